@@ -1,0 +1,167 @@
+/*
+ * rtod.h -- C-ABI of the MI355X-native RTOD per-slide outlier detector.
+ *
+ * Plain C types only: pointers, sizes, status codes.  No exception crosses
+ * this boundary; every call returns an rtod_status.  The library owns all
+ * device memory, its HIP stream and its HIP graph; the caller owns every host
+ * array it passes.  One context per host thread; contexts are independent.
+ *
+ * Semantics (identical to the reference's ods slide loop):
+ *   - window of W points, slide of S points, W % S == 0 (the reference uses
+ *     unit_num = W / S and silently corrupts otherwise, optixScan.cpp:1037);
+ *   - global point g lives in ring slot g mod W; rtod_prime loads slots
+ *     0..W-1, slide t writes slots ((t-1) mod (W/S))*S + [0,S)
+ *     (optixScan.cpp:1020-1033, :1053-1057, :1106);
+ *   - a window point p is an outlier iff #{q : d2(p,q) <= R*R} <= K, the
+ *     count including p (optixScan.cu:158-199, optixScan.cpp:1095-1099), with
+ *     d2 in FP32: dx*dx, then fmaf(dy,dy,.), fmaf(dz,dz,.);
+ *   - outliers are returned as ring slots, sorted ascending (the reference's
+ *     h_outlier_list holds the same set unordered, optixScan.cpp:1097).
+ */
+#ifndef RTOD_H
+#define RTOD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RTOD_API __attribute__((visibility("default")))
+
+typedef enum rtod_status {
+    RTOD_OK = 0,
+    RTOD_ERR_INVALID_ARGUMENT = 1, /* bad config / pointer / size            */
+    RTOD_ERR_UNSUPPORTED = 2,      /* valid but not supported by this build  */
+    RTOD_ERR_STATE = 3,            /* call out of order (e.g. slide before prime) */
+    RTOD_ERR_HIP = 4,              /* HIP runtime error (device lost, ...)   */
+    RTOD_ERR_NOMEM = 5,            /* device or pinned host allocation failed */
+    RTOD_ERR_INTERNAL = 6          /* device-side consistency check failed   */
+} rtod_status;
+
+typedef enum rtod_index_kind {
+    RTOD_INDEX_AUTO = 0,         /* cell stencil (fastest measured)           */
+    RTOD_INDEX_CELL_STENCIL = 1, /* cell-sorted window + sampled key table    */
+    RTOD_INDEX_LBVH = 2          /* Morton-code LBVH over candidate boxes     */
+} rtod_index_kind;
+
+typedef enum rtod_sched_kind {
+    RTOD_SCHED_PLAIN = 0,  /* candidates range-counted in cell order          */
+    RTOD_SCHED_LEARNED = 1 /* MLP-predicted ordering/bucketing, exact verify  */
+} rtod_sched_kind;
+
+/* flags */
+#define RTOD_FLAG_PHASE_TIMES 0x1 /* launch phases one by one between HIP
+                                     events and fill rtod_times per phase  */
+#define RTOD_FLAG_NO_GRAPH 0x2    /* launch kernels directly, no hipGraph  */
+
+/* Replaces the compile-time DIMENSION/MK/OPTIMIZATION macros and the runtime
+ * flags --window --slide --R --K of the reference (optixScan.h:4-28,
+ * optixScan.cpp:149-212).  dim in {1,2,3}; 1 <= S <= W, W % S == 0;
+ * K >= 1; R finite with 1e-18 <= R <= 1e18. */
+typedef struct rtod_config {
+    int32_t dim;
+    int32_t window;
+    int32_t slide;
+    float R;
+    int32_t K;
+    int32_t index_kind;   /* rtod_index_kind */
+    int32_t sched_kind;   /* rtod_sched_kind */
+    float miss_threshold; /* learned scheduler: fall back to the plain
+                             schedule when the miss rate exceeds this      */
+    int32_t flags;        /* RTOD_FLAG_*                                   */
+    int32_t device;       /* HIP device ordinal                            */
+} rtod_config;
+
+/* Per-slide time breakdown in milliseconds, the reference Timer buckets
+ * (timer.h:16-24, printed by showTimeNew timer.h:102-117). */
+typedef struct rtod_times {
+    double copy_h2d;       /* new slide into the device ring              */
+    double prepare_cell;   /* cell keys, LDS radix sort, per-cell counts,
+                              dense fast-accept, candidate compaction     */
+    double build_index;    /* cell-stencil index / LBVH build             */
+    double detect_outlier; /* range counts with early exit                */
+    double copy_d2h;       /* outlier compaction + list to host           */
+    double total;          /* host wall clock, entry to list in host mem  */
+} rtod_times;
+
+/* Counters mirroring the reference's DEBUG_INFO / "BVH Node / slide" output
+ * (optixScan.cpp:1176-1187, optixScan.cu:71-74). */
+typedef struct rtod_stats {
+    int64_t slides;
+    int64_t candidates_last;    /* points not fast-accepted, last slide    */
+    int64_t candidates_total;
+    int64_t pair_tests_last;    /* predicate evaluations, last slide       */
+    int64_t pair_tests_total;
+    int64_t outliers_last;
+    int64_t outliers_total;
+    int64_t device_bytes;       /* device memory owned by the context      */
+    int64_t host_pinned_bytes;  /* pinned host memory owned by the context */
+    int64_t predictor_misses;   /* learned scheduler: mispredicted points  */
+    int64_t predictor_points;   /* learned scheduler: predicted points     */
+    int64_t fallbacks;          /* slides run on the plain schedule after
+                                   the miss rate exceeded the threshold   */
+    int32_t index_kind;         /* index actually used                     */
+    int32_t sched_kind;         /* schedule actually used last slide       */
+} rtod_stats;
+
+typedef struct rtod_ctx rtod_ctx;
+
+/* Allocates the device ring, sorted-window buffers, batch ring, pinned
+ * staging and output buffers, and the stream.  Replaces data_h2d,
+ * initialize_params and initialize_cell (optixScan.cpp:232-238, :674-724,
+ * :960-965). */
+RTOD_API rtod_status rtod_create(const rtod_config *cfg, rtod_ctx **out);
+
+/* Loads the first W points ([W][dim] row-major) into ring slots 0..W-1 and
+ * builds the cell structure.  Untimed, like the reference's initial window
+ * load (optixScan.cpp:1020-1033).  May be called again to restart. */
+RTOD_API rtod_status rtod_prime(rtod_ctx *ctx, const float *pts, int64_t n_pts);
+
+/* One slide: new_pts [S][dim] row-major; writes the window's outliers
+ * (ring slots, ascending) to outlier_slots (capacity >= W) and their number
+ * to *n_outliers; times may be NULL.  Replaces the body of the while loop of
+ * detect_outlier (optixScan.cpp:1045-1110). */
+RTOD_API rtod_status rtod_slide(rtod_ctx *ctx, const float *new_pts, int32_t *outlier_slots,
+                                int32_t *n_outliers, rtod_times *times);
+
+/* Phase-level entry points: the same slide split where the reference's loop
+ * body calls out.  Call in this order; rtod_slide == all five.
+ *   rtod_copy_new_points  <- cudaMemcpy of the new slide (optixScan.cpp:1051-1058)
+ *   rtod_prepare_cell     <- prepare_c_non_points_queue   (optixScan.cpp:967-1017)
+ *   rtod_build_index      <- rebuild_gas / kGenAABB       (optixScan.cpp:378-426,
+ *                                                          aabb.cu:35-45)
+ *   rtod_detect_outlier   <- launch                       (optixScan.cpp:727-750)
+ *   rtod_copy_outlier     <- D2H + select count <= K      (optixScan.cpp:1083-1101) */
+RTOD_API rtod_status rtod_copy_new_points(rtod_ctx *ctx, const float *new_pts);
+RTOD_API rtod_status rtod_prepare_cell(rtod_ctx *ctx);
+RTOD_API rtod_status rtod_build_index(rtod_ctx *ctx);
+RTOD_API rtod_status rtod_detect_outlier(rtod_ctx *ctx);
+RTOD_API rtod_status rtod_copy_outlier(rtod_ctx *ctx, int32_t *outlier_slots, int32_t *n_outliers);
+
+/* Counters (reference DEBUG_INFO counters and [Mem] lines,
+ * optixScan.cpp:66-93, :752-815, :1170-1187). */
+RTOD_API rtod_status rtod_get_stats(const rtod_ctx *ctx, rtod_stats *out);
+
+/* Average device duration (ms) per launch of each kernel recorded with HIP
+ * events on the context's stream while RTOD_FLAG_PHASE_TIMES is set.
+ * names[i] (static strings) / avg_ms[i] / launches[i] for i < *n (in: capacity). */
+RTOD_API rtod_status rtod_kernel_times(const rtod_ctx *ctx, const char **names, double *avg_ms,
+                                       int64_t *launches, int32_t *n);
+RTOD_API rtod_status rtod_reset_kernel_times(rtod_ctx *ctx);
+
+/* Frees everything (reference cleanup, optixScan.cpp:883-932). */
+RTOD_API void rtod_destroy(rtod_ctx *ctx);
+
+RTOD_API const char *rtod_status_str(rtod_status s);
+
+/* Last error message of this thread (static storage, never NULL). */
+RTOD_API const char *rtod_last_error(void);
+
+/* Library version string. */
+RTOD_API const char *rtod_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,104 @@
+/*
+ * rtod_oracle.h -- CPU restatement of RTOD's per-slide outlier detection.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path and
+ * the timed CPU baseline; only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product path (rtod_amd/)
+ * never links or calls it.
+ *
+ * Contract (shared with the HIP path):
+ *   - a window point p is an OUTLIER iff  #{q in window : d2(p,q) <= r2} <= K,
+ *     the count including p itself
+ *     (reference: src/ods/optixScan.cu:158-199 counts the ray's own box,
+ *      src/ods/optixScan.cpp:1095-1099 selects count <= K,
+ *      src/ods/optixScan.cpp:817-838 brute-force checker, "include itself").
+ *   - d2 is FP32:  dx = p.x - q.x; d2 = dx*dx; d2 = fmaf(dy,dy,d2);
+ *     d2 = fmaf(dz,dz,d2);  r2 = R*R in FP32.  (The reference tests FP64
+ *     `|dx| < R` / `sqdist < R2`, optixScan.cu:170-189; semantics mode 1 of
+ *     rtod_o_brute restates that rule so disagreements can be counted.)
+ *   - ring semantics: global index g lives in ring slot g mod W; slide t
+ *     (t = 1, 2, ...) holds x[tS, tS+W)  (optixScan.cpp:1019-1126).
+ *   - outliers are reported as ring-slot ids sorted ascending (the reference
+ *     reports the same set in arbitrary order, optixScan.cpp:1097).
+ *
+ * Parity status of each function is stated next to it and in the tests that
+ * pin it (tests/test_oracle.py).
+ */
+#ifndef RTOD_ORACLE_H
+#define RTOD_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* FP32 squared distance with explicit FMA (the contract predicate). */
+float rtod_o_d2(const float *p, const float *q, int dim);
+
+/* Deterministic generators (xoshiro256** seeded by splitmix64, Box-Muller in
+ * double, cast to f32).  Random walk per axis, spikes of magnitude U(lo,hi)
+ * along a random direction with probability p_spike. */
+void rtod_o_gen_walk(uint64_t seed, int64_t n, int dim, double start,
+                     double sigma, double p_spike, double spike_lo,
+                     double spike_hi, float *out);
+/* Mixture of ncomp anisotropic Gaussians, centres U[lo,hi]^dim, per-axis
+ * sigma U(sig_lo,sig_hi), linear centre drift of total length `drift` over
+ * the stream, fraction noise_frac of uniform background in [lo,hi]^dim. */
+void rtod_o_gen_gmm(uint64_t seed, int64_t n, int dim, int ncomp, double lo,
+                    double hi, double sig_lo, double sig_hi, double noise_frac,
+                    double drift, float *out);
+
+/* Brute force over one window `win` ([W][dim], slot order).
+ * semantics 0: contract predicate (FP32 d2 <= r2).
+ * semantics 1: the reference's FP64 strict rule (1-D |dx| < R, else
+ *              sqdist < R*R in double) -- src/ods/optixScan.cu:170-189.
+ * Writes sorted outlier slots, optional per-slot counts capped at K+1.
+ * Returns the number of outliers.  Follows optixScan.cpp:817-838. */
+int64_t rtod_o_brute(const float *win, int64_t W, int dim, float R, int K,
+                     int semantics, int nthreads, int32_t *out_slots,
+                     int32_t *out_counts);
+
+/* 1-D: sorted-window oracle.  For sorted values the neighbour set of x_i is a
+ * contiguous index interval [lo, hi] (the rounded predicate is monotone in
+ * |x_i - x_j|), so count = hi - lo + 1 from two binary searches. */
+int64_t rtod_o_sorted1d(const float *win, int64_t W, float R, int K,
+                        int32_t *out_slots, int32_t *out_counts);
+
+/* Grid oracle: the reference's grid-filter algorithm restated with exact
+ * semantics (optixScan.cpp:947-1017 cell map + dense rule, optixScan.cu:158-199
+ * count with early exit).  Cells of side R(1-1e-5)/sqrt(d) keyed by
+ * floor(x / c) in double; a cell holding >= K+1 points is dense (all of them
+ * are inliers); every other point is range-counted over the stencil cells
+ * with early exit at K+1.  Cells are per-cell FIFOs of ring slots updated
+ * incrementally each slide. */
+typedef struct rtod_o_grid rtod_o_grid;
+rtod_o_grid *rtod_o_grid_create(int dim, int64_t W, int64_t S, float R, int K,
+                                int nthreads);
+int rtod_o_grid_prime(rtod_o_grid *g, const float *pts);
+int64_t rtod_o_grid_slide(rtod_o_grid *g, const float *new_pts,
+                          int32_t *out_slots);
+/* stats: [0] candidates last slide, [1] active cells, [2] pair tests last
+ * slide, [3] dense cells last slide */
+void rtod_o_grid_stats(const rtod_o_grid *g, int64_t *out4);
+void rtod_o_grid_destroy(rtod_o_grid *g);
+
+/* Cell key of a point as the oracle computes it (packing documented in the
+ * source); exposed for tests. */
+uint64_t rtod_o_cell_key(const float *p, int dim, float R);
+
+/* Drive T slides of stream xs[(W + T*S) * dim].
+ * method 0 grid, 1 brute (contract), 2 sorted1d, 3 brute (reference FP64).
+ * n_out[T] per-slide outlier counts; out[] concatenated sorted slot lists
+ * (capacity cap).  *us_per_slide (nullable) = mean wall time of the slide
+ * calls (method 0) or of the per-window detections.  Returns total listed,
+ * or -1 on overflow / bad argument. */
+int64_t rtod_o_run_stream(int method, const float *xs, int dim, int64_t W,
+                          int64_t S, int64_t T, float R, int K, int nthreads,
+                          int32_t *n_out, int32_t *out, int64_t cap,
+                          double *us_per_slide);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

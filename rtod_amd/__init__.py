@@ -1,0 +1,229 @@
+"""rtod_amd -- MI355X-native RTOD per-slide outlier detection.
+
+Python host mirror of the reference's ods slide loop over the C-ABI of
+``include/rtod.h`` (library ``rtod_amd/librtod.so``, built by
+``rtod_amd/build.py``).  The GPU path is the only path: there is no CPU
+fallback, and loading fails loudly when the HIP library is missing.
+
+Reference mapping (src/ods/optixScan.cpp):
+  Detector(...)              <- parse_args / initialize_params / initialize_cell
+                                (:149-212, :674-724, :960-965)
+  Detector.prime(pts)        <- initial window load of detect_outlier (:1020-1033)
+  Detector.slide(new_pts)    <- one iteration of the slide loop (:1045-1110)
+  Detector.copy_new_points / prepare_cell / build_index / detect_outlier /
+  copy_outlier               <- the loop body's calls (:1051-1101)
+  Detector.times             <- Timer buckets (timer.h:16-24, :102-117)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librtod.so")
+
+INDEX_AUTO, INDEX_CELL_STENCIL, INDEX_LBVH = 0, 1, 2
+SCHED_PLAIN, SCHED_LEARNED = 0, 1
+FLAG_PHASE_TIMES, FLAG_NO_GRAPH = 0x1, 0x2
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "unsupported", 3: "call out of order",
+          4: "HIP runtime error", 5: "out of memory", 6: "internal error"}
+
+
+class RtodError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class _Config(C.Structure):
+    _fields_ = [("dim", C.c_int32), ("window", C.c_int32), ("slide", C.c_int32),
+                ("R", C.c_float), ("K", C.c_int32), ("index_kind", C.c_int32),
+                ("sched_kind", C.c_int32), ("miss_threshold", C.c_float),
+                ("flags", C.c_int32), ("device", C.c_int32)]
+
+
+class _Times(C.Structure):
+    _fields_ = [("copy_h2d", C.c_double), ("prepare_cell", C.c_double),
+                ("build_index", C.c_double), ("detect_outlier", C.c_double),
+                ("copy_d2h", C.c_double), ("total", C.c_double)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("slides", C.c_int64), ("candidates_last", C.c_int64),
+                ("candidates_total", C.c_int64), ("pair_tests_last", C.c_int64),
+                ("pair_tests_total", C.c_int64), ("outliers_last", C.c_int64),
+                ("outliers_total", C.c_int64), ("device_bytes", C.c_int64),
+                ("host_pinned_bytes", C.c_int64), ("predictor_misses", C.c_int64),
+                ("predictor_points", C.c_int64), ("fallbacks", C.c_int64),
+                ("index_kind", C.c_int32), ("sched_kind", C.c_int32)]
+
+
+# (name, restype, argtypes) of every symbol include/rtod.h declares
+_F32P = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_I32P = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+SIGNATURES = [
+    ("rtod_create", C.c_int, [C.POINTER(_Config), C.POINTER(C.c_void_p)]),
+    ("rtod_prime", C.c_int, [C.c_void_p, _F32P, C.c_int64]),
+    ("rtod_slide", C.c_int, [C.c_void_p, _F32P, _I32P, C.POINTER(C.c_int32),
+                             C.POINTER(_Times)]),
+    ("rtod_copy_new_points", C.c_int, [C.c_void_p, _F32P]),
+    ("rtod_prepare_cell", C.c_int, [C.c_void_p]),
+    ("rtod_build_index", C.c_int, [C.c_void_p]),
+    ("rtod_detect_outlier", C.c_int, [C.c_void_p]),
+    ("rtod_copy_outlier", C.c_int, [C.c_void_p, _I32P, C.POINTER(C.c_int32)]),
+    ("rtod_get_stats", C.c_int, [C.c_void_p, C.POINTER(_Stats)]),
+    ("rtod_kernel_times", C.c_int, [C.c_void_p, C.POINTER(C.c_char_p),
+                                    C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_int32)]),
+    ("rtod_reset_kernel_times", C.c_int, [C.c_void_p]),
+    ("rtod_destroy", None, [C.c_void_p]),
+    ("rtod_status_str", C.c_char_p, [C.c_int]),
+    ("rtod_last_error", C.c_char_p, []),
+    ("rtod_version", C.c_char_p, []),
+]
+
+_lib = None
+
+
+def lib():
+    """Load librtod.so (raises if it was not built -- there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                               f"`python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        raise RtodError(status, lib().rtod_last_error().decode())
+
+
+@dataclass
+class Times:
+    """Per-slide breakdown in ms, reference Timer bucket names."""
+    copy_h2d: float
+    prepare_cell: float
+    build_index: float
+    detect_outlier: float
+    copy_d2h: float
+    total: float
+
+
+class Detector:
+    """One sliding-window stream on one GPU (the reference's ScanState)."""
+
+    def __init__(self, dim: int, window: int, slide: int, R: float, K: int,
+                 index_kind: int = INDEX_AUTO, sched_kind: int = SCHED_PLAIN,
+                 miss_threshold: float = 0.25, flags: int = 0, device: int = 0):
+        self.dim, self.W, self.S, self.R, self.K = dim, window, slide, R, K
+        cfg = _Config(dim, window, slide, R, K, index_kind, sched_kind, miss_threshold, flags,
+                      device)
+        h = C.c_void_p()
+        _check(lib().rtod_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._out = np.empty(window, np.int32)
+        self._times = _Times()
+
+    # -- lifecycle ---------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            lib().rtod_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- stream ------------------------------------------------------------
+    def prime(self, pts: np.ndarray) -> None:
+        pts = np.ascontiguousarray(pts, np.float32).reshape(-1)
+        if pts.size != self.W * self.dim:
+            raise ValueError("prime needs exactly W points")
+        _check(lib().rtod_prime(self._h, pts, self.W))
+
+    def slide(self, new_pts: np.ndarray) -> np.ndarray:
+        """Outlier ring slots (ascending) of the window after this slide."""
+        new_pts = np.ascontiguousarray(new_pts, np.float32).reshape(-1)
+        if new_pts.size != self.S * self.dim:
+            raise ValueError("slide needs exactly S points")
+        n = C.c_int32(0)
+        _check(lib().rtod_slide(self._h, new_pts, self._out, C.byref(n), C.byref(self._times)))
+        return self._out[:n.value].copy()
+
+    @property
+    def times(self) -> Times:
+        t = self._times
+        return Times(t.copy_h2d, t.prepare_cell, t.build_index, t.detect_outlier, t.copy_d2h,
+                     t.total)
+
+    # -- phase-level API (reference loop body) ------------------------------
+    def copy_new_points(self, new_pts: np.ndarray) -> None:
+        new_pts = np.ascontiguousarray(new_pts, np.float32).reshape(-1)
+        if new_pts.size != self.S * self.dim:
+            raise ValueError("slide needs exactly S points")
+        _check(lib().rtod_copy_new_points(self._h, new_pts))
+
+    def prepare_cell(self) -> None:
+        _check(lib().rtod_prepare_cell(self._h))
+
+    def build_index(self) -> None:
+        _check(lib().rtod_build_index(self._h))
+
+    def detect_outlier(self) -> None:
+        _check(lib().rtod_detect_outlier(self._h))
+
+    def copy_outlier(self) -> np.ndarray:
+        n = C.c_int32(0)
+        _check(lib().rtod_copy_outlier(self._h, self._out, C.byref(n)))
+        return self._out[:n.value].copy()
+
+    # -- counters ----------------------------------------------------------
+    def stats(self) -> dict:
+        s = _Stats()
+        _check(lib().rtod_get_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in _Stats._fields_}
+
+    def kernel_times(self) -> dict:
+        cap = 16
+        names = (C.c_char_p * cap)()
+        avg = (C.c_double * cap)()
+        cnt = (C.c_int64 * cap)()
+        n = C.c_int32(cap)
+        _check(lib().rtod_kernel_times(self._h, names, avg, cnt, C.byref(n)))
+        return {names[i].decode(): (avg[i], cnt[i]) for i in range(n.value)}
+
+    def reset_kernel_times(self) -> None:
+        _check(lib().rtod_reset_kernel_times(self._h))
+
+
+def run_stream(xs: np.ndarray, W: int, S: int, T: int, R: float, K: int, **kw):
+    """Per-slide outlier slot lists for slides 1..T of stream xs ([N][dim])."""
+    xs = np.ascontiguousarray(xs, np.float32)
+    dim = xs.shape[1]
+    with Detector(dim, W, S, R, K, **kw) as det:
+        det.prime(xs[:W])
+        return [det.slide(xs[W + (t - 1) * S: W + t * S]) for t in range(1, T + 1)]
+
+
+def version() -> str:
+    return lib().rtod_version().decode()

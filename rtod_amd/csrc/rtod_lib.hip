@@ -1,0 +1,758 @@
+// rtod_lib.hip -- context management and the C-ABI of include/rtod.h.
+//
+// One context = one window stream on one GPU: its own HIP stream, device
+// arena and pinned staging.  A slide is one hipGraph launch of the plain
+// path's kernels (rtod_plain.h) and one stream synchronisation; the graph is
+// parameter-free because every per-slide quantity (ring offset, ping-pong
+// parity, batch-ring entries, staging buffer) derives from the device-side
+// slide counter that the detect kernel advances.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/rtod.h"
+#include "rtod_plain.h"
+
+using namespace rtod;
+
+namespace {
+
+thread_local std::string g_last_error = "";
+
+void set_error(const std::string &s) { g_last_error = s; }
+
+#define RTOD_HIP(call)                                                                            \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) {                                                                   \
+            set_error(std::string(#call) + ": " + hipGetErrorString(e_));                         \
+            return RTOD_ERR_HIP;                                                                  \
+        }                                                                                         \
+    } while (0)
+
+double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+enum KernelId { kKIngest = 0, kKMergePass, kKMergeWindow, kKClassify, kKDetect, kKCount };
+const char *kKernelNames[kKCount] = {"ingest_sort", "merge_pass", "merge_window", "classify",
+                                     "detect"};
+
+// LDS-sort tile shapes, picked by the batch size S
+enum SortShape { kSort256x2 = 0, kSort256x8 = 1, kSort1024x8 = 2 };
+int sort_tile(int shape) { return shape == kSort256x2 ? 512 : shape == kSort256x8 ? 2048 : 8192; }
+
+} // namespace
+
+struct rtod_ctx {
+    rtod_config cfg{};
+    int dim = 1, W = 0, S = 0, m = 0, K = 0;
+    int sort_shape = 0, CH = 0;
+    int detect_blocks = 512;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    Dev d{};
+    float *d_prime = nullptr; // device copy of the initial window
+    float *h_in = nullptr;
+    HostOut *h_out = nullptr;
+    std::vector<void *> dev_allocs;
+    int64_t dev_bytes = 0, pinned_bytes = 0;
+    bool primed = false;
+    bool staged = false;   // phase API: new points copied, kernels pending
+    int phase = 0;         // phase API progress
+    int64_t host_t = 0;
+    hipEvent_t ev[8] = {};
+    double ksum[kKCount] = {};
+    int64_t kcnt[kKCount] = {};
+    // stats
+    int64_t cand_last = 0, cand_total = 0, tests_last = 0, tests_total = 0;
+    int64_t out_last = 0, out_total = 0;
+};
+
+namespace {
+
+template <typename T> rtod_status dalloc(rtod_ctx *c, T **p, size_t n)
+{
+    void *q = nullptr;
+    size_t bytes = n * sizeof(T);
+    if (bytes == 0)
+        bytes = 16;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        return RTOD_ERR_NOMEM;
+    }
+    c->dev_allocs.push_back(q);
+    c->dev_bytes += (int64_t)bytes;
+    *p = (T *)q;
+    return RTOD_OK;
+}
+
+#define RTOD_TRY(x)                                                                               \
+    do {                                                                                          \
+        rtod_status s_ = (x);                                                                     \
+        if (s_ != RTOD_OK)                                                                        \
+            return s_;                                                                            \
+    } while (0)
+
+template <int DIM> void launch_ingest(rtod_ctx *c, hipStream_t s, int mode, const float *src,
+                                      int grid, int dst_batch)
+{
+    switch (c->sort_shape) {
+    case kSort256x2:
+        hipLaunchKernelGGL((k_ingest_sort<DIM, 256, 2>), dim3(grid), dim3(256), 0, s, c->d, mode,
+                           src, dst_batch);
+        break;
+    case kSort256x8:
+        hipLaunchKernelGGL((k_ingest_sort<DIM, 256, 8>), dim3(grid), dim3(256), 0, s, c->d, mode,
+                           src, dst_batch);
+        break;
+    default:
+        hipLaunchKernelGGL((k_ingest_sort<DIM, 1024, 8>), dim3(grid), dim3(1024), 0, s, c->d,
+                           mode, src, dst_batch);
+        break;
+    }
+}
+
+void launch_ingest_dim(rtod_ctx *c, hipStream_t s, int mode, const float *src, int grid,
+                       int dst_batch)
+{
+    if (c->dim == 1)
+        launch_ingest<1>(c, s, mode, src, grid, dst_batch);
+    else if (c->dim == 2)
+        launch_ingest<2>(c, s, mode, src, grid, dst_batch);
+    else
+        launch_ingest<3>(c, s, mode, src, grid, dst_batch);
+}
+
+// merge passes turning chunk-sorted runs of length c->CH into sorted
+// segments of seg_len (the batch), last pass into the batch ring
+void launch_batch_merges(rtod_ctx *c, hipStream_t s, int mode, int n_total)
+{
+    const uint64_t *sk = c->d.tkey[0];
+    const int32_t *ss = c->d.tslot[0];
+    int src = 0;
+    for (int L = c->CH; L < c->S; L *= 2) {
+        const bool last = 2 * L >= c->S;
+        uint64_t *dk = last ? c->d.bkey : c->d.tkey[src ^ 1];
+        int32_t *ds = last ? c->d.bslot : c->d.tslot[src ^ 1];
+        hipLaunchKernelGGL(k_merge_pass, dim3((n_total + 255) / 256), dim3(256), 0, s, c->d, sk,
+                           ss, dk, ds, n_total, c->S, L, last ? 1 : 0, mode);
+        src ^= 1;
+        sk = c->d.tkey[src];
+        ss = c->d.tslot[src];
+    }
+}
+
+void launch_merge_window(rtod_ctx *c, hipStream_t s)
+{
+    const int grid = (c->W + 1023) / 1024;
+    if (c->dim == 1)
+        hipLaunchKernelGGL(k_merge_window<1>, dim3(grid), dim3(256), 0, s, c->d);
+    else if (c->dim == 2)
+        hipLaunchKernelGGL(k_merge_window<2>, dim3(grid), dim3(256), 0, s, c->d);
+    else
+        hipLaunchKernelGGL(k_merge_window<3>, dim3(grid), dim3(256), 0, s, c->d);
+}
+
+void launch_classify(rtod_ctx *c, hipStream_t s)
+{
+    const int grid = (c->W + 255) / 256;
+    if (c->dim == 1)
+        hipLaunchKernelGGL(k_classify<1>, dim3(grid), dim3(256), 0, s, c->d);
+    else if (c->dim == 2)
+        hipLaunchKernelGGL(k_classify<2>, dim3(grid), dim3(256), 0, s, c->d);
+    else
+        hipLaunchKernelGGL(k_classify<3>, dim3(grid), dim3(256), 0, s, c->d);
+}
+
+void launch_detect(rtod_ctx *c, hipStream_t s)
+{
+    const int grid = c->detect_blocks;
+    if (c->dim == 1)
+        hipLaunchKernelGGL(k_detect<1>, dim3(grid), dim3(kDetectNT), 0, s, c->d);
+    else if (c->dim == 2)
+        hipLaunchKernelGGL(k_detect<2>, dim3(grid), dim3(kDetectNT), 0, s, c->d);
+    else
+        hipLaunchKernelGGL(k_detect<3>, dim3(grid), dim3(kDetectNT), 0, s, c->d);
+}
+
+// The slide's kernel sequence.  ev != nullptr: record an event before the
+// first kernel and after each phase (prepare: ingest + merges, merge_window,
+// classify; detect).
+void launch_slide(rtod_ctx *c, hipStream_t s, hipEvent_t *ev)
+{
+    if (ev)
+        (void)hipEventRecord(ev[0], s);
+    launch_ingest_dim(c, s, kModeSlide, nullptr, c->d.cpb, c->d.cpb == 1 ? 1 : 0);
+    if (ev)
+        (void)hipEventRecord(ev[1], s);
+    if (c->d.cpb > 1)
+        launch_batch_merges(c, s, kModeSlide, c->S);
+    if (ev)
+        (void)hipEventRecord(ev[2], s);
+    launch_merge_window(c, s);
+    if (ev)
+        (void)hipEventRecord(ev[3], s);
+    launch_classify(c, s);
+    if (ev)
+        (void)hipEventRecord(ev[4], s);
+    launch_detect(c, s);
+    if (ev)
+        (void)hipEventRecord(ev[5], s);
+}
+
+rtod_status build_graph(rtod_ctx *c)
+{
+    if (c->gexec)
+        return RTOD_OK;
+    RTOD_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    launch_slide(c, c->stream, nullptr);
+    hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
+    if (e != hipSuccess) {
+        set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        return RTOD_ERR_HIP;
+    }
+    RTOD_HIP(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    return RTOD_OK;
+}
+
+rtod_status validate(const rtod_config *cfg)
+{
+    if (!cfg) {
+        set_error("config is NULL");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (cfg->dim < 1 || cfg->dim > 3) {
+        set_error("dim must be 1, 2 or 3");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (cfg->window < 1 || cfg->window > (1 << 28)) {
+        set_error("window must be in [1, 2^28]");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (cfg->slide < 1 || cfg->slide > cfg->window || cfg->window % cfg->slide != 0) {
+        set_error("slide must divide window (1 <= S <= W, W % S == 0)");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (cfg->K < 1) {
+        set_error("K must be >= 1");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (!std::isfinite(cfg->R) || cfg->R < 1e-18f || cfg->R > 1e18f) {
+        set_error("R must be finite and in [1e-18, 1e18]");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (cfg->index_kind < 0 || cfg->index_kind > 2 || cfg->sched_kind < 0 ||
+        cfg->sched_kind > 1) {
+        set_error("unknown index_kind or sched_kind");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (cfg->index_kind == RTOD_INDEX_LBVH || cfg->sched_kind == RTOD_SCHED_LEARNED) {
+        set_error("LBVH index and learned scheduler are not available in this build");
+        return RTOD_ERR_UNSUPPORTED;
+    }
+    return RTOD_OK;
+}
+
+rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out)
+{
+    RTOD_TRY(validate(cfg));
+    rtod_ctx *c = new (std::nothrow) rtod_ctx();
+    if (!c)
+        return RTOD_ERR_NOMEM;
+    *out = c;
+    c->cfg = *cfg;
+    c->dim = cfg->dim;
+    c->W = cfg->window;
+    c->S = cfg->slide;
+    c->m = c->W / c->S;
+    c->K = cfg->K;
+    c->device = cfg->device;
+    RTOD_HIP(hipSetDevice(c->device));
+    RTOD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto &e : c->ev)
+        RTOD_HIP(hipEventCreate(&e));
+
+    c->sort_shape = c->S <= 512 ? kSort256x2 : c->S <= 2048 ? kSort256x8 : kSort1024x8;
+    c->CH = sort_tile(c->sort_shape);
+
+    Dev &d = c->d;
+    d.dim = c->dim;
+    d.W = c->W;
+    d.S = c->S;
+    d.m = c->m;
+    d.K = c->K;
+    d.cpb = (c->S + c->CH - 1) / c->CH;
+    int sl = 4;
+    while (((int64_t)c->W + (1ll << sl) - 1) >> sl > kMaxSamples)
+        ++sl;
+    d.stride_log2 = sl;
+    d.nsamp = (int)(((int64_t)c->W + (1ll << sl) - 1) >> sl);
+    const float R = cfg->R;
+    d.r2 = R * R;
+    const double cside = (double)R * (1.0 - 1e-5) / std::sqrt((double)c->dim);
+    d.inv_c = 1.0 / cside;
+    d.Rp = (double)R * (1.0 + 1e-5);
+
+    const size_t W = c->W, S = c->S, dim = c->dim, m = c->m;
+    const size_t tmpn = W > S ? W : S;
+    RTOD_TRY(dalloc(c, &d.ring, dim * W));
+    for (int p = 0; p < 2; ++p) {
+        RTOD_TRY(dalloc(c, &d.akey[p], W));
+        RTOD_TRY(dalloc(c, &d.aslot[p], W));
+        RTOD_TRY(dalloc(c, &d.axyz[p], dim * W));
+        RTOD_TRY(dalloc(c, &d.tkey[p], tmpn));
+        RTOD_TRY(dalloc(c, &d.tslot[p], tmpn));
+    }
+    RTOD_TRY(dalloc(c, &d.bkey, (m + 1) * S));
+    RTOD_TRY(dalloc(c, &d.bslot, (m + 1) * S));
+    RTOD_TRY(dalloc(c, &d.samples, (size_t)d.nsamp));
+    RTOD_TRY(dalloc(c, &d.cand, W));
+    RTOD_TRY(dalloc(c, &d.cand_cnt, W));
+    RTOD_TRY(dalloc(c, &d.bitmap, (W + 31) / 32));
+    RTOD_TRY(dalloc(c, &d.ctl, 1));
+    RTOD_TRY(dalloc(c, &c->d_prime, dim * W));
+    RTOD_HIP(hipMemsetAsync(d.bitmap, 0, sizeof(uint32_t) * ((W + 31) / 32), c->stream));
+    RTOD_HIP(hipMemsetAsync(d.ctl, 0, sizeof(Ctl), c->stream));
+
+    const size_t in_bytes = 2 * S * dim * sizeof(float);
+    const size_t out_bytes = sizeof(HostOut) + W * sizeof(int32_t);
+    RTOD_HIP(hipHostMalloc((void **)&c->h_in, in_bytes, hipHostMallocMapped | hipHostMallocCoherent));
+    RTOD_HIP(hipHostMalloc((void **)&c->h_out, out_bytes, hipHostMallocMapped | hipHostMallocCoherent));
+    c->pinned_bytes = (int64_t)(in_bytes + out_bytes);
+    std::memset(c->h_in, 0, in_bytes);
+    std::memset((void *)c->h_out, 0, out_bytes);
+    d.h_in = c->h_in;
+    d.h_out = c->h_out;
+    RTOD_HIP(hipStreamSynchronize(c->stream));
+    return RTOD_OK;
+}
+
+template <int DIM> void launch_gather(rtod_ctx *c, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gather<DIM>, dim3((c->W + 255) / 256), dim3(256), 0, s, c->d, 0);
+}
+
+rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n)
+{
+    if (!pts || n != c->W) {
+        set_error("prime needs exactly W points");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    RTOD_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    RTOD_HIP(hipMemcpyAsync(c->d_prime, pts, sizeof(float) * (size_t)c->W * c->dim,
+                            hipMemcpyHostToDevice, s));
+    // sort each initial batch (generation b -> batch-ring entry b)
+    launch_ingest_dim(c, s, kModePrime, c->d_prime, c->m * c->d.cpb, c->d.cpb == 1 ? 1 : 0);
+    if (c->d.cpb > 1)
+        launch_batch_merges(c, s, kModePrime, c->W);
+    // merge the m sorted batches into the sorted window A[0]
+    {
+        const uint64_t *sk = c->d.bkey;
+        const int32_t *ss = c->d.bslot;
+        int src = 0;
+        int L = c->S;
+        do {
+            const bool last = 2 * (int64_t)L >= c->W;
+            uint64_t *dk = last ? c->d.akey[0] : c->d.tkey[src ^ 1];
+            int32_t *ds = last ? c->d.aslot[0] : c->d.tslot[src ^ 1];
+            hipLaunchKernelGGL(k_merge_pass, dim3((c->W + 255) / 256), dim3(256), 0, s, c->d, sk,
+                               ss, dk, ds, c->W, c->W, L, 0, (int)kModePrime);
+            src ^= 1;
+            sk = c->d.tkey[src];
+            ss = c->d.tslot[src];
+            if (last)
+                break;
+            L *= 2;
+        } while (true);
+    }
+    if (c->dim == 1)
+        launch_gather<1>(c, s);
+    else if (c->dim == 2)
+        launch_gather<2>(c, s);
+    else
+        launch_gather<3>(c, s);
+    RTOD_HIP(hipMemsetAsync(c->d.bitmap, 0, sizeof(uint32_t) * ((c->W + 31) / 32), s));
+    RTOD_HIP(hipMemsetAsync(c->d.ctl, 0, sizeof(Ctl), s));
+    RTOD_HIP(hipGetLastError());
+    RTOD_HIP(hipStreamSynchronize(s));
+    c->h_out->slide = 0;
+    c->host_t = 0;
+    c->primed = true;
+    c->staged = false;
+    c->phase = 0;
+    return RTOD_OK;
+}
+
+rtod_status collect(rtod_ctx *c, int32_t *outs, int32_t *n_out)
+{
+    HostOut *ho = c->h_out;
+    if (ho->slide != c->host_t + 1) {
+        set_error("device slide counter out of step with the host");
+        return RTOD_ERR_INTERNAL;
+    }
+    if (ho->err != 0) {
+        set_error("device consistency check failed (code " + std::to_string(ho->err) + ")");
+        c->host_t++;
+        return RTOD_ERR_INTERNAL;
+    }
+    const int n = ho->n_out;
+    if (n < 0 || n > c->W) {
+        set_error("outlier count out of range");
+        return RTOD_ERR_INTERNAL;
+    }
+    if (outs && n)
+        std::memcpy(outs, ho->list, sizeof(int32_t) * (size_t)n);
+    if (n_out)
+        *n_out = n;
+    c->host_t++;
+    c->cand_last = ho->n_cand;
+    c->cand_total += ho->n_cand;
+    c->tests_last = (int64_t)ho->pair_tests;
+    c->tests_total += (int64_t)ho->pair_tests;
+    c->out_last = n;
+    c->out_total += n;
+    return RTOD_OK;
+}
+
+void accumulate_kernel_times(rtod_ctx *c, double *phase)
+{
+    float ms[5];
+    for (int i = 0; i < 5; ++i) {
+        ms[i] = 0.f;
+        (void)hipEventElapsedTime(&ms[i], c->ev[i], c->ev[i + 1]);
+    }
+    c->ksum[kKIngest] += ms[0];
+    c->kcnt[kKIngest]++;
+    if (c->d.cpb > 1) {
+        c->ksum[kKMergePass] += ms[1];
+        c->kcnt[kKMergePass]++;
+    }
+    c->ksum[kKMergeWindow] += ms[2];
+    c->kcnt[kKMergeWindow]++;
+    c->ksum[kKClassify] += ms[3];
+    c->kcnt[kKClassify]++;
+    c->ksum[kKDetect] += ms[4];
+    c->kcnt[kKDetect]++;
+    if (phase) {
+        phase[0] = (double)ms[0] + ms[1] + ms[2] + ms[3]; // prepare cell
+        phase[1] = 0.0;                                   // build index (stencil: none)
+        phase[2] = (double)ms[4];                         // detect (+ compaction tail)
+    }
+}
+
+rtod_status slide_impl(rtod_ctx *c, const float *new_pts, int32_t *outs, int32_t *n_out,
+                       rtod_times *times)
+{
+    const double t_entry = now_ms();
+    if (!c->primed) {
+        set_error("rtod_slide before rtod_prime");
+        return RTOD_ERR_STATE;
+    }
+    if (!new_pts || !outs || !n_out) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (c->staged) {
+        set_error("a phase-API slide is in progress");
+        return RTOD_ERR_STATE;
+    }
+    RTOD_HIP(hipSetDevice(c->device));
+    const size_t nf = (size_t)c->S * c->dim;
+    std::memcpy(c->h_in + (size_t)(c->host_t & 1) * nf, new_pts, nf * sizeof(float));
+    const double t_h2d = now_ms();
+    double phase[3] = {0, 0, 0};
+    const bool timed = (c->cfg.flags & RTOD_FLAG_PHASE_TIMES) != 0;
+    if (timed) {
+        launch_slide(c, c->stream, c->ev);
+        RTOD_HIP(hipGetLastError());
+        RTOD_HIP(hipStreamSynchronize(c->stream));
+        accumulate_kernel_times(c, phase);
+    } else if (!(c->cfg.flags & RTOD_FLAG_NO_GRAPH)) {
+        RTOD_TRY(build_graph(c));
+        RTOD_HIP(hipGraphLaunch(c->gexec, c->stream));
+        RTOD_HIP(hipStreamSynchronize(c->stream));
+    } else {
+        launch_slide(c, c->stream, nullptr);
+        RTOD_HIP(hipGetLastError());
+        RTOD_HIP(hipStreamSynchronize(c->stream));
+    }
+    const double t_dev = now_ms();
+    RTOD_TRY(collect(c, outs, n_out));
+    const double t_end = now_ms();
+    if (times) {
+        times->copy_h2d = t_h2d - t_entry;
+        times->prepare_cell = phase[0];
+        times->build_index = phase[1];
+        times->detect_outlier = phase[2];
+        times->copy_d2h = t_end - t_dev;
+        times->total = t_end - t_entry;
+    }
+    return RTOD_OK;
+}
+
+} // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+RTOD_API rtod_status rtod_create(const rtod_config *cfg, rtod_ctx **out)
+{
+    if (!out) {
+        set_error("out is NULL");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    rtod_ctx *c = nullptr;
+    rtod_status s;
+    try {
+        s = create_impl(cfg, &c);
+    } catch (...) {
+        s = RTOD_ERR_INTERNAL;
+    }
+    if (s != RTOD_OK) {
+        if (c)
+            rtod_destroy(c);
+        return s;
+    }
+    *out = c;
+    return RTOD_OK;
+}
+
+RTOD_API rtod_status rtod_prime(rtod_ctx *ctx, const float *pts, int64_t n_pts)
+{
+    if (!ctx) {
+        set_error("ctx is NULL");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    try {
+        return prime_impl(ctx, pts, n_pts);
+    } catch (...) {
+        return RTOD_ERR_INTERNAL;
+    }
+}
+
+RTOD_API rtod_status rtod_slide(rtod_ctx *ctx, const float *new_pts, int32_t *outlier_slots,
+                                int32_t *n_outliers, rtod_times *times)
+{
+    if (!ctx) {
+        set_error("ctx is NULL");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    try {
+        return slide_impl(ctx, new_pts, outlier_slots, n_outliers, times);
+    } catch (...) {
+        return RTOD_ERR_INTERNAL;
+    }
+}
+
+RTOD_API rtod_status rtod_copy_new_points(rtod_ctx *c, const float *new_pts)
+{
+    if (!c || !new_pts) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (!c->primed || c->phase != 0) {
+        set_error("copy_new_points out of order");
+        return RTOD_ERR_STATE;
+    }
+    const size_t nf = (size_t)c->S * c->dim;
+    std::memcpy(c->h_in + (size_t)(c->host_t & 1) * nf, new_pts, nf * sizeof(float));
+    c->staged = true;
+    c->phase = 1;
+    return RTOD_OK;
+}
+
+RTOD_API rtod_status rtod_prepare_cell(rtod_ctx *c)
+{
+    if (!c) {
+        set_error("ctx is NULL");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (c->phase != 1) {
+        set_error("prepare_cell out of order");
+        return RTOD_ERR_STATE;
+    }
+    RTOD_HIP(hipSetDevice(c->device));
+    launch_ingest_dim(c, c->stream, kModeSlide, nullptr, c->d.cpb, c->d.cpb == 1 ? 1 : 0);
+    if (c->d.cpb > 1)
+        launch_batch_merges(c, c->stream, kModeSlide, c->S);
+    launch_merge_window(c, c->stream);
+    launch_classify(c, c->stream);
+    RTOD_HIP(hipGetLastError());
+    RTOD_HIP(hipStreamSynchronize(c->stream));
+    c->phase = 2;
+    return RTOD_OK;
+}
+
+RTOD_API rtod_status rtod_build_index(rtod_ctx *c)
+{
+    if (!c) {
+        set_error("ctx is NULL");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (c->phase != 2) {
+        set_error("build_index out of order");
+        return RTOD_ERR_STATE;
+    }
+    // cell stencil: the cell-sorted window and its sample table were built by
+    // the merge in prepare_cell; nothing left to do
+    c->phase = 3;
+    return RTOD_OK;
+}
+
+RTOD_API rtod_status rtod_detect_outlier(rtod_ctx *c)
+{
+    if (!c) {
+        set_error("ctx is NULL");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (c->phase != 3) {
+        set_error("detect_outlier out of order");
+        return RTOD_ERR_STATE;
+    }
+    RTOD_HIP(hipSetDevice(c->device));
+    launch_detect(c, c->stream);
+    RTOD_HIP(hipGetLastError());
+    RTOD_HIP(hipStreamSynchronize(c->stream));
+    c->phase = 4;
+    return RTOD_OK;
+}
+
+RTOD_API rtod_status rtod_copy_outlier(rtod_ctx *c, int32_t *outs, int32_t *n_out)
+{
+    if (!c || !outs || !n_out) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (c->phase != 4) {
+        set_error("copy_outlier out of order");
+        return RTOD_ERR_STATE;
+    }
+    rtod_status s = collect(c, outs, n_out);
+    c->phase = 0;
+    c->staged = false;
+    return s;
+}
+
+RTOD_API rtod_status rtod_get_stats(const rtod_ctx *c, rtod_stats *o)
+{
+    if (!c || !o) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    std::memset(o, 0, sizeof(*o));
+    o->slides = c->host_t;
+    o->candidates_last = c->cand_last;
+    o->candidates_total = c->cand_total;
+    o->pair_tests_last = c->tests_last;
+    o->pair_tests_total = c->tests_total;
+    o->outliers_last = c->out_last;
+    o->outliers_total = c->out_total;
+    o->device_bytes = c->dev_bytes;
+    o->host_pinned_bytes = c->pinned_bytes;
+    o->index_kind = RTOD_INDEX_CELL_STENCIL;
+    o->sched_kind = RTOD_SCHED_PLAIN;
+    return RTOD_OK;
+}
+
+RTOD_API rtod_status rtod_kernel_times(const rtod_ctx *c, const char **names, double *avg_ms,
+                                       int64_t *launches, int32_t *n)
+{
+    if (!c || !n) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    const int cap = *n;
+    int k = 0;
+    for (int i = 0; i < kKCount && k < cap; ++i) {
+        if (names)
+            names[k] = kKernelNames[i];
+        if (avg_ms)
+            avg_ms[k] = c->kcnt[i] ? c->ksum[i] / (double)c->kcnt[i] : 0.0;
+        if (launches)
+            launches[k] = c->kcnt[i];
+        ++k;
+    }
+    *n = k;
+    return RTOD_OK;
+}
+
+RTOD_API rtod_status rtod_reset_kernel_times(rtod_ctx *c)
+{
+    if (!c) {
+        set_error("ctx is NULL");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    for (int i = 0; i < kKCount; ++i) {
+        c->ksum[i] = 0.0;
+        c->kcnt[i] = 0;
+    }
+    return RTOD_OK;
+}
+
+RTOD_API void rtod_destroy(rtod_ctx *c)
+{
+    if (!c)
+        return;
+    (void)hipSetDevice(c->device);
+    if (c->stream)
+        (void)hipStreamSynchronize(c->stream);
+    if (c->gexec)
+        (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph)
+        (void)hipGraphDestroy(c->graph);
+    for (auto &e : c->ev)
+        if (e)
+            (void)hipEventDestroy(e);
+    for (void *p : c->dev_allocs)
+        (void)hipFree(p);
+    if (c->h_in)
+        (void)hipHostFree(c->h_in);
+    if (c->h_out)
+        (void)hipHostFree(c->h_out);
+    if (c->stream)
+        (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+RTOD_API const char *rtod_status_str(rtod_status s)
+{
+    switch (s) {
+    case RTOD_OK:
+        return "ok";
+    case RTOD_ERR_INVALID_ARGUMENT:
+        return "invalid argument";
+    case RTOD_ERR_UNSUPPORTED:
+        return "unsupported";
+    case RTOD_ERR_STATE:
+        return "call out of order";
+    case RTOD_ERR_HIP:
+        return "HIP runtime error";
+    case RTOD_ERR_NOMEM:
+        return "out of memory";
+    case RTOD_ERR_INTERNAL:
+        return "internal error";
+    }
+    return "unknown status";
+}
+
+RTOD_API const char *rtod_last_error(void) { return g_last_error.c_str(); }
+
+RTOD_API const char *rtod_version(void) { return "rtod-mi355x 0.1 (gfx950)"; }
+
+} // extern "C"

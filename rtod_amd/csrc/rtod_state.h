@@ -1,0 +1,71 @@
+// rtod_state.h -- device-resident state of one detector context.
+//
+// HBM layout (W window, S slide, d dims, m = W/S):
+//   ring      f32 [d][W]        window coordinates by ring slot (SoA), updated
+//                               in place each slide (reference: params.points,
+//                               optixScan.cpp:232-238, :1053-1057)
+//   akey/aslot/axyz [2]         the window sorted by (cell key, arrival): the
+//                               cell-stencil index; ping-pong by slide parity
+//   bkey/bslot [(m+1)*S]        each batch of S arrivals sorted by cell key,
+//                               kept until it expires m slides later
+//   tkey/tslot [2][max(W,S)]    scratch runs for chunked sorts
+//   samples  u64 [W/stride]     every stride-th key of the sorted window
+//   cand     i32 [W]            candidate positions (sorted-window index)
+//   bitmap   u32 [W/32]         outlier flags by ring slot
+//   ctl                         slide counter, candidate count, grid ticket
+// Pinned host memory, device-mapped:
+//   h_in  f32 [2][S*d]          new slide staging (row-major, as given)
+//   h_out                       outlier count + ascending slot list + counters
+#pragma once
+
+#include <stdint.h>
+
+namespace rtod {
+
+struct Ctl {
+    uint64_t t;            // slides completed since prime
+    int32_t n_cand;        // candidates of the current slide
+    uint32_t ticket;       // last-workgroup ticket of the detect kernel
+    unsigned long long pair_tests;
+    int32_t err;
+    int32_t pad;
+};
+
+struct HostOut {
+    int32_t n_out;
+    int32_t n_cand;
+    int64_t slide;         // t + 1 after the slide's detect kernel
+    unsigned long long pair_tests;
+    int32_t err;
+    int32_t pad;
+    int32_t list[1];       // W entries follow
+};
+
+struct LbvhState;
+
+struct Dev {
+    int dim, W, S, m, K;
+    int cpb;          // sort chunks per batch
+    int nsamp;        // samples in the sorted-window sample table
+    int stride_log2;  // sample stride
+    float r2;
+    double inv_c;     // 1 / cell side
+    double Rp;        // R (1 + 1e-5): stencil half-width
+    float *ring;
+    uint64_t *akey[2];
+    int32_t *aslot[2];
+    float *axyz[2];
+    uint64_t *bkey;
+    int32_t *bslot;
+    uint64_t *tkey[2];
+    int32_t *tslot[2];
+    uint64_t *samples;
+    int32_t *cand;
+    int32_t *cand_cnt;  // per-candidate counts (capped), for stats/learning
+    uint32_t *bitmap;
+    Ctl *ctl;
+    const float *h_in;
+    HostOut *h_out;
+};
+
+} // namespace rtod

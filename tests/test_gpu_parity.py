@@ -1,0 +1,208 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle.
+
+Bar: bit-exact outlier sets on every slide (ring slots, ascending).  Oracles:
+brute force (small windows), the grid oracle (same algorithm, any size) and
+the 1-D sorted-window oracle (independent method, full C2 size).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import rtod_amd as R
+
+pytestmark = pytest.mark.gpu
+
+
+def case(name):
+    print(f"[case] {name}", flush=True)
+
+
+def assert_same(got, want, name):
+    assert len(got) == len(want), f"{name}: {len(got)} slides vs {len(want)}"
+    for t, (g, w) in enumerate(zip(got, want), 1):
+        if not np.array_equal(g, w):
+            only_g = sorted(set(g.tolist()) - set(w.tolist()))[:8]
+            only_w = sorted(set(w.tolist()) - set(g.tolist()))[:8]
+            raise AssertionError(f"{name}: slide {t}: gpu-only {only_g} oracle-only {only_w}")
+
+
+def test_small_walk_vs_brute():
+    case("walk1d W=2000 S=100 K=10")
+    W, S, T = 2000, 100, 40
+    xs = O.gen_walk(5, W + T * S)
+    assert_same(R.run_stream(xs, W, S, T, 0.45, 10), O.run_stream(xs, W, S, T, 0.45, 10, "brute"),
+                "walk1d small")
+
+
+def test_c1_full_stream_vs_grid_and_brute():
+    wl = O.C1
+    case(wl.name)
+    xs = wl.data()
+    got = R.run_stream(xs, wl.W, wl.S, wl.T, wl.R, wl.K)
+    assert_same(got, O.run_stream(xs, wl.W, wl.S, wl.T, wl.R, wl.K, "grid"), "C1 vs grid")
+    assert_same(got[:20], O.run_stream(xs, wl.W, wl.S, 20, wl.R, wl.K, "brute"), "C1 vs brute")
+
+
+def test_c2_vs_sorted1d_and_grid():
+    wl = O.C2
+    case(wl.name)
+    xs = wl.data()
+    T = 40
+    got = R.run_stream(xs, wl.W, wl.S, T, wl.R, wl.K)
+    assert_same(got, O.run_stream(xs, wl.W, wl.S, T, wl.R, wl.K, "grid"), "C2 vs grid")
+    assert_same(got[:10], O.run_stream(xs, wl.W, wl.S, 10, wl.R, wl.K, "sorted1d"),
+                "C2 vs sorted1d")
+
+
+def test_c3_vs_grid_and_brute():
+    wl = O.C3
+    case(wl.name)
+    xs = wl.data()
+    T = 120
+    got = R.run_stream(xs, wl.W, wl.S, T, wl.R, wl.K)
+    assert_same(got, O.run_stream(xs, wl.W, wl.S, T, wl.R, wl.K, "grid"), "C3 vs grid")
+    assert_same(got[:5], O.run_stream(xs, wl.W, wl.S, 5, wl.R, wl.K, "brute"), "C3 vs brute")
+
+
+@pytest.mark.parametrize("kind", ["walk", "gmm"])
+def test_2d_vs_brute(kind):
+    case(f"2-D {kind}")
+    W, S, T = 3000, 150, 20
+    xs = O.gen_walk(21, W + T * S, 2) if kind == "walk" else O.gen_gmm(22, W + T * S, 2)
+    Rr = 0.3 if kind == "walk" else 1.5
+    assert_same(R.run_stream(xs, W, S, T, Rr, 20), O.run_stream(xs, W, S, T, Rr, 20, "brute"),
+                f"2-D {kind}")
+
+
+@pytest.mark.parametrize("W,S,K", [(1000, 1000, 5), (999, 1, 3), (4096, 64, 4096),
+                                   (5000, 2500, 1), (40000, 20000, 50)])
+def test_shapes_vs_oracle(W, S, K):
+    case(f"shape W={W} S={S} K={K}")
+    T = 3 if S > 1 else 50
+    xs = O.gen_walk(31, W + T * S)
+    want = O.run_stream(xs, W, S, T, 0.45, K, "sorted1d")
+    assert_same(R.run_stream(xs, W, S, T, 0.45, K), want, f"W={W} S={S} K={K}")
+
+
+def test_lattice_closed_form():
+    """x_i = i*h, h = 0.25, R = 0.5 exactly: interior points have exactly
+    2*R/h + 1 = 5 neighbours (<= includes distance R); K=4 flags only the
+    window's edge-free lattice ends; K=5 flags every point."""
+    case("lattice")
+    h, W, S, T = 0.25, 256, 64, 6
+    xs = (np.arange(W + T * S, dtype=np.float64) * h).astype(np.float32).reshape(-1, 1)
+    for K in (4, 5):
+        got = R.run_stream(xs, W, S, T, 0.5, K)
+        want = O.run_stream(xs, W, S, T, 0.5, K, "brute")
+        assert_same(got, want, f"lattice K={K}")
+        if K == 5:
+            assert all(len(g) == W for g in got)
+        else:  # the two lowest and two highest values of each window
+            assert all(len(g) == 4 for g in got)
+
+
+def test_duplicates_and_far_points():
+    case("duplicates")
+    W, S, T, K = 1200, 100, 12, 30
+    rng = np.random.default_rng(7)
+    xs = rng.integers(0, 40, size=(W + T * S, 3)).astype(np.float32) * 0.25
+    xs[::97] += 1000.0  # isolated far points: always outliers
+    got = R.run_stream(xs, W, S, T, 0.3, K)
+    assert_same(got, O.run_stream(xs, W, S, T, 0.3, K, "brute"), "duplicates")
+
+
+def test_nonfinite_points():
+    """NaN/inf points have no neighbours, themselves included (d2 is NaN or
+    inf), so they are outliers and count for nobody -- as in brute force."""
+    case("nonfinite")
+    W, S, T, K = 800, 100, 6, 5
+    xs = O.gen_walk(41, W + T * S, 2)
+    xs[17] = [np.nan, 1.0]
+    xs[300] = [np.inf, 2.0]
+    xs[905] = [-np.inf, -np.inf]
+    xs[1200] = [np.nan, np.nan]
+    got = R.run_stream(xs, W, S, T, 0.45, K)
+    assert_same(got, O.run_stream(xs, W, S, T, 0.45, K, "brute"), "nonfinite")
+
+
+def test_huge_coordinates_clamped_cells():
+    """Coordinates beyond the cell-key range land in clamped cells, which are
+    never fast-accepted; results stay exact."""
+    case("clamped")
+    W, S, T, K = 600, 60, 5, 3
+    rng = np.random.default_rng(9)
+    xs = np.concatenate([rng.normal(0, 1, (W + T * S, 1)),
+                         rng.normal(0, 1, (W + T * S, 1))], 1).astype(np.float32)
+    xs[::3, 0] += np.float32(3e9)
+    got = R.run_stream(xs, W, S, T, 0.5, K)
+    assert_same(got, O.run_stream(xs, W, S, T, 0.5, K, "brute"), "clamped")
+
+
+def test_phase_api_equals_slide():
+    case("phase api")
+    W, S, T, K = 5000, 250, 10, 20
+    xs = O.gen_gmm(51, W + T * S, 3)
+    want = R.run_stream(xs, W, S, T, 1.9, K)
+    with R.Detector(3, W, S, 1.9, K) as det:
+        det.prime(xs[:W])
+        got = []
+        for t in range(1, T + 1):
+            det.copy_new_points(xs[W + (t - 1) * S: W + t * S])
+            det.prepare_cell()
+            det.build_index()
+            det.detect_outlier()
+            got.append(det.copy_outlier())
+        with pytest.raises(R.RtodError):
+            det.detect_outlier()  # out of order
+    assert_same(got, want, "phase api")
+
+
+@pytest.mark.parametrize("flags", [R.FLAG_NO_GRAPH, R.FLAG_PHASE_TIMES])
+def test_launch_modes_agree(flags):
+    case(f"flags={flags}")
+    wl = O.C1
+    xs = wl.data()
+    T = 30
+    want = O.run_stream(xs, wl.W, wl.S, T, wl.R, wl.K, "grid")
+    with R.Detector(1, wl.W, wl.S, wl.R, wl.K, flags=flags) as det:
+        det.prime(xs[:wl.W])
+        got = [det.slide(xs[wl.W + (t - 1) * wl.S: wl.W + t * wl.S]) for t in range(1, T + 1)]
+        if flags & R.FLAG_PHASE_TIMES:
+            kt = det.kernel_times()
+            assert kt["detect"][1] == T and kt["detect"][0] > 0
+            assert det.times.total > 0
+    assert_same(got, want, f"flags={flags}")
+
+
+def test_reprime_restarts_stream():
+    case("reprime")
+    W, S, T, K = 2000, 200, 5, 10
+    xs = O.gen_walk(61, W + T * S)
+    ys = O.gen_walk(62, W + T * S)
+    with R.Detector(1, W, S, 0.45, K) as det:
+        det.prime(xs[:W])
+        for t in range(1, 3):
+            det.slide(xs[W + (t - 1) * S: W + t * S])
+        det.prime(ys[:W])
+        got = [det.slide(ys[W + (t - 1) * S: W + t * S]) for t in range(1, T + 1)]
+    assert_same(got, O.run_stream(ys, W, S, T, 0.45, K, "brute"), "reprime")
+
+
+def test_stats_counters():
+    case("stats")
+    wl = O.C1
+    xs = wl.data()
+    g = O.Grid(1, wl.W, wl.S, wl.R, wl.K)
+    g.prime(xs[:wl.W])
+    with R.Detector(1, wl.W, wl.S, wl.R, wl.K) as det:
+        det.prime(xs[:wl.W])
+        for t in range(1, 6):
+            new = xs[wl.W + (t - 1) * wl.S: wl.W + t * wl.S]
+            out = det.slide(new)
+            g.slide(new)
+        st = det.stats()
+    assert st["slides"] == 5
+    assert st["outliers_last"] == len(out)
+    # same cells (same key function) -> same candidate set size
+    assert st["candidates_last"] == g.stats()["candidates"]
+    assert st["device_bytes"] > 0 and st["host_pinned_bytes"] > 0

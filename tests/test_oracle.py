@@ -1,0 +1,253 @@
+"""Pins the CPU oracle (CPU only).
+
+Every check is chosen so that a plausible mistake fails it: a `<` for `<=`
+(lattices with R exactly on a lattice distance), a forgotten self count
+(K = count thresholds on lattices), a contracted or double-precision predicate
+(exact rounding of the FMA chain), a wrong ring slot (sliding vs from-scratch
+windows, golden slot lists), a wrong cell rule (grid vs brute force vs the
+independent sorted-window method on clustered, duplicated, clamped and
+non-finite data).
+"""
+import json
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+# ---------------------------------------------------------------- predicate
+def _round_f32(x: Fraction) -> np.float32:
+    """Correct round-to-nearest-even of an exact rational to float32."""
+    f = np.float32(float(x))
+    best = None
+    for c in (np.nextafter(f, np.float32(-np.inf)), f, np.nextafter(f, np.float32(np.inf))):
+        err = abs(Fraction(float(c)) - x)
+        key = (err, int(np.float32(c).view(np.uint32)) & 1)
+        if best is None or key < best[0]:
+            best = (key, c)
+    return np.float32(best[1])
+
+
+def _d2_exact(p, q):
+    p = [np.float32(v) for v in p]
+    q = [np.float32(v) for v in q]
+    dx = np.float32(p[0] - q[0])
+    d2 = _round_f32(Fraction(float(dx)) ** 2)
+    for a in range(1, len(p)):
+        dy = np.float32(p[a] - q[a])
+        d2 = _round_f32(Fraction(float(dy)) ** 2 + Fraction(float(d2)))
+    return d2
+
+
+def test_predicate_rounds_like_fma_chain():
+    rng = np.random.default_rng(1)
+    for dim in (1, 2, 3):
+        for _ in range(400):
+            p = rng.normal(0, 1, dim).astype(np.float32) * np.float32(rng.choice([1e-3, 1, 1e3]))
+            q = p + rng.normal(0, 1e-2, dim).astype(np.float32)
+            assert O.d2(p, q) == _d2_exact(p, q)
+
+
+def test_predicate_fma_differs_from_double_sometimes():
+    """The contract is the FP32 FMA chain, not a double sum rounded once:
+    find inputs where the two differ and check the oracle takes the chain."""
+    rng = np.random.default_rng(2)
+    seen = 0
+    for _ in range(20000):
+        p = rng.uniform(-1, 1, 3).astype(np.float32)
+        q = rng.uniform(-1, 1, 3).astype(np.float32)
+        dbl = np.float32(sum((float(a) - float(b)) ** 2 for a, b in zip(p, q)))
+        ex = _d2_exact(p, q)
+        if dbl != ex:
+            seen += 1
+            assert O.d2(p, q) == ex
+    assert seen > 0
+
+
+# ------------------------------------------------------------ closed forms
+def test_lattice_1d_counts_and_le():
+    h, W = 0.25, 64
+    win = (np.arange(W) * h).astype(np.float32).reshape(-1, 1)
+    out, cnt = O.brute(win, 0.5, 100, counts=True)
+    # <= R: j in i-2..i+2 (distance 0.5 == R counts); self included
+    want = np.minimum(np.arange(W), 2) + np.minimum(W - 1 - np.arange(W), 2) + 1
+    assert np.array_equal(cnt, want)
+    assert O.brute(win, 0.5, 4).tolist() == [0, 1, W - 2, W - 1]
+    assert len(O.brute(win, 0.5, 5)) == W
+    # the reference's strict FP64 rule |dx| < R: only i-1..i+1
+    _, cnt_ref = O.brute(win, 0.5, 100, semantics=1, counts=True)
+    want_ref = np.minimum(np.arange(W), 1) + np.minimum(W - 1 - np.arange(W), 1) + 1
+    assert np.array_equal(cnt_ref, want_ref)
+
+
+def test_lattice_3d_cube():
+    n = 6
+    g = np.stack(np.meshgrid(*(np.arange(n),) * 3, indexing="ij"), -1).reshape(-1, 3)
+    win = g.astype(np.float32)
+    # R = 1: self + face neighbours (diagonals are sqrt(2) > 1)
+    _, cnt = O.brute(win, 1.0, 100, counts=True)
+    want = 1 + sum(((g[:, a] > 0).astype(int) + (g[:, a] < n - 1).astype(int)) for a in range(3))
+    assert np.array_equal(cnt, want)
+    interior = np.all((g > 0) & (g < n - 1), axis=1)
+    assert np.array_equal(O.brute(win, 1.0, 6), np.nonzero(~interior)[0])
+
+
+def test_isolated_dense_and_k_ge_w():
+    rng = np.random.default_rng(3)
+    cluster = np.zeros((60, 2), np.float32)
+    far = rng.uniform(100, 200, (5, 2)).astype(np.float32)
+    win = np.concatenate([cluster, far])
+    out = O.brute(win, 0.1, 50)
+    assert out.tolist() == list(range(60, 65))  # cluster of 60 >= K+1 identical points
+    assert len(O.brute(win, 0.1, 65)) == 65     # K >= W: everything
+
+
+# ------------------------------------------------- independent method agree
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("R,K", [(0.1, 5), (0.45, 50), (1.0, 200)])
+def test_sorted1d_equals_brute(seed, R, K):
+    win = O.gen_walk(seed, 3000, 1)
+    a, ca = O.brute(win, R, K, counts=True)
+    b, cb = O.sorted1d(win, R, K, counts=True)
+    assert np.array_equal(a, b)
+    assert np.array_equal(np.minimum(ca, K + 1), cb)
+
+
+def _grid_vs_brute(xs, W, S, T, R, K, nthreads=1):
+    g = O.run_stream(xs, W, S, T, R, K, "grid", nthreads=nthreads)
+    b = O.run_stream(xs, W, S, T, R, K, "brute")
+    for t, (x, y) in enumerate(zip(g, b), 1):
+        assert np.array_equal(x, y), f"slide {t}"
+
+
+@pytest.mark.parametrize("dim,kind,R,K", [(1, "walk", 0.45, 10), (2, "walk", 0.3, 20),
+                                          (3, "walk", 0.3, 20), (2, "gmm", 1.5, 20),
+                                          (3, "gmm", 1.9, 50)])
+def test_grid_equals_brute(dim, kind, R, K):
+    W, S, T = 2400, 120, 25
+    xs = O.gen_walk(7, W + T * S, dim) if kind == "walk" else O.gen_gmm(8, W + T * S, dim)
+    _grid_vs_brute(xs, W, S, T, R, K)
+    _grid_vs_brute(xs, W, S, T, R, K, nthreads=4)
+
+
+def test_grid_edge_data():
+    rng = np.random.default_rng(4)
+    W, S, T = 900, 90, 8
+    xs = rng.integers(0, 12, (W + T * S, 2)).astype(np.float32) * np.float32(0.1)
+    xs[::50] += np.float32(5e9)           # clamped cells
+    xs[7] = [np.nan, 0.0]                 # non-finite
+    xs[333] = [np.inf, np.inf]
+    _grid_vs_brute(xs, W, S, T, 0.15, 12)
+    _grid_vs_brute(xs, W, S, T, 0.15, 1000)
+
+
+def test_grid_shapes():
+    xs = O.gen_walk(9, 6000, 1)
+    for W, S in ((1000, 1000), (999, 1), (1000, 500)):
+        T = min(20, (xs.shape[0] - W) // S)
+        _grid_vs_brute(xs, W, S, T, 0.45, 8)
+
+
+def test_grid_invalid_configs():
+    for args in ((0, 100, 10, 0.5, 5), (1, 100, 30, 0.5, 5), (1, 100, 10, 0.0, 5),
+                 (1, 100, 10, 0.5, 0), (4, 100, 10, 0.5, 5)):
+        with pytest.raises(ValueError):
+            O.Grid(*args)
+
+
+# -------------------------------------------------------------- invariants
+def test_permutation_invariance():
+    rng = np.random.default_rng(5)
+    win = O.gen_gmm(10, 2000, 3)
+    perm = rng.permutation(2000)
+    a = O.brute(win, 1.9, 30)
+    b = O.brute(win[perm], 1.9, 30)
+    assert np.array_equal(np.sort(perm[b]), a)
+
+
+def test_monotone_in_r_and_k():
+    win = O.gen_walk(11, 4000, 1)
+    prev = None
+    for R in (0.1, 0.2, 0.4, 0.8):
+        s = set(O.brute(win, R, 40).tolist())
+        assert prev is None or s <= prev
+        prev = s
+    prev = None
+    for K in (5, 20, 40, 80):
+        s = set(O.brute(win, 0.3, K).tolist())
+        assert prev is None or prev <= s
+        prev = s
+
+
+def test_cells_have_bounded_diameter():
+    """Two points in one unclamped cell always pass the predicate, also at
+    large magnitudes and next to cell boundaries: the dense rule is exact."""
+    rng = np.random.default_rng(12)
+    for dim, R in ((1, 0.45), (2, 0.3), (3, 1.9)):
+        c = R * (1 - 1e-5) / np.sqrt(dim)
+        for scale in (1.0, 1e3, 1e5):
+            k = rng.integers(-1000, 1000, (400, dim)) * scale
+            base = (k * c).astype(np.float64)
+            p = (base + rng.uniform(-1e-6, 1, (400, dim)) * c).astype(np.float32)
+            q = (base + rng.uniform(0, 1 + 1e-6, (400, dim)) * c).astype(np.float32)
+            checked = 0
+            for a, b in zip(p, q):
+                key = O.cell_key(a, R)
+                if key == O.cell_key(b, R) and _unclamped(key, dim):
+                    assert O.d2(a, b) <= np.float32(R) * np.float32(R)
+                    checked += 1
+            assert checked > 0 or scale * 1000 * c > 2 ** (20 if dim == 3 else 30) * c
+
+
+def _unclamped(key, dim):
+    F = 21 if dim == 3 else 31
+    M = (1 << F) - 1
+    return all(0 < (key >> (F * (dim - 1 - a))) & M < M for a in range(dim))
+
+
+def test_cell_key_packing():
+    R = 1.0
+    c = (1 - 1e-5)
+    assert O.cell_key(np.float32([0.5]), R) == 2 ** 30
+    assert O.cell_key(np.float32([-0.5]), R) == 2 ** 30 - 1
+    assert O.cell_key(np.float32([2.5 * c]), R) == 2 ** 30 + 2
+    k3 = O.cell_key(np.float32([0.1, 0.1, -0.1]), R * np.sqrt(3))
+    assert k3 == (2 ** 20 << 42) | (2 ** 20 << 21) | (2 ** 20 - 1)
+    big = O.cell_key(np.float32([3e38]), 1.0)
+    assert big == 2 ** 31 - 1  # clamped to the top field value
+
+
+# ----------------------------------------------------------------- goldens
+def _golden():
+    with open(os.path.join(HERE, "golden", "golden.json")) as f:
+        return json.load(f)
+
+
+def test_generators_match_golden():
+    import hashlib
+
+    g = _golden()["generators"]
+    for wl in (O.C1, O.C2, O.C3):
+        xs = wl.data()
+        assert hashlib.sha256(xs.tobytes()).hexdigest() == g[wl.name]["sha256"], wl.name
+        assert xs.shape == (wl.N, wl.dim)
+
+
+def test_streams_match_golden():
+    gs = _golden()["streams"]
+    for wl in (O.C1, O.C3):
+        gd = gs[wl.name]
+        xs = wl.data()
+        got = O.run_stream(xs, wl.W, wl.S, gd["slides"], wl.R, wl.K, "grid")
+        for t, (a, b) in enumerate(zip(got, gd["outliers"]), 1):
+            assert a.tolist() == b, f"{wl.name} slide {t}"
+
+
+def test_workload_constants():
+    assert O.C1.T == 380 and O.C2.T == 189 and O.C3.T == 1130
+    assert (O.C2.W, O.C2.S, O.C2.K, O.C2.R) == (100_000, 5_000, 50, 0.45)
