@@ -51,9 +51,8 @@ typedef enum rtod_sched_kind {
 } rtod_sched_kind;
 
 /* flags */
-#define RTOD_FLAG_PHASE_TIMES 0x1 /* launch phases one by one between HIP
-                                     events and fill rtod_times per phase  */
-#define RTOD_FLAG_NO_GRAPH 0x2    /* launch kernels directly, no hipGraph  */
+#define RTOD_FLAG_PHASE_TIMES 0x1 /* record HIP events between the slide's
+                                     kernels and fill rtod_times per phase  */
 
 /* Replaces the compile-time DIMENSION/MK/OPTIMIZATION macros and the runtime
  * flags --window --slide --R --K of the reference (optixScan.h:4-28,

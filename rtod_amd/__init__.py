@@ -27,7 +27,7 @@ LIB_PATH = os.path.join(_HERE, "librtod.so")
 
 INDEX_AUTO, INDEX_CELL_STENCIL, INDEX_LBVH = 0, 1, 2
 SCHED_PLAIN, SCHED_LEARNED = 0, 1
-FLAG_PHASE_TIMES, FLAG_NO_GRAPH = 0x1, 0x2
+FLAG_PHASE_TIMES = 0x1
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "unsupported", 3: "call out of order",
           4: "HIP runtime error", 5: "out of memory", 6: "internal error"}

@@ -50,8 +50,10 @@ template <int DIM> struct CellBits {
     static constexpr uint64_t FMASK = (1ull << FIELD) - 1;
 };
 
+// The clamped value fits int32 (|k| <= 2^30), so the conversion is the
+// single native f64 -> i32 instruction.
 template <int DIM>
-__device__ __forceinline__ int64_t cell_coord(double v, double inv_c)
+__device__ __forceinline__ int cell_coord(double v, double inv_c)
 {
     constexpr int B = CellBits<DIM>::B;
     double f = floor(v * inv_c);
@@ -61,7 +63,7 @@ __device__ __forceinline__ int64_t cell_coord(double v, double inv_c)
         f = lo;
     else if (!(f < hi))
         f = hi;
-    return (int64_t)f;
+    return (int)f;
 }
 
 template <int DIM>
@@ -82,9 +84,9 @@ __device__ __forceinline__ uint64_t pack_key(int64_t k0, int64_t k1, int64_t k2)
 template <int DIM>
 __device__ __forceinline__ uint64_t cell_key(const float (&p)[3], double inv_c)
 {
-    int64_t k0 = cell_coord<DIM>((double)p[0], inv_c);
-    int64_t k1 = DIM > 1 ? cell_coord<DIM>((double)p[1], inv_c) : 0;
-    int64_t k2 = DIM > 2 ? cell_coord<DIM>((double)p[2], inv_c) : 0;
+    const int64_t k0 = cell_coord<DIM>((double)p[0], inv_c);
+    const int64_t k1 = DIM > 1 ? cell_coord<DIM>((double)p[1], inv_c) : 0;
+    const int64_t k2 = DIM > 2 ? cell_coord<DIM>((double)p[2], inv_c) : 0;
     return pack_key<DIM>(k0, k1, k2);
 }
 
@@ -156,6 +158,31 @@ __device__ __forceinline__ uint64_t wave_and(uint64_t x)
     for (int d = 1; d < kWave; d <<= 1)
         x &= __shfl_xor(x, d, kWave);
     return x;
+}
+
+// Block-wide copy of n elements global -> LDS with U loads in flight per
+// thread: all U loads of a round are issued before any LDS store, so the
+// copy costs ceil(n / (U * blockDim)) memory round trips instead of one per
+// element per thread.
+template <int U, typename T>
+__device__ __forceinline__ void copy_to_lds(T *__restrict__ s, const T *__restrict__ g, int n)
+{
+    const int nt = blockDim.x;
+    for (int base = threadIdx.x; base < n; base += U * nt) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = base + u * nt;
+            if (i < n)
+                v[u] = g[i];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = base + u * nt;
+            if (i < n)
+                s[i] = v[u];
+        }
+    }
 }
 
 // first index in [lo,hi) with a[i] >= k (hi if none)

@@ -1,11 +1,12 @@
 // rtod_lib.hip -- context management and the C-ABI of include/rtod.h.
 //
 // One context = one window stream on one GPU: its own HIP stream, device
-// arena and pinned staging.  A slide is one hipGraph launch of the plain
-// path's kernels (rtod_plain.h) and one stream synchronisation; the graph is
-// parameter-free because every per-slide quantity (ring offset, ping-pong
-// parity, batch-ring entries, staging buffer) derives from the device-side
-// slide counter that the detect kernel advances.
+// arena and pinned staging.  A slide is four direct kernel launches of the
+// plain path (rtod_plain.h) and one stream synchronisation.  Per-slide
+// quantities (ring offset, ping-pong parity, batch-ring entries, staging
+// buffer) travel as kernel arguments.  Direct launches are used rather than
+// a hipGraph: on ROCm 7.2 / MI355X a graph of four small kernels measured
+// 16 us per launch against 6 us for the same four direct launches.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -46,21 +47,32 @@ enum KernelId { kKIngest = 0, kKMergePass, kKMergeWindow, kKClassify, kKDetect, 
 const char *kKernelNames[kKCount] = {"ingest_sort", "merge_pass", "merge_window", "classify",
                                      "detect"};
 
-// LDS-sort tile shapes, picked by the batch size S
-enum SortShape { kSort256x2 = 0, kSort256x8 = 1, kSort1024x8 = 2 };
-int sort_tile(int shape) { return shape == kSort256x2 ? 512 : shape == kSort256x8 ? 2048 : 8192; }
+// LDS-sort tile shapes (threads x items per thread), picked by S and dim:
+// the smallest tile holding the whole batch, else the largest that fits
+// 160 KB of LDS (coordinates/counters pool, keys, 16-bit values)
+struct SortShape {
+    int nt, it;
+};
+constexpr SortShape kSortShapes[4] = {{256, 2}, {512, 4}, {1024, 4}, {1024, 8}};
+
+int pick_sort_shape(int S, int dim)
+{
+    const int top = dim == 3 ? 2 : 3; // (1024, 8) x 3 floats exceeds LDS
+    for (int i = 0; i <= top; ++i)
+        if (kSortShapes[i].nt * kSortShapes[i].it >= S)
+            return i;
+    return top;
+}
 
 } // namespace
 
 struct rtod_ctx {
     rtod_config cfg{};
     int dim = 1, W = 0, S = 0, m = 0, K = 0;
-    int sort_shape = 0, CH = 0;
+    int sort_shape = 0, CH = 512;
     int detect_blocks = 512;
     int device = 0;
     hipStream_t stream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
     Dev d{};
     float *d_prime = nullptr; // device copy of the initial window
     float *h_in = nullptr;
@@ -105,126 +117,146 @@ template <typename T> rtod_status dalloc(rtod_ctx *c, T **p, size_t n)
             return s_;                                                                            \
     } while (0)
 
-template <int DIM> void launch_ingest(rtod_ctx *c, hipStream_t s, int mode, const float *src,
-                                      int grid, int dst_batch)
+template <int DIM> void launch_ingest(rtod_ctx *c, hipStream_t s, const Step &st, int mode,
+                                      const float *src, int grid, int dst_batch)
 {
+    const dim3 g(grid);
     switch (c->sort_shape) {
-    case kSort256x2:
-        hipLaunchKernelGGL((k_ingest_sort<DIM, 256, 2>), dim3(grid), dim3(256), 0, s, c->d, mode,
-                           src, dst_batch);
+    case 0:
+        hipLaunchKernelGGL((k_ingest_sort<DIM, 256, 2>), g, dim3(256), 0, s, c->d, st, mode, src,
+                           dst_batch);
         break;
-    case kSort256x8:
-        hipLaunchKernelGGL((k_ingest_sort<DIM, 256, 8>), dim3(grid), dim3(256), 0, s, c->d, mode,
+    case 1:
+        hipLaunchKernelGGL((k_ingest_sort<DIM, 512, 4>), g, dim3(512), 0, s, c->d, st, mode, src,
+                           dst_batch);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_ingest_sort<DIM, 1024, 4>), g, dim3(1024), 0, s, c->d, st, mode,
                            src, dst_batch);
         break;
     default:
-        hipLaunchKernelGGL((k_ingest_sort<DIM, 1024, 8>), dim3(grid), dim3(1024), 0, s, c->d,
-                           mode, src, dst_batch);
+        if constexpr (DIM < 3)
+            hipLaunchKernelGGL((k_ingest_sort<DIM, 1024, 8>), g, dim3(1024), 0, s, c->d, st, mode,
+                               src, dst_batch);
         break;
     }
 }
 
-void launch_ingest_dim(rtod_ctx *c, hipStream_t s, int mode, const float *src, int grid,
-                       int dst_batch)
+void launch_ingest_dim(rtod_ctx *c, hipStream_t s, const Step &st, int mode, const float *src,
+                       int grid, int dst_batch)
 {
     if (c->dim == 1)
-        launch_ingest<1>(c, s, mode, src, grid, dst_batch);
+        launch_ingest<1>(c, s, st, mode, src, grid, dst_batch);
     else if (c->dim == 2)
-        launch_ingest<2>(c, s, mode, src, grid, dst_batch);
+        launch_ingest<2>(c, s, st, mode, src, grid, dst_batch);
     else
-        launch_ingest<3>(c, s, mode, src, grid, dst_batch);
+        launch_ingest<3>(c, s, st, mode, src, grid, dst_batch);
+}
+
+void launch_merge_pass(rtod_ctx *c, hipStream_t s, const Step &st, const uint64_t *sk,
+                       const int32_t *ss, const float *sx, size_t sxs, const int32_t *sp,
+                       uint64_t *dk, int32_t *ds, float *dx, size_t dxs, int32_t *dp, int n_total,
+                       int seg_len, int L, int dst_batch, int mode)
+{
+    const dim3 g((n_total + 255) / 256), b(256);
+    if (c->dim == 1)
+        hipLaunchKernelGGL(k_merge_pass<1>, g, b, 0, s, c->d, st, sk, ss, sx, sxs, sp, dk, ds, dx,
+                           dxs, dp, n_total, seg_len, L, dst_batch, mode);
+    else if (c->dim == 2)
+        hipLaunchKernelGGL(k_merge_pass<2>, g, b, 0, s, c->d, st, sk, ss, sx, sxs, sp, dk, ds, dx,
+                           dxs, dp, n_total, seg_len, L, dst_batch, mode);
+    else
+        hipLaunchKernelGGL(k_merge_pass<3>, g, b, 0, s, c->d, st, sk, ss, sx, sxs, sp, dk, ds, dx,
+                           dxs, dp, n_total, seg_len, L, dst_batch, mode);
 }
 
 // merge passes turning chunk-sorted runs of length c->CH into sorted
-// segments of seg_len (the batch), last pass into the batch ring
-void launch_batch_merges(rtod_ctx *c, hipStream_t s, int mode, int n_total)
+// segments of S (the batches), last pass into the batch ring
+void launch_batch_merges(rtod_ctx *c, hipStream_t s, const Step &st, int mode, int n_total)
 {
-    const uint64_t *sk = c->d.tkey[0];
-    const int32_t *ss = c->d.tslot[0];
     int src = 0;
+    const size_t bxs = (size_t)(c->m + 1) * c->S;
     for (int L = c->CH; L < c->S; L *= 2) {
         const bool last = 2 * L >= c->S;
-        uint64_t *dk = last ? c->d.bkey : c->d.tkey[src ^ 1];
-        int32_t *ds = last ? c->d.bslot : c->d.tslot[src ^ 1];
-        hipLaunchKernelGGL(k_merge_pass, dim3((n_total + 255) / 256), dim3(256), 0, s, c->d, sk,
-                           ss, dk, ds, n_total, c->S, L, last ? 1 : 0, mode);
+        launch_merge_pass(c, s, st, c->d.tkey[src], c->d.tslot[src], c->d.txyz[src],
+                          (size_t)c->d.tmpn, nullptr, last ? c->d.bkey : c->d.tkey[src ^ 1],
+                          last ? c->d.bslot : c->d.tslot[src ^ 1],
+                          last ? c->d.bxyz : c->d.txyz[src ^ 1], last ? bxs : (size_t)c->d.tmpn,
+                          nullptr, n_total, c->S, L, last ? 1 : 0, mode);
         src ^= 1;
-        sk = c->d.tkey[src];
-        ss = c->d.tslot[src];
     }
 }
 
-void launch_merge_window(rtod_ctx *c, hipStream_t s)
+void launch_merge_window(rtod_ctx *c, hipStream_t s, const Step &st)
 {
     const int grid = (c->W + 1023) / 1024;
     if (c->dim == 1)
-        hipLaunchKernelGGL(k_merge_window<1>, dim3(grid), dim3(256), 0, s, c->d);
+        hipLaunchKernelGGL(k_merge_window<1>, dim3(grid), dim3(256), 0, s, c->d, st);
     else if (c->dim == 2)
-        hipLaunchKernelGGL(k_merge_window<2>, dim3(grid), dim3(256), 0, s, c->d);
+        hipLaunchKernelGGL(k_merge_window<2>, dim3(grid), dim3(256), 0, s, c->d, st);
     else
-        hipLaunchKernelGGL(k_merge_window<3>, dim3(grid), dim3(256), 0, s, c->d);
+        hipLaunchKernelGGL(k_merge_window<3>, dim3(grid), dim3(256), 0, s, c->d, st);
 }
 
-void launch_classify(rtod_ctx *c, hipStream_t s)
+void launch_classify(rtod_ctx *c, hipStream_t s, const Step &st)
 {
     const int grid = (c->W + 255) / 256;
     if (c->dim == 1)
-        hipLaunchKernelGGL(k_classify<1>, dim3(grid), dim3(256), 0, s, c->d);
+        hipLaunchKernelGGL(k_classify<1>, dim3(grid), dim3(256), 0, s, c->d, st);
     else if (c->dim == 2)
-        hipLaunchKernelGGL(k_classify<2>, dim3(grid), dim3(256), 0, s, c->d);
+        hipLaunchKernelGGL(k_classify<2>, dim3(grid), dim3(256), 0, s, c->d, st);
     else
-        hipLaunchKernelGGL(k_classify<3>, dim3(grid), dim3(256), 0, s, c->d);
+        hipLaunchKernelGGL(k_classify<3>, dim3(grid), dim3(256), 0, s, c->d, st);
 }
 
-void launch_detect(rtod_ctx *c, hipStream_t s)
+void launch_detect(rtod_ctx *c, hipStream_t s, const Step &st)
 {
     const int grid = c->detect_blocks;
     if (c->dim == 1)
-        hipLaunchKernelGGL(k_detect<1>, dim3(grid), dim3(kDetectNT), 0, s, c->d);
+        hipLaunchKernelGGL(k_detect<1>, dim3(grid), dim3(kDetectNT), 0, s, c->d, st);
     else if (c->dim == 2)
-        hipLaunchKernelGGL(k_detect<2>, dim3(grid), dim3(kDetectNT), 0, s, c->d);
+        hipLaunchKernelGGL(k_detect<2>, dim3(grid), dim3(kDetectNT), 0, s, c->d, st);
     else
-        hipLaunchKernelGGL(k_detect<3>, dim3(grid), dim3(kDetectNT), 0, s, c->d);
+        hipLaunchKernelGGL(k_detect<3>, dim3(grid), dim3(kDetectNT), 0, s, c->d, st);
+}
+
+Step make_step(const rtod_ctx *c)
+{
+    Step st;
+    const uint64_t t = (uint64_t)c->host_t;
+    st.t = t;
+    st.u = (int)(t % (uint64_t)c->m);
+    st.par = (int)(t & 1);
+    st.eold = (int)(t % (uint64_t)(c->m + 1));
+    st.enew = (int)(((uint64_t)c->m + t) % (uint64_t)(c->m + 1));
+    st.buf = (int)(t & 1);
+    return st;
 }
 
 // The slide's kernel sequence.  ev != nullptr: record an event before the
-// first kernel and after each phase (prepare: ingest + merges, merge_window,
-// classify; detect).
+// first kernel and after each one (ingest, batch merges, merge_window,
+// classify, detect).
 void launch_slide(rtod_ctx *c, hipStream_t s, hipEvent_t *ev)
 {
+    const Step st = make_step(c);
     if (ev)
         (void)hipEventRecord(ev[0], s);
-    launch_ingest_dim(c, s, kModeSlide, nullptr, c->d.cpb, c->d.cpb == 1 ? 1 : 0);
+    launch_ingest_dim(c, s, st, kModeSlide, nullptr, c->d.cpb, c->d.cpb == 1 ? 1 : 0);
     if (ev)
         (void)hipEventRecord(ev[1], s);
     if (c->d.cpb > 1)
-        launch_batch_merges(c, s, kModeSlide, c->S);
+        launch_batch_merges(c, s, st, kModeSlide, c->S);
     if (ev)
         (void)hipEventRecord(ev[2], s);
-    launch_merge_window(c, s);
+    launch_merge_window(c, s, st);
     if (ev)
         (void)hipEventRecord(ev[3], s);
-    launch_classify(c, s);
+    launch_classify(c, s, st);
     if (ev)
         (void)hipEventRecord(ev[4], s);
-    launch_detect(c, s);
+    launch_detect(c, s, st);
     if (ev)
         (void)hipEventRecord(ev[5], s);
-}
-
-rtod_status build_graph(rtod_ctx *c)
-{
-    if (c->gexec)
-        return RTOD_OK;
-    RTOD_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    launch_slide(c, c->stream, nullptr);
-    hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
-    if (e != hipSuccess) {
-        set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-        return RTOD_ERR_HIP;
-    }
-    RTOD_HIP(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
-    return RTOD_OK;
 }
 
 rtod_status validate(const rtod_config *cfg)
@@ -284,8 +316,8 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out)
     for (auto &e : c->ev)
         RTOD_HIP(hipEventCreate(&e));
 
-    c->sort_shape = c->S <= 512 ? kSort256x2 : c->S <= 2048 ? kSort256x8 : kSort1024x8;
-    c->CH = sort_tile(c->sort_shape);
+    c->sort_shape = pick_sort_shape(c->S, c->dim);
+    c->CH = kSortShapes[c->sort_shape].nt * kSortShapes[c->sort_shape].it;
 
     Dev &d = c->d;
     d.dim = c->dim;
@@ -299,6 +331,11 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out)
         ++sl;
     d.stride_log2 = sl;
     d.nsamp = (int)(((int64_t)c->W + (1ll << sl) - 1) >> sl);
+    int bl = 3;
+    while (((int64_t)c->S + (1ll << bl) - 1) >> bl > 512)
+        ++bl;
+    d.bs_log2 = bl;
+    d.nbs = (int)(((int64_t)c->S + (1ll << bl) - 1) >> bl);
     const float R = cfg->R;
     d.r2 = R * R;
     const double cside = (double)R * (1.0 - 1e-5) / std::sqrt((double)c->dim);
@@ -308,16 +345,22 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out)
     const size_t W = c->W, S = c->S, dim = c->dim, m = c->m;
     const size_t tmpn = W > S ? W : S;
     RTOD_TRY(dalloc(c, &d.ring, dim * W));
+    d.tmpn = (int)tmpn;
     for (int p = 0; p < 2; ++p) {
         RTOD_TRY(dalloc(c, &d.akey[p], W));
         RTOD_TRY(dalloc(c, &d.aslot[p], W));
         RTOD_TRY(dalloc(c, &d.axyz[p], dim * W));
+        RTOD_TRY(dalloc(c, &d.apos[p], W));
         RTOD_TRY(dalloc(c, &d.tkey[p], tmpn));
         RTOD_TRY(dalloc(c, &d.tslot[p], tmpn));
+        RTOD_TRY(dalloc(c, &d.txyz[p], dim * tmpn));
     }
     RTOD_TRY(dalloc(c, &d.bkey, (m + 1) * S));
     RTOD_TRY(dalloc(c, &d.bslot, (m + 1) * S));
-    RTOD_TRY(dalloc(c, &d.samples, (size_t)d.nsamp));
+    RTOD_TRY(dalloc(c, &d.bsamp, (m + 1) * (size_t)d.nbs));
+    RTOD_TRY(dalloc(c, &d.bxyz, dim * (m + 1) * S));
+    RTOD_TRY(dalloc(c, &d.samples[0], (size_t)d.nsamp));
+    RTOD_TRY(dalloc(c, &d.samples[1], (size_t)d.nsamp));
     RTOD_TRY(dalloc(c, &d.cand, W));
     RTOD_TRY(dalloc(c, &d.cand_cnt, W));
     RTOD_TRY(dalloc(c, &d.bitmap, (W + 31) / 32));
@@ -339,11 +382,6 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out)
     return RTOD_OK;
 }
 
-template <int DIM> void launch_gather(rtod_ctx *c, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_gather<DIM>, dim3((c->W + 255) / 256), dim3(256), 0, s, c->d, 0);
-}
-
 rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n)
 {
     if (!pts || n != c->W) {
@@ -352,38 +390,47 @@ rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n)
     }
     RTOD_HIP(hipSetDevice(c->device));
     hipStream_t s = c->stream;
+    Step st{};
     RTOD_HIP(hipMemcpyAsync(c->d_prime, pts, sizeof(float) * (size_t)c->W * c->dim,
                             hipMemcpyHostToDevice, s));
     // sort each initial batch (generation b -> batch-ring entry b)
-    launch_ingest_dim(c, s, kModePrime, c->d_prime, c->m * c->d.cpb, c->d.cpb == 1 ? 1 : 0);
+    launch_ingest_dim(c, s, st, kModePrime, c->d_prime, c->m * c->d.cpb, c->d.cpb == 1 ? 1 : 0);
     if (c->d.cpb > 1)
-        launch_batch_merges(c, s, kModePrime, c->W);
-    // merge the m sorted batches into the sorted window A[0]
+        launch_batch_merges(c, s, st, kModePrime, c->W);
+    // merge the m sorted batches (entries 0..m-1 are contiguous) into the
+    // sorted window A[0]; batch positions ride along: the first pass derives
+    // them from the index inside the batch ring, later passes copy them
     {
+        const size_t bxs = (size_t)(c->m + 1) * c->S;
         const uint64_t *sk = c->d.bkey;
         const int32_t *ss = c->d.bslot;
+        const float *sx = c->d.bxyz;
+        size_t sxs = bxs;
+        const int32_t *sp = nullptr;
         int src = 0;
         int L = c->S;
-        do {
+        for (;;) {
             const bool last = 2 * (int64_t)L >= c->W;
             uint64_t *dk = last ? c->d.akey[0] : c->d.tkey[src ^ 1];
             int32_t *ds = last ? c->d.aslot[0] : c->d.tslot[src ^ 1];
-            hipLaunchKernelGGL(k_merge_pass, dim3((c->W + 255) / 256), dim3(256), 0, s, c->d, sk,
-                               ss, dk, ds, c->W, c->W, L, 0, (int)kModePrime);
+            float *dx = last ? c->d.axyz[0] : c->d.txyz[src ^ 1];
+            const size_t dxs = last ? (size_t)c->W : (size_t)c->d.tmpn;
+            int32_t *dp = last ? c->d.apos[0] : ((src ^ 1) ? c->d.apos[1] : c->d.cand_cnt);
+            launch_merge_pass(c, s, st, sk, ss, sx, sxs, sp, dk, ds, dx, dxs, dp, c->W, c->W, L, 0,
+                              (int)kModePrime);
             src ^= 1;
             sk = c->d.tkey[src];
             ss = c->d.tslot[src];
+            sx = c->d.txyz[src];
+            sxs = (size_t)c->d.tmpn;
+            sp = dp;
             if (last)
                 break;
             L *= 2;
-        } while (true);
+        }
     }
-    if (c->dim == 1)
-        launch_gather<1>(c, s);
-    else if (c->dim == 2)
-        launch_gather<2>(c, s);
-    else
-        launch_gather<3>(c, s);
+    hipLaunchKernelGGL(k_window_samples, dim3((c->d.nsamp + 255) / 256), dim3(256), 0, s, c->d,
+                       0);
     RTOD_HIP(hipMemsetAsync(c->d.bitmap, 0, sizeof(uint32_t) * ((c->W + 31) / 32), s));
     RTOD_HIP(hipMemsetAsync(c->d.ctl, 0, sizeof(Ctl), s));
     RTOD_HIP(hipGetLastError());
@@ -480,10 +527,6 @@ rtod_status slide_impl(rtod_ctx *c, const float *new_pts, int32_t *outs, int32_t
         RTOD_HIP(hipGetLastError());
         RTOD_HIP(hipStreamSynchronize(c->stream));
         accumulate_kernel_times(c, phase);
-    } else if (!(c->cfg.flags & RTOD_FLAG_NO_GRAPH)) {
-        RTOD_TRY(build_graph(c));
-        RTOD_HIP(hipGraphLaunch(c->gexec, c->stream));
-        RTOD_HIP(hipStreamSynchronize(c->stream));
     } else {
         launch_slide(c, c->stream, nullptr);
         RTOD_HIP(hipGetLastError());
@@ -588,11 +631,12 @@ RTOD_API rtod_status rtod_prepare_cell(rtod_ctx *c)
         return RTOD_ERR_STATE;
     }
     RTOD_HIP(hipSetDevice(c->device));
-    launch_ingest_dim(c, c->stream, kModeSlide, nullptr, c->d.cpb, c->d.cpb == 1 ? 1 : 0);
+    const Step st = make_step(c);
+    launch_ingest_dim(c, c->stream, st, kModeSlide, nullptr, c->d.cpb, c->d.cpb == 1 ? 1 : 0);
     if (c->d.cpb > 1)
-        launch_batch_merges(c, c->stream, kModeSlide, c->S);
-    launch_merge_window(c, c->stream);
-    launch_classify(c, c->stream);
+        launch_batch_merges(c, c->stream, st, kModeSlide, c->S);
+    launch_merge_window(c, c->stream, st);
+    launch_classify(c, c->stream, st);
     RTOD_HIP(hipGetLastError());
     RTOD_HIP(hipStreamSynchronize(c->stream));
     c->phase = 2;
@@ -626,7 +670,7 @@ RTOD_API rtod_status rtod_detect_outlier(rtod_ctx *c)
         return RTOD_ERR_STATE;
     }
     RTOD_HIP(hipSetDevice(c->device));
-    launch_detect(c, c->stream);
+    launch_detect(c, c->stream, make_step(c));
     RTOD_HIP(hipGetLastError());
     RTOD_HIP(hipStreamSynchronize(c->stream));
     c->phase = 4;
@@ -712,10 +756,6 @@ RTOD_API void rtod_destroy(rtod_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream)
         (void)hipStreamSynchronize(c->stream);
-    if (c->gexec)
-        (void)hipGraphExecDestroy(c->gexec);
-    if (c->graph)
-        (void)hipGraphDestroy(c->graph);
     for (auto &e : c->ev)
         if (e)
             (void)hipEventDestroy(e);

@@ -1,8 +1,8 @@
 // rtod_plain.h -- the plain (cell-stencil) per-slide path.
 //
-// One slide = five device phases, all state in HBM, one host round trip:
+// One slide = four kernel launches, all state in HBM, one host round trip:
 //   k_ingest_sort   new slide from pinned staging into the ring; cell keys;
-//                   LDS-staged stable radix sort of the batch by key
+//                   stable 4-bit LDS radix sort of the batch by relative key
 //   k_merge_pass    (only when S exceeds one LDS tile) merges sorted runs
 //   k_merge_window  incremental rebuild of the cell-sorted window:
 //                   window - expired batch + new batch, by rank computation;
@@ -27,142 +27,335 @@ namespace rtod {
 
 enum SortMode { kModeSlide = 0, kModePrime = 1 };
 
-// batch-ring entry of the batch written by the current slide (generation m+t)
-__device__ __forceinline__ int entry_new(const Dev &d, uint64_t t)
+// largest sample table of a sorted window (entries); the stride grows with W
+constexpr int kMaxSamples = 4096;
+
+// ---------------------------------------------------------------------------
+// Key fields: non-negative per-axis cell coordinates (k + 2^B), most
+// significant axis first.
+// ---------------------------------------------------------------------------
+template <int DIM>
+__device__ __forceinline__ void key_fields(uint64_t key, uint64_t (&f)[3])
 {
-    return (int)((uint64_t(d.m) + t) % uint64_t(d.m + 1));
+    constexpr int F = CellBits<DIM>::FIELD;
+    constexpr uint64_t M = CellBits<DIM>::FMASK;
+    f[0] = f[1] = f[2] = 0;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a)
+        f[a] = (key >> (F * (DIM - 1 - a))) & M;
 }
-// batch-ring entry of the batch expiring at this slide (generation t)
-__device__ __forceinline__ int entry_old(const Dev &d, uint64_t t)
+
+template <int DIM>
+__device__ __forceinline__ uint64_t fields_key(const uint64_t (&f)[3])
 {
-    return (int)(t % uint64_t(d.m + 1));
+    constexpr int F = CellBits<DIM>::FIELD;
+    uint64_t k = 0;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a)
+        k |= f[a] << (F * (DIM - 1 - a));
+    return k;
+}
+
+__device__ __forceinline__ int bit_length(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of the block's NT*IT 64-bit words by bits
+// [sh0, sh0 + nbits), 4 bits per pass, wave-level multi-split.
+// Arrangement: item i of lane l in wave w is element w*64*IT + i*64 + l, in
+// arrival order.  Per item, four ballots give every lane the set of lanes
+// holding its digit; its rank is the running count of that digit in the
+// wave (kept in lane `digit`) plus the lanes below it in the set.  Only the
+// 16 x NW (digit, wave) totals go through LDS, scanned digit-major by one
+// wave, then items scatter to their ranks and are reloaded wave-striped.
+// Out: s_key holds the sorted order (and the registers its striping).
+// ---------------------------------------------------------------------------
+template <int NT, int IT>
+__device__ __forceinline__ void block_radix_sort(uint64_t (&k)[IT], int sh0, int nbits,
+                                                 uint64_t *s_key, int *s_hist)
+{
+    constexpr int NW = NT / kWave;
+    static_assert(16 * NW <= 4 * kWave, "digit table scanned by one wave");
+    const int lane = lane_id(), wid = threadIdx.x / kWave;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (int sh = sh0; sh < sh0 + nbits; sh += 4) {
+        int cnt = 0; // lane d < 16: items with digit d so far in this wave
+        uint32_t rank[IT];
+        uint32_t dig[IT];
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const uint32_t dg = (uint32_t)(k[i] >> sh) & 15u;
+            dig[i] = dg;
+            const uint64_t b0 = __ballot(dg & 1u), b1 = __ballot(dg & 2u);
+            const uint64_t b2 = __ballot(dg & 4u), b3 = __ballot(dg & 8u);
+            const uint64_t m = ((dg & 1u) ? b0 : ~b0) & ((dg & 2u) ? b1 : ~b1) &
+                               ((dg & 4u) ? b2 : ~b2) & ((dg & 8u) ? b3 : ~b3);
+            const int before = __shfl(cnt, (int)dg, kWave);
+            rank[i] = (uint32_t)(before + __popcll(m & lt));
+            const uint64_t md = ((lane & 1) ? b0 : ~b0) & ((lane & 2) ? b1 : ~b1) &
+                                ((lane & 4) ? b2 : ~b2) & ((lane & 8) ? b3 : ~b3);
+            cnt += __popcll(md);
+        }
+        if (lane < 16)
+            s_hist[lane * NW + wid] = cnt;
+        __syncthreads();
+        if (wid == 0) { // exclusive scan of the 16*NW totals, 4 per lane
+            int x[4], sum = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int e = lane * 4 + q;
+                x[q] = e < 16 * NW ? s_hist[e] : 0;
+                sum += x[q];
+            }
+            const int incl = wave_incl_scan(sum);
+            int run = incl - sum;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int e = lane * 4 + q;
+                if (e < 16 * NW)
+                    s_hist[e] = run;
+                run += x[q];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < IT; ++i)
+            s_key[s_hist[dig[i] * NW + wid] + (int)rank[i]] = k[i];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < IT; ++i)
+            k[i] = s_key[wid * kWave * IT + i * kWave + lane];
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
-// k_ingest_sort: one workgroup per LDS tile of CH = NT*IT points.
-// Slide mode: points come from the pinned staging buffer of parity t, go to
-// ring slots (t mod m)*S + j.  Prime mode: segment b of the initial window
-// (slots b*S + j) comes from a device buffer.
-// Stable LSD sort over the key bits that vary inside the tile, one split
-// pass per bit (block scan of the zero flags); ties keep arrival order.
+// k_ingest_sort: one workgroup of NT threads per LDS tile of CH = NT*IT
+// points.  Slide mode: the new slide comes from the pinned staging buffer
+// st.buf (one round of coalesced 16-byte loads), goes to ring slots
+// st.u*S + j.  Prime mode: segment b of the initial window (slots b*S + j)
+// comes from a device buffer.  The batch is sorted by (cell key, arrival):
+// keys are re-expressed relative to the tile's per-axis minimum and packed
+// into exactly the bits their ranges need, so a slide spanning a few dozen
+// cells sorts in two 4-bit passes.
+// Output: keys, slots, coordinates in sorted order (batch entry, or a
+// scratch run when the batch spans several tiles) + the batch sample table.
 // ---------------------------------------------------------------------------
 template <int DIM, int NT, int IT>
-__global__ __launch_bounds__(NT) void k_ingest_sort(Dev d, int mode, const float *src_dev,
+__global__ __launch_bounds__(NT) void k_ingest_sort(Dev d, Step st, int mode, const float *src_dev,
                                                     int dst_batch)
 {
-    constexpr int CH = NT * IT;
+    constexpr int CH = NT * IT, NW = NT / kWave;
+    constexpr int IB = 13; // bits of the tile-local index (CH <= 8192)
+    static_assert(CH <= (1 << IB), "tile index bits");
+    __shared__ __attribute__((aligned(16))) float s_coord[CH * DIM];
     __shared__ uint64_t s_key[CH];
-    __shared__ int32_t s_val[CH];
-    __shared__ int s_scan[NT / kWave];
-    __shared__ uint64_t s_red[2][NT / kWave];
+    __shared__ uint16_t s_val[CH];
+    __shared__ int s_hist[16 * NW];
+    __shared__ uint64_t s_red[2 * 3][NW];
 
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
     const int seg = blockIdx.x / d.cpb;
     const int chunk = blockIdx.x % d.cpb;
-    const uint64_t t = (mode == kModeSlide) ? d.ctl->t : 0;
-    const int slot0 = (mode == kModeSlide) ? (int)(t % uint64_t(d.m)) * d.S : seg * d.S;
-    const float *src = (mode == kModeSlide) ? d.h_in + (size_t)(t & 1) * d.S * DIM
+    const int slot0 = (mode == kModeSlide) ? st.u * d.S : seg * d.S;
+    const float *src = (mode == kModeSlide) ? d.h_in + (size_t)st.buf * d.S * DIM
                                             : src_dev + (size_t)seg * d.S * DIM;
     const int j0 = chunk * CH;
     const int n = min(CH, d.S - j0);
 
-    // stage 1: coalesced loads, ring update, keys into LDS
-    uint64_t lor = 0, land = ~0ull;
-    for (int l = tid; l < CH; l += NT) {
-        uint64_t key = ~0ull;
+    // stage 1: the tile's n*DIM floats into LDS, one round of loads
+    {
+        const float *g = src + (size_t)j0 * DIM;
+        const int nf = n * DIM;
+        if ((((uintptr_t)g) & 15) == 0) {
+            const int n4 = nf >> 2;
+            copy_to_lds<(CH * DIM + 4 * NT - 1) / (4 * NT)>(reinterpret_cast<float4 *>(s_coord),
+                                                            reinterpret_cast<const float4 *>(g),
+                                                            n4);
+            for (int q = (n4 << 2) + tid; q < nf; q += NT)
+                s_coord[q] = g[q];
+        } else {
+            copy_to_lds<(CH * DIM + NT - 1) / NT>(s_coord, g, nf);
+        }
+    }
+    __syncthreads();
+
+    // wave-striped items: ring update, keys, per-axis min/max of key fields
+    uint64_t k[IT];
+    uint64_t fmin[3] = {~0ull, ~0ull, ~0ull}, fmax[3] = {0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int l = wid * kWave * IT + i * kWave + lane;
+        k[i] = ~0ull;
         if (l < n) {
-            const int j = j0 + l;
             float p[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-            for (int a = 0; a < DIM; ++a)
-                p[a] = src[(size_t)j * DIM + a];
+            for (int a = 0; a < DIM; ++a) {
+                p[a] = s_coord[l * DIM + a];
+                d.ring[(size_t)a * d.W + slot0 + j0 + l] = p[a];
+            }
+            k[i] = cell_key<DIM>(p, d.inv_c);
+            uint64_t f[3];
+            key_fields<DIM>(k[i], f);
 #pragma unroll
-            for (int a = 0; a < DIM; ++a)
-                d.ring[(size_t)a * d.W + slot0 + j] = p[a];
-            key = cell_key<DIM>(p, d.inv_c);
-            lor |= key;
-            land &= key;
+            for (int a = 0; a < DIM; ++a) {
+                fmin[a] = f[a] < fmin[a] ? f[a] : fmin[a];
+                fmax[a] = f[a] > fmax[a] ? f[a] : fmax[a];
+            }
         }
-        s_key[l] = key;
     }
-    lor = wave_or(lor);
-    land = wave_and(land);
-    if (lane_id() == 0) {
-        s_red[0][tid / kWave] = lor;
-        s_red[1][tid / kWave] = land;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+        for (int dd = 1; dd < kWave; dd <<= 1) {
+            const uint64_t y0 = __shfl_xor(fmin[a], dd, kWave);
+            const uint64_t y1 = __shfl_xor(fmax[a], dd, kWave);
+            fmin[a] = y0 < fmin[a] ? y0 : fmin[a];
+            fmax[a] = y1 > fmax[a] ? y1 : fmax[a];
+        }
+        if (lane == 0) {
+            s_red[2 * a][wid] = fmin[a];
+            s_red[2 * a + 1][wid] = fmax[a];
+        }
     }
     __syncthreads();
-    uint64_t vary_or = 0, vary_and = ~0ull;
+    int w[3] = {0, 0, 0};
 #pragma unroll
-    for (int w = 0; w < NT / kWave; ++w) {
-        vary_or |= s_red[0][w];
-        vary_and &= s_red[1][w];
+    for (int a = 0; a < DIM; ++a) {
+        uint64_t mn = ~0ull, mx = 0;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+            mn = s_red[2 * a][q] < mn ? s_red[2 * a][q] : mn;
+            mx = s_red[2 * a + 1][q] > mx ? s_red[2 * a + 1][q] : mx;
+        }
+        fmin[a] = mn;
+        w[a] = bit_length(mx - mn);
     }
-    uint64_t varying = (n > 0) ? (vary_or ^ vary_and) : 0;
-
-    // stage 2: blocked arrangement in registers
-    uint64_t k[IT];
-    int32_t v[IT];
+    const int nbits = w[0] + w[1] + w[2];
+    // Sort words (relative key bits << IB) | tile index.  Keys up to 51 bits
+    // sort in one stage.  Wider keys (cells spread beyond 2^51, possible
+    // only in 2-D/3-D with clamped extremes) take two stable LSD stages: the
+    // low 51 bits first, then the high bits, recomputed from the
+    // coordinates, with the stage-1 order as the index.
+    constexpr int KB = 64 - IB;
+    constexpr uint64_t IMASK = (1ull << IB) - 1ull;
+    const bool packed = nbits <= KB;
+    auto rel_of = [&](const float *pc) -> uint64_t {
+        const float p3[3] = {pc[0], DIM > 1 ? pc[1] : 0.f, DIM > 2 ? pc[2] : 0.f};
+        uint64_t f[3];
+        key_fields<DIM>(cell_key<DIM>(p3, d.inv_c), f);
+        uint64_t rel = 0;
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+            rel = (rel << w[a]) | (f[a] - fmin[a]);
+        return rel;
+    };
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
-        k[i] = s_key[tid * IT + i];
-        v[i] = tid * IT + i;
+        const int l = wid * kWave * IT + i * kWave + lane;
+        uint64_t rel = ~0ull; // padding: all ones, sorts last
+        if (l < n) {
+            uint64_t f[3];
+            key_fields<DIM>(k[i], f);
+            rel = 0;
+#pragma unroll
+            for (int a = 0; a < DIM; ++a)
+                rel = (rel << w[a]) | (f[a] - fmin[a]);
+        }
+        k[i] = ((packed ? rel : (rel & ((1ull << KB) - 1ull))) << IB) | (uint64_t)l;
+        s_key[l] = k[i]; // the arrival order stands when no bit varies
     }
     __syncthreads();
-
-    while (varying) {
-        const int bit = __builtin_ctzll(varying);
-        varying &= varying - 1;
-        int zeros = 0;
+    block_radix_sort<NT, IT>(k, IB, packed ? nbits : KB, s_key, s_hist);
+    if (!packed) {
+        // stage-1 order p -> tile index, then sort by the high key bits
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const int e = tid + i * NT;
+            s_val[e] = (uint16_t)(s_key[e] & IMASK);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const int p = wid * kWave * IT + i * kWave + lane;
+            const int l = s_val[p];
+            const uint64_t hi = (l < n) ? (rel_of(&s_coord[l * DIM]) >> KB) : ~0ull;
+            k[i] = (hi << IB) | (uint64_t)p;
+        }
+        __syncthreads();
 #pragma unroll
         for (int i = 0; i < IT; ++i)
-            zeros += ((k[i] >> bit) & 1ull) ? 0 : 1;
-        int total_z;
-        int zp = block_excl_scan<NT>(zeros, s_scan, total_z);
-        int op = total_z + tid * IT - zp;
-#pragma unroll
-        for (int i = 0; i < IT; ++i) {
-            int pos = ((k[i] >> bit) & 1ull) ? op++ : zp++;
-            s_key[pos] = k[i];
-            s_val[pos] = v[i];
-        }
+            s_key[wid * kWave * IT + i * kWave + lane] = k[i];
         __syncthreads();
-#pragma unroll
-        for (int i = 0; i < IT; ++i) {
-            k[i] = s_key[tid * IT + i];
-            v[i] = s_val[tid * IT + i];
-        }
-        __syncthreads();
+        block_radix_sort<NT, IT>(k, IB, nbits - KB, s_key, s_hist);
     }
 
-    // output (padding keys sorted last and dropped)
-    size_t base;
+    // output in sorted order, striped (coalesced)
+    const size_t stride_b = (size_t)(d.m + 1) * d.S;
+    const int e = (mode == kModeSlide) ? st.enew : seg;
+    uint64_t *ok;
+    int32_t *os;
+    float *ox;
+    size_t xs;
     if (dst_batch) {
-        int e = (mode == kModeSlide) ? entry_new(d, t) : seg;
-        base = (size_t)e * d.S + j0;
+        ok = d.bkey + (size_t)e * d.S;
+        os = d.bslot + (size_t)e * d.S;
+        ox = d.bxyz + (size_t)e * d.S;
+        xs = stride_b;
     } else {
-        base = (size_t)seg * d.S + j0;
+        ok = d.tkey[0] + (size_t)seg * d.S + j0;
+        os = d.tslot[0] + (size_t)seg * d.S + j0;
+        ox = d.txyz[0] + (size_t)seg * d.S + j0;
+        xs = (size_t)d.tmpn;
     }
-    uint64_t *ok = dst_batch ? d.bkey : d.tkey[0];
-    int32_t *os = dst_batch ? d.bslot : d.tslot[0];
+    const int bmask = (1 << d.bs_log2) - 1;
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
-        int pos = tid * IT + i;
-        if (pos < n) {
-            ok[base + pos] = k[i];
-            os[base + pos] = slot0 + j0 + v[i];
+        const int pos = tid + i * NT;
+        if (pos >= n)
+            break;
+        const uint64_t word = s_key[pos];
+        int l;
+        uint64_t key;
+        if (packed) {
+            l = (int)(word & IMASK);
+            const uint64_t rel = word >> IB;
+            uint64_t f[3] = {0, 0, 0};
+            int sh = 0;
+#pragma unroll
+            for (int a = DIM - 1; a >= 0; --a) {
+                f[a] = fmin[a] + ((rel >> sh) & ((1ull << w[a]) - 1ull));
+                sh += w[a];
+            }
+            key = fields_key<DIM>(f);
+        } else {
+            l = s_val[word & IMASK];
+            const float p3[3] = {s_coord[l * DIM], DIM > 1 ? s_coord[l * DIM + 1] : 0.f,
+                                 DIM > 2 ? s_coord[l * DIM + 2] : 0.f};
+            key = cell_key<DIM>(p3, d.inv_c);
         }
+        ok[pos] = key;
+        os[pos] = slot0 + j0 + l;
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+            ox[(size_t)a * xs + pos] = s_coord[l * DIM + a];
+        if (dst_batch && (pos & bmask) == 0) // a batch is one tile: j0 == 0
+            d.bsamp[(size_t)e * d.nbs + (pos >> d.bs_log2)] = key;
     }
 }
 
 // ---------------------------------------------------------------------------
 // k_merge_pass: within each segment of seg_len elements, merge sorted runs of
 // length L pairwise into runs of 2L (stable: the earlier run wins ties).
-// dst_batch: write segment s of the result into its batch-ring entry.
+// Carries slots, coordinates (SoA planes, strides sxs/dxs) and optionally
+// batch positions.  dst_batch: segment s goes to batch-ring entry
+// (slide: st.enew, prime: s) and fills its sample table.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_merge_pass(Dev d, const uint64_t *sk, const int32_t *ss,
-                                                    uint64_t *dk, int32_t *ds, int n_total,
-                                                    int seg_len, int L, int dst_batch, int mode)
+template <int DIM>
+__global__ __launch_bounds__(256) void k_merge_pass(Dev d, Step st, const uint64_t *sk,
+                                                    const int32_t *ss, const float *sx, size_t sxs,
+                                                    const int32_t *sp, uint64_t *dk, int32_t *ds,
+                                                    float *dx, size_t dxs, int32_t *dp,
+                                                    int n_total, int seg_len, int L, int dst_batch,
+                                                    int mode)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_total)
@@ -185,131 +378,236 @@ __global__ __launch_bounds__(256) void k_merge_pass(Dev d, const uint64_t *sk, c
     }
     size_t o;
     if (dst_batch) {
-        int e = (mode == kModeSlide) ? entry_new(d, d.ctl->t) : seg;
+        const int e = (mode == kModeSlide) ? st.enew : seg;
         o = (size_t)e * d.S + out;
+        if ((out & ((1 << d.bs_log2) - 1)) == 0)
+            d.bsamp[(size_t)e * d.nbs + (out >> d.bs_log2)] = key;
     } else {
         o = (size_t)base + out;
     }
     dk[o] = key;
     ds[o] = ss[i];
-}
-
-// gather coordinates of a sorted (key, slot) array from the ring
-template <int DIM>
-__global__ __launch_bounds__(256) void k_gather(Dev d, int par)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.W)
-        return;
-    const int s = d.aslot[par][i];
 #pragma unroll
     for (int a = 0; a < DIM; ++a)
-        d.axyz[par][(size_t)a * d.W + i] = d.ring[(size_t)a * d.W + s];
-    if ((i & ((1 << d.stride_log2) - 1)) == 0)
-        d.samples[i >> d.stride_log2] = d.akey[par][i];
+        dx[(size_t)a * dxs + o] = sx[(size_t)a * sxs + i];
+    if (dp) // batch position: from the source, or the index inside its batch
+        dp[o] = sp ? sp[i] : (i % d.S);
+}
+
+// sample table of a freshly built sorted window
+__global__ __launch_bounds__(256) void k_window_samples(Dev d, int par)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < d.nsamp)
+        d.samples[par][s] = d.akey[par][(size_t)s << d.stride_log2];
+}
+
+// Wave-cooperative bound over a sorted table in LDS (n <= 4096 entries): a
+// 64-ary search, every lane probing one position per round.  Returns the
+// number of entries < key (lower) or <= key (upper), in every lane.
+__device__ __forceinline__ int wave_count_lds(const uint64_t *tab, int n, uint64_t key, bool upper)
+{
+    const int lane = lane_id();
+    int lo = 0, len = n; // answer in [lo, lo + len]
+    while (len > 0) {
+        const int step = (len + kWave - 1) / kWave; // probes lane*step
+        const int pos = lo + lane * step;
+        bool below = false;
+        if (lane * step < len) {
+            const uint64_t x = tab[pos];
+            below = upper ? (x <= key) : (x < key);
+        }
+        const int c = __popcll(__ballot(below)); // probes that are below
+        if (c == 0)
+            return lo;
+        // the answer lies after probe c-1 and at or before probe c
+        lo = lo + (c - 1) * step + 1;
+        len = min(step - 1, n - lo);
+        if (step == 1)
+            return lo;
+    }
+    return lo;
 }
 
 // ---------------------------------------------------------------------------
 // k_merge_window: A' = (A - E) + B, all sorted by (key, arrival).
-//   A  current sorted window, E the expiring batch (a subset of A, sorted by
-//   key then slot), B the new batch sorted by key.
-//   For a survivor a at index i: pos = i - #E before a + #B with key < key_a.
-//   For b at index j in B:        pos = j + #A with key <= k_b - #E with key <= k_b.
-// Workgroup c owns the A chunk [i0, i1) and the b with
-// key(A[i0]) <= k_b < key(A[i1]); six leader threads narrow every search
-// range once, the chunk's keys sit in LDS.  Output position multiples of the
-// sample stride also fill the sample table of the cell-stencil index.
+//   A  current sorted window, E the expiring batch (a subset of A), B the new
+//   batch, both batches sorted by key with arrival order on ties.
+//   survivor a at index i:  pos = i - #E before a + #B with key < key_a
+//   b at index j of B:      pos = j + #A with key <= k_b - #E with key <= k_b
+// Workgroup c merges the A chunk [i0, i1) (six chunk-boundary searches, the
+// B and E key ranges the chunk touches staged in LDS) and places an equal
+// share of B, ranked against A and E through their sample tables in LDS,
+// so no workgroup waits on a long tail of new points.  #E before A[i0] is
+// A[i0]'s own batch position when A[i0] expires.  Output positions that are
+// multiples of the sample stride fill A''s sample table.
 // ---------------------------------------------------------------------------
 template <int DIM>
-__global__ __launch_bounds__(256) void k_merge_window(Dev d)
+__global__ __launch_bounds__(256) void k_merge_window(Dev d, Step st)
 {
-    constexpr int NT = 256, IT = 4, T = NT * IT;
+    constexpr int NT = 256, IT = 4, T = NT * IT, CAP = 2048, MAXBS = 512;
     __shared__ uint64_t s_k[T];
-    __shared__ int s_scan[NT / kWave];
-    __shared__ int s_par[8];
+    __shared__ uint64_t s_bs[MAXBS], s_es[MAXBS];
+    __shared__ uint64_t s_as[kMaxSamples];
+    __shared__ uint64_t s_br[CAP];
+    __shared__ int s_rc[IT][NT / kWave];
+    __shared__ int s_res[8];
+    __shared__ uint64_t s_kn;
+    __shared__ int s_pos0, s_slot0;
 
-    const int tid = threadIdx.x;
-    const uint64_t t = d.ctl->t;
+    const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
     const int W = d.W, S = d.S;
-    const int par = (int)(t & 1);
+    const int par = st.par;
     const uint64_t *Ak = d.akey[par];
     const int32_t *As = d.aslot[par];
+    const int32_t *Ap = d.apos[par];
     const float *Ax = d.axyz[par];
     uint64_t *Nk = d.akey[par ^ 1];
     int32_t *Ns = d.aslot[par ^ 1];
+    int32_t *Np = d.apos[par ^ 1];
     float *Nx = d.axyz[par ^ 1];
-    const size_t eo = (size_t)entry_old(d, t) * S, en = (size_t)entry_new(d, t) * S;
-    const uint64_t *Ek = d.bkey + eo;
-    const int32_t *Es = d.bslot + eo;
-    const uint64_t *Bk = d.bkey + en;
-    const int32_t *Bs = d.bslot + en;
-    const int exp_lo = (int)(t % uint64_t(d.m)) * S;
+    uint64_t *Nsamp = d.samples[par ^ 1];
+    const uint64_t *Ek = d.bkey + (size_t)st.eold * S;
+    const uint64_t *Bk = d.bkey + (size_t)st.enew * S;
+    const int32_t *Bs = d.bslot + (size_t)st.enew * S;
+    const float *Bx = d.bxyz + (size_t)st.enew * S;
+    const size_t bxs = (size_t)(d.m + 1) * S;
+    const uint64_t *Esamp = d.bsamp + (size_t)st.eold * d.nbs;
+    const uint64_t *Bsamp = d.bsamp + (size_t)st.enew * d.nbs;
+    const int exp_lo = st.u * S;
     const int i0 = blockIdx.x * T;
     const int i1 = min(W, i0 + T);
     const int n = i1 - i0;
     const bool first = blockIdx.x == 0;
     const bool last = i1 == W;
-    const int smask = (1 << d.stride_log2) - 1;
+    const int sl2 = d.stride_log2, smask = (1 << sl2) - 1;
+    const int bsl = d.bs_log2, nbs = d.nbs;
+    // this workgroup's share of B
+    const int bper = (S + gridDim.x - 1) / gridDim.x;
+    const int j0 = min(S, (int)blockIdx.x * bper), j1 = min(S, j0 + bper);
 
     if (blockIdx.x == 0 && tid == 0)
         d.ctl->n_cand = 0;
 
-    for (int l = tid; l < n; l += NT)
-        s_k[l] = Ak[i0 + l];
-    if (tid < 6) {
-        const uint64_t k0 = Ak[i0];
-        int r = 0;
-        if (tid == 0) {
-            const int s0 = As[i0];
-            const bool ex = s0 >= exp_lo && s0 < exp_lo + S;
-            r = ex ? lower_bound_pair(Ek, Es, 0, S, k0, s0) : upper_bound_u64(Ek, 0, S, k0);
-        } else if (tid == 1) {
-            r = lower_bound_u64(Bk, 0, S, k0);
-        } else if (tid == 2) {
-            r = lower_bound_u64(Bk, 0, S, Ak[i1 - 1]);
-        } else if (tid == 3) {
-            r = last ? S : lower_bound_u64(Bk, 0, S, Ak[i1]);
-        } else if (tid == 4) {
-            r = first ? 0 : lower_bound_u64(Ek, 0, S, k0);
-        } else {
-            r = last ? S : lower_bound_u64(Ek, 0, S, Ak[i1]);
-        }
-        s_par[tid] = r;
-    }
-    __syncthreads();
-    const int e_before = s_par[0];
-    const int b_lo = s_par[1], b_hi = s_par[2];
-    const int own_lo = first ? 0 : b_lo, own_hi = s_par[3];
-    const int e_lo = s_par[4], e_hi = s_par[5];
-
-    // survivors of A (blocked items so the expired prefix is a block scan)
-    int32_t sl[IT];
+    // round 1: the chunk (keys, slots, batch positions, coordinates; striped,
+    // item i of thread t is chunk element i*NT + t), the share of B, and the
+    // sample tables of A, B and E
+    int32_t sl[IT], ps[IT];
+    float cx[IT][3];
+    uint64_t kk[IT];
     bool ex[IT];
-    int nex = 0;
+    int below[IT];
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
-        const int l = tid * IT + i;
+        const int l = i * NT + tid;
         ex[i] = false;
         sl[i] = 0;
+        ps[i] = 0;
+        kk[i] = 0;
+        cx[i][0] = cx[i][1] = cx[i][2] = 0.f;
         if (l < n) {
+            kk[i] = Ak[i0 + l];
             sl[i] = As[i0 + l];
+            ps[i] = Ap[i0 + l];
+#pragma unroll
+            for (int a = 0; a < DIM; ++a)
+                cx[i][a] = Ax[(size_t)a * W + i0 + l];
             ex[i] = sl[i] >= exp_lo && sl[i] < exp_lo + S;
-            nex += ex[i] ? 1 : 0;
         }
     }
-    int tot;
-    int eb = e_before + block_excl_scan<NT>(nex, s_scan, tot);
+    // B share: one item per thread per round (bper <= NT in practice)
+    const int jb = j0 + tid;
+    uint64_t bk = 0;
+    int32_t bsl_ = 0;
+    float bx[3] = {0.f, 0.f, 0.f};
+    if (jb < j1) {
+        bk = Bk[jb];
+        bsl_ = Bs[jb];
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+            bx[a] = Bx[(size_t)a * bxs + jb];
+    }
+    copy_to_lds<2>(s_bs, Bsamp, nbs);
+    copy_to_lds<2>(s_es, Esamp, nbs);
+    copy_to_lds<kMaxSamples / NT>(s_as, d.samples[par], d.nsamp);
+    if (tid == 0) {
+        s_kn = last ? 0 : Ak[i1];
+        s_pos0 = Ap[i0];
+        s_slot0 = As[i0];
+    }
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
-        const int l = tid * IT + i;
-        if (l >= n)
-            continue;
-        if (ex[i]) {
-            ++eb;
-            continue;
+        const int l = i * NT + tid;
+        if (l < n)
+            s_k[l] = kk[i];
+        const uint64_t m = __ballot(ex[i]);
+        below[i] = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0)
+            s_rc[i][wid] = __popcll(m);
+    }
+    __syncthreads();
+
+    // five chunk-boundary searches on the batch sample tables, one wave each
+    // (waves 0,1 take two):  0: #E <= key(A[i0])  1: lb(B, key(A[i0]))
+    //   2: lb(B, key(A[i1-1]))  3: lb(E, key(A[i0]))  4: lb(E, key(A[i1]))
+    const uint64_t k0 = s_k[0], kl = s_k[n - 1], kn = s_kn;
+    const int bstride = 1 << bsl;
+    for (int q = wid; q < 5; q += NT / kWave) {
+        const bool onE = (q == 0 || q >= 3);
+        const bool upper = q == 0;
+        const uint64_t key = (q == 2) ? kl : (q == 4) ? kn : k0;
+        const int c = wave_count_lds(onE ? s_es : s_bs, nbs, key, upper);
+        // finish inside one stride of the batch: lanes probe it in one round
+        int res = 0;
+        if (c > 0) {
+            const int L = ((c - 1) << bsl) + 1, H = min(c << bsl, S);
+            int cnt = 0;
+            for (int r0 = 0; r0 < bstride - 1; r0 += kWave) {
+                const int idx = L + r0 + lane;
+                bool hit = false;
+                if (r0 + lane < bstride - 1 && idx < H) {
+                    const uint64_t x = onE ? Ek[idx] : Bk[idx];
+                    hit = upper ? (x <= key) : (x < key);
+                }
+                cnt += __popcll(__ballot(hit));
+            }
+            res = L + cnt;
         }
-        const uint64_t key = s_k[l];
-        const int rb = lower_bound_u64(Bk, b_lo, b_hi, key);
+        if (lane == 0)
+            s_res[q] = res;
+    }
+    __syncthreads();
+    const bool ex0 = s_slot0 >= exp_lo && s_slot0 < exp_lo + S;
+    const int e_before = ex0 ? s_pos0 : s_res[0];
+    const int b_lo = s_res[1], b_hi = s_res[2];
+    (void)first;
+
+    // stage the B range the survivors search
+    const int nb = b_hi - b_lo;
+    const bool bfit = nb <= CAP;
+    if (bfit)
+        copy_to_lds<8>(s_br, Bk + b_lo, nb);
+    __syncthreads();
+
+    // survivors of A; #E before element l = e_before + expired elements of
+    // the chunk in earlier rows, earlier waves of this row, earlier lanes
+    int row_base = 0;
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        int in_row = 0, row_tot = 0;
+#pragma unroll
+        for (int q = 0; q < NT / kWave; ++q) {
+            in_row += (q < wid) ? s_rc[i][q] : 0;
+            row_tot += s_rc[i][q];
+        }
+        const int eb = e_before + row_base + in_row + below[i];
+        row_base += row_tot;
+        const int l = i * NT + tid;
+        if (l >= n || ex[i])
+            continue;
+        const uint64_t key = kk[i];
+        const int rb = bfit ? b_lo + lower_bound_u64(s_br, 0, nb, key)
+                            : lower_bound_u64(Bk, b_lo, b_hi, key);
         const int np = i0 + l - eb + rb;
         if ((unsigned)np >= (unsigned)W) { // impossible unless state is corrupt
             atomicOr(&d.ctl->err, 1);
@@ -317,39 +615,52 @@ __global__ __launch_bounds__(256) void k_merge_window(Dev d)
         }
         Nk[np] = key;
         Ns[np] = sl[i];
+        Np[np] = ps[i];
 #pragma unroll
         for (int a = 0; a < DIM; ++a)
-            Nx[(size_t)a * W + np] = Ax[(size_t)a * W + i0 + l];
+            Nx[(size_t)a * W + np] = cx[i][a];
         if ((np & smask) == 0)
-            d.samples[np >> d.stride_log2] = key;
+            Nsamp[np >> sl2] = key;
     }
 
-    // new points owned by this chunk
-    for (int j = own_lo + tid; j < own_hi; j += NT) {
-        const uint64_t key = Bk[j];
-        const int32_t s = Bs[j];
-        int lo = 0, hi = n;
-        while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (s_k[mid] <= key)
-                lo = mid + 1;
-            else
-                hi = mid;
+    // this workgroup's share of B: #A <= k_b via A's sample table, #E <= k_b
+    // via E's, each finished by a short search inside one stride
+    for (int j = jb; j < j1; j += NT) {
+        uint64_t key = bk;
+        int32_t sb = bsl_;
+        float px[3] = {bx[0], bx[1], bx[2]};
+        if (j != jb) { // more than NT per workgroup: later rounds load here
+            key = Bk[j];
+            sb = Bs[j];
+#pragma unroll
+            for (int a = 0; a < DIM; ++a)
+                px[a] = Bx[(size_t)a * bxs + j];
         }
-        const int ubA = i0 + lo;
-        const int ubE = upper_bound_u64(Ek, e_lo, e_hi, key);
+        int ca = upper_bound_u64(s_as, 0, d.nsamp, key); // samples <= key
+        int ubA = 0;
+        if (ca > 0) {
+            const int L = ((ca - 1) << sl2) + 1, H = min(ca << sl2, W);
+            ubA = upper_bound_u64(Ak, L, H, key);
+        }
+        int ce = upper_bound_u64(s_es, 0, nbs, key);
+        int ubE = 0;
+        if (ce > 0) {
+            const int L = ((ce - 1) << bsl) + 1, H = min(ce << bsl, S);
+            ubE = upper_bound_u64(Ek, L, H, key);
+        }
         const int np = j + ubA - ubE;
         if ((unsigned)np >= (unsigned)W) {
             atomicOr(&d.ctl->err, 2);
             continue;
         }
         Nk[np] = key;
-        Ns[np] = s;
+        Ns[np] = sb;
+        Np[np] = j;
 #pragma unroll
         for (int a = 0; a < DIM; ++a)
-            Nx[(size_t)a * W + np] = d.ring[(size_t)a * W + s];
+            Nx[(size_t)a * W + np] = px[a];
         if ((np & smask) == 0)
-            d.samples[np >> d.stride_log2] = key;
+            Nsamp[np >> sl2] = key;
     }
 }
 
@@ -357,30 +668,53 @@ __global__ __launch_bounds__(256) void k_merge_window(Dev d)
 // k_classify: a point is fast-accepted iff its cell is unclamped and holds
 // >= K+1 points (reference: cells with num > K are skipped, optixScan.cpp:
 // 987-993).  The cell is a run of equal keys in the sorted window: find the
-// run start s within [i-K, i] and test key[s+K] == key.  Other points are
-// appended to the candidate list, one atomic per wavefront.
+// run start s within [i-K, i] and test key[s+K] == key.  The keys around the
+// tile sit in LDS when K is small.  Other points are appended to the
+// candidate list, one atomic per wavefront.
 // ---------------------------------------------------------------------------
+constexpr int kClassifyNT = 256;
+constexpr int kClassifyHalo = 512;
+
 template <int DIM>
-__global__ __launch_bounds__(256) void k_classify(Dev d)
+__global__ __launch_bounds__(kClassifyNT) void k_classify(Dev d, Step st)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t t = d.ctl->t;
-    const uint64_t *Nk = d.akey[(t & 1) ^ 1];
+    constexpr int NT = kClassifyNT, HALO = kClassifyHalo;
+    __shared__ uint64_t s_t[NT + 2 * HALO];
+    const int i0 = blockIdx.x * NT;
+    const int i = i0 + threadIdx.x;
+    const uint64_t *Nk = d.akey[st.par ^ 1];
     const int W = d.W, K = d.K;
+    const bool tiled = K <= HALO;
+    // tile = keys [i0 - K, i0 + NT + K) clipped to [0, W)
+    const int t0 = max(0, i0 - K), t1 = min(W, i0 + NT + K);
+    if (tiled)
+        copy_to_lds<(NT + 2 * HALO) / NT>(s_t, Nk + t0, t1 - t0);
+    __syncthreads();
     bool cand = false;
     if (i < W) {
-        const uint64_t key = Nk[i];
+        const uint64_t key = tiled ? s_t[i - t0] : Nk[i];
         bool dense = false;
         if (K < W && key_guaranteed<DIM>(key)) {
             int lo = max(0, i - K), hi = i;
-            while (lo < hi) {
-                int mid = (lo + hi) >> 1;
-                if (Nk[mid] == key)
-                    hi = mid;
-                else
-                    lo = mid + 1;
+            if (tiled) {
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_t[mid - t0] == key)
+                        hi = mid;
+                    else
+                        lo = mid + 1;
+                }
+                dense = (lo + K < W) && (s_t[lo + K - t0] == key);
+            } else {
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (Nk[mid] == key)
+                        hi = mid;
+                    else
+                        lo = mid + 1;
+                }
+                dense = (lo + K < W) && (Nk[lo + K] == key);
             }
-            dense = (lo + K < W) && (Nk[lo + K] == key);
         }
         cand = !dense;
     }
@@ -404,7 +738,7 @@ __global__ __launch_bounds__(256) void k_classify(Dev d)
 // (k0[,k1]) fixed with the last axis spanning a contiguous key range.
 // ---------------------------------------------------------------------------
 template <int DIM> struct Stencil {
-    int64_t lo[3], hi[3];
+    int lo[3], hi[3];
     int nrows, n1;
 };
 
@@ -454,16 +788,97 @@ __device__ __forceinline__ void row_keys(const Stencil<DIM> &st, int r, uint64_t
 }
 
 constexpr int kDetectNT = 256;
-constexpr int kMaxSamples = 4096;
+
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Row bounds of a candidate's stencil: lane b < 2*nrows computes bound b
+// (row b>>1, upper if b&1).  The sample table in LDS narrows each bound to
+// one stride; the strides are then finished cooperatively: the 64 lanes walk
+// the (bound, offset) pairs 64 at a time, eight batches of loads in flight,
+// and each batch's ballot is credited to the bound's owner lane.
+template <int DIM>
+__device__ __forceinline__ int row_bounds(const Dev &d, const uint64_t *Nk,
+                                          const uint64_t *s_samp, const Stencil<DIM> &st,
+                                          int *w_L)
+{
+    constexpr int U = 8;
+    const int lane = lane_id();
+    const int W = d.W, sl2 = d.stride_log2;
+    const int stride = 1 << sl2;
+    const int nb = 2 * st.nrows;
+    uint64_t key = 0;
+    int L = 0;
+    if (lane < nb) {
+        uint64_t klo, khi;
+        row_keys<DIM>(st, lane >> 1, klo, khi);
+        const bool upper = lane & 1;
+        key = upper ? khi : klo;
+        int lo = 0, hi = d.nsamp;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const bool right = upper ? (s_samp[mid] <= key) : (s_samp[mid] < key);
+            if (right)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        L = (lo > 0) ? ((lo - 1) << sl2) + 1 : 0;
+        w_L[lane] = (lo > 0) ? L : -1; // -1: the bound is 0, nothing to probe
+    }
+    wave_sync_lds();
+    // bound b's undecided positions: [L_b, L_b + stride - 1) clipped to W
+    const int niter = (nb * stride + kWave - 1) / kWave;
+    int my_cnt = 0;
+    for (int q0 = 0; q0 < niter; q0 += U) {
+        uint64_t xv[U];
+        bool ok[U];
+        int bb[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = (q0 + u) * kWave + lane;
+            const int b = e >> sl2, r = e & (stride - 1);
+            bb[u] = b;
+            const int Lb = (b < nb) ? w_L[b] : -1;
+            ok[u] = (q0 + u < niter) && Lb >= 0 && r < stride - 1 && Lb + r < W;
+            xv[u] = ok[u] ? Nk[Lb + r] : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = q0 + u;
+            if (q >= niter)
+                break;
+            const uint64_t kb = __shfl(key, bb[u] & 63, kWave);
+            const bool less = ok[u] && ((bb[u] & 1) ? (xv[u] <= kb) : (xv[u] < kb));
+            const uint64_t m = __ballot(less);
+            const int b0 = (q * kWave) >> sl2;
+            if (stride >= kWave) {
+                if (lane == b0)
+                    my_cnt += __popcll(m);
+            } else {
+                const int g = lane - b0;
+                if (g >= 0 && g < (kWave >> sl2))
+                    my_cnt += __popcll(m & (((1ull << stride) - 1ull) << (g * stride)));
+            }
+        }
+    }
+    int res = 0;
+    if (lane < nb)
+        res = (L > 0) ? L + my_cnt : 0;
+    return res;
+}
 
 // Exact neighbour count of the candidate at sorted position p, capped: stops
 // as soon as the count exceeds K.  Wave-cooperative (all 64 lanes call it
-// with the same p).  Returns min(count, first multiple-of-64 chunk where it
-// exceeded K); *tests = predicate evaluations.
+// with the same p).  *tests += predicate evaluations.
 template <int DIM>
 __device__ __forceinline__ int range_count(const Dev &d, const uint64_t *Nk, const float *Nx,
-                                           const uint64_t *s_samp, int *w_start, int *w_off,
-                                           int p, long long &tests)
+                                           const uint64_t *s_samp, int *w_a, int *w_b, int p,
+                                           long long &tests)
 {
     const int lane = lane_id();
     const int W = d.W, K = d.K;
@@ -478,29 +893,23 @@ __device__ __forceinline__ int range_count(const Dev &d, const uint64_t *Nk, con
             atomicOr(&d.ctl->err, 4);
         st.nrows = 32;
     }
-
-    // row bounds: lanes [0,nrows) lower bounds, lanes [32,32+nrows) upper
-    const int r = lane & 31;
-    int bnd = 0;
-    if (r < st.nrows) {
-        uint64_t klo, khi;
-        row_keys<DIM>(st, r, klo, khi);
-        bnd = (lane < 32) ? bound2<false>(Nk, W, s_samp, d.nsamp, d.stride_log2, klo)
-                          : bound2<true>(Nk, W, s_samp, d.nsamp, d.stride_log2, khi);
-    }
-    const int ub = __shfl(bnd, r + 32, kWave);
-    int len = (lane < 32 && r < st.nrows) ? (ub - bnd) : 0;
+    const int bnd = row_bounds<DIM>(d, Nk, s_samp, st, w_a);
+    wave_sync_lds();
+    // lane r < nrows: row r spans [bound 2r, bound 2r+1)
+    const int r = lane;
+    const int lo_b = __shfl(bnd, (2 * r) & 63, kWave);
+    const int hi_b = __shfl(bnd, (2 * r + 1) & 63, kWave);
+    const int len = (r < st.nrows) ? (hi_b - lo_b) : 0;
     const int incl = wave_incl_scan(len);
     const int total = __shfl(incl, 63, kWave);
     if (total <= K) // even counting every stencil point it stays <= K
         return total;
-    if (lane < 32) {
-        w_start[lane] = bnd;
-        w_off[lane] = incl - len;
+    wave_sync_lds();
+    if (r < st.nrows) {
+        w_a[r] = lo_b;
+        w_b[r] = incl - len;
     }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_sync_lds();
 
     int cnt = 0;
     for (int base = 0; base < total; base += kWave) {
@@ -509,17 +918,17 @@ __device__ __forceinline__ int range_count(const Dev &d, const uint64_t *Nk, con
         if (idx < total) {
             int j;
             if constexpr (DIM == 1) {
-                j = w_start[0] + idx;
+                j = w_a[0] + idx;
             } else {
                 int lo = 0, hi = st.nrows - 1;
                 while (lo < hi) { // last row with off <= idx
-                    int mid = (lo + hi + 1) >> 1;
-                    if (w_off[mid] <= idx)
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (w_b[mid] <= idx)
                         lo = mid;
                     else
                         hi = mid - 1;
                 }
-                j = w_start[lo] + (idx - w_off[lo]);
+                j = w_a[lo] + (idx - w_b[lo]);
             }
             const float qx = Nx[j];
             const float qy = DIM > 1 ? Nx[(size_t)W + j] : 0.f;
@@ -536,47 +945,48 @@ __device__ __forceinline__ int range_count(const Dev &d, const uint64_t *Nk, con
 
 // ---------------------------------------------------------------------------
 // k_detect: one wavefront per candidate (grid-stride over the candidate
-// list), then the last workgroup to finish compacts the bitmap.
+// list).  Only ceil(n_cand / 4) workgroups take part (the rest leave at
+// once); the last of them to finish compacts the outlier bitmap into an
+// ascending slot list written straight into pinned host memory.
 // ---------------------------------------------------------------------------
+constexpr int kCompactRegs = 16;
+
 template <int DIM>
-__global__ __launch_bounds__(kDetectNT) void k_detect(Dev d)
+__global__ __launch_bounds__(kDetectNT) void k_detect(Dev d, Step st)
 {
     constexpr int NW = kDetectNT / kWave;
     __shared__ uint64_t s_samp[kMaxSamples];
-    __shared__ int s_rows[NW][2][32];
+    __shared__ int s_rows[NW][2][64];
     __shared__ int s_scan[NW];
     __shared__ int s_last;
     __shared__ unsigned long long s_tests;
 
     const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
-    const uint64_t t = d.ctl->t;
-    const int par = (int)(t & 1) ^ 1;
+    const int par = st.par ^ 1;
     const uint64_t *Nk = d.akey[par];
     const int32_t *Ns = d.aslot[par];
     const float *Nx = d.axyz[par];
     const int n_cand = d.ctl->n_cand;
     const int K = d.K;
+    const int busy = min((int)gridDim.x, max(1, (n_cand + NW - 1) / NW));
+    if ((int)blockIdx.x >= busy)
+        return;
 
     if (tid == 0)
         s_tests = 0;
-    const bool busy = blockIdx.x * NW < n_cand;
-    if (busy)
-        for (int i = tid; i < d.nsamp; i += kDetectNT)
-            s_samp[i] = d.samples[i];
+    copy_to_lds<kMaxSamples / kDetectNT>(s_samp, d.samples[par], d.nsamp);
     __syncthreads();
 
     long long tests = 0;
-    if (busy) {
-        for (int ci = blockIdx.x * NW + wid; ci < n_cand; ci += gridDim.x * NW) {
-            const int p = __builtin_amdgcn_readfirstlane(d.cand[ci]);
-            const int cnt = range_count<DIM>(d, Nk, Nx, s_samp, s_rows[wid][0], s_rows[wid][1], p,
-                                             tests);
-            if (lane == 0) {
-                d.cand_cnt[ci] = min(cnt, K + 1);
-                if (cnt <= K) {
-                    const int slot = Ns[p];
-                    atomicOr(&d.bitmap[slot >> 5], 1u << (slot & 31));
-                }
+    for (int ci = blockIdx.x * NW + wid; ci < n_cand; ci += busy * NW) {
+        const int p = __builtin_amdgcn_readfirstlane(d.cand[ci]);
+        const int cnt =
+            range_count<DIM>(d, Nk, Nx, s_samp, s_rows[wid][0], s_rows[wid][1], p, tests);
+        if (lane == 0) {
+            d.cand_cnt[ci] = min(cnt, K + 1);
+            if (cnt <= K) {
+                const int slot = Ns[p];
+                atomicOr(&d.bitmap[slot >> 5], 1u << (slot & 31));
             }
         }
     }
@@ -587,29 +997,62 @@ __global__ __launch_bounds__(kDetectNT) void k_detect(Dev d)
         if (s_tests)
             atomicAdd(&d.ctl->pair_tests, s_tests);
         __threadfence();
-        s_last = (atomicAdd(&d.ctl->ticket, 1u) == gridDim.x - 1);
+        s_last = (atomicAdd(&d.ctl->ticket, 1u) == (unsigned)busy - 1);
     }
     __syncthreads();
     if (!s_last)
         return;
     __threadfence();
 
-    // last workgroup: ascending slot list from the bitmap, bitmap cleared
+    // last workgroup: ascending slot list from the bitmap, bitmap cleared.
+    // Each thread owns a contiguous run of words; loads are issued together.
     const int nwords = (d.W + 31) >> 5;
     const int per = (nwords + kDetectNT - 1) / kDetectNT;
     const int w0 = min(nwords, tid * per), w1 = min(nwords, w0 + per);
-    int mine = 0;
-    for (int w = w0; w < w1; ++w)
-        mine += __popc(__hip_atomic_load(&d.bitmap[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    int total;
-    int pos = block_excl_scan<kDetectNT>(mine, s_scan, total);
     HostOut *ho = d.h_out;
-    for (int w = w0; w < w1; ++w) {
-        uint32_t bits = atomicExch(&d.bitmap[w], 0u);
-        while (bits) {
-            const int b = __builtin_ctz(bits);
-            bits &= bits - 1;
-            ho->list[pos++] = (w << 5) + b;
+    int total = 0;
+    if (per <= kCompactRegs) {
+        uint32_t wv[kCompactRegs];
+        int mine = 0;
+#pragma unroll
+        for (int k = 0; k < kCompactRegs; ++k) {
+            wv[k] = 0;
+            if (w0 + k < w1)
+                wv[k] = __hip_atomic_load(&d.bitmap[w0 + k], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+        }
+#pragma unroll
+        for (int k = 0; k < kCompactRegs; ++k)
+            mine += __popc(wv[k]);
+        int pos = block_excl_scan<kDetectNT>(mine, s_scan, total);
+#pragma unroll
+        for (int k = 0; k < kCompactRegs; ++k) {
+            uint32_t bits = wv[k];
+            if (bits)
+                __hip_atomic_store(&d.bitmap[w0 + k], 0u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            while (bits) {
+                const int b = __builtin_ctz(bits);
+                bits &= bits - 1;
+                ho->list[pos++] = ((w0 + k) << 5) + b;
+            }
+        }
+    } else {
+        int mine = 0;
+        for (int w = w0; w < w1; ++w)
+            mine += __popc(__hip_atomic_load(&d.bitmap[w], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT));
+        int pos = block_excl_scan<kDetectNT>(mine, s_scan, total);
+        for (int w = w0; w < w1; ++w) {
+            uint32_t bits =
+                __hip_atomic_load(&d.bitmap[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (bits)
+                __hip_atomic_store(&d.bitmap[w], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            while (bits) {
+                const int b = __builtin_ctz(bits);
+                bits &= bits - 1;
+                ho->list[pos++] = (w << 5) + b;
+            }
         }
     }
     __syncthreads();
@@ -620,11 +1063,11 @@ __global__ __launch_bounds__(kDetectNT) void k_detect(Dev d)
         ho->n_cand = n_cand;
         ho->pair_tests = pt;
         ho->err = __hip_atomic_load(&d.ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ho->slide = (int64_t)(t + 1);
+        ho->slide = (int64_t)(st.t + 1);
         d.ctl->err = 0;
         d.ctl->pair_tests = 0;
         d.ctl->ticket = 0;
-        d.ctl->t = t + 1;
+        d.ctl->t = st.t + 1;
         __threadfence_system();
     }
 }

@@ -6,10 +6,14 @@
 //                               optixScan.cpp:232-238, :1053-1057)
 //   akey/aslot/axyz [2]         the window sorted by (cell key, arrival): the
 //                               cell-stencil index; ping-pong by slide parity
+//   apos [2][W]                 position of each sorted-window point inside
+//                               its own sorted batch
 //   bkey/bslot [(m+1)*S]        each batch of S arrivals sorted by cell key,
 //                               kept until it expires m slides later
-//   tkey/tslot [2][max(W,S)]    scratch runs for chunked sorts
-//   samples  u64 [W/stride]     every stride-th key of the sorted window
+//   bxyz f32 [d][(m+1)*S]       coordinates of each sorted batch (SoA)
+//   bsamp [(m+1)*nbs]           every 2^bs_log2-th key of each sorted batch
+//   tkey/tslot/txyz [2]         scratch runs for chunked sorts (max(W,S))
+//   samples  u64 [2][W/stride]  every stride-th key of each sorted window
 //   cand     i32 [W]            candidate positions (sorted-window index)
 //   bitmap   u32 [W/32]         outlier flags by ring slot
 //   ctl                         slide counter, candidate count, grid ticket
@@ -41,13 +45,23 @@ struct HostOut {
     int32_t list[1];       // W entries follow
 };
 
-struct LbvhState;
+// Per-slide quantities, passed by value to every kernel of the slide.
+struct Step {
+    uint64_t t; // slides completed before this one
+    int u;      // ring batch index t mod m: slots [u*S, u*S+S) expire and refill
+    int par;    // sorted-window parity: read [par], write [par^1]
+    int eold;   // batch-ring entry of the expiring batch (generation t)
+    int enew;   // batch-ring entry of the new batch (generation m+t)
+    int buf;    // pinned staging buffer holding the new slide
+};
 
 struct Dev {
     int dim, W, S, m, K;
     int cpb;          // sort chunks per batch
     int nsamp;        // samples in the sorted-window sample table
     int stride_log2;  // sample stride
+    int nbs;          // samples per sorted batch
+    int bs_log2;      // batch sample stride
     float r2;
     double inv_c;     // 1 / cell side
     double Rp;        // R (1 + 1e-5): stencil half-width
@@ -55,11 +69,16 @@ struct Dev {
     uint64_t *akey[2];
     int32_t *aslot[2];
     float *axyz[2];
+    int32_t *apos[2];
     uint64_t *bkey;
     int32_t *bslot;
+    uint64_t *bsamp;
+    float *bxyz;
     uint64_t *tkey[2];
     int32_t *tslot[2];
-    uint64_t *samples;
+    float *txyz[2];
+    int tmpn;         // plane stride of txyz
+    uint64_t *samples[2];
     int32_t *cand;
     int32_t *cand_cnt;  // per-candidate counts (capped), for stats/learning
     uint32_t *bitmap;

@@ -138,6 +138,20 @@ def test_huge_coordinates_clamped_cells():
     assert_same(got, O.run_stream(xs, W, S, T, 0.5, K, "brute"), "clamped")
 
 
+def test_wide_keys_two_stage_sort():
+    """3-D points at both clamp extremes of every axis make the batch's
+    relative cell keys 63 bits wide, which takes the two-stage sort."""
+    case("wide keys")
+    W, S, T, K = 1500, 500, 6, 4
+    rng = np.random.default_rng(10)
+    xs = rng.normal(0, 1, (W + T * S, 3)).astype(np.float32)
+    sel = rng.integers(0, 3, (W + T * S, 3))
+    xs[sel == 1] += np.float32(1e7)
+    xs[sel == 2] -= np.float32(1e7)
+    got = R.run_stream(xs, W, S, T, 1.0, K)
+    assert_same(got, O.run_stream(xs, W, S, T, 1.0, K, "brute"), "wide keys")
+
+
 def test_phase_api_equals_slide():
     case("phase api")
     W, S, T, K = 5000, 250, 10, 20
@@ -157,7 +171,7 @@ def test_phase_api_equals_slide():
     assert_same(got, want, "phase api")
 
 
-@pytest.mark.parametrize("flags", [R.FLAG_NO_GRAPH, R.FLAG_PHASE_TIMES])
+@pytest.mark.parametrize("flags", [0, R.FLAG_PHASE_TIMES])
 def test_launch_modes_agree(flags):
     case(f"flags={flags}")
     wl = O.C1
