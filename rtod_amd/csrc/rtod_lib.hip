@@ -77,6 +77,7 @@ struct rtod_ctx {
     float *d_prime = nullptr; // device copy of the initial window
     float *h_in = nullptr;
     HostOut *h_out = nullptr;
+    const float *dev_src = nullptr; // device-resident new slide of this call
     std::vector<void *> dev_allocs;
     int64_t dev_bytes = 0, pinned_bytes = 0;
     bool primed = false;
@@ -222,7 +223,7 @@ void launch_detect(rtod_ctx *c, hipStream_t s, const Step &st)
 
 Step make_step(const rtod_ctx *c)
 {
-    Step st;
+    Step st{};
     const uint64_t t = (uint64_t)c->host_t;
     st.t = t;
     st.u = (int)(t % (uint64_t)c->m);
@@ -230,6 +231,7 @@ Step make_step(const rtod_ctx *c)
     st.eold = (int)(t % (uint64_t)(c->m + 1));
     st.enew = (int)(((uint64_t)c->m + t) % (uint64_t)(c->m + 1));
     st.buf = (int)(t & 1);
+    st.src = c->dev_src ? c->dev_src : c->h_in + (size_t)st.buf * c->S * c->dim;
     return st;
 }
 
@@ -500,38 +502,18 @@ void accumulate_kernel_times(rtod_ctx *c, double *phase)
     }
 }
 
-rtod_status slide_impl(rtod_ctx *c, const float *new_pts, int32_t *outs, int32_t *n_out,
-                       rtod_times *times)
+// One slide whose new points are already where the ingest kernel reads them
+// (pinned staging, or c->dev_src in HBM).
+rtod_status run_slide(rtod_ctx *c, double t_entry, double t_h2d, int32_t *outs, int32_t *n_out,
+                      rtod_times *times)
 {
-    const double t_entry = now_ms();
-    if (!c->primed) {
-        set_error("rtod_slide before rtod_prime");
-        return RTOD_ERR_STATE;
-    }
-    if (!new_pts || !outs || !n_out) {
-        set_error("NULL argument");
-        return RTOD_ERR_INVALID_ARGUMENT;
-    }
-    if (c->staged) {
-        set_error("a phase-API slide is in progress");
-        return RTOD_ERR_STATE;
-    }
-    RTOD_HIP(hipSetDevice(c->device));
-    const size_t nf = (size_t)c->S * c->dim;
-    std::memcpy(c->h_in + (size_t)(c->host_t & 1) * nf, new_pts, nf * sizeof(float));
-    const double t_h2d = now_ms();
     double phase[3] = {0, 0, 0};
     const bool timed = (c->cfg.flags & RTOD_FLAG_PHASE_TIMES) != 0;
-    if (timed) {
-        launch_slide(c, c->stream, c->ev);
-        RTOD_HIP(hipGetLastError());
-        RTOD_HIP(hipStreamSynchronize(c->stream));
+    launch_slide(c, c->stream, timed ? c->ev : nullptr);
+    RTOD_HIP(hipGetLastError());
+    RTOD_HIP(hipStreamSynchronize(c->stream));
+    if (timed)
         accumulate_kernel_times(c, phase);
-    } else {
-        launch_slide(c, c->stream, nullptr);
-        RTOD_HIP(hipGetLastError());
-        RTOD_HIP(hipStreamSynchronize(c->stream));
-    }
     const double t_dev = now_ms();
     RTOD_TRY(collect(c, outs, n_out));
     const double t_end = now_ms();
@@ -544,6 +526,53 @@ rtod_status slide_impl(rtod_ctx *c, const float *new_pts, int32_t *outs, int32_t
         times->total = t_end - t_entry;
     }
     return RTOD_OK;
+}
+
+rtod_status slide_checks(rtod_ctx *c, const void *new_pts, int32_t *outs, int32_t *n_out)
+{
+    if (!c->primed) {
+        set_error("rtod_slide before rtod_prime");
+        return RTOD_ERR_STATE;
+    }
+    if (!new_pts || !outs || !n_out) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (c->staged) {
+        set_error("a phase-API slide is in progress");
+        return RTOD_ERR_STATE;
+    }
+    return RTOD_OK;
+}
+
+rtod_status slide_impl(rtod_ctx *c, const float *new_pts, int32_t *outs, int32_t *n_out,
+                       rtod_times *times)
+{
+    const double t_entry = now_ms();
+    RTOD_TRY(slide_checks(c, new_pts, outs, n_out));
+    RTOD_HIP(hipSetDevice(c->device));
+    const size_t nf = (size_t)c->S * c->dim;
+    std::memcpy(c->h_in + (size_t)(c->host_t & 1) * nf, new_pts, nf * sizeof(float));
+    c->dev_src = nullptr;
+    return run_slide(c, t_entry, now_ms(), outs, n_out, times);
+}
+
+rtod_status slide_device_impl(rtod_ctx *c, const float *dev_pts, int32_t *outs, int32_t *n_out,
+                              rtod_times *times)
+{
+    const double t_entry = now_ms();
+    RTOD_TRY(slide_checks(c, dev_pts, outs, n_out));
+    RTOD_HIP(hipSetDevice(c->device));
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, dev_pts) != hipSuccess || attr.type != hipMemoryTypeDevice) {
+        (void)hipGetLastError();
+        set_error("rtod_slide_device needs a device (HBM) pointer");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    c->dev_src = dev_pts;
+    rtod_status s = run_slide(c, t_entry, t_entry, outs, n_out, times);
+    c->dev_src = nullptr;
+    return s;
 }
 
 } // namespace
@@ -598,6 +627,21 @@ RTOD_API rtod_status rtod_slide(rtod_ctx *ctx, const float *new_pts, int32_t *ou
     }
     try {
         return slide_impl(ctx, new_pts, outlier_slots, n_outliers, times);
+    } catch (...) {
+        return RTOD_ERR_INTERNAL;
+    }
+}
+
+RTOD_API rtod_status rtod_slide_device(rtod_ctx *ctx, const float *dev_new_pts,
+                                       int32_t *outlier_slots, int32_t *n_outliers,
+                                       rtod_times *times)
+{
+    if (!ctx) {
+        set_error("ctx is NULL");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    try {
+        return slide_device_impl(ctx, dev_new_pts, outlier_slots, n_outliers, times);
     } catch (...) {
         return RTOD_ERR_INTERNAL;
     }

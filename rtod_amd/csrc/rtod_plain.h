@@ -130,9 +130,9 @@ __device__ __forceinline__ void block_radix_sort(uint64_t (&k)[IT], int sh0, int
 
 // ---------------------------------------------------------------------------
 // k_ingest_sort: one workgroup of NT threads per LDS tile of CH = NT*IT
-// points.  Slide mode: the new slide comes from the pinned staging buffer
-// st.buf (one round of coalesced 16-byte loads), goes to ring slots
-// st.u*S + j.  Prime mode: segment b of the initial window (slots b*S + j)
+// points.  Slide mode: the new slide comes from st.src -- the pinned staging
+// buffer or a caller's HBM buffer -- in one round of coalesced 16-byte
+// loads and goes to ring slots st.u*S + j.  Prime mode: segment b of the initial window (slots b*S + j)
 // comes from a device buffer.  The batch is sorted by (cell key, arrival):
 // keys are re-expressed relative to the tile's per-axis minimum and packed
 // into exactly the bits their ranges need, so a slide spanning a few dozen
@@ -157,8 +157,7 @@ __global__ __launch_bounds__(NT) void k_ingest_sort(Dev d, Step st, int mode, co
     const int seg = blockIdx.x / d.cpb;
     const int chunk = blockIdx.x % d.cpb;
     const int slot0 = (mode == kModeSlide) ? st.u * d.S : seg * d.S;
-    const float *src = (mode == kModeSlide) ? d.h_in + (size_t)st.buf * d.S * DIM
-                                            : src_dev + (size_t)seg * d.S * DIM;
+    const float *src = (mode == kModeSlide) ? st.src : src_dev + (size_t)seg * d.S * DIM;
     const int j0 = chunk * CH;
     const int n = min(CH, d.S - j0);
 

@@ -53,6 +53,7 @@ struct Step {
     int eold;   // batch-ring entry of the expiring batch (generation t)
     int enew;   // batch-ring entry of the new batch (generation m+t)
     int buf;    // pinned staging buffer holding the new slide
+    const float *src; // the new slide ([S][d] row-major): pinned staging or HBM
 };
 
 struct Dev {

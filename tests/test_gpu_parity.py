@@ -188,6 +188,20 @@ def test_launch_modes_agree(flags):
     assert_same(got, want, f"flags={flags}")
 
 
+def test_slide_device_equals_slide():
+    case("slide_device")
+    W, S, T, K = 6000, 300, 15, 20
+    xs = O.gen_gmm(71, W + T * S, 3)
+    want = R.run_stream(xs, W, S, T, 1.9, K)
+    with R.DeviceStream(xs) as ds, R.Detector(3, W, S, 1.9, K) as det:
+        det.prime(xs[:W])
+        got = [det.slide_device(ds.at(W + (t - 1) * S)) for t in range(1, T + 1)]
+        with pytest.raises(R.RtodError) as e:  # a host pointer is refused
+            det.slide_device(xs.ctypes.data)
+        assert e.value.status == 1
+    assert_same(got, want, "slide_device")
+
+
 def test_reprime_restarts_stream():
     case("reprime")
     W, S, T, K = 2000, 200, 5, 10
