@@ -56,14 +56,14 @@ def stream(name: str, slides: int) -> np.ndarray:
     return oracle.gen_walk(seed, n, dim) if gen == "walk" else oracle.gen_gmm(seed, n, dim)
 
 
-def run_gpu(xs, W, S, R, K, warmup, steps, flags=0, sched=0, device_input=True):
+def run_gpu(xs, W, S, R, K, warmup, steps, flags=0, sched=0, device_input=True, index=0):
     """device_input: the stream sits in HBM (uploaded once, untimed) and each
     slide is fed by device pointer; else each slide's points come from host
     memory through the pinned staging buffer (PCIe inside the timed call)."""
     import rtod_amd
 
     dim = xs.shape[1]
-    det = rtod_amd.Detector(dim, W, S, R, K, flags=flags, sched_kind=sched)
+    det = rtod_amd.Detector(dim, W, S, R, K, flags=flags, sched_kind=sched, index_kind=index)
     ds = rtod_amd.DeviceStream(xs[: W + (warmup + steps) * S]) if device_input else None
     det.prime(xs[:W])
     totals, outs = [], []
@@ -129,6 +129,26 @@ def measured_traffic(workload: str, kernel: str):
                 if tok.startswith("traffic_bytes="):
                     return int(tok.split("=")[1])
     return None
+
+
+def index_comparison(names, steps):
+    """Per-slide time of the cell-stencil index and of the LBVH index on the
+    same streams (the north star picks whichever measures faster)."""
+    import rtod_amd
+
+    out = {}
+    for name in names:
+        dim, N, W, S, R, K, gen, seed = WORKLOADS[name]
+        T = min(10000, (N - W) // S)
+        xs = stream(name, T)
+        n = min(steps, T - 10)
+        st = run_gpu(xs, W, S, R, K, 10, n, index=rtod_amd.INDEX_CELL_STENCIL)
+        lb = run_gpu(xs, W, S, R, K, 10, n, index=rtod_amd.INDEX_LBVH)
+        out[name] = {"cell_stencil_us": round(st["lib_ms"] * 1e3, 3),
+                     "lbvh_us": round(lb["lib_ms"] * 1e3, 3),
+                     "faster": "cell_stencil" if st["lib_ms"] <= lb["lib_ms"] else "lbvh"}
+        log(f"[bench] index comparison {name}: {out[name]}")
+    return out
 
 
 def other_workloads(names, steps, cpu_slides, with_cpu):
@@ -214,6 +234,10 @@ def main() -> int:
         others = other_workloads(["walk1d-W10k-S500", "gmm3d-W10k-S500"], 120,
                                  args.cpu_slides, not args.no_cpu)
 
+    indexes = None
+    if not args.no_others and args.workload == "walk1d-W100k-S5k":
+        indexes = index_comparison(["walk1d-W100k-S5k", "gmm3d-W10k-S500"], 60)
+
     cpu = None
     if not args.no_cpu:
         us_mt, us_st, threads = cpu_baseline(xs, W, S, R, K, args.cpu_slides)
@@ -263,6 +287,8 @@ def main() -> int:
         line["speedup_vs_cpu"] = round(cpu["value"] / value, 2)
     if others:
         line["other_workloads"] = others
+    if indexes:
+        line["index_comparison"] = indexes
     print(json.dumps(line), flush=True)
     return 0
 

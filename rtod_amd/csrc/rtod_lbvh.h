@@ -1,0 +1,412 @@
+// rtod_lbvh.h -- LBVH index over the candidates (RTOD_INDEX_LBVH).
+//
+// The reference rebuilds an OptiX GAS over one box [c-R, c+R] per candidate
+// every slide (kGenAABB + optixAccelBuild, aabb.cu:5-45, optixScan.cpp:
+// 378-426) and casts one zero-length ray from every window point; each box a
+// ray starts in runs the exact test and atomically bumps the candidate's
+// count, skipping candidates already above K (optixScan.cu:158-199).  This
+// index does the same on MI355X:
+//   k_lbvh_codes   Morton code of each candidate from its cell fields
+//   k_lbvh_sort    stable LDS tile sort of (code, candidate) pairs
+//   k_lbvh_merge   pairwise merge of sorted tiles (when U > one tile)
+//   k_lbvh_build   Karras radix-tree hierarchy (one thread per internal node)
+//   k_lbvh_boxes   leaf boxes rounded outward, unions bottom-up (second
+//                  arrival at a node continues)
+//   k_lbvh_query   every window point walks the tree (LDS stack); at a leaf:
+//                  early-out if the count already exceeds K, exact FP32 test,
+//                  atomicAdd -- counts stay exact up to K+1
+//   k_lbvh_select  count <= K -> outlier bit; last workgroup compacts
+// The number of candidates U lives on the device; kernels are sized for W
+// and exit past U.
+#pragma once
+
+#include "rtod_plain.h"
+
+namespace rtod {
+
+// device arrays of the index (allocated for RTOD_INDEX_LBVH contexts)
+struct Lbvh {
+    uint64_t *code[2]; // sorted Morton codes (ping-pong for merges)
+    int32_t *item[2];  // candidate index of each sorted code
+    int32_t *left, *right; // internal node children: >= 0 internal, < 0 leaf ~i
+    int32_t *parent;   // [2U]: internal nodes 0..U-2, leaves at U-1+i
+    float *box;        // [2U][6] lo xyz, hi xyz (internal then leaves)
+    int32_t *flag;     // [U] bottom-up arrival counters
+    int32_t *count;    // [U] neighbour counts per sorted leaf
+    int ntiles_log2;   // merge passes for U up to W
+};
+
+// Morton interleave of up to three 21-bit (3-D) or two 31-bit (2-D) fields
+__device__ __forceinline__ uint64_t spread3(uint64_t x)
+{
+    x &= 0x1fffffull;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+
+__device__ __forceinline__ uint64_t spread2(uint64_t x)
+{
+    x &= 0x7fffffffull;
+    x = (x | x << 16) & 0x0000ffff0000ffffull;
+    x = (x | x << 8) & 0x00ff00ff00ff00ffull;
+    x = (x | x << 4) & 0x0f0f0f0f0f0f0f0full;
+    x = (x | x << 2) & 0x3333333333333333ull;
+    x = (x | x << 1) & 0x5555555555555555ull;
+    return x;
+}
+
+template <int DIM> __device__ __forceinline__ uint64_t morton_of_key(uint64_t key)
+{
+    uint64_t f[3];
+    key_fields<DIM>(key, f);
+    if constexpr (DIM == 1)
+        return f[0];
+    else if constexpr (DIM == 2)
+        return (spread2(f[0]) << 1) | spread2(f[1]);
+    else
+        return (spread3(f[0]) << 2) | (spread3(f[1]) << 1) | spread3(f[2]);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_lbvh_codes(Dev d, Step st, Lbvh L)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int U = d.ctl->n_cand;
+    if (i >= U)
+        return;
+    const uint64_t *Nk = d.akey[st.par ^ 1];
+    L.code[0][i] = morton_of_key<DIM>(Nk[d.cand[i]]);
+    L.item[0][i] = i;
+}
+
+// Stable sort of one tile of CH (code, item) pairs: relative codes packed
+// with the tile index, two LSD stages when the codes span more than 51 bits.
+template <int NT, int IT>
+__global__ __launch_bounds__(NT) void k_lbvh_sort(Dev d, Lbvh L)
+{
+    constexpr int CH = NT * IT, NW = NT / kWave, IB = 13, KB = 64 - IB;
+    constexpr uint64_t IMASK = (1ull << IB) - 1ull;
+    static_assert(CH <= (1 << IB), "tile index bits");
+    __shared__ uint64_t s_key[CH];
+    __shared__ uint64_t s_code[CH];
+    __shared__ uint16_t s_val[CH];
+    __shared__ int s_hist[16 * NW];
+    __shared__ uint64_t s_red[2][NW];
+    const int U = d.ctl->n_cand;
+    const int j0 = blockIdx.x * CH;
+    if (j0 >= U)
+        return;
+    const int n = min(CH, U - j0);
+    const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
+    uint64_t k[IT];
+    uint64_t mn = ~0ull, mx = 0;
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int l = wid * kWave * IT + i * kWave + lane;
+        k[i] = ~0ull;
+        if (l < n) {
+            k[i] = L.code[0][j0 + l];
+            mn = k[i] < mn ? k[i] : mn;
+            mx = k[i] > mx ? k[i] : mx;
+        }
+        s_code[l] = k[i];
+    }
+    for (int dd = 1; dd < kWave; dd <<= 1) {
+        const uint64_t a = __shfl_xor(mn, dd, kWave), b = __shfl_xor(mx, dd, kWave);
+        mn = a < mn ? a : mn;
+        mx = b > mx ? b : mx;
+    }
+    if (lane == 0) {
+        s_red[0][wid] = mn;
+        s_red[1][wid] = mx;
+    }
+    __syncthreads();
+    mn = ~0ull;
+    mx = 0;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+        mn = s_red[0][q] < mn ? s_red[0][q] : mn;
+        mx = s_red[1][q] > mx ? s_red[1][q] : mx;
+    }
+    const int nbits = bit_length(mx - mn);
+    const bool packed = nbits <= KB;
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int l = wid * kWave * IT + i * kWave + lane;
+        const uint64_t rel = (l < n) ? k[i] - mn : ~0ull;
+        k[i] = ((packed ? rel : (rel & ((1ull << KB) - 1ull))) << IB) | (uint64_t)l;
+        s_key[l] = k[i];
+    }
+    __syncthreads();
+    block_radix_sort<NT, IT>(k, IB, packed ? nbits : KB, s_key, s_hist);
+    if (!packed) {
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const int e = tid + i * NT;
+            s_val[e] = (uint16_t)(s_key[e] & IMASK);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const int p = wid * kWave * IT + i * kWave + lane;
+            const int l = s_val[p];
+            const uint64_t hi = (l < n) ? ((s_code[l] - mn) >> KB) : ~0ull;
+            k[i] = (hi << IB) | (uint64_t)p;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < IT; ++i)
+            s_key[wid * kWave * IT + i * kWave + lane] = k[i];
+        __syncthreads();
+        block_radix_sort<NT, IT>(k, IB, nbits - KB, s_key, s_hist);
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int pos = tid + i * NT;
+        if (pos >= n)
+            break;
+        const uint64_t w = s_key[pos];
+        const int l = packed ? (int)(w & IMASK) : (int)s_val[w & IMASK];
+        L.code[1][j0 + pos] = s_code[l];
+        L.item[1][j0 + pos] = L.item[0][j0 + l];
+    }
+}
+
+// merge sorted runs of length Lr of (code, item), stable; src -> dst
+__global__ __launch_bounds__(256) void k_lbvh_merge(Dev d, Lbvh L, int Lr, int src)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int U = d.ctl->n_cand;
+    if (i >= U)
+        return; // Lr >= U still copies (pr * Lr >= U): buffers stay in step
+    const uint64_t *sk = L.code[src];
+    const int r = i / Lr, pr = r ^ 1;
+    const uint64_t key = sk[i];
+    int out;
+    if (pr * Lr >= U) {
+        out = i;
+    } else {
+        const int p0 = pr * Lr, p1 = min(pr * Lr + Lr, U);
+        const int rank = (r & 1) ? upper_bound_u64(sk, p0, p1, key) - p0
+                                 : lower_bound_u64(sk, p0, p1, key) - p0;
+        out = min(r, pr) * Lr + (i - r * Lr) + rank;
+    }
+    L.code[src ^ 1][out] = key;
+    L.item[src ^ 1][out] = L.item[src][i];
+}
+
+// copy the sorted codes back to buffer 1 when the merges ended in buffer 0
+__global__ __launch_bounds__(256) void k_lbvh_settle(Dev d, Lbvh L, int src)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int U = d.ctl->n_cand;
+    if (src == 1 || i >= U)
+        return;
+    L.code[1][i] = L.code[0][i];
+    L.item[1][i] = L.item[0][i];
+}
+
+// common prefix length of the augmented keys (code, index); -1 out of range
+__device__ __forceinline__ int lbvh_delta(const uint64_t *code, int U, int i, int j)
+{
+    if (j < 0 || j >= U)
+        return -1;
+    const uint64_t a = code[i], b = code[j];
+    if (a != b)
+        return __builtin_clzll(a ^ b);
+    return 64 + __builtin_clz((uint32_t)(i ^ j));
+}
+
+// Karras (2012) hierarchy: internal node i covers a key range [min(i,j),
+// max(i,j)] found by exponential + binary search; its split is the last
+// position sharing more than the range's common prefix.
+__global__ __launch_bounds__(256) void k_lbvh_build(Dev d, Lbvh L)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int U = d.ctl->n_cand;
+    if (i == 0 && U > 0)
+        L.parent[0] = -1;
+    if (i >= U - 1)
+        return;
+    const uint64_t *c = L.code[1];
+    const int dl = lbvh_delta(c, U, i, i + 1) - lbvh_delta(c, U, i, i - 1);
+    const int dir = dl >= 0 ? 1 : -1;
+    const int dmin = lbvh_delta(c, U, i, i - dir);
+    int lmax = 2;
+    while (lbvh_delta(c, U, i, i + lmax * dir) > dmin)
+        lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (lbvh_delta(c, U, i, i + (l + t) * dir) > dmin)
+            l += t;
+    const int j = i + l * dir;
+    const int dnode = lbvh_delta(c, U, i, j);
+    int s = 0;
+    for (int div = 2;; div *= 2) {
+        const int t = (l + div - 1) / div;
+        if (lbvh_delta(c, U, i, i + (s + t) * dir) > dnode)
+            s += t;
+        if (t == 1)
+            break;
+    }
+    const int gamma = i + s * dir + min(dir, 0);
+    const int lo = min(i, j), hi = max(i, j);
+    const int left = (lo == gamma) ? ~gamma : gamma;
+    const int right = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+    L.left[i] = left;
+    L.right[i] = right;
+    L.parent[left >= 0 ? left : (U - 1 + ~left)] = i;
+    L.parent[right >= 0 ? right : (U - 1 + ~right)] = i;
+    L.flag[i] = 0;
+}
+
+// Leaf boxes [x - R', x + R'] rounded outward, then unions up the tree: the
+// second child to reach a node computes its box and continues.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_lbvh_boxes(Dev d, Step st, Lbvh L)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int U = d.ctl->n_cand;
+    if (i >= U)
+        return;
+    const float *Nx = d.axyz[st.par ^ 1];
+    const int p = d.cand[L.item[1][i]];
+    const int leaf = U - 1 + i;
+    float *bx = L.box + (size_t)leaf * 6;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float lo = 0.f, hi = 0.f;
+        if (a < DIM) {
+            const double x = (double)Nx[(size_t)a * d.W + p];
+            lo = __double2float_rd(x - d.Rp);
+            hi = __double2float_ru(x + d.Rp);
+        }
+        bx[a] = lo;
+        bx[3 + a] = hi;
+    }
+    L.count[i] = 0;
+    __threadfence();
+    int node = L.parent[leaf];
+    while (node >= 0) {
+        if (atomicAdd(&L.flag[node], 1) == 0)
+            return; // the sibling subtree finishes this node
+        __threadfence();
+        const int a0 = L.left[node], b0 = L.right[node];
+        const float *ba = L.box + (size_t)(a0 >= 0 ? a0 : U - 1 + ~a0) * 6;
+        const float *bb = L.box + (size_t)(b0 >= 0 ? b0 : U - 1 + ~b0) * 6;
+        float *bn = L.box + (size_t)node * 6;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float la = __hip_atomic_load(&ba[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const float lb = __hip_atomic_load(&bb[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const float ha =
+                __hip_atomic_load(&ba[3 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const float hb =
+                __hip_atomic_load(&bb[3 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            bn[a] = fminf(la, lb);
+            bn[3 + a] = fmaxf(ha, hb);
+        }
+        __threadfence();
+        node = L.parent[node];
+    }
+}
+
+constexpr int kLbvhStack = 64;
+
+// Every window point (sorted order, so neighbouring threads walk similar
+// paths) descends the tree; at a leaf whose box holds it: early-out when the
+// candidate's count already exceeds K (racy read; counts only grow), exact
+// FP32 test, atomicAdd.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_lbvh_query(Dev d, Step st, Lbvh L)
+{
+    __shared__ int s_stack[256 * kLbvhStack];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int U = d.ctl->n_cand;
+    if (U == 0)
+        return;
+    const float *Nx = d.axyz[st.par ^ 1];
+    const bool active = q < d.W;
+    float x[3] = {0.f, 0.f, 0.f};
+    if (active)
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+            x[a] = Nx[(size_t)a * d.W + q];
+    const int K = d.K;
+    int *stk = s_stack + threadIdx.x; // stride 256: conflict-free
+    int sp = 0;
+    int node = (U == 1) ? ~0 : 0;
+    long long tests = 0;
+    for (; active;) {
+        const int slot = node >= 0 ? node : U - 1 + ~node;
+        const float *b = L.box + (size_t)slot * 6;
+        bool in = true;
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+            in = in && (x[a] >= b[a]) && (x[a] <= b[3 + a]);
+        if (in) {
+            if (node < 0) {
+                const int leaf = ~node;
+                if (__hip_atomic_load(&L.count[leaf], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT) <= K) {
+                    const int p = d.cand[L.item[1][leaf]];
+                    const float qx = Nx[p];
+                    const float qy = DIM > 1 ? Nx[(size_t)d.W + p] : 0.f;
+                    const float qz = DIM > 2 ? Nx[(size_t)2 * d.W + p] : 0.f;
+                    ++tests;
+                    if (dist2<DIM>(x, qx, qy, qz) <= d.r2)
+                        atomicAdd(&L.count[leaf], 1);
+                }
+            } else if (sp + 2 <= kLbvhStack) {
+                stk[(sp++) * 256] = L.right[node];
+                node = L.left[node];
+                continue;
+            } else {
+                atomicOr(&d.ctl->err, 8); // stack overflow: cannot happen below depth 64
+            }
+        }
+        if (sp == 0)
+            break;
+        node = stk[(--sp) * 256];
+    }
+    for (int dd = 1; dd < kWave; dd <<= 1)
+        tests += __shfl_xor(tests, dd, kWave);
+    if (lane_id() == 0 && tests)
+        atomicAdd(&d.ctl->pair_tests, (unsigned long long)tests);
+}
+
+// outlier bits from the leaf counts, then the last workgroup compacts
+__global__ __launch_bounds__(kDetectNT) void k_lbvh_select(Dev d, Step st, Lbvh L)
+{
+    __shared__ int s_scan[kDetectNT / kWave];
+    __shared__ int s_last;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int U = d.ctl->n_cand;
+    const int busy = min((int)gridDim.x, max(1, (U + kDetectNT - 1) / kDetectNT));
+    if ((int)blockIdx.x >= busy)
+        return;
+    if (i < U) {
+        const int c = L.count[i];
+        d.cand_cnt[L.item[1][i]] = min(c, d.K + 1);
+        if (c <= d.K) {
+            const int slot = d.aslot[st.par ^ 1][d.cand[L.item[1][i]]];
+            atomicOr(&d.bitmap[slot >> 5], 1u << (slot & 31));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        s_last = (atomicAdd(&d.ctl->ticket, 1u) == (unsigned)busy - 1);
+    }
+    __syncthreads();
+    if (!s_last)
+        return;
+    __threadfence();
+    finish_slide(d, st, U, s_scan);
+}
+
+} // namespace rtod

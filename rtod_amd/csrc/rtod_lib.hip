@@ -18,7 +18,7 @@
 #include <vector>
 
 #include "../../include/rtod.h"
-#include "rtod_plain.h"
+#include "rtod_lbvh.h"
 
 using namespace rtod;
 
@@ -43,9 +43,18 @@ double now_ms()
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
-enum KernelId { kKIngest = 0, kKMergePass, kKMergeWindow, kKClassify, kKDetect, kKCount };
+enum KernelId {
+    kKIngest = 0,
+    kKMergePass,
+    kKMergeWindow,
+    kKClassify,
+    kKDetect,
+    kKLbvhBuild,
+    kKLbvhQuery,
+    kKCount
+};
 const char *kKernelNames[kKCount] = {"ingest_sort", "merge_pass", "merge_window", "classify",
-                                     "detect"};
+                                     "detect",      "lbvh_build", "lbvh_query"};
 
 // LDS-sort tile shapes (threads x items per thread), picked by S and dim:
 // the smallest tile holding the whole batch, else the largest that fits
@@ -78,6 +87,9 @@ struct rtod_ctx {
     float *h_in = nullptr;
     HostOut *h_out = nullptr;
     const float *dev_src = nullptr; // device-resident new slide of this call
+    bool lbvh = false;
+    Lbvh L{};
+    int lbvh_passes = 0;
     std::vector<void *> dev_allocs;
     int64_t dev_bytes = 0, pinned_bytes = 0;
     bool primed = false;
@@ -221,6 +233,46 @@ void launch_detect(rtod_ctx *c, hipStream_t s, const Step &st)
         hipLaunchKernelGGL(k_detect<3>, dim3(grid), dim3(kDetectNT), 0, s, c->d, st);
 }
 
+// LBVH index: codes, tile sort, merges up to U = W, hierarchy, boxes
+void launch_lbvh_build(rtod_ctx *c, hipStream_t s, const Step &st)
+{
+    const dim3 gw((c->W + 255) / 256), b(256);
+    if (c->dim == 1)
+        hipLaunchKernelGGL(k_lbvh_codes<1>, gw, b, 0, s, c->d, st, c->L);
+    else if (c->dim == 2)
+        hipLaunchKernelGGL(k_lbvh_codes<2>, gw, b, 0, s, c->d, st, c->L);
+    else
+        hipLaunchKernelGGL(k_lbvh_codes<3>, gw, b, 0, s, c->d, st, c->L);
+    constexpr int CH = 1024 * 4;
+    hipLaunchKernelGGL((k_lbvh_sort<1024, 4>), dim3((c->W + CH - 1) / CH), dim3(1024), 0, s,
+                       c->d, c->L);
+    int src = 1;
+    for (int p = 0, Lr = CH; p < c->lbvh_passes; ++p, Lr *= 2) {
+        hipLaunchKernelGGL(k_lbvh_merge, gw, b, 0, s, c->d, c->L, Lr, src);
+        src ^= 1;
+    }
+    hipLaunchKernelGGL(k_lbvh_settle, gw, b, 0, s, c->d, c->L, src);
+    hipLaunchKernelGGL(k_lbvh_build, gw, b, 0, s, c->d, c->L);
+    if (c->dim == 1)
+        hipLaunchKernelGGL(k_lbvh_boxes<1>, gw, b, 0, s, c->d, st, c->L);
+    else if (c->dim == 2)
+        hipLaunchKernelGGL(k_lbvh_boxes<2>, gw, b, 0, s, c->d, st, c->L);
+    else
+        hipLaunchKernelGGL(k_lbvh_boxes<3>, gw, b, 0, s, c->d, st, c->L);
+}
+
+void launch_lbvh_detect(rtod_ctx *c, hipStream_t s, const Step &st)
+{
+    const dim3 gw((c->W + 255) / 256), b(256);
+    if (c->dim == 1)
+        hipLaunchKernelGGL(k_lbvh_query<1>, gw, b, 0, s, c->d, st, c->L);
+    else if (c->dim == 2)
+        hipLaunchKernelGGL(k_lbvh_query<2>, gw, b, 0, s, c->d, st, c->L);
+    else
+        hipLaunchKernelGGL(k_lbvh_query<3>, gw, b, 0, s, c->d, st, c->L);
+    hipLaunchKernelGGL(k_lbvh_select, gw, dim3(kDetectNT), 0, s, c->d, st, c->L);
+}
+
 Step make_step(const rtod_ctx *c)
 {
     Step st{};
@@ -256,9 +308,20 @@ void launch_slide(rtod_ctx *c, hipStream_t s, hipEvent_t *ev)
     launch_classify(c, s, st);
     if (ev)
         (void)hipEventRecord(ev[4], s);
-    launch_detect(c, s, st);
-    if (ev)
-        (void)hipEventRecord(ev[5], s);
+    if (c->lbvh) {
+        launch_lbvh_build(c, s, st);
+        if (ev)
+            (void)hipEventRecord(ev[5], s);
+        launch_lbvh_detect(c, s, st);
+        if (ev)
+            (void)hipEventRecord(ev[6], s);
+    } else {
+        launch_detect(c, s, st);
+        if (ev) {
+            (void)hipEventRecord(ev[5], s);
+            (void)hipEventRecord(ev[6], s);
+        }
+    }
 }
 
 rtod_status validate(const rtod_config *cfg)
@@ -292,8 +355,8 @@ rtod_status validate(const rtod_config *cfg)
         set_error("unknown index_kind or sched_kind");
         return RTOD_ERR_INVALID_ARGUMENT;
     }
-    if (cfg->index_kind == RTOD_INDEX_LBVH || cfg->sched_kind == RTOD_SCHED_LEARNED) {
-        set_error("LBVH index and learned scheduler are not available in this build");
+    if (cfg->sched_kind == RTOD_SCHED_LEARNED) {
+        set_error("the learned scheduler is not available in this build");
         return RTOD_ERR_UNSUPPORTED;
     }
     return RTOD_OK;
@@ -371,6 +434,24 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out)
     RTOD_HIP(hipMemsetAsync(d.bitmap, 0, sizeof(uint32_t) * ((W + 31) / 32), c->stream));
     RTOD_HIP(hipMemsetAsync(d.ctl, 0, sizeof(Ctl), c->stream));
 
+    c->lbvh = cfg->index_kind == RTOD_INDEX_LBVH;
+    if (c->lbvh) {
+        Lbvh &L = c->L;
+        for (int p = 0; p < 2; ++p) {
+            RTOD_TRY(dalloc(c, &L.code[p], W));
+            RTOD_TRY(dalloc(c, &L.item[p], W));
+        }
+        RTOD_TRY(dalloc(c, &L.left, W));
+        RTOD_TRY(dalloc(c, &L.right, W));
+        RTOD_TRY(dalloc(c, &L.parent, 2 * W));
+        RTOD_TRY(dalloc(c, &L.box, 2 * W * 6));
+        RTOD_TRY(dalloc(c, &L.flag, W));
+        RTOD_TRY(dalloc(c, &L.count, W));
+        int passes = 0;
+        for (size_t Lr = 4096; Lr < W; Lr *= 2)
+            ++passes;
+        c->lbvh_passes = passes;
+    }
     const size_t in_bytes = 2 * S * dim * sizeof(float);
     const size_t out_bytes = sizeof(HostOut) + W * sizeof(int32_t);
     RTOD_HIP(hipHostMalloc((void **)&c->h_in, in_bytes, hipHostMallocMapped | hipHostMallocCoherent));
@@ -478,8 +559,8 @@ rtod_status collect(rtod_ctx *c, int32_t *outs, int32_t *n_out)
 
 void accumulate_kernel_times(rtod_ctx *c, double *phase)
 {
-    float ms[5];
-    for (int i = 0; i < 5; ++i) {
+    float ms[6];
+    for (int i = 0; i < 6; ++i) {
         ms[i] = 0.f;
         (void)hipEventElapsedTime(&ms[i], c->ev[i], c->ev[i + 1]);
     }
@@ -493,12 +574,19 @@ void accumulate_kernel_times(rtod_ctx *c, double *phase)
     c->kcnt[kKMergeWindow]++;
     c->ksum[kKClassify] += ms[3];
     c->kcnt[kKClassify]++;
-    c->ksum[kKDetect] += ms[4];
-    c->kcnt[kKDetect]++;
+    if (c->lbvh) {
+        c->ksum[kKLbvhBuild] += ms[4];
+        c->kcnt[kKLbvhBuild]++;
+        c->ksum[kKLbvhQuery] += ms[5];
+        c->kcnt[kKLbvhQuery]++;
+    } else {
+        c->ksum[kKDetect] += ms[4];
+        c->kcnt[kKDetect]++;
+    }
     if (phase) {
-        phase[0] = (double)ms[0] + ms[1] + ms[2] + ms[3]; // prepare cell
-        phase[1] = 0.0;                                   // build index (stencil: none)
-        phase[2] = (double)ms[4];                         // detect (+ compaction tail)
+        phase[0] = (double)ms[0] + ms[1] + ms[2] + ms[3];     // prepare cell
+        phase[1] = c->lbvh ? (double)ms[4] : 0.0;              // build index
+        phase[2] = c->lbvh ? (double)ms[5] : (double)ms[4];    // detect (+ compaction)
     }
 }
 
@@ -698,7 +786,13 @@ RTOD_API rtod_status rtod_build_index(rtod_ctx *c)
         return RTOD_ERR_STATE;
     }
     // cell stencil: the cell-sorted window and its sample table were built by
-    // the merge in prepare_cell; nothing left to do
+    // the merge in prepare_cell; LBVH: build the tree over the candidates
+    if (c->lbvh) {
+        RTOD_HIP(hipSetDevice(c->device));
+        launch_lbvh_build(c, c->stream, make_step(c));
+        RTOD_HIP(hipGetLastError());
+        RTOD_HIP(hipStreamSynchronize(c->stream));
+    }
     c->phase = 3;
     return RTOD_OK;
 }
@@ -714,7 +808,10 @@ RTOD_API rtod_status rtod_detect_outlier(rtod_ctx *c)
         return RTOD_ERR_STATE;
     }
     RTOD_HIP(hipSetDevice(c->device));
-    launch_detect(c, c->stream, make_step(c));
+    if (c->lbvh)
+        launch_lbvh_detect(c, c->stream, make_step(c));
+    else
+        launch_detect(c, c->stream, make_step(c));
     RTOD_HIP(hipGetLastError());
     RTOD_HIP(hipStreamSynchronize(c->stream));
     c->phase = 4;
@@ -753,7 +850,7 @@ RTOD_API rtod_status rtod_get_stats(const rtod_ctx *c, rtod_stats *o)
     o->outliers_total = c->out_total;
     o->device_bytes = c->dev_bytes;
     o->host_pinned_bytes = c->pinned_bytes;
-    o->index_kind = RTOD_INDEX_CELL_STENCIL;
+    o->index_kind = c->lbvh ? RTOD_INDEX_LBVH : RTOD_INDEX_CELL_STENCIL;
     o->sched_kind = RTOD_SCHED_PLAIN;
     return RTOD_OK;
 }

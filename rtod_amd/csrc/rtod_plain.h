@@ -943,68 +943,17 @@ __device__ __forceinline__ int range_count(const Dev &d, const uint64_t *Nk, con
 }
 
 // ---------------------------------------------------------------------------
-// k_detect: one wavefront per candidate (grid-stride over the candidate
-// list).  Only ceil(n_cand / 4) workgroups take part (the rest leave at
-// once); the last of them to finish compacts the outlier bitmap into an
-// ascending slot list written straight into pinned host memory.
+// Slide epilogue, run by every thread of the last workgroup to finish the
+// detect phase: ascending slot list from the outlier bitmap written straight
+// into pinned host memory, bitmap cleared, counters published, device slide
+// counter advanced.  Each thread owns a contiguous run of bitmap words; its
+// loads are issued together.
 // ---------------------------------------------------------------------------
 constexpr int kCompactRegs = 16;
 
-template <int DIM>
-__global__ __launch_bounds__(kDetectNT) void k_detect(Dev d, Step st)
+__device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int n_cand, int *s_scan)
 {
-    constexpr int NW = kDetectNT / kWave;
-    __shared__ uint64_t s_samp[kMaxSamples];
-    __shared__ int s_rows[NW][2][64];
-    __shared__ int s_scan[NW];
-    __shared__ int s_last;
-    __shared__ unsigned long long s_tests;
-
-    const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
-    const int par = st.par ^ 1;
-    const uint64_t *Nk = d.akey[par];
-    const int32_t *Ns = d.aslot[par];
-    const float *Nx = d.axyz[par];
-    const int n_cand = d.ctl->n_cand;
-    const int K = d.K;
-    const int busy = min((int)gridDim.x, max(1, (n_cand + NW - 1) / NW));
-    if ((int)blockIdx.x >= busy)
-        return;
-
-    if (tid == 0)
-        s_tests = 0;
-    copy_to_lds<kMaxSamples / kDetectNT>(s_samp, d.samples[par], d.nsamp);
-    __syncthreads();
-
-    long long tests = 0;
-    for (int ci = blockIdx.x * NW + wid; ci < n_cand; ci += busy * NW) {
-        const int p = __builtin_amdgcn_readfirstlane(d.cand[ci]);
-        const int cnt =
-            range_count<DIM>(d, Nk, Nx, s_samp, s_rows[wid][0], s_rows[wid][1], p, tests);
-        if (lane == 0) {
-            d.cand_cnt[ci] = min(cnt, K + 1);
-            if (cnt <= K) {
-                const int slot = Ns[p];
-                atomicOr(&d.bitmap[slot >> 5], 1u << (slot & 31));
-            }
-        }
-    }
-    if (lane == 0 && tests)
-        atomicAdd(&s_tests, (unsigned long long)tests);
-    __syncthreads();
-    if (tid == 0) {
-        if (s_tests)
-            atomicAdd(&d.ctl->pair_tests, s_tests);
-        __threadfence();
-        s_last = (atomicAdd(&d.ctl->ticket, 1u) == (unsigned)busy - 1);
-    }
-    __syncthreads();
-    if (!s_last)
-        return;
-    __threadfence();
-
-    // last workgroup: ascending slot list from the bitmap, bitmap cleared.
-    // Each thread owns a contiguous run of words; loads are issued together.
+    const int tid = threadIdx.x;
     const int nwords = (d.W + 31) >> 5;
     const int per = (nwords + kDetectNT - 1) / kDetectNT;
     const int w0 = min(nwords, tid * per), w1 = min(nwords, w0 + per);
@@ -1069,6 +1018,68 @@ __global__ __launch_bounds__(kDetectNT) void k_detect(Dev d, Step st)
         d.ctl->t = st.t + 1;
         __threadfence_system();
     }
+}
+
+// ---------------------------------------------------------------------------
+// k_detect: one wavefront per candidate (grid-stride over the candidate
+// list).  Only ceil(n_cand / 4) workgroups take part (the rest leave at
+// once); the last of them to finish compacts the outlier bitmap into an
+// ascending slot list written straight into pinned host memory.
+// ---------------------------------------------------------------------------
+template <int DIM>
+__global__ __launch_bounds__(kDetectNT) void k_detect(Dev d, Step st)
+{
+    constexpr int NW = kDetectNT / kWave;
+    __shared__ uint64_t s_samp[kMaxSamples];
+    __shared__ int s_rows[NW][2][64];
+    __shared__ int s_scan[NW];
+    __shared__ int s_last;
+    __shared__ unsigned long long s_tests;
+
+    const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
+    const int par = st.par ^ 1;
+    const uint64_t *Nk = d.akey[par];
+    const int32_t *Ns = d.aslot[par];
+    const float *Nx = d.axyz[par];
+    const int n_cand = d.ctl->n_cand;
+    const int K = d.K;
+    const int busy = min((int)gridDim.x, max(1, (n_cand + NW - 1) / NW));
+    if ((int)blockIdx.x >= busy)
+        return;
+
+    if (tid == 0)
+        s_tests = 0;
+    copy_to_lds<kMaxSamples / kDetectNT>(s_samp, d.samples[par], d.nsamp);
+    __syncthreads();
+
+    long long tests = 0;
+    for (int ci = blockIdx.x * NW + wid; ci < n_cand; ci += busy * NW) {
+        const int p = __builtin_amdgcn_readfirstlane(d.cand[ci]);
+        const int cnt =
+            range_count<DIM>(d, Nk, Nx, s_samp, s_rows[wid][0], s_rows[wid][1], p, tests);
+        if (lane == 0) {
+            d.cand_cnt[ci] = min(cnt, K + 1);
+            if (cnt <= K) {
+                const int slot = Ns[p];
+                atomicOr(&d.bitmap[slot >> 5], 1u << (slot & 31));
+            }
+        }
+    }
+    if (lane == 0 && tests)
+        atomicAdd(&s_tests, (unsigned long long)tests);
+    __syncthreads();
+    if (tid == 0) {
+        if (s_tests)
+            atomicAdd(&d.ctl->pair_tests, s_tests);
+        __threadfence();
+        s_last = (atomicAdd(&d.ctl->ticket, 1u) == (unsigned)busy - 1);
+    }
+    __syncthreads();
+    if (!s_last)
+        return;
+    __threadfence();
+
+    finish_slide(d, st, n_cand, s_scan);
 }
 
 } // namespace rtod

@@ -234,3 +234,59 @@ def test_stats_counters():
     # same cells (same key function) -> same candidate set size
     assert st["candidates_last"] == g.stats()["candidates"]
     assert st["device_bytes"] > 0 and st["host_pinned_bytes"] > 0
+
+
+# ---------------------------------------------------------------- LBVH index
+LBVH_CASES = [
+    ("walk1d W=2000", lambda: O.gen_walk(5, 2000 + 30 * 100), 2000, 100, 30, 0.45, 10),
+    ("gmm3d W=5000", lambda: O.gen_gmm(13, 5000 + 20 * 250, 3), 5000, 250, 20, 1.9, 50),
+    ("gmm2d W=3000", lambda: O.gen_gmm(22, 3000 + 20 * 150, 2), 3000, 150, 20, 1.5, 20),
+    ("walk3d W=2400", lambda: O.gen_walk(7, 2400 + 15 * 120, 3), 2400, 120, 15, 0.3, 20),
+]
+
+
+@pytest.mark.parametrize("name,gen,W,S,T,Rr,K", LBVH_CASES, ids=[c[0] for c in LBVH_CASES])
+def test_lbvh_vs_brute(name, gen, W, S, T, Rr, K):
+    case(f"lbvh {name}")
+    xs = gen()
+    got = R.run_stream(xs, W, S, T, Rr, K, index_kind=R.INDEX_LBVH)
+    assert_same(got, O.run_stream(xs, W, S, T, Rr, K, "brute"), f"lbvh {name}")
+
+
+def test_lbvh_c3_vs_grid():
+    wl = O.C3
+    case("lbvh " + wl.name)
+    xs = wl.data()
+    T = 30
+    got = R.run_stream(xs, wl.W, wl.S, T, wl.R, wl.K, index_kind=R.INDEX_LBVH)
+    assert_same(got, O.run_stream(xs, wl.W, wl.S, T, wl.R, wl.K, "grid"), "lbvh C3")
+
+
+def test_lbvh_c2_multi_tile_vs_grid():
+    """C2 with R small enough that candidates exceed one 4096-entry sort tile
+    (merge passes of the LBVH sort run)."""
+    wl = O.C2
+    case("lbvh C2 small R")
+    xs = wl.data()
+    T, Rr, K = 4, 0.02, 200
+    got = R.run_stream(xs, wl.W, wl.S, T, Rr, K, index_kind=R.INDEX_LBVH)
+    with R.Detector(1, wl.W, wl.S, Rr, K, index_kind=R.INDEX_LBVH) as det:
+        det.prime(xs[:wl.W])
+        det.slide(xs[wl.W: wl.W + wl.S])
+        assert det.stats()["candidates_last"] > 4096
+    assert_same(got, O.run_stream(xs, wl.W, wl.S, T, Rr, K, "grid"), "lbvh C2 small R")
+
+
+@pytest.mark.parametrize("index_kind", [R.INDEX_CELL_STENCIL, R.INDEX_LBVH])
+def test_no_candidates_and_edge_data(index_kind):
+    case(f"edge index={index_kind}")
+    W, S, T = 1000, 100, 4
+    same = np.zeros((W + T * S, 2), np.float32)  # one dense cell: no candidates
+    got = R.run_stream(same, W, S, T, 0.5, 10, index_kind=index_kind)
+    assert all(len(g) == 0 for g in got)
+    xs = O.gen_walk(41, W + T * S, 2)
+    xs[17] = [np.nan, 1.0]
+    xs[300] = [np.inf, 2.0]
+    xs[905] = [-np.inf, -np.inf]
+    got = R.run_stream(xs, W, S, T, 0.45, 5, index_kind=index_kind)
+    assert_same(got, O.run_stream(xs, W, S, T, 0.45, 5, "brute"), "edge data")
