@@ -98,8 +98,11 @@ typedef struct rtod_stats {
     int64_t host_pinned_bytes;  /* pinned host memory owned by the context */
     int64_t predictor_misses;   /* learned scheduler: mispredicted points  */
     int64_t predictor_points;   /* learned scheduler: predicted points     */
-    int64_t fallbacks;          /* slides run on the plain schedule after
-                                   the miss rate exceeded the threshold   */
+    int64_t fallbacks;          /* slides run on the plain schedule because
+                                   the previous slide's miss rate exceeded
+                                   the threshold (or no prediction yet)   */
+    int64_t train_steps;        /* learned scheduler: optimiser steps      */
+    double train_loss_last;     /* learned scheduler: mean loss, last slide */
     int32_t index_kind;         /* index actually used                     */
     int32_t sched_kind;         /* schedule actually used last slide       */
 } rtod_stats;
@@ -157,6 +160,32 @@ RTOD_API rtod_status rtod_get_stats(const rtod_ctx *ctx, rtod_stats *out);
 RTOD_API rtod_status rtod_kernel_times(const rtod_ctx *ctx, const char **names, double *avg_ms,
                                        int64_t *launches, int32_t *n);
 RTOD_API rtod_status rtod_reset_kernel_times(rtod_ctx *ctx);
+
+/* ---- Learned scheduler (RTOD_SCHED_LEARNED) ------------------------------
+ * No reference counterpart: the reference schedules every range count the
+ * same way.  The predictor is an MLP 6 -> 32 -> 32 -> 1 whose RTOD_MLP_NPARAMS
+ * FP32 parameters are laid out W1[6][32] b1[32] W2[32][32] b2[32] W3[32] b3
+ * (row index = input unit).  It only orders the exact range counts; outlier
+ * sets are identical to RTOD_SCHED_PLAIN's on every slide. */
+#define RTOD_MLP_NPARAMS 1313
+
+/* Copies the context's current predictor parameters (n == RTOD_MLP_NPARAMS). */
+RTOD_API rtod_status rtod_get_predictor(rtod_ctx *ctx, float *params, int32_t n);
+
+/* Replaces them, clears the optimiser state; frozen != 0 stops online
+ * training (for A/B runs against a fixed or untrained predictor). */
+RTOD_API rtod_status rtod_set_predictor(rtod_ctx *ctx, const float *params, int32_t n,
+                                        int32_t frozen);
+
+/* Predictor forward pass on the GPU's matrix cores, the device code the
+ * detector runs: X [n][6] features -> out [n] in (0,1).  Host arrays. */
+RTOD_API rtod_status rtod_mlp_forward(int32_t device, const float *params, const float *X,
+                                      int32_t n, float *out);
+
+/* Gradient of the mean squared error over n (1..128) samples, by the
+ * detector's training kernels: grad [RTOD_MLP_NPARAMS], *loss = mean error. */
+RTOD_API rtod_status rtod_mlp_grad(int32_t device, const float *params, const float *X,
+                                   const float *y, int32_t n, float *grad, float *loss);
 
 /* Frees everything (reference cleanup, optixScan.cpp:883-932). */
 RTOD_API void rtod_destroy(rtod_ctx *ctx);

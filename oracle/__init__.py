@@ -72,6 +72,15 @@ def lib():
         L.rtod_o_grid_destroy.argtypes = [C.c_void_p]
         L.rtod_o_cell_key.restype = C.c_uint64
         L.rtod_o_cell_key.argtypes = [f32p, C.c_int, C.c_float]
+        L.rtod_o_mlp_forward.restype = None
+        L.rtod_o_mlp_forward.argtypes = [f32p, f32p, C.c_int, C.c_int, f32p]
+        L.rtod_o_mlp_grad.restype = C.c_float
+        L.rtod_o_mlp_grad.argtypes = [f32p, f32p, f32p, C.c_int, C.c_int, f32p]
+        L.rtod_o_mlp_loss64.restype = C.c_double
+        L.rtod_o_mlp_loss64.argtypes = [np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS"),
+                                        f32p, f32p, C.c_int]
+        L.rtod_o_bf16_round.restype = C.c_float
+        L.rtod_o_bf16_round.argtypes = [C.c_float]
         L.rtod_o_run_stream.restype = C.c_int64
         L.rtod_o_run_stream.argtypes = [C.c_int, f32p, C.c_int, C.c_int64, C.c_int64, C.c_int64,
                                         C.c_float, C.c_int, C.c_int, i32p, i32p, C.c_int64,
@@ -228,3 +237,35 @@ class Workload:
 C1 = Workload("walk1d-W10k-S500", 1, 200_000, 10_000, 500, 0.45, 50, "walk", 1)
 C2 = Workload("walk1d-W100k-S5k", 1, 1_048_572, 100_000, 5_000, 0.45, 50, "walk", 2)
 C3 = Workload("gmm3d-W10k-S500", 3, 575_000, 10_000, 500, 1.9, 50, "gmm", 3)
+
+
+# --------------------------------------------------------------------------
+# predictor MLP reference (learned scheduler)
+# --------------------------------------------------------------------------
+MLP_NP = 1313
+
+
+def mlp_forward(params: np.ndarray, X: np.ndarray, bf16: bool = True) -> np.ndarray:
+    X = np.ascontiguousarray(X, np.float32).reshape(-1, 6)
+    out = np.empty(X.shape[0], np.float32)
+    lib().rtod_o_mlp_forward(np.ascontiguousarray(params, np.float32), X.ravel(), X.shape[0],
+                             int(bf16), out)
+    return out
+
+
+def mlp_grad(params: np.ndarray, X: np.ndarray, y: np.ndarray, bf16: bool = True):
+    X = np.ascontiguousarray(X, np.float32).reshape(-1, 6)
+    g = np.empty(MLP_NP, np.float32)
+    loss = lib().rtod_o_mlp_grad(np.ascontiguousarray(params, np.float32), X.ravel(),
+                                 np.ascontiguousarray(y, np.float32), X.shape[0], int(bf16), g)
+    return float(loss), g
+
+
+def mlp_loss64(params: np.ndarray, X: np.ndarray, y: np.ndarray) -> float:
+    X = np.ascontiguousarray(X, np.float32).reshape(-1, 6)
+    return float(lib().rtod_o_mlp_loss64(np.ascontiguousarray(params, np.float64), X.ravel(),
+                                         np.ascontiguousarray(y, np.float32), X.shape[0]))
+
+
+def bf16_round(x: float) -> float:
+    return float(lib().rtod_o_bf16_round(x))

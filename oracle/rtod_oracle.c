@@ -973,3 +973,135 @@ int64_t rtod_o_run_stream(int method, const float *xs, int dim, int64_t W, int64
     free(buf);
     return total;
 }
+
+/* ------------------------------------------------------------------ */
+/* MLP reference (learned scheduler's predictor)                        */
+/* ------------------------------------------------------------------ */
+#define NIN 6
+#define NH 32
+#define OFF_W1 0
+#define OFF_B1 (NIN * NH)
+#define OFF_W2 (OFF_B1 + NH)
+#define OFF_B2 (OFF_W2 + NH * NH)
+#define OFF_W3 (OFF_B2 + NH)
+#define OFF_B3 (OFF_W3 + NH)
+
+float rtod_o_bf16_round(float x)
+{
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) { /* inf / nan: truncate, keep nan quiet */
+        if (u & 0x007fffffu)
+            u |= 0x00400000u;
+        u &= 0xffff0000u;
+    } else {
+        const uint32_t lsb = (u >> 16) & 1u;
+        u += 0x7fffu + lsb;
+        u &= 0xffff0000u;
+    }
+    float r;
+    memcpy(&r, &u, 4);
+    return r;
+}
+
+static inline float q(float x, int bf16) { return bf16 ? rtod_o_bf16_round(x) : x; }
+
+/* one sample forward; keeps pre-activations for the backward pass */
+static float mlp_fwd1(const float *P, const float *x, int bf16, float *a1, float *h1, float *a2,
+                      float *h2)
+{
+    for (int j = 0; j < NH; j++) {
+        float s = 0.f;
+        for (int i = 0; i < NIN; i++)
+            s += q(x[i], bf16) * q(P[OFF_W1 + i * NH + j], bf16);
+        a1[j] = s + P[OFF_B1 + j];
+        h1[j] = a1[j] > 0.f ? a1[j] : 0.f;
+    }
+    for (int j = 0; j < NH; j++) {
+        float s = 0.f;
+        for (int i = 0; i < NH; i++)
+            s += q(h1[i], bf16) * q(P[OFF_W2 + i * NH + j], bf16);
+        a2[j] = s + P[OFF_B2 + j];
+        h2[j] = a2[j] > 0.f ? a2[j] : 0.f;
+    }
+    float z = 0.f; /* output layer: FP32 dot product on both paths */
+    for (int i = 0; i < NH; i++)
+        z += h2[i] * P[OFF_W3 + i];
+    z += P[OFF_B3];
+    return 1.f / (1.f + expf(-z));
+}
+
+void rtod_o_mlp_forward(const float *params, const float *X, int n, int bf16, float *pred)
+{
+    float a1[NH], h1[NH], a2[NH], h2[NH];
+    for (int s = 0; s < n; s++)
+        pred[s] = mlp_fwd1(params, X + s * NIN, bf16, a1, h1, a2, h2);
+}
+
+float rtod_o_mlp_grad(const float *params, const float *X, const float *y, int n, int bf16,
+                      float *grad)
+{
+    const float *P = params;
+    memset(grad, 0, sizeof(float) * RTOD_O_MLP_NP);
+    float a1[NH], h1[NH], a2[NH], h2[NH], d2[NH], d1[NH];
+    double loss = 0.0;
+    for (int s = 0; s < n; s++) {
+        const float *x = X + s * NIN;
+        const float yh = mlp_fwd1(P, x, bf16, a1, h1, a2, h2);
+        const float e = yh - y[s];
+        loss += (double)e * e;
+        const float dz = 2.f * e * yh * (1.f - yh) / (float)n;
+        /* layer 3 */
+        for (int i = 0; i < NH; i++)
+            grad[OFF_W3 + i] += h2[i] * dz;
+        grad[OFF_B3] += dz;
+        for (int i = 0; i < NH; i++)
+            d2[i] = a2[i] > 0.f ? dz * P[OFF_W3 + i] : 0.f;
+        /* layer 2 */
+        for (int i = 0; i < NH; i++)
+            for (int j = 0; j < NH; j++)
+                grad[OFF_W2 + i * NH + j] += q(h1[i], bf16) * q(d2[j], bf16);
+        for (int j = 0; j < NH; j++)
+            grad[OFF_B2 + j] += d2[j];
+        for (int i = 0; i < NH; i++) {
+            float t = 0.f;
+            for (int j = 0; j < NH; j++)
+                t += q(d2[j], bf16) * q(P[OFF_W2 + i * NH + j], bf16);
+            d1[i] = a1[i] > 0.f ? t : 0.f;
+        }
+        /* layer 1 */
+        for (int i = 0; i < NIN; i++)
+            for (int j = 0; j < NH; j++)
+                grad[OFF_W1 + i * NH + j] += q(x[i], bf16) * q(d1[j], bf16);
+        for (int j = 0; j < NH; j++)
+            grad[OFF_B1 + j] += d1[j];
+    }
+    return (float)(loss / (double)n);
+}
+
+double rtod_o_mlp_loss64(const double *P, const float *X, const float *y, int n)
+{
+    double loss = 0.0;
+    for (int s = 0; s < n; s++) {
+        const float *x = X + s * NIN;
+        double h1[NH], h2[NH];
+        for (int j = 0; j < NH; j++) {
+            double t = P[OFF_B1 + j];
+            for (int i = 0; i < NIN; i++)
+                t += (double)x[i] * P[OFF_W1 + i * NH + j];
+            h1[j] = t > 0.0 ? t : 0.0;
+        }
+        for (int j = 0; j < NH; j++) {
+            double t = P[OFF_B2 + j];
+            for (int i = 0; i < NH; i++)
+                t += h1[i] * P[OFF_W2 + i * NH + j];
+            h2[j] = t > 0.0 ? t : 0.0;
+        }
+        double z = P[OFF_B3];
+        for (int i = 0; i < NH; i++)
+            z += h2[i] * P[OFF_W3 + i];
+        const double yh = 1.0 / (1.0 + exp(-z));
+        loss += (yh - y[s]) * (yh - y[s]);
+    }
+    return loss / (double)n;
+}

@@ -98,6 +98,26 @@ int64_t rtod_o_run_stream(int method, const float *xs, int dim, int64_t W,
                           int32_t *n_out, int32_t *out, int64_t cap,
                           double *us_per_slide);
 
+/* ------------------------------------------------------------------ */
+/* Learned scheduler's predictor: MLP 6 -> 32 (ReLU) -> 32 (ReLU) -> 1     */
+/* (sigmoid), loss = mean squared error.  Parameters, flat (RTOD_O_MLP_NP  */
+/* floats): W1[6][32] b1[32] W2[32][32] b2[32] W3[32] b3.                  */
+/* bf16 != 0 rounds the operands of the two hidden layers' products       */
+/* (inputs, W1, h1, W2 forward; h1, d2, W2, x, d1 backward) to bfloat16    */
+/* (round to nearest even) and accumulates in FP32, as the GPU's MFMA path */
+/* does; the output layer (h2 . W3) and all bias/elementwise terms stay    */
+/* FP32.  bf16 == 0 is plain FP32.  The GPU kernels are checked against    */
+/* this; this is checked against central differences of the FP64 loss.    */
+/* ------------------------------------------------------------------ */
+#define RTOD_O_MLP_NP 1313
+void rtod_o_mlp_forward(const float *params, const float *X /*[n][6]*/, int n, int bf16,
+                        float *pred /*[n]*/);
+/* gradient of the mean loss over the n samples; returns the loss */
+float rtod_o_mlp_grad(const float *params, const float *X, const float *y, int n, int bf16,
+                      float *grad /*[NP]*/);
+double rtod_o_mlp_loss64(const double *params, const float *X, const float *y, int n);
+float rtod_o_bf16_round(float x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,7 +59,10 @@ class _Stats(C.Structure):
                 ("outliers_total", C.c_int64), ("device_bytes", C.c_int64),
                 ("host_pinned_bytes", C.c_int64), ("predictor_misses", C.c_int64),
                 ("predictor_points", C.c_int64), ("fallbacks", C.c_int64),
+                ("train_steps", C.c_int64), ("train_loss_last", C.c_double),
                 ("index_kind", C.c_int32), ("sched_kind", C.c_int32)]
+
+MLP_NPARAMS = 1313
 
 
 # (name, restype, argtypes) of every symbol include/rtod.h declares
@@ -82,6 +85,11 @@ SIGNATURES = [
                                     C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                     C.POINTER(C.c_int32)]),
     ("rtod_reset_kernel_times", C.c_int, [C.c_void_p]),
+    ("rtod_get_predictor", C.c_int, [C.c_void_p, _F32P, C.c_int32]),
+    ("rtod_set_predictor", C.c_int, [C.c_void_p, _F32P, C.c_int32, C.c_int32]),
+    ("rtod_mlp_forward", C.c_int, [C.c_int32, _F32P, _F32P, C.c_int32, _F32P]),
+    ("rtod_mlp_grad", C.c_int, [C.c_int32, _F32P, _F32P, _F32P, C.c_int32, _F32P,
+                                C.POINTER(C.c_float)]),
     ("rtod_destroy", None, [C.c_void_p]),
     ("rtod_status_str", C.c_char_p, [C.c_int]),
     ("rtod_last_error", C.c_char_p, []),
@@ -221,6 +229,17 @@ class Detector:
         _check(lib().rtod_kernel_times(self._h, names, avg, cnt, C.byref(n)))
         return {names[i].decode(): (avg[i], cnt[i]) for i in range(n.value)}
 
+    def get_predictor(self) -> np.ndarray:
+        """Current predictor parameters (learned schedule only)."""
+        out = np.empty(MLP_NPARAMS, np.float32)
+        _check(lib().rtod_get_predictor(self._h, out, MLP_NPARAMS))
+        return out
+
+    def set_predictor(self, params: np.ndarray, frozen: bool = False) -> None:
+        """Replace the predictor (optimiser state cleared); frozen stops training."""
+        p = np.ascontiguousarray(params, np.float32).ravel()
+        _check(lib().rtod_set_predictor(self._h, p, p.size, 1 if frozen else 0))
+
     def reset_kernel_times(self) -> None:
         _check(lib().rtod_reset_kernel_times(self._h))
 
@@ -263,6 +282,26 @@ class DeviceStream:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def mlp_forward(params: np.ndarray, X: np.ndarray, device: int = 0) -> np.ndarray:
+    """The detector's predictor forward pass (matrix cores) on given features."""
+    X = np.ascontiguousarray(X, np.float32).reshape(-1, 6)
+    out = np.empty(X.shape[0], np.float32)
+    _check(lib().rtod_mlp_forward(device, np.ascontiguousarray(params, np.float32).ravel(),
+                                  X.ravel(), X.shape[0], out))
+    return out
+
+
+def mlp_grad(params: np.ndarray, X: np.ndarray, y: np.ndarray, device: int = 0):
+    """Loss and gradient by the detector's training kernels (n <= 128)."""
+    X = np.ascontiguousarray(X, np.float32).reshape(-1, 6)
+    g = np.empty(MLP_NPARAMS, np.float32)
+    loss = C.c_float(0.0)
+    _check(lib().rtod_mlp_grad(device, np.ascontiguousarray(params, np.float32).ravel(),
+                               X.ravel(), np.ascontiguousarray(y, np.float32).ravel(),
+                               X.shape[0], g, C.byref(loss)))
+    return float(loss.value), g
 
 
 def version() -> str:

@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "librtod.so")
 ODS = os.path.join(HERE, "ods")
 
 HIP_SOURCES = ["rtod_lib.hip"]
-HIP_DEPS = ["rtod_device.h", "rtod_state.h", "rtod_plain.h", "rtod_lbvh.h"]
+HIP_DEPS = ["rtod_device.h", "rtod_state.h", "rtod_plain.h", "rtod_lbvh.h", "rtod_learned.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

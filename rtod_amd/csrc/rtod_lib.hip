@@ -88,6 +88,7 @@ struct rtod_ctx {
     HostOut *h_out = nullptr;
     const float *dev_src = nullptr; // device-resident new slide of this call
     bool lbvh = false;
+    bool learned = false;
     Lbvh L{};
     int lbvh_passes = 0;
     std::vector<void *> dev_allocs;
@@ -102,6 +103,9 @@ struct rtod_ctx {
     // stats
     int64_t cand_last = 0, cand_total = 0, tests_last = 0, tests_total = 0;
     int64_t out_last = 0, out_total = 0;
+    int64_t misses_total = 0, pred_points = 0, fallbacks = 0, train_steps = 0;
+    double train_loss_last = 0.0;
+    bool used_pred_last = false;
 };
 
 namespace {
@@ -202,7 +206,8 @@ void launch_batch_merges(rtod_ctx *c, hipStream_t s, const Step &st, int mode, i
 
 void launch_merge_window(rtod_ctx *c, hipStream_t s, const Step &st)
 {
-    const int grid = (c->W + 1023) / 1024;
+    // learned: one more workgroup trains the predictor beside the merge
+    const int grid = (c->W + 1023) / 1024 + (c->learned ? 1 : 0);
     if (c->dim == 1)
         hipLaunchKernelGGL(k_merge_window<1>, dim3(grid), dim3(256), 0, s, c->d, st);
     else if (c->dim == 2)
@@ -211,26 +216,44 @@ void launch_merge_window(rtod_ctx *c, hipStream_t s, const Step &st)
         hipLaunchKernelGGL(k_merge_window<3>, dim3(grid), dim3(256), 0, s, c->d, st);
 }
 
-void launch_classify(rtod_ctx *c, hipStream_t s, const Step &st)
+template <int DIM> void launch_classify_dim(rtod_ctx *c, hipStream_t s, const Step &st)
 {
     const int grid = (c->W + 255) / 256;
-    if (c->dim == 1)
-        hipLaunchKernelGGL(k_classify<1>, dim3(grid), dim3(256), 0, s, c->d, st);
-    else if (c->dim == 2)
-        hipLaunchKernelGGL(k_classify<2>, dim3(grid), dim3(256), 0, s, c->d, st);
+    if (c->learned)
+        hipLaunchKernelGGL((k_classify<DIM, true>), dim3(grid), dim3(kClassifyNT), 0, s, c->d,
+                           st);
     else
-        hipLaunchKernelGGL(k_classify<3>, dim3(grid), dim3(256), 0, s, c->d, st);
+        hipLaunchKernelGGL((k_classify<DIM, false>), dim3(grid), dim3(kClassifyNT), 0, s, c->d,
+                           st);
+}
+
+void launch_classify(rtod_ctx *c, hipStream_t s, const Step &st)
+{
+    if (c->dim == 1)
+        launch_classify_dim<1>(c, s, st);
+    else if (c->dim == 2)
+        launch_classify_dim<2>(c, s, st);
+    else
+        launch_classify_dim<3>(c, s, st);
+}
+
+template <int DIM> void launch_detect_dim(rtod_ctx *c, hipStream_t s, const Step &st)
+{
+    const int grid = c->detect_blocks;
+    if (c->learned)
+        hipLaunchKernelGGL((k_detect<DIM, true>), dim3(grid), dim3(kDetectNT), 0, s, c->d, st);
+    else
+        hipLaunchKernelGGL((k_detect<DIM, false>), dim3(grid), dim3(kDetectNT), 0, s, c->d, st);
 }
 
 void launch_detect(rtod_ctx *c, hipStream_t s, const Step &st)
 {
-    const int grid = c->detect_blocks;
     if (c->dim == 1)
-        hipLaunchKernelGGL(k_detect<1>, dim3(grid), dim3(kDetectNT), 0, s, c->d, st);
+        launch_detect_dim<1>(c, s, st);
     else if (c->dim == 2)
-        hipLaunchKernelGGL(k_detect<2>, dim3(grid), dim3(kDetectNT), 0, s, c->d, st);
+        launch_detect_dim<2>(c, s, st);
     else
-        hipLaunchKernelGGL(k_detect<3>, dim3(grid), dim3(kDetectNT), 0, s, c->d, st);
+        launch_detect_dim<3>(c, s, st);
 }
 
 // LBVH index: codes, tile sort, merges up to U = W, hierarchy, boxes
@@ -356,10 +379,44 @@ rtod_status validate(const rtod_config *cfg)
         return RTOD_ERR_INVALID_ARGUMENT;
     }
     if (cfg->sched_kind == RTOD_SCHED_LEARNED) {
-        set_error("the learned scheduler is not available in this build");
-        return RTOD_ERR_UNSUPPORTED;
+        if (cfg->index_kind == RTOD_INDEX_LBVH) {
+            set_error("the learned scheduler orders the cell-stencil range counts; "
+                      "it does not apply to the LBVH index");
+            return RTOD_ERR_UNSUPPORTED;
+        }
+        if (cfg->window > (1 << 22)) {
+            set_error("the learned scheduler's replay (720 bytes per window point) is "
+                      "limited to W <= 2^22");
+            return RTOD_ERR_UNSUPPORTED;
+        }
+        if (!(cfg->miss_threshold >= 0.f && cfg->miss_threshold <= 1.f)) {
+            set_error("miss_threshold must be in [0, 1] (0 selects the default 0.25)");
+            return RTOD_ERR_INVALID_ARGUMENT;
+        }
     }
     return RTOD_OK;
+}
+
+// He-uniform hidden weights from a fixed seed, zero biases: the untrained
+// network outputs ~0.5 everywhere (every candidate a probable outlier, i.e.
+// the plain schedule) until training moves it.
+std::vector<float> initial_predictor()
+{
+    std::vector<float> p(kMlpNP, 0.f);
+    uint64_t x = 0x9e3779b97f4a7c15ull;
+    auto uni = [&x]() {
+        x ^= x << 13;
+        x ^= x >> 7;
+        x ^= x << 17;
+        return (double)(x >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+    };
+    for (int i = 0; i < kMlpIn * kMlpH; ++i)
+        p[kOffW1 + i] = (float)(uni() * std::sqrt(6.0 / kMlpIn));
+    for (int i = 0; i < kMlpH * kMlpH; ++i)
+        p[kOffW2 + i] = (float)(uni() * std::sqrt(6.0 / kMlpH));
+    for (int i = 0; i < kMlpH; ++i)
+        p[kOffW3 + i] = (float)(uni() * std::sqrt(6.0 / kMlpH) * 0.1);
+    return p;
 }
 
 rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out)
@@ -452,6 +509,26 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out)
             ++passes;
         c->lbvh_passes = passes;
     }
+    c->learned = cfg->sched_kind == RTOD_SCHED_LEARNED;
+    d.lr.on = c->learned ? 1 : 0;
+    if (c->learned) {
+        Learn &lr = d.lr;
+        lr.miss_thr = cfg->miss_threshold > 0.f ? cfg->miss_threshold : 0.25f;
+        RTOD_TRY(dalloc(c, &lr.params, (size_t)kMlpNP));
+        RTOD_TRY(dalloc(c, &lr.adam_m, (size_t)kMlpNP));
+        RTOD_TRY(dalloc(c, &lr.adam_v, (size_t)kMlpNP));
+        RTOD_TRY(dalloc(c, &lr.rep_x, (size_t)kReplaySlides * W * kFeatStride));
+        RTOD_TRY(dalloc(c, &lr.rep_c, (size_t)kReplaySlides * W));
+        RTOD_TRY(dalloc(c, &lr.rep_n, (size_t)2 * kReplaySlides));
+        RTOD_TRY(dalloc(c, &lr.lc, 1));
+        RTOD_HIP(hipMemsetAsync(lr.rep_n, 0, sizeof(int32_t) * 2 * kReplaySlides, c->stream));
+        RTOD_HIP(hipMemsetAsync(lr.lc, 0, sizeof(LearnCtl), c->stream));
+        std::vector<float> p0 = initial_predictor();
+        RTOD_HIP(hipMemcpyAsync(lr.params, p0.data(), sizeof(float) * kMlpNP,
+                                hipMemcpyHostToDevice, c->stream));
+        RTOD_HIP(hipMemsetAsync(lr.adam_m, 0, sizeof(float) * kMlpNP, c->stream));
+        RTOD_HIP(hipMemsetAsync(lr.adam_v, 0, sizeof(float) * kMlpNP, c->stream));
+    }
     const size_t in_bytes = 2 * S * dim * sizeof(float);
     const size_t out_bytes = sizeof(HostOut) + W * sizeof(int32_t);
     RTOD_HIP(hipHostMalloc((void **)&c->h_in, in_bytes, hipHostMallocMapped | hipHostMallocCoherent));
@@ -516,6 +593,10 @@ rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n)
                        0);
     RTOD_HIP(hipMemsetAsync(c->d.bitmap, 0, sizeof(uint32_t) * ((c->W + 31) / 32), s));
     RTOD_HIP(hipMemsetAsync(c->d.ctl, 0, sizeof(Ctl), s));
+    if (c->learned) { // replay and per-slide counters restart; weights stay
+        RTOD_HIP(hipMemsetAsync(c->d.lr.rep_n, 0, sizeof(int32_t) * 2 * kReplaySlides, s));
+        RTOD_HIP(hipMemsetAsync(c->d.lr.lc, 0, kLearnCtlResetBytes, s));
+    }
     RTOD_HIP(hipGetLastError());
     RTOD_HIP(hipStreamSynchronize(s));
     c->h_out->slide = 0;
@@ -554,6 +635,16 @@ rtod_status collect(rtod_ctx *c, int32_t *outs, int32_t *n_out)
     c->tests_total += (int64_t)ho->pair_tests;
     c->out_last = n;
     c->out_total += n;
+    if (c->learned) {
+        c->misses_total += ho->misses;
+        c->pred_points += ho->n_cand;
+        c->used_pred_last = ho->used_pred != 0;
+        if (!ho->used_pred)
+            c->fallbacks++;
+        c->train_steps += ho->train_steps;
+        if (ho->train_steps > 0)
+            c->train_loss_last = ho->train_loss;
+    }
     return RTOD_OK;
 }
 
@@ -851,7 +942,134 @@ RTOD_API rtod_status rtod_get_stats(const rtod_ctx *c, rtod_stats *o)
     o->device_bytes = c->dev_bytes;
     o->host_pinned_bytes = c->pinned_bytes;
     o->index_kind = c->lbvh ? RTOD_INDEX_LBVH : RTOD_INDEX_CELL_STENCIL;
-    o->sched_kind = RTOD_SCHED_PLAIN;
+    o->sched_kind = c->used_pred_last ? RTOD_SCHED_LEARNED : RTOD_SCHED_PLAIN;
+    o->predictor_misses = c->misses_total;
+    o->predictor_points = c->pred_points;
+    o->fallbacks = c->fallbacks;
+    o->train_steps = c->train_steps;
+    o->train_loss_last = c->train_loss_last;
+    return RTOD_OK;
+}
+
+RTOD_API rtod_status rtod_get_predictor(rtod_ctx *c, float *params, int32_t n)
+{
+    if (!c || !params) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (!c->learned) {
+        set_error("the context has no predictor (sched_kind is not LEARNED)");
+        return RTOD_ERR_STATE;
+    }
+    if (n != kMlpNP) {
+        set_error("predictor has RTOD_MLP_NPARAMS parameters");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    RTOD_HIP(hipSetDevice(c->device));
+    RTOD_HIP(hipMemcpyAsync(params, c->d.lr.params, sizeof(float) * kMlpNP,
+                            hipMemcpyDeviceToHost, c->stream));
+    RTOD_HIP(hipStreamSynchronize(c->stream));
+    return RTOD_OK;
+}
+
+RTOD_API rtod_status rtod_set_predictor(rtod_ctx *c, const float *params, int32_t n,
+                                        int32_t frozen)
+{
+    if (!c || !params) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (!c->learned) {
+        set_error("the context has no predictor (sched_kind is not LEARNED)");
+        return RTOD_ERR_STATE;
+    }
+    if (n != kMlpNP) {
+        set_error("predictor has RTOD_MLP_NPARAMS parameters");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (c->staged) {
+        set_error("a phase-API slide is in progress");
+        return RTOD_ERR_STATE;
+    }
+    RTOD_HIP(hipSetDevice(c->device));
+    const Learn &lr = c->d.lr;
+    RTOD_HIP(hipMemcpyAsync(lr.params, params, sizeof(float) * kMlpNP, hipMemcpyHostToDevice,
+                            c->stream));
+    RTOD_HIP(hipMemsetAsync(lr.adam_m, 0, sizeof(float) * kMlpNP, c->stream));
+    RTOD_HIP(hipMemsetAsync(lr.adam_v, 0, sizeof(float) * kMlpNP, c->stream));
+    const int32_t kept[2] = {0, frozen ? 1 : 0}; // adam_t, frozen
+    RTOD_HIP(hipMemcpyAsync(&lr.lc->adam_t, kept, sizeof(kept), hipMemcpyHostToDevice,
+                            c->stream));
+    RTOD_HIP(hipStreamSynchronize(c->stream));
+    return RTOD_OK;
+}
+
+namespace {
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf()
+    {
+        if (p)
+            (void)hipFree(p);
+    }
+};
+} // namespace
+
+RTOD_API rtod_status rtod_mlp_forward(int32_t device, const float *params, const float *X,
+                                      int32_t n, float *out)
+{
+    if (!params || (n > 0 && (!X || !out))) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (n < 0 || n > (1 << 24)) {
+        set_error("n must be in [0, 2^24]");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (n == 0)
+        return RTOD_OK;
+    RTOD_HIP(hipSetDevice(device));
+    DevBuf bp, bx, bo;
+    RTOD_HIP(hipMalloc(&bp.p, sizeof(float) * kMlpNP));
+    RTOD_HIP(hipMalloc(&bx.p, sizeof(float) * kMlpIn * (size_t)n));
+    RTOD_HIP(hipMalloc(&bo.p, sizeof(float) * (size_t)n));
+    RTOD_HIP(hipMemcpy(bp.p, params, sizeof(float) * kMlpNP, hipMemcpyHostToDevice));
+    RTOD_HIP(hipMemcpy(bx.p, X, sizeof(float) * kMlpIn * (size_t)n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_mlp_forward, dim3((n + 255) / 256), dim3(256), 0, nullptr,
+                       (const float *)bp.p, (const float *)bx.p, n, (float *)bo.p);
+    RTOD_HIP(hipGetLastError());
+    RTOD_HIP(hipMemcpy(out, bo.p, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+    return RTOD_OK;
+}
+
+RTOD_API rtod_status rtod_mlp_grad(int32_t device, const float *params, const float *X,
+                                   const float *y, int32_t n, float *grad, float *loss)
+{
+    if (!params || !X || !y || !grad || !loss) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (n < 1 || n > kTrainRows) {
+        set_error("n must be in [1, 128]");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    RTOD_HIP(hipSetDevice(device));
+    DevBuf bp, bx, by, bg;
+    RTOD_HIP(hipMalloc(&bp.p, sizeof(float) * kMlpNP));
+    RTOD_HIP(hipMalloc(&bx.p, sizeof(float) * kMlpIn * (size_t)n));
+    RTOD_HIP(hipMalloc(&by.p, sizeof(float) * (size_t)n));
+    RTOD_HIP(hipMalloc(&bg.p, sizeof(float) * (kMlpNP + 1)));
+    RTOD_HIP(hipMemcpy(bp.p, params, sizeof(float) * kMlpNP, hipMemcpyHostToDevice));
+    RTOD_HIP(hipMemcpy(bx.p, X, sizeof(float) * kMlpIn * (size_t)n, hipMemcpyHostToDevice));
+    RTOD_HIP(hipMemcpy(by.p, y, sizeof(float) * (size_t)n, hipMemcpyHostToDevice));
+    float *g = (float *)bg.p;
+    hipLaunchKernelGGL(k_mlp_grad, dim3(1), dim3(kTrainNT), 0, nullptr, (const float *)bp.p,
+                       (const float *)bx.p, (const float *)by.p, n, g, g + kMlpNP);
+    RTOD_HIP(hipGetLastError());
+    std::vector<float> h(kMlpNP + 1);
+    RTOD_HIP(hipMemcpy(h.data(), g, sizeof(float) * (kMlpNP + 1), hipMemcpyDeviceToHost));
+    std::memcpy(grad, h.data(), sizeof(float) * kMlpNP);
+    *loss = h[kMlpNP];
     return RTOD_OK;
 }
 

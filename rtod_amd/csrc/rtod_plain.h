@@ -21,6 +21,7 @@
 #pragma once
 
 #include "rtod_device.h"
+#include "rtod_learned.h"
 #include "rtod_state.h"
 
 namespace rtod {
@@ -447,8 +448,8 @@ __global__ __launch_bounds__(256) void k_merge_window(Dev d, Step st)
     constexpr int NT = 256, IT = 4, T = NT * IT, CAP = 2048, MAXBS = 512;
     __shared__ uint64_t s_k[T];
     __shared__ uint64_t s_bs[MAXBS], s_es[MAXBS];
-    __shared__ uint64_t s_as[kMaxSamples];
-    __shared__ uint64_t s_br[CAP];
+    __shared__ __attribute__((aligned(16))) uint64_t s_as[kMaxSamples];
+    __shared__ __attribute__((aligned(16))) uint64_t s_br[CAP];
     __shared__ int s_rc[IT][NT / kWave];
     __shared__ int s_res[8];
     __shared__ uint64_t s_kn;
@@ -456,6 +457,15 @@ __global__ __launch_bounds__(256) void k_merge_window(Dev d, Step st)
 
     const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
     const int W = d.W, S = d.S;
+    // a learned context appends one training workgroup to the merge grid
+    const int nmb = (W + T - 1) / T;
+    static_assert(4 * sizeof(TrainWaveLds) <= sizeof(s_as), "training LDS");
+    static_assert(sizeof(TrainLds) <= sizeof(s_br), "training LDS");
+    if ((int)blockIdx.x >= nmb) {
+        if (d.lr.on)
+            train_block(d, nmb, s_as, s_br);
+        return;
+    }
     const int par = st.par;
     const uint64_t *Ak = d.akey[par];
     const int32_t *As = d.aslot[par];
@@ -482,11 +492,14 @@ __global__ __launch_bounds__(256) void k_merge_window(Dev d, Step st)
     const int sl2 = d.stride_log2, smask = (1 << sl2) - 1;
     const int bsl = d.bs_log2, nbs = d.nbs;
     // this workgroup's share of B
-    const int bper = (S + gridDim.x - 1) / gridDim.x;
+    const int bper = (S + nmb - 1) / nmb;
     const int j0 = min(S, (int)blockIdx.x * bper), j1 = min(S, j0 + bper);
 
-    if (blockIdx.x == 0 && tid == 0)
+    if (blockIdx.x == 0 && tid == 0) {
         d.ctl->n_cand = 0;
+        if (d.lr.on)
+            d.lr.lc->n_cand_i = 0;
+    }
 
     // round 1: the chunk (keys, slots, batch positions, coordinates; striped,
     // item i of thread t is chunk element i*NT + t), the share of B, and the
@@ -661,6 +674,11 @@ __global__ __launch_bounds__(256) void k_merge_window(Dev d, Step st)
         if ((np & smask) == 0)
             Nsamp[np >> sl2] = key;
     }
+    if (d.lr.on) { // tells the training workgroup when to stop stepping
+        __syncthreads();
+        if (tid == 0)
+            atomicAdd(&d.lr.lc->mw_done, 1);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -670,63 +688,128 @@ __global__ __launch_bounds__(256) void k_merge_window(Dev d, Step st)
 // run start s within [i-K, i] and test key[s+K] == key.  The keys around the
 // tile sit in LDS when K is small.  Other points are appended to the
 // candidate list, one atomic per wavefront.
+// LEARN: each wavefront holding candidates also computes their features
+// (own-cell count, the two neighbouring cells' counts along the last axis,
+// offsets from the cell centre), runs the predictor on the matrix cores, and
+// files predicted outliers at the front of the list (ctl->n_cand) and
+// predicted inliers at the back (index W-1-k, lc->n_cand_i); the features go
+// to the replay slot of this slide under the same list index.
 // ---------------------------------------------------------------------------
 constexpr int kClassifyNT = 256;
 constexpr int kClassifyHalo = 512;
 
-template <int DIM>
+template <int DIM, bool LEARN>
 __global__ __launch_bounds__(kClassifyNT) void k_classify(Dev d, Step st)
 {
-    constexpr int NT = kClassifyNT, HALO = kClassifyHalo;
+    constexpr int NT = kClassifyNT, HALO = kClassifyHalo, NW = NT / kWave;
     __shared__ uint64_t s_t[NT + 2 * HALO];
+    __shared__ __attribute__((aligned(16))) __bf16 s_h[LEARN ? NW : 1][16 * kMlpH];
+    __shared__ float s_z[LEARN ? NW : 1][kWave];
     const int i0 = blockIdx.x * NT;
     const int i = i0 + threadIdx.x;
     const uint64_t *Nk = d.akey[st.par ^ 1];
     const int W = d.W, K = d.K;
-    const bool tiled = K <= HALO;
-    // tile = keys [i0 - K, i0 + NT + K) clipped to [0, W)
-    const int t0 = max(0, i0 - K), t1 = min(W, i0 + NT + K);
+    // keys read: [i-K, i] for the run start; learned features reach 2K+2
+    const int H = LEARN ? 2 * K + 2 : K;
+    const bool tiled = H <= HALO;
+    const int t0 = max(0, i0 - H), t1 = min(W, i0 + NT + H);
     if (tiled)
         copy_to_lds<(NT + 2 * HALO) / NT>(s_t, Nk + t0, t1 - t0);
     __syncthreads();
+    auto kat = [&](int j) -> uint64_t { return tiled ? s_t[j - t0] : Nk[j]; };
+    // first index in [max(0, p-cap), p] holding `key` (key[p] == key)
+    auto run_lo = [&](int p, uint64_t key, int cap) {
+        int lo = max(0, p - cap), hi = p;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (kat(mid) == key)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        return lo;
+    };
+    // last index in [p, min(W-1, p+cap)] holding `key` (key[p] == key)
+    auto run_hi = [&](int p, uint64_t key, int cap) {
+        int lo = p, hi = min(W - 1, p + cap);
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (kat(mid) == key)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        return lo;
+    };
     bool cand = false;
+    uint64_t key = 0;
     if (i < W) {
-        const uint64_t key = tiled ? s_t[i - t0] : Nk[i];
+        key = kat(i);
         bool dense = false;
         if (K < W && key_guaranteed<DIM>(key)) {
-            int lo = max(0, i - K), hi = i;
-            if (tiled) {
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (s_t[mid - t0] == key)
-                        hi = mid;
-                    else
-                        lo = mid + 1;
-                }
-                dense = (lo + K < W) && (s_t[lo + K - t0] == key);
-            } else {
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (Nk[mid] == key)
-                        hi = mid;
-                    else
-                        lo = mid + 1;
-                }
-                dense = (lo + K < W) && (Nk[lo + K] == key);
-            }
+            const int lo = run_lo(i, key, K);
+            dense = (lo + K < W) && (kat(lo + K) == key);
         }
         cand = !dense;
     }
-    const uint64_t mask = __ballot(cand);
-    const int cnt = __popcll(mask);
-    if (cnt) {
-        int base = 0;
-        if (lane_id() == 0)
-            base = atomicAdd(&d.ctl->n_cand, cnt);
-        base = __shfl(base, 0, kWave);
+    const int lane = lane_id();
+    if constexpr (!LEARN) {
+        const uint64_t mask = __ballot(cand);
+        const int cnt = __popcll(mask);
+        if (cnt) {
+            int base = 0;
+            if (lane == 0)
+                base = atomicAdd(&d.ctl->n_cand, cnt);
+            base = __shfl(base, 0, kWave);
+            if (cand) {
+                const uint64_t below = mask & ((1ull << lane) - 1ull);
+                d.cand[base + __popcll(below)] = i;
+            }
+        }
+    } else {
+        if (__ballot(cand) == 0)
+            return;
+        const int wid = threadIdx.x / kWave;
+        constexpr uint64_t FM = CellBits<DIM>::FMASK;
+        const float capf = 1.f / (float)(K + 1);
+        float f[kMlpIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         if (cand) {
-            const uint64_t below = mask & ((1ull << lane_id()) - 1ull);
-            d.cand[base + __popcll(below)] = i;
+            const int a = run_lo(i, key, K), b = run_hi(i, key, K);
+            f[0] = (float)min(b - a + 1, K + 1) * capf;
+            const uint64_t last = key & FM;
+            if (last > 0 && a > 0 && kat(a - 1) == key - 1)
+                f[1] = (float)min(a - run_lo(a - 1, key - 1, K), K + 1) * capf;
+            if (last < FM && b + 1 < W && kat(b + 1) == key + 1)
+                f[2] = (float)min(run_hi(b + 1, key + 1, K) - b, K + 1) * capf;
+            const float *Nx = d.axyz[st.par ^ 1];
+#pragma unroll
+            for (int ax = 0; ax < DIM; ++ax) {
+                const double u = (double)Nx[(size_t)ax * W + i] * d.inv_c;
+                const float off = (float)(u - floor(u)) - 0.5f;
+                f[3 + ax] = (off >= -0.5f && off <= 0.5f) ? off : 0.f; // non-finite -> 0
+            }
+        }
+        const float yh = mlp_infer_wave(d.lr.params, f, s_h[wid], s_z[wid]);
+        const bool pin = cand && yh >= kInlierCut;
+        const bool pout = cand && !pin;
+        const uint64_t mo = __ballot(pout), mi = __ballot(pin);
+        int bo = 0, bi = 0;
+        if (lane == 0) {
+            if (mo)
+                bo = atomicAdd(&d.ctl->n_cand, __popcll(mo));
+            if (mi)
+                bi = atomicAdd(&d.lr.lc->n_cand_i, __popcll(mi));
+        }
+        bo = __shfl(bo, 0, kWave);
+        bi = __shfl(bi, 0, kWave);
+        if (cand) {
+            const uint64_t lt = (1ull << lane) - 1ull;
+            const int li = pout ? bo + __popcll(mo & lt) : W - 1 - (bi + __popcll(mi & lt));
+            d.cand[li] = i;
+            float *rx = d.lr.rep_x +
+                        ((size_t)(st.t % kReplaySlides) * W + li) * kFeatStride;
+            *reinterpret_cast<float4 *>(rx) = make_float4(f[0], f[1], f[2], f[3]);
+            *reinterpret_cast<float4 *>(rx + 4) = make_float4(f[4], f[5], 0.f, 0.f);
         }
     }
 }
@@ -871,22 +954,18 @@ __device__ __forceinline__ int row_bounds(const Dev &d, const uint64_t *Nk,
     return res;
 }
 
-// Exact neighbour count of the candidate at sorted position p, capped: stops
-// as soon as the count exceeds K.  Wave-cooperative (all 64 lanes call it
-// with the same p).  *tests += predicate evaluations.
+// Exact count of the window points q in the cells of stencil `st` with
+// d2(c, q) <= r2, capped: stops as soon as the count exceeds K.
+// Wave-cooperative (all 64 lanes call it with the same arguments).
+// tests += predicate evaluations.
 template <int DIM>
-__device__ __forceinline__ int range_count(const Dev &d, const uint64_t *Nk, const float *Nx,
-                                           const uint64_t *s_samp, int *w_a, int *w_b, int p,
-                                           long long &tests)
+__device__ __forceinline__ int range_count_st(const Dev &d, const uint64_t *Nk, const float *Nx,
+                                              const uint64_t *s_samp, int *w_a, int *w_b,
+                                              const float (&c)[3], Stencil<DIM> st,
+                                              long long &tests)
 {
     const int lane = lane_id();
     const int W = d.W, K = d.K;
-    float c[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-    for (int a = 0; a < DIM; ++a)
-        c[a] = Nx[(size_t)a * W + p];
-    Stencil<DIM> st;
-    make_stencil<DIM>(c, d.inv_c, d.Rp, st);
     if (st.nrows > 32) { // cannot happen: <= 5 cells per axis
         if (lane == 0)
             atomicOr(&d.ctl->err, 4);
@@ -940,6 +1019,63 @@ __device__ __forceinline__ int range_count(const Dev &d, const uint64_t *Nk, con
             break;
     }
     return cnt;
+}
+
+template <int DIM>
+__device__ __forceinline__ void load_point(const float *Nx, int W, int p, float (&c)[3])
+{
+    c[0] = c[1] = c[2] = 0.f;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a)
+        c[a] = Nx[(size_t)a * W + p];
+}
+
+// Capped neighbour count of the candidate at sorted position p over its full
+// stencil.
+template <int DIM>
+__device__ __forceinline__ int range_count(const Dev &d, const uint64_t *Nk, const float *Nx,
+                                           const uint64_t *s_samp, int *w_a, int *w_b, int p,
+                                           long long &tests)
+{
+    float c[3];
+    load_point<DIM>(Nx, d.W, p, c);
+    Stencil<DIM> st;
+    make_stencil<DIM>(c, d.inv_c, d.Rp, st);
+    return range_count_st<DIM>(d, Nk, Nx, s_samp, w_a, w_b, c, st, tests);
+}
+
+// Near-first count for a probable inlier (2-D and 3-D): the stencil cut to the
+// 3^d cells around the candidate's own cell first; only if that subset holds
+// <= K neighbours is the full stencil counted.  A subset count > K proves
+// count > K, so the capped result equals range_count's.
+template <int DIM>
+__device__ __forceinline__ int range_count_near_first(const Dev &d, const uint64_t *Nk,
+                                                      const float *Nx, const uint64_t *s_samp,
+                                                      int *w_a, int *w_b, int p,
+                                                      long long &tests)
+{
+    float c[3];
+    load_point<DIM>(Nx, d.W, p, c);
+    Stencil<DIM> full;
+    make_stencil<DIM>(c, d.inv_c, d.Rp, full);
+    Stencil<DIM> nr = full;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+        const int own = cell_coord<DIM>((double)c[a], d.inv_c);
+        nr.lo[a] = max(full.lo[a], own - 1);
+        nr.hi[a] = min(full.hi[a], own + 1);
+    }
+    if constexpr (DIM == 2) {
+        nr.nrows = nr.hi[0] - nr.lo[0] + 1;
+    } else if constexpr (DIM == 3) {
+        nr.n1 = nr.hi[1] - nr.lo[1] + 1;
+        nr.nrows = (nr.hi[0] - nr.lo[0] + 1) * nr.n1;
+    }
+    const int cnt = range_count_st<DIM>(d, Nk, Nx, s_samp, w_a, w_b, c, nr, tests);
+    if (cnt > d.K)
+        return cnt;
+    wave_sync_lds();
+    return range_count_st<DIM>(d, Nk, Nx, s_samp, w_a, w_b, c, full, tests);
 }
 
 // ---------------------------------------------------------------------------
@@ -1025,8 +1161,14 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int n
 // list).  Only ceil(n_cand / 4) workgroups take part (the rest leave at
 // once); the last of them to finish compacts the outlier bitmap into an
 // ascending slot list written straight into pinned host memory.
+// LEARN: the list is the predicted-outlier bucket [0, nO) followed by the
+// predicted-inlier bucket (indices W-1, W-2, ...): the long full counts are
+// handed out first.  When the schedule is on (lc->use_pred), predicted
+// inliers take the near-first count (2-D/3-D).  Every count is exact; each
+// is stored in the replay, and mispredictions are counted for the fallback
+// rule.
 // ---------------------------------------------------------------------------
-template <int DIM>
+template <int DIM, bool LEARN>
 __global__ __launch_bounds__(kDetectNT) void k_detect(Dev d, Step st)
 {
     constexpr int NW = kDetectNT / kWave;
@@ -1035,42 +1177,63 @@ __global__ __launch_bounds__(kDetectNT) void k_detect(Dev d, Step st)
     __shared__ int s_scan[NW];
     __shared__ int s_last;
     __shared__ unsigned long long s_tests;
+    __shared__ int s_miss;
 
     const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
     const int par = st.par ^ 1;
     const uint64_t *Nk = d.akey[par];
     const int32_t *Ns = d.aslot[par];
     const float *Nx = d.axyz[par];
-    const int n_cand = d.ctl->n_cand;
-    const int K = d.K;
+    const int W = d.W, K = d.K;
+    const int n_o = d.ctl->n_cand;
+    const int n_cand = LEARN ? n_o + d.lr.lc->n_cand_i : n_o;
+    const bool use_pred = LEARN && d.lr.lc->use_pred != 0;
     const int busy = min((int)gridDim.x, max(1, (n_cand + NW - 1) / NW));
     if ((int)blockIdx.x >= busy)
         return;
 
-    if (tid == 0)
+    if (tid == 0) {
         s_tests = 0;
+        s_miss = 0;
+    }
     copy_to_lds<kMaxSamples / kDetectNT>(s_samp, d.samples[par], d.nsamp);
     __syncthreads();
 
     long long tests = 0;
+    int misses = 0;
     for (int ci = blockIdx.x * NW + wid; ci < n_cand; ci += busy * NW) {
-        const int p = __builtin_amdgcn_readfirstlane(d.cand[ci]);
-        const int cnt =
-            range_count<DIM>(d, Nk, Nx, s_samp, s_rows[wid][0], s_rows[wid][1], p, tests);
+        const int li = (!LEARN || ci < n_o) ? ci : W - 1 - (ci - n_o);
+        const int p = __builtin_amdgcn_readfirstlane(d.cand[li]);
+        int cnt;
+        if (LEARN && DIM > 1 && use_pred && ci >= n_o)
+            cnt = range_count_near_first<DIM>(d, Nk, Nx, s_samp, s_rows[wid][0], s_rows[wid][1],
+                                              p, tests);
+        else
+            cnt = range_count<DIM>(d, Nk, Nx, s_samp, s_rows[wid][0], s_rows[wid][1], p,
+                                   tests);
         if (lane == 0) {
-            d.cand_cnt[ci] = min(cnt, K + 1);
+            const int capped = min(cnt, K + 1);
+            d.cand_cnt[ci] = capped;
             if (cnt <= K) {
                 const int slot = Ns[p];
                 atomicOr(&d.bitmap[slot >> 5], 1u << (slot & 31));
+            }
+            if constexpr (LEARN) {
+                d.lr.rep_c[(size_t)(st.t % kReplaySlides) * W + li] = capped;
+                misses += ((ci >= n_o) != (cnt > K)) ? 1 : 0;
             }
         }
     }
     if (lane == 0 && tests)
         atomicAdd(&s_tests, (unsigned long long)tests);
+    if (LEARN && lane == 0 && misses)
+        atomicAdd(&s_miss, misses);
     __syncthreads();
     if (tid == 0) {
         if (s_tests)
             atomicAdd(&d.ctl->pair_tests, s_tests);
+        if (LEARN && s_miss)
+            atomicAdd(&d.lr.lc->misses, s_miss);
         __threadfence();
         s_last = (atomicAdd(&d.ctl->ticket, 1u) == (unsigned)busy - 1);
     }
@@ -1079,6 +1242,29 @@ __global__ __launch_bounds__(kDetectNT) void k_detect(Dev d, Step st)
         return;
     __threadfence();
 
+    if constexpr (LEARN) {
+        if (tid == 0) {
+            LearnCtl *lc = d.lr.lc;
+            const int slot = (int)(st.t % kReplaySlides);
+            const int n_i = n_cand - n_o;
+            const int miss =
+                __hip_atomic_load(&lc->misses, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            d.lr.rep_n[2 * slot] = n_o;
+            d.lr.rep_n[2 * slot + 1] = n_i;
+            HostOut *ho = d.h_out;
+            ho->misses = miss;
+            ho->used_pred = use_pred ? 1 : 0;
+            ho->train_steps = lc->steps_last;
+            ho->train_loss = lc->loss_last;
+            // fallback rule: schedule the next slide by prediction only if
+            // this slide's miss rate is within the threshold
+            if (n_cand > 0)
+                lc->use_pred = ((float)miss <= d.lr.miss_thr * (float)n_cand) ? 1 : 0;
+            lc->misses = 0;
+            lc->mw_done = 0;
+            lc->steps_last = 0;
+        }
+    }
     finish_slide(d, st, n_cand, s_scan);
 }
 

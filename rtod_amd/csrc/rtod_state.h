@@ -17,6 +17,12 @@
 //   cand     i32 [W]            candidate positions (sorted-window index)
 //   bitmap   u32 [W/32]         outlier flags by ring slot
 //   ctl                         slide counter, candidate count, grid ticket
+// Learned scheduler (only when configured):
+//   params/adam_m/adam_v f32 [1313]   predictor MLP, FP32 master + Adam moments
+//   rep_x f32 [20][W][8]        replay: features of each candidate, by slide
+//                               (slot t mod 20) and candidate-list index
+//   rep_c i32 [20][W]           replay: its exact neighbour count, capped K+1
+//   rep_n i32 [20][2]           replay: bucket sizes of the stored slide
 // Pinned host memory, device-mapped:
 //   h_in  f32 [2][S*d]          new slide staging (row-major, as given)
 //   h_out                       outlier count + ascending slot list + counters
@@ -41,8 +47,38 @@ struct HostOut {
     int64_t slide;         // t + 1 after the slide's detect kernel
     unsigned long long pair_tests;
     int32_t err;
+    int32_t misses;        // learned: mispredicted candidates of the slide
+    int32_t used_pred;     // learned: the slide was scheduled by prediction
+    int32_t train_steps;   // learned: optimiser steps taken during the slide
+    float train_loss;      // learned: mean loss of those steps
     int32_t pad;
     int32_t list[1];       // W entries follow
+};
+
+// Device-side state of the learned scheduler.
+struct LearnCtl {
+    // cleared by rtod_prime
+    int32_t n_cand_i;      // predicted-inlier bucket size (current slide)
+    int32_t use_pred;      // schedule the current slide by prediction
+    int32_t misses;        // mispredictions (current slide)
+    int32_t mw_done;       // merge workgroups finished (current slide)
+    int32_t steps_last;    // optimiser steps of the last training block
+    float loss_last;       // their mean loss
+    // kept across rtod_prime
+    int32_t adam_t;        // optimiser steps since the weights were set
+    int32_t frozen;        // 1: no training
+};
+constexpr int kLearnCtlResetBytes = 6 * 4;
+
+struct Learn {
+    int on;                // learned schedule configured
+    float miss_thr;        // fall back to the plain schedule above this rate
+    float *params;         // [kMlpNP] FP32 master weights
+    float *adam_m, *adam_v;
+    float *rep_x;          // [kReplaySlides][W][kFeatStride]
+    int32_t *rep_c;        // [kReplaySlides][W]
+    int32_t *rep_n;        // [kReplaySlides][2]
+    LearnCtl *lc;
 };
 
 // Per-slide quantities, passed by value to every kernel of the slide.
@@ -86,6 +122,7 @@ struct Dev {
     Ctl *ctl;
     const float *h_in;
     HostOut *h_out;
+    Learn lr;
 };
 
 } // namespace rtod

@@ -60,7 +60,39 @@ def test_null_arguments_rejected():
 
 
 def test_struct_layouts_match_header():
-    # rtod_config: 10 x 4 bytes; rtod_times: 6 doubles; rtod_stats: 12 int64 + 2 int32
+    # rtod_config: 10 x 4 bytes; rtod_times: 6 doubles;
+    # rtod_stats: 13 int64 + 1 double + 2 int32
     assert C.sizeof(R._Config) == 40
     assert C.sizeof(R._Times) == 48
-    assert C.sizeof(R._Stats) == 12 * 8 + 2 * 4
+    assert C.sizeof(R._Stats) == 14 * 8 + 2 * 4
+
+
+@pytest.mark.parametrize("kw,status", [
+    # the learned schedule orders cell-stencil range counts only
+    (dict(index_kind=R.INDEX_LBVH), 2),
+    (dict(miss_threshold=-0.1), 1),
+    (dict(miss_threshold=1.5), 1),
+    (dict(miss_threshold=float("nan")), 1),
+    (dict(window=(1 << 22) + 16, slide=16), 2),   # replay size limit
+])
+def test_learned_config_rejected_without_device(kw, status):
+    base = dict(dim=3, window=1000, slide=100, R=1.0, K=5, sched_kind=R.SCHED_LEARNED)
+    base.update(kw)
+    with pytest.raises(R.RtodError) as e:
+        R.Detector(**base)
+    assert e.value.status == status
+
+
+def test_predictor_entry_points_reject_bad_arguments_without_device():
+    import numpy as np
+    lib = R.lib()
+    p = np.zeros(R.MLP_NPARAMS, np.float32)
+    x = np.zeros(6 * 4, np.float32)
+    y = np.zeros(4, np.float32)
+    g = np.zeros(R.MLP_NPARAMS, np.float32)
+    loss = C.c_float()
+    assert lib.rtod_get_predictor(None, p, R.MLP_NPARAMS) == 1
+    assert lib.rtod_set_predictor(None, p, R.MLP_NPARAMS, 0) == 1
+    assert lib.rtod_mlp_grad(0, p, x, y, 0, g, C.byref(loss)) == 1     # n < 1
+    assert lib.rtod_mlp_grad(0, p, x, y, 129, g, C.byref(loss)) == 1   # n > 128
+    assert lib.rtod_mlp_forward(0, p, x, -1, y) == 1

@@ -251,3 +251,98 @@ def test_streams_match_golden():
 def test_workload_constants():
     assert O.C1.T == 380 and O.C2.T == 189 and O.C3.T == 1130
     assert (O.C2.W, O.C2.S, O.C2.K, O.C2.R) == (100_000, 5_000, 50, 0.45)
+
+
+# ---------------------------------------------------------------------------
+# predictor MLP reference (learned scheduler)
+# ---------------------------------------------------------------------------
+def _bf16_rne_exact(x: float) -> float:
+    """Round-to-nearest-even onto the bfloat16 grid, by exact rationals."""
+    if x == 0.0 or not np.isfinite(x):
+        return x
+    s = -1.0 if x < 0 else 1.0
+    a = Fraction(abs(x))
+    e = 0
+    while a >= 2:
+        a /= 2
+        e += 1
+    while a < 1:
+        a *= 2
+        e -= 1
+    scaled = a * 128            # 7 explicit mantissa bits: grid step 1/128
+    lo = scaled.numerator // scaled.denominator
+    rem = scaled - lo
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and lo % 2 == 1):
+        lo += 1
+    return s * float(Fraction(lo, 128) * Fraction(2) ** e)
+
+
+def test_bf16_round_is_nearest_even():
+    f32 = lambda v: float(np.float32(v))
+    ties = [1 + 2 ** -8, 1 + 3 * 2 ** -8, -(1 + 2 ** -8), 3 * 2 ** -9 * (1 + 2 ** -8)]
+    assert O.bf16_round(ties[0]) == 1.0                    # tie -> even (down)
+    assert O.bf16_round(ties[1]) == 1 + 2 ** -6            # tie -> even (up)
+    assert O.bf16_round(f32(1 + 2 ** -8 + 2 ** -20)) == 1 + 2 ** -7
+    rng = np.random.default_rng(3)
+    vals = list(rng.normal(0, 10, 2000).astype(np.float32)) + ties
+    vals += list((rng.integers(-(2 ** 20), 2 ** 20, 500) * 2.0 ** -16).astype(np.float32))
+    for v in vals:
+        assert O.bf16_round(float(v)) == _bf16_rne_exact(f32(v)), v
+    assert np.isnan(O.bf16_round(float("nan")))
+    assert O.bf16_round(float("inf")) == float("inf")
+
+
+def _rand_params(rng, scale=0.4):
+    return rng.normal(0, scale, O.MLP_NP).astype(np.float32)
+
+
+def test_mlp_gradient_matches_central_differences():
+    rng = np.random.default_rng(11)
+    P = _rand_params(rng)
+    X = rng.uniform(-0.5, 1.0, (48, 6)).astype(np.float32)
+    y = rng.uniform(0, 1, 48).astype(np.float32)
+    loss, g = O.mlp_grad(P, X, y, bf16=False)
+    assert abs(loss - O.mlp_loss64(P.astype(np.float64), X, y)) < 1e-6
+    h = 1e-5
+    P64 = P.astype(np.float64)
+    fd = np.empty(O.MLP_NP)
+    for i in range(O.MLP_NP):
+        Pp = P64.copy(); Pp[i] += h
+        Pm = P64.copy(); Pm[i] -= h
+        fd[i] = (O.mlp_loss64(Pp, X, y) - O.mlp_loss64(Pm, X, y)) / (2 * h)
+    err = np.abs(g - fd)
+    tol = 1e-4 * np.abs(fd).max() + 1e-3 * np.abs(fd)
+    # a ReLU kink within h of a sample may spoil a handful of differences
+    assert np.count_nonzero(err > tol) <= 3
+    assert np.abs(fd).max() > 1e-3
+
+
+def test_mlp_closed_form_constant_network():
+    # all weights zero, output bias z0: yh = sigmoid(z0) for every input;
+    # the only non-zero gradient entry is b3 = mean(2 e yh (1-yh))
+    z0 = 0.3
+    P = np.zeros(O.MLP_NP, np.float32)
+    P[-1] = z0
+    rng = np.random.default_rng(5)
+    X = rng.uniform(-1, 1, (20, 6)).astype(np.float32)
+    y = rng.uniform(0, 1, 20).astype(np.float32)
+    yh = 1 / (1 + np.exp(-z0))
+    assert np.allclose(O.mlp_forward(P, X), yh, rtol=1e-6)
+    loss, g = O.mlp_grad(P, X, y)
+    assert np.isclose(loss, np.mean((yh - y) ** 2), rtol=1e-5)
+    assert np.isclose(g[-1], np.mean(2 * (yh - y) * yh * (1 - yh)), rtol=1e-5)
+    assert np.count_nonzero(g[:-1]) == 0
+
+
+def test_mlp_bf16_emulation_close_to_fp32():
+    rng = np.random.default_rng(8)
+    P = _rand_params(rng)
+    X = rng.uniform(-0.5, 1.0, (128, 6)).astype(np.float32)
+    y = rng.uniform(0, 1, 128).astype(np.float32)
+    a = O.mlp_forward(P, X, bf16=True)
+    b = O.mlp_forward(P, X, bf16=False)
+    assert np.abs(a - b).max() < 2e-2 and np.abs(a - b).max() > 0
+    _, ga = O.mlp_grad(P, X, y, bf16=True)
+    _, gb = O.mlp_grad(P, X, y, bf16=False)
+    assert 0 < np.abs(ga - gb).max() < 0.25 * np.abs(gb).max()
+    assert np.dot(ga, gb) / (np.linalg.norm(ga) * np.linalg.norm(gb)) > 0.99
