@@ -16,6 +16,11 @@ Extra objects on the line:
                 8 TB/s HBM peak, algorithmic bytes B = 4*d*(W+S) + 4*|O|
   cpu_baseline  the grid oracle (same algorithm, C, pthreads) timed on this
                 host's cores on a bounded sample of the same stream
+  learned       the same workloads with the learned work scheduler (online-
+                trained MLP ordering the exact range counts): per-slide time,
+                speed-up over the plain schedule, predictor miss rate, slides
+                that fell back to the plain schedule, and the number of slides
+                whose outlier list differs from the plain schedule's (must be 0)
 """
 from __future__ import annotations
 
@@ -56,7 +61,8 @@ def stream(name: str, slides: int) -> np.ndarray:
     return oracle.gen_walk(seed, n, dim) if gen == "walk" else oracle.gen_gmm(seed, n, dim)
 
 
-def run_gpu(xs, W, S, R, K, warmup, steps, flags=0, sched=0, device_input=True, index=0):
+def run_gpu(xs, W, S, R, K, warmup, steps, flags=0, sched=0, device_input=True, index=0,
+            keep_lists=False):
     """device_input: the stream sits in HBM (uploaded once, untimed) and each
     slide is fed by device pointer; else each slide's points come from host
     memory through the pinned staging buffer (PCIe inside the timed call)."""
@@ -66,20 +72,26 @@ def run_gpu(xs, W, S, R, K, warmup, steps, flags=0, sched=0, device_input=True, 
     det = rtod_amd.Detector(dim, W, S, R, K, flags=flags, sched_kind=sched, index_kind=index)
     ds = rtod_amd.DeviceStream(xs[: W + (warmup + steps) * S]) if device_input else None
     det.prime(xs[:W])
-    totals, outs = [], []
+    totals, outs, lists = [], [], []
     t_outer = 0.0
+    st0 = None
     for t in range(1, warmup + steps + 1):
         row = W + (t - 1) * S
         t0 = time.perf_counter()
         o = det.slide_device(ds.at(row)) if ds else det.slide(xs[row: row + S])
         dt = time.perf_counter() - t0
+        if t == warmup:
+            st0 = det.stats()
         if t > warmup:
             totals.append(det.times.total)
             outs.append(len(o))
             t_outer += dt
+            if keep_lists:
+                lists.append(o.copy())
     res = {"lib_ms": float(np.mean(totals)), "outer_ms": t_outer / max(1, steps) * 1e3,
            "outliers_mean": float(np.mean(outs)) if outs else 0.0,
-           "kernels": det.kernel_times(), "stats": det.stats()}
+           "kernels": det.kernel_times(), "stats": det.stats(), "stats_warm": st0,
+           "lists": lists}
     det.close()
     if ds:
         ds.close()
@@ -151,6 +163,46 @@ def index_comparison(names, steps):
     return out
 
 
+def learned_comparison(names, train_slides, steps):
+    """Plain vs learned schedule on the same slides: the learned detector
+    first trains online for `train_slides` slides (untimed warm-up, as the
+    plain run's warm-up), then both are timed over `steps` slides."""
+    import rtod_amd
+
+    out = {}
+    for name in names:
+        dim, N, W, S, R, K, gen, seed = WORKLOADS[name]
+        xs = stream(name, train_slides + steps)
+        pl = run_gpu(xs, W, S, R, K, train_slides, steps, keep_lists=True)
+        le = run_gpu(xs, W, S, R, K, train_slides, steps, sched=rtod_amd.SCHED_LEARNED,
+                     keep_lists=True)
+        mism = sum(0 if np.array_equal(a, b) else 1 for a, b in zip(pl["lists"], le["lists"]))
+        s1, s0 = le["stats"], le["stats_warm"]
+        pts = max(1, s1["predictor_points"] - s0["predictor_points"])
+        # kernel durations of both schedules (separate event-timed passes)
+        kp = run_gpu(xs, W, S, R, K, train_slides, min(steps, 60),
+                     flags=rtod_amd.FLAG_PHASE_TIMES)["kernels"]
+        kl = run_gpu(xs, W, S, R, K, train_slides, min(steps, 60),
+                     flags=rtod_amd.FLAG_PHASE_TIMES, sched=rtod_amd.SCHED_LEARNED)["kernels"]
+        ent = {"plain_us": round(pl["lib_ms"] * 1e3, 3),
+               "learned_us": round(le["lib_ms"] * 1e3, 3),
+               "speedup_vs_plain": round(pl["lib_ms"] / le["lib_ms"], 3),
+               "mismatched_slides": mism,
+               "miss_rate": round((s1["predictor_misses"] - s0["predictor_misses"]) / pts, 4),
+               "fallback_slides": s1["fallbacks"] - s0["fallbacks"],
+               "train_slides": train_slides,
+               "train_steps": s1["train_steps"],
+               "pair_tests_per_slide": {
+                   "plain": round((pl["stats"]["pair_tests_total"] -
+                                   pl["stats_warm"]["pair_tests_total"]) / steps),
+                   "learned": round((s1["pair_tests_total"] - s0["pair_tests_total"]) / steps)},
+               "kernels_ms_plain": {k: round(v[0], 5) for k, v in kp.items() if v[1] > 0},
+               "kernels_ms_learned": {k: round(v[0], 5) for k, v in kl.items() if v[1] > 0}}
+        out[name] = ent
+        log(f"[bench] learned {name}: {ent}")
+    return out
+
+
 def other_workloads(names, steps, cpu_slides, with_cpu):
     out = {}
     for name in names:
@@ -184,6 +236,10 @@ def main() -> int:
                     help="skip the 1-D W=10k and 3-D workloads")
     ap.add_argument("--workload", default="walk1d-W100k-S5k", choices=sorted(WORKLOADS),
                     help="profiling runs only: the JSON line always names the workload run")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-schedule comparison")
+    ap.add_argument("--train-slides", type=int, default=200,
+                    help="online-training warm-up of the learned schedule")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run only a plain 64 MiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -238,6 +294,13 @@ def main() -> int:
     if not args.no_others and args.workload == "walk1d-W100k-S5k":
         indexes = index_comparison(["walk1d-W100k-S5k", "gmm3d-W10k-S500"], 60)
 
+    learned = None
+    if not args.no_learned:
+        names = [args.workload]
+        if not args.no_others and args.workload == "walk1d-W100k-S5k":
+            names += ["walk1d-W10k-S500", "gmm3d-W10k-S500"]
+        learned = learned_comparison(names, args.train_slides, args.steps)
+
     cpu = None
     if not args.no_cpu:
         us_mt, us_st, threads = cpu_baseline(xs, W, S, R, K, args.cpu_slides)
@@ -289,6 +352,8 @@ def main() -> int:
         line["other_workloads"] = others
     if indexes:
         line["index_comparison"] = indexes
+    if learned:
+        line["learned"] = learned
     print(json.dumps(line), flush=True)
     return 0
 

@@ -187,6 +187,52 @@ RTOD_API rtod_status rtod_mlp_forward(int32_t device, const float *params, const
 RTOD_API rtod_status rtod_mlp_grad(int32_t device, const float *params, const float *X,
                                    const float *y, int32_t n, float *grad, float *loss);
 
+/* ---- Batched streams ----------------------------------------------------
+ * No reference counterpart: the reference runs one stream per process.  A
+ * batch advances n independent streams (each its own dim, W, S, R, K; plain
+ * schedule, cell-stencil index) by one slide per step: each phase of the
+ * slide is ONE launch covering every stream, and the step's launches are
+ * replayed from a hipGraph.  Per stream the semantics are exactly
+ * rtod_slide's. */
+typedef struct rtod_batch rtod_batch;
+
+typedef struct rtod_batch_tuning {
+    int32_t detect_points_per_wg; /* window points per detect workgroup     */
+    int32_t reserved;
+    float cell_scale[3];          /* cell side multiplier by dim (1-D in
+                                     [1/16,1], 2-D in [1/4,1], 3-D 1)       */
+    double us_per_step;           /* autotune: measured mean step time      */
+} rtod_batch_tuning;
+
+/* cfgs[n]: per-stream configurations (sched PLAIN, index AUTO or
+ * CELL_STENCIL; device and flags ignored, `device` used). */
+RTOD_API rtod_status rtod_batch_create(const rtod_config *cfgs, int32_t n, int32_t device,
+                                       rtod_batch **out);
+/* pts[i]: the first W_i points of stream i ([W_i][dim_i], host). */
+RTOD_API rtod_status rtod_batch_prime(rtod_batch *b, const float *const *pts);
+/* One slide of every stream: new_pts[i] [S_i][dim_i] (host).  Outliers of
+ * stream i (ring slots, ascending) are out_slots[out_offsets[i] ..
+ * out_offsets[i+1]); out_offsets has n+1 entries, out_slots capacity sum W_i. */
+RTOD_API rtod_status rtod_batch_step(rtod_batch *b, const float *const *new_pts,
+                                     int32_t *out_offsets, int32_t *out_slots);
+/* Device-resident streams: stream i's points [n_points[i]][dim_i] stay in HBM
+ * (device pointers); slide t reads rows W_i + (t-1) S_i ...  NULL unbinds. */
+RTOD_API rtod_status rtod_batch_bind_device(rtod_batch *b, const float *const *dev_streams,
+                                            const int64_t *n_points);
+RTOD_API rtod_status rtod_batch_prime_device(rtod_batch *b);
+RTOD_API rtod_status rtod_batch_step_device(rtod_batch *b, int32_t *out_offsets,
+                                            int32_t *out_slots);
+/* The autotuner: replays the first `slides` slides of the bound device
+ * streams under each candidate (detect workgroup size x cell scale of the
+ * 1-D/2-D streams), keeps the fastest, and leaves the batch primed from
+ * slide 0 with it.  Results do not depend on the tuning. */
+RTOD_API rtod_status rtod_batch_autotune(rtod_batch *b, int32_t slides, rtod_batch_tuning *out);
+RTOD_API rtod_status rtod_batch_get_tuning(const rtod_batch *b, rtod_batch_tuning *out);
+/* Sets a tuning; the batch must be primed again afterwards. */
+RTOD_API rtod_status rtod_batch_set_tuning(rtod_batch *b, const rtod_batch_tuning *t);
+RTOD_API rtod_status rtod_batch_stream_stats(const rtod_batch *b, int32_t i, rtod_stats *out);
+RTOD_API void rtod_batch_destroy(rtod_batch *b);
+
 /* Frees everything (reference cleanup, optixScan.cpp:883-932). */
 RTOD_API void rtod_destroy(rtod_ctx *ctx);
 

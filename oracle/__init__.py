@@ -6,8 +6,8 @@ baseline, never the product.  The product path (``rtod_amd``) does not import
 it and fails loudly when its HIP library is missing.
 
 The C restatement lives in ``oracle/rtod_oracle.c`` (built by
-``oracle/Makefile``); this module binds it with ctypes and adds the workload
-generators of the measured configurations.
+``oracle/Makefile``); this module binds it with ctypes.  The synthetic
+streams come from ``rtod_amd.synth`` and are re-exported here.
 
 Parity pins (tests/test_oracle.py):
   * brute force  -- closed-form lattices, isolated points, K >= W, dense
@@ -15,13 +15,13 @@ Parity pins (tests/test_oracle.py):
   * sorted1d     -- equals brute force on random streams (independent method);
   * grid         -- equals brute force and sorted1d slide by slide, is
     invariant under permutation of the window, monotone in R and K;
-  * generators   -- pinned by the SHA-256 of their output (tests/golden).
+  * generators   -- (rtod_amd.synth) pinned by the SHA-256 of their output
+    (tests/golden).
 """
 from __future__ import annotations
 
 import ctypes as C
 import os
-from dataclasses import dataclass
 
 import numpy as np
 
@@ -47,13 +47,6 @@ def lib():
         i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
         L.rtod_o_d2.restype = C.c_float
         L.rtod_o_d2.argtypes = [f32p, f32p, C.c_int]
-        L.rtod_o_gen_walk.restype = None
-        L.rtod_o_gen_walk.argtypes = [C.c_uint64, C.c_int64, C.c_int, C.c_double, C.c_double,
-                                      C.c_double, C.c_double, C.c_double, f32p]
-        L.rtod_o_gen_gmm.restype = None
-        L.rtod_o_gen_gmm.argtypes = [C.c_uint64, C.c_int64, C.c_int, C.c_int, C.c_double,
-                                     C.c_double, C.c_double, C.c_double, C.c_double,
-                                     C.c_double, f32p]
         L.rtod_o_brute.restype = C.c_int64
         L.rtod_o_brute.argtypes = [f32p, C.c_int64, C.c_int, C.c_float, C.c_int, C.c_int,
                                    C.c_int, i32p, C.c_void_p]
@@ -90,22 +83,10 @@ def lib():
 
 
 # --------------------------------------------------------------------------
-# generators
+# generators: the measured streams come from the product's synthetic-data
+# library (inputs, not checked code); re-exported for the tests
 # --------------------------------------------------------------------------
-def gen_walk(seed: int, n: int, dim: int = 1, start: float = 10.0, sigma: float = 0.05,
-             p_spike: float = 0.002, spike_lo: float = 2.0, spike_hi: float = 5.0) -> np.ndarray:
-    out = np.empty(n * dim, np.float32)
-    lib().rtod_o_gen_walk(seed, n, dim, start, sigma, p_spike, spike_lo, spike_hi, out)
-    return out.reshape(n, dim)
-
-
-def gen_gmm(seed: int, n: int, dim: int = 3, ncomp: int = 8, lo: float = 0.0, hi: float = 100.0,
-            sig_lo: float = 0.5, sig_hi: float = 3.0, noise_frac: float = 0.005,
-            drift: float = 5.0) -> np.ndarray:
-    out = np.empty(n * dim, np.float32)
-    lib().rtod_o_gen_gmm(seed, n, dim, ncomp, lo, hi, sig_lo, sig_hi, noise_frac, drift, out)
-    return out.reshape(n, dim)
-
+from rtod_amd.synth import C1, C2, C3, Workload, gen_gmm, gen_walk  # noqa: E402,F401
 
 # --------------------------------------------------------------------------
 # detection
@@ -211,32 +192,6 @@ def run_stream(xs: np.ndarray, W: int, S: int, T: int, R: float, K: int, method:
 # --------------------------------------------------------------------------
 # measured configurations (synthetic, seeded)
 # --------------------------------------------------------------------------
-@dataclass(frozen=True)
-class Workload:
-    name: str
-    dim: int
-    N: int
-    W: int
-    S: int
-    R: float
-    K: int
-    kind: str  # "walk" | "gmm"
-    seed: int
-
-    @property
-    def T(self) -> int:
-        """Number of slides: min(10000, floor((N-W)/S)) (optixScan.cpp:1036,1045)."""
-        return min(10000, (self.N - self.W) // self.S)
-
-    def data(self) -> np.ndarray:
-        if self.kind == "walk":
-            return gen_walk(self.seed, self.N, self.dim)
-        return gen_gmm(self.seed, self.N, self.dim)
-
-
-C1 = Workload("walk1d-W10k-S500", 1, 200_000, 10_000, 500, 0.45, 50, "walk", 1)
-C2 = Workload("walk1d-W100k-S5k", 1, 1_048_572, 100_000, 5_000, 0.45, 50, "walk", 2)
-C3 = Workload("gmm3d-W10k-S500", 3, 575_000, 10_000, 500, 1.9, 50, "gmm", 3)
 
 
 # --------------------------------------------------------------------------

@@ -36,19 +36,6 @@ extern "C" {
 /* FP32 squared distance with explicit FMA (the contract predicate). */
 float rtod_o_d2(const float *p, const float *q, int dim);
 
-/* Deterministic generators (xoshiro256** seeded by splitmix64, Box-Muller in
- * double, cast to f32).  Random walk per axis, spikes of magnitude U(lo,hi)
- * along a random direction with probability p_spike. */
-void rtod_o_gen_walk(uint64_t seed, int64_t n, int dim, double start,
-                     double sigma, double p_spike, double spike_lo,
-                     double spike_hi, float *out);
-/* Mixture of ncomp anisotropic Gaussians, centres U[lo,hi]^dim, per-axis
- * sigma U(sig_lo,sig_hi), linear centre drift of total length `drift` over
- * the stream, fraction noise_frac of uniform background in [lo,hi]^dim. */
-void rtod_o_gen_gmm(uint64_t seed, int64_t n, int dim, int ncomp, double lo,
-                    double hi, double sig_lo, double sig_hi, double noise_frac,
-                    double drift, float *out);
-
 /* Brute force over one window `win` ([W][dim], slot order).
  * semantics 0: contract predicate (FP32 d2 <= r2).
  * semantics 1: the reference's FP64 strict rule (1-D |dx| < R, else

@@ -65,6 +65,14 @@ class _Stats(C.Structure):
 MLP_NPARAMS = 1313
 
 
+class _BatchTuning(C.Structure):
+    _fields_ = [("detect_points_per_wg", C.c_int32), ("reserved", C.c_int32),
+                ("cell_scale", C.c_float * 3), ("us_per_step", C.c_double)]
+
+
+_PP = C.POINTER(C.c_void_p)
+
+
 # (name, restype, argtypes) of every symbol include/rtod.h declares
 _F32P = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _I32P = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -90,6 +98,18 @@ SIGNATURES = [
     ("rtod_mlp_forward", C.c_int, [C.c_int32, _F32P, _F32P, C.c_int32, _F32P]),
     ("rtod_mlp_grad", C.c_int, [C.c_int32, _F32P, _F32P, _F32P, C.c_int32, _F32P,
                                 C.POINTER(C.c_float)]),
+    ("rtod_batch_create", C.c_int, [C.POINTER(_Config), C.c_int32, C.c_int32,
+                                    C.POINTER(C.c_void_p)]),
+    ("rtod_batch_prime", C.c_int, [C.c_void_p, _PP]),
+    ("rtod_batch_step", C.c_int, [C.c_void_p, _PP, _I32P, _I32P]),
+    ("rtod_batch_bind_device", C.c_int, [C.c_void_p, _PP, C.POINTER(C.c_int64)]),
+    ("rtod_batch_prime_device", C.c_int, [C.c_void_p]),
+    ("rtod_batch_step_device", C.c_int, [C.c_void_p, _I32P, _I32P]),
+    ("rtod_batch_autotune", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(_BatchTuning)]),
+    ("rtod_batch_get_tuning", C.c_int, [C.c_void_p, C.POINTER(_BatchTuning)]),
+    ("rtod_batch_set_tuning", C.c_int, [C.c_void_p, C.POINTER(_BatchTuning)]),
+    ("rtod_batch_stream_stats", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(_Stats)]),
+    ("rtod_batch_destroy", None, [C.c_void_p]),
     ("rtod_destroy", None, [C.c_void_p]),
     ("rtod_status_str", C.c_char_p, [C.c_int]),
     ("rtod_last_error", C.c_char_p, []),
@@ -282,6 +302,103 @@ class DeviceStream:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def _tuning_dict(t: _BatchTuning) -> dict:
+    return {"detect_points_per_wg": t.detect_points_per_wg,
+            "cell_scale": [float(x) for x in t.cell_scale], "us_per_step": t.us_per_step}
+
+
+class Batch:
+    """Many independent streams advanced together: each phase of a slide is
+    one launch over all streams, the step one hipGraph replay (rtod_batch_*).
+    streams: list of (dim, W, S, R, K)."""
+
+    def __init__(self, streams, device: int = 0):
+        n = len(streams)
+        cfgs = (_Config * n)()
+        for i, (dim, W, S, R, K) in enumerate(streams):
+            cfgs[i] = _Config(dim, W, S, R, K, INDEX_AUTO, SCHED_PLAIN, 0.25, 0, device)
+        self.streams = [tuple(x) for x in streams]
+        self.n = n
+        h = C.c_void_p()
+        _check(lib().rtod_batch_create(cfgs, n, device, C.byref(h)))
+        self._h = h
+        self._offs = np.zeros(n + 1, np.int32)
+        self._slots = np.zeros(sum(s[1] for s in streams), np.int32)
+        self._keep = []
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            lib().rtod_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _ptrs(arrays):
+        arrs = [np.ascontiguousarray(a, np.float32) for a in arrays]
+        return arrs, (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+    def _split(self):
+        o = self._offs
+        return [self._slots[o[i]:o[i + 1]].copy() for i in range(self.n)]
+
+    def prime(self, windows) -> None:
+        arrs, pp = self._ptrs(windows)
+        for a, (dim, W, S, R, K) in zip(arrs, self.streams):
+            assert a.size == W * dim
+        _check(lib().rtod_batch_prime(self._h, C.cast(pp, _PP)))
+
+    def step(self, new_pts):
+        arrs, pp = self._ptrs(new_pts)
+        for a, (dim, W, S, R, K) in zip(arrs, self.streams):
+            assert a.size == S * dim
+        _check(lib().rtod_batch_step(self._h, C.cast(pp, _PP), self._offs, self._slots))
+        return self._split()
+
+    def bind_device(self, dev_ptrs, n_points) -> None:
+        """dev_ptrs: device addresses of each stream's [n][dim] float32 points."""
+        pp = (C.c_void_p * self.n)(*dev_ptrs)
+        npts = (C.c_int64 * self.n)(*n_points)
+        self._keep = [pp, npts]
+        _check(lib().rtod_batch_bind_device(self._h, C.cast(pp, _PP), npts))
+
+    def prime_device(self) -> None:
+        _check(lib().rtod_batch_prime_device(self._h))
+
+    def step_device(self):
+        _check(lib().rtod_batch_step_device(self._h, self._offs, self._slots))
+        return self._split()
+
+    def autotune(self, slides: int) -> dict:
+        t = _BatchTuning()
+        _check(lib().rtod_batch_autotune(self._h, slides, C.byref(t)))
+        return _tuning_dict(t)
+
+    def get_tuning(self) -> dict:
+        t = _BatchTuning()
+        _check(lib().rtod_batch_get_tuning(self._h, C.byref(t)))
+        return _tuning_dict(t)
+
+    def set_tuning(self, detect_points_per_wg: int, cell_scale=(1.0, 1.0, 1.0)) -> None:
+        t = _BatchTuning(detect_points_per_wg, 0, (C.c_float * 3)(*cell_scale), 0.0)
+        _check(lib().rtod_batch_set_tuning(self._h, C.byref(t)))
+
+    def stream_stats(self, i: int) -> dict:
+        st = _Stats()
+        _check(lib().rtod_batch_stream_stats(self._h, i, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in _Stats._fields_}
 
 
 def mlp_forward(params: np.ndarray, X: np.ndarray, device: int = 0) -> np.ndarray:

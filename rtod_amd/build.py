@@ -62,9 +62,23 @@ def build_ods() -> str:
     return ODS
 
 
+SYNTH = os.path.join(HERE, "librtod_synth.so")
+
+
+def build_synth() -> str:
+    """Synthetic-stream generators (plain C, gcc)."""
+    src = os.path.join(HERE, "host", "synth.c")
+    tmp = SYNTH + ".tmp"
+    _run(["gcc", "-O2", "-std=c11", "-ffp-contract=off", "-fPIC", "-shared",
+          "-fvisibility=hidden", "-o", tmp, src, "-lm"])
+    os.replace(tmp, SYNTH)
+    return SYNTH
+
+
 def build_all() -> None:
     build_lib()
     build_ods()
+    build_synth()
 
 
 if __name__ == "__main__":

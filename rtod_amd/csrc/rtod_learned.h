@@ -498,7 +498,7 @@ __device__ __forceinline__ void train_block(const Dev &d, int n_merge, void *lds
 __global__ __launch_bounds__(256) void k_mlp_forward(const float *P, const float *X, int n,
                                                      float *out)
 {
-    __shared__ __bf16 s_h[4][16 * kMlpH];
+    __shared__ __attribute__((aligned(16))) __bf16 s_h[4][16 * kMlpH];
     __shared__ float s_z[4][kWave];
     const int wid = threadIdx.x / kWave;
     const int i = blockIdx.x * 256 + threadIdx.x;

@@ -9,6 +9,7 @@
 // 16 us per launch against 6 us for the same four direct launches.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -18,6 +19,7 @@
 #include <vector>
 
 #include "../../include/rtod.h"
+#include "rtod_batch.h"
 #include "rtod_lbvh.h"
 
 using namespace rtod;
@@ -63,6 +65,10 @@ struct SortShape {
     int nt, it;
 };
 constexpr SortShape kSortShapes[4] = {{256, 2}, {512, 4}, {1024, 4}, {1024, 8}};
+constexpr int kBatchSortShape = 2; // batched streams: the 1024 x 4 tile
+static_assert(kSortShapes[kBatchSortShape].nt == kBatchSortNT &&
+                  kSortShapes[kBatchSortShape].it == kBatchSortIT,
+              "batched ingest tile");
 
 int pick_sort_shape(int S, int dim)
 {
@@ -82,6 +88,7 @@ struct rtod_ctx {
     int detect_blocks = 512;
     int device = 0;
     hipStream_t stream = nullptr;
+    bool own_stream = true; // false: a batch member on the batch's stream
     Dev d{};
     float *d_prime = nullptr; // device copy of the initial window
     float *h_in = nullptr;
@@ -419,7 +426,14 @@ std::vector<float> initial_predictor()
     return p;
 }
 
-rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out)
+// Options of contexts created as members of a batch.
+struct CreateOpts {
+    int sort_shape = -1;          // -1: picked from S and dim
+    double cell_scale = 1.0;      // cell side multiplier (<= 1)
+    hipStream_t stream = nullptr; // borrowed stream
+};
+
+rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts &opt = {})
 {
     RTOD_TRY(validate(cfg));
     rtod_ctx *c = new (std::nothrow) rtod_ctx();
@@ -434,11 +448,16 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out)
     c->K = cfg->K;
     c->device = cfg->device;
     RTOD_HIP(hipSetDevice(c->device));
-    RTOD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    if (opt.stream) {
+        c->stream = opt.stream;
+        c->own_stream = false;
+    } else {
+        RTOD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    }
     for (auto &e : c->ev)
         RTOD_HIP(hipEventCreate(&e));
 
-    c->sort_shape = pick_sort_shape(c->S, c->dim);
+    c->sort_shape = opt.sort_shape >= 0 ? opt.sort_shape : pick_sort_shape(c->S, c->dim);
     c->CH = kSortShapes[c->sort_shape].nt * kSortShapes[c->sort_shape].it;
 
     Dev &d = c->d;
@@ -460,7 +479,8 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out)
     d.nbs = (int)(((int64_t)c->S + (1ll << bl) - 1) >> bl);
     const float R = cfg->R;
     d.r2 = R * R;
-    const double cside = (double)R * (1.0 - 1e-5) / std::sqrt((double)c->dim);
+    const double cside =
+        (double)R * (1.0 - 1e-5) / std::sqrt((double)c->dim) * opt.cell_scale;
     d.inv_c = 1.0 / cside;
     d.Rp = (double)R * (1.0 + 1e-5);
 
@@ -542,7 +562,8 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out)
     return RTOD_OK;
 }
 
-rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n)
+rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n,
+                       hipMemcpyKind kind = hipMemcpyHostToDevice)
 {
     if (!pts || n != c->W) {
         set_error("prime needs exactly W points");
@@ -551,8 +572,7 @@ rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n)
     RTOD_HIP(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     Step st{};
-    RTOD_HIP(hipMemcpyAsync(c->d_prime, pts, sizeof(float) * (size_t)c->W * c->dim,
-                            hipMemcpyHostToDevice, s));
+    RTOD_HIP(hipMemcpyAsync(c->d_prime, pts, sizeof(float) * (size_t)c->W * c->dim, kind, s));
     // sort each initial batch (generation b -> batch-ring entry b)
     launch_ingest_dim(c, s, st, kModePrime, c->d_prime, c->m * c->d.cpb, c->d.cpb == 1 ? 1 : 0);
     if (c->d.cpb > 1)
@@ -1124,7 +1144,7 @@ RTOD_API void rtod_destroy(rtod_ctx *c)
         (void)hipHostFree(c->h_in);
     if (c->h_out)
         (void)hipHostFree(c->h_out);
-    if (c->stream)
+    if (c->stream && c->own_stream)
         (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1153,5 +1173,567 @@ RTOD_API const char *rtod_status_str(rtod_status s)
 RTOD_API const char *rtod_last_error(void) { return g_last_error.c_str(); }
 
 RTOD_API const char *rtod_version(void) { return "rtod-mi355x 0.1 (gfx950)"; }
+
+} // extern "C"
+
+// ===========================================================================
+// Batched streams: many contexts advanced together by one persistent kernel
+// (rtod_batch.h) replayed from a hipGraph.
+// ===========================================================================
+struct rtod_batch {
+    int n = 0, device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<rtod_ctx *> ctx;
+    std::vector<rtod_config> cfg;
+    rtod_batch_tuning tuning{};
+    BatchStream *d_streams = nullptr;
+    BatchItem *d_items = nullptr;
+    struct Launch {
+        int kind, dim, first, count;
+    };
+    std::vector<Launch> launches;
+    int n_items = 0;
+    std::vector<const float *> src; // device-resident streams, empty: host input
+    std::vector<int64_t> npts;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool plan_ok = false, primed = false;
+    int64_t t = 0;
+};
+
+namespace {
+
+void batch_free_plan(rtod_batch *b)
+{
+    if (b->exec)
+        (void)hipGraphExecDestroy(b->exec);
+    if (b->graph)
+        (void)hipGraphDestroy(b->graph);
+    if (b->d_items)
+        (void)hipFree(b->d_items);
+    if (b->d_streams)
+        (void)hipFree(b->d_streams);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->d_items = nullptr;
+    b->d_streams = nullptr;
+    b->plan_ok = false;
+}
+
+void batch_free_members(rtod_batch *b)
+{
+    for (auto *c : b->ctx)
+        rtod_destroy(c);
+    b->ctx.clear();
+}
+
+rtod_status batch_create_members(rtod_batch *b)
+{
+    batch_free_members(b);
+    for (int i = 0; i < b->n; ++i) {
+        CreateOpts opt;
+        opt.sort_shape = kBatchSortShape;
+        opt.cell_scale = b->tuning.cell_scale[b->cfg[i].dim - 1];
+        opt.stream = b->stream;
+        rtod_ctx *c = nullptr;
+        rtod_status st = create_impl(&b->cfg[i], &c, opt);
+        if (st != RTOD_OK) {
+            if (c)
+                rtod_destroy(c);
+            return st;
+        }
+        b->ctx.push_back(c);
+    }
+    b->primed = false;
+    return RTOD_OK;
+}
+
+template <int DIM> void *batch_kernel(int kind)
+{
+    switch (kind) {
+    case 0:
+        return reinterpret_cast<void *>(&k_batch_ingest<DIM>);
+    case 1:
+        return reinterpret_cast<void *>(&k_batch_merge_pass<DIM>);
+    case 2:
+        return reinterpret_cast<void *>(&k_batch_merge_window<DIM>);
+    case 3:
+        return reinterpret_cast<void *>(&k_batch_classify<DIM>);
+    default:
+        return reinterpret_cast<void *>(&k_batch_detect<DIM>);
+    }
+}
+
+// A step: per dimensionality present, a chain of launches -- ingest, merge
+// pass 0..P-1 (only streams whose slide spans several sort tiles), window
+// merge, classify, detect -- each over all of the group's streams, larger
+// windows first.  Items of one launch are contiguous in one device table;
+// the chains are independent branches of one hipGraph.
+rtod_status batch_build_plan(rtod_batch *b)
+{
+    batch_free_plan(b);
+    std::vector<BatchStream> hs(b->n);
+    std::vector<int> passes(b->n, 0);
+    for (int i = 0; i < b->n; ++i) {
+        rtod_ctx *c = b->ctx[i];
+        for (int L = kBatchCH; L < c->S; L *= 2)
+            ++passes[i];
+        if (passes[i] > kBatchMaxPasses) {
+            set_error("batch stream needs too many merge passes (S too large)");
+            return RTOD_ERR_UNSUPPORTED;
+        }
+        std::memset(&hs[i], 0, sizeof(BatchStream));
+        hs[i].d = c->d;
+        hs[i].src_base = b->src.empty() ? nullptr : b->src[i];
+    }
+    const int q = b->tuning.detect_points_per_wg;
+    std::vector<BatchItem> items;
+    b->launches.clear();
+    for (int dim = 1; dim <= 3; ++dim) {
+        std::vector<int> order;
+        int pmax = 0;
+        for (int i = 0; i < b->n; ++i)
+            if (b->ctx[i]->dim == dim) {
+                order.push_back(i);
+                pmax = std::max(pmax, passes[i]);
+            }
+        if (order.empty())
+            continue;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return b->ctx[x]->W > b->ctx[y]->W; });
+        auto add_launch = [&](int kind, auto blocks_of, auto arg_of, int pass) {
+            const int first = (int)items.size();
+            for (int i : order) {
+                const int nb = blocks_of(i);
+                for (int k = 0; k < nb; ++k)
+                    items.push_back(BatchItem{i, k, arg_of(i), pass});
+            }
+            const int cnt = (int)items.size() - first;
+            if (cnt > 0)
+                b->launches.push_back({kind, dim, first, cnt});
+        };
+        add_launch(0, [&](int i) { return b->ctx[i]->d.cpb; }, [](int) { return 0; }, 0);
+        for (int k = 0; k < pmax; ++k)
+            add_launch(
+                1, [&](int i) { return passes[i] > k ? (b->ctx[i]->S + 255) / 256 : 0; },
+                [&](int) { return kBatchCH << k; }, k);
+        add_launch(2, [&](int i) { return (b->ctx[i]->W + kMergeT - 1) / kMergeT; },
+                   [](int) { return 0; }, 0);
+        add_launch(3, [&](int i) { return (b->ctx[i]->W + kClassifyNT - 1) / kClassifyNT; },
+                   [](int) { return 0; }, 0);
+        auto ndb = [&](int i) {
+            return std::max(1, std::min(4096, (b->ctx[i]->W + q - 1) / q));
+        };
+        add_launch(4, ndb, ndb, 0);
+    }
+    b->n_items = (int)items.size();
+    RTOD_HIP(hipMalloc(&b->d_streams, sizeof(BatchStream) * hs.size()));
+    RTOD_HIP(hipMalloc(&b->d_items, sizeof(BatchItem) * items.size()));
+    RTOD_HIP(hipMemcpy(b->d_streams, hs.data(), sizeof(BatchStream) * hs.size(),
+                       hipMemcpyHostToDevice));
+    RTOD_HIP(hipMemcpy(b->d_items, items.data(), sizeof(BatchItem) * items.size(),
+                       hipMemcpyHostToDevice));
+    RTOD_HIP(hipGraphCreate(&b->graph, 0));
+    hipGraphNode_t prev[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (const auto &l : b->launches) {
+        const BatchItem *it = b->d_items + l.first;
+        const BatchStream *st = b->d_streams;
+        void *args[2] = {(void *)&it, (void *)&st};
+        hipKernelNodeParams kp{};
+        kp.func = l.dim == 1 ? batch_kernel<1>(l.kind)
+                  : l.dim == 2 ? batch_kernel<2>(l.kind)
+                               : batch_kernel<3>(l.kind);
+        kp.gridDim = dim3(l.count);
+        kp.blockDim = dim3(l.kind == 0 ? kBatchSortNT : kBatchNT);
+        kp.sharedMemBytes = 0;
+        kp.kernelParams = args;
+        kp.extra = nullptr;
+        hipGraphNode_t node;
+        RTOD_HIP(hipGraphAddKernelNode(&node, b->graph, prev[l.dim] ? &prev[l.dim] : nullptr,
+                                       prev[l.dim] ? 1 : 0, &kp));
+        prev[l.dim] = node;
+    }
+    RTOD_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->plan_ok = true;
+    return RTOD_OK;
+}
+
+rtod_status batch_check_tuning(const rtod_batch_tuning *t)
+{
+    if (t->detect_points_per_wg < 1 || t->detect_points_per_wg > (1 << 20)) {
+        set_error("tuning: detect_points_per_wg in [1, 2^20]");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    // smaller cells keep the dense rule exact; the stencil must stay within
+    // 32 rows: 2-D down to 1/4, 3-D fixed at 1
+    if (!(t->cell_scale[0] >= 0.0625f && t->cell_scale[0] <= 1.f) ||
+        !(t->cell_scale[1] >= 0.25f && t->cell_scale[1] <= 1.f) || t->cell_scale[2] != 1.f) {
+        set_error("tuning: cell_scale 1-D in [1/16, 1], 2-D in [1/4, 1], 3-D 1");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    return RTOD_OK;
+}
+
+rtod_status batch_apply_tuning(rtod_batch *b, const rtod_batch_tuning &t)
+{
+    RTOD_TRY(batch_check_tuning(&t));
+    bool cells = false;
+    for (int a = 0; a < 3; ++a)
+        cells = cells || t.cell_scale[a] != b->tuning.cell_scale[a];
+    b->tuning = t;
+    if (cells)
+        RTOD_TRY(batch_create_members(b)); // keys change: state must be primed again
+    return batch_build_plan(b);
+}
+
+rtod_status batch_prime_device(rtod_batch *b)
+{
+    for (int i = 0; i < b->n; ++i)
+        RTOD_TRY(prime_impl(b->ctx[i], b->src[i], b->cfg[i].window, hipMemcpyDeviceToDevice));
+    b->t = 0;
+    b->primed = true;
+    return RTOD_OK;
+}
+
+rtod_status batch_launch_collect(rtod_batch *b, int32_t *offsets, int32_t *slots)
+{
+    if (!b->plan_ok)
+        RTOD_TRY(batch_build_plan(b));
+    RTOD_HIP(hipGraphLaunch(b->exec, b->stream));
+    RTOD_HIP(hipStreamSynchronize(b->stream));
+    int64_t off = 0;
+    if (offsets)
+        offsets[0] = 0;
+    rtod_status first_err = RTOD_OK;
+    for (int i = 0; i < b->n; ++i) {
+        int32_t k = 0;
+        rtod_status st = collect(b->ctx[i], slots ? slots + off : nullptr, &k);
+        if (st != RTOD_OK && first_err == RTOD_OK)
+            first_err = st;
+        off += k;
+        if (offsets)
+            offsets[i + 1] = (int32_t)off;
+    }
+    b->t++;
+    return first_err;
+}
+
+rtod_status batch_step_device(rtod_batch *b, int32_t *offsets, int32_t *slots)
+{
+    for (int i = 0; i < b->n; ++i)
+        if ((int64_t)b->cfg[i].window + (b->t + 1) * (int64_t)b->cfg[i].slide > b->npts[i]) {
+            set_error("a bound device stream has no points left for this step");
+            return RTOD_ERR_INVALID_ARGUMENT;
+        }
+    return batch_launch_collect(b, offsets, slots);
+}
+
+} // namespace
+
+extern "C" {
+
+RTOD_API rtod_status rtod_batch_create(const rtod_config *cfgs, int32_t n, int32_t device,
+                                       rtod_batch **out)
+{
+    if (!out) {
+        set_error("out is NULL");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    if (!cfgs || n < 1 || n > 65536) {
+        set_error("need 1..65536 stream configurations");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    for (int i = 0; i < n; ++i) {
+        RTOD_TRY(validate(&cfgs[i]));
+        if (cfgs[i].sched_kind != RTOD_SCHED_PLAIN || cfgs[i].index_kind == RTOD_INDEX_LBVH) {
+            set_error("batched streams run the plain schedule on the cell-stencil index");
+            return RTOD_ERR_UNSUPPORTED;
+        }
+    }
+    rtod_batch *b = new (std::nothrow) rtod_batch();
+    if (!b)
+        return RTOD_ERR_NOMEM;
+    rtod_status st = RTOD_OK;
+    try {
+        b->n = n;
+        b->device = device;
+        b->cfg.assign(cfgs, cfgs + n);
+        for (auto &c : b->cfg) {
+            c.device = device;
+            c.flags = 0;
+        }
+        b->tuning.detect_points_per_wg = 64;
+        b->tuning.cell_scale[0] = b->tuning.cell_scale[1] = b->tuning.cell_scale[2] = 1.f;
+        b->tuning.us_per_step = 0.0;
+        st = [&]() -> rtod_status {
+            RTOD_HIP(hipSetDevice(device));
+            RTOD_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+            return batch_create_members(b);
+        }();
+    } catch (...) {
+        st = RTOD_ERR_INTERNAL;
+    }
+    if (st != RTOD_OK) {
+        rtod_batch_destroy(b);
+        return st;
+    }
+    *out = b;
+    return RTOD_OK;
+}
+
+RTOD_API rtod_status rtod_batch_prime(rtod_batch *b, const float *const *pts)
+{
+    if (!b || !pts) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    for (int i = 0; i < b->n; ++i)
+        if (!pts[i]) {
+            set_error("NULL stream pointer");
+            return RTOD_ERR_INVALID_ARGUMENT;
+        }
+    try {
+        RTOD_HIP(hipSetDevice(b->device));
+        for (int i = 0; i < b->n; ++i)
+            RTOD_TRY(prime_impl(b->ctx[i], pts[i], b->cfg[i].window));
+        b->t = 0;
+        b->primed = true;
+        return RTOD_OK;
+    } catch (...) {
+        return RTOD_ERR_INTERNAL;
+    }
+}
+
+RTOD_API rtod_status rtod_batch_step(rtod_batch *b, const float *const *new_pts,
+                                     int32_t *out_offsets, int32_t *out_slots)
+{
+    if (!b || !new_pts || !out_offsets || !out_slots) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (!b->primed) {
+        set_error("rtod_batch_step before rtod_batch_prime");
+        return RTOD_ERR_STATE;
+    }
+    if (!b->src.empty()) {
+        set_error("the batch reads bound device streams: use rtod_batch_step_device");
+        return RTOD_ERR_STATE;
+    }
+    for (int i = 0; i < b->n; ++i)
+        if (!new_pts[i]) {
+            set_error("NULL stream pointer");
+            return RTOD_ERR_INVALID_ARGUMENT;
+        }
+    try {
+        RTOD_HIP(hipSetDevice(b->device));
+        for (int i = 0; i < b->n; ++i) {
+            rtod_ctx *c = b->ctx[i];
+            const size_t nf = (size_t)c->S * c->dim;
+            std::memcpy(c->h_in + (size_t)(c->host_t & 1) * nf, new_pts[i], nf * sizeof(float));
+        }
+        return batch_launch_collect(b, out_offsets, out_slots);
+    } catch (...) {
+        return RTOD_ERR_INTERNAL;
+    }
+}
+
+RTOD_API rtod_status rtod_batch_bind_device(rtod_batch *b, const float *const *dev_streams,
+                                            const int64_t *n_points)
+{
+    if (!b) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    try {
+        RTOD_HIP(hipSetDevice(b->device));
+        if (!dev_streams) { // back to host input
+            b->src.clear();
+            b->npts.clear();
+            b->plan_ok = false;
+            return RTOD_OK;
+        }
+        if (!n_points) {
+            set_error("NULL argument");
+            return RTOD_ERR_INVALID_ARGUMENT;
+        }
+        for (int i = 0; i < b->n; ++i) {
+            hipPointerAttribute_t attr{};
+            if (!dev_streams[i] || hipPointerGetAttributes(&attr, dev_streams[i]) != hipSuccess ||
+                attr.type != hipMemoryTypeDevice) {
+                (void)hipGetLastError();
+                set_error("rtod_batch_bind_device needs device (HBM) pointers");
+                return RTOD_ERR_INVALID_ARGUMENT;
+            }
+            if (n_points[i] < b->cfg[i].window) {
+                set_error("a bound stream holds fewer than W points");
+                return RTOD_ERR_INVALID_ARGUMENT;
+            }
+        }
+        b->src.assign(dev_streams, dev_streams + b->n);
+        b->npts.assign(n_points, n_points + b->n);
+        b->plan_ok = false;
+        return RTOD_OK;
+    } catch (...) {
+        return RTOD_ERR_INTERNAL;
+    }
+}
+
+RTOD_API rtod_status rtod_batch_prime_device(rtod_batch *b)
+{
+    if (!b) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (b->src.empty()) {
+        set_error("no device streams bound");
+        return RTOD_ERR_STATE;
+    }
+    try {
+        RTOD_HIP(hipSetDevice(b->device));
+        return batch_prime_device(b);
+    } catch (...) {
+        return RTOD_ERR_INTERNAL;
+    }
+}
+
+RTOD_API rtod_status rtod_batch_step_device(rtod_batch *b, int32_t *out_offsets,
+                                            int32_t *out_slots)
+{
+    if (!b || !out_offsets || !out_slots) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (b->src.empty()) {
+        set_error("no device streams bound");
+        return RTOD_ERR_STATE;
+    }
+    if (!b->primed) {
+        set_error("rtod_batch_step_device before priming");
+        return RTOD_ERR_STATE;
+    }
+    try {
+        RTOD_HIP(hipSetDevice(b->device));
+        return batch_step_device(b, out_offsets, out_slots);
+    } catch (...) {
+        return RTOD_ERR_INTERNAL;
+    }
+}
+
+RTOD_API rtod_status rtod_batch_get_tuning(const rtod_batch *b, rtod_batch_tuning *out)
+{
+    if (!b || !out) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    *out = b->tuning;
+    return RTOD_OK;
+}
+
+RTOD_API rtod_status rtod_batch_set_tuning(rtod_batch *b, const rtod_batch_tuning *t)
+{
+    if (!b || !t) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    try {
+        RTOD_HIP(hipSetDevice(b->device));
+        RTOD_TRY(batch_check_tuning(t));
+        RTOD_TRY(batch_apply_tuning(b, *t));
+        b->primed = false; // every stream restarts from rtod_batch_prime*
+        return RTOD_OK;
+    } catch (...) {
+        return RTOD_ERR_INTERNAL;
+    }
+}
+
+RTOD_API rtod_status rtod_batch_autotune(rtod_batch *b, int32_t slides, rtod_batch_tuning *out)
+{
+    if (!b) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (b->src.empty()) {
+        set_error("autotuning replays the bound device streams: bind them first");
+        return RTOD_ERR_STATE;
+    }
+    if (slides < 2 || slides > 1000) {
+        set_error("slides must be in [2, 1000]");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    for (int i = 0; i < b->n; ++i)
+        if ((int64_t)b->cfg[i].window + (int64_t)slides * b->cfg[i].slide > b->npts[i]) {
+            set_error("a bound stream is shorter than W + slides*S");
+            return RTOD_ERR_INVALID_ARGUMENT;
+        }
+    try {
+        RTOD_HIP(hipSetDevice(b->device));
+        const int qs[4] = {16, 64, 256, 1024};
+        const float c1[3] = {1.f, 0.5f, 0.25f};
+        rtod_batch_tuning best{};
+        double best_us = 1e300;
+        std::vector<int32_t> offs(b->n + 1);
+        int64_t cap = 0;
+        for (auto &c : b->cfg)
+            cap += c.window;
+        std::vector<int32_t> slots((size_t)cap);
+        for (float cs : c1)
+            for (int q : qs) {
+                    rtod_batch_tuning t{};
+                    t.detect_points_per_wg = q;
+                    t.cell_scale[0] = cs;
+                    t.cell_scale[1] = cs;
+                    t.cell_scale[2] = 1.f;
+                    RTOD_TRY(batch_apply_tuning(b, t));
+                    RTOD_TRY(batch_prime_device(b));
+                    double sum = 0.0;
+                    int cnt = 0;
+                    for (int k = 0; k < slides; ++k) {
+                        const double t0 = now_ms();
+                        RTOD_TRY(batch_step_device(b, offs.data(), slots.data()));
+                        if (k > 0) { // the first step warms caches and code
+                            sum += now_ms() - t0;
+                            ++cnt;
+                        }
+                    }
+                    const double us = sum / cnt * 1e3;
+                    if (us < best_us) {
+                        best_us = us;
+                        best = t;
+                    }
+            }
+        best.us_per_step = best_us;
+        RTOD_TRY(batch_apply_tuning(b, best));
+        RTOD_TRY(batch_prime_device(b));
+        if (out)
+            *out = best;
+        return RTOD_OK;
+    } catch (...) {
+        return RTOD_ERR_INTERNAL;
+    }
+}
+
+RTOD_API rtod_status rtod_batch_stream_stats(const rtod_batch *b, int32_t i, rtod_stats *out)
+{
+    if (!b || !out || i < 0 || i >= b->n) {
+        set_error("NULL argument or stream index out of range");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    return rtod_get_stats(b->ctx[i], out);
+}
+
+RTOD_API void rtod_batch_destroy(rtod_batch *b)
+{
+    if (!b)
+        return;
+    (void)hipSetDevice(b->device);
+    if (b->stream)
+        (void)hipStreamSynchronize(b->stream);
+    batch_free_plan(b);
+    batch_free_members(b);
+    if (b->stream)
+        (void)hipStreamDestroy(b->stream);
+    delete b;
+}
 
 } // extern "C"

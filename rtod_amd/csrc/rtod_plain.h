@@ -141,22 +141,31 @@ __device__ __forceinline__ void block_radix_sort(uint64_t (&k)[IT], int sh0, int
 // Output: keys, slots, coordinates in sorted order (batch entry, or a
 // scratch run when the batch spans several tiles) + the batch sample table.
 // ---------------------------------------------------------------------------
+template <int DIM, int NT, int IT> struct IngestLds {
+    alignas(16) float coord[NT * IT * DIM];
+    uint64_t key[NT * IT];
+    uint16_t val[NT * IT];
+    int hist[16 * (NT / kWave)];
+    uint64_t red[2 * 3][NT / kWave];
+};
+
 template <int DIM, int NT, int IT>
-__global__ __launch_bounds__(NT) void k_ingest_sort(Dev d, Step st, int mode, const float *src_dev,
-                                                    int dst_batch)
+__device__ __forceinline__ void ingest_sort_body(const Dev &d, const Step &st, int mode,
+                                                 const float *src_dev, int dst_batch, int bid,
+                                                 IngestLds<DIM, NT, IT> &lds)
 {
     constexpr int CH = NT * IT, NW = NT / kWave;
     constexpr int IB = 13; // bits of the tile-local index (CH <= 8192)
     static_assert(CH <= (1 << IB), "tile index bits");
-    __shared__ __attribute__((aligned(16))) float s_coord[CH * DIM];
-    __shared__ uint64_t s_key[CH];
-    __shared__ uint16_t s_val[CH];
-    __shared__ int s_hist[16 * NW];
-    __shared__ uint64_t s_red[2 * 3][NW];
+    float *s_coord = lds.coord;
+    uint64_t *s_key = lds.key;
+    uint16_t *s_val = lds.val;
+    int *s_hist = lds.hist;
+    auto &s_red = lds.red;
 
     const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
-    const int seg = blockIdx.x / d.cpb;
-    const int chunk = blockIdx.x % d.cpb;
+    const int seg = bid / d.cpb;
+    const int chunk = bid % d.cpb;
     const int slot0 = (mode == kModeSlide) ? st.u * d.S : seg * d.S;
     const float *src = (mode == kModeSlide) ? st.src : src_dev + (size_t)seg * d.S * DIM;
     const int j0 = chunk * CH;
@@ -342,6 +351,14 @@ __global__ __launch_bounds__(NT) void k_ingest_sort(Dev d, Step st, int mode, co
     }
 }
 
+template <int DIM, int NT, int IT>
+__global__ __launch_bounds__(NT) void k_ingest_sort(Dev d, Step st, int mode, const float *src_dev,
+                                                    int dst_batch)
+{
+    __shared__ IngestLds<DIM, NT, IT> lds;
+    ingest_sort_body<DIM, NT, IT>(d, st, mode, src_dev, dst_batch, blockIdx.x, lds);
+}
+
 // ---------------------------------------------------------------------------
 // k_merge_pass: within each segment of seg_len elements, merge sorted runs of
 // length L pairwise into runs of 2L (stable: the earlier run wins ties).
@@ -350,14 +367,14 @@ __global__ __launch_bounds__(NT) void k_ingest_sort(Dev d, Step st, int mode, co
 // (slide: st.enew, prime: s) and fills its sample table.
 // ---------------------------------------------------------------------------
 template <int DIM>
-__global__ __launch_bounds__(256) void k_merge_pass(Dev d, Step st, const uint64_t *sk,
-                                                    const int32_t *ss, const float *sx, size_t sxs,
-                                                    const int32_t *sp, uint64_t *dk, int32_t *ds,
-                                                    float *dx, size_t dxs, int32_t *dp,
-                                                    int n_total, int seg_len, int L, int dst_batch,
-                                                    int mode)
+__device__ __forceinline__ void merge_pass_body(const Dev &d, const Step &st, const uint64_t *sk,
+                                                const int32_t *ss, const float *sx, size_t sxs,
+                                                const int32_t *sp, uint64_t *dk, int32_t *ds,
+                                                float *dx, size_t dxs, int32_t *dp, int n_total,
+                                                int seg_len, int L, int dst_batch, int mode,
+                                                int bid)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = bid * 256 + threadIdx.x;
     if (i >= n_total)
         return;
     const int seg = i / seg_len;
@@ -392,6 +409,18 @@ __global__ __launch_bounds__(256) void k_merge_pass(Dev d, Step st, const uint64
         dx[(size_t)a * dxs + o] = sx[(size_t)a * sxs + i];
     if (dp) // batch position: from the source, or the index inside its batch
         dp[o] = sp ? sp[i] : (i % d.S);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_merge_pass(Dev d, Step st, const uint64_t *sk,
+                                                    const int32_t *ss, const float *sx, size_t sxs,
+                                                    const int32_t *sp, uint64_t *dk, int32_t *ds,
+                                                    float *dx, size_t dxs, int32_t *dp,
+                                                    int n_total, int seg_len, int L, int dst_batch,
+                                                    int mode)
+{
+    merge_pass_body<DIM>(d, st, sk, ss, sx, sxs, sp, dk, ds, dx, dxs, dp, n_total, seg_len, L,
+                         dst_batch, mode, blockIdx.x);
 }
 
 // sample table of a freshly built sorted window
@@ -442,26 +471,42 @@ __device__ __forceinline__ int wave_count_lds(const uint64_t *tab, int n, uint64
 // A[i0]'s own batch position when A[i0] expires.  Output positions that are
 // multiples of the sample stride fill A''s sample table.
 // ---------------------------------------------------------------------------
+constexpr int kMergeT = 1024; // window elements per merge workgroup
+
+struct MergeWindowLds {
+    static constexpr int NT = 256, IT = 4, T = NT * IT, CAP = 2048, MAXBS = 512;
+    uint64_t k[T];
+    uint64_t bs[MAXBS], es[MAXBS];
+    alignas(16) uint64_t as[kMaxSamples];
+    alignas(16) uint64_t br[CAP];
+    int rc[IT][NT / kWave];
+    int res[8];
+    uint64_t kn;
+    int pos0, slot0;
+};
+
 template <int DIM>
-__global__ __launch_bounds__(256) void k_merge_window(Dev d, Step st)
+__device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, int bid,
+                                                  MergeWindowLds &lds)
 {
-    constexpr int NT = 256, IT = 4, T = NT * IT, CAP = 2048, MAXBS = 512;
-    __shared__ uint64_t s_k[T];
-    __shared__ uint64_t s_bs[MAXBS], s_es[MAXBS];
-    __shared__ __attribute__((aligned(16))) uint64_t s_as[kMaxSamples];
-    __shared__ __attribute__((aligned(16))) uint64_t s_br[CAP];
-    __shared__ int s_rc[IT][NT / kWave];
-    __shared__ int s_res[8];
-    __shared__ uint64_t s_kn;
-    __shared__ int s_pos0, s_slot0;
+    constexpr int NT = 256, IT = 4, T = NT * IT, CAP = 2048;
+    static_assert(T == kMergeT, "merge tile");
+    uint64_t *s_k = lds.k;
+    uint64_t *s_bs = lds.bs, *s_es = lds.es;
+    uint64_t *s_as = lds.as;
+    uint64_t *s_br = lds.br;
+    auto &s_rc = lds.rc;
+    int *s_res = lds.res;
+    uint64_t &s_kn = lds.kn;
+    int &s_pos0 = lds.pos0, &s_slot0 = lds.slot0;
 
     const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
     const int W = d.W, S = d.S;
     // a learned context appends one training workgroup to the merge grid
     const int nmb = (W + T - 1) / T;
-    static_assert(4 * sizeof(TrainWaveLds) <= sizeof(s_as), "training LDS");
-    static_assert(sizeof(TrainLds) <= sizeof(s_br), "training LDS");
-    if ((int)blockIdx.x >= nmb) {
+    static_assert(4 * sizeof(TrainWaveLds) <= sizeof(lds.as), "training LDS");
+    static_assert(sizeof(TrainLds) <= sizeof(lds.br), "training LDS");
+    if (bid >= nmb) {
         if (d.lr.on)
             train_block(d, nmb, s_as, s_br);
         return;
@@ -484,18 +529,18 @@ __global__ __launch_bounds__(256) void k_merge_window(Dev d, Step st)
     const uint64_t *Esamp = d.bsamp + (size_t)st.eold * d.nbs;
     const uint64_t *Bsamp = d.bsamp + (size_t)st.enew * d.nbs;
     const int exp_lo = st.u * S;
-    const int i0 = blockIdx.x * T;
+    const int i0 = bid * T;
     const int i1 = min(W, i0 + T);
     const int n = i1 - i0;
-    const bool first = blockIdx.x == 0;
+    const bool first = bid == 0;
     const bool last = i1 == W;
     const int sl2 = d.stride_log2, smask = (1 << sl2) - 1;
     const int bsl = d.bs_log2, nbs = d.nbs;
     // this workgroup's share of B
     const int bper = (S + nmb - 1) / nmb;
-    const int j0 = min(S, (int)blockIdx.x * bper), j1 = min(S, j0 + bper);
+    const int j0 = min(S, bid * bper), j1 = min(S, j0 + bper);
 
-    if (blockIdx.x == 0 && tid == 0) {
+    if (bid == 0 && tid == 0) {
         d.ctl->n_cand = 0;
         if (d.lr.on)
             d.lr.lc->n_cand_i = 0;
@@ -681,6 +726,13 @@ __global__ __launch_bounds__(256) void k_merge_window(Dev d, Step st)
     }
 }
 
+template <int DIM>
+__global__ __launch_bounds__(256) void k_merge_window(Dev d, Step st)
+{
+    __shared__ MergeWindowLds lds;
+    merge_window_body<DIM>(d, st, blockIdx.x, lds);
+}
+
 // ---------------------------------------------------------------------------
 // k_classify: a point is fast-accepted iff its cell is unclamped and holds
 // >= K+1 points (reference: cells with num > K are skipped, optixScan.cpp:
@@ -698,14 +750,22 @@ __global__ __launch_bounds__(256) void k_merge_window(Dev d, Step st)
 constexpr int kClassifyNT = 256;
 constexpr int kClassifyHalo = 512;
 
+template <bool LEARN> struct ClassifyLds {
+    static constexpr int NW = kClassifyNT / kWave;
+    uint64_t t[kClassifyNT + 2 * kClassifyHalo];
+    alignas(16) __bf16 h[LEARN ? NW : 1][16 * kMlpH];
+    float z[LEARN ? NW : 1][kWave];
+};
+
 template <int DIM, bool LEARN>
-__global__ __launch_bounds__(kClassifyNT) void k_classify(Dev d, Step st)
+__device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int bid,
+                                              ClassifyLds<LEARN> &lds)
 {
-    constexpr int NT = kClassifyNT, HALO = kClassifyHalo, NW = NT / kWave;
-    __shared__ uint64_t s_t[NT + 2 * HALO];
-    __shared__ __attribute__((aligned(16))) __bf16 s_h[LEARN ? NW : 1][16 * kMlpH];
-    __shared__ float s_z[LEARN ? NW : 1][kWave];
-    const int i0 = blockIdx.x * NT;
+    constexpr int NT = kClassifyNT, HALO = kClassifyHalo;
+    uint64_t *s_t = lds.t;
+    auto &s_h = lds.h;
+    auto &s_z = lds.z;
+    const int i0 = bid * NT;
     const int i = i0 + threadIdx.x;
     const uint64_t *Nk = d.akey[st.par ^ 1];
     const int W = d.W, K = d.K;
@@ -812,6 +872,13 @@ __global__ __launch_bounds__(kClassifyNT) void k_classify(Dev d, Step st)
             *reinterpret_cast<float4 *>(rx + 4) = make_float4(f[4], f[5], 0.f, 0.f);
         }
     }
+}
+
+template <int DIM, bool LEARN>
+__global__ __launch_bounds__(kClassifyNT) void k_classify(Dev d, Step st)
+{
+    __shared__ ClassifyLds<LEARN> lds;
+    classify_body<DIM, LEARN>(d, st, blockIdx.x, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1168,16 +1235,29 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int n
 // is stored in the replay, and mispredictions are counted for the fallback
 // rule.
 // ---------------------------------------------------------------------------
+struct DetectLds {
+    static constexpr int NW = kDetectNT / kWave;
+    uint64_t samp[kMaxSamples];
+    int rows[NW][2][64];
+    int scan[NW];
+    int last;
+    int miss;
+    unsigned long long tests;
+};
+
+// bid / nblk: this workgroup and the number of workgroups of the detect
+// phase of this stream
 template <int DIM, bool LEARN>
-__global__ __launch_bounds__(kDetectNT) void k_detect(Dev d, Step st)
+__device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bid, int nblk,
+                                            DetectLds &lds)
 {
     constexpr int NW = kDetectNT / kWave;
-    __shared__ uint64_t s_samp[kMaxSamples];
-    __shared__ int s_rows[NW][2][64];
-    __shared__ int s_scan[NW];
-    __shared__ int s_last;
-    __shared__ unsigned long long s_tests;
-    __shared__ int s_miss;
+    uint64_t *s_samp = lds.samp;
+    auto &s_rows = lds.rows;
+    int *s_scan = lds.scan;
+    int &s_last = lds.last;
+    unsigned long long &s_tests = lds.tests;
+    int &s_miss = lds.miss;
 
     const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
     const int par = st.par ^ 1;
@@ -1188,8 +1268,8 @@ __global__ __launch_bounds__(kDetectNT) void k_detect(Dev d, Step st)
     const int n_o = d.ctl->n_cand;
     const int n_cand = LEARN ? n_o + d.lr.lc->n_cand_i : n_o;
     const bool use_pred = LEARN && d.lr.lc->use_pred != 0;
-    const int busy = min((int)gridDim.x, max(1, (n_cand + NW - 1) / NW));
-    if ((int)blockIdx.x >= busy)
+    const int busy = min(nblk, max(1, (n_cand + NW - 1) / NW));
+    if (bid >= busy)
         return;
 
     if (tid == 0) {
@@ -1201,7 +1281,7 @@ __global__ __launch_bounds__(kDetectNT) void k_detect(Dev d, Step st)
 
     long long tests = 0;
     int misses = 0;
-    for (int ci = blockIdx.x * NW + wid; ci < n_cand; ci += busy * NW) {
+    for (int ci = bid * NW + wid; ci < n_cand; ci += busy * NW) {
         const int li = (!LEARN || ci < n_o) ? ci : W - 1 - (ci - n_o);
         const int p = __builtin_amdgcn_readfirstlane(d.cand[li]);
         int cnt;
@@ -1266,6 +1346,13 @@ __global__ __launch_bounds__(kDetectNT) void k_detect(Dev d, Step st)
         }
     }
     finish_slide(d, st, n_cand, s_scan);
+}
+
+template <int DIM, bool LEARN>
+__global__ __launch_bounds__(kDetectNT) void k_detect(Dev d, Step st)
+{
+    __shared__ DetectLds lds;
+    detect_body<DIM, LEARN>(d, st, blockIdx.x, gridDim.x, lds);
 }
 
 } // namespace rtod

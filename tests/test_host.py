@@ -96,3 +96,28 @@ def test_predictor_entry_points_reject_bad_arguments_without_device():
     assert lib.rtod_mlp_grad(0, p, x, y, 0, g, C.byref(loss)) == 1     # n < 1
     assert lib.rtod_mlp_grad(0, p, x, y, 129, g, C.byref(loss)) == 1   # n > 128
     assert lib.rtod_mlp_forward(0, p, x, -1, y) == 1
+
+
+@pytest.mark.parametrize("streams,status", [
+    ([], 1),                                            # no streams
+    ([(1, 100, 30, 0.5, 5)], 1),                        # W % S != 0
+    ([(3, 1000, 100, 1.0, 5), (4, 100, 10, 0.5, 5)], 1),  # bad dim in the second
+])
+def test_batch_create_rejects_bad_configs_without_device(streams, status):
+    with pytest.raises(R.RtodError) as e:
+        R.Batch(streams)
+    assert e.value.status == status
+
+
+def test_batch_rejects_learned_and_lbvh_without_device():
+    import numpy as np
+    lib = R.lib()
+    for idx, sch in [(R.INDEX_LBVH, R.SCHED_PLAIN), (R.INDEX_AUTO, R.SCHED_LEARNED)]:
+        cfgs = (R._Config * 1)(R._Config(3, 1000, 100, 1.0, 5, idx, sch, 0.25, 0, 0))
+        h = C.c_void_p()
+        assert lib.rtod_batch_create(cfgs, 1, 0, C.byref(h)) == 2
+    assert lib.rtod_batch_create(None, 1, 0, C.byref(C.c_void_p())) == 1
+    assert lib.rtod_batch_prime(None, None) == 1
+    z = np.zeros(4, np.int32)
+    assert lib.rtod_batch_step_device(None, z, z) == 1
+    lib.rtod_batch_destroy(None)  # no-op
