@@ -16,6 +16,10 @@ Extra objects on the line:
                 8 TB/s HBM peak, algorithmic bytes B = 4*d*(W+S) + 4*|O|
   cpu_baseline  the grid oracle (same algorithm, C, pthreads) timed on this
                 host's cores on a bounded sample of the same stream
+  batched       256 independent streams of uneven dim/W/S/R/K (the batched
+                workload): per-step time of the batched API after its
+                autotuner, its roofline over all streams' bytes, and the grid
+                oracle over the same streams on the host cores
   learned       the same workloads with the learned work scheduler (online-
                 trained MLP ordering the exact range counts): per-slide time,
                 speed-up over the plain schedule, predictor miss rate, slides
@@ -43,22 +47,24 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-WORKLOADS = {
+def _workloads():
+    from rtod_amd import synth
+
     # name: (dim, N, W, S, R, K, generator, seed)
-    "walk1d-W100k-S5k": (1, 1_048_572, 100_000, 5_000, 0.45, 50, "walk", 2),
-    "walk1d-W10k-S500": (1, 200_000, 10_000, 500, 0.45, 50, "walk", 1),
-    "gmm3d-W10k-S500": (3, 575_000, 10_000, 500, 1.9, 50, "gmm", 3),
-}
+    return {w.name: (w.dim, w.N, w.W, w.S, w.R, w.K, w.kind, w.seed)
+            for w in synth.WORKLOADS.values()}
+
+
+WORKLOADS = _workloads()
 
 
 def stream(name: str, slides: int) -> np.ndarray:
-    import oracle
+    from rtod_amd import synth
 
-    dim, N, W, S, R, K, gen, seed = WORKLOADS[name]
     # the measured streams have N points; longer runs extend the same seeded
     # generator
-    n = max(N, W + slides * S)
-    return oracle.gen_walk(seed, n, dim) if gen == "walk" else oracle.gen_gmm(seed, n, dim)
+    dim, N, W, S = WORKLOADS[name][:4]
+    return synth.WORKLOADS[name].data(W + slides * S)
 
 
 def run_gpu(xs, W, S, R, K, warmup, steps, flags=0, sched=0, device_input=True, index=0,
@@ -203,6 +209,79 @@ def learned_comparison(names, train_slides, steps):
     return out
 
 
+def batched(n_streams, warmup, steps, tune_slides, cpu_slides, with_cpu):
+    """The batched workload: n_streams independent streams (dim ~ U{1,2,3},
+    W log-uniform on [2k, 100k], S = 1-5% of W, K in {20, 50}, R set for a
+    median neighbour count ~2K), resident in HBM, advanced together."""
+    import rtod_amd
+    from rtod_amd import synth
+
+    slides = tune_slides + warmup + steps
+    t0 = time.perf_counter()
+    descs, data = synth.batch_workload(n_streams, slides=slides)
+    log(f"[bench] batched: {n_streams} streams generated in {time.perf_counter() - t0:.1f} s, "
+        f"sum W {sum(d.W for d in descs)}, sum S {sum(d.S for d in descs)}")
+    cfg = [(d.dim, d.W, d.S, d.R, d.K) for d in descs]
+    dss = [rtod_amd.DeviceStream(x) for x in data]
+    b = rtod_amd.Batch(cfg)
+    b.bind_device([ds.ptr.value for ds in dss], [x.shape[0] for x in data])
+    tuning = b.autotune(tune_slides)
+    log(f"[bench] batched autotune: {tuning}")
+    # the autotuner leaves the batch primed at slide 0: warm up, then time
+    outs = 0
+    for _ in range(warmup):
+        b.step_device()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        lists = b.step_device()
+        outs += sum(len(x) for x in lists)
+    dt = (time.perf_counter() - t0) / steps
+    b.close()
+    for ds in dss:
+        ds.close()
+    us = dt * 1e6
+    out_per_step = outs / steps
+    bytes_step = sum(4 * d.dim * (d.W + d.S) for d in descs) + 4 * out_per_step
+    achieved = bytes_step / dt / 1e9
+    ent = {"workload": f"batch-{n_streams} streams: dim U{{1,2,3}}, W log-U[2k,100k], "
+                       f"S 1-5% of W, K in {{20,50}}, R for median count ~2K",
+           "streams": n_streams, "sum_W": int(sum(d.W for d in descs)),
+           "sum_S": int(sum(d.S for d in descs)),
+           "us_per_step": round(us, 2),
+           "new_points_per_s": round(sum(d.S for d in descs) / dt),
+           "outliers_per_step": round(out_per_step, 1),
+           "tuning": tuning,
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                        "algorithmic_bytes": int(bytes_step)}}
+    if with_cpu:
+        ent["cpu_baseline"] = batched_cpu(descs, data, cpu_slides)
+        ent["speedup_vs_cpu"] = round(ent["cpu_baseline"]["value"] / us, 2)
+    log(f"[bench] batched: {ent}")
+    return ent
+
+
+def batched_cpu(descs, data, slides):
+    """Grid oracle over the same streams, one stream per task on a pool of
+    host threads (ctypes releases the GIL); wall time of `slides` steps."""
+    import oracle
+    from concurrent.futures import ThreadPoolExecutor
+
+    threads = min(os.cpu_count() or 1, 16)
+    grids = [oracle.Grid(d.dim, d.W, d.S, d.R, d.K) for d in descs]
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda i: grids[i].prime(data[i][:descs[i].W]), range(len(descs))))
+        t0 = time.perf_counter()
+        for t in range(1, slides + 1):
+            list(ex.map(lambda i: grids[i].slide(
+                data[i][descs[i].W + (t - 1) * descs[i].S: descs[i].W + t * descs[i].S]),
+                range(len(descs))))
+        dt = (time.perf_counter() - t0) / slides
+    return {"value": round(dt * 1e6, 1), "unit": "us/step", "cores": threads, "kind": "port",
+            "sample": f"grid oracle (C), every stream's first {slides} slides, streams spread "
+                      f"over a {threads}-thread pool, one barrier per step"}
+
+
 def other_workloads(names, steps, cpu_slides, with_cpu):
     out = {}
     for name in names:
@@ -236,6 +315,9 @@ def main() -> int:
                     help="skip the 1-D W=10k and 3-D workloads")
     ap.add_argument("--workload", default="walk1d-W100k-S5k", choices=sorted(WORKLOADS),
                     help="profiling runs only: the JSON line always names the workload run")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-streams measurement")
+    ap.add_argument("--batch-streams", type=int, default=256)
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-schedule comparison")
     ap.add_argument("--train-slides", type=int, default=200,
@@ -293,6 +375,10 @@ def main() -> int:
     indexes = None
     if not args.no_others and args.workload == "walk1d-W100k-S5k":
         indexes = index_comparison(["walk1d-W100k-S5k", "gmm3d-W10k-S500"], 60)
+
+    batch = None
+    if not args.no_batched and args.workload == "walk1d-W100k-S5k":
+        batch = batched(args.batch_streams, 5, 40, 4, 3, not args.no_cpu)
 
     learned = None
     if not args.no_learned:
@@ -354,6 +440,8 @@ def main() -> int:
         line["index_comparison"] = indexes
     if learned:
         line["learned"] = learned
+    if batch:
+        line["batched"] = batch
     print(json.dumps(line), flush=True)
     return 0
 
