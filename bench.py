@@ -318,6 +318,8 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-streams measurement")
     ap.add_argument("--batch-streams", type=int, default=256)
+    ap.add_argument("--batched-only", action="store_true",
+                    help="profiling runs: only the batched measurement, its JSON object")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-schedule comparison")
     ap.add_argument("--train-slides", type=int, default=200,
@@ -331,6 +333,10 @@ def main() -> int:
         return 2
     if args.calib_copy:
         calib_copy()
+        return 0
+    if args.batched_only:
+        print(json.dumps(batched(args.batch_streams, min(args.warmup, 5), args.steps, 4, 3,
+                                 not args.no_cpu)), flush=True)
         return 0
 
     dim, _, W, S, R, K, _, _ = WORKLOADS[args.workload]

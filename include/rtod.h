@@ -96,8 +96,10 @@ typedef struct rtod_stats {
     int64_t outliers_total;
     int64_t device_bytes;       /* device memory owned by the context      */
     int64_t host_pinned_bytes;  /* pinned host memory owned by the context */
-    int64_t predictor_misses;   /* learned scheduler: mispredicted points  */
-    int64_t predictor_points;   /* learned scheduler: predicted points     */
+    int64_t predictor_misses;   /* learned scheduler: mispredicted cell
+                                   units (a unit: <= 64 candidates of one
+                                   sparse cell, counted together)         */
+    int64_t predictor_points;   /* learned scheduler: predicted units      */
     int64_t fallbacks;          /* slides run on the plain schedule because
                                    the previous slide's miss rate exceeded
                                    the threshold (or no prediction yet)   */

@@ -102,7 +102,7 @@ __global__ __launch_bounds__(kBatchNT) void k_batch_classify(const BatchItem *it
     __shared__ ClassifyLds<false> lds;
     const BatchItem it = items[blockIdx.x];
     const BatchStream &bs = streams[it.stream];
-    classify_body<DIM, false>(bs.d, batch_step(bs), it.bid, lds);
+    classify_body<DIM, kClsUnits>(bs.d, batch_step(bs), it.bid, lds);
 }
 
 template <int DIM>

@@ -7,10 +7,10 @@
 // outcome.  Both hidden layers run on the matrix cores as
 // v_mfma_f32_16x16x32_bf16 (BF16 operands, FP32 accumulation); the output
 // layer, biases and activations are FP32.  Training is online: a single
-// workgroup takes Adam steps on minibatches drawn from the exact counts of
-// the previous 20 slides.  It rides in the merge kernel's grid as one extra
-// workgroup, on a CU the latency-bound merge leaves idle, and keeps stepping
-// while the merge workgroups are still running.
+// workgroup, launched on a stream of its own after each slide, takes Adam
+// steps on minibatches drawn from the exact counts of the previous slides
+// (20 kept) while the next slide's latency-bound ingest and merge leave CUs
+// idle; new weights are published through a double buffer.
 //
 // MFMA 16x16x32 lane layout (lane l of 64): A operand holds row l%16,
 // k = (l/16)*8 + i (i < 8); B operand holds column l%16, the same k; the
@@ -48,7 +48,7 @@ constexpr int kFeatStride = 8;      // floats per stored feature row
 constexpr float kInlierCut = 0.75f; // predicted inlier iff output >= this
 constexpr int kTrainNT = 256;       // training workgroup: 4 waves
 constexpr int kTrainRows = 128;     // samples per step: 32 per wave
-constexpr int kTrainMaxSteps = 8;   // per slide
+constexpr int kTrainSteps = 2;      // Adam steps per training launch
 constexpr float kAdamLr = 5e-3f, kAdamB1 = 0.9f, kAdamB2 = 0.999f, kAdamEps = 1e-7f;
 
 __device__ __forceinline__ f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c)
@@ -57,6 +57,9 @@ __device__ __forceinline__ f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c)
 }
 
 __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
+
+// logistic function from the hardware exp2 and reciprocal (~1 ulp each)
+__device__ __forceinline__ float sigmoidf(float z) { return __frcp_rn(1.f + __expf(-z)); }
 
 __device__ __forceinline__ float mlp_target(int cnt, int K)
 {
@@ -125,56 +128,77 @@ __device__ __forceinline__ float cross_group_sum(float x)
 }
 
 // ---------------------------------------------------------------------------
-// Inference for one wavefront: lane l's features f -> predicted output.
-// The 64 lanes form four 16-row tiles.  s_h: 16*32 bf16, s_z: 64 floats,
-// both private to the wave.  Every lane of the wave must call it.
+// Inference for up to 64 rows of features held by one wavefront in LDS:
+// s_x[r][0..7] (zero padded) for r < m -> s_z[r] = the pre-sigmoid output.
+// Only ceil(m/16) 16-row tiles run, all tiles of a layer back to back, so a
+// layer costs one LDS round trip.  s_h: 64*32 bf16, private to the wave.
+// Every lane of the wave must call it (m uniform).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float mlp_infer_wave(const float *P, const float (&f)[kMlpIn],
-                                                __bf16 *s_h, float *s_z)
+__device__ __forceinline__ void mlp_infer_rows(const float *P, const float (*s_x)[kFeatStride],
+                                               int m, __bf16 *s_h, float *s_z)
 {
     const int l = lane_id(), n0 = l & 15, grp = l >> 4;
+    const int tiles = (m + 15) >> 4;
     const bf16x8 w1a = b_w1(P, 0), w1b = b_w1(P, 1);
     const bf16x8 w2a = b_w2(P, 0), w2b = b_w2(P, 1);
     const float b1a = P[kOffB1 + n0], b1b = P[kOffB1 + n0 + 16];
     const float b2a = P[kOffB2 + n0], b2b = P[kOffB2 + n0 + 16];
     const float w3a = P[kOffW3 + n0], w3b = P[kOffW3 + n0 + 16];
-    const float b3 = P[kOffB3];
     const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+    // layer 1: A = feature rows (k < 8 in lane group 0, zero elsewhere)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        float x[kMlpIn];
+        if (r < tiles) {
+            bf16x8 a;
 #pragma unroll
-        for (int j = 0; j < kMlpIn; ++j)
-            x[j] = __shfl(f[j], r * 16 + n0, kWave);
-        bf16x8 a;
+            for (int i = 0; i < 8; ++i)
+                a[i] = to_bf16(grp == 0 ? s_x[r * 16 + n0][i] : 0.f);
+            const f32x4 c0 = mfma_bf16(a, w1a, zero);
+            const f32x4 c1 = mfma_bf16(a, w1b, zero);
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
-            a[i] = to_bf16((grp == 0 && i < kMlpIn) ? x[i < kMlpIn ? i : 0] : 0.f);
-        const f32x4 c0 = mfma_bf16(a, w1a, zero);
-        const f32x4 c1 = mfma_bf16(a, w1b, zero);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int row = grp * 4 + v;
-            s_h[row * kMlpH + n0] = to_bf16(fmaxf(c0[v] + b1a, 0.f));
-            s_h[row * kMlpH + n0 + 16] = to_bf16(fmaxf(c1[v] + b1b, 0.f));
-        }
-        wave_lds_fence();
-        const bf16x8 a2 = *reinterpret_cast<const bf16x8 *>(&s_h[n0 * kMlpH + grp * 8]);
-        wave_lds_fence();
-        const f32x4 d0 = mfma_bf16(a2, w2a, zero);
-        const f32x4 d1 = mfma_bf16(a2, w2b, zero);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            float part = fmaxf(d0[v] + b2a, 0.f) * w3a + fmaxf(d1[v] + b2b, 0.f) * w3b;
-            part = group16_sum(part);
-            if (n0 == 0)
-                s_z[r * 16 + grp * 4 + v] = part;
+            for (int v = 0; v < 4; ++v) {
+                const int row = r * 16 + grp * 4 + v;
+                s_h[row * kMlpH + n0] = to_bf16(fmaxf(c0[v] + b1a, 0.f));
+                s_h[row * kMlpH + n0 + 16] = to_bf16(fmaxf(c1[v] + b1b, 0.f));
+            }
         }
     }
     wave_lds_fence();
-    const float z = s_z[l] + b3;
+    // layer 2 and the output dot product
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        if (r < tiles) {
+            const bf16x8 a2 =
+                *reinterpret_cast<const bf16x8 *>(&s_h[(r * 16 + n0) * kMlpH + grp * 8]);
+            const f32x4 d0 = mfma_bf16(a2, w2a, zero);
+            const f32x4 d1 = mfma_bf16(a2, w2b, zero);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                float part = fmaxf(d0[v] + b2a, 0.f) * w3a + fmaxf(d1[v] + b2b, 0.f) * w3b;
+                part = group16_sum(part);
+                if (n0 == 0)
+                    s_z[r * 16 + grp * 4 + v] = part;
+            }
+        }
+    }
     wave_lds_fence();
-    return 1.f / (1.f + expf(-z));
+}
+
+// One wavefront, lane l's features f -> predicted output (all 64 rows).
+// s_x: 64 x 8 floats, s_h: 64*32 bf16, s_z: 64 floats, private to the wave.
+__device__ __forceinline__ float mlp_infer_wave(const float *P, const float (&f)[kMlpIn],
+                                                float (*s_x)[kFeatStride], __bf16 *s_h,
+                                                float *s_z)
+{
+    const int l = lane_id();
+#pragma unroll
+    for (int j = 0; j < kFeatStride; ++j)
+        s_x[l][j] = j < kMlpIn ? f[j < kMlpIn ? j : 0] : 0.f;
+    wave_lds_fence();
+    mlp_infer_rows(P, s_x, kWave, s_h, s_z);
+    const float z = s_z[l] + P[kOffB3];
+    wave_lds_fence();
+    return sigmoidf(z);
 }
 
 // ---------------------------------------------------------------------------
@@ -192,7 +216,6 @@ struct TrainLds {
     float p[kMlpNP + 3]; // weights
     float g[kMlpNP + 3]; // gradient of the current step
     float loss;
-    int stop;
     int pref[kReplaySlides + 1];
 };
 
@@ -257,7 +280,7 @@ __device__ __forceinline__ float mlp_wave_grads(const float *P, float *G, TrainW
             const float z = group16_sum(part) + b3;
             const int row = r * 16 + grp * 4 + v;
             const bool valid = row0 + row < n_total;
-            const float yh = 1.f / (1.f + expf(-z));
+            const float yh = sigmoidf(z);
             const float e = yh - w.ys[row];
             dz[r][v] = valid ? 2.f * e * yh * (1.f - yh) * inv_n : 0.f;
             if (valid && n0 == 0)
@@ -375,23 +398,26 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x)
 }
 
 // ---------------------------------------------------------------------------
-// The training workgroup (kTrainNT threads).  Draws kTrainRows samples per
-// step uniformly from the replay of the previous slides, takes Adam steps
-// until the merge workgroups of this slide have all finished (at least one,
-// at most kTrainMaxSteps), then writes the weights back.  n_merge: the
-// number of merge workgroups.  lds_wave >= 4 * sizeof(TrainWaveLds),
-// lds_blk >= sizeof(TrainLds).
+// k_train: one workgroup of kTrainNT threads, launched on its own stream
+// after a slide, overlapping the next slide's ingest and merge on CUs they
+// leave idle.  `steps` Adam steps, each on kTrainRows samples drawn uniformly
+// from the replay, skipping the slots of slides t_next and t_next+1 (those
+// the next slides may be rewriting meanwhile).  The FP32 master weights and
+// the moments are the trainer's own; the result is published to the
+// inactive half of the double-buffered weights and then made current, so a
+// classify never waits for training and never reads a half-written set.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void train_block(const Dev &d, int n_merge, void *lds_wave,
-                                            void *lds_blk)
+__global__ __launch_bounds__(kTrainNT) void k_train(Dev d, uint64_t t_next, int steps)
 {
+    __shared__ TrainWaveLds lds_wave[kTrainNT / kWave];
+    __shared__ TrainLds s;
     const Learn &L = d.lr;
     LearnCtl *lc = L.lc;
     const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
-    TrainLds &s = *reinterpret_cast<TrainLds *>(lds_blk);
-    TrainWaveLds &w = reinterpret_cast<TrainWaveLds *>(lds_wave)[wid];
+    TrainWaveLds &w = lds_wave[wid];
     if (lc->frozen)
         return;
+    const int skip0 = (int)(t_next % kReplaySlides), skip1 = (int)((t_next + 1) % kReplaySlides);
     constexpr int PER = (kMlpNP + kTrainNT - 1) / kTrainNT;
     float am[PER], av[PER];
 #pragma unroll
@@ -409,26 +435,30 @@ __device__ __forceinline__ void train_block(const Dev &d, int n_merge, void *lds
         int acc = 0;
         s.pref[0] = 0;
         for (int j = 0; j < kReplaySlides; ++j) {
-            acc += L.rep_n[2 * j] + L.rep_n[2 * j + 1];
+            if (j != skip0 && j != skip1)
+                acc += L.rep_n[2 * j] + L.rep_n[2 * j + 1];
             s.pref[j + 1] = acc;
         }
         s.loss = 0.f;
-        s.stop = 0;
     }
     __syncthreads();
     const int N = s.pref[kReplaySlides];
     if (N <= 0)
         return;
     int t_adam = lc->adam_t;
-    int steps = 0;
     const int W = d.W;
-    for (;;) {
+    for (int it = 0; it < steps; ++it) {
         if (lane < 32) {
             const uint32_t h = mix32((uint32_t)t_adam * 0x9e3779b9u + (uint32_t)(wid * 32 + lane));
             const int u = (int)(h % (uint32_t)N);
-            int j = 0;
-            while (j + 1 < kReplaySlides && s.pref[j + 1] <= u)
-                ++j;
+            int j = 0, hi = kReplaySlides - 1; // last slot with pref[j] <= u
+            while (j < hi) {
+                const int mid = (j + hi + 1) >> 1;
+                if (s.pref[mid] <= u)
+                    j = mid;
+                else
+                    hi = mid - 1;
+            }
             const int off = u - s.pref[j];
             const int nO = L.rep_n[2 * j];
             const int li = off < nO ? off : W - 1 - (off - nO);
@@ -466,30 +496,36 @@ __device__ __forceinline__ void train_block(const Dev &d, int n_merge, void *lds
                 s.p[p] -= kAdamLr * mh / (sqrtf(vh) + kAdamEps);
             }
         }
-        ++steps;
-        if (tid == 0) {
-            const int done =
-                __hip_atomic_load(&lc->mw_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s.stop = (steps >= kTrainMaxSteps || done >= n_merge) ? 1 : 0;
-        }
         __syncthreads();
-        if (s.stop)
-            break;
     }
+    const int cur = __hip_atomic_load(&lc->pub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    float *pub = L.pub + (size_t)((cur + 1) & 1) * kMlpNP;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int p = tid + q * kTrainNT;
         if (p < kMlpNP) {
             L.params[p] = s.p[p];
+            pub[p] = s.p[p];
             L.adam_m[p] = am[q];
             L.adam_v[p] = av[q];
         }
     }
+    __threadfence();
+    __syncthreads();
     if (tid == 0) {
         lc->adam_t = t_adam;
-        lc->steps_last = steps;
+        lc->steps_total += steps;
         lc->loss_last = s.loss / (float)(steps * kTrainRows);
+        __threadfence();
+        __hip_atomic_store(&lc->pub, cur + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+
+// the weights classify uses: the published half of the double buffer
+__device__ __forceinline__ const float *current_weights(const Learn &L)
+{
+    const int cur = __hip_atomic_load(&L.lc->pub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return L.pub + (size_t)(cur & 1) * kMlpNP;
 }
 
 // ---------------------------------------------------------------------------
@@ -498,7 +534,8 @@ __device__ __forceinline__ void train_block(const Dev &d, int n_merge, void *lds
 __global__ __launch_bounds__(256) void k_mlp_forward(const float *P, const float *X, int n,
                                                      float *out)
 {
-    __shared__ __attribute__((aligned(16))) __bf16 s_h[4][16 * kMlpH];
+    __shared__ __attribute__((aligned(16))) __bf16 s_h[4][kWave * kMlpH];
+    __shared__ float s_x[4][kWave][kFeatStride];
     __shared__ float s_z[4][kWave];
     const int wid = threadIdx.x / kWave;
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -506,7 +543,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float *P, const float
 #pragma unroll
     for (int j = 0; j < kMlpIn; ++j)
         f[j] = i < n ? X[(size_t)i * kMlpIn + j] : 0.f;
-    const float y = mlp_infer_wave(P, f, s_h[wid], s_z[wid]);
+    const float y = mlp_infer_wave(P, f, s_x[wid], s_h[wid], s_z[wid]);
     if (i < n)
         out[i] = y;
 }

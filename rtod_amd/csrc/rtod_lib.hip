@@ -89,6 +89,9 @@ struct rtod_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = true; // false: a batch member on the batch's stream
+    hipStream_t train_stream = nullptr; // learned: the predictor's training launches
+    hipEvent_t train_ev = nullptr;
+    bool train_pending = false;
     Dev d{};
     float *d_prime = nullptr; // device copy of the initial window
     float *h_in = nullptr;
@@ -213,8 +216,7 @@ void launch_batch_merges(rtod_ctx *c, hipStream_t s, const Step &st, int mode, i
 
 void launch_merge_window(rtod_ctx *c, hipStream_t s, const Step &st)
 {
-    // learned: one more workgroup trains the predictor beside the merge
-    const int grid = (c->W + 1023) / 1024 + (c->learned ? 1 : 0);
+    const int grid = (c->W + 1023) / 1024;
     if (c->dim == 1)
         hipLaunchKernelGGL(k_merge_window<1>, dim3(grid), dim3(256), 0, s, c->d, st);
     else if (c->dim == 2)
@@ -226,12 +228,15 @@ void launch_merge_window(rtod_ctx *c, hipStream_t s, const Step &st)
 template <int DIM> void launch_classify_dim(rtod_ctx *c, hipStream_t s, const Step &st)
 {
     const int grid = (c->W + 255) / 256;
-    if (c->learned)
-        hipLaunchKernelGGL((k_classify<DIM, true>), dim3(grid), dim3(kClassifyNT), 0, s, c->d,
-                           st);
+    if (c->lbvh) // the LBVH index is built over the per-point candidate list
+        hipLaunchKernelGGL((k_classify<DIM, kClsPoints>), dim3(grid), dim3(kClassifyNT), 0, s,
+                           c->d, st);
+    else if (c->learned)
+        hipLaunchKernelGGL((k_classify<DIM, kClsLearned>), dim3(grid), dim3(kClassifyNT), 0, s,
+                           c->d, st);
     else
-        hipLaunchKernelGGL((k_classify<DIM, false>), dim3(grid), dim3(kClassifyNT), 0, s, c->d,
-                           st);
+        hipLaunchKernelGGL((k_classify<DIM, kClsUnits>), dim3(grid), dim3(kClassifyNT), 0, s,
+                           c->d, st);
 }
 
 void launch_classify(rtod_ctx *c, hipStream_t s, const Step &st)
@@ -504,6 +509,7 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
     RTOD_TRY(dalloc(c, &d.samples[0], (size_t)d.nsamp));
     RTOD_TRY(dalloc(c, &d.samples[1], (size_t)d.nsamp));
     RTOD_TRY(dalloc(c, &d.cand, W));
+    RTOD_TRY(dalloc(c, &d.units, W));
     RTOD_TRY(dalloc(c, &d.cand_cnt, W));
     RTOD_TRY(dalloc(c, &d.bitmap, (W + 31) / 32));
     RTOD_TRY(dalloc(c, &d.ctl, 1));
@@ -535,6 +541,7 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
         Learn &lr = d.lr;
         lr.miss_thr = cfg->miss_threshold > 0.f ? cfg->miss_threshold : 0.25f;
         RTOD_TRY(dalloc(c, &lr.params, (size_t)kMlpNP));
+        RTOD_TRY(dalloc(c, &lr.pub, (size_t)2 * kMlpNP));
         RTOD_TRY(dalloc(c, &lr.adam_m, (size_t)kMlpNP));
         RTOD_TRY(dalloc(c, &lr.adam_v, (size_t)kMlpNP));
         RTOD_TRY(dalloc(c, &lr.rep_x, (size_t)kReplaySlides * W * kFeatStride));
@@ -546,6 +553,11 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
         std::vector<float> p0 = initial_predictor();
         RTOD_HIP(hipMemcpyAsync(lr.params, p0.data(), sizeof(float) * kMlpNP,
                                 hipMemcpyHostToDevice, c->stream));
+        for (int h = 0; h < 2; ++h)
+            RTOD_HIP(hipMemcpyAsync(lr.pub + (size_t)h * kMlpNP, p0.data(),
+                                    sizeof(float) * kMlpNP, hipMemcpyHostToDevice, c->stream));
+        RTOD_HIP(hipStreamCreateWithFlags(&c->train_stream, hipStreamNonBlocking));
+        RTOD_HIP(hipEventCreateWithFlags(&c->train_ev, hipEventDisableTiming));
         RTOD_HIP(hipMemsetAsync(lr.adam_m, 0, sizeof(float) * kMlpNP, c->stream));
         RTOD_HIP(hipMemsetAsync(lr.adam_v, 0, sizeof(float) * kMlpNP, c->stream));
     }
@@ -570,6 +582,10 @@ rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n,
         return RTOD_ERR_INVALID_ARGUMENT;
     }
     RTOD_HIP(hipSetDevice(c->device));
+    if (c->train_stream) { // the replay is about to be cleared
+        RTOD_HIP(hipStreamSynchronize(c->train_stream));
+        c->train_pending = false;
+    }
     hipStream_t s = c->stream;
     Step st{};
     RTOD_HIP(hipMemcpyAsync(c->d_prime, pts, sizeof(float) * (size_t)c->W * c->dim, kind, s));
@@ -657,13 +673,13 @@ rtod_status collect(rtod_ctx *c, int32_t *outs, int32_t *n_out)
     c->out_total += n;
     if (c->learned) {
         c->misses_total += ho->misses;
-        c->pred_points += ho->n_cand;
+        c->pred_points += ho->n_units;
         c->used_pred_last = ho->used_pred != 0;
         if (!ho->used_pred)
             c->fallbacks++;
-        c->train_steps += ho->train_steps;
-        if (ho->train_steps > 0)
+        if (ho->train_steps > c->train_steps)
             c->train_loss_last = ho->train_loss;
+        c->train_steps = ho->train_steps;
     }
     return RTOD_OK;
 }
@@ -701,6 +717,31 @@ void accumulate_kernel_times(rtod_ctx *c, double *phase)
     }
 }
 
+// Learned schedule: one training launch on the trainer's stream after a
+// slide, unless the previous one is still running.  It overlaps the next
+// slide's kernels; the weights it publishes are picked up by whichever
+// classify starts after it finishes.
+rtod_status maybe_train(rtod_ctx *c)
+{
+    if (!c->learned)
+        return RTOD_OK;
+    if (c->train_pending) {
+        const hipError_t q = hipEventQuery(c->train_ev);
+        if (q == hipErrorNotReady) {
+            (void)hipGetLastError(); // not an error: still running
+            return RTOD_OK;
+        }
+        RTOD_HIP(q);
+        c->train_pending = false;
+    }
+    hipLaunchKernelGGL(k_train, dim3(1), dim3(kTrainNT), 0, c->train_stream, c->d,
+                       (uint64_t)c->host_t, kTrainSteps);
+    RTOD_HIP(hipGetLastError());
+    RTOD_HIP(hipEventRecord(c->train_ev, c->train_stream));
+    c->train_pending = true;
+    return RTOD_OK;
+}
+
 // One slide whose new points are already where the ingest kernel reads them
 // (pinned staging, or c->dev_src in HBM).
 rtod_status run_slide(rtod_ctx *c, double t_entry, double t_h2d, int32_t *outs, int32_t *n_out,
@@ -716,6 +757,7 @@ rtod_status run_slide(rtod_ctx *c, double t_entry, double t_h2d, int32_t *outs, 
     const double t_dev = now_ms();
     RTOD_TRY(collect(c, outs, n_out));
     const double t_end = now_ms();
+    RTOD_TRY(maybe_train(c));
     if (times) {
         times->copy_h2d = t_h2d - t_entry;
         times->prepare_cell = phase[0];
@@ -942,6 +984,8 @@ RTOD_API rtod_status rtod_copy_outlier(rtod_ctx *c, int32_t *outs, int32_t *n_ou
     rtod_status s = collect(c, outs, n_out);
     c->phase = 0;
     c->staged = false;
+    if (s == RTOD_OK)
+        s = maybe_train(c);
     return s;
 }
 
@@ -986,6 +1030,8 @@ RTOD_API rtod_status rtod_get_predictor(rtod_ctx *c, float *params, int32_t n)
         return RTOD_ERR_INVALID_ARGUMENT;
     }
     RTOD_HIP(hipSetDevice(c->device));
+    RTOD_HIP(hipStreamSynchronize(c->train_stream));
+    c->train_pending = false;
     RTOD_HIP(hipMemcpyAsync(params, c->d.lr.params, sizeof(float) * kMlpNP,
                             hipMemcpyDeviceToHost, c->stream));
     RTOD_HIP(hipStreamSynchronize(c->stream));
@@ -1012,9 +1058,14 @@ RTOD_API rtod_status rtod_set_predictor(rtod_ctx *c, const float *params, int32_
         return RTOD_ERR_STATE;
     }
     RTOD_HIP(hipSetDevice(c->device));
+    RTOD_HIP(hipStreamSynchronize(c->train_stream));
+    c->train_pending = false;
     const Learn &lr = c->d.lr;
     RTOD_HIP(hipMemcpyAsync(lr.params, params, sizeof(float) * kMlpNP, hipMemcpyHostToDevice,
                             c->stream));
+    for (int h = 0; h < 2; ++h)
+        RTOD_HIP(hipMemcpyAsync(lr.pub + (size_t)h * kMlpNP, params, sizeof(float) * kMlpNP,
+                                hipMemcpyHostToDevice, c->stream));
     RTOD_HIP(hipMemsetAsync(lr.adam_m, 0, sizeof(float) * kMlpNP, c->stream));
     RTOD_HIP(hipMemsetAsync(lr.adam_v, 0, sizeof(float) * kMlpNP, c->stream));
     const int32_t kept[2] = {0, frozen ? 1 : 0}; // adam_t, frozen
@@ -1135,6 +1186,12 @@ RTOD_API void rtod_destroy(rtod_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream)
         (void)hipStreamSynchronize(c->stream);
+    if (c->train_stream) {
+        (void)hipStreamSynchronize(c->train_stream);
+        (void)hipStreamDestroy(c->train_stream);
+    }
+    if (c->train_ev)
+        (void)hipEventDestroy(c->train_ev);
     for (auto &e : c->ev)
         if (e)
             (void)hipEventDestroy(e);

@@ -20,6 +20,8 @@
 // D2H + select (optixScan.cpp:1083-1101).
 #pragma once
 
+#include <climits>
+
 #include "rtod_device.h"
 #include "rtod_learned.h"
 #include "rtod_state.h"
@@ -502,15 +504,7 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
 
     const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
     const int W = d.W, S = d.S;
-    // a learned context appends one training workgroup to the merge grid
-    const int nmb = (W + T - 1) / T;
-    static_assert(4 * sizeof(TrainWaveLds) <= sizeof(lds.as), "training LDS");
-    static_assert(sizeof(TrainLds) <= sizeof(lds.br), "training LDS");
-    if (bid >= nmb) {
-        if (d.lr.on)
-            train_block(d, nmb, s_as, s_br);
-        return;
-    }
+    const int nmb = (W + T - 1) / T; // merge workgroups
     const int par = st.par;
     const uint64_t *Ak = d.akey[par];
     const int32_t *As = d.aslot[par];
@@ -542,8 +536,9 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
 
     if (bid == 0 && tid == 0) {
         d.ctl->n_cand = 0;
+        d.ctl->n_units = 0;
         if (d.lr.on)
-            d.lr.lc->n_cand_i = 0;
+            d.lr.lc->n_units_i = 0;
     }
 
     // round 1: the chunk (keys, slots, batch positions, coordinates; striped,
@@ -719,11 +714,6 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
         if ((np & smask) == 0)
             Nsamp[np >> sl2] = key;
     }
-    if (d.lr.on) { // tells the training workgroup when to stop stepping
-        __syncthreads();
-        if (tid == 0)
-            atomicAdd(&d.lr.lc->mw_done, 1);
-    }
 }
 
 template <int DIM>
@@ -738,39 +728,50 @@ __global__ __launch_bounds__(256) void k_merge_window(Dev d, Step st)
 // >= K+1 points (reference: cells with num > K are skipped, optixScan.cpp:
 // 987-993).  The cell is a run of equal keys in the sorted window: find the
 // run start s within [i-K, i] and test key[s+K] == key.  The keys around the
-// tile sit in LDS when K is small.  Other points are appended to the
-// candidate list, one atomic per wavefront.
-// LEARN: each wavefront holding candidates also computes their features
-// (own-cell count, the two neighbouring cells' counts along the last axis,
-// offsets from the cell centre), runs the predictor on the matrix cores, and
-// files predicted outliers at the front of the list (ctl->n_cand) and
-// predicted inliers at the back (index W-1-k, lc->n_cand_i); the features go
-// to the replay slot of this slide under the same list index.
+// tile sit in LDS when K is small.  The other points -- all points of the
+// sparse cells -- are the candidates.
+//   kClsPoints  appends every candidate to the per-point list (LBVH index).
+//   kClsUnits   appends cell units: a candidate starts a unit where its cell
+//               run starts or at a multiple of kUnitMax; the unit runs to the
+//               end of the cell or of that aligned block, so a unit is at
+//               most kUnitMax candidates sharing one cell.  One atomic per
+//               wavefront.
+//   kClsLearned kClsUnits plus, per unit, the predictor's features (own-cell
+//               count, the two neighbouring cells' counts along the last
+//               axis, the unit's first point's offsets from the cell centre)
+//               and its prediction on the matrix cores: probable-outlier
+//               units are filed at the front of the list (ctl->n_units),
+//               probable-inlier units at the back (index W-1-k,
+//               lc->n_units_i); features go to the slide's replay slot under
+//               the unit's list index.
 // ---------------------------------------------------------------------------
 constexpr int kClassifyNT = 256;
 constexpr int kClassifyHalo = 512;
+enum ClassifyMode { kClsPoints = 0, kClsUnits = 1, kClsLearned = 2 };
+constexpr int kUnitMax = 16; // candidates per cell unit (a power of two)
 
-template <bool LEARN> struct ClassifyLds {
+template <bool MLP> struct ClassifyLds {
     static constexpr int NW = kClassifyNT / kWave;
     uint64_t t[kClassifyNT + 2 * kClassifyHalo];
-    alignas(16) __bf16 h[LEARN ? NW : 1][16 * kMlpH];
-    float z[LEARN ? NW : 1][kWave];
+    alignas(16) __bf16 h[MLP ? NW : 1][MLP ? kWave * kMlpH : 1];
+    float x[MLP ? NW : 1][MLP ? kWave : 1][kFeatStride];
+    float z[MLP ? NW : 1][kWave];
 };
 
-template <int DIM, bool LEARN>
+template <int DIM, int MODE>
 __device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int bid,
-                                              ClassifyLds<LEARN> &lds)
+                                              ClassifyLds<MODE == kClsLearned> &lds)
 {
     constexpr int NT = kClassifyNT, HALO = kClassifyHalo;
+    constexpr bool LEARN = MODE == kClsLearned;
     uint64_t *s_t = lds.t;
-    auto &s_h = lds.h;
-    auto &s_z = lds.z;
     const int i0 = bid * NT;
     const int i = i0 + threadIdx.x;
     const uint64_t *Nk = d.akey[st.par ^ 1];
     const int W = d.W, K = d.K;
-    // keys read: [i-K, i] for the run start; learned features reach 2K+2
-    const int H = LEARN ? 2 * K + 2 : K;
+    // keys read: [i-K, i] for the run start, [i, i+63] for a unit's end;
+    // learned features reach 2K+2 either side
+    const int H = LEARN ? max(2 * K + 2, kUnitMax) : max(K, kUnitMax);
     const bool tiled = H <= HALO;
     const int t0 = max(0, i0 - H), t1 = min(W, i0 + NT + H);
     if (tiled)
@@ -813,7 +814,8 @@ __device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int 
         cand = !dense;
     }
     const int lane = lane_id();
-    if constexpr (!LEARN) {
+    const uint64_t lt = (1ull << lane) - 1ull;
+    if constexpr (MODE == kClsPoints) {
         const uint64_t mask = __ballot(cand);
         const int cnt = __popcll(mask);
         if (cnt) {
@@ -821,64 +823,95 @@ __device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int 
             if (lane == 0)
                 base = atomicAdd(&d.ctl->n_cand, cnt);
             base = __shfl(base, 0, kWave);
-            if (cand) {
-                const uint64_t below = mask & ((1ull << lane) - 1ull);
-                d.cand[base + __popcll(below)] = i;
-            }
+            if (cand)
+                d.cand[base + __popcll(mask & lt)] = i;
         }
+        return;
     } else {
-        if (__ballot(cand) == 0)
+        const uint64_t cmask = __ballot(cand);
+        if (cmask == 0)
             return;
-        const int wid = threadIdx.x / kWave;
-        constexpr uint64_t FM = CellBits<DIM>::FMASK;
-        const float capf = 1.f / (float)(K + 1);
-        float f[kMlpIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        if (cand) {
-            const int a = run_lo(i, key, K), b = run_hi(i, key, K);
-            f[0] = (float)min(b - a + 1, K + 1) * capf;
-            const uint64_t last = key & FM;
-            if (last > 0 && a > 0 && kat(a - 1) == key - 1)
-                f[1] = (float)min(a - run_lo(a - 1, key - 1, K), K + 1) * capf;
-            if (last < FM && b + 1 < W && kat(b + 1) == key + 1)
-                f[2] = (float)min(run_hi(b + 1, key + 1, K) - b, K + 1) * capf;
-            const float *Nx = d.axyz[st.par ^ 1];
+        if (lane == 0)
+            atomicAdd(&d.ctl->n_cand, __popcll(cmask));
+        // unit starts and lengths
+        const bool ustart = cand && (i == 0 || (i & (kUnitMax - 1)) == 0 || kat(i - 1) != key);
+        int ulen = 0;
+        if (ustart) {
+            const int end = min(W, (i | (kUnitMax - 1)) + 1); // end of the aligned block
+            ulen = run_hi(i, key, end - 1 - i) - i + 1;
+        }
+        if constexpr (MODE == kClsUnits) {
+            const uint64_t m = __ballot(ustart);
+            int base = 0;
+            if (lane == 0 && m)
+                base = atomicAdd(&d.ctl->n_units, __popcll(m));
+            base = __shfl(base, 0, kWave);
+            if (ustart)
+                d.units[base + __popcll(m & lt)] = make_int2(i, ulen);
+        } else {
+            const int wid = threadIdx.x / kWave;
+            constexpr uint64_t FM = CellBits<DIM>::FMASK;
+            const float capf = 1.f / (float)(K + 1);
+            float f[kMlpIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            if (ustart) {
+                const int a = run_lo(i, key, K), b = run_hi(i, key, K);
+                f[0] = (float)min(b - a + 1, K + 1) * capf;
+                const uint64_t last = key & FM;
+                if (last > 0 && a > 0 && kat(a - 1) == key - 1)
+                    f[1] = (float)min(a - run_lo(a - 1, key - 1, K), K + 1) * capf;
+                if (last < FM && b + 1 < W && kat(b + 1) == key + 1)
+                    f[2] = (float)min(run_hi(b + 1, key + 1, K) - b, K + 1) * capf;
+                const float *Nx = d.axyz[st.par ^ 1];
 #pragma unroll
-            for (int ax = 0; ax < DIM; ++ax) {
-                const double u = (double)Nx[(size_t)ax * W + i] * d.inv_c;
-                const float off = (float)(u - floor(u)) - 0.5f;
-                f[3 + ax] = (off >= -0.5f && off <= 0.5f) ? off : 0.f; // non-finite -> 0
+                for (int ax = 0; ax < DIM; ++ax) {
+                    const double u = (double)Nx[(size_t)ax * W + i] * d.inv_c;
+                    const float off = (float)(u - floor(u)) - 0.5f;
+                    f[3 + ax] = (off >= -0.5f && off <= 0.5f) ? off : 0.f; // non-finite -> 0
+                }
             }
-        }
-        const float yh = mlp_infer_wave(d.lr.params, f, s_h[wid], s_z[wid]);
-        const bool pin = cand && yh >= kInlierCut;
-        const bool pout = cand && !pin;
-        const uint64_t mo = __ballot(pout), mi = __ballot(pin);
-        int bo = 0, bi = 0;
-        if (lane == 0) {
-            if (mo)
-                bo = atomicAdd(&d.ctl->n_cand, __popcll(mo));
-            if (mi)
-                bi = atomicAdd(&d.lr.lc->n_cand_i, __popcll(mi));
-        }
-        bo = __shfl(bo, 0, kWave);
-        bi = __shfl(bi, 0, kWave);
-        if (cand) {
-            const uint64_t lt = (1ull << lane) - 1ull;
-            const int li = pout ? bo + __popcll(mo & lt) : W - 1 - (bi + __popcll(mi & lt));
-            d.cand[li] = i;
-            float *rx = d.lr.rep_x +
-                        ((size_t)(st.t % kReplaySlides) * W + li) * kFeatStride;
-            *reinterpret_cast<float4 *>(rx) = make_float4(f[0], f[1], f[2], f[3]);
-            *reinterpret_cast<float4 *>(rx + 4) = make_float4(f[4], f[5], 0.f, 0.f);
+            // the wave's unit starts, compacted into rows 0..m-1
+            const uint64_t sm = __ballot(ustart);
+            const int m = __popcll(sm);
+            const int row = __popcll(sm & lt);
+            if (ustart) {
+#pragma unroll
+                for (int j = 0; j < kFeatStride; ++j)
+                    lds.x[wid][row][j] = j < kMlpIn ? f[j < kMlpIn ? j : 0] : 0.f;
+            }
+            wave_lds_fence();
+            const float *P = current_weights(d.lr);
+            mlp_infer_rows(P, lds.x[wid], m, lds.h[wid], lds.z[wid]);
+            const float yh = ustart ? sigmoidf(lds.z[wid][row] + P[kOffB3]) : 0.f;
+            wave_lds_fence();
+            const bool pin = ustart && yh >= kInlierCut;
+            const bool pout = ustart && !pin;
+            const uint64_t mo = __ballot(pout), mi = __ballot(pin);
+            int bo = 0, bi = 0;
+            if (lane == 0) {
+                if (mo)
+                    bo = atomicAdd(&d.ctl->n_units, __popcll(mo));
+                if (mi)
+                    bi = atomicAdd(&d.lr.lc->n_units_i, __popcll(mi));
+            }
+            bo = __shfl(bo, 0, kWave);
+            bi = __shfl(bi, 0, kWave);
+            if (ustart) {
+                const int li = pout ? bo + __popcll(mo & lt) : W - 1 - (bi + __popcll(mi & lt));
+                d.units[li] = make_int2(i, ulen);
+                float *rx = d.lr.rep_x +
+                            ((size_t)(st.t % kReplaySlides) * W + li) * kFeatStride;
+                *reinterpret_cast<float4 *>(rx) = make_float4(f[0], f[1], f[2], f[3]);
+                *reinterpret_cast<float4 *>(rx + 4) = make_float4(f[4], f[5], 0.f, 0.f);
+            }
         }
     }
 }
 
-template <int DIM, bool LEARN>
+template <int DIM, int MODE>
 __global__ __launch_bounds__(kClassifyNT) void k_classify(Dev d, Step st)
 {
-    __shared__ ClassifyLds<LEARN> lds;
-    classify_body<DIM, LEARN>(d, st, blockIdx.x, lds);
+    __shared__ ClassifyLds<MODE == kClsLearned> lds;
+    classify_body<DIM, MODE>(d, st, blockIdx.x, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1021,19 +1054,43 @@ __device__ __forceinline__ int row_bounds(const Dev &d, const uint64_t *Nk,
     return res;
 }
 
-// Exact count of the window points q in the cells of stencil `st` with
-// d2(c, q) <= r2, capped: stops as soon as the count exceeds K.
-// Wave-cooperative (all 64 lanes call it with the same arguments).
-// tests += predicate evaluations.
+__device__ __forceinline__ int wave_min_i(int x)
+{
+#pragma unroll
+    for (int m = 1; m < kWave; m <<= 1)
+        x = min(x, __shfl_xor(x, m, kWave));
+    return x;
+}
+
+__device__ __forceinline__ int wave_max_i(int x)
+{
+#pragma unroll
+    for (int m = 1; m < kWave; m <<= 1)
+        x = max(x, __shfl_xor(x, m, kWave));
+    return x;
+}
+
+__device__ __forceinline__ float lane_f(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// Counts of a cell unit's candidates (<= kUnitMax) over the window points in the cells of
+// stencil `st` (a superset of each candidate's own stencil; the predicate
+// decides).  Lane c < n holds candidate c's coordinates in pc and receives
+// its count, exact up to K+1 at least.  Chunks of 64 stencil points stream
+// through the lanes; each chunk is tested against every candidate (one
+// ballot + popcount per candidate) and the scan stops once every
+// candidate's count exceeds K.  Wave-cooperative; tests += evaluations.
 template <int DIM>
-__device__ __forceinline__ int range_count_st(const Dev &d, const uint64_t *Nk, const float *Nx,
-                                              const uint64_t *s_samp, int *w_a, int *w_b,
-                                              const float (&c)[3], Stencil<DIM> st,
-                                              long long &tests)
+__device__ __forceinline__ int unit_count(const Dev &d, const uint64_t *Nk, const float *Nx,
+                                          const uint64_t *s_samp, int *w_a, int *w_b,
+                                          const float (&pc)[3], int n, Stencil<DIM> st,
+                                          long long &tests)
 {
     const int lane = lane_id();
     const int W = d.W, K = d.K;
-    if (st.nrows > 32) { // cannot happen: <= 5 cells per axis
+    if (st.nrows > 32) { // cannot happen: the stencils stay within 32 rows
         if (lane == 0)
             atomicOr(&d.ctl->err, 4);
         st.nrows = 32;
@@ -1047,8 +1104,6 @@ __device__ __forceinline__ int range_count_st(const Dev &d, const uint64_t *Nk, 
     const int len = (r < st.nrows) ? (hi_b - lo_b) : 0;
     const int incl = wave_incl_scan(len);
     const int total = __shfl(incl, 63, kWave);
-    if (total <= K) // even counting every stencil point it stays <= K
-        return total;
     wave_sync_lds();
     if (r < st.nrows) {
         w_a[r] = lo_b;
@@ -1059,8 +1114,9 @@ __device__ __forceinline__ int range_count_st(const Dev &d, const uint64_t *Nk, 
     int cnt = 0;
     for (int base = 0; base < total; base += kWave) {
         const int idx = base + lane;
-        bool hit = false;
-        if (idx < total) {
+        float qx = 0.f, qy = 0.f, qz = 0.f;
+        const bool have = idx < total;
+        if (have) {
             int j;
             if constexpr (DIM == 1) {
                 j = w_a[0] + idx;
@@ -1075,60 +1131,70 @@ __device__ __forceinline__ int range_count_st(const Dev &d, const uint64_t *Nk, 
                 }
                 j = w_a[lo] + (idx - w_b[lo]);
             }
-            const float qx = Nx[j];
-            const float qy = DIM > 1 ? Nx[(size_t)W + j] : 0.f;
-            const float qz = DIM > 2 ? Nx[(size_t)2 * W + j] : 0.f;
-            hit = dist2<DIM>(c, qx, qy, qz) <= d.r2;
+            qx = Nx[j];
+            if constexpr (DIM > 1)
+                qy = Nx[(size_t)W + j];
+            if constexpr (DIM > 2)
+                qz = Nx[(size_t)2 * W + j];
         }
-        cnt += __popcll(__ballot(hit));
-        tests += min(kWave, total - base);
-        if (cnt > K)
+        for (int c = 0; c < n; ++c) {
+            const float p3[3] = {lane_f(pc[0], c), DIM > 1 ? lane_f(pc[1], c) : 0.f,
+                                 DIM > 2 ? lane_f(pc[2], c) : 0.f};
+            const bool hit = have && dist2<DIM>(p3, qx, qy, qz) <= d.r2;
+            const int h = __popcll(__ballot(hit));
+            cnt += (lane == c) ? h : 0;
+        }
+        tests += (long long)min(kWave, total - base) * n;
+        if (__ballot(lane < n && cnt <= K) == 0)
             break;
     }
     return cnt;
 }
 
+// Stencil covering every candidate of a unit: per axis the union of
+// [floor((x - R') / c), floor((x + R') / c)] over lanes < n.
 template <int DIM>
-__device__ __forceinline__ void load_point(const float *Nx, int W, int p, float (&c)[3])
+__device__ __forceinline__ Stencil<DIM> unit_stencil(const Dev &d, const float (&pc)[3], int n)
 {
-    c[0] = c[1] = c[2] = 0.f;
-#pragma unroll
-    for (int a = 0; a < DIM; ++a)
-        c[a] = Nx[(size_t)a * W + p];
-}
-
-// Capped neighbour count of the candidate at sorted position p over its full
-// stencil.
-template <int DIM>
-__device__ __forceinline__ int range_count(const Dev &d, const uint64_t *Nk, const float *Nx,
-                                           const uint64_t *s_samp, int *w_a, int *w_b, int p,
-                                           long long &tests)
-{
-    float c[3];
-    load_point<DIM>(Nx, d.W, p, c);
+    const int lane = lane_id();
     Stencil<DIM> st;
-    make_stencil<DIM>(c, d.inv_c, d.Rp, st);
-    return range_count_st<DIM>(d, Nk, Nx, s_samp, w_a, w_b, c, st, tests);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        st.lo[a] = 0;
+        st.hi[a] = 0;
+    }
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+        int lo = INT_MAX, hi = INT_MIN;
+        if (lane < n) {
+            lo = cell_coord<DIM>((double)pc[a] - d.Rp, d.inv_c);
+            hi = cell_coord<DIM>((double)pc[a] + d.Rp, d.inv_c);
+        }
+        st.lo[a] = wave_min_i(lo);
+        st.hi[a] = wave_max_i(hi);
+    }
+    if constexpr (DIM == 1) {
+        st.nrows = 1;
+        st.n1 = 1;
+    } else if constexpr (DIM == 2) {
+        st.nrows = st.hi[0] - st.lo[0] + 1;
+        st.n1 = 1;
+    } else {
+        st.n1 = st.hi[1] - st.lo[1] + 1;
+        st.nrows = (st.hi[0] - st.lo[0] + 1) * st.n1;
+    }
+    return st;
 }
 
-// Near-first count for a probable inlier (2-D and 3-D): the stencil cut to the
-// 3^d cells around the candidate's own cell first; only if that subset holds
-// <= K neighbours is the full stencil counted.  A subset count > K proves
-// count > K, so the capped result equals range_count's.
+// The unit's own cell and its neighbours (own +- 1 per axis) within `full`.
 template <int DIM>
-__device__ __forceinline__ int range_count_near_first(const Dev &d, const uint64_t *Nk,
-                                                      const float *Nx, const uint64_t *s_samp,
-                                                      int *w_a, int *w_b, int p,
-                                                      long long &tests)
+__device__ __forceinline__ Stencil<DIM> near_stencil(const Dev &d, const Stencil<DIM> &full,
+                                                     const float (&pc)[3])
 {
-    float c[3];
-    load_point<DIM>(Nx, d.W, p, c);
-    Stencil<DIM> full;
-    make_stencil<DIM>(c, d.inv_c, d.Rp, full);
     Stencil<DIM> nr = full;
 #pragma unroll
     for (int a = 0; a < DIM; ++a) {
-        const int own = cell_coord<DIM>((double)c[a], d.inv_c);
+        const int own = __builtin_amdgcn_readfirstlane(cell_coord<DIM>((double)pc[a], d.inv_c));
         nr.lo[a] = max(full.lo[a], own - 1);
         nr.hi[a] = min(full.hi[a], own + 1);
     }
@@ -1138,11 +1204,7 @@ __device__ __forceinline__ int range_count_near_first(const Dev &d, const uint64
         nr.n1 = nr.hi[1] - nr.lo[1] + 1;
         nr.nrows = (nr.hi[0] - nr.lo[0] + 1) * nr.n1;
     }
-    const int cnt = range_count_st<DIM>(d, Nk, Nx, s_samp, w_a, w_b, c, nr, tests);
-    if (cnt > d.K)
-        return cnt;
-    wave_sync_lds();
-    return range_count_st<DIM>(d, Nk, Nx, s_samp, w_a, w_b, c, full, tests);
+    return nr;
 }
 
 // ---------------------------------------------------------------------------
@@ -1224,16 +1286,18 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int n
 }
 
 // ---------------------------------------------------------------------------
-// k_detect: one wavefront per candidate (grid-stride over the candidate
-// list).  Only ceil(n_cand / 4) workgroups take part (the rest leave at
-// once); the last of them to finish compacts the outlier bitmap into an
+// k_detect: one wavefront per cell unit (grid-stride over the unit list):
+// the unit's candidates are counted together over their union stencil
+// (unit_count).  Only ceil(n_units / 4) workgroups take part (the rest leave
+// at once); the last of them to finish compacts the outlier bitmap into an
 // ascending slot list written straight into pinned host memory.
-// LEARN: the list is the predicted-outlier bucket [0, nO) followed by the
-// predicted-inlier bucket (indices W-1, W-2, ...): the long full counts are
-// handed out first.  When the schedule is on (lc->use_pred), predicted
-// inliers take the near-first count (2-D/3-D).  Every count is exact; each
-// is stored in the replay, and mispredictions are counted for the fallback
-// rule.
+// LEARN: the list is the probable-outlier units [0, nO) followed by the
+// probable-inlier units (indices W-1, W-2, ...): the long full counts are
+// handed out first.  When the schedule is on (lc->use_pred), a probable-
+// inlier unit is first counted over its own cell and the neighbouring
+// cells only; the full stencil is counted only if some candidate is still
+// at <= K.  Every count is exact; each unit's smallest count goes to the
+// replay, and mispredicted units are counted for the fallback rule.
 // ---------------------------------------------------------------------------
 struct DetectLds {
     static constexpr int NW = kDetectNT / kWave;
@@ -1265,10 +1329,10 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
     const int32_t *Ns = d.aslot[par];
     const float *Nx = d.axyz[par];
     const int W = d.W, K = d.K;
-    const int n_o = d.ctl->n_cand;
-    const int n_cand = LEARN ? n_o + d.lr.lc->n_cand_i : n_o;
+    const int n_o = d.ctl->n_units;
+    const int n_units = LEARN ? n_o + d.lr.lc->n_units_i : n_o;
     const bool use_pred = LEARN && d.lr.lc->use_pred != 0;
-    const int busy = min(nblk, max(1, (n_cand + NW - 1) / NW));
+    const int busy = min(nblk, max(1, (n_units + NW - 1) / NW));
     if (bid >= busy)
         return;
 
@@ -1281,28 +1345,45 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
 
     long long tests = 0;
     int misses = 0;
-    for (int ci = bid * NW + wid; ci < n_cand; ci += busy * NW) {
-        const int li = (!LEARN || ci < n_o) ? ci : W - 1 - (ci - n_o);
-        const int p = __builtin_amdgcn_readfirstlane(d.cand[li]);
+    for (int u = bid * NW + wid; u < n_units; u += busy * NW) {
+        const int li = (!LEARN || u < n_o) ? u : W - 1 - (u - n_o);
+        const int2 un = d.units[li];
+        const int start = __builtin_amdgcn_readfirstlane(un.x);
+        const int n = __builtin_amdgcn_readfirstlane(un.y);
+        float pc[3] = {0.f, 0.f, 0.f};
+        if (lane < n) {
+#pragma unroll
+            for (int a = 0; a < DIM; ++a)
+                pc[a] = Nx[(size_t)a * W + start + lane];
+        }
+        const Stencil<DIM> full = unit_stencil<DIM>(d, pc, n);
         int cnt;
-        if (LEARN && DIM > 1 && use_pred && ci >= n_o)
-            cnt = range_count_near_first<DIM>(d, Nk, Nx, s_samp, s_rows[wid][0], s_rows[wid][1],
-                                              p, tests);
-        else
-            cnt = range_count<DIM>(d, Nk, Nx, s_samp, s_rows[wid][0], s_rows[wid][1], p,
-                                   tests);
-        if (lane == 0) {
-            const int capped = min(cnt, K + 1);
-            d.cand_cnt[ci] = capped;
-            if (cnt <= K) {
-                const int slot = Ns[p];
-                atomicOr(&d.bitmap[slot >> 5], 1u << (slot & 31));
+        if (LEARN && use_pred && u >= n_o) {
+            cnt = unit_count<DIM>(d, Nk, Nx, s_samp, s_rows[wid][0], s_rows[wid][1], pc, n,
+                                  near_stencil<DIM>(d, full, pc), tests);
+            if (__ballot(lane < n && cnt <= K) != 0) {
+                wave_sync_lds();
+                cnt = unit_count<DIM>(d, Nk, Nx, s_samp, s_rows[wid][0], s_rows[wid][1], pc,
+                                      n, full, tests);
             }
+        } else {
+            cnt = unit_count<DIM>(d, Nk, Nx, s_samp, s_rows[wid][0], s_rows[wid][1], pc, n,
+                                  full, tests);
+        }
+        const int capped = lane < n ? min(cnt, K + 1) : K + 1;
+        if (lane < n && cnt <= K) {
+            const int slot = Ns[start + lane];
+            atomicOr(&d.bitmap[slot >> 5], 1u << (slot & 31));
+        }
+        const int umin = wave_min_i(capped);
+        if (lane == 0) {
+            d.cand_cnt[u] = umin;
             if constexpr (LEARN) {
-                d.lr.rep_c[(size_t)(st.t % kReplaySlides) * W + li] = capped;
-                misses += ((ci >= n_o) != (cnt > K)) ? 1 : 0;
+                d.lr.rep_c[(size_t)(st.t % kReplaySlides) * W + li] = umin;
+                misses += ((u >= n_o) != (umin > K)) ? 1 : 0;
             }
         }
+        wave_sync_lds();
     }
     if (lane == 0 && tests)
         atomicAdd(&s_tests, (unsigned long long)tests);
@@ -1326,26 +1407,25 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
         if (tid == 0) {
             LearnCtl *lc = d.lr.lc;
             const int slot = (int)(st.t % kReplaySlides);
-            const int n_i = n_cand - n_o;
             const int miss =
                 __hip_atomic_load(&lc->misses, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             d.lr.rep_n[2 * slot] = n_o;
-            d.lr.rep_n[2 * slot + 1] = n_i;
+            d.lr.rep_n[2 * slot + 1] = n_units - n_o;
             HostOut *ho = d.h_out;
             ho->misses = miss;
             ho->used_pred = use_pred ? 1 : 0;
-            ho->train_steps = lc->steps_last;
+            ho->train_steps = lc->steps_total;
             ho->train_loss = lc->loss_last;
             // fallback rule: schedule the next slide by prediction only if
-            // this slide's miss rate is within the threshold
-            if (n_cand > 0)
-                lc->use_pred = ((float)miss <= d.lr.miss_thr * (float)n_cand) ? 1 : 0;
+            // this slide's miss rate (over units) is within the threshold
+            if (n_units > 0)
+                lc->use_pred = ((float)miss <= d.lr.miss_thr * (float)n_units) ? 1 : 0;
             lc->misses = 0;
-            lc->mw_done = 0;
-            lc->steps_last = 0;
         }
     }
-    finish_slide(d, st, n_cand, s_scan);
+    if (tid == 0)
+        d.h_out->n_units = n_units;
+    finish_slide(d, st, d.ctl->n_cand, s_scan);
 }
 
 template <int DIM, bool LEARN>

@@ -14,14 +14,19 @@
 //   bsamp [(m+1)*nbs]           every 2^bs_log2-th key of each sorted batch
 //   tkey/tslot/txyz [2]         scratch runs for chunked sorts (max(W,S))
 //   samples  u64 [2][W/stride]  every stride-th key of each sorted window
-//   cand     i32 [W]            candidate positions (sorted-window index)
+//   cand     i32 [W]            candidate positions (sorted-window index; the
+//                               LBVH index's per-point list)
+//   units    i32x2 [W]          cell units (start, len <= 64): runs of one
+//                               sparse cell's candidates in the sorted window
 //   bitmap   u32 [W/32]         outlier flags by ring slot
 //   ctl                         slide counter, candidate count, grid ticket
 // Learned scheduler (only when configured):
 //   params/adam_m/adam_v f32 [1313]   predictor MLP, FP32 master + Adam moments
-//   rep_x f32 [20][W][8]        replay: features of each candidate, by slide
-//                               (slot t mod 20) and candidate-list index
-//   rep_c i32 [20][W]           replay: its exact neighbour count, capped K+1
+//   pub f32 [2][1313]           published weights (classify reads pub[lc.pub&1])
+//   rep_x f32 [20][W][8]        replay: features of each cell unit, by slide
+//                               (slot t mod 20) and unit-list index
+//   rep_c i32 [20][W]           replay: the unit's smallest exact neighbour
+//                               count, capped at K+1
 //   rep_n i32 [20][2]           replay: bucket sizes of the stored slide
 // Pinned host memory, device-mapped:
 //   h_in  f32 [2][S*d]          new slide staging (row-major, as given)
@@ -38,7 +43,7 @@ struct Ctl {
     uint32_t ticket;       // last-workgroup ticket of the detect kernel
     unsigned long long pair_tests;
     int32_t err;
-    int32_t pad;
+    int32_t n_units;       // cell units of the current slide
 };
 
 struct HostOut {
@@ -49,35 +54,36 @@ struct HostOut {
     int32_t err;
     int32_t misses;        // learned: mispredicted candidates of the slide
     int32_t used_pred;     // learned: the slide was scheduled by prediction
-    int32_t train_steps;   // learned: optimiser steps taken during the slide
+    int32_t train_steps;   // learned: optimiser steps since creation
     float train_loss;      // learned: mean loss of those steps
-    int32_t pad;
+    int32_t n_units;       // cell units of the slide
     int32_t list[1];       // W entries follow
 };
 
 // Device-side state of the learned scheduler.
 struct LearnCtl {
     // cleared by rtod_prime
-    int32_t n_cand_i;      // predicted-inlier bucket size (current slide)
+    int32_t n_units_i;     // predicted-inlier unit bucket size (current slide)
     int32_t use_pred;      // schedule the current slide by prediction
     int32_t misses;        // mispredictions (current slide)
-    int32_t mw_done;       // merge workgroups finished (current slide)
-    int32_t steps_last;    // optimiser steps of the last training block
-    float loss_last;       // their mean loss
+    float loss_last;       // mean loss of the last training workgroup
     // kept across rtod_prime
     int32_t adam_t;        // optimiser steps since the weights were set
     int32_t frozen;        // 1: no training
+    int32_t steps_total;   // optimiser steps since creation
+    int32_t pub;           // the current half of the published weights: pub & 1
 };
-constexpr int kLearnCtlResetBytes = 6 * 4;
+constexpr int kLearnCtlResetBytes = 4 * 4;
 
 struct Learn {
     int on;                // learned schedule configured
     float miss_thr;        // fall back to the plain schedule above this rate
-    float *params;         // [kMlpNP] FP32 master weights
+    float *params;         // [kMlpNP] FP32 master weights (trainer's own)
+    float *pub;            // [2][kMlpNP] published weights, double-buffered
     float *adam_m, *adam_v;
-    float *rep_x;          // [kReplaySlides][W][kFeatStride]
-    int32_t *rep_c;        // [kReplaySlides][W]
-    int32_t *rep_n;        // [kReplaySlides][2]
+    float *rep_x;          // [kReplaySlides][W][kFeatStride] features by unit
+    int32_t *rep_c;        // [kReplaySlides][W] the unit's minimum count, capped
+    int32_t *rep_n;        // [kReplaySlides][2] bucket sizes
     LearnCtl *lc;
 };
 
@@ -117,7 +123,8 @@ struct Dev {
     int tmpn;         // plane stride of txyz
     uint64_t *samples[2];
     int32_t *cand;
-    int32_t *cand_cnt;  // per-candidate counts (capped), for stats/learning
+    int2 *units;        // cell units: {start, len}
+    int32_t *cand_cnt;  // per-unit minimum count (capped)
     uint32_t *bitmap;
     Ctl *ctl;
     const float *h_in;

@@ -113,8 +113,9 @@ def test_learned_exact_c3_vs_grid():
     T = 150
     got, st = _learned_stream(xs, wl.W, wl.S, T, wl.R, wl.K)
     assert_same(got, O.run_stream(xs, wl.W, wl.S, T, wl.R, wl.K, "grid"), "C3 learned vs grid")
-    assert st["train_steps"] >= T - 1      # >= 1 step per slide once replay data exists
-    assert st["predictor_points"] == st["candidates_total"]
+    # >= 1 step per slide once replay data exists (reported up to a slide late)
+    assert st["train_steps"] >= T - 3
+    assert 0 < st["predictor_points"] <= st["candidates_total"]   # cell units
     print(f"C3 learned: misses {st['predictor_misses'] / st['predictor_points']:.3f} "
           f"fallback slides {st['fallbacks']} steps {st['train_steps']} "
           f"loss {st['train_loss_last']:.4f}", flush=True)
