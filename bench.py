@@ -375,8 +375,8 @@ def main() -> int:
 
     others = None
     if not args.no_others and args.workload == "walk1d-W100k-S5k":
-        others = other_workloads(["walk1d-W10k-S500", "gmm3d-W10k-S500"], 120,
-                                 args.cpu_slides, not args.no_cpu)
+        others = other_workloads(["walk1d-W10k-S500", "gmm3d-W10k-S500", "gmm3d-W100k-S5k"],
+                                 120, args.cpu_slides, not args.no_cpu)
 
     indexes = None
     if not args.no_others and args.workload == "walk1d-W100k-S5k":
@@ -390,7 +390,7 @@ def main() -> int:
     if not args.no_learned:
         names = [args.workload]
         if not args.no_others and args.workload == "walk1d-W100k-S5k":
-            names += ["walk1d-W10k-S500", "gmm3d-W10k-S500"]
+            names += ["walk1d-W10k-S500", "gmm3d-W10k-S500", "gmm3d-W100k-S5k"]
         learned = learned_comparison(names, args.train_slides, args.steps)
 
     cpu = None

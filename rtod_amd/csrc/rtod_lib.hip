@@ -85,7 +85,7 @@ struct rtod_ctx {
     rtod_config cfg{};
     int dim = 1, W = 0, S = 0, m = 0, K = 0;
     int sort_shape = 0, CH = 512;
-    int detect_blocks = 512;
+    int detect_blocks = 1024;
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = true; // false: a batch member on the batch's stream
@@ -472,7 +472,7 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
     d.m = c->m;
     d.K = c->K;
     d.cpb = (c->S + c->CH - 1) / c->CH;
-    int sl = 4;
+    int sl = 2; // sample stride >= 4, table <= kMaxSamples entries
     while (((int64_t)c->W + (1ll << sl) - 1) >> sl > kMaxSamples)
         ++sl;
     d.stride_log2 = sl;

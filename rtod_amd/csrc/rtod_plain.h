@@ -749,10 +749,13 @@ constexpr int kClassifyNT = 256;
 constexpr int kClassifyHalo = 512;
 enum ClassifyMode { kClsPoints = 0, kClsUnits = 1, kClsLearned = 2 };
 constexpr int kUnitMax = 16; // candidates per cell unit (a power of two)
+static_assert(kUnitMax == 16, "unit reductions span lanes 0..15");
 
 template <bool MLP> struct ClassifyLds {
     static constexpr int NW = kClassifyNT / kWave;
     uint64_t t[kClassifyNT + 2 * kClassifyHalo];
+    int wc[NW][3]; // per wave: candidates, units (front list), units (back list)
+    int base[3];
     alignas(16) __bf16 h[MLP ? NW : 1][MLP ? kWave * kMlpH : 1];
     float x[MLP ? NW : 1][MLP ? kWave : 1][kFeatStride];
     float z[MLP ? NW : 1][kWave];
@@ -828,76 +831,85 @@ __device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int 
         }
         return;
     } else {
+        // per wave: candidate count, unit starts (and, learned, their
+        // buckets); one atomic per counter per workgroup
+        const int wid = threadIdx.x / kWave;
         const uint64_t cmask = __ballot(cand);
-        if (cmask == 0)
-            return;
-        if (lane == 0)
-            atomicAdd(&d.ctl->n_cand, __popcll(cmask));
-        // unit starts and lengths
-        const bool ustart = cand && (i == 0 || (i & (kUnitMax - 1)) == 0 || kat(i - 1) != key);
+        const bool ustart =
+            cand && (i == 0 || (i & (kUnitMax - 1)) == 0 || kat(i - 1) != key);
         int ulen = 0;
         if (ustart) {
             const int end = min(W, (i | (kUnitMax - 1)) + 1); // end of the aligned block
             ulen = run_hi(i, key, end - 1 - i) - i + 1;
         }
-        if constexpr (MODE == kClsUnits) {
-            const uint64_t m = __ballot(ustart);
-            int base = 0;
-            if (lane == 0 && m)
-                base = atomicAdd(&d.ctl->n_units, __popcll(m));
-            base = __shfl(base, 0, kWave);
-            if (ustart)
-                d.units[base + __popcll(m & lt)] = make_int2(i, ulen);
-        } else {
-            const int wid = threadIdx.x / kWave;
-            constexpr uint64_t FM = CellBits<DIM>::FMASK;
-            const float capf = 1.f / (float)(K + 1);
-            float f[kMlpIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            if (ustart) {
-                const int a = run_lo(i, key, K), b = run_hi(i, key, K);
-                f[0] = (float)min(b - a + 1, K + 1) * capf;
-                const uint64_t last = key & FM;
-                if (last > 0 && a > 0 && kat(a - 1) == key - 1)
-                    f[1] = (float)min(a - run_lo(a - 1, key - 1, K), K + 1) * capf;
-                if (last < FM && b + 1 < W && kat(b + 1) == key + 1)
-                    f[2] = (float)min(run_hi(b + 1, key + 1, K) - b, K + 1) * capf;
-                const float *Nx = d.axyz[st.par ^ 1];
+        uint64_t mo = __ballot(ustart), mi = 0; // front / back list members
+        float f[kMlpIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if constexpr (LEARN) {
+            if (mo) { // the wave holds unit starts: features and prediction
+                constexpr uint64_t FM = CellBits<DIM>::FMASK;
+                const float capf = 1.f / (float)(K + 1);
+                if (ustart) {
+                    const int a = run_lo(i, key, K), b = run_hi(i, key, K);
+                    f[0] = (float)min(b - a + 1, K + 1) * capf;
+                    const uint64_t last = key & FM;
+                    if (last > 0 && a > 0 && kat(a - 1) == key - 1)
+                        f[1] = (float)min(a - run_lo(a - 1, key - 1, K), K + 1) * capf;
+                    if (last < FM && b + 1 < W && kat(b + 1) == key + 1)
+                        f[2] = (float)min(run_hi(b + 1, key + 1, K) - b, K + 1) * capf;
+                    const float *Nx = d.axyz[st.par ^ 1];
 #pragma unroll
-                for (int ax = 0; ax < DIM; ++ax) {
-                    const double u = (double)Nx[(size_t)ax * W + i] * d.inv_c;
-                    const float off = (float)(u - floor(u)) - 0.5f;
-                    f[3 + ax] = (off >= -0.5f && off <= 0.5f) ? off : 0.f; // non-finite -> 0
+                    for (int ax = 0; ax < DIM; ++ax) {
+                        const double u = (double)Nx[(size_t)ax * W + i] * d.inv_c;
+                        const float off = (float)(u - floor(u)) - 0.5f;
+                        f[3 + ax] = (off >= -0.5f && off <= 0.5f) ? off : 0.f; // non-finite -> 0
+                    }
                 }
-            }
-            // the wave's unit starts, compacted into rows 0..m-1
-            const uint64_t sm = __ballot(ustart);
-            const int m = __popcll(sm);
-            const int row = __popcll(sm & lt);
-            if (ustart) {
+                // the wave's unit starts, compacted into rows 0..m-1
+                const int m = __popcll(mo);
+                const int row = __popcll(mo & lt);
+                if (ustart) {
 #pragma unroll
-                for (int j = 0; j < kFeatStride; ++j)
-                    lds.x[wid][row][j] = j < kMlpIn ? f[j < kMlpIn ? j : 0] : 0.f;
+                    for (int j = 0; j < kFeatStride; ++j)
+                        lds.x[wid][row][j] = j < kMlpIn ? f[j < kMlpIn ? j : 0] : 0.f;
+                }
+                wave_lds_fence();
+                const float *P = current_weights(d.lr);
+                mlp_infer_rows(P, lds.x[wid], m, lds.h[wid], lds.z[wid]);
+                const float yh = ustart ? sigmoidf(lds.z[wid][row] + P[kOffB3]) : 0.f;
+                wave_lds_fence();
+                const bool pin = ustart && yh >= kInlierCut;
+                mi = __ballot(pin);
+                mo = __ballot(ustart && !pin);
             }
-            wave_lds_fence();
-            const float *P = current_weights(d.lr);
-            mlp_infer_rows(P, lds.x[wid], m, lds.h[wid], lds.z[wid]);
-            const float yh = ustart ? sigmoidf(lds.z[wid][row] + P[kOffB3]) : 0.f;
-            wave_lds_fence();
-            const bool pin = ustart && yh >= kInlierCut;
-            const bool pout = ustart && !pin;
-            const uint64_t mo = __ballot(pout), mi = __ballot(pin);
-            int bo = 0, bi = 0;
-            if (lane == 0) {
-                if (mo)
-                    bo = atomicAdd(&d.ctl->n_units, __popcll(mo));
-                if (mi)
-                    bi = atomicAdd(&d.lr.lc->n_units_i, __popcll(mi));
-            }
-            bo = __shfl(bo, 0, kWave);
-            bi = __shfl(bi, 0, kWave);
-            if (ustart) {
-                const int li = pout ? bo + __popcll(mo & lt) : W - 1 - (bi + __popcll(mi & lt));
-                d.units[li] = make_int2(i, ulen);
+        }
+        if (lane == 0) {
+            lds.wc[wid][0] = __popcll(cmask);
+            lds.wc[wid][1] = __popcll(mo);
+            lds.wc[wid][2] = __popcll(mi);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tot[3] = {0, 0, 0};
+            for (int w = 0; w < ClassifyLds<LEARN>::NW; ++w)
+                for (int q = 0; q < 3; ++q)
+                    tot[q] += lds.wc[w][q];
+            if (tot[0])
+                atomicAdd(&d.ctl->n_cand, tot[0]);
+            lds.base[1] = tot[1] ? atomicAdd(&d.ctl->n_units, tot[1]) : 0;
+            if constexpr (LEARN)
+                lds.base[2] = tot[2] ? atomicAdd(&d.lr.lc->n_units_i, tot[2]) : 0;
+        }
+        __syncthreads();
+        int bo = lds.base[1], bi = LEARN ? lds.base[2] : 0;
+        for (int w = 0; w < wid; ++w) {
+            bo += lds.wc[w][1];
+            bi += lds.wc[w][2];
+        }
+        if (ustart) {
+            const bool front = (mo >> lane) & 1ull;
+            const int li = front ? bo + __popcll(mo & lt) : W - 1 - (bi + __popcll(mi & lt));
+            d.units[li] = make_int2(i, ulen);
+            if constexpr (LEARN) {
                 float *rx = d.lr.rep_x +
                             ((size_t)(st.t % kReplaySlides) * W + li) * kFeatStride;
                 *reinterpret_cast<float4 *>(rx) = make_float4(f[0], f[1], f[2], f[3]);
@@ -1054,20 +1066,21 @@ __device__ __forceinline__ int row_bounds(const Dev &d, const uint64_t *Nk,
     return res;
 }
 
-__device__ __forceinline__ int wave_min_i(int x)
+// min / max over lanes 0..15 (a unit's candidates), broadcast to the wave
+__device__ __forceinline__ int unit_min_i(int x)
 {
 #pragma unroll
-    for (int m = 1; m < kWave; m <<= 1)
+    for (int m = 1; m < 16; m <<= 1)
         x = min(x, __shfl_xor(x, m, kWave));
-    return x;
+    return __builtin_amdgcn_readfirstlane(x);
 }
 
-__device__ __forceinline__ int wave_max_i(int x)
+__device__ __forceinline__ int unit_max_i(int x)
 {
 #pragma unroll
-    for (int m = 1; m < kWave; m <<= 1)
+    for (int m = 1; m < 16; m <<= 1)
         x = max(x, __shfl_xor(x, m, kWave));
-    return x;
+    return __builtin_amdgcn_readfirstlane(x);
 }
 
 __device__ __forceinline__ float lane_f(float v, int l)
@@ -1111,41 +1124,56 @@ __device__ __forceinline__ int unit_count(const Dev &d, const uint64_t *Nk, cons
     }
     wave_sync_lds();
 
+    // PF chunks at a time: their row lookups and coordinate loads are
+    // issued together, so a group costs one memory round trip
+    constexpr int PF = 8;
     int cnt = 0;
-    for (int base = 0; base < total; base += kWave) {
-        const int idx = base + lane;
-        float qx = 0.f, qy = 0.f, qz = 0.f;
-        const bool have = idx < total;
-        if (have) {
-            int j;
-            if constexpr (DIM == 1) {
-                j = w_a[0] + idx;
-            } else {
-                int lo = 0, hi = st.nrows - 1;
-                while (lo < hi) { // last row with off <= idx
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (w_b[mid] <= idx)
-                        lo = mid;
-                    else
-                        hi = mid - 1;
+    for (int base0 = 0; base0 < total; base0 += PF * kWave) {
+        float qx[PF], qy[PF], qz[PF];
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            const int idx = base0 + k * kWave + lane;
+            qx[k] = qy[k] = qz[k] = 0.f;
+            if (idx < total) {
+                int j;
+                if constexpr (DIM == 1) {
+                    j = w_a[0] + idx;
+                } else {
+                    int lo = 0, hi = st.nrows - 1;
+                    while (lo < hi) { // last row with off <= idx
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (w_b[mid] <= idx)
+                            lo = mid;
+                        else
+                            hi = mid - 1;
+                    }
+                    j = w_a[lo] + (idx - w_b[lo]);
                 }
-                j = w_a[lo] + (idx - w_b[lo]);
+                qx[k] = Nx[j];
+                if constexpr (DIM > 1)
+                    qy[k] = Nx[(size_t)W + j];
+                if constexpr (DIM > 2)
+                    qz[k] = Nx[(size_t)2 * W + j];
             }
-            qx = Nx[j];
-            if constexpr (DIM > 1)
-                qy = Nx[(size_t)W + j];
-            if constexpr (DIM > 2)
-                qz = Nx[(size_t)2 * W + j];
         }
-        for (int c = 0; c < n; ++c) {
-            const float p3[3] = {lane_f(pc[0], c), DIM > 1 ? lane_f(pc[1], c) : 0.f,
-                                 DIM > 2 ? lane_f(pc[2], c) : 0.f};
-            const bool hit = have && dist2<DIM>(p3, qx, qy, qz) <= d.r2;
-            const int h = __popcll(__ballot(hit));
-            cnt += (lane == c) ? h : 0;
+        bool done = false;
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            const int base = base0 + k * kWave;
+            if (done || base >= total)
+                break;
+            const bool have = base + lane < total;
+            for (int c = 0; c < n; ++c) {
+                const float p3[3] = {lane_f(pc[0], c), DIM > 1 ? lane_f(pc[1], c) : 0.f,
+                                     DIM > 2 ? lane_f(pc[2], c) : 0.f};
+                const bool hit = have && dist2<DIM>(p3, qx[k], qy[k], qz[k]) <= d.r2;
+                const int h = __popcll(__ballot(hit));
+                cnt += (lane == c) ? h : 0;
+            }
+            tests += (long long)min(kWave, total - base) * n;
+            done = __ballot(lane < n && cnt <= K) == 0;
         }
-        tests += (long long)min(kWave, total - base) * n;
-        if (__ballot(lane < n && cnt <= K) == 0)
+        if (done)
             break;
     }
     return cnt;
@@ -1170,8 +1198,8 @@ __device__ __forceinline__ Stencil<DIM> unit_stencil(const Dev &d, const float (
             lo = cell_coord<DIM>((double)pc[a] - d.Rp, d.inv_c);
             hi = cell_coord<DIM>((double)pc[a] + d.Rp, d.inv_c);
         }
-        st.lo[a] = wave_min_i(lo);
-        st.hi[a] = wave_max_i(hi);
+        st.lo[a] = unit_min_i(lo);
+        st.hi[a] = unit_max_i(hi);
     }
     if constexpr (DIM == 1) {
         st.nrows = 1;
@@ -1294,9 +1322,9 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int n
 // LEARN: the list is the probable-outlier units [0, nO) followed by the
 // probable-inlier units (indices W-1, W-2, ...): the long full counts are
 // handed out first.  When the schedule is on (lc->use_pred), a probable-
-// inlier unit is first counted over its own cell and the neighbouring
-// cells only; the full stencil is counted only if some candidate is still
-// at <= K.  Every count is exact; each unit's smallest count goes to the
+// inlier unit (2-D/3-D) is first counted over its own cell and the
+// neighbouring cells only; the full stencil is counted only if some
+// candidate is still at <= K.  (In 1-D those cells are most of the stencil.)  Every count is exact; each unit's smallest count goes to the
 // replay, and mispredicted units are counted for the fallback rule.
 // ---------------------------------------------------------------------------
 struct DetectLds {
@@ -1358,7 +1386,7 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
         }
         const Stencil<DIM> full = unit_stencil<DIM>(d, pc, n);
         int cnt;
-        if (LEARN && use_pred && u >= n_o) {
+        if (LEARN && DIM > 1 && use_pred && u >= n_o) {
             cnt = unit_count<DIM>(d, Nk, Nx, s_samp, s_rows[wid][0], s_rows[wid][1], pc, n,
                                   near_stencil<DIM>(d, full, pc), tests);
             if (__ballot(lane < n && cnt <= K) != 0) {
@@ -1375,7 +1403,7 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
             const int slot = Ns[start + lane];
             atomicOr(&d.bitmap[slot >> 5], 1u << (slot & 31));
         }
-        const int umin = wave_min_i(capped);
+        const int umin = unit_min_i(capped);
         if (lane == 0) {
             d.cand_cnt[u] = umin;
             if constexpr (LEARN) {

@@ -5,6 +5,8 @@ Single streams:
   C2 walk1d-W100k-S5k   1-D random walk, 1,048,572 points, W=100k, S=5k, R=0.45, K=50
   C3 gmm3d-W10k-S500    3-D mixture of 8 drifting Gaussians + 0.5% noise,
                         575,000 points, W=10k, S=500, R=1.9, K=50
+  gmm3d-W100k-S5k       the same generator, 2.1M points, W=100k, S=5k, R=0.9,
+                        K=50 (detection throughput-bound)
 Batched (C4): `batch_workload` draws n independent streams of uneven shape --
 dim ~ U{1,2,3}, W log-uniform on [2k, 100k] rounded to a multiple of S,
 S = 1-5% of W, K in {20, 50} -- with R set on the first window so that the
@@ -84,7 +86,9 @@ class Workload:
 C1 = Workload("walk1d-W10k-S500", 1, 200_000, 10_000, 500, 0.45, 50, "walk", 1)
 C2 = Workload("walk1d-W100k-S5k", 1, 1_048_572, 100_000, 5_000, 0.45, 50, "walk", 2)
 C3 = Workload("gmm3d-W10k-S500", 3, 575_000, 10_000, 500, 1.9, 50, "gmm", 3)
-WORKLOADS = {w.name: w for w in (C1, C2, C3)}
+# a large 3-D window: ~90k candidates per slide, detection throughput-bound
+C3L = Workload("gmm3d-W100k-S5k", 3, 2_100_000, 100_000, 5_000, 0.9, 50, "gmm", 11)
+WORKLOADS = {w.name: w for w in (C1, C2, C3, C3L)}
 
 
 @dataclass(frozen=True)
