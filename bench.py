@@ -209,38 +209,65 @@ def learned_comparison(names, train_slides, steps):
     return out
 
 
-def batched(n_streams, warmup, steps, tune_slides, cpu_slides, with_cpu):
+def _batch_run(descs, data, sched, warmup, steps, tune_slides):
+    """Autotune, warm up (the learned schedule trains meanwhile), then time
+    `steps` steps; per-step digests of all outlier lists for comparison."""
+    import hashlib
+
+    import rtod_amd
+
+    cfg = [(d.dim, d.W, d.S, d.R, d.K) for d in descs]
+    dss = [rtod_amd.DeviceStream(x) for x in data]
+    b = rtod_amd.Batch(cfg, sched_kind=sched)
+    b.bind_device([ds.ptr.value for ds in dss], [x.shape[0] for x in data])
+    tuning = b.autotune(tune_slides)
+    for _ in range(warmup):
+        b.step_device()
+    outs, digests, t_tot = 0, [], 0.0
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        lists = b.step_device()
+        t_tot += time.perf_counter() - t0
+        outs += sum(len(x) for x in lists)
+        h = hashlib.sha256()
+        for x in lists:
+            h.update(x.tobytes())
+            h.update(b"|")
+        digests.append(h.hexdigest())
+    st = [b.stream_stats(i) for i in range(len(descs))]
+    b.close()
+    for ds in dss:
+        ds.close()
+    return t_tot / steps, outs / steps, digests, tuning, st
+
+
+def batched(n_streams, warmup, steps, tune_slides, cpu_slides, with_cpu, learned_warmup=40):
     """The batched workload: n_streams independent streams (dim ~ U{1,2,3},
     W log-uniform on [2k, 100k], S = 1-5% of W, K in {20, 50}, R set for a
-    median neighbour count ~2K), resident in HBM, advanced together."""
+    median neighbour count ~2K), resident in HBM, advanced together; plain,
+    then the learned schedule on the same steps."""
     import rtod_amd
     from rtod_amd import synth
 
-    slides = tune_slides + warmup + steps
+    slides = tune_slides + max(warmup, learned_warmup) + steps
     t0 = time.perf_counter()
     descs, data = synth.batch_workload(n_streams, slides=slides)
     log(f"[bench] batched: {n_streams} streams generated in {time.perf_counter() - t0:.1f} s, "
         f"sum W {sum(d.W for d in descs)}, sum S {sum(d.S for d in descs)}")
-    cfg = [(d.dim, d.W, d.S, d.R, d.K) for d in descs]
-    dss = [rtod_amd.DeviceStream(x) for x in data]
-    b = rtod_amd.Batch(cfg)
-    b.bind_device([ds.ptr.value for ds in dss], [x.shape[0] for x in data])
-    tuning = b.autotune(tune_slides)
+    dt, out_per_step, dig_p, tuning, _ = _batch_run(descs, data, rtod_amd.SCHED_PLAIN,
+                                                    learned_warmup, steps, tune_slides)
     log(f"[bench] batched autotune: {tuning}")
-    # the autotuner leaves the batch primed at slide 0: warm up, then time
-    outs = 0
-    for _ in range(warmup):
-        b.step_device()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        lists = b.step_device()
-        outs += sum(len(x) for x in lists)
-    dt = (time.perf_counter() - t0) / steps
-    b.close()
-    for ds in dss:
-        ds.close()
+    dt_l, _, dig_l, tuning_l, st_l = _batch_run(descs, data, rtod_amd.SCHED_LEARNED,
+                                                learned_warmup, steps, tune_slides)
+    pts = max(1, sum(s["predictor_points"] for s in st_l))
+    learned = {"us_per_step": round(dt_l * 1e6, 2),
+               "speedup_vs_plain": round(dt / dt_l, 3),
+               "mismatched_steps": sum(1 for a, c in zip(dig_p, dig_l) if a != c),
+               "miss_rate": round(sum(s["predictor_misses"] for s in st_l) / pts, 4),
+               "fallback_stream_slides": int(sum(s["fallbacks"] for s in st_l)),
+               "train_steps": int(max(s["train_steps"] for s in st_l)),
+               "tuning": tuning_l}
     us = dt * 1e6
-    out_per_step = outs / steps
     bytes_step = sum(4 * d.dim * (d.W + d.S) for d in descs) + 4 * out_per_step
     achieved = bytes_step / dt / 1e9
     ent = {"workload": f"batch-{n_streams} streams: dim U{{1,2,3}}, W log-U[2k,100k], "
@@ -251,6 +278,7 @@ def batched(n_streams, warmup, steps, tune_slides, cpu_slides, with_cpu):
            "new_points_per_s": round(sum(d.S for d in descs) / dt),
            "outliers_per_step": round(out_per_step, 1),
            "tuning": tuning,
+           "learned": learned,
            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                         "algorithmic_bytes": int(bytes_step)}}

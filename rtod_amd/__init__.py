@@ -312,13 +312,16 @@ def _tuning_dict(t: _BatchTuning) -> dict:
 class Batch:
     """Many independent streams advanced together: each phase of a slide is
     one launch over all streams, the step one hipGraph replay (rtod_batch_*).
-    streams: list of (dim, W, S, R, K)."""
+    streams: list of (dim, W, S, R, K); sched_kind LEARNED shares one
+    online-trained predictor among all streams."""
 
-    def __init__(self, streams, device: int = 0):
+    def __init__(self, streams, device: int = 0, sched_kind: int = SCHED_PLAIN,
+                 miss_threshold: float = 0.25):
         n = len(streams)
         cfgs = (_Config * n)()
         for i, (dim, W, S, R, K) in enumerate(streams):
-            cfgs[i] = _Config(dim, W, S, R, K, INDEX_AUTO, SCHED_PLAIN, 0.25, 0, device)
+            cfgs[i] = _Config(dim, W, S, R, K, INDEX_AUTO, sched_kind, miss_threshold, 0,
+                              device)
         self.streams = [tuple(x) for x in streams]
         self.n = n
         h = C.c_void_p()

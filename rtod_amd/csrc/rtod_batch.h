@@ -95,24 +95,116 @@ __global__ __launch_bounds__(kBatchNT) void k_batch_merge_window(const BatchItem
     merge_window_body<DIM>(bs.d, batch_step(bs), it.bid, lds);
 }
 
-template <int DIM>
+template <int DIM, int MODE>
 __global__ __launch_bounds__(kBatchNT) void k_batch_classify(const BatchItem *items,
                                                             const BatchStream *streams)
 {
-    __shared__ ClassifyLds<false> lds;
+    __shared__ ClassifyLds<MODE == kClsLearned> lds;
     const BatchItem it = items[blockIdx.x];
     const BatchStream &bs = streams[it.stream];
-    classify_body<DIM, kClsUnits>(bs.d, batch_step(bs), it.bid, lds);
+    classify_body<DIM, MODE>(bs.d, batch_step(bs), it.bid, lds);
 }
 
-template <int DIM>
-__global__ __launch_bounds__(kBatchNT) void k_batch_detect(const BatchItem *items,
+template <int DIM, bool LEARN>
+__global__ __launch_bounds__(kDetectNT) void k_batch_detect(const BatchItem *items,
                                                           const BatchStream *streams)
 {
     __shared__ DetectLds lds;
     const BatchItem it = items[blockIdx.x];
     const BatchStream &bs = streams[it.stream];
-    detect_body<DIM, false>(bs.d, batch_step(bs), it.bid, it.arg, lds);
+    detect_body<DIM, LEARN>(bs.d, batch_step(bs), it.bid, it.arg, lds);
+}
+
+// The batch's shared predictor (learned batches), trained on every stream's
+// replay: a stream is drawn with probability proportional to its stored
+// units, then one of them uniformly; the slots of slides t_next and
+// t_next+1 are skipped (the next steps may be rewriting them).  n <=
+// kTrainMaxStreams.
+__global__ __launch_bounds__(kTrainNT) void k_train_batch(const BatchStream *streams, int n,
+                                                         Learn shared, uint64_t t_next,
+                                                         int steps)
+{
+    constexpr int PER = kTrainMaxStreams / kTrainNT;
+    __shared__ TrainWaveLds lds_wave[kTrainNT / kWave];
+    __shared__ TrainLds s;
+    __shared__ int s_pref[kTrainMaxStreams + 1];
+    __shared__ int s_scan[kTrainNT / kWave];
+    if (shared.ls->frozen)
+        return;
+    const int tid = threadIdx.x;
+    const int skip0 = (int)(t_next % kReplaySlides), skip1 = (int)((t_next + 1) % kReplaySlides);
+    int tot[PER], mine = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = tid * PER + q;
+        tot[q] = 0;
+        if (i < n) {
+            const int32_t *rn = streams[i].d.lr.rep_n;
+            for (int j = 0; j < kReplaySlides; ++j)
+                if (j != skip0 && j != skip1)
+                    tot[q] += rn[2 * j] + rn[2 * j + 1];
+        }
+        mine += tot[q];
+    }
+    int total = 0;
+    int run = block_excl_scan<kTrainNT>(mine, s_scan, total);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = tid * PER + q;
+        if (i < n)
+            s_pref[i] = run;
+        run += tot[q];
+    }
+    if (tid == 0)
+        s_pref[n] = total;
+    __syncthreads();
+    if (total <= 0)
+        return;
+    train_core(shared, steps, lds_wave, s, [&](uint32_t h, float *xs, float &y) {
+        const int u = (int)(h % (uint32_t)total);
+        int si = 0, hi = n - 1; // last stream with s_pref[si] <= u
+        while (si < hi) {
+            const int mid = (si + hi + 1) >> 1;
+            if (s_pref[mid] <= u)
+                si = mid;
+            else
+                hi = mid - 1;
+        }
+        const Dev &d = streams[si].d;
+        int off = u - s_pref[si];
+        int4 rn4[kReplaySlides / 2];
+        const int4 *src = reinterpret_cast<const int4 *>(d.lr.rep_n);
+#pragma unroll
+        for (int k = 0; k < kReplaySlides / 2; ++k)
+            rn4[k] = src[k];
+        int slot = 0, n_o = 0;
+#pragma unroll
+        for (int k = 0; k < kReplaySlides / 2; ++k) {
+#pragma unroll
+            for (int h2 = 0; h2 < 2; ++h2) {
+                const int j = 2 * k + h2;
+                const int o = h2 ? rn4[k].z : rn4[k].x, in = h2 ? rn4[k].w : rn4[k].y;
+                const int c = (j == skip0 || j == skip1) ? 0 : o + in;
+                if (off >= 0 && off < c) {
+                    slot = j;
+                    n_o = o;
+                    off = -1 - off; // found: remember the in-slot index
+                } else if (off >= 0) {
+                    off -= c;
+                }
+            }
+        }
+        if (off >= 0) { // counts changed underneath: an empty sample
+#pragma unroll
+            for (int j = 0; j < kFeatStride; ++j)
+                xs[j] = 0.f;
+            y = 0.f;
+            return;
+        }
+        const int in_slot = -1 - off;
+        load_replay_row(d.lr, d.W, d.K, slot,
+                        in_slot < n_o ? in_slot : d.W - 1 - (in_slot - n_o), xs, y);
+    });
 }
 
 } // namespace rtod

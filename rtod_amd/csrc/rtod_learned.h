@@ -49,6 +49,7 @@ constexpr float kInlierCut = 0.75f; // predicted inlier iff output >= this
 constexpr int kTrainNT = 256;       // training workgroup: 4 waves
 constexpr int kTrainRows = 128;     // samples per step: 32 per wave
 constexpr int kTrainSteps = 2;      // Adam steps per training launch
+constexpr int kTrainStepsBatch = 8; // per launch after a batch step (a step is ms long)
 constexpr float kAdamLr = 5e-3f, kAdamB1 = 0.9f, kAdamB2 = 0.999f, kAdamEps = 1e-7f;
 
 __device__ __forceinline__ f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c)
@@ -218,6 +219,7 @@ struct TrainLds {
     float loss;
     int pref[kReplaySlides + 1];
 };
+constexpr int kTrainMaxStreams = 1024; // a batch's replays sampled by one trainer
 
 // Gradient of the mean loss over n_total samples, for the 32 samples of this
 // wave (rows row0 .. row0+31; rows >= n_total contribute nothing).  w.xs and
@@ -398,26 +400,21 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x)
 }
 
 // ---------------------------------------------------------------------------
-// k_train: one workgroup of kTrainNT threads, launched on its own stream
-// after a slide, overlapping the next slide's ingest and merge on CUs they
-// leave idle.  `steps` Adam steps, each on kTrainRows samples drawn uniformly
-// from the replay, skipping the slots of slides t_next and t_next+1 (those
-// the next slides may be rewriting meanwhile).  The FP32 master weights and
-// the moments are the trainer's own; the result is published to the
-// inactive half of the double-buffered weights and then made current, so a
-// classify never waits for training and never reads a half-written set.
+// Training: one workgroup of kTrainNT threads, launched on a stream of its
+// own after a slide, overlapping the next slide's ingest and merge on CUs
+// they leave idle.  `steps` Adam steps, each on kTrainRows samples drawn by
+// `sample` (which fills a wave's 32 rows of xs / ys).  The FP32 master
+// weights and the moments are the trainer's own; the result is published to
+// the inactive half of the double-buffered weights and then made current,
+// so a classify never waits for training and never reads a half-written set.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kTrainNT) void k_train(Dev d, uint64_t t_next, int steps)
+template <typename Sample>
+__device__ __forceinline__ void train_core(const Learn &L, int steps, TrainWaveLds *lds_wave,
+                                           TrainLds &s, Sample &&sample)
 {
-    __shared__ TrainWaveLds lds_wave[kTrainNT / kWave];
-    __shared__ TrainLds s;
-    const Learn &L = d.lr;
-    LearnCtl *lc = L.lc;
+    LearnShared *ls = L.ls;
     const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
     TrainWaveLds &w = lds_wave[wid];
-    if (lc->frozen)
-        return;
-    const int skip0 = (int)(t_next % kReplaySlides), skip1 = (int)((t_next + 1) % kReplaySlides);
     constexpr int PER = (kMlpNP + kTrainNT - 1) / kTrainNT;
     float am[PER], av[PER];
 #pragma unroll
@@ -431,49 +428,14 @@ __global__ __launch_bounds__(kTrainNT) void k_train(Dev d, uint64_t t_next, int 
             av[q] = L.adam_v[p];
         }
     }
-    if (tid == 0) {
-        int acc = 0;
-        s.pref[0] = 0;
-        for (int j = 0; j < kReplaySlides; ++j) {
-            if (j != skip0 && j != skip1)
-                acc += L.rep_n[2 * j] + L.rep_n[2 * j + 1];
-            s.pref[j + 1] = acc;
-        }
+    if (tid == 0)
         s.loss = 0.f;
-    }
     __syncthreads();
-    const int N = s.pref[kReplaySlides];
-    if (N <= 0)
-        return;
-    int t_adam = lc->adam_t;
-    const int W = d.W;
+    int t_adam = ls->adam_t;
     for (int it = 0; it < steps; ++it) {
         if (lane < 32) {
             const uint32_t h = mix32((uint32_t)t_adam * 0x9e3779b9u + (uint32_t)(wid * 32 + lane));
-            const int u = (int)(h % (uint32_t)N);
-            int j = 0, hi = kReplaySlides - 1; // last slot with pref[j] <= u
-            while (j < hi) {
-                const int mid = (j + hi + 1) >> 1;
-                if (s.pref[mid] <= u)
-                    j = mid;
-                else
-                    hi = mid - 1;
-            }
-            const int off = u - s.pref[j];
-            const int nO = L.rep_n[2 * j];
-            const int li = off < nO ? off : W - 1 - (off - nO);
-            const size_t e = (size_t)j * W + li;
-            const float4 x0 = *reinterpret_cast<const float4 *>(&L.rep_x[e * kFeatStride]);
-            const float2 x1 = *reinterpret_cast<const float2 *>(&L.rep_x[e * kFeatStride + 4]);
-            w.xs[lane][0] = x0.x;
-            w.xs[lane][1] = x0.y;
-            w.xs[lane][2] = x0.z;
-            w.xs[lane][3] = x0.w;
-            w.xs[lane][4] = x1.x;
-            w.xs[lane][5] = x1.y;
-            w.xs[lane][6] = 0.f;
-            w.xs[lane][7] = 0.f;
-            w.ys[lane] = mlp_target(L.rep_c[e], d.K);
+            sample(h, w.xs[lane], w.ys[lane]);
         }
         wave_lds_fence();
         float sse = mlp_wave_grads(s.p, s.g, w, wid * 32, kTrainRows);
@@ -498,7 +460,7 @@ __global__ __launch_bounds__(kTrainNT) void k_train(Dev d, uint64_t t_next, int 
         }
         __syncthreads();
     }
-    const int cur = __hip_atomic_load(&lc->pub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int cur = __hip_atomic_load(&ls->pub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     float *pub = L.pub + (size_t)((cur + 1) & 1) * kMlpNP;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -513,18 +475,77 @@ __global__ __launch_bounds__(kTrainNT) void k_train(Dev d, uint64_t t_next, int 
     __threadfence();
     __syncthreads();
     if (tid == 0) {
-        lc->adam_t = t_adam;
-        lc->steps_total += steps;
-        lc->loss_last = s.loss / (float)(steps * kTrainRows);
+        ls->adam_t = t_adam;
+        ls->steps_total += steps;
+        ls->loss_last = s.loss / (float)(steps * kTrainRows);
         __threadfence();
-        __hip_atomic_store(&lc->pub, cur + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ls->pub, cur + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+
+// one replay row: features of unit-list index li in slot j, target from its
+// capped count
+__device__ __forceinline__ void load_replay_row(const Learn &L, int W, int K, int j, int li,
+                                                float *xs, float &y)
+{
+    const size_t e = (size_t)j * W + li;
+    const float4 x0 = *reinterpret_cast<const float4 *>(&L.rep_x[e * kFeatStride]);
+    const float2 x1 = *reinterpret_cast<const float2 *>(&L.rep_x[e * kFeatStride + 4]);
+    xs[0] = x0.x;
+    xs[1] = x0.y;
+    xs[2] = x0.z;
+    xs[3] = x0.w;
+    xs[4] = x1.x;
+    xs[5] = x1.y;
+    xs[6] = 0.f;
+    xs[7] = 0.f;
+    y = mlp_target(L.rep_c[e], K);
+}
+
+// One stream's replay, uniform over its stored units, slots of slides
+// t_next and t_next+1 skipped (the next slides may be rewriting them).
+__global__ __launch_bounds__(kTrainNT) void k_train(Dev d, uint64_t t_next, int steps)
+{
+    __shared__ TrainWaveLds lds_wave[kTrainNT / kWave];
+    __shared__ TrainLds s;
+    const Learn &L = d.lr;
+    if (L.ls->frozen)
+        return;
+    const int skip0 = (int)(t_next % kReplaySlides), skip1 = (int)((t_next + 1) % kReplaySlides);
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        s.pref[0] = 0;
+        for (int j = 0; j < kReplaySlides; ++j) {
+            if (j != skip0 && j != skip1)
+                acc += L.rep_n[2 * j] + L.rep_n[2 * j + 1];
+            s.pref[j + 1] = acc;
+        }
+    }
+    __syncthreads();
+    const int N = s.pref[kReplaySlides];
+    if (N <= 0)
+        return;
+    const int W = d.W, K = d.K;
+    train_core(L, steps, lds_wave, s, [&](uint32_t h, float *xs, float &y) {
+        const int u = (int)(h % (uint32_t)N);
+        int j = 0, hi = kReplaySlides - 1; // last slot with pref[j] <= u
+        while (j < hi) {
+            const int mid = (j + hi + 1) >> 1;
+            if (s.pref[mid] <= u)
+                j = mid;
+            else
+                hi = mid - 1;
+        }
+        const int off = u - s.pref[j];
+        const int nO = L.rep_n[2 * j];
+        load_replay_row(L, W, K, j, off < nO ? off : W - 1 - (off - nO), xs, y);
+    });
 }
 
 // the weights classify uses: the published half of the double buffer
 __device__ __forceinline__ const float *current_weights(const Learn &L)
 {
-    const int cur = __hip_atomic_load(&L.lc->pub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const int cur = __hip_atomic_load(&L.ls->pub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     return L.pub + (size_t)(cur & 1) * kMlpNP;
 }
 

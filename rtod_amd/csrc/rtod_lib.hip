@@ -85,7 +85,7 @@ struct rtod_ctx {
     rtod_config cfg{};
     int dim = 1, W = 0, S = 0, m = 0, K = 0;
     int sort_shape = 0, CH = 512;
-    int detect_blocks = 1024;
+    int detect_blocks = 512;
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = true; // false: a batch member on the batch's stream
@@ -436,6 +436,7 @@ struct CreateOpts {
     int sort_shape = -1;          // -1: picked from S and dim
     double cell_scale = 1.0;      // cell side multiplier (<= 1)
     hipStream_t stream = nullptr; // borrowed stream
+    bool trainer = true;          // learned: own training stream (batches share one)
 };
 
 rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts &opt = {})
@@ -548,16 +549,20 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
         RTOD_TRY(dalloc(c, &lr.rep_c, (size_t)kReplaySlides * W));
         RTOD_TRY(dalloc(c, &lr.rep_n, (size_t)2 * kReplaySlides));
         RTOD_TRY(dalloc(c, &lr.lc, 1));
+        RTOD_TRY(dalloc(c, &lr.ls, 1));
         RTOD_HIP(hipMemsetAsync(lr.rep_n, 0, sizeof(int32_t) * 2 * kReplaySlides, c->stream));
         RTOD_HIP(hipMemsetAsync(lr.lc, 0, sizeof(LearnCtl), c->stream));
+        RTOD_HIP(hipMemsetAsync(lr.ls, 0, sizeof(LearnShared), c->stream));
         std::vector<float> p0 = initial_predictor();
         RTOD_HIP(hipMemcpyAsync(lr.params, p0.data(), sizeof(float) * kMlpNP,
                                 hipMemcpyHostToDevice, c->stream));
         for (int h = 0; h < 2; ++h)
             RTOD_HIP(hipMemcpyAsync(lr.pub + (size_t)h * kMlpNP, p0.data(),
                                     sizeof(float) * kMlpNP, hipMemcpyHostToDevice, c->stream));
-        RTOD_HIP(hipStreamCreateWithFlags(&c->train_stream, hipStreamNonBlocking));
-        RTOD_HIP(hipEventCreateWithFlags(&c->train_ev, hipEventDisableTiming));
+        if (opt.trainer) {
+            RTOD_HIP(hipStreamCreateWithFlags(&c->train_stream, hipStreamNonBlocking));
+            RTOD_HIP(hipEventCreateWithFlags(&c->train_ev, hipEventDisableTiming));
+        }
         RTOD_HIP(hipMemsetAsync(lr.adam_m, 0, sizeof(float) * kMlpNP, c->stream));
         RTOD_HIP(hipMemsetAsync(lr.adam_v, 0, sizeof(float) * kMlpNP, c->stream));
     }
@@ -631,7 +636,7 @@ rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n,
     RTOD_HIP(hipMemsetAsync(c->d.ctl, 0, sizeof(Ctl), s));
     if (c->learned) { // replay and per-slide counters restart; weights stay
         RTOD_HIP(hipMemsetAsync(c->d.lr.rep_n, 0, sizeof(int32_t) * 2 * kReplaySlides, s));
-        RTOD_HIP(hipMemsetAsync(c->d.lr.lc, 0, kLearnCtlResetBytes, s));
+        RTOD_HIP(hipMemsetAsync(c->d.lr.lc, 0, sizeof(LearnCtl), s));
     }
     RTOD_HIP(hipGetLastError());
     RTOD_HIP(hipStreamSynchronize(s));
@@ -723,7 +728,7 @@ void accumulate_kernel_times(rtod_ctx *c, double *phase)
 // classify starts after it finishes.
 rtod_status maybe_train(rtod_ctx *c)
 {
-    if (!c->learned)
+    if (!c->learned || !c->train_stream)
         return RTOD_OK;
     if (c->train_pending) {
         const hipError_t q = hipEventQuery(c->train_ev);
@@ -1030,7 +1035,8 @@ RTOD_API rtod_status rtod_get_predictor(rtod_ctx *c, float *params, int32_t n)
         return RTOD_ERR_INVALID_ARGUMENT;
     }
     RTOD_HIP(hipSetDevice(c->device));
-    RTOD_HIP(hipStreamSynchronize(c->train_stream));
+    if (c->train_stream)
+        RTOD_HIP(hipStreamSynchronize(c->train_stream));
     c->train_pending = false;
     RTOD_HIP(hipMemcpyAsync(params, c->d.lr.params, sizeof(float) * kMlpNP,
                             hipMemcpyDeviceToHost, c->stream));
@@ -1058,7 +1064,8 @@ RTOD_API rtod_status rtod_set_predictor(rtod_ctx *c, const float *params, int32_
         return RTOD_ERR_STATE;
     }
     RTOD_HIP(hipSetDevice(c->device));
-    RTOD_HIP(hipStreamSynchronize(c->train_stream));
+    if (c->train_stream)
+        RTOD_HIP(hipStreamSynchronize(c->train_stream));
     c->train_pending = false;
     const Learn &lr = c->d.lr;
     RTOD_HIP(hipMemcpyAsync(lr.params, params, sizeof(float) * kMlpNP, hipMemcpyHostToDevice,
@@ -1069,7 +1076,7 @@ RTOD_API rtod_status rtod_set_predictor(rtod_ctx *c, const float *params, int32_
     RTOD_HIP(hipMemsetAsync(lr.adam_m, 0, sizeof(float) * kMlpNP, c->stream));
     RTOD_HIP(hipMemsetAsync(lr.adam_v, 0, sizeof(float) * kMlpNP, c->stream));
     const int32_t kept[2] = {0, frozen ? 1 : 0}; // adam_t, frozen
-    RTOD_HIP(hipMemcpyAsync(&lr.lc->adam_t, kept, sizeof(kept), hipMemcpyHostToDevice,
+    RTOD_HIP(hipMemcpyAsync(&lr.ls->adam_t, kept, sizeof(kept), hipMemcpyHostToDevice,
                             c->stream));
     RTOD_HIP(hipStreamSynchronize(c->stream));
     return RTOD_OK;
@@ -1256,6 +1263,13 @@ struct rtod_batch {
     hipGraphExec_t exec = nullptr;
     bool plan_ok = false, primed = false;
     int64_t t = 0;
+    // learned batches: one predictor shared by every stream
+    bool learned = false;
+    Learn shared{};
+    std::vector<void *> shared_allocs;
+    hipStream_t train_stream = nullptr;
+    hipEvent_t train_ev = nullptr;
+    bool train_pending = false;
 };
 
 namespace {
@@ -1284,6 +1298,38 @@ void batch_free_members(rtod_batch *b)
     b->ctx.clear();
 }
 
+// Learned batches: one launch of the shared predictor's trainer after a
+// step, unless the previous one is still running.
+rtod_status batch_maybe_train(rtod_batch *b)
+{
+    if (!b->learned)
+        return RTOD_OK;
+    if (b->train_pending) {
+        const hipError_t q = hipEventQuery(b->train_ev);
+        if (q == hipErrorNotReady) {
+            (void)hipGetLastError(); // not an error: still running
+            return RTOD_OK;
+        }
+        RTOD_HIP(q);
+        b->train_pending = false;
+    }
+    hipLaunchKernelGGL(k_train_batch, dim3(1), dim3(kTrainNT), 0, b->train_stream, b->d_streams,
+                       b->n, b->shared, (uint64_t)b->t, kTrainStepsBatch);
+    RTOD_HIP(hipGetLastError());
+    RTOD_HIP(hipEventRecord(b->train_ev, b->train_stream));
+    b->train_pending = true;
+    return RTOD_OK;
+}
+
+rtod_status batch_sync_trainer(rtod_batch *b)
+{
+    if (b->train_stream) {
+        RTOD_HIP(hipStreamSynchronize(b->train_stream));
+        b->train_pending = false;
+    }
+    return RTOD_OK;
+}
+
 rtod_status batch_create_members(rtod_batch *b)
 {
     batch_free_members(b);
@@ -1292,6 +1338,7 @@ rtod_status batch_create_members(rtod_batch *b)
         opt.sort_shape = kBatchSortShape;
         opt.cell_scale = b->tuning.cell_scale[b->cfg[i].dim - 1];
         opt.stream = b->stream;
+        opt.trainer = false;
         rtod_ctx *c = nullptr;
         rtod_status st = create_impl(&b->cfg[i], &c, opt);
         if (st != RTOD_OK) {
@@ -1299,13 +1346,21 @@ rtod_status batch_create_members(rtod_batch *b)
                 rtod_destroy(c);
             return st;
         }
+        if (b->learned) { // the batch's predictor replaces the member's own
+            Learn &lr = c->d.lr;
+            lr.params = b->shared.params;
+            lr.pub = b->shared.pub;
+            lr.adam_m = b->shared.adam_m;
+            lr.adam_v = b->shared.adam_v;
+            lr.ls = b->shared.ls;
+        }
         b->ctx.push_back(c);
     }
     b->primed = false;
     return RTOD_OK;
 }
 
-template <int DIM> void *batch_kernel(int kind)
+template <int DIM> void *batch_kernel(int kind, bool learned)
 {
     switch (kind) {
     case 0:
@@ -1315,9 +1370,11 @@ template <int DIM> void *batch_kernel(int kind)
     case 2:
         return reinterpret_cast<void *>(&k_batch_merge_window<DIM>);
     case 3:
-        return reinterpret_cast<void *>(&k_batch_classify<DIM>);
+        return learned ? reinterpret_cast<void *>(&k_batch_classify<DIM, kClsLearned>)
+                       : reinterpret_cast<void *>(&k_batch_classify<DIM, kClsUnits>);
     default:
-        return reinterpret_cast<void *>(&k_batch_detect<DIM>);
+        return learned ? reinterpret_cast<void *>(&k_batch_detect<DIM, true>)
+                       : reinterpret_cast<void *>(&k_batch_detect<DIM, false>);
     }
 }
 
@@ -1397,11 +1454,11 @@ rtod_status batch_build_plan(rtod_batch *b)
         const BatchStream *st = b->d_streams;
         void *args[2] = {(void *)&it, (void *)&st};
         hipKernelNodeParams kp{};
-        kp.func = l.dim == 1 ? batch_kernel<1>(l.kind)
-                  : l.dim == 2 ? batch_kernel<2>(l.kind)
-                               : batch_kernel<3>(l.kind);
+        kp.func = l.dim == 1 ? batch_kernel<1>(l.kind, b->learned)
+                  : l.dim == 2 ? batch_kernel<2>(l.kind, b->learned)
+                               : batch_kernel<3>(l.kind, b->learned);
         kp.gridDim = dim3(l.count);
-        kp.blockDim = dim3(l.kind == 0 ? kBatchSortNT : kBatchNT);
+        kp.blockDim = dim3(l.kind == 0 ? kBatchSortNT : l.kind == 4 ? kDetectNT : kBatchNT);
         kp.sharedMemBytes = 0;
         kp.kernelParams = args;
         kp.extra = nullptr;
@@ -1434,6 +1491,7 @@ rtod_status batch_check_tuning(const rtod_batch_tuning *t)
 rtod_status batch_apply_tuning(rtod_batch *b, const rtod_batch_tuning &t)
 {
     RTOD_TRY(batch_check_tuning(&t));
+    RTOD_TRY(batch_sync_trainer(b));
     bool cells = false;
     for (int a = 0; a < 3; ++a)
         cells = cells || t.cell_scale[a] != b->tuning.cell_scale[a];
@@ -1445,6 +1503,7 @@ rtod_status batch_apply_tuning(rtod_batch *b, const rtod_batch_tuning &t)
 
 rtod_status batch_prime_device(rtod_batch *b)
 {
+    RTOD_TRY(batch_sync_trainer(b));
     for (int i = 0; i < b->n; ++i)
         RTOD_TRY(prime_impl(b->ctx[i], b->src[i], b->cfg[i].window, hipMemcpyDeviceToDevice));
     b->t = 0;
@@ -1472,6 +1531,8 @@ rtod_status batch_launch_collect(rtod_batch *b, int32_t *offsets, int32_t *slots
             offsets[i + 1] = (int32_t)off;
     }
     b->t++;
+    if (first_err == RTOD_OK)
+        first_err = batch_maybe_train(b);
     return first_err;
 }
 
@@ -1503,10 +1564,18 @@ RTOD_API rtod_status rtod_batch_create(const rtod_config *cfgs, int32_t n, int32
     }
     for (int i = 0; i < n; ++i) {
         RTOD_TRY(validate(&cfgs[i]));
-        if (cfgs[i].sched_kind != RTOD_SCHED_PLAIN || cfgs[i].index_kind == RTOD_INDEX_LBVH) {
-            set_error("batched streams run the plain schedule on the cell-stencil index");
+        if (cfgs[i].index_kind == RTOD_INDEX_LBVH) {
+            set_error("batched streams run on the cell-stencil index");
             return RTOD_ERR_UNSUPPORTED;
         }
+        if (cfgs[i].sched_kind != cfgs[0].sched_kind) {
+            set_error("the streams of a batch share one schedule (all PLAIN or all LEARNED)");
+            return RTOD_ERR_UNSUPPORTED;
+        }
+    }
+    if (cfgs[0].sched_kind == RTOD_SCHED_LEARNED && n > kTrainMaxStreams) {
+        set_error("a learned batch holds at most 1024 streams");
+        return RTOD_ERR_UNSUPPORTED;
     }
     rtod_batch *b = new (std::nothrow) rtod_batch();
     if (!b)
@@ -1523,9 +1592,35 @@ RTOD_API rtod_status rtod_batch_create(const rtod_config *cfgs, int32_t n, int32
         b->tuning.detect_points_per_wg = 64;
         b->tuning.cell_scale[0] = b->tuning.cell_scale[1] = b->tuning.cell_scale[2] = 1.f;
         b->tuning.us_per_step = 0.0;
+        b->learned = cfgs[0].sched_kind == RTOD_SCHED_LEARNED;
         st = [&]() -> rtod_status {
             RTOD_HIP(hipSetDevice(device));
             RTOD_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+            if (b->learned) {
+                auto alloc = [&](void **p, size_t bytes) -> rtod_status {
+                    RTOD_HIP(hipMalloc(p, bytes));
+                    b->shared_allocs.push_back(*p);
+                    return RTOD_OK;
+                };
+                Learn &sh = b->shared;
+                RTOD_TRY(alloc((void **)&sh.params, sizeof(float) * kMlpNP));
+                RTOD_TRY(alloc((void **)&sh.pub, sizeof(float) * 2 * kMlpNP));
+                RTOD_TRY(alloc((void **)&sh.adam_m, sizeof(float) * kMlpNP));
+                RTOD_TRY(alloc((void **)&sh.adam_v, sizeof(float) * kMlpNP));
+                RTOD_TRY(alloc((void **)&sh.ls, sizeof(LearnShared)));
+                sh.on = 1;
+                const std::vector<float> p0 = initial_predictor();
+                RTOD_HIP(hipMemcpy(sh.params, p0.data(), sizeof(float) * kMlpNP,
+                                   hipMemcpyHostToDevice));
+                for (int h = 0; h < 2; ++h)
+                    RTOD_HIP(hipMemcpy(sh.pub + (size_t)h * kMlpNP, p0.data(),
+                                       sizeof(float) * kMlpNP, hipMemcpyHostToDevice));
+                RTOD_HIP(hipMemset(sh.adam_m, 0, sizeof(float) * kMlpNP));
+                RTOD_HIP(hipMemset(sh.adam_v, 0, sizeof(float) * kMlpNP));
+                RTOD_HIP(hipMemset(sh.ls, 0, sizeof(LearnShared)));
+                RTOD_HIP(hipStreamCreateWithFlags(&b->train_stream, hipStreamNonBlocking));
+                RTOD_HIP(hipEventCreateWithFlags(&b->train_ev, hipEventDisableTiming));
+            }
             return batch_create_members(b);
         }();
     } catch (...) {
@@ -1552,6 +1647,7 @@ RTOD_API rtod_status rtod_batch_prime(rtod_batch *b, const float *const *pts)
         }
     try {
         RTOD_HIP(hipSetDevice(b->device));
+        RTOD_TRY(batch_sync_trainer(b));
         for (int i = 0; i < b->n; ++i)
             RTOD_TRY(prime_impl(b->ctx[i], pts[i], b->cfg[i].window));
         b->t = 0;
@@ -1786,8 +1882,16 @@ RTOD_API void rtod_batch_destroy(rtod_batch *b)
     (void)hipSetDevice(b->device);
     if (b->stream)
         (void)hipStreamSynchronize(b->stream);
+    if (b->train_stream) {
+        (void)hipStreamSynchronize(b->train_stream);
+        (void)hipStreamDestroy(b->train_stream);
+    }
+    if (b->train_ev)
+        (void)hipEventDestroy(b->train_ev);
     batch_free_plan(b);
     batch_free_members(b);
+    for (void *p : b->shared_allocs)
+        (void)hipFree(p);
     if (b->stream)
         (void)hipStreamDestroy(b->stream);
     delete b;

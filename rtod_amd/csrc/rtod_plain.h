@@ -981,7 +981,9 @@ __device__ __forceinline__ void row_keys(const Stencil<DIM> &st, int r, uint64_t
     }
 }
 
-constexpr int kDetectNT = 256;
+// 8 wavefronts share one LDS copy of the sample table: about 24 resident
+// wavefronts per CU (VGPR-bound) instead of 16 with 256-thread workgroups
+constexpr int kDetectNT = 512;
 
 __device__ __forceinline__ void wave_sync_lds()
 {
@@ -1442,8 +1444,8 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
             HostOut *ho = d.h_out;
             ho->misses = miss;
             ho->used_pred = use_pred ? 1 : 0;
-            ho->train_steps = lc->steps_total;
-            ho->train_loss = lc->loss_last;
+            ho->train_steps = d.lr.ls->steps_total;
+            ho->train_loss = d.lr.ls->loss_last;
             // fallback rule: schedule the next slide by prediction only if
             // this slide's miss rate (over units) is within the threshold
             if (n_units > 0)

@@ -60,31 +60,35 @@ struct HostOut {
     int32_t list[1];       // W entries follow
 };
 
-// Device-side state of the learned scheduler.
+// Device-side state of the learned scheduler: per stream ...
 struct LearnCtl {
-    // cleared by rtod_prime
     int32_t n_units_i;     // predicted-inlier unit bucket size (current slide)
     int32_t use_pred;      // schedule the current slide by prediction
     int32_t misses;        // mispredictions (current slide)
-    float loss_last;       // mean loss of the last training workgroup
-    // kept across rtod_prime
+    int32_t pad;
+};
+
+// ... and of the predictor, which several streams may share (a batch).
+struct LearnShared {
+    float loss_last;       // mean loss of the last training launch
     int32_t adam_t;        // optimiser steps since the weights were set
     int32_t frozen;        // 1: no training
     int32_t steps_total;   // optimiser steps since creation
     int32_t pub;           // the current half of the published weights: pub & 1
+    int32_t pad[3];
 };
-constexpr int kLearnCtlResetBytes = 4 * 4;
 
 struct Learn {
     int on;                // learned schedule configured
     float miss_thr;        // fall back to the plain schedule above this rate
-    float *params;         // [kMlpNP] FP32 master weights (trainer's own)
+    float *params;         // [kMlpNP] FP32 master weights (the trainer's own)
     float *pub;            // [2][kMlpNP] published weights, double-buffered
     float *adam_m, *adam_v;
     float *rep_x;          // [kReplaySlides][W][kFeatStride] features by unit
     int32_t *rep_c;        // [kReplaySlides][W] the unit's minimum count, capped
     int32_t *rep_n;        // [kReplaySlides][2] bucket sizes
-    LearnCtl *lc;
+    LearnCtl *lc;          // this stream's
+    LearnShared *ls;       // the predictor's
 };
 
 // Per-slide quantities, passed by value to every kernel of the slide.

@@ -142,3 +142,21 @@ def test_batch_errors():
             b.step_device()
         with pytest.raises(R.RtodError):       # 3-D cell scale must stay 1
             b.set_tuning(64, (1.0, 1.0, 0.5))
+
+
+def test_learned_batch_exact_and_trains():
+    case("learned batch, 12 streams, device input")
+    T = 30
+    descs, data = _streams(MIXED, T, seed0=700)
+    want = _oracle(descs, data, T)
+    with R.Batch(descs, sched_kind=R.SCHED_LEARNED) as b:
+        ds = _bind(b, descs, data)
+        b.prime_device()
+        got = [b.step_device() for _ in range(T)]
+        stats = [b.stream_stats(i) for i in range(len(descs))]
+        b.close()
+        for d in ds:
+            d.close()
+    _check(got, want, "learned batch")
+    assert all(st["predictor_points"] > 0 for st in stats if st["candidates_total"] > 0)
+    assert max(st["train_steps"] for st in stats) > 0  # the shared predictor trained

@@ -109,13 +109,17 @@ def test_batch_create_rejects_bad_configs_without_device(streams, status):
     assert e.value.status == status
 
 
-def test_batch_rejects_learned_and_lbvh_without_device():
+def test_batch_rejects_lbvh_and_mixed_schedules_without_device():
     import numpy as np
     lib = R.lib()
-    for idx, sch in [(R.INDEX_LBVH, R.SCHED_PLAIN), (R.INDEX_AUTO, R.SCHED_LEARNED)]:
+    for idx, sch in [(R.INDEX_LBVH, R.SCHED_PLAIN), (R.INDEX_LBVH, R.SCHED_LEARNED)]:
         cfgs = (R._Config * 1)(R._Config(3, 1000, 100, 1.0, 5, idx, sch, 0.25, 0, 0))
         h = C.c_void_p()
         assert lib.rtod_batch_create(cfgs, 1, 0, C.byref(h)) == 2
+    # one schedule per batch
+    cfgs = (R._Config * 2)(R._Config(3, 1000, 100, 1.0, 5, 0, R.SCHED_PLAIN, 0.25, 0, 0),
+                           R._Config(3, 1000, 100, 1.0, 5, 0, R.SCHED_LEARNED, 0.25, 0, 0))
+    assert lib.rtod_batch_create(cfgs, 2, 0, C.byref(C.c_void_p())) == 2
     assert lib.rtod_batch_create(None, 1, 0, C.byref(C.c_void_p())) == 1
     assert lib.rtod_batch_prime(None, None) == 1
     z = np.zeros(4, np.int32)
