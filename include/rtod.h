@@ -138,6 +138,19 @@ RTOD_API rtod_status rtod_slide_device(rtod_ctx *ctx, const float *dev_new_pts,
                                        int32_t *outlier_slots, int32_t *n_outliers,
                                        rtod_times *times);
 
+/* n_slides consecutive slides from HBM: slide i takes the S points at
+ * dev_pts + i*S*dim.  No counterpart in the reference, whose loop
+ * (optixScan.cpp:1045-1110) runs one slide per iteration; this runs the same
+ * slides software-pipelined: slide i+1's new batch is sorted on a second HIP
+ * stream while slide i is merged, classified and counted, and the host stays
+ * one slide ahead.  Results equal those of n_slides rtod_slide_device calls:
+ * slide i's outliers (ring slots, ascending) are
+ * out_slots[out_offsets[i] .. out_offsets[i+1]); out_offsets holds
+ * n_slides + 1 entries and out_slots at least n_slides * W (slots_cap). */
+RTOD_API rtod_status rtod_slides_device(rtod_ctx *ctx, const float *dev_pts, int32_t n_slides,
+                                        int32_t *out_offsets, int32_t *out_slots,
+                                        int64_t slots_cap);
+
 /* Phase-level entry points: the same slide split where the reference's loop
  * body calls out.  Call in this order; rtod_slide == all five.
  *   rtod_copy_new_points  <- cudaMemcpy of the new slide (optixScan.cpp:1051-1058)

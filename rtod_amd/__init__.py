@@ -83,6 +83,8 @@ SIGNATURES = [
                              C.POINTER(_Times)]),
     ("rtod_slide_device", C.c_int, [C.c_void_p, C.c_void_p, _I32P, C.POINTER(C.c_int32),
                                     C.POINTER(_Times)]),
+    ("rtod_slides_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, _I32P, _I32P,
+                                     C.c_int64]),
     ("rtod_copy_new_points", C.c_int, [C.c_void_p, _F32P]),
     ("rtod_prepare_cell", C.c_int, [C.c_void_p]),
     ("rtod_build_index", C.c_int, [C.c_void_p]),
@@ -206,6 +208,15 @@ class Detector:
         _check(lib().rtod_slide_device(self._h, C.c_void_p(dev_ptr), self._out, C.byref(n),
                                        C.byref(self._times)))
         return self._out[:n.value].copy()
+
+    def slides_device(self, dev_ptr: int, n_slides: int) -> list[np.ndarray]:
+        """n_slides consecutive slides from HBM (slide i at dev_ptr + i*S*dim
+        floats), pipelined; the same lists as n_slides slide_device() calls."""
+        offs = np.zeros(n_slides + 1, np.int32)
+        slots = np.empty(max(1, n_slides * self.W), np.int32)
+        _check(lib().rtod_slides_device(self._h, C.c_void_p(dev_ptr), n_slides, offs, slots,
+                                        slots.size))
+        return [slots[offs[i]:offs[i + 1]].copy() for i in range(n_slides)]
 
     @property
     def times(self) -> Times:

@@ -44,8 +44,9 @@ __device__ __forceinline__ Step batch_step(const BatchStream &bs)
     st.t = t;
     st.u = (int)(t % m);
     st.par = (int)(t & 1);
-    st.eold = (int)(t % (m + 1));
-    st.enew = (int)((m + t) % (m + 1));
+    const uint64_t ne = (uint64_t)d.nent;
+    st.eold = (int)(t % ne);
+    st.enew = (int)((m + t) % ne);
     st.buf = (int)(t & 1);
     st.src = bs.src_base ? bs.src_base + ((size_t)d.W + (size_t)t * d.S) * d.dim
                          : d.h_in + (size_t)st.buf * d.S * d.dim;
@@ -78,7 +79,7 @@ __global__ __launch_bounds__(kBatchNT) void k_batch_merge_pass(const BatchItem *
     const Step st = batch_step(bs);
     const int L = it.arg, src = it.pass & 1;
     const bool last = 2 * L >= d.S;
-    const size_t bxs = (size_t)(d.m + 1) * d.S;
+    const size_t bxs = (size_t)d.nent * d.S;
     merge_pass_body<DIM>(d, st, d.tkey[src], d.tslot[src], d.txyz[src], (size_t)d.tmpn, nullptr,
                          last ? d.bkey : d.tkey[src ^ 1], last ? d.bslot : d.tslot[src ^ 1],
                          last ? d.bxyz : d.txyz[src ^ 1], last ? bxs : (size_t)d.tmpn, nullptr,

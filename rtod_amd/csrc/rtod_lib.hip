@@ -58,26 +58,36 @@ enum KernelId {
 const char *kKernelNames[kKCount] = {"ingest_sort", "merge_pass", "merge_window", "classify",
                                      "detect",      "lbvh_build", "lbvh_query"};
 
-// LDS-sort tile shapes (threads x items per thread), picked by S and dim:
-// the smallest tile holding the whole batch, else the largest that fits
-// 160 KB of LDS (coordinates/counters pool, keys, 16-bit values)
+// LDS-sort tile shapes (threads x items per thread).  A batch of up to
+// kRunMergeCap points is sorted as tiles of 1024 points on as many
+// workgroups (one CU sorts a tile; a whole 8k-point tile on one CU costs
+// several microseconds per 4-bit pass), then merged by one k_merge_runs
+// launch.  Larger batches use the largest tile that fits 160 KB of LDS
+// (coordinates/counters pool, keys, 16-bit values) and pairwise merge passes.
 struct SortShape {
     int nt, it;
 };
-constexpr SortShape kSortShapes[4] = {{256, 2}, {512, 4}, {1024, 4}, {1024, 8}};
-constexpr int kBatchSortShape = 2; // batched streams: the 1024 x 4 tile
+constexpr SortShape kSortShapes[5] = {{256, 2}, {256, 4}, {512, 4}, {1024, 4}, {1024, 8}};
+constexpr int kBatchSortShape = 3; // batched streams: the 1024 x 4 tile
 static_assert(kSortShapes[kBatchSortShape].nt == kBatchSortNT &&
                   kSortShapes[kBatchSortShape].it == kBatchSortIT,
               "batched ingest tile");
+constexpr int kRunTileShape = 1; // 1024-point tiles merged by k_merge_runs
+static_assert(kSortShapes[kRunTileShape].nt * kSortShapes[kRunTileShape].it == kRunLen,
+              "runs merged by k_merge_runs");
 
 int pick_sort_shape(int S, int dim)
 {
-    const int top = dim == 3 ? 2 : 3; // (1024, 8) x 3 floats exceeds LDS
-    for (int i = 0; i <= top; ++i)
-        if (kSortShapes[i].nt * kSortShapes[i].it >= S)
-            return i;
-    return top;
+    const int one = kSortShapes[0].nt * kSortShapes[0].it;
+    if (S <= one)
+        return 0;
+    if (S <= kRunMergeCap)
+        return kRunTileShape;
+    return dim == 3 ? 3 : 4; // (1024, 8) x 3 floats exceeds LDS
 }
+
+// k_merge_runs covers a batch of several tiles in one launch
+bool use_run_merge(int S, int CH) { return CH == kRunLen && S > CH && S <= kRunMergeCap; }
 
 } // namespace
 
@@ -92,6 +102,8 @@ struct rtod_ctx {
     hipStream_t train_stream = nullptr; // learned: the predictor's training launches
     hipEvent_t train_ev = nullptr;
     bool train_pending = false;
+    int64_t train_t_next = 0; // the pending training launch skips replay slots
+                              // of slides train_t_next and train_t_next + 1
     Dev d{};
     float *d_prime = nullptr; // device copy of the initial window
     float *h_in = nullptr;
@@ -108,6 +120,10 @@ struct rtod_ctx {
     int phase = 0;         // phase API progress
     int64_t host_t = 0;
     hipEvent_t ev[8] = {};
+    // pipelined slides: the ingest stage's stream and per-parity events
+    // (ingest done, window merge done, slide done)
+    hipStream_t ing_stream = nullptr;
+    hipEvent_t ev_ing[2] = {}, ev_mw[2] = {}, ev_done[2] = {};
     double ksum[kKCount] = {};
     int64_t kcnt[kKCount] = {};
     // stats
@@ -154,10 +170,14 @@ template <int DIM> void launch_ingest(rtod_ctx *c, hipStream_t s, const Step &st
                            dst_batch);
         break;
     case 1:
-        hipLaunchKernelGGL((k_ingest_sort<DIM, 512, 4>), g, dim3(512), 0, s, c->d, st, mode, src,
+        hipLaunchKernelGGL((k_ingest_sort<DIM, 256, 4>), g, dim3(256), 0, s, c->d, st, mode, src,
                            dst_batch);
         break;
     case 2:
+        hipLaunchKernelGGL((k_ingest_sort<DIM, 512, 4>), g, dim3(512), 0, s, c->d, st, mode, src,
+                           dst_batch);
+        break;
+    case 3:
         hipLaunchKernelGGL((k_ingest_sort<DIM, 1024, 4>), g, dim3(1024), 0, s, c->d, st, mode,
                            src, dst_batch);
         break;
@@ -197,12 +217,24 @@ void launch_merge_pass(rtod_ctx *c, hipStream_t s, const Step &st, const uint64_
                            dxs, dp, n_total, seg_len, L, dst_batch, mode);
 }
 
-// merge passes turning chunk-sorted runs of length c->CH into sorted
-// segments of S (the batches), last pass into the batch ring
+// chunk-sorted runs of length c->CH into sorted segments of S (the
+// batches) in the batch ring: one k_merge_runs launch, or pairwise merge
+// passes for batches beyond its LDS staging
 void launch_batch_merges(rtod_ctx *c, hipStream_t s, const Step &st, int mode, int n_total)
 {
+    if (use_run_merge(c->S, c->CH)) {
+        const int wps = (c->S + kRunMergeNT - 1) / kRunMergeNT;
+        const dim3 g((n_total / c->S) * wps), b(kRunMergeNT);
+        if (c->dim == 1)
+            hipLaunchKernelGGL(k_merge_runs<1>, g, b, 0, s, c->d, st, mode);
+        else if (c->dim == 2)
+            hipLaunchKernelGGL(k_merge_runs<2>, g, b, 0, s, c->d, st, mode);
+        else
+            hipLaunchKernelGGL(k_merge_runs<3>, g, b, 0, s, c->d, st, mode);
+        return;
+    }
     int src = 0;
-    const size_t bxs = (size_t)(c->m + 1) * c->S;
+    const size_t bxs = (size_t)c->d.nent * c->S;
     for (int L = c->CH; L < c->S; L *= 2) {
         const bool last = 2 * L >= c->S;
         launch_merge_pass(c, s, st, c->d.tkey[src], c->d.tslot[src], c->d.txyz[src],
@@ -308,26 +340,39 @@ void launch_lbvh_detect(rtod_ctx *c, hipStream_t s, const Step &st)
     hipLaunchKernelGGL(k_lbvh_select, gw, dim3(kDetectNT), 0, s, c->d, st, c->L);
 }
 
-Step make_step(const rtod_ctx *c)
+// slide t's quantities; src: its new points in HBM, or nullptr for the
+// pinned staging buffer
+Step make_step_at(const rtod_ctx *c, uint64_t t, const float *src)
 {
     Step st{};
-    const uint64_t t = (uint64_t)c->host_t;
+    const uint64_t ne = (uint64_t)c->d.nent;
     st.t = t;
     st.u = (int)(t % (uint64_t)c->m);
     st.par = (int)(t & 1);
-    st.eold = (int)(t % (uint64_t)(c->m + 1));
-    st.enew = (int)(((uint64_t)c->m + t) % (uint64_t)(c->m + 1));
+    st.eold = (int)(t % ne);
+    st.enew = (int)(((uint64_t)c->m + t) % ne);
     st.buf = (int)(t & 1);
-    st.src = c->dev_src ? c->dev_src : c->h_in + (size_t)st.buf * c->S * c->dim;
+    st.src = src ? src : c->h_in + (size_t)st.buf * c->S * c->dim;
     return st;
 }
 
-// The slide's kernel sequence.  ev != nullptr: record an event before the
-// first kernel and after each one (ingest, batch merges, merge_window,
-// classify, detect).
-void launch_slide(rtod_ctx *c, hipStream_t s, hipEvent_t *ev)
+Step make_step(const rtod_ctx *c) { return make_step_at(c, (uint64_t)c->host_t, c->dev_src); }
+
+// the host-out buffer slide t writes (its parity)
+HostOut *host_out_of(const rtod_ctx *c, int64_t t)
 {
-    const Step st = make_step(c);
+    return reinterpret_cast<HostOut *>(reinterpret_cast<char *>(c->h_out) +
+                                       (size_t)(t & 1) * (size_t)c->d.h_out_bytes);
+}
+
+// The slide's kernel sequence in two stages.  Ingest stage: the new batch
+// sorted into its batch-ring entry (ingest, batch merges).  Core stage:
+// window merge, classify, [LBVH build], detect.  ev != nullptr: record an
+// event before the first kernel and after each one; ev_merged != nullptr:
+// record it after the window merge (the batch-ring entries it read are then
+// free for a later ingest).
+void launch_ingest_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t *ev)
+{
     if (ev)
         (void)hipEventRecord(ev[0], s);
     launch_ingest_dim(c, s, st, kModeSlide, nullptr, c->d.cpb, c->d.cpb == 1 ? 1 : 0);
@@ -337,9 +382,16 @@ void launch_slide(rtod_ctx *c, hipStream_t s, hipEvent_t *ev)
         launch_batch_merges(c, s, st, kModeSlide, c->S);
     if (ev)
         (void)hipEventRecord(ev[2], s);
+}
+
+void launch_core_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t *ev,
+                       hipEvent_t ev_merged)
+{
     launch_merge_window(c, s, st);
     if (ev)
         (void)hipEventRecord(ev[3], s);
+    if (ev_merged)
+        (void)hipEventRecord(ev_merged, s);
     launch_classify(c, s, st);
     if (ev)
         (void)hipEventRecord(ev[4], s);
@@ -357,6 +409,13 @@ void launch_slide(rtod_ctx *c, hipStream_t s, hipEvent_t *ev)
             (void)hipEventRecord(ev[6], s);
         }
     }
+}
+
+void launch_slide(rtod_ctx *c, hipStream_t s, hipEvent_t *ev)
+{
+    const Step st = make_step(c);
+    launch_ingest_stage(c, s, st, ev);
+    launch_core_stage(c, s, st, ev, nullptr);
 }
 
 rtod_status validate(const rtod_config *cfg)
@@ -503,10 +562,12 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
         RTOD_TRY(dalloc(c, &d.tslot[p], tmpn));
         RTOD_TRY(dalloc(c, &d.txyz[p], dim * tmpn));
     }
-    RTOD_TRY(dalloc(c, &d.bkey, (m + 1) * S));
-    RTOD_TRY(dalloc(c, &d.bslot, (m + 1) * S));
-    RTOD_TRY(dalloc(c, &d.bsamp, (m + 1) * (size_t)d.nbs));
-    RTOD_TRY(dalloc(c, &d.bxyz, dim * (m + 1) * S));
+    d.nent = (int)m + 2;
+    const size_t ne = (size_t)d.nent;
+    RTOD_TRY(dalloc(c, &d.bkey, ne * S));
+    RTOD_TRY(dalloc(c, &d.bslot, ne * S));
+    RTOD_TRY(dalloc(c, &d.bsamp, ne * (size_t)d.nbs));
+    RTOD_TRY(dalloc(c, &d.bxyz, dim * ne * S));
     RTOD_TRY(dalloc(c, &d.samples[0], (size_t)d.nsamp));
     RTOD_TRY(dalloc(c, &d.samples[1], (size_t)d.nsamp));
     RTOD_TRY(dalloc(c, &d.cand, W));
@@ -567,14 +628,17 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
         RTOD_HIP(hipMemsetAsync(lr.adam_v, 0, sizeof(float) * kMlpNP, c->stream));
     }
     const size_t in_bytes = 2 * S * dim * sizeof(float);
-    const size_t out_bytes = sizeof(HostOut) + W * sizeof(int32_t);
+    // two host-out buffers (slide parity), each aligned to 256 bytes
+    const size_t out_bytes = (sizeof(HostOut) + W * sizeof(int32_t) + 255) & ~(size_t)255;
     RTOD_HIP(hipHostMalloc((void **)&c->h_in, in_bytes, hipHostMallocMapped | hipHostMallocCoherent));
-    RTOD_HIP(hipHostMalloc((void **)&c->h_out, out_bytes, hipHostMallocMapped | hipHostMallocCoherent));
-    c->pinned_bytes = (int64_t)(in_bytes + out_bytes);
+    RTOD_HIP(hipHostMalloc((void **)&c->h_out, 2 * out_bytes,
+                           hipHostMallocMapped | hipHostMallocCoherent));
+    c->pinned_bytes = (int64_t)(in_bytes + 2 * out_bytes);
     std::memset(c->h_in, 0, in_bytes);
-    std::memset((void *)c->h_out, 0, out_bytes);
+    std::memset((void *)c->h_out, 0, 2 * out_bytes);
     d.h_in = c->h_in;
     d.h_out = c->h_out;
+    d.h_out_bytes = (int64_t)out_bytes;
     RTOD_HIP(hipStreamSynchronize(c->stream));
     return RTOD_OK;
 }
@@ -602,7 +666,7 @@ rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n,
     // sorted window A[0]; batch positions ride along: the first pass derives
     // them from the index inside the batch ring, later passes copy them
     {
-        const size_t bxs = (size_t)(c->m + 1) * c->S;
+        const size_t bxs = (size_t)c->d.nent * c->S;
         const uint64_t *sk = c->d.bkey;
         const int32_t *ss = c->d.bslot;
         const float *sx = c->d.bxyz;
@@ -640,7 +704,8 @@ rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n,
     }
     RTOD_HIP(hipGetLastError());
     RTOD_HIP(hipStreamSynchronize(s));
-    c->h_out->slide = 0;
+    host_out_of(c, 0)->slide = 0;
+    host_out_of(c, 1)->slide = 0;
     c->host_t = 0;
     c->primed = true;
     c->staged = false;
@@ -650,7 +715,7 @@ rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n,
 
 rtod_status collect(rtod_ctx *c, int32_t *outs, int32_t *n_out)
 {
-    HostOut *ho = c->h_out;
+    HostOut *ho = host_out_of(c, c->host_t);
     if (ho->slide != c->host_t + 1) {
         set_error("device slide counter out of step with the host");
         return RTOD_ERR_INTERNAL;
@@ -744,6 +809,16 @@ rtod_status maybe_train(rtod_ctx *c)
     RTOD_HIP(hipGetLastError());
     RTOD_HIP(hipEventRecord(c->train_ev, c->train_stream));
     c->train_pending = true;
+    c->train_t_next = c->host_t;
+    return RTOD_OK;
+}
+
+// Slide t rewrites its replay slot (classify, detect on stream s): past the
+// pending training launch's two skipped slides, s first waits for it.
+rtod_status guard_trainer(rtod_ctx *c, int64_t t, hipStream_t s)
+{
+    if (c->train_pending && t >= c->train_t_next + 2)
+        RTOD_HIP(hipStreamWaitEvent(s, c->train_ev, 0));
     return RTOD_OK;
 }
 
@@ -754,6 +829,7 @@ rtod_status run_slide(rtod_ctx *c, double t_entry, double t_h2d, int32_t *outs, 
 {
     double phase[3] = {0, 0, 0};
     const bool timed = (c->cfg.flags & RTOD_FLAG_PHASE_TIMES) != 0;
+    RTOD_TRY(guard_trainer(c, c->host_t, c->stream));
     launch_slide(c, c->stream, timed ? c->ev : nullptr);
     RTOD_HIP(hipGetLastError());
     RTOD_HIP(hipStreamSynchronize(c->stream));
@@ -818,6 +894,82 @@ rtod_status slide_device_impl(rtod_ctx *c, const float *dev_pts, int32_t *outs, 
     c->dev_src = dev_pts;
     rtod_status s = run_slide(c, t_entry, t_entry, outs, n_out, times);
     c->dev_src = nullptr;
+    return s;
+}
+
+rtod_status ensure_pipeline(rtod_ctx *c)
+{
+    if (c->ing_stream)
+        return RTOD_OK;
+    RTOD_HIP(hipStreamCreateWithFlags(&c->ing_stream, hipStreamNonBlocking));
+    for (int p = 0; p < 2; ++p) {
+        RTOD_HIP(hipEventCreateWithFlags(&c->ev_ing[p], hipEventDisableTiming));
+        RTOD_HIP(hipEventCreateWithFlags(&c->ev_mw[p], hipEventDisableTiming));
+        RTOD_HIP(hipEventCreateWithFlags(&c->ev_done[p], hipEventDisableTiming));
+    }
+    return RTOD_OK;
+}
+
+// Slide t = t0 + i of a pipelined run.  Its ingest stage goes on the ingest
+// stream, after slide t-2's window merge has read the batch-ring entry it
+// fills (enew(t) == eold(t-2) with m + 2 entries); its core stage goes on
+// the context's stream after its ingest.  Host-out buffer: t's parity.
+rtod_status enqueue_pipelined(rtod_ctx *c, int64_t t0, int64_t i, const float *src)
+{
+    const uint64_t t = (uint64_t)(t0 + i);
+    const int p = (int)(t & 1);
+    const Step st = make_step_at(c, t, src);
+    if (i >= 2)
+        RTOD_HIP(hipStreamWaitEvent(c->ing_stream, c->ev_mw[p], 0));
+    launch_ingest_stage(c, c->ing_stream, st, nullptr);
+    RTOD_HIP(hipEventRecord(c->ev_ing[p], c->ing_stream));
+    RTOD_HIP(hipStreamWaitEvent(c->stream, c->ev_ing[p], 0));
+    RTOD_TRY(guard_trainer(c, (int64_t)t, c->stream));
+    launch_core_stage(c, c->stream, st, nullptr, c->ev_mw[p]);
+    RTOD_HIP(hipEventRecord(c->ev_done[p], c->stream));
+    RTOD_HIP(hipGetLastError());
+    return RTOD_OK;
+}
+
+// n slides from HBM with slide i+1's ingest beside slide i's core stage and
+// the host one slide ahead: after slide i finishes, its list is collected
+// and slide i+2 is enqueued.  The results are those of n rtod_slide_device
+// calls.  On an error both streams are drained before returning.
+rtod_status slides_device_impl(rtod_ctx *c, const float *dev, int32_t n, int32_t *offs,
+                               int32_t *slots)
+{
+    RTOD_TRY(ensure_pipeline(c));
+    const int64_t t0 = c->host_t;
+    const size_t per = (size_t)c->S * c->dim;
+    rtod_status s = RTOD_OK;
+    int32_t enq = 0;
+    for (; enq < n && enq < 2 && s == RTOD_OK; ++enq)
+        s = enqueue_pipelined(c, t0, enq, dev + (size_t)enq * per);
+    int64_t used = 0;
+    offs[0] = 0;
+    for (int32_t i = 0; i < n && s == RTOD_OK; ++i) {
+        const hipError_t e = hipEventSynchronize(c->ev_done[(t0 + i) & 1]);
+        if (e != hipSuccess) {
+            set_error(std::string("HIP error: ") + hipGetErrorString(e));
+            s = RTOD_ERR_HIP;
+            break;
+        }
+        int32_t k = 0;
+        s = collect(c, slots + used, &k);
+        if (s != RTOD_OK)
+            break;
+        used += k;
+        offs[i + 1] = (int32_t)used;
+        s = maybe_train(c);
+        if (s == RTOD_OK && enq < n) {
+            s = enqueue_pipelined(c, t0, enq, dev + (size_t)enq * per);
+            ++enq;
+        }
+    }
+    if (s != RTOD_OK) {
+        (void)hipStreamSynchronize(c->ing_stream);
+        (void)hipStreamSynchronize(c->stream);
+    }
     return s;
 }
 
@@ -893,6 +1045,42 @@ RTOD_API rtod_status rtod_slide_device(rtod_ctx *ctx, const float *dev_new_pts,
     }
 }
 
+RTOD_API rtod_status rtod_slides_device(rtod_ctx *ctx, const float *dev_pts, int32_t n_slides,
+                                        int32_t *out_offsets, int32_t *out_slots,
+                                        int64_t slots_cap)
+{
+    if (!ctx || !dev_pts || !out_offsets || !out_slots) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (n_slides < 1 || slots_cap < (int64_t)n_slides * ctx->W) {
+        set_error("n_slides must be >= 1 and out_slots hold n_slides * W entries");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (!ctx->primed || ctx->staged) {
+        set_error(!ctx->primed ? "rtod_slides_device before rtod_prime"
+                               : "a phase-API slide is in progress");
+        return RTOD_ERR_STATE;
+    }
+    if (!ctx->own_stream) {
+        set_error("a batch member's slides run through its batch");
+        return RTOD_ERR_UNSUPPORTED;
+    }
+    try {
+        RTOD_HIP(hipSetDevice(ctx->device));
+        hipPointerAttribute_t attr{};
+        if (hipPointerGetAttributes(&attr, dev_pts) != hipSuccess ||
+            attr.type != hipMemoryTypeDevice) {
+            (void)hipGetLastError();
+            set_error("rtod_slides_device needs a device (HBM) pointer");
+            return RTOD_ERR_INVALID_ARGUMENT;
+        }
+        return slides_device_impl(ctx, dev_pts, n_slides, out_offsets, out_slots);
+    } catch (...) {
+        return RTOD_ERR_INTERNAL;
+    }
+}
+
 RTOD_API rtod_status rtod_copy_new_points(rtod_ctx *c, const float *new_pts)
 {
     if (!c || !new_pts) {
@@ -921,6 +1109,7 @@ RTOD_API rtod_status rtod_prepare_cell(rtod_ctx *c)
         return RTOD_ERR_STATE;
     }
     RTOD_HIP(hipSetDevice(c->device));
+    RTOD_TRY(guard_trainer(c, c->host_t, c->stream));
     const Step st = make_step(c);
     launch_ingest_dim(c, c->stream, st, kModeSlide, nullptr, c->d.cpb, c->d.cpb == 1 ? 1 : 0);
     if (c->d.cpb > 1)
@@ -1202,6 +1391,14 @@ RTOD_API void rtod_destroy(rtod_ctx *c)
     for (auto &e : c->ev)
         if (e)
             (void)hipEventDestroy(e);
+    if (c->ing_stream) {
+        (void)hipStreamSynchronize(c->ing_stream);
+        (void)hipStreamDestroy(c->ing_stream);
+    }
+    for (int p = 0; p < 2; ++p)
+        for (hipEvent_t e : {c->ev_ing[p], c->ev_mw[p], c->ev_done[p]})
+            if (e)
+                (void)hipEventDestroy(e);
     for (void *p : c->dev_allocs)
         (void)hipFree(p);
     if (c->h_in)
@@ -1270,6 +1467,7 @@ struct rtod_batch {
     hipStream_t train_stream = nullptr;
     hipEvent_t train_ev = nullptr;
     bool train_pending = false;
+    int64_t train_t_next = 0; // the pending launch skips steps train_t_next, +1
 };
 
 namespace {
@@ -1318,6 +1516,7 @@ rtod_status batch_maybe_train(rtod_batch *b)
     RTOD_HIP(hipGetLastError());
     RTOD_HIP(hipEventRecord(b->train_ev, b->train_stream));
     b->train_pending = true;
+    b->train_t_next = b->t;
     return RTOD_OK;
 }
 
@@ -1515,6 +1714,10 @@ rtod_status batch_launch_collect(rtod_batch *b, int32_t *offsets, int32_t *slots
 {
     if (!b->plan_ok)
         RTOD_TRY(batch_build_plan(b));
+    // a step past the pending training launch's skipped replay slots waits
+    // for it before rewriting one
+    if (b->train_pending && b->t >= b->train_t_next + 2)
+        RTOD_HIP(hipStreamWaitEvent(b->stream, b->train_ev, 0));
     RTOD_HIP(hipGraphLaunch(b->exec, b->stream));
     RTOD_HIP(hipStreamSynchronize(b->stream));
     int64_t off = 0;

@@ -300,7 +300,7 @@ __device__ __forceinline__ void ingest_sort_body(const Dev &d, const Step &st, i
     }
 
     // output in sorted order, striped (coalesced)
-    const size_t stride_b = (size_t)(d.m + 1) * d.S;
+    const size_t stride_b = (size_t)d.nent * d.S;
     const int e = (mode == kModeSlide) ? st.enew : seg;
     uint64_t *ok;
     int32_t *os;
@@ -425,6 +425,89 @@ __global__ __launch_bounds__(256) void k_merge_pass(Dev d, Step st, const uint64
                          dst_batch, mode, blockIdx.x);
 }
 
+// ---------------------------------------------------------------------------
+// k_merge_runs: one launch turning each batch segment's tile-sorted runs
+// (length L = kRunLen, the last one shorter) into the sorted batch: ring
+// entry and sample table.  The segment's keys (S <= kRunMergeCap) are
+// staged in LDS.  Element i of run r lands at
+//     (i - r*L) + sum_{r' < r} #(run r' <= key) + sum_{r' > r} #(run r' < key)
+// which is stable: an earlier run holds earlier arrivals.  One element per
+// thread; its counts in all the other runs are found together, bit by bit
+// from L down (the count is the largest c whose element c-1 passes), so each
+// of the log2(L)+1 steps issues up to kRunMax independent LDS reads.  Slots
+// and coordinates are loaded before the staging barrier.
+// ---------------------------------------------------------------------------
+constexpr int kRunMergeNT = 1024, kRunLen = 1024, kRunMergeCap = 8192;
+constexpr int kRunMax = kRunMergeCap / kRunLen;
+
+template <int DIM>
+__device__ __forceinline__ void merge_runs_body(const Dev &d, const Step &st, int mode, int bid,
+                                                uint64_t *s_k)
+{
+    constexpr int L = kRunLen;
+    const int S = d.S;
+    const int wps = (S + kRunMergeNT - 1) / kRunMergeNT; // workgroups per segment
+    const int seg = bid / wps, w = bid % wps;
+    const size_t sxs = (size_t)d.tmpn;
+    const uint64_t *sk = d.tkey[0] + (size_t)seg * S;
+    const int i = w * kRunMergeNT + threadIdx.x;
+    const bool on = i < S;
+    int32_t sl = 0;
+    float cx[3] = {0.f, 0.f, 0.f};
+    if (on) {
+        sl = d.tslot[0][(size_t)seg * S + i];
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+            cx[a] = d.txyz[0][(size_t)a * sxs + (size_t)seg * S + i];
+    }
+    copy_to_lds<kRunMergeCap / kRunMergeNT>(s_k, sk, S);
+    __syncthreads();
+    if (!on)
+        return;
+    const uint64_t key = s_k[i];
+    const int r = i / L;
+    const int nruns = (S + L - 1) / L;
+    int c[kRunMax], len[kRunMax];
+#pragma unroll
+    for (int q = 0; q < kRunMax; ++q) {
+        c[q] = 0;
+        len[q] = (q < nruns && q != r) ? min(L, S - q * L) : 0;
+    }
+#pragma unroll
+    for (int b = L; b >= 1; b >>= 1) {
+        uint64_t x[kRunMax];
+#pragma unroll
+        for (int q = 0; q < kRunMax; ++q)
+            x[q] = (c[q] + b <= len[q]) ? s_k[q * L + c[q] + b - 1] : ~0ull;
+#pragma unroll
+        for (int q = 0; q < kRunMax; ++q) {
+            const bool pass = (c[q] + b <= len[q]) && (q < r ? x[q] <= key : x[q] < key);
+            c[q] += pass ? b : 0;
+        }
+    }
+    int o = i - r * L;
+#pragma unroll
+    for (int q = 0; q < kRunMax; ++q)
+        o += c[q];
+    const int e = (mode == kModeSlide) ? st.enew : seg;
+    const size_t bxs = (size_t)d.nent * S;
+    const size_t ob = (size_t)e * S + o;
+    d.bkey[ob] = key;
+    d.bslot[ob] = sl;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a)
+        d.bxyz[(size_t)a * bxs + ob] = cx[a];
+    if ((o & ((1 << d.bs_log2) - 1)) == 0)
+        d.bsamp[(size_t)e * d.nbs + (o >> d.bs_log2)] = key;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kRunMergeNT) void k_merge_runs(Dev d, Step st, int mode)
+{
+    __shared__ uint64_t s_k[kRunMergeCap];
+    merge_runs_body<DIM>(d, st, mode, blockIdx.x, s_k);
+}
+
 // sample table of a freshly built sorted window
 __global__ __launch_bounds__(256) void k_window_samples(Dev d, int par)
 {
@@ -519,7 +602,7 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
     const uint64_t *Bk = d.bkey + (size_t)st.enew * S;
     const int32_t *Bs = d.bslot + (size_t)st.enew * S;
     const float *Bx = d.bxyz + (size_t)st.enew * S;
-    const size_t bxs = (size_t)(d.m + 1) * S;
+    const size_t bxs = (size_t)d.nent * S;
     const uint64_t *Esamp = d.bsamp + (size_t)st.eold * d.nbs;
     const uint64_t *Bsamp = d.bsamp + (size_t)st.enew * d.nbs;
     const int exp_lo = st.u * S;
@@ -1252,7 +1335,7 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int n
     const int nwords = (d.W + 31) >> 5;
     const int per = (nwords + kDetectNT - 1) / kDetectNT;
     const int w0 = min(nwords, tid * per), w1 = min(nwords, w0 + per);
-    HostOut *ho = d.h_out;
+    HostOut *ho = host_out(d, st.t);
     int total = 0;
     if (per <= kCompactRegs) {
         uint32_t wv[kCompactRegs];
@@ -1441,7 +1524,7 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
                 __hip_atomic_load(&lc->misses, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             d.lr.rep_n[2 * slot] = n_o;
             d.lr.rep_n[2 * slot + 1] = n_units - n_o;
-            HostOut *ho = d.h_out;
+            HostOut *ho = host_out(d, st.t);
             ho->misses = miss;
             ho->used_pred = use_pred ? 1 : 0;
             ho->train_steps = d.lr.ls->steps_total;
@@ -1454,7 +1537,7 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
         }
     }
     if (tid == 0)
-        d.h_out->n_units = n_units;
+        host_out(d, st.t)->n_units = n_units;
     finish_slide(d, st, d.ctl->n_cand, s_scan);
 }
 

@@ -104,6 +104,8 @@ struct Step {
 
 struct Dev {
     int dim, W, S, m, K;
+    int nent;         // batch-ring entries, m + 2: the next slide's batch can be
+                      // ingested while this slide's merge reads its two
     int cpb;          // sort chunks per batch
     int nsamp;        // samples in the sorted-window sample table
     int stride_log2;  // sample stride
@@ -132,8 +134,17 @@ struct Dev {
     uint32_t *bitmap;
     Ctl *ctl;
     const float *h_in;
-    HostOut *h_out;
+    HostOut *h_out;      // two host-out buffers, by slide parity
+    int64_t h_out_bytes; // their stride
     Learn lr;
 };
+
+// the host-out buffer of slide t: consecutive slides alternate, so a slide
+// can finish while the host still reads the previous one's list
+__device__ __forceinline__ HostOut *host_out(const Dev &d, uint64_t t)
+{
+    return reinterpret_cast<HostOut *>(reinterpret_cast<char *>(d.h_out) +
+                                       (size_t)(t & 1) * (size_t)d.h_out_bytes);
+}
 
 } // namespace rtod

@@ -74,6 +74,19 @@ def test_2d_vs_brute(kind):
                 f"2-D {kind}")
 
 
+@pytest.mark.parametrize("dim,W,S", [(1, 2050, 1025), (2, 12291, 4097), (3, 16384, 8192),
+                                     (1, 24576, 8192)])
+def test_multi_tile_batches_vs_grid(dim, W, S):
+    # batches of several 1024-point sort tiles, merged in one pass (two runs
+    # with a one-point tail, five runs, eight full runs), priming included
+    case(f"multi-tile dim={dim} W={W} S={S}")
+    T = 4
+    xs = O.gen_walk(61 + dim, W + T * S, dim) if dim < 3 else O.gen_gmm(64, W + T * S, 3)
+    Rr = {1: 0.45, 2: 0.4, 3: 1.9}[dim]
+    got = R.run_stream(xs, W, S, T, Rr, 30)
+    assert_same(got, O.run_stream(xs, W, S, T, Rr, 30, "grid"), f"multi-tile dim={dim} S={S}")
+
+
 @pytest.mark.parametrize("W,S,K", [(1000, 1000, 5), (999, 1, 3), (4096, 64, 4096),
                                    (5000, 2500, 1), (40000, 20000, 50)])
 def test_shapes_vs_oracle(W, S, K):
@@ -200,6 +213,67 @@ def test_slide_device_equals_slide():
             det.slide_device(xs.ctypes.data)
         assert e.value.status == 1
     assert_same(got, want, "slide_device")
+
+
+PIPE_CASES = [
+    ("C2", O.C2, 40, {}),
+    ("C3", O.C3, 60, {}),
+    ("C1 learned", O.C1, 60, dict(sched_kind=R.SCHED_LEARNED)),
+    ("C3 lbvh", O.C3, 12, dict(index_kind=R.INDEX_LBVH)),
+]
+
+
+@pytest.mark.parametrize("name,wl,T,kw", PIPE_CASES, ids=[c[0] for c in PIPE_CASES])
+def test_pipelined_slides_vs_grid(name, wl, T, kw):
+    """rtod_slides_device (the next slide's ingest on a second stream, the
+    host one slide ahead) against the grid oracle, mixed with single slides
+    on the same context: 3 single, T-8 pipelined, 2 single, 3 pipelined."""
+    case(f"pipelined {name}")
+    xs = wl.data(wl.W + T * wl.S)
+    want = O.run_stream(xs, wl.W, wl.S, T, wl.R, wl.K, "grid")
+    W, S = wl.W, wl.S
+    got = []
+    with R.DeviceStream(xs) as ds, R.Detector(wl.dim, W, S, wl.R, wl.K, **kw) as det:
+        det.prime(xs[:W])
+        t = 0
+
+        def single(n):
+            nonlocal t
+            for _ in range(n):
+                got.append(det.slide_device(ds.at(W + t * S)))
+                t += 1
+
+        def piped(n):
+            nonlocal t
+            got.extend(det.slides_device(ds.at(W + t * S), n))
+            t += n
+
+        single(3)
+        piped(T - 8)
+        single(2)
+        piped(3)
+        assert det.stats()["slides"] == T
+    assert_same(got, want, f"pipelined {name}")
+
+
+def test_pipelined_slides_argument_checks():
+    case("pipelined args")
+    W, S = 2000, 200
+    xs = O.gen_walk(81, W + 4 * S)
+    with R.DeviceStream(xs) as ds, R.Detector(1, W, S, 0.45, 10) as det:
+        offs = np.zeros(3, np.int32)
+        slots = np.zeros(2 * W, np.int32)
+        L = R.lib()
+        # before prime
+        assert L.rtod_slides_device(det._h, ds.at(W), 2, offs, slots, slots.size) == 3
+        det.prime(xs[:W])
+        assert L.rtod_slides_device(det._h, ds.at(W), 2, offs, slots, slots.size - 1) == 1
+        assert L.rtod_slides_device(det._h, ds.at(W), 0, offs, slots, slots.size) == 1
+        assert L.rtod_slides_device(det._h, xs.ctypes.data, 2, offs, slots, slots.size) == 1
+        # nothing ran: the stream continues from slide 1
+        assert det.stats()["slides"] == 0
+        got = det.slides_device(ds.at(W), 2)
+    assert_same(got, O.run_stream(xs, W, S, 2, 0.45, 10, "grid"), "pipelined after errors")
 
 
 def test_reprime_restarts_stream():
