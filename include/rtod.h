@@ -141,9 +141,11 @@ RTOD_API rtod_status rtod_slide_device(rtod_ctx *ctx, const float *dev_new_pts,
 /* n_slides consecutive slides from HBM: slide i takes the S points at
  * dev_pts + i*S*dim.  No counterpart in the reference, whose loop
  * (optixScan.cpp:1045-1110) runs one slide per iteration; this runs the same
- * slides software-pipelined: slide i+1's new batch is sorted on a second HIP
- * stream while slide i is merged, classified and counted, and the host stays
- * one slide ahead.  Results equal those of n_slides rtod_slide_device calls:
+ * slides software-pipelined: slide i+1's new batch is sorted by extra
+ * workgroups of slide i's kernel launches (S <= 8192; larger slides sort
+ * between them), up to four slides are queued, and the host collects each
+ * slide's list as it completes.  Results equal those of n_slides
+ * rtod_slide_device calls:
  * slide i's outliers (ring slots, ascending) are
  * out_slots[out_offsets[i] .. out_offsets[i+1]); out_offsets holds
  * n_slides + 1 entries and out_slots at least n_slides * W (slots_cap). */

@@ -118,9 +118,8 @@ __global__ __launch_bounds__(kDetectNT) void k_batch_detect(const BatchItem *ite
 
 // The batch's shared predictor (learned batches), trained on every stream's
 // replay: a stream is drawn with probability proportional to its stored
-// units, then one of them uniformly; the slots of slides t_next and
-// t_next+1 are skipped (the next steps may be rewriting them).  n <=
-// kTrainMaxStreams.
+// units, then one of them uniformly; the slots of the steps from t_next on
+// are skipped (replay_skipped).  n <= kTrainMaxStreams.
 __global__ __launch_bounds__(kTrainNT) void k_train_batch(const BatchStream *streams, int n,
                                                          Learn shared, uint64_t t_next,
                                                          int steps)
@@ -133,7 +132,6 @@ __global__ __launch_bounds__(kTrainNT) void k_train_batch(const BatchStream *str
     if (shared.ls->frozen)
         return;
     const int tid = threadIdx.x;
-    const int skip0 = (int)(t_next % kReplaySlides), skip1 = (int)((t_next + 1) % kReplaySlides);
     int tot[PER], mine = 0;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -142,7 +140,7 @@ __global__ __launch_bounds__(kTrainNT) void k_train_batch(const BatchStream *str
         if (i < n) {
             const int32_t *rn = streams[i].d.lr.rep_n;
             for (int j = 0; j < kReplaySlides; ++j)
-                if (j != skip0 && j != skip1)
+                if (!replay_skipped(j, t_next))
                     tot[q] += rn[2 * j] + rn[2 * j + 1];
         }
         mine += tot[q];
@@ -185,7 +183,7 @@ __global__ __launch_bounds__(kTrainNT) void k_train_batch(const BatchStream *str
             for (int h2 = 0; h2 < 2; ++h2) {
                 const int j = 2 * k + h2;
                 const int o = h2 ? rn4[k].z : rn4[k].x, in = h2 ? rn4[k].w : rn4[k].y;
-                const int c = (j == skip0 || j == skip1) ? 0 : o + in;
+                const int c = replay_skipped(j, t_next) ? 0 : o + in;
                 if (off >= 0 && off < c) {
                     slot = j;
                     n_o = o;

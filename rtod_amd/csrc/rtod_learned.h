@@ -502,8 +502,21 @@ __device__ __forceinline__ void load_replay_row(const Learn &L, int W, int K, in
     y = mlp_target(L.rep_c[e], K);
 }
 
-// One stream's replay, uniform over its stored units, slots of slides
-// t_next and t_next+1 skipped (the next slides may be rewriting them).
+// Replay slots a training launch leaves alone: those of slides t_next ..
+// t_next + kReplaySkip - 1, which may be queued or running while it trains
+// (up to kHostOuts slides are in flight); a slide past them waits for the
+// launch before it rewrites its slot.
+constexpr int kReplaySkip = kHostOuts;
+static_assert(kReplaySkip < kReplaySlides, "replay left to train on");
+
+__device__ __forceinline__ bool replay_skipped(int j, uint64_t t_next)
+{
+    const int first = (int)(t_next % kReplaySlides);
+    return (j - first + kReplaySlides) % kReplaySlides < kReplaySkip;
+}
+
+// One stream's replay, uniform over its stored units, the slots of the
+// slides from t_next on skipped (replay_skipped).
 __global__ __launch_bounds__(kTrainNT) void k_train(Dev d, uint64_t t_next, int steps)
 {
     __shared__ TrainWaveLds lds_wave[kTrainNT / kWave];
@@ -511,12 +524,11 @@ __global__ __launch_bounds__(kTrainNT) void k_train(Dev d, uint64_t t_next, int 
     const Learn &L = d.lr;
     if (L.ls->frozen)
         return;
-    const int skip0 = (int)(t_next % kReplaySlides), skip1 = (int)((t_next + 1) % kReplaySlides);
     if (threadIdx.x == 0) {
         int acc = 0;
         s.pref[0] = 0;
         for (int j = 0; j < kReplaySlides; ++j) {
-            if (j != skip0 && j != skip1)
+            if (!replay_skipped(j, t_next))
                 acc += L.rep_n[2 * j] + L.rep_n[2 * j + 1];
             s.pref[j + 1] = acc;
         }

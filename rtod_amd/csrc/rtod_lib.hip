@@ -89,6 +89,14 @@ int pick_sort_shape(int S, int dim)
 // k_merge_runs covers a batch of several tiles in one launch
 bool use_run_merge(int S, int CH) { return CH == kRunLen && S > CH && S <= kRunMergeCap; }
 
+// the next slide's ingest stage can ride in this slide's core launches:
+// 256-thread sort tiles (the merge's workgroup size), at most one merge
+// launch (k_merge_runs, the detect's workgroup size)
+bool ride_along(int sort_shape, int S, int CH)
+{
+    return kSortShapes[sort_shape].nt == 256 && (S <= CH || use_run_merge(S, CH));
+}
+
 } // namespace
 
 struct rtod_ctx {
@@ -102,8 +110,8 @@ struct rtod_ctx {
     hipStream_t train_stream = nullptr; // learned: the predictor's training launches
     hipEvent_t train_ev = nullptr;
     bool train_pending = false;
-    int64_t train_t_next = 0; // the pending training launch skips replay slots
-                              // of slides train_t_next and train_t_next + 1
+    int64_t train_t_next = 0; // the pending training launch skips the replay
+                              // slots of kReplaySkip slides from this one
     Dev d{};
     float *d_prime = nullptr; // device copy of the initial window
     float *h_in = nullptr;
@@ -120,10 +128,7 @@ struct rtod_ctx {
     int phase = 0;         // phase API progress
     int64_t host_t = 0;
     hipEvent_t ev[8] = {};
-    // pipelined slides: the ingest stage's stream and per-parity events
-    // (ingest done, window merge done, slide done)
-    hipStream_t ing_stream = nullptr;
-    hipEvent_t ev_ing[2] = {}, ev_mw[2] = {}, ev_done[2] = {};
+    bool ride = false; // pipelined slides: next ingest rides in core launches
     double ksum[kKCount] = {};
     int64_t kcnt[kKCount] = {};
     // stats
@@ -223,8 +228,7 @@ void launch_merge_pass(rtod_ctx *c, hipStream_t s, const Step &st, const uint64_
 void launch_batch_merges(rtod_ctx *c, hipStream_t s, const Step &st, int mode, int n_total)
 {
     if (use_run_merge(c->S, c->CH)) {
-        const int wps = (c->S + kRunMergeNT - 1) / kRunMergeNT;
-        const dim3 g((n_total / c->S) * wps), b(kRunMergeNT);
+        const dim3 g((n_total / c->S) * run_merge_wgs(c->S)), b(kRunMergeNT);
         if (c->dim == 1)
             hipLaunchKernelGGL(k_merge_runs<1>, g, b, 0, s, c->d, st, mode);
         else if (c->dim == 2)
@@ -246,15 +250,33 @@ void launch_batch_merges(rtod_ctx *c, hipStream_t s, const Step &st, int mode, i
     }
 }
 
-void launch_merge_window(rtod_ctx *c, hipStream_t s, const Step &st)
+// slide st's window merge; next != nullptr: the next slide's tile sorts
+// ride along as extra workgroups (ride_along)
+template <int DIM>
+void launch_merge_window_dim(rtod_ctx *c, hipStream_t s, const Step &st, const Step *next)
 {
-    const int grid = (c->W + 1023) / 1024;
-    if (c->dim == 1)
-        hipLaunchKernelGGL(k_merge_window<1>, dim3(grid), dim3(256), 0, s, c->d, st);
-    else if (c->dim == 2)
-        hipLaunchKernelGGL(k_merge_window<2>, dim3(grid), dim3(256), 0, s, c->d, st);
+    const int nmb = (c->W + 1023) / 1024;
+    if (!next) {
+        hipLaunchKernelGGL(k_merge_window<DIM>, dim3(nmb), dim3(256), 0, s, c->d, st);
+        return;
+    }
+    const dim3 g(nmb + c->d.cpb);
+    if (c->sort_shape == 0)
+        hipLaunchKernelGGL((k_merge_window_ingest<DIM, 2>), g, dim3(256), 0, s, c->d, st, *next,
+                           nmb);
     else
-        hipLaunchKernelGGL(k_merge_window<3>, dim3(grid), dim3(256), 0, s, c->d, st);
+        hipLaunchKernelGGL((k_merge_window_ingest<DIM, 4>), g, dim3(256), 0, s, c->d, st, *next,
+                           nmb);
+}
+
+void launch_merge_window(rtod_ctx *c, hipStream_t s, const Step &st, const Step *next = nullptr)
+{
+    if (c->dim == 1)
+        launch_merge_window_dim<1>(c, s, st, next);
+    else if (c->dim == 2)
+        launch_merge_window_dim<2>(c, s, st, next);
+    else
+        launch_merge_window_dim<3>(c, s, st, next);
 }
 
 template <int DIM> void launch_classify_dim(rtod_ctx *c, hipStream_t s, const Step &st)
@@ -281,23 +303,36 @@ void launch_classify(rtod_ctx *c, hipStream_t s, const Step &st)
         launch_classify_dim<3>(c, s, st);
 }
 
-template <int DIM> void launch_detect_dim(rtod_ctx *c, hipStream_t s, const Step &st)
+// slide st's detect; runs != nullptr: that slide's run merge rides along as
+// extra workgroups
+template <int DIM>
+void launch_detect_dim(rtod_ctx *c, hipStream_t s, const Step &st, const Step *runs)
 {
     const int grid = c->detect_blocks;
+    if (runs) {
+        const dim3 g(grid + run_merge_wgs(c->S));
+        if (c->learned)
+            hipLaunchKernelGGL((k_detect_runs<DIM, true>), g, dim3(kDetectNT), 0, s, c->d, st,
+                               *runs, grid);
+        else
+            hipLaunchKernelGGL((k_detect_runs<DIM, false>), g, dim3(kDetectNT), 0, s, c->d, st,
+                               *runs, grid);
+        return;
+    }
     if (c->learned)
         hipLaunchKernelGGL((k_detect<DIM, true>), dim3(grid), dim3(kDetectNT), 0, s, c->d, st);
     else
         hipLaunchKernelGGL((k_detect<DIM, false>), dim3(grid), dim3(kDetectNT), 0, s, c->d, st);
 }
 
-void launch_detect(rtod_ctx *c, hipStream_t s, const Step &st)
+void launch_detect(rtod_ctx *c, hipStream_t s, const Step &st, const Step *runs = nullptr)
 {
     if (c->dim == 1)
-        launch_detect_dim<1>(c, s, st);
+        launch_detect_dim<1>(c, s, st, runs);
     else if (c->dim == 2)
-        launch_detect_dim<2>(c, s, st);
+        launch_detect_dim<2>(c, s, st, runs);
     else
-        launch_detect_dim<3>(c, s, st);
+        launch_detect_dim<3>(c, s, st, runs);
 }
 
 // LBVH index: codes, tile sort, merges up to U = W, hierarchy, boxes
@@ -358,19 +393,17 @@ Step make_step_at(const rtod_ctx *c, uint64_t t, const float *src)
 
 Step make_step(const rtod_ctx *c) { return make_step_at(c, (uint64_t)c->host_t, c->dev_src); }
 
-// the host-out buffer slide t writes (its parity)
+// the host-out buffer slide t writes
 HostOut *host_out_of(const rtod_ctx *c, int64_t t)
 {
     return reinterpret_cast<HostOut *>(reinterpret_cast<char *>(c->h_out) +
-                                       (size_t)(t & 1) * (size_t)c->d.h_out_bytes);
+                                       (size_t)(t % kHostOuts) * (size_t)c->d.h_out_bytes);
 }
 
 // The slide's kernel sequence in two stages.  Ingest stage: the new batch
 // sorted into its batch-ring entry (ingest, batch merges).  Core stage:
 // window merge, classify, [LBVH build], detect.  ev != nullptr: record an
-// event before the first kernel and after each one; ev_merged != nullptr:
-// record it after the window merge (the batch-ring entries it read are then
-// free for a later ingest).
+// event before the first kernel and after each one.
 void launch_ingest_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t *ev)
 {
     if (ev)
@@ -384,17 +417,19 @@ void launch_ingest_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t 
         (void)hipEventRecord(ev[2], s);
 }
 
+// next != nullptr (pipelined slides, ride_along): the next slide's ingest
+// stage rides in this stage's launches -- its tile sorts in the window
+// merge's, its run merge in the detect's (after the LBVH detect on its own).
 void launch_core_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t *ev,
-                       hipEvent_t ev_merged)
+                       const Step *next)
 {
-    launch_merge_window(c, s, st);
+    launch_merge_window(c, s, st, next);
     if (ev)
         (void)hipEventRecord(ev[3], s);
-    if (ev_merged)
-        (void)hipEventRecord(ev_merged, s);
     launch_classify(c, s, st);
     if (ev)
         (void)hipEventRecord(ev[4], s);
+    const bool runs = next && c->d.cpb > 1;
     if (c->lbvh) {
         launch_lbvh_build(c, s, st);
         if (ev)
@@ -402,8 +437,10 @@ void launch_core_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t *e
         launch_lbvh_detect(c, s, st);
         if (ev)
             (void)hipEventRecord(ev[6], s);
+        if (runs)
+            launch_batch_merges(c, s, *next, kModeSlide, c->S);
     } else {
-        launch_detect(c, s, st);
+        launch_detect(c, s, st, runs ? next : nullptr);
         if (ev) {
             (void)hipEventRecord(ev[5], s);
             (void)hipEventRecord(ev[6], s);
@@ -524,6 +561,7 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
 
     c->sort_shape = opt.sort_shape >= 0 ? opt.sort_shape : pick_sort_shape(c->S, c->dim);
     c->CH = kSortShapes[c->sort_shape].nt * kSortShapes[c->sort_shape].it;
+    c->ride = ride_along(c->sort_shape, c->S, c->CH);
 
     Dev &d = c->d;
     d.dim = c->dim;
@@ -628,14 +666,15 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
         RTOD_HIP(hipMemsetAsync(lr.adam_v, 0, sizeof(float) * kMlpNP, c->stream));
     }
     const size_t in_bytes = 2 * S * dim * sizeof(float);
-    // two host-out buffers (slide parity), each aligned to 256 bytes
+    // kHostOuts host-out buffers (slide t: t % kHostOuts), each aligned to
+    // 256 bytes
     const size_t out_bytes = (sizeof(HostOut) + W * sizeof(int32_t) + 255) & ~(size_t)255;
     RTOD_HIP(hipHostMalloc((void **)&c->h_in, in_bytes, hipHostMallocMapped | hipHostMallocCoherent));
-    RTOD_HIP(hipHostMalloc((void **)&c->h_out, 2 * out_bytes,
+    RTOD_HIP(hipHostMalloc((void **)&c->h_out, kHostOuts * out_bytes,
                            hipHostMallocMapped | hipHostMallocCoherent));
-    c->pinned_bytes = (int64_t)(in_bytes + 2 * out_bytes);
+    c->pinned_bytes = (int64_t)(in_bytes + kHostOuts * out_bytes);
     std::memset(c->h_in, 0, in_bytes);
-    std::memset((void *)c->h_out, 0, 2 * out_bytes);
+    std::memset((void *)c->h_out, 0, kHostOuts * out_bytes);
     d.h_in = c->h_in;
     d.h_out = c->h_out;
     d.h_out_bytes = (int64_t)out_bytes;
@@ -704,8 +743,8 @@ rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n,
     }
     RTOD_HIP(hipGetLastError());
     RTOD_HIP(hipStreamSynchronize(s));
-    host_out_of(c, 0)->slide = 0;
-    host_out_of(c, 1)->slide = 0;
+    for (int k = 0; k < kHostOuts; ++k)
+        host_out_of(c, k)->slide = 0;
     c->host_t = 0;
     c->primed = true;
     c->staged = false;
@@ -814,11 +853,21 @@ rtod_status maybe_train(rtod_ctx *c)
 }
 
 // Slide t rewrites its replay slot (classify, detect on stream s): past the
-// pending training launch's two skipped slides, s first waits for it.
+// pending training launch's skipped slides (kReplaySkip), s first waits for
+// it.
 rtod_status guard_trainer(rtod_ctx *c, int64_t t, hipStream_t s)
 {
-    if (c->train_pending && t >= c->train_t_next + 2)
-        RTOD_HIP(hipStreamWaitEvent(s, c->train_ev, 0));
+    if (!c->train_pending || t < c->train_t_next + kReplaySkip)
+        return RTOD_OK;
+    const hipError_t q = hipEventQuery(c->train_ev);
+    if (q == hipSuccess) {
+        c->train_pending = false;
+        return RTOD_OK;
+    }
+    if (q != hipErrorNotReady)
+        RTOD_HIP(q);
+    (void)hipGetLastError(); // still running: not an error
+    RTOD_HIP(hipStreamWaitEvent(s, c->train_ev, 0));
     return RTOD_OK;
 }
 
@@ -897,79 +946,79 @@ rtod_status slide_device_impl(rtod_ctx *c, const float *dev_pts, int32_t *outs, 
     return s;
 }
 
-rtod_status ensure_pipeline(rtod_ctx *c)
+// Wait for slide t's stamp in its host-out buffer (written last, after a
+// system-scope release, by the slide's detect).  The stream is queried now
+// and then, so a failed launch surfaces as an error instead of a hang.
+rtod_status wait_slide(rtod_ctx *c, int64_t t)
 {
-    if (c->ing_stream)
-        return RTOD_OK;
-    RTOD_HIP(hipStreamCreateWithFlags(&c->ing_stream, hipStreamNonBlocking));
-    for (int p = 0; p < 2; ++p) {
-        RTOD_HIP(hipEventCreateWithFlags(&c->ev_ing[p], hipEventDisableTiming));
-        RTOD_HIP(hipEventCreateWithFlags(&c->ev_mw[p], hipEventDisableTiming));
-        RTOD_HIP(hipEventCreateWithFlags(&c->ev_done[p], hipEventDisableTiming));
+    const int64_t *stamp = &host_out_of(c, t)->slide;
+    const int64_t want = t + 1;
+    for (uint64_t spin = 1;; ++spin) {
+        if (__atomic_load_n(stamp, __ATOMIC_ACQUIRE) == want)
+            return RTOD_OK;
+        if ((spin & 1023) == 0) {
+            const hipError_t q = hipStreamQuery(c->stream);
+            if (q == hipSuccess) {
+                if (__atomic_load_n(stamp, __ATOMIC_ACQUIRE) == want)
+                    return RTOD_OK;
+                set_error("the stream went idle without completing the slide");
+                return RTOD_ERR_INTERNAL;
+            }
+            if (q != hipErrorNotReady) {
+                set_error(std::string("HIP error: ") + hipGetErrorString(q));
+                return RTOD_ERR_HIP;
+            }
+            (void)hipGetLastError(); // not ready: not an error
+        }
+        __builtin_ia32_pause();
     }
-    return RTOD_OK;
 }
 
-// Slide t = t0 + i of a pipelined run.  Its ingest stage goes on the ingest
-// stream, after slide t-2's window merge has read the batch-ring entry it
-// fills (enew(t) == eold(t-2) with m + 2 entries); its core stage goes on
-// the context's stream after its ingest.  Host-out buffer: t's parity.
-rtod_status enqueue_pipelined(rtod_ctx *c, int64_t t0, int64_t i, const float *src)
-{
-    const uint64_t t = (uint64_t)(t0 + i);
-    const int p = (int)(t & 1);
-    const Step st = make_step_at(c, t, src);
-    if (i >= 2)
-        RTOD_HIP(hipStreamWaitEvent(c->ing_stream, c->ev_mw[p], 0));
-    launch_ingest_stage(c, c->ing_stream, st, nullptr);
-    RTOD_HIP(hipEventRecord(c->ev_ing[p], c->ing_stream));
-    RTOD_HIP(hipStreamWaitEvent(c->stream, c->ev_ing[p], 0));
-    RTOD_TRY(guard_trainer(c, (int64_t)t, c->stream));
-    launch_core_stage(c, c->stream, st, nullptr, c->ev_mw[p]);
-    RTOD_HIP(hipEventRecord(c->ev_done[p], c->stream));
-    RTOD_HIP(hipGetLastError());
-    return RTOD_OK;
-}
-
-// n slides from HBM with slide i+1's ingest beside slide i's core stage and
-// the host one slide ahead: after slide i finishes, its list is collected
-// and slide i+2 is enqueued.  The results are those of n rtod_slide_device
-// calls.  On an error both streams are drained before returning.
+// n slides from HBM on the context's stream, up to kHostOuts in flight: the
+// host enqueues slide i once slide i - kHostOuts is collected (polling its
+// stamp), and slide i+1's ingest stage rides in slide i's core launches
+// (ride_along; else it follows them).  The results are those of n
+// rtod_slide_device calls.  On an error the stream is drained first.
 rtod_status slides_device_impl(rtod_ctx *c, const float *dev, int32_t n, int32_t *offs,
                                int32_t *slots)
 {
-    RTOD_TRY(ensure_pipeline(c));
     const int64_t t0 = c->host_t;
     const size_t per = (size_t)c->S * c->dim;
-    rtod_status s = RTOD_OK;
-    int32_t enq = 0;
-    for (; enq < n && enq < 2 && s == RTOD_OK; ++enq)
-        s = enqueue_pipelined(c, t0, enq, dev + (size_t)enq * per);
     int64_t used = 0;
+    int32_t done = 0;
     offs[0] = 0;
-    for (int32_t i = 0; i < n && s == RTOD_OK; ++i) {
-        const hipError_t e = hipEventSynchronize(c->ev_done[(t0 + i) & 1]);
-        if (e != hipSuccess) {
-            set_error(std::string("HIP error: ") + hipGetErrorString(e));
-            s = RTOD_ERR_HIP;
-            break;
-        }
+    auto collect_next = [&]() -> rtod_status {
+        RTOD_TRY(wait_slide(c, c->host_t));
         int32_t k = 0;
-        s = collect(c, slots + used, &k);
-        if (s != RTOD_OK)
-            break;
+        RTOD_TRY(collect(c, slots + used, &k));
         used += k;
-        offs[i + 1] = (int32_t)used;
-        s = maybe_train(c);
-        if (s == RTOD_OK && enq < n) {
-            s = enqueue_pipelined(c, t0, enq, dev + (size_t)enq * per);
-            ++enq;
+        offs[++done] = (int32_t)used;
+        return maybe_train(c);
+    };
+    rtod_status s = RTOD_OK;
+    launch_ingest_stage(c, c->stream, make_step_at(c, (uint64_t)t0, dev), nullptr);
+    for (int32_t i = 0; i < n && s == RTOD_OK; ++i) {
+        if (i >= kHostOuts && (s = collect_next()) != RTOD_OK)
+            break;
+        const uint64_t t = (uint64_t)(t0 + i);
+        const Step st = make_step_at(c, t, dev + (size_t)i * per);
+        const bool more = i + 1 < n;
+        const Step nx = more ? make_step_at(c, t + 1, dev + (size_t)(i + 1) * per) : Step{};
+        if ((s = guard_trainer(c, (int64_t)t, c->stream)) != RTOD_OK)
+            break;
+        launch_core_stage(c, c->stream, st, nullptr, more && c->ride ? &nx : nullptr);
+        if (more && !c->ride)
+            launch_ingest_stage(c, c->stream, nx, nullptr);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            set_error(std::string("launch: ") + hipGetErrorString(e));
+            s = RTOD_ERR_HIP;
         }
     }
-    if (s != RTOD_OK) {
-        (void)hipStreamSynchronize(c->ing_stream);
+    while (s == RTOD_OK && done < n)
+        s = collect_next();
+    if (s != RTOD_OK)
         (void)hipStreamSynchronize(c->stream);
-    }
     return s;
 }
 
@@ -1391,14 +1440,6 @@ RTOD_API void rtod_destroy(rtod_ctx *c)
     for (auto &e : c->ev)
         if (e)
             (void)hipEventDestroy(e);
-    if (c->ing_stream) {
-        (void)hipStreamSynchronize(c->ing_stream);
-        (void)hipStreamDestroy(c->ing_stream);
-    }
-    for (int p = 0; p < 2; ++p)
-        for (hipEvent_t e : {c->ev_ing[p], c->ev_mw[p], c->ev_done[p]})
-            if (e)
-                (void)hipEventDestroy(e);
     for (void *p : c->dev_allocs)
         (void)hipFree(p);
     if (c->h_in)
@@ -1467,7 +1508,7 @@ struct rtod_batch {
     hipStream_t train_stream = nullptr;
     hipEvent_t train_ev = nullptr;
     bool train_pending = false;
-    int64_t train_t_next = 0; // the pending launch skips steps train_t_next, +1
+    int64_t train_t_next = 0; // the pending launch skips kReplaySkip steps from here
 };
 
 namespace {
@@ -1716,7 +1757,7 @@ rtod_status batch_launch_collect(rtod_batch *b, int32_t *offsets, int32_t *slots
         RTOD_TRY(batch_build_plan(b));
     // a step past the pending training launch's skipped replay slots waits
     // for it before rewriting one
-    if (b->train_pending && b->t >= b->train_t_next + 2)
+    if (b->train_pending && b->t >= b->train_t_next + kReplaySkip)
         RTOD_HIP(hipStreamWaitEvent(b->stream, b->train_ev, 0));
     RTOD_HIP(hipGraphLaunch(b->exec, b->stream));
     RTOD_HIP(hipStreamSynchronize(b->stream));

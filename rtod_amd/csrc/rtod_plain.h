@@ -431,74 +431,100 @@ __global__ __launch_bounds__(256) void k_merge_pass(Dev d, Step st, const uint64
 // entry and sample table.  The segment's keys (S <= kRunMergeCap) are
 // staged in LDS.  Element i of run r lands at
 //     (i - r*L) + sum_{r' < r} #(run r' <= key) + sum_{r' > r} #(run r' < key)
-// which is stable: an earlier run holds earlier arrivals.  One element per
-// thread; its counts in all the other runs are found together, bit by bit
-// from L down (the count is the largest c whose element c-1 passes), so each
-// of the log2(L)+1 steps issues up to kRunMax independent LDS reads.  Slots
-// and coordinates are loaded before the staging barrier.
+// which is stable: an earlier run holds earlier arrivals.  A thread places
+// kRunMergeIT elements (strided by the workgroup size); an element's counts
+// in all the other runs are found together, bit by bit from L down (the
+// count is the largest c whose element c-1 passes), so each of the
+// log2(L)+1 steps issues up to kRunMax independent LDS reads per element.
+// Slots and coordinates are loaded before the staging barrier.
 // ---------------------------------------------------------------------------
-constexpr int kRunMergeNT = 1024, kRunLen = 1024, kRunMergeCap = 8192;
+constexpr int kRunMergeNT = 512, kRunMergeIT = 2, kRunLen = 1024, kRunMergeCap = 8192;
 constexpr int kRunMax = kRunMergeCap / kRunLen;
+constexpr int kRunMergeT = kRunMergeNT * kRunMergeIT; // elements per workgroup
+
+// workgroups of k_merge_runs per batch segment of S points
+__host__ __device__ constexpr int run_merge_wgs(int S) { return (S + kRunMergeT - 1) / kRunMergeT; }
 
 template <int DIM>
 __device__ __forceinline__ void merge_runs_body(const Dev &d, const Step &st, int mode, int bid,
                                                 uint64_t *s_k)
 {
-    constexpr int L = kRunLen;
+    constexpr int L = kRunLen, IT = kRunMergeIT;
     const int S = d.S;
-    const int wps = (S + kRunMergeNT - 1) / kRunMergeNT; // workgroups per segment
+    const int wps = run_merge_wgs(S);
     const int seg = bid / wps, w = bid % wps;
     const size_t sxs = (size_t)d.tmpn;
     const uint64_t *sk = d.tkey[0] + (size_t)seg * S;
-    const int i = w * kRunMergeNT + threadIdx.x;
-    const bool on = i < S;
-    int32_t sl = 0;
-    float cx[3] = {0.f, 0.f, 0.f};
-    if (on) {
-        sl = d.tslot[0][(size_t)seg * S + i];
+    int32_t sl[IT];
+    float cx[IT][3];
 #pragma unroll
-        for (int a = 0; a < DIM; ++a)
-            cx[a] = d.txyz[0][(size_t)a * sxs + (size_t)seg * S + i];
+    for (int k = 0; k < IT; ++k) {
+        const int i = w * kRunMergeT + k * kRunMergeNT + threadIdx.x;
+        sl[k] = 0;
+        cx[k][0] = cx[k][1] = cx[k][2] = 0.f;
+        if (i < S) {
+            sl[k] = d.tslot[0][(size_t)seg * S + i];
+#pragma unroll
+            for (int a = 0; a < DIM; ++a)
+                cx[k][a] = d.txyz[0][(size_t)a * sxs + (size_t)seg * S + i];
+        }
     }
     copy_to_lds<kRunMergeCap / kRunMergeNT>(s_k, sk, S);
     __syncthreads();
-    if (!on)
-        return;
-    const uint64_t key = s_k[i];
-    const int r = i / L;
     const int nruns = (S + L - 1) / L;
-    int c[kRunMax], len[kRunMax];
+    const int e = (mode == kModeSlide) ? st.enew : seg;
+    const size_t bxs = (size_t)d.nent * S;
+    // the elements' searches advance together: IT * kRunMax independent
+    // LDS reads per step
+    uint64_t key[IT];
+    int r[IT], c[IT][kRunMax], len[IT][kRunMax];
 #pragma unroll
-    for (int q = 0; q < kRunMax; ++q) {
-        c[q] = 0;
-        len[q] = (q < nruns && q != r) ? min(L, S - q * L) : 0;
+    for (int k = 0; k < IT; ++k) {
+        const int i = w * kRunMergeT + k * kRunMergeNT + threadIdx.x;
+        const bool on = i < S;
+        key[k] = on ? s_k[i] : 0ull;
+        r[k] = on ? i / L : 0;
+#pragma unroll
+        for (int q = 0; q < kRunMax; ++q) {
+            c[k][q] = 0;
+            len[k][q] = (on && q < nruns && q != r[k]) ? min(L, S - q * L) : 0;
+        }
     }
 #pragma unroll
     for (int b = L; b >= 1; b >>= 1) {
-        uint64_t x[kRunMax];
+        uint64_t x[IT][kRunMax];
+#pragma unroll
+        for (int k = 0; k < IT; ++k)
+#pragma unroll
+            for (int q = 0; q < kRunMax; ++q)
+                x[k][q] = (c[k][q] + b <= len[k][q]) ? s_k[q * L + c[k][q] + b - 1] : ~0ull;
+#pragma unroll
+        for (int k = 0; k < IT; ++k)
+#pragma unroll
+            for (int q = 0; q < kRunMax; ++q) {
+                const bool pass = (c[k][q] + b <= len[k][q]) &&
+                                  (q < r[k] ? x[k][q] <= key[k] : x[k][q] < key[k]);
+                c[k][q] += pass ? b : 0;
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+        const int i = w * kRunMergeT + k * kRunMergeNT + threadIdx.x;
+        if (i >= S)
+            break;
+        int o = i - r[k] * L;
 #pragma unroll
         for (int q = 0; q < kRunMax; ++q)
-            x[q] = (c[q] + b <= len[q]) ? s_k[q * L + c[q] + b - 1] : ~0ull;
+            o += c[k][q];
+        const size_t ob = (size_t)e * S + o;
+        d.bkey[ob] = key[k];
+        d.bslot[ob] = sl[k];
 #pragma unroll
-        for (int q = 0; q < kRunMax; ++q) {
-            const bool pass = (c[q] + b <= len[q]) && (q < r ? x[q] <= key : x[q] < key);
-            c[q] += pass ? b : 0;
-        }
+        for (int a = 0; a < DIM; ++a)
+            d.bxyz[(size_t)a * bxs + ob] = cx[k][a];
+        if ((o & ((1 << d.bs_log2) - 1)) == 0)
+            d.bsamp[(size_t)e * d.nbs + (o >> d.bs_log2)] = key[k];
     }
-    int o = i - r * L;
-#pragma unroll
-    for (int q = 0; q < kRunMax; ++q)
-        o += c[q];
-    const int e = (mode == kModeSlide) ? st.enew : seg;
-    const size_t bxs = (size_t)d.nent * S;
-    const size_t ob = (size_t)e * S + o;
-    d.bkey[ob] = key;
-    d.bslot[ob] = sl;
-#pragma unroll
-    for (int a = 0; a < DIM; ++a)
-        d.bxyz[(size_t)a * bxs + ob] = cx[a];
-    if ((o & ((1 << d.bs_log2) - 1)) == 0)
-        d.bsamp[(size_t)e * d.nbs + (o >> d.bs_log2)] = key;
 }
 
 template <int DIM>
@@ -1381,6 +1407,10 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int n
             }
         }
     }
+    // the slide stamp is written last, after one system-scope release by the
+    // thread that writes it (cumulative over the workgroup's list writes,
+    // ordered before it by the barrier): a host that sees the stamp
+    // (polling, acquire) sees the list and the counters
     __syncthreads();
     if (tid == 0) {
         const unsigned long long pt =
@@ -1389,12 +1419,13 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int n
         ho->n_cand = n_cand;
         ho->pair_tests = pt;
         ho->err = __hip_atomic_load(&d.ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ho->slide = (int64_t)(st.t + 1);
         d.ctl->err = 0;
         d.ctl->pair_tests = 0;
         d.ctl->ticket = 0;
         d.ctl->t = st.t + 1;
         __threadfence_system();
+        __hip_atomic_store(&ho->slide, (int64_t)(st.t + 1), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -1546,6 +1577,44 @@ __global__ __launch_bounds__(kDetectNT) void k_detect(Dev d, Step st)
 {
     __shared__ DetectLds lds;
     detect_body<DIM, LEARN>(d, st, blockIdx.x, gridDim.x, lds);
+}
+
+// ---------------------------------------------------------------------------
+// Core-stage launches carrying the next slide's ingest stage in extra
+// workgroups (pipelined slides on one stream): slide t's window merge with
+// slide t+1's tile sorts, slide t's detect with slide t+1's run merge.  The
+// halves touch disjoint memory: the ingest writes batch-ring entry enew(t+1)
+// and the tile scratch; slide t's merge reads entries enew(t) and eold(t)
+// (distinct from enew(t+1) with m + 2 entries) and its detect the window.
+// ---------------------------------------------------------------------------
+template <int DIM, int IT>
+__global__ __launch_bounds__(256) void k_merge_window_ingest(Dev d, Step st, Step st_next,
+                                                            int nmb)
+{
+    __shared__ union {
+        MergeWindowLds mw;
+        IngestLds<DIM, 256, IT> ing;
+    } lds;
+    if ((int)blockIdx.x < nmb)
+        merge_window_body<DIM>(d, st, blockIdx.x, lds.mw);
+    else
+        ingest_sort_body<DIM, 256, IT>(d, st_next, kModeSlide, nullptr, d.cpb == 1 ? 1 : 0,
+                                       blockIdx.x - nmb, lds.ing);
+}
+
+static_assert(kRunMergeNT == kDetectNT, "the run merge rides in detect workgroups");
+
+template <int DIM, bool LEARN>
+__global__ __launch_bounds__(kDetectNT) void k_detect_runs(Dev d, Step st, Step st_next, int ndet)
+{
+    __shared__ union {
+        DetectLds det;
+        uint64_t runs[kRunMergeCap];
+    } lds;
+    if ((int)blockIdx.x < ndet)
+        detect_body<DIM, LEARN>(d, st, blockIdx.x, ndet, lds.det);
+    else
+        merge_runs_body<DIM>(d, st_next, kModeSlide, blockIdx.x - ndet, lds.runs);
 }
 
 } // namespace rtod

@@ -134,17 +134,19 @@ struct Dev {
     uint32_t *bitmap;
     Ctl *ctl;
     const float *h_in;
-    HostOut *h_out;      // two host-out buffers, by slide parity
+    HostOut *h_out;      // kHostOuts host-out buffers, slide t uses t % kHostOuts
     int64_t h_out_bytes; // their stride
     Learn lr;
 };
 
-// the host-out buffer of slide t: consecutive slides alternate, so a slide
-// can finish while the host still reads the previous one's list
+// the host-out buffer of slide t: consecutive slides rotate through
+// kHostOuts buffers, so up to that many slides can be in flight while the
+// host reads the oldest one's list
+constexpr int kHostOuts = 4;
 __device__ __forceinline__ HostOut *host_out(const Dev &d, uint64_t t)
 {
     return reinterpret_cast<HostOut *>(reinterpret_cast<char *>(d.h_out) +
-                                       (size_t)(t & 1) * (size_t)d.h_out_bytes);
+                                       (size_t)(t % kHostOuts) * (size_t)d.h_out_bytes);
 }
 
 } // namespace rtod
