@@ -608,6 +608,11 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
     RTOD_TRY(dalloc(c, &d.bxyz, dim * ne * S));
     RTOD_TRY(dalloc(c, &d.samples[0], (size_t)d.nsamp));
     RTOD_TRY(dalloc(c, &d.samples[1], (size_t)d.nsamp));
+    d.nsamp4 = d.stride_log2 > 2 ? (int)(((int64_t)c->W + 3) >> 2) : 0;
+    if (d.nsamp4) {
+        RTOD_TRY(dalloc(c, &d.samples4[0], (size_t)d.nsamp4));
+        RTOD_TRY(dalloc(c, &d.samples4[1], (size_t)d.nsamp4));
+    }
     RTOD_TRY(dalloc(c, &d.cand, W));
     RTOD_TRY(dalloc(c, &d.units, W));
     RTOD_TRY(dalloc(c, &d.cand_cnt, W));
@@ -733,7 +738,8 @@ rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n,
             L *= 2;
         }
     }
-    hipLaunchKernelGGL(k_window_samples, dim3((c->d.nsamp + 255) / 256), dim3(256), 0, s, c->d,
+    hipLaunchKernelGGL(k_window_samples,
+                       dim3((std::max(c->d.nsamp, c->d.nsamp4) + 255) / 256), dim3(256), 0, s, c->d,
                        0);
     RTOD_HIP(hipMemsetAsync(c->d.bitmap, 0, sizeof(uint32_t) * ((c->W + 31) / 32), s));
     RTOD_HIP(hipMemsetAsync(c->d.ctl, 0, sizeof(Ctl), s));

@@ -540,6 +540,8 @@ __global__ __launch_bounds__(256) void k_window_samples(Dev d, int par)
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < d.nsamp)
         d.samples[par][s] = d.akey[par][(size_t)s << d.stride_log2];
+    if (s < d.nsamp4)
+        d.samples4[par][s] = d.akey[par][(size_t)s << 2];
 }
 
 // Wave-cooperative bound over a sorted table in LDS (n <= 4096 entries): a
@@ -624,6 +626,7 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
     int32_t *Np = d.apos[par ^ 1];
     float *Nx = d.axyz[par ^ 1];
     uint64_t *Nsamp = d.samples[par ^ 1];
+    uint64_t *Nsamp4 = d.samples4[par ^ 1];
     const uint64_t *Ek = d.bkey + (size_t)st.eold * S;
     const uint64_t *Bk = d.bkey + (size_t)st.enew * S;
     const int32_t *Bs = d.bslot + (size_t)st.enew * S;
@@ -750,6 +753,24 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
         copy_to_lds<8>(s_br, Bk + b_lo, nb);
     __syncthreads();
 
+    // B rank of each of the thread's items (#B < key), the IT searches of the
+    // staged range advancing together bit by bit (independent LDS reads)
+    int rbl[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i)
+        rbl[i] = 0;
+    if (bfit) {
+        for (int b = nb > 0 ? 1 << (31 - __builtin_clz(nb)) : 0; b >= 1; b >>= 1) {
+            uint64_t x[IT];
+#pragma unroll
+            for (int i = 0; i < IT; ++i)
+                x[i] = (rbl[i] + b <= nb) ? s_br[rbl[i] + b - 1] : ~0ull;
+#pragma unroll
+            for (int i = 0; i < IT; ++i)
+                rbl[i] += (rbl[i] + b <= nb && x[i] < kk[i]) ? b : 0;
+        }
+    }
+
     // survivors of A; #E before element l = e_before + expired elements of
     // the chunk in earlier rows, earlier waves of this row, earlier lanes
     int row_base = 0;
@@ -767,8 +788,7 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
         if (l >= n || ex[i])
             continue;
         const uint64_t key = kk[i];
-        const int rb = bfit ? b_lo + lower_bound_u64(s_br, 0, nb, key)
-                            : lower_bound_u64(Bk, b_lo, b_hi, key);
+        const int rb = bfit ? b_lo + rbl[i] : lower_bound_u64(Bk, b_lo, b_hi, key);
         const int np = i0 + l - eb + rb;
         if ((unsigned)np >= (unsigned)W) { // impossible unless state is corrupt
             atomicOr(&d.ctl->err, 1);
@@ -782,6 +802,8 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
             Nx[(size_t)a * W + np] = cx[i][a];
         if ((np & smask) == 0)
             Nsamp[np >> sl2] = key;
+        if (Nsamp4 && (np & 3) == 0)
+            Nsamp4[np >> 2] = key;
     }
 
     // this workgroup's share of B: #A <= k_b via A's sample table, #E <= k_b
@@ -797,13 +819,13 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
             for (int a = 0; a < DIM; ++a)
                 px[a] = Bx[(size_t)a * bxs + j];
         }
-        int ca = upper_bound_u64(s_as, 0, d.nsamp, key); // samples <= key
+        const int ca = upper_bound_u64(s_as, 0, d.nsamp, key); // samples <= key
         int ubA = 0;
         if (ca > 0) {
             const int L = ((ca - 1) << sl2) + 1, H = min(ca << sl2, W);
             ubA = upper_bound_u64(Ak, L, H, key);
         }
-        int ce = upper_bound_u64(s_es, 0, nbs, key);
+        const int ce = upper_bound_u64(s_es, 0, nbs, key);
         int ubE = 0;
         if (ce > 0) {
             const int L = ((ce - 1) << bsl) + 1, H = min(ce << bsl, S);
@@ -822,6 +844,8 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
             Nx[(size_t)a * W + np] = px[a];
         if ((np & smask) == 0)
             Nsamp[np >> sl2] = key;
+        if (Nsamp4 && (np & 3) == 0)
+            Nsamp4[np >> 2] = key;
     }
 }
 
@@ -1101,64 +1125,39 @@ __device__ __forceinline__ void wave_sync_lds()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Row bounds of a candidate's stencil: lane b < 2*nrows computes bound b
-// (row b>>1, upper if b&1).  The sample table in LDS narrows each bound to
-// one stride; the strides are then finished cooperatively: the 64 lanes walk
-// the (bound, offset) pairs 64 at a time, eight batches of loads in flight,
-// and each batch's ballot is credited to the bound's owner lane.
-template <int DIM>
-__device__ __forceinline__ int row_bounds(const Dev &d, const uint64_t *Nk,
-                                          const uint64_t *s_samp, const Stencil<DIM> &st,
-                                          int *w_L)
+// Finishes bounds inside one gap of a sorted table each: bound b's answer
+// lies past tab[w_L[b] - 1] and at or before tab[w_L[b] + 2^sl2 - 2]; the
+// 2^sl2 - 1 positions from w_L[b] are probed (w_L[b] < 0: nothing to probe,
+// positions >= lim do not exist).  The 64 lanes walk the (bound, offset)
+// pairs 64 at a time, U batches of loads in flight, and each batch's ballot
+// is credited to the bound's owner lane: lane b returns how many of its
+// positions pass (tab[i] < key for a lower bound, <= key for an upper one).
+template <int U>
+__device__ __forceinline__ int refine_bounds(const uint64_t *tab, const int *w_L, int lim, int nb,
+                                             int sl2, uint64_t key)
 {
-    constexpr int U = 8;
     const int lane = lane_id();
-    const int W = d.W, sl2 = d.stride_log2;
     const int stride = 1 << sl2;
-    const int nb = 2 * st.nrows;
-    uint64_t key = 0;
-    int L = 0;
-    if (lane < nb) {
-        uint64_t klo, khi;
-        row_keys<DIM>(st, lane >> 1, klo, khi);
-        const bool upper = lane & 1;
-        key = upper ? khi : klo;
-        int lo = 0, hi = d.nsamp;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            const bool right = upper ? (s_samp[mid] <= key) : (s_samp[mid] < key);
-            if (right)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        L = (lo > 0) ? ((lo - 1) << sl2) + 1 : 0;
-        w_L[lane] = (lo > 0) ? L : -1; // -1: the bound is 0, nothing to probe
-    }
-    wave_sync_lds();
-    // bound b's undecided positions: [L_b, L_b + stride - 1) clipped to W
     const int niter = (nb * stride + kWave - 1) / kWave;
     int my_cnt = 0;
     for (int q0 = 0; q0 < niter; q0 += U) {
+        // a position with nothing to probe loads as ~0, which passes neither
+        // test (keys use at most 63 bits)
         uint64_t xv[U];
-        bool ok[U];
-        int bb[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int e = (q0 + u) * kWave + lane;
             const int b = e >> sl2, r = e & (stride - 1);
-            bb[u] = b;
             const int Lb = (b < nb) ? w_L[b] : -1;
-            ok[u] = (q0 + u < niter) && Lb >= 0 && r < stride - 1 && Lb + r < W;
-            xv[u] = ok[u] ? Nk[Lb + r] : 0ull;
+            const bool ok = (q0 + u < niter) && Lb >= 0 && r < stride - 1 && Lb + r < lim;
+            xv[u] = ok ? tab[Lb + r] : ~0ull;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int q = q0 + u;
-            if (q >= niter)
-                break;
-            const uint64_t kb = __shfl(key, bb[u] & 63, kWave);
-            const bool less = ok[u] && ((bb[u] & 1) ? (xv[u] <= kb) : (xv[u] < kb));
+            const int b = (q * kWave + lane) >> sl2;
+            const uint64_t kb = __shfl(key, b & 63, kWave);
+            const bool less = (b & 1) ? (xv[u] <= kb) : (xv[u] < kb);
             const uint64_t m = __ballot(less);
             const int b0 = (q * kWave) >> sl2;
             if (stride >= kWave) {
@@ -1171,10 +1170,61 @@ __device__ __forceinline__ int row_bounds(const Dev &d, const uint64_t *Nk,
             }
         }
     }
-    int res = 0;
+    return my_cnt;
+}
+
+// Row bounds of a candidate's stencil: lane b < 2*nrows computes bound b
+// (row b>>1, upper if b&1).  The sample table in LDS (stride S1 = 2^sl2)
+// narrows each bound to one gap of S1 - 1 keys.  With S1 <= 4 those keys are
+// probed directly; else the every-4th-key table narrows the gap to 3 keys
+// first (S1/4 - 1 probes, then 3): two short coalesced rounds instead of
+// S1 - 1 probes per bound, which is most of a 3-D unit's memory traffic.
+template <int DIM>
+__device__ __forceinline__ int row_bounds(const Dev &d, const uint64_t *Nk, const uint64_t *Nk4,
+                                          const uint64_t *s_samp, const Stencil<DIM> &st,
+                                          int *w_L)
+{
+    const int lane = lane_id();
+    const int W = d.W, sl2 = d.stride_log2;
+    const int nb = 2 * st.nrows;
+    uint64_t key = 0;
+    int lo = 0;
+    if (lane < nb) {
+        uint64_t klo, khi;
+        row_keys<DIM>(st, lane >> 1, klo, khi);
+        const bool upper = lane & 1;
+        key = upper ? khi : klo;
+        int hi = d.nsamp;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const bool right = upper ? (s_samp[mid] <= key) : (s_samp[mid] < key);
+            if (right)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+    }
+    // the answer is 0 when no sample passes; else it lies in
+    // [(lo-1)*S1 + 1, lo*S1], element (lo-1)*S1 passing
+    if (!Nk4) {
+        if (lane < nb)
+            w_L[lane] = lo > 0 ? ((lo - 1) << sl2) + 1 : -1;
+        wave_sync_lds();
+        const int c = refine_bounds<8>(Nk, w_L, W, nb, sl2, key);
+        return (lane < nb && lo > 0) ? ((lo - 1) << sl2) + 1 + c : 0;
+    }
     if (lane < nb)
-        res = (L > 0) ? L + my_cnt : 0;
-    return res;
+        w_L[lane] = lo > 0 ? ((lo - 1) << (sl2 - 2)) + 1 : -1;
+    wave_sync_lds();
+    const int c4 = refine_bounds<8>(Nk4, w_L, d.nsamp4, nb, sl2 - 2, key);
+    // element base passes, element base + 4 does not (or is past the end)
+    const int base = lo > 0 ? ((lo - 1) << sl2) + 4 * c4 : 0;
+    wave_sync_lds();
+    if (lane < nb)
+        w_L[lane] = lo > 0 ? base + 1 : -1;
+    wave_sync_lds();
+    const int c1 = refine_bounds<8>(Nk, w_L, W, nb, 2, key);
+    return (lane < nb && lo > 0) ? base + 1 + c1 : 0;
 }
 
 // min / max over lanes 0..15 (a unit's candidates), broadcast to the wave
@@ -1207,8 +1257,9 @@ __device__ __forceinline__ float lane_f(float v, int l)
 // ballot + popcount per candidate) and the scan stops once every
 // candidate's count exceeds K.  Wave-cooperative; tests += evaluations.
 template <int DIM>
-__device__ __forceinline__ int unit_count(const Dev &d, const uint64_t *Nk, const float *Nx,
-                                          const uint64_t *s_samp, int *w_a, int *w_b,
+__device__ __forceinline__ int unit_count(const Dev &d, const uint64_t *Nk, const uint64_t *Nk4,
+                                          const float *Nx, const uint64_t *s_samp, int *w_a,
+                                          int *w_b,
                                           const float (&pc)[3], int n, Stencil<DIM> st,
                                           long long &tests)
 {
@@ -1219,7 +1270,7 @@ __device__ __forceinline__ int unit_count(const Dev &d, const uint64_t *Nk, cons
             atomicOr(&d.ctl->err, 4);
         st.nrows = 32;
     }
-    const int bnd = row_bounds<DIM>(d, Nk, s_samp, st, w_a);
+    const int bnd = row_bounds<DIM>(d, Nk, Nk4, s_samp, st, w_a);
     wave_sync_lds();
     // lane r < nrows: row r spans [bound 2r, bound 2r+1)
     const int r = lane;
@@ -1363,6 +1414,13 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int n
     const int w0 = min(nwords, tid * per), w1 = min(nwords, w0 + per);
     HostOut *ho = host_out(d, st.t);
     int total = 0;
+    // the slide's counters, requested with the bitmap words
+    unsigned long long pt = 0;
+    int err = 0;
+    if (tid == 0) {
+        pt = __hip_atomic_load(&d.ctl->pair_tests, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        err = __hip_atomic_load(&d.ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     if (per <= kCompactRegs) {
         uint32_t wv[kCompactRegs];
         int mine = 0;
@@ -1413,12 +1471,10 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int n
     // (polling, acquire) sees the list and the counters
     __syncthreads();
     if (tid == 0) {
-        const unsigned long long pt =
-            __hip_atomic_load(&d.ctl->pair_tests, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ho->n_out = total;
         ho->n_cand = n_cand;
         ho->pair_tests = pt;
-        ho->err = __hip_atomic_load(&d.ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ho->err = err;
         d.ctl->err = 0;
         d.ctl->pair_tests = 0;
         d.ctl->ticket = 0;
@@ -1470,6 +1526,7 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
     const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
     const int par = st.par ^ 1;
     const uint64_t *Nk = d.akey[par];
+    const uint64_t *Nk4 = d.nsamp4 ? d.samples4[par] : nullptr;
     const int32_t *Ns = d.aslot[par];
     const float *Nx = d.axyz[par];
     const int W = d.W, K = d.K;
@@ -1484,41 +1541,57 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
         s_tests = 0;
         s_miss = 0;
     }
-    copy_to_lds<kMaxSamples / kDetectNT>(s_samp, d.samples[par], d.nsamp);
-    __syncthreads();
-
-    long long tests = 0;
-    int misses = 0;
-    for (int u = bid * NW + wid; u < n_units; u += busy * NW) {
-        const int li = (!LEARN || u < n_o) ? u : W - 1 - (u - n_o);
-        const int2 un = d.units[li];
-        const int start = __builtin_amdgcn_readfirstlane(un.x);
-        const int n = __builtin_amdgcn_readfirstlane(un.y);
-        float pc[3] = {0.f, 0.f, 0.f};
+    // a unit's list entry, then its candidates' coordinates and slots: the
+    // first unit's are requested around the sample-table staging, each
+    // next unit's entry while the current one is counted
+    auto list_index = [&](int uu) { return (!LEARN || uu < n_o) ? uu : W - 1 - (uu - n_o); };
+    const int ustep = busy * NW;
+    int u = bid * NW + wid;
+    int2 un = make_int2(0, 0);
+    if (u < n_units)
+        un = d.units[list_index(u)];
+    float pc[3] = {0.f, 0.f, 0.f};
+    int32_t cslot = 0;
+    auto load_candidates = [&](int2 e) {
+        const int start = __builtin_amdgcn_readfirstlane(e.x);
+        const int n = __builtin_amdgcn_readfirstlane(e.y);
         if (lane < n) {
 #pragma unroll
             for (int a = 0; a < DIM; ++a)
                 pc[a] = Nx[(size_t)a * W + start + lane];
+            cslot = Ns[start + lane];
         }
+    };
+    copy_to_lds<kMaxSamples / kDetectNT>(s_samp, d.samples[par], d.nsamp);
+    if (u < n_units)
+        load_candidates(un);
+    __syncthreads();
+
+    long long tests = 0;
+    int misses = 0;
+    for (; u < n_units; u += ustep) {
+        const int li = list_index(u);
+        const int n = __builtin_amdgcn_readfirstlane(un.y);
+        int2 un_next = make_int2(0, 0);
+        if (u + ustep < n_units)
+            un_next = d.units[list_index(u + ustep)];
         const Stencil<DIM> full = unit_stencil<DIM>(d, pc, n);
         int cnt;
         if (LEARN && DIM > 1 && use_pred && u >= n_o) {
-            cnt = unit_count<DIM>(d, Nk, Nx, s_samp, s_rows[wid][0], s_rows[wid][1], pc, n,
+            cnt = unit_count<DIM>(d, Nk, Nk4, Nx, s_samp, s_rows[wid][0], s_rows[wid][1], pc, n,
                                   near_stencil<DIM>(d, full, pc), tests);
             if (__ballot(lane < n && cnt <= K) != 0) {
                 wave_sync_lds();
-                cnt = unit_count<DIM>(d, Nk, Nx, s_samp, s_rows[wid][0], s_rows[wid][1], pc,
+                cnt = unit_count<DIM>(d, Nk, Nk4, Nx, s_samp, s_rows[wid][0], s_rows[wid][1], pc,
                                       n, full, tests);
             }
         } else {
-            cnt = unit_count<DIM>(d, Nk, Nx, s_samp, s_rows[wid][0], s_rows[wid][1], pc, n,
+            cnt = unit_count<DIM>(d, Nk, Nk4, Nx, s_samp, s_rows[wid][0], s_rows[wid][1], pc, n,
                                   full, tests);
         }
         const int capped = lane < n ? min(cnt, K + 1) : K + 1;
-        if (lane < n && cnt <= K) {
-            const int slot = Ns[start + lane];
-            atomicOr(&d.bitmap[slot >> 5], 1u << (slot & 31));
-        }
+        if (lane < n && cnt <= K)
+            atomicOr(&d.bitmap[cslot >> 5], 1u << (cslot & 31));
         const int umin = unit_min_i(capped);
         if (lane == 0) {
             d.cand_cnt[u] = umin;
@@ -1528,6 +1601,9 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
             }
         }
         wave_sync_lds();
+        un = un_next;
+        if (u + ustep < n_units)
+            load_candidates(un);
     }
     if (lane == 0 && tests)
         atomicAdd(&s_tests, (unsigned long long)tests);

@@ -128,6 +128,9 @@ struct Dev {
     float *txyz[2];
     int tmpn;         // plane stride of txyz
     uint64_t *samples[2];
+    uint64_t *samples4[2]; // every 4th key of the sorted window (when the
+                           // sample stride exceeds 4; else null)
+    int nsamp4;            // entries of samples4 (0: none)
     int32_t *cand;
     int2 *units;        // cell units: {start, len}
     int32_t *cand_cnt;  // per-unit minimum count (capped)
