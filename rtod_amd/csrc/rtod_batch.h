@@ -107,7 +107,7 @@ __global__ __launch_bounds__(kBatchNT) void k_batch_classify(const BatchItem *it
 }
 
 template <int DIM, bool LEARN>
-__global__ __launch_bounds__(kDetectNT) void k_batch_detect(const BatchItem *items,
+__global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_batch_detect(const BatchItem *items,
                                                           const BatchStream *streams)
 {
     __shared__ DetectLds lds;

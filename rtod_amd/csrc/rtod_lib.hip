@@ -103,7 +103,7 @@ struct rtod_ctx {
     rtod_config cfg{};
     int dim = 1, W = 0, S = 0, m = 0, K = 0;
     int sort_shape = 0, CH = 512;
-    int detect_blocks = 512;
+    int detect_blocks = 768; // three resident detect workgroups per CU
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = true; // false: a batch member on the batch's stream
@@ -311,11 +311,12 @@ void launch_detect_dim(rtod_ctx *c, hipStream_t s, const Step &st, const Step *r
     const int grid = c->detect_blocks;
     if (runs) {
         const dim3 g(grid + run_merge_wgs(c->S));
+        const size_t lds = detect_runs_lds(c->S);
         if (c->learned)
-            hipLaunchKernelGGL((k_detect_runs<DIM, true>), g, dim3(kDetectNT), 0, s, c->d, st,
+            hipLaunchKernelGGL((k_detect_runs<DIM, true>), g, dim3(kDetectNT), lds, s, c->d, st,
                                *runs, grid);
         else
-            hipLaunchKernelGGL((k_detect_runs<DIM, false>), g, dim3(kDetectNT), 0, s, c->d, st,
+            hipLaunchKernelGGL((k_detect_runs<DIM, false>), g, dim3(kDetectNT), lds, s, c->d, st,
                                *runs, grid);
         return;
     }

@@ -1114,9 +1114,12 @@ __device__ __forceinline__ void row_keys(const Stencil<DIM> &st, int r, uint64_t
     }
 }
 
-// 8 wavefronts share one LDS copy of the sample table: about 24 resident
-// wavefronts per CU (VGPR-bound) instead of 16 with 256-thread workgroups
+// 8 wavefronts share one LDS copy of the sample table; registers are capped
+// for 6 wavefronts per SIMD, so three workgroups (24 wavefronts) fit a CU:
+// the per-unit work is a chain of dependent memory round trips, and more
+// resident units hide more of it
 constexpr int kDetectNT = 512;
+#define RTOD_DETECT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(6, 8)))
 
 __device__ __forceinline__ void wave_sync_lds()
 {
@@ -1649,7 +1652,7 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
 }
 
 template <int DIM, bool LEARN>
-__global__ __launch_bounds__(kDetectNT) void k_detect(Dev d, Step st)
+__global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_detect(Dev d, Step st)
 {
     __shared__ DetectLds lds;
     detect_body<DIM, LEARN>(d, st, blockIdx.x, gridDim.x, lds);
@@ -1680,17 +1683,26 @@ __global__ __launch_bounds__(256) void k_merge_window_ingest(Dev d, Step st, Ste
 
 static_assert(kRunMergeNT == kDetectNT, "the run merge rides in detect workgroups");
 
-template <int DIM, bool LEARN>
-__global__ __launch_bounds__(kDetectNT) void k_detect_runs(Dev d, Step st, Step st_next, int ndet)
+// LDS of k_detect_runs, set at launch: the detect's, or the next slide's
+// S keys when larger (dynamic, so a 5000-point slide takes 40 KB, not the
+// 64 KB of the largest slide, and three workgroups still fit a CU)
+__host__ __device__ constexpr size_t detect_runs_lds(int S)
 {
-    __shared__ union {
-        DetectLds det;
-        uint64_t runs[kRunMergeCap];
-    } lds;
+    return sizeof(DetectLds) > (size_t)S * sizeof(uint64_t) ? sizeof(DetectLds)
+                                                             : (size_t)S * sizeof(uint64_t);
+}
+
+template <int DIM, bool LEARN>
+__global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_detect_runs(Dev d, Step st,
+                                                                                Step st_next,
+                                                                                int ndet)
+{
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
     if ((int)blockIdx.x < ndet)
-        detect_body<DIM, LEARN>(d, st, blockIdx.x, ndet, lds.det);
+        detect_body<DIM, LEARN>(d, st, blockIdx.x, ndet, *reinterpret_cast<DetectLds *>(dyn_lds));
     else
-        merge_runs_body<DIM>(d, st_next, kModeSlide, blockIdx.x - ndet, lds.runs);
+        merge_runs_body<DIM>(d, st_next, kModeSlide, blockIdx.x - ndet,
+                             reinterpret_cast<uint64_t *>(dyn_lds));
 }
 
 } // namespace rtod
