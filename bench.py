@@ -104,24 +104,29 @@ def run_gpu(xs, W, S, R, K, warmup, steps, flags=0, sched=0, device_input=True, 
     return res
 
 
-def run_gpu_pipelined(xs, W, S, R, K, warmup, steps, sched=0, index=0):
+def run_gpu_pipelined(xs, W, S, R, K, warmup, steps, sched=0, index=0, flags=0):
     """The same slides through rtod_slides_device: slide t+1's new batch is
-    sorted on a second HIP stream while slide t is merged, classified and
-    counted, and the host stays one slide ahead.  Stream in HBM; `warmup`
-    slides in one call, then `steps` slides in one timed call."""
+    sorted by extra workgroups of slide t's launches, up to four slides are
+    queued and the host collects each list as it completes.  Stream in HBM;
+    `warmup` slides in one call, then `steps` slides in one timed call.
+    flags=FLAG_PHASE_TIMES: HIP events around each kernel (a separate,
+    slide-by-slide measuring pass)."""
     import rtod_amd
 
     dim = xs.shape[1]
-    det = rtod_amd.Detector(dim, W, S, R, K, sched_kind=sched, index_kind=index)
+    det = rtod_amd.Detector(dim, W, S, R, K, sched_kind=sched, index_kind=index, flags=flags)
     ds = rtod_amd.DeviceStream(xs[: W + (warmup + steps) * S])
     det.prime(xs[:W])
     if warmup:
         det.slides_device(ds.at(W), warmup)
     st0 = det.stats()
     t0 = time.perf_counter()
+    if flags:
+        det.reset_kernel_times()
     lists = det.slides_device(ds.at(W + warmup * S), steps)
     dt = time.perf_counter() - t0
-    res = {"us": dt / steps * 1e6, "lists": lists, "stats": det.stats(), "stats_warm": st0}
+    res = {"us": dt / steps * 1e6, "lists": lists, "stats": det.stats(), "stats_warm": st0,
+           "kernels": det.kernel_times()}
     det.close()
     ds.close()
     return res
@@ -165,7 +170,7 @@ def measured_traffic(workload: str, kernel: str):
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("kernel=") and f"k_{kernel}" in line:
+        if line.startswith("kernel=") and f"k_{kernel}<" in line:
             for tok in line.split():
                 if tok.startswith("traffic_bytes="):
                     return int(tok.split("=")[1])
@@ -418,10 +423,15 @@ def main() -> int:
     pcie = run_gpu(xs, W, S, R, K, args.warmup, args.steps, device_input=False)
     log(f"[bench] plain, inputs from host memory (PCIe inside the call): "
         f"{pcie['lib_ms'] * 1e3:.1f} us/slide")
-    # per-kernel device durations: a separate pass with events between phases
-    timed = run_gpu(xs, W, S, R, K, min(args.warmup, 10), min(args.steps, 60),
-                    flags=rtod_amd.FLAG_PHASE_TIMES)
+    # per-kernel device durations: separate passes with HIP events around each
+    # kernel, of the pipelined launches (the measured `value`) and of the
+    # one-slide-per-call ones
+    timed = run_gpu_pipelined(xs, W, S, R, K, min(args.warmup, 10), min(args.steps, 60),
+                              flags=rtod_amd.FLAG_PHASE_TIMES)
     kern = {k: v[0] for k, v in timed["kernels"].items() if v[1] > 0}
+    timed_seq = run_gpu(xs, W, S, R, K, min(args.warmup, 10), min(args.steps, 60),
+                        flags=rtod_amd.FLAG_PHASE_TIMES)
+    kern_seq = {k: v[0] for k, v in timed_seq["kernels"].items() if v[1] > 0}
     dom = max(kern, key=kern.get)
     dom_ms = kern[dom]
     n_out = plain["outliers_mean"]
@@ -431,8 +441,10 @@ def main() -> int:
     alg = {"ingest_sort": 8 * dim * S,                  # new slide in, ring out
            "merge_pass": 8 * dim * S,
            "merge_window": 8 * dim * W,                 # window in, window out
+           "merge_window_ingest": 8 * dim * (W + S),    # + the next slide's tile sorts
            "classify": 4 * dim * W,                     # one pass over the window
-           "detect": 4 * dim * n_cand + 4 * n_out}      # candidates in, list out
+           "detect": 4 * dim * n_cand + 4 * n_out,      # candidates in, list out
+           "detect_runs": 4 * dim * n_cand + 4 * n_out + 8 * dim * S}  # + next run merge
     b_dom = alg.get(dom, 4 * dim * (W + S))
     achieved = b_dom / (dom_ms * 1e-3) / 1e9
     b_slide = 4 * dim * (W + S) + 4 * n_out
@@ -498,6 +510,7 @@ def main() -> int:
         "outliers_per_slide": plain["outliers_mean"],
         "candidates_per_slide": plain["stats"]["candidates_last"],
         "kernels_ms": {k: round(v, 5) for k, v in kern.items()},
+        "kernels_ms_one_slide_per_call": {k: round(v, 5) for k, v in kern_seq.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 3),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,

@@ -97,7 +97,7 @@ typedef struct rtod_stats {
     int64_t device_bytes;       /* device memory owned by the context      */
     int64_t host_pinned_bytes;  /* pinned host memory owned by the context */
     int64_t predictor_misses;   /* learned scheduler: mispredicted cell
-                                   units (a unit: <= 64 candidates of one
+                                   units (a unit: <= 16 candidates of one
                                    sparse cell, counted together)         */
     int64_t predictor_points;   /* learned scheduler: predicted units      */
     int64_t fallbacks;          /* slides run on the plain schedule because
@@ -172,7 +172,10 @@ RTOD_API rtod_status rtod_copy_outlier(rtod_ctx *ctx, int32_t *outlier_slots, in
 RTOD_API rtod_status rtod_get_stats(const rtod_ctx *ctx, rtod_stats *out);
 
 /* Average device duration (ms) per launch of each kernel recorded with HIP
- * events on the context's stream while RTOD_FLAG_PHASE_TIMES is set.
+ * events on the context's stream while RTOD_FLAG_PHASE_TIMES is set (with
+ * rtod_slides_device the flag also waits on every slide: a measuring mode;
+ * its window merge and detect launches carry the next slide's sort and are
+ * named "merge_window_ingest" and "detect_runs").
  * names[i] (static strings) / avg_ms[i] / launches[i] for i < *n (in: capacity). */
 RTOD_API rtod_status rtod_kernel_times(const rtod_ctx *ctx, const char **names, double *avg_ms,
                                        int64_t *launches, int32_t *n);

@@ -53,10 +53,14 @@ enum KernelId {
     kKDetect,
     kKLbvhBuild,
     kKLbvhQuery,
+    kKMergeWindowIngest, // pipelined: window merge + the next slide's tile sorts
+    kKDetectRuns,        // pipelined: detect + the next slide's run merge
     kKCount
 };
-const char *kKernelNames[kKCount] = {"ingest_sort", "merge_pass", "merge_window", "classify",
-                                     "detect",      "lbvh_build", "lbvh_query"};
+const char *kKernelNames[kKCount] = {"ingest_sort",         "merge_pass",  "merge_window",
+                                     "classify",            "detect",      "lbvh_build",
+                                     "lbvh_query",          "merge_window_ingest",
+                                     "detect_runs"};
 
 // LDS-sort tile shapes (threads x items per thread).  A batch of up to
 // kRunMergeCap points is sorted as tiles of 1024 points on as many
@@ -414,8 +418,6 @@ void launch_ingest_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t 
         (void)hipEventRecord(ev[1], s);
     if (c->d.cpb > 1)
         launch_batch_merges(c, s, st, kModeSlide, c->S);
-    if (ev)
-        (void)hipEventRecord(ev[2], s);
 }
 
 // next != nullptr (pipelined slides, ride_along): the next slide's ingest
@@ -424,6 +426,8 @@ void launch_ingest_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t 
 void launch_core_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t *ev,
                        const Step *next)
 {
+    if (ev)
+        (void)hipEventRecord(ev[2], s);
     launch_merge_window(c, s, st, next);
     if (ev)
         (void)hipEventRecord(ev[3], s);
@@ -800,6 +804,32 @@ rtod_status collect(rtod_ctx *c, int32_t *outs, int32_t *n_out)
     return RTOD_OK;
 }
 
+// Core-stage kernel durations of a pipelined slide (events 2..6): the window
+// merge and the detect carry the next slide's ingest work when `fused`.
+void accumulate_core_times(rtod_ctx *c, bool fused)
+{
+    float ms[4];
+    for (int i = 0; i < 4; ++i) {
+        ms[i] = 0.f;
+        (void)hipEventElapsedTime(&ms[i], c->ev[2 + i], c->ev[3 + i]);
+    }
+    const int mw = fused ? kKMergeWindowIngest : kKMergeWindow;
+    c->ksum[mw] += ms[0];
+    c->kcnt[mw]++;
+    c->ksum[kKClassify] += ms[1];
+    c->kcnt[kKClassify]++;
+    if (c->lbvh) {
+        c->ksum[kKLbvhBuild] += ms[2];
+        c->kcnt[kKLbvhBuild]++;
+        c->ksum[kKLbvhQuery] += ms[3];
+        c->kcnt[kKLbvhQuery]++;
+    } else {
+        const int dk = (fused && c->d.cpb > 1) ? kKDetectRuns : kKDetect;
+        c->ksum[dk] += ms[2];
+        c->kcnt[dk]++;
+    }
+}
+
 void accumulate_kernel_times(rtod_ctx *c, double *phase)
 {
     float ms[6];
@@ -986,11 +1016,14 @@ rtod_status wait_slide(rtod_ctx *c, int64_t t)
 // stamp), and slide i+1's ingest stage rides in slide i's core launches
 // (ride_along; else it follows them).  The results are those of n
 // rtod_slide_device calls.  On an error the stream is drained first.
+// RTOD_FLAG_PHASE_TIMES: HIP events around the core kernels, and each slide
+// is waited for before the next is enqueued (a measuring mode).
 rtod_status slides_device_impl(rtod_ctx *c, const float *dev, int32_t n, int32_t *offs,
                                int32_t *slots)
 {
     const int64_t t0 = c->host_t;
     const size_t per = (size_t)c->S * c->dim;
+    const bool timed = (c->cfg.flags & RTOD_FLAG_PHASE_TIMES) != 0;
     int64_t used = 0;
     int32_t done = 0;
     offs[0] = 0;
@@ -1013,13 +1046,22 @@ rtod_status slides_device_impl(rtod_ctx *c, const float *dev, int32_t n, int32_t
         const Step nx = more ? make_step_at(c, t + 1, dev + (size_t)(i + 1) * per) : Step{};
         if ((s = guard_trainer(c, (int64_t)t, c->stream)) != RTOD_OK)
             break;
-        launch_core_stage(c, c->stream, st, nullptr, more && c->ride ? &nx : nullptr);
+        const bool fused = more && c->ride;
+        launch_core_stage(c, c->stream, st, timed ? c->ev : nullptr, fused ? &nx : nullptr);
         if (more && !c->ride)
             launch_ingest_stage(c, c->stream, nx, nullptr);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
             set_error(std::string("launch: ") + hipGetErrorString(e));
             s = RTOD_ERR_HIP;
+        } else if (timed) {
+            const hipError_t q = hipStreamSynchronize(c->stream);
+            if (q != hipSuccess) {
+                set_error(std::string("HIP error: ") + hipGetErrorString(q));
+                s = RTOD_ERR_HIP;
+            } else {
+                accumulate_core_times(c, fused);
+            }
         }
     }
     while (s == RTOD_OK && done < n)
