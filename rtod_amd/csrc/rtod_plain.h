@@ -1321,24 +1321,30 @@ __device__ __forceinline__ int unit_count(const Dev &d, const uint64_t *Nk, cons
                     qz[k] = Nx[(size_t)2 * W + j];
             }
         }
-        bool done = false;
+        // candidate by candidate over the group's chunks: its coordinates are
+        // broadcast once per group, and it stops as soon as its own count
+        // exceeds K (counts stay exact up to K+1)
+        const int nk = min(PF, (total - base0 + kWave - 1) / kWave);
+        for (int c = 0; c < n; ++c) {
+            const int cc = __builtin_amdgcn_readlane(cnt, c);
+            if (cc > K)
+                continue;
+            const float p3[3] = {lane_f(pc[0], c), DIM > 1 ? lane_f(pc[1], c) : 0.f,
+                                 DIM > 2 ? lane_f(pc[2], c) : 0.f};
+            int h = 0;
 #pragma unroll
-        for (int k = 0; k < PF; ++k) {
-            const int base = base0 + k * kWave;
-            if (done || base >= total)
-                break;
-            const bool have = base + lane < total;
-            for (int c = 0; c < n; ++c) {
-                const float p3[3] = {lane_f(pc[0], c), DIM > 1 ? lane_f(pc[1], c) : 0.f,
-                                     DIM > 2 ? lane_f(pc[2], c) : 0.f};
-                const bool hit = have && dist2<DIM>(p3, qx[k], qy[k], qz[k]) <= d.r2;
-                const int h = __popcll(__ballot(hit));
-                cnt += (lane == c) ? h : 0;
+            for (int k = 0; k < PF; ++k) {
+                if (k >= nk || cc + h > K)
+                    break;
+                const int base = base0 + k * kWave;
+                const bool hit =
+                    base + lane < total && dist2<DIM>(p3, qx[k], qy[k], qz[k]) <= d.r2;
+                h += __popcll(__ballot(hit));
+                tests += min(kWave, total - base);
             }
-            tests += (long long)min(kWave, total - base) * n;
-            done = __ballot(lane < n && cnt <= K) == 0;
+            cnt += (lane == c) ? h : 0;
         }
-        if (done)
+        if (__ballot(lane < n && cnt <= K) == 0)
             break;
     }
     return cnt;
