@@ -476,19 +476,19 @@ __device__ __forceinline__ void merge_runs_body(const Dev &d, const Step &st, in
     const size_t bxs = (size_t)d.nent * S;
     // the elements' searches advance together: IT * kRunMax independent
     // LDS reads per step
+    // run q's length (0 past the last run; an element skips its own run)
+    auto run_len = [&](int q) { return q < nruns ? min(L, S - q * L) : 0; };
     uint64_t key[IT];
-    int r[IT], c[IT][kRunMax], len[IT][kRunMax];
+    int r[IT], c[IT][kRunMax];
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
         const int i = w * kRunMergeT + k * kRunMergeNT + threadIdx.x;
         const bool on = i < S;
         key[k] = on ? s_k[i] : 0ull;
-        r[k] = on ? i / L : 0;
+        r[k] = on ? i / L : kRunMax; // an idle thread searches nothing
 #pragma unroll
-        for (int q = 0; q < kRunMax; ++q) {
+        for (int q = 0; q < kRunMax; ++q)
             c[k][q] = 0;
-            len[k][q] = (on && q < nruns && q != r[k]) ? min(L, S - q * L) : 0;
-        }
     }
 #pragma unroll
     for (int b = L; b >= 1; b >>= 1) {
@@ -496,14 +496,16 @@ __device__ __forceinline__ void merge_runs_body(const Dev &d, const Step &st, in
 #pragma unroll
         for (int k = 0; k < IT; ++k)
 #pragma unroll
-            for (int q = 0; q < kRunMax; ++q)
-                x[k][q] = (c[k][q] + b <= len[k][q]) ? s_k[q * L + c[k][q] + b - 1] : ~0ull;
+            for (int q = 0; q < kRunMax; ++q) {
+                const bool live = q != r[k] && r[k] < kRunMax && c[k][q] + b <= run_len(q);
+                x[k][q] = live ? s_k[q * L + c[k][q] + b - 1] : ~0ull;
+            }
 #pragma unroll
         for (int k = 0; k < IT; ++k)
 #pragma unroll
             for (int q = 0; q < kRunMax; ++q) {
-                const bool pass = (c[k][q] + b <= len[k][q]) &&
-                                  (q < r[k] ? x[k][q] <= key[k] : x[k][q] < key[k]);
+                const bool live = q != r[k] && r[k] < kRunMax && c[k][q] + b <= run_len(q);
+                const bool pass = live && (q < r[k] ? x[k][q] <= key[k] : x[k][q] < key[k]);
                 c[k][q] += pass ? b : 0;
             }
     }
@@ -1290,8 +1292,9 @@ __device__ __forceinline__ int unit_count(const Dev &d, const uint64_t *Nk, cons
     wave_sync_lds();
 
     // PF chunks at a time: their row lookups and coordinate loads are
-    // issued together, so a group costs one memory round trip
-    constexpr int PF = 8;
+    // issued together, so a group costs one memory round trip (4 in 3-D:
+    // three coordinate planes per chunk, within the detect's register cap)
+    constexpr int PF = DIM == 3 ? 4 : 8;
     int cnt = 0;
     for (int base0 = 0; base0 < total; base0 += PF * kWave) {
         float qx[PF], qy[PF], qz[PF];
