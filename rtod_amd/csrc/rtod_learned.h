@@ -135,16 +135,39 @@ __device__ __forceinline__ float cross_group_sum(float x)
 // layer costs one LDS round trip.  s_h: 64*32 bf16, private to the wave.
 // Every lane of the wave must call it (m uniform).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void mlp_infer_rows(const float *P, const float (*s_x)[kFeatStride],
+// A lane's slices of the weights for inference (MFMA B operands, biases,
+// output weights), loaded once so the loads can be issued early.
+struct MlpRegs {
+    bf16x8 w1a, w1b, w2a, w2b;
+    float b1a, b1b, b2a, b2b, w3a, w3b, b3;
+};
+
+__device__ __forceinline__ MlpRegs mlp_regs(const float *P)
+{
+    const int n0 = lane_id() & 15;
+    MlpRegs r;
+    r.w1a = b_w1(P, 0);
+    r.w1b = b_w1(P, 1);
+    r.w2a = b_w2(P, 0);
+    r.w2b = b_w2(P, 1);
+    r.b1a = P[kOffB1 + n0];
+    r.b1b = P[kOffB1 + n0 + 16];
+    r.b2a = P[kOffB2 + n0];
+    r.b2b = P[kOffB2 + n0 + 16];
+    r.w3a = P[kOffW3 + n0];
+    r.w3b = P[kOffW3 + n0 + 16];
+    r.b3 = P[kOffB3];
+    return r;
+}
+
+__device__ __forceinline__ void mlp_infer_rows(const MlpRegs &Wr, const float (*s_x)[kFeatStride],
                                                int m, __bf16 *s_h, float *s_z)
 {
     const int l = lane_id(), n0 = l & 15, grp = l >> 4;
     const int tiles = (m + 15) >> 4;
-    const bf16x8 w1a = b_w1(P, 0), w1b = b_w1(P, 1);
-    const bf16x8 w2a = b_w2(P, 0), w2b = b_w2(P, 1);
-    const float b1a = P[kOffB1 + n0], b1b = P[kOffB1 + n0 + 16];
-    const float b2a = P[kOffB2 + n0], b2b = P[kOffB2 + n0 + 16];
-    const float w3a = P[kOffW3 + n0], w3b = P[kOffW3 + n0 + 16];
+    const bf16x8 w1a = Wr.w1a, w1b = Wr.w1b, w2a = Wr.w2a, w2b = Wr.w2b;
+    const float b1a = Wr.b1a, b1b = Wr.b1b, b2a = Wr.b2a, b2b = Wr.b2b;
+    const float w3a = Wr.w3a, w3b = Wr.w3b;
     const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
     // layer 1: A = feature rows (k < 8 in lane group 0, zero elsewhere)
 #pragma unroll
@@ -196,8 +219,9 @@ __device__ __forceinline__ float mlp_infer_wave(const float *P, const float (&f)
     for (int j = 0; j < kFeatStride; ++j)
         s_x[l][j] = j < kMlpIn ? f[j < kMlpIn ? j : 0] : 0.f;
     wave_lds_fence();
-    mlp_infer_rows(P, s_x, kWave, s_h, s_z);
-    const float z = s_z[l] + P[kOffB3];
+    const MlpRegs Wr = mlp_regs(P);
+    mlp_infer_rows(Wr, s_x, kWave, s_h, s_z);
+    const float z = s_z[l] + Wr.b3;
     wave_lds_fence();
     return sigmoidf(z);
 }

@@ -476,19 +476,19 @@ __device__ __forceinline__ void merge_runs_body(const Dev &d, const Step &st, in
     const size_t bxs = (size_t)d.nent * S;
     // the elements' searches advance together: IT * kRunMax independent
     // LDS reads per step
-    // run q's length (0 past the last run; an element skips its own run)
-    auto run_len = [&](int q) { return q < nruns ? min(L, S - q * L) : 0; };
     uint64_t key[IT];
-    int r[IT], c[IT][kRunMax];
+    int r[IT], c[IT][kRunMax], len[IT][kRunMax];
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
         const int i = w * kRunMergeT + k * kRunMergeNT + threadIdx.x;
         const bool on = i < S;
         key[k] = on ? s_k[i] : 0ull;
-        r[k] = on ? i / L : kRunMax; // an idle thread searches nothing
+        r[k] = on ? i / L : 0;
 #pragma unroll
-        for (int q = 0; q < kRunMax; ++q)
+        for (int q = 0; q < kRunMax; ++q) {
             c[k][q] = 0;
+            len[k][q] = (on && q < nruns && q != r[k]) ? min(L, S - q * L) : 0;
+        }
     }
 #pragma unroll
     for (int b = L; b >= 1; b >>= 1) {
@@ -496,16 +496,14 @@ __device__ __forceinline__ void merge_runs_body(const Dev &d, const Step &st, in
 #pragma unroll
         for (int k = 0; k < IT; ++k)
 #pragma unroll
-            for (int q = 0; q < kRunMax; ++q) {
-                const bool live = q != r[k] && r[k] < kRunMax && c[k][q] + b <= run_len(q);
-                x[k][q] = live ? s_k[q * L + c[k][q] + b - 1] : ~0ull;
-            }
+            for (int q = 0; q < kRunMax; ++q)
+                x[k][q] = (c[k][q] + b <= len[k][q]) ? s_k[q * L + c[k][q] + b - 1] : ~0ull;
 #pragma unroll
         for (int k = 0; k < IT; ++k)
 #pragma unroll
             for (int q = 0; q < kRunMax; ++q) {
-                const bool live = q != r[k] && r[k] < kRunMax && c[k][q] + b <= run_len(q);
-                const bool pass = live && (q < r[k] ? x[k][q] <= key[k] : x[k][q] < key[k]);
+                const bool pass = (c[k][q] + b <= len[k][q]) &&
+                                  (q < r[k] ? x[k][q] <= key[k] : x[k][q] < key[k]);
                 c[k][q] += pass ? b : 0;
             }
     }
@@ -894,6 +892,7 @@ template <bool MLP> struct ClassifyLds {
     alignas(16) __bf16 h[MLP ? NW : 1][MLP ? kWave * kMlpH : 1];
     float x[MLP ? NW : 1][MLP ? kWave : 1][kFeatStride];
     float z[MLP ? NW : 1][kWave];
+    float w[MLP ? kMlpNP : 1]; // the predictor's weights
 };
 
 template <int DIM, int MODE>
@@ -912,8 +911,46 @@ __device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int 
     const int H = LEARN ? max(2 * K + 2, kUnitMax) : max(K, kUnitMax);
     const bool tiled = H <= HALO;
     const int t0 = max(0, i0 - H), t1 = min(W, i0 + NT + H);
-    if (tiled)
+    // learned: this point's coordinates are requested first, and wave 0
+    // stages the predictor's published weights in LDS while the other waves
+    // stage the key tile, so the two chains of round trips overlap
+    float xi[3] = {0.f, 0.f, 0.f};
+    if constexpr (LEARN) {
+        if (i < W) {
+            const float *Nx = d.axyz[st.par ^ 1];
+#pragma unroll
+            for (int ax = 0; ax < DIM; ++ax)
+                xi[ax] = Nx[(size_t)ax * W + i];
+        }
+        const int wv = threadIdx.x / kWave;
+        if (wv == 0) { // all of a lane's loads in one round trip
+            constexpr int PER = (kMlpNP + kWave - 1) / kWave;
+            const int lane = threadIdx.x;
+            const float *P = current_weights(d.lr);
+            float v[PER];
+#pragma unroll
+            for (int u = 0; u < PER; ++u)
+                v[u] = lane + u * kWave < kMlpNP ? P[lane + u * kWave] : 0.f;
+#pragma unroll
+            for (int u = 0; u < PER; ++u)
+                if (lane + u * kWave < kMlpNP)
+                    lds.w[lane + u * kWave] = v[u];
+        } else if (tiled) {
+            constexpr int NTT = NT - kWave;
+            for (int q = threadIdx.x - kWave; q < t1 - t0; q += 8 * NTT) {
+                uint64_t v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    v[u] = q + u * NTT < t1 - t0 ? Nk[t0 + q + u * NTT] : 0ull;
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (q + u * NTT < t1 - t0)
+                        s_t[q + u * NTT] = v[u];
+            }
+        }
+    } else if (tiled) {
         copy_to_lds<(NT + 2 * HALO) / NT>(s_t, Nk + t0, t1 - t0);
+    }
     __syncthreads();
     auto kat = [&](int j) -> uint64_t { return tiled ? s_t[j - t0] : Nk[j]; };
     // first index in [max(0, p-cap), p] holding `key` (key[p] == key)
@@ -991,10 +1028,9 @@ __device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int 
                         f[1] = (float)min(a - run_lo(a - 1, key - 1, K), K + 1) * capf;
                     if (last < FM && b + 1 < W && kat(b + 1) == key + 1)
                         f[2] = (float)min(run_hi(b + 1, key + 1, K) - b, K + 1) * capf;
-                    const float *Nx = d.axyz[st.par ^ 1];
 #pragma unroll
                     for (int ax = 0; ax < DIM; ++ax) {
-                        const double u = (double)Nx[(size_t)ax * W + i] * d.inv_c;
+                        const double u = (double)xi[ax] * d.inv_c;
                         const float off = (float)(u - floor(u)) - 0.5f;
                         f[3 + ax] = (off >= -0.5f && off <= 0.5f) ? off : 0.f; // non-finite -> 0
                     }
@@ -1008,9 +1044,9 @@ __device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int 
                         lds.x[wid][row][j] = j < kMlpIn ? f[j < kMlpIn ? j : 0] : 0.f;
                 }
                 wave_lds_fence();
-                const float *P = current_weights(d.lr);
-                mlp_infer_rows(P, lds.x[wid], m, lds.h[wid], lds.z[wid]);
-                const float yh = ustart ? sigmoidf(lds.z[wid][row] + P[kOffB3]) : 0.f;
+                const MlpRegs Wr = mlp_regs(lds.w);
+                mlp_infer_rows(Wr, lds.x[wid], m, lds.h[wid], lds.z[wid]);
+                const float yh = ustart ? sigmoidf(lds.z[wid][row] + Wr.b3) : 0.f;
                 wave_lds_fence();
                 const bool pin = ustart && yh >= kInlierCut;
                 mi = __ballot(pin);
