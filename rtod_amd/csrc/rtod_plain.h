@@ -432,13 +432,15 @@ __global__ __launch_bounds__(256) void k_merge_pass(Dev d, Step st, const uint64
 // staged in LDS.  Element i of run r lands at
 //     (i - r*L) + sum_{r' < r} #(run r' <= key) + sum_{r' > r} #(run r' < key)
 // which is stable: an earlier run holds earlier arrivals.  A thread places
-// kRunMergeIT elements (strided by the workgroup size); an element's counts
+// kRunMergeIT elements (strided by the workgroup size; one, since the
+// search is bound by a CU's LDS bandwidth and more workgroups spread it);
+// an element's counts
 // in all the other runs are found together, bit by bit from L down (the
 // count is the largest c whose element c-1 passes), so each of the
 // log2(L)+1 steps issues up to kRunMax independent LDS reads per element.
 // Slots and coordinates are loaded before the staging barrier.
 // ---------------------------------------------------------------------------
-constexpr int kRunMergeNT = 512, kRunMergeIT = 2, kRunLen = 1024, kRunMergeCap = 8192;
+constexpr int kRunMergeNT = 512, kRunMergeIT = 1, kRunLen = 1024, kRunMergeCap = 8192;
 constexpr int kRunMax = kRunMergeCap / kRunLen;
 constexpr int kRunMergeT = kRunMergeNT * kRunMergeIT; // elements per workgroup
 
