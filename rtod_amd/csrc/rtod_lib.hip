@@ -259,7 +259,7 @@ void launch_batch_merges(rtod_ctx *c, hipStream_t s, const Step &st, int mode, i
 template <int DIM>
 void launch_merge_window_dim(rtod_ctx *c, hipStream_t s, const Step &st, const Step *next)
 {
-    const int nmb = (c->W + 1023) / 1024;
+    const int nmb = (c->W + kMergeT - 1) / kMergeT;
     if (!next) {
         hipLaunchKernelGGL(k_merge_window<DIM>, dim3(nmb), dim3(256), 0, s, c->d, st);
         return;

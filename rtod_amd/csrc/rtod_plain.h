@@ -586,10 +586,11 @@ __device__ __forceinline__ int wave_count_lds(const uint64_t *tab, int n, uint64
 // A[i0]'s own batch position when A[i0] expires.  Output positions that are
 // multiples of the sample stride fill A''s sample table.
 // ---------------------------------------------------------------------------
-constexpr int kMergeT = 1024; // window elements per merge workgroup
+constexpr int kMergeT = 512; // window elements per merge workgroup (half the chunk
+                              // of 1024 measured 8% faster on the 1-D headline)
 
 struct MergeWindowLds {
-    static constexpr int NT = 256, IT = 4, T = NT * IT, CAP = 2048, MAXBS = 512;
+    static constexpr int NT = 256, IT = 2, T = NT * IT, CAP = 2048, MAXBS = 512;
     uint64_t k[T];
     uint64_t bs[MAXBS], es[MAXBS];
     alignas(16) uint64_t as[kMaxSamples];
@@ -604,7 +605,7 @@ template <int DIM>
 __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, int bid,
                                                   MergeWindowLds &lds)
 {
-    constexpr int NT = 256, IT = 4, T = NT * IT, CAP = 2048;
+    constexpr int NT = 256, IT = 2, T = NT * IT, CAP = 2048;
     static_assert(T == kMergeT, "merge tile");
     uint64_t *s_k = lds.k;
     uint64_t *s_bs = lds.bs, *s_es = lds.es;
