@@ -1322,7 +1322,7 @@ __device__ __forceinline__ int unit_count(const Dev &d, const uint64_t *Nk, cons
     const int hi_b = __shfl(bnd, (2 * r + 1) & 63, kWave);
     const int len = (r < st.nrows) ? (hi_b - lo_b) : 0;
     const int incl = wave_incl_scan(len);
-    const int total = __shfl(incl, 63, kWave);
+    const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, kWave));
     wave_sync_lds();
     if (r < st.nrows) {
         w_a[r] = lo_b;
@@ -1336,11 +1336,14 @@ __device__ __forceinline__ int unit_count(const Dev &d, const uint64_t *Nk, cons
     constexpr int PF = DIM == 3 ? 4 : 8;
     int cnt = 0;
     for (int base0 = 0; base0 < total; base0 += PF * kWave) {
+        // a lane past the stencil's end holds NaN, which no predicate passes
+        // (as a NaN coordinate in the window passes none)
         float qx[PF], qy[PF], qz[PF];
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
             const int idx = base0 + k * kWave + lane;
-            qx[k] = qy[k] = qz[k] = 0.f;
+            qx[k] = __builtin_nanf("");
+            qy[k] = qz[k] = 0.f;
             if (idx < total) {
                 int j;
                 if constexpr (DIM == 1) {
@@ -1364,28 +1367,30 @@ __device__ __forceinline__ int unit_count(const Dev &d, const uint64_t *Nk, cons
             }
         }
         // candidate by candidate over the group's chunks: its coordinates are
-        // broadcast once per group, and it stops as soon as its own count
-        // exceeds K (counts stay exact up to K+1)
+        // broadcast once per group, and it skips the groups after its own
+        // count exceeds K (counts stay exact up to K+1)
         const int nk = min(PF, (total - base0 + kWave - 1) / kWave);
+        int pairs = 0; // predicate evaluations of this group (uniform)
         for (int c = 0; c < n; ++c) {
             const int cc = __builtin_amdgcn_readlane(cnt, c);
             if (cc > K)
                 continue;
             const float p3[3] = {lane_f(pc[0], c), DIM > 1 ? lane_f(pc[1], c) : 0.f,
                                  DIM > 2 ? lane_f(pc[2], c) : 0.f};
+            // the group's chunks are counted whole: branch-free, the
+            // predicate's lane mask feeds the popcount directly
             int h = 0;
 #pragma unroll
             for (int k = 0; k < PF; ++k) {
-                if (k >= nk || cc + h > K)
-                    break;
-                const int base = base0 + k * kWave;
-                const bool hit =
-                    base + lane < total && dist2<DIM>(p3, qx[k], qy[k], qz[k]) <= d.r2;
-                h += __popcll(__ballot(hit));
-                tests += min(kWave, total - base);
+                if (k < nk) { // uniform
+                    const bool hit = dist2<DIM>(p3, qx[k], qy[k], qz[k]) <= d.r2;
+                    h += __builtin_popcountll(__builtin_amdgcn_ballot_w64(hit));
+                }
             }
+            pairs += min(nk * kWave, total - base0);
             cnt += (lane == c) ? h : 0;
         }
+        tests += pairs;
         if (__ballot(lane < n && cnt <= K) == 0)
             break;
     }
