@@ -113,7 +113,7 @@ __global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_batch_detec
     __shared__ DetectLds lds;
     const BatchItem it = items[blockIdx.x];
     const BatchStream &bs = streams[it.stream];
-    detect_body<DIM, LEARN>(bs.d, batch_step(bs), it.bid, it.arg, lds);
+    detect_body<DIM, LEARN>(bs.d, batch_step(bs), it.bid, it.arg, true, lds);
 }
 
 // The batch's shared predictor (learned batches), trained on every stream's

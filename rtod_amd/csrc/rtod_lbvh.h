@@ -406,7 +406,7 @@ __global__ __launch_bounds__(kDetectNT) void k_lbvh_select(Dev d, Step st, Lbvh 
     if (!s_last)
         return;
     __threadfence();
-    finish_slide(d, st, U, s_scan);
+    finish_slide<kDetectNT>(d, st, U, s_scan);
 }
 
 } // namespace rtod

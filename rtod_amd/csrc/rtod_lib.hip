@@ -255,32 +255,37 @@ void launch_batch_merges(rtod_ctx *c, hipStream_t s, const Step &st, int mode, i
 }
 
 // slide st's window merge; next != nullptr: the next slide's tile sorts
-// ride along as extra workgroups (ride_along)
+// ride along as extra workgroups (ride_along); prev != nullptr: so does the
+// epilogue of the previous slide, whose detect left it (deferred finish)
 template <int DIM>
-void launch_merge_window_dim(rtod_ctx *c, hipStream_t s, const Step &st, const Step *next)
+void launch_merge_window_dim(rtod_ctx *c, hipStream_t s, const Step &st, const Step *next,
+                             const Step *prev)
 {
     const int nmb = (c->W + kMergeT - 1) / kMergeT;
-    if (!next) {
+    if (!next && !prev) {
         hipLaunchKernelGGL(k_merge_window<DIM>, dim3(nmb), dim3(256), 0, s, c->d, st);
         return;
     }
-    const dim3 g(nmb + c->d.cpb);
+    const dim3 g((prev ? 1 : 0) + nmb + (next ? c->d.cpb : 0));
+    const Step nx = next ? *next : Step{}, pv = prev ? *prev : Step{};
+    const int fin = prev ? 1 : 0;
     if (c->sort_shape == 0)
-        hipLaunchKernelGGL((k_merge_window_ingest<DIM, 2>), g, dim3(256), 0, s, c->d, st, *next,
-                           nmb);
+        hipLaunchKernelGGL((k_merge_window_ingest<DIM, 2>), g, dim3(256), 0, s, c->d, st, nx, pv,
+                           nmb, fin);
     else
-        hipLaunchKernelGGL((k_merge_window_ingest<DIM, 4>), g, dim3(256), 0, s, c->d, st, *next,
-                           nmb);
+        hipLaunchKernelGGL((k_merge_window_ingest<DIM, 4>), g, dim3(256), 0, s, c->d, st, nx, pv,
+                           nmb, fin);
 }
 
-void launch_merge_window(rtod_ctx *c, hipStream_t s, const Step &st, const Step *next = nullptr)
+void launch_merge_window(rtod_ctx *c, hipStream_t s, const Step &st, const Step *next = nullptr,
+                         const Step *prev = nullptr)
 {
     if (c->dim == 1)
-        launch_merge_window_dim<1>(c, s, st, next);
+        launch_merge_window_dim<1>(c, s, st, next, prev);
     else if (c->dim == 2)
-        launch_merge_window_dim<2>(c, s, st, next);
+        launch_merge_window_dim<2>(c, s, st, next, prev);
     else
-        launch_merge_window_dim<3>(c, s, st, next);
+        launch_merge_window_dim<3>(c, s, st, next, prev);
 }
 
 template <int DIM> void launch_classify_dim(rtod_ctx *c, hipStream_t s, const Step &st)
@@ -308,9 +313,10 @@ void launch_classify(rtod_ctx *c, hipStream_t s, const Step &st)
 }
 
 // slide st's detect; runs != nullptr: that slide's run merge rides along as
-// extra workgroups
+// extra workgroups (and the epilogue is left to the next launch); fin: the
+// detect runs the slide's epilogue itself
 template <int DIM>
-void launch_detect_dim(rtod_ctx *c, hipStream_t s, const Step &st, const Step *runs)
+void launch_detect_dim(rtod_ctx *c, hipStream_t s, const Step &st, const Step *runs, bool fin)
 {
     const int grid = c->detect_blocks;
     if (runs) {
@@ -324,20 +330,28 @@ void launch_detect_dim(rtod_ctx *c, hipStream_t s, const Step &st, const Step *r
                                *runs, grid);
         return;
     }
+    const int f = fin ? 1 : 0;
     if (c->learned)
-        hipLaunchKernelGGL((k_detect<DIM, true>), dim3(grid), dim3(kDetectNT), 0, s, c->d, st);
+        hipLaunchKernelGGL((k_detect<DIM, true>), dim3(grid), dim3(kDetectNT), 0, s, c->d, st, f);
     else
-        hipLaunchKernelGGL((k_detect<DIM, false>), dim3(grid), dim3(kDetectNT), 0, s, c->d, st);
+        hipLaunchKernelGGL((k_detect<DIM, false>), dim3(grid), dim3(kDetectNT), 0, s, c->d, st, f);
 }
 
-void launch_detect(rtod_ctx *c, hipStream_t s, const Step &st, const Step *runs = nullptr)
+void launch_detect(rtod_ctx *c, hipStream_t s, const Step &st, const Step *runs = nullptr,
+                   bool fin = true)
 {
     if (c->dim == 1)
-        launch_detect_dim<1>(c, s, st, runs);
+        launch_detect_dim<1>(c, s, st, runs, fin);
     else if (c->dim == 2)
-        launch_detect_dim<2>(c, s, st, runs);
+        launch_detect_dim<2>(c, s, st, runs, fin);
     else
-        launch_detect_dim<3>(c, s, st, runs);
+        launch_detect_dim<3>(c, s, st, runs, fin);
+}
+
+// slide st's epilogue on its own, after a detect that left it
+void launch_finish(rtod_ctx *c, hipStream_t s, const Step &st)
+{
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, c->d, st);
 }
 
 // LBVH index: codes, tile sort, merges up to U = W, hierarchy, boxes
@@ -423,12 +437,16 @@ void launch_ingest_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t 
 // next != nullptr (pipelined slides, ride_along): the next slide's ingest
 // stage rides in this stage's launches -- its tile sorts in the window
 // merge's, its run merge in the detect's (after the LBVH detect on its own).
+// Cell-stencil index only: prev != nullptr, the previous slide's epilogue
+// rides in the window merge's launch; defer, this slide's detect leaves its
+// own epilogue to the next slide's window merge (a run's last slide must
+// not defer).
 void launch_core_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t *ev,
-                       const Step *next)
+                       const Step *next, const Step *prev = nullptr, bool defer = false)
 {
     if (ev)
         (void)hipEventRecord(ev[2], s);
-    launch_merge_window(c, s, st, next);
+    launch_merge_window(c, s, st, next, prev);
     if (ev)
         (void)hipEventRecord(ev[3], s);
     launch_classify(c, s, st);
@@ -445,7 +463,7 @@ void launch_core_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t *e
         if (runs)
             launch_batch_merges(c, s, *next, kModeSlide, c->S);
     } else {
-        launch_detect(c, s, st, runs ? next : nullptr);
+        launch_detect(c, s, st, runs ? next : nullptr, !defer);
         if (ev) {
             (void)hipEventRecord(ev[5], s);
             (void)hipEventRecord(ev[6], s);
@@ -1037,6 +1055,10 @@ rtod_status slides_device_impl(rtod_ctx *c, const float *dev, int32_t n, int32_t
     };
     rtod_status s = RTOD_OK;
     launch_ingest_stage(c, c->stream, make_step_at(c, (uint64_t)t0, dev), nullptr);
+    // cell-stencil index: every slide but the last leaves its epilogue to
+    // the next slide's window-merge launch; pending is that slide
+    Step pending{};
+    bool has_pending = false;
     for (int32_t i = 0; i < n && s == RTOD_OK; ++i) {
         if (i >= kHostOuts && (s = collect_next()) != RTOD_OK)
             break;
@@ -1047,7 +1069,11 @@ rtod_status slides_device_impl(rtod_ctx *c, const float *dev, int32_t n, int32_t
         if ((s = guard_trainer(c, (int64_t)t, c->stream)) != RTOD_OK)
             break;
         const bool fused = more && c->ride;
-        launch_core_stage(c, c->stream, st, timed ? c->ev : nullptr, fused ? &nx : nullptr);
+        const bool defer = more && !c->lbvh;
+        launch_core_stage(c, c->stream, st, timed ? c->ev : nullptr, fused ? &nx : nullptr,
+                          has_pending ? &pending : nullptr, defer);
+        pending = st;
+        has_pending = defer;
         if (more && !c->ride)
             launch_ingest_stage(c, c->stream, nx, nullptr);
         const hipError_t e = hipGetLastError();
@@ -1064,6 +1090,8 @@ rtod_status slides_device_impl(rtod_ctx *c, const float *dev, int32_t n, int32_t
             }
         }
     }
+    if (has_pending) // stopped early: the last launched slide's epilogue
+        launch_finish(c, c->stream, pending);
     while (s == RTOD_OK && done < n)
         s = collect_next();
     if (s != RTOD_OK)

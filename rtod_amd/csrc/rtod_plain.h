@@ -601,9 +601,11 @@ struct MergeWindowLds {
     int pos0, slot0;
 };
 
+// reset: workgroup 0 clears the slide's classify counters (left to the
+// previous slide's epilogue when that runs in the same launch)
 template <int DIM>
 __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, int bid,
-                                                  MergeWindowLds &lds)
+                                                  MergeWindowLds &lds, bool reset = true)
 {
     constexpr int NT = 256, IT = 2, T = NT * IT, CAP = 2048;
     static_assert(T == kMergeT, "merge tile");
@@ -649,7 +651,7 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
     const int bper = (S + nmb - 1) / nmb;
     const int j0 = min(S, bid * bper), j1 = min(S, j0 + bper);
 
-    if (bid == 0 && tid == 0) {
+    if (reset && bid == 0 && tid == 0) {
         d.ctl->n_cand = 0;
         d.ctl->n_units = 0;
         if (d.lr.on)
@@ -1454,29 +1456,29 @@ __device__ __forceinline__ Stencil<DIM> near_stencil(const Dev &d, const Stencil
 }
 
 // ---------------------------------------------------------------------------
-// Slide epilogue, run by every thread of the last workgroup to finish the
-// detect phase: ascending slot list from the outlier bitmap written straight
-// into pinned host memory, bitmap cleared, counters published, device slide
-// counter advanced.  Each thread owns a contiguous run of bitmap words; its
-// loads are issued together.
+// Slide epilogue, run by every thread of one workgroup of NT threads once
+// the slide's detect phase is complete (the last detect workgroup, or a
+// workgroup of the next slide's first launch): ascending slot list from the
+// outlier bitmap written straight into pinned host memory, bitmap cleared,
+// counters published and reset for the next slide, device slide counter
+// advanced.  Each thread owns a contiguous run of bitmap words; its loads
+// are issued together.
 // ---------------------------------------------------------------------------
 constexpr int kCompactRegs = 16;
 
+template <int NT>
 __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int n_cand, int *s_scan)
 {
     const int tid = threadIdx.x;
     const int nwords = (d.W + 31) >> 5;
-    const int per = (nwords + kDetectNT - 1) / kDetectNT;
+    const int per = (nwords + NT - 1) / NT;
     const int w0 = min(nwords, tid * per), w1 = min(nwords, w0 + per);
     HostOut *ho = host_out(d, st.t);
     int total = 0;
     // the slide's counters, requested with the bitmap words
     unsigned long long pt = 0;
-    int err = 0;
-    if (tid == 0) {
+    if (tid == 0)
         pt = __hip_atomic_load(&d.ctl->pair_tests, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        err = __hip_atomic_load(&d.ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
     if (per <= kCompactRegs) {
         uint32_t wv[kCompactRegs];
         int mine = 0;
@@ -1490,7 +1492,7 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int n
 #pragma unroll
         for (int k = 0; k < kCompactRegs; ++k)
             mine += __popc(wv[k]);
-        int pos = block_excl_scan<kDetectNT>(mine, s_scan, total);
+        int pos = block_excl_scan<NT>(mine, s_scan, total);
 #pragma unroll
         for (int k = 0; k < kCompactRegs; ++k) {
             uint32_t bits = wv[k];
@@ -1508,7 +1510,7 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int n
         for (int w = w0; w < w1; ++w)
             mine += __popc(__hip_atomic_load(&d.bitmap[w], __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT));
-        int pos = block_excl_scan<kDetectNT>(mine, s_scan, total);
+        int pos = block_excl_scan<NT>(mine, s_scan, total);
         for (int w = w0; w < w1; ++w) {
             uint32_t bits =
                 __hip_atomic_load(&d.bitmap[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1525,15 +1527,21 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int n
     // thread that writes it (cumulative over the workgroup's list writes,
     // ordered before it by the barrier): a host that sees the stamp
     // (polling, acquire) sees the list and the counters
+    // (err is taken by an exchange: the next slide's window merge may be
+    // running beside this workgroup, and a flag it raises is reported with
+    // this slide or the next, never lost)
     __syncthreads();
     if (tid == 0) {
         ho->n_out = total;
         ho->n_cand = n_cand;
         ho->pair_tests = pt;
-        ho->err = err;
-        d.ctl->err = 0;
+        ho->err = atomicExch(&d.ctl->err, 0);
         d.ctl->pair_tests = 0;
         d.ctl->ticket = 0;
+        d.ctl->n_cand = 0;
+        d.ctl->n_units = 0;
+        if (d.lr.on)
+            d.lr.lc->n_units_i = 0;
         d.ctl->t = st.t + 1;
         __threadfence_system();
         __hip_atomic_store(&ho->slide, (int64_t)(st.t + 1), __ATOMIC_RELAXED,
@@ -1541,12 +1549,59 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int n
     }
 }
 
+// The learned schedule's per-slide bookkeeping (replay bucket sizes, miss
+// count, the fallback rule for the next slide), then finish_slide.  n_o /
+// n_units: the slide's probable-outlier and total unit counts; use_pred:
+// whether its detect followed the prediction.
+template <int NT>
+__device__ __forceinline__ void slide_epilogue(const Dev &d, const Step &st, int n_o, int n_units,
+                                               bool learn, bool use_pred, int *s_scan)
+{
+    if (threadIdx.x == 0) {
+        HostOut *ho = host_out(d, st.t);
+        if (learn) {
+            LearnCtl *lc = d.lr.lc;
+            const int slot = (int)(st.t % kReplaySlides);
+            const int miss =
+                __hip_atomic_load(&lc->misses, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            d.lr.rep_n[2 * slot] = n_o;
+            d.lr.rep_n[2 * slot + 1] = n_units - n_o;
+            ho->misses = miss;
+            ho->used_pred = use_pred ? 1 : 0;
+            ho->train_steps = d.lr.ls->steps_total;
+            ho->train_loss = d.lr.ls->loss_last;
+            // fallback rule: schedule the next slide by prediction only if
+            // this slide's miss rate (over units) is within the threshold
+            if (n_units > 0)
+                lc->use_pred = ((float)miss <= d.lr.miss_thr * (float)n_units) ? 1 : 0;
+            lc->misses = 0;
+        }
+        ho->n_units = n_units;
+    }
+    finish_slide<NT>(d, st, d.ctl->n_cand, s_scan);
+}
+
+// The epilogue of slide st run after its detect launch (pipelined slides: a
+// workgroup of the next slide's window-merge launch), from the counters the
+// slide's classify and detect left in memory.
+template <int NT>
+__device__ __forceinline__ void finish_deferred(const Dev &d, const Step &st, int *s_scan)
+{
+    const bool learn = d.lr.on != 0;
+    const int n_o = d.ctl->n_units;
+    const int n_units = learn ? n_o + d.lr.lc->n_units_i : n_o;
+    const bool use_pred = learn && d.lr.lc->use_pred != 0;
+    slide_epilogue<NT>(d, st, n_o, n_units, learn, use_pred, s_scan);
+}
+
 // ---------------------------------------------------------------------------
 // k_detect: one wavefront per cell unit (grid-stride over the unit list):
 // the unit's candidates are counted together over their union stencil
-// (unit_count).  Only ceil(n_units / 4) workgroups take part (the rest leave
-// at once); the last of them to finish compacts the outlier bitmap into an
-// ascending slot list written straight into pinned host memory.
+// (unit_count).  Only ceil(n_units / 8) workgroups take part (the rest leave
+// at once).  fin: the last of them to finish runs the slide epilogue
+// (slide_epilogue: the outlier bitmap compacted into an ascending slot list
+// written straight into pinned host memory); otherwise the next launch on
+// the stream runs it (finish_deferred).
 // LEARN: the list is the probable-outlier units [0, nO) followed by the
 // probable-inlier units (indices W-1, W-2, ...): the long full counts are
 // handed out first.  When the schedule is on (lc->use_pred), a probable-
@@ -1569,7 +1624,7 @@ struct DetectLds {
 // phase of this stream
 template <int DIM, bool LEARN>
 __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bid, int nblk,
-                                            DetectLds &lds)
+                                            bool fin, DetectLds &lds)
 {
     constexpr int NW = kDetectNT / kWave;
     uint64_t *s_samp = lds.samp;
@@ -1671,67 +1726,71 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
             atomicAdd(&d.ctl->pair_tests, s_tests);
         if (LEARN && s_miss)
             atomicAdd(&d.lr.lc->misses, s_miss);
-        __threadfence();
-        s_last = (atomicAdd(&d.ctl->ticket, 1u) == (unsigned)busy - 1);
+        if (fin) {
+            __threadfence();
+            s_last = (atomicAdd(&d.ctl->ticket, 1u) == (unsigned)busy - 1);
+        }
     }
+    if (!fin)
+        return;
     __syncthreads();
     if (!s_last)
         return;
     __threadfence();
-
-    if constexpr (LEARN) {
-        if (tid == 0) {
-            LearnCtl *lc = d.lr.lc;
-            const int slot = (int)(st.t % kReplaySlides);
-            const int miss =
-                __hip_atomic_load(&lc->misses, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            d.lr.rep_n[2 * slot] = n_o;
-            d.lr.rep_n[2 * slot + 1] = n_units - n_o;
-            HostOut *ho = host_out(d, st.t);
-            ho->misses = miss;
-            ho->used_pred = use_pred ? 1 : 0;
-            ho->train_steps = d.lr.ls->steps_total;
-            ho->train_loss = d.lr.ls->loss_last;
-            // fallback rule: schedule the next slide by prediction only if
-            // this slide's miss rate (over units) is within the threshold
-            if (n_units > 0)
-                lc->use_pred = ((float)miss <= d.lr.miss_thr * (float)n_units) ? 1 : 0;
-            lc->misses = 0;
-        }
-    }
-    if (tid == 0)
-        host_out(d, st.t)->n_units = n_units;
-    finish_slide(d, st, d.ctl->n_cand, s_scan);
+    slide_epilogue<kDetectNT>(d, st, n_o, n_units, LEARN, use_pred, s_scan);
 }
 
 template <int DIM, bool LEARN>
-__global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_detect(Dev d, Step st)
+__global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_detect(Dev d, Step st,
+                                                                           int fin)
 {
     __shared__ DetectLds lds;
-    detect_body<DIM, LEARN>(d, st, blockIdx.x, gridDim.x, lds);
+    detect_body<DIM, LEARN>(d, st, blockIdx.x, gridDim.x, fin != 0, lds);
 }
 
 // ---------------------------------------------------------------------------
-// Core-stage launches carrying the next slide's ingest stage in extra
+// Core-stage launches carrying work of the neighbouring slides in extra
 // workgroups (pipelined slides on one stream): slide t's window merge with
-// slide t+1's tile sorts, slide t's detect with slide t+1's run merge.  The
-// halves touch disjoint memory: the ingest writes batch-ring entry enew(t+1)
-// and the tile scratch; slide t's merge reads entries enew(t) and eold(t)
-// (distinct from enew(t+1) with m + 2 entries) and its detect the window.
+// slide t-1's epilogue and slide t+1's tile sorts, slide t's detect with
+// slide t+1's run merge.  The parts touch disjoint memory: the ingest writes
+// batch-ring entry enew(t+1) and the tile scratch; slide t's merge reads
+// entries enew(t) and eold(t) (distinct from enew(t+1) with m + 2 entries)
+// and writes the other window buffer; slide t-1's epilogue reads and clears
+// the outlier bitmap and the slide counters (slide t's classify, the next
+// launch, is their first user) and writes host-out buffer t-1.
 // ---------------------------------------------------------------------------
+// fin_prev: workgroup 0 runs slide st_prev's epilogue; then nmb merge
+// workgroups, then the tile sorts of st_next (as many as the grid has left)
 template <int DIM, int IT>
 __global__ __launch_bounds__(256) void k_merge_window_ingest(Dev d, Step st, Step st_next,
-                                                            int nmb)
+                                                            Step st_prev, int nmb, int fin_prev)
 {
     __shared__ union {
         MergeWindowLds mw;
         IngestLds<DIM, 256, IT> ing;
+        int scan[256 / kWave];
     } lds;
-    if ((int)blockIdx.x < nmb)
-        merge_window_body<DIM>(d, st, blockIdx.x, lds.mw);
+    int bid = blockIdx.x;
+    if (fin_prev) {
+        if (bid == 0) {
+            finish_deferred<256>(d, st_prev, lds.scan);
+            return;
+        }
+        --bid;
+    }
+    if (bid < nmb)
+        merge_window_body<DIM>(d, st, bid, lds.mw, !fin_prev);
     else
         ingest_sort_body<DIM, 256, IT>(d, st_next, kModeSlide, nullptr, d.cpb == 1 ? 1 : 0,
-                                       blockIdx.x - nmb, lds.ing);
+                                       bid - nmb, lds.ing);
+}
+
+// slide st's epilogue in a launch of its own (a pipelined run stopped
+// before the slide that would carry it)
+__global__ __launch_bounds__(256) void k_finish(Dev d, Step st)
+{
+    __shared__ int s_scan[256 / kWave];
+    finish_deferred<256>(d, st, s_scan);
 }
 
 static_assert(kRunMergeNT == kDetectNT, "the run merge rides in detect workgroups");
@@ -1752,7 +1811,8 @@ __global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_detect_runs
 {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     if ((int)blockIdx.x < ndet)
-        detect_body<DIM, LEARN>(d, st, blockIdx.x, ndet, *reinterpret_cast<DetectLds *>(dyn_lds));
+        detect_body<DIM, LEARN>(d, st, blockIdx.x, ndet, false,
+                                *reinterpret_cast<DetectLds *>(dyn_lds));
     else
         merge_runs_body<DIM>(d, st_next, kModeSlide, blockIdx.x - ndet,
                              reinterpret_cast<uint64_t *>(dyn_lds));
