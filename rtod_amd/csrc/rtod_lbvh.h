@@ -379,34 +379,19 @@ __global__ __launch_bounds__(256) void k_lbvh_query(Dev d, Step st, Lbvh L)
         atomicAdd(&d.ctl->pair_tests, (unsigned long long)tests);
 }
 
-// outlier bits from the leaf counts, then the last workgroup compacts
+// outlier bits from the leaf counts (the slide epilogue, k_finish, follows)
 __global__ __launch_bounds__(kDetectNT) void k_lbvh_select(Dev d, Step st, Lbvh L)
 {
-    __shared__ int s_scan[kDetectNT / kWave];
-    __shared__ int s_last;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int U = d.ctl->n_cand;
-    const int busy = min((int)gridDim.x, max(1, (U + kDetectNT - 1) / kDetectNT));
-    if ((int)blockIdx.x >= busy)
-        return;
     if (i < U) {
         const int c = L.count[i];
         d.cand_cnt[L.item[1][i]] = min(c, d.K + 1);
         if (c <= d.K) {
             const int slot = d.aslot[st.par ^ 1][d.cand[L.item[1][i]]];
-            atomicOr(&d.bitmap[slot >> 5], 1u << (slot & 31));
+            atomicOr(&bitmap_of(d, st.t)[slot >> 5], 1u << (slot & 31));
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        s_last = (atomicAdd(&d.ctl->ticket, 1u) == (unsigned)busy - 1);
-    }
-    __syncthreads();
-    if (!s_last)
-        return;
-    __threadfence();
-    finish_slide<kDetectNT>(d, st, U, s_scan);
 }
 
 } // namespace rtod

@@ -266,9 +266,9 @@ void launch_merge_window_dim(rtod_ctx *c, hipStream_t s, const Step &st, const S
         hipLaunchKernelGGL(k_merge_window<DIM>, dim3(nmb), dim3(256), 0, s, c->d, st);
         return;
     }
-    const dim3 g((prev ? 1 : 0) + nmb + (next ? c->d.cpb : 0));
+    const int fin = prev ? c->d.fin_parts : 0;
+    const dim3 g(fin + nmb + (next ? c->d.cpb : 0));
     const Step nx = next ? *next : Step{}, pv = prev ? *prev : Step{};
-    const int fin = prev ? 1 : 0;
     if (c->sort_shape == 0)
         hipLaunchKernelGGL((k_merge_window_ingest<DIM, 2>), g, dim3(256), 0, s, c->d, st, nx, pv,
                            nmb, fin);
@@ -351,7 +351,7 @@ void launch_detect(rtod_ctx *c, hipStream_t s, const Step &st, const Step *runs 
 // slide st's epilogue on its own, after a detect that left it
 void launch_finish(rtod_ctx *c, hipStream_t s, const Step &st)
 {
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, c->d, st);
+    hipLaunchKernelGGL(k_finish, dim3(c->d.fin_parts), dim3(256), 0, s, c->d, st);
 }
 
 // LBVH index: codes, tile sort, merges up to U = W, hierarchy, boxes
@@ -392,6 +392,7 @@ void launch_lbvh_detect(rtod_ctx *c, hipStream_t s, const Step &st)
     else
         hipLaunchKernelGGL(k_lbvh_query<3>, gw, b, 0, s, c->d, st, c->L);
     hipLaunchKernelGGL(k_lbvh_select, gw, dim3(kDetectNT), 0, s, c->d, st, c->L);
+    launch_finish(c, s, st);
 }
 
 // slide t's quantities; src: its new points in HBM, or nullptr for the
@@ -438,9 +439,9 @@ void launch_ingest_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t 
 // stage rides in this stage's launches -- its tile sorts in the window
 // merge's, its run merge in the detect's (after the LBVH detect on its own).
 // Cell-stencil index only: prev != nullptr, the previous slide's epilogue
-// rides in the window merge's launch; defer, this slide's detect leaves its
-// own epilogue to the next slide's window merge (a run's last slide must
-// not defer).
+// rides in the window merge's launch; defer, this slide's epilogue is left
+// to the next slide's window merge (a run's last slide must not defer),
+// else it follows the detect in a launch of its own.
 void launch_core_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t *ev,
                        const Step *next, const Step *prev = nullptr, bool defer = false)
 {
@@ -463,7 +464,9 @@ void launch_core_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t *e
         if (runs)
             launch_batch_merges(c, s, *next, kModeSlide, c->S);
     } else {
-        launch_detect(c, s, st, runs ? next : nullptr, !defer);
+        launch_detect(c, s, st, runs ? next : nullptr, false);
+        if (!defer)
+            launch_finish(c, s, st);
         if (ev) {
             (void)hipEventRecord(ev[5], s);
             (void)hipEventRecord(ev[6], s);
@@ -639,10 +642,13 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
     RTOD_TRY(dalloc(c, &d.cand, W));
     RTOD_TRY(dalloc(c, &d.units, W));
     RTOD_TRY(dalloc(c, &d.cand_cnt, W));
-    RTOD_TRY(dalloc(c, &d.bitmap, (W + 31) / 32));
+    d.bm_words = (((W + 31) / 32) + 63) / 64 * 64;
+    // one epilogue part per chunk of the bitmap
+    d.fin_parts = std::min(32, std::max(1, (int)(((W + 31) / 32 + kChunkWords - 1) / kChunkWords)));
+    RTOD_TRY(dalloc(c, &d.bitmap, (size_t)2 * d.bm_words));
     RTOD_TRY(dalloc(c, &d.ctl, 1));
     RTOD_TRY(dalloc(c, &c->d_prime, dim * W));
-    RTOD_HIP(hipMemsetAsync(d.bitmap, 0, sizeof(uint32_t) * ((W + 31) / 32), c->stream));
+    RTOD_HIP(hipMemsetAsync(d.bitmap, 0, sizeof(uint32_t) * 2 * d.bm_words, c->stream));
     RTOD_HIP(hipMemsetAsync(d.ctl, 0, sizeof(Ctl), c->stream));
 
     c->lbvh = cfg->index_kind == RTOD_INDEX_LBVH;
@@ -764,7 +770,7 @@ rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n,
     hipLaunchKernelGGL(k_window_samples,
                        dim3((std::max(c->d.nsamp, c->d.nsamp4) + 255) / 256), dim3(256), 0, s, c->d,
                        0);
-    RTOD_HIP(hipMemsetAsync(c->d.bitmap, 0, sizeof(uint32_t) * ((c->W + 31) / 32), s));
+    RTOD_HIP(hipMemsetAsync(c->d.bitmap, 0, sizeof(uint32_t) * 2 * c->d.bm_words, s));
     RTOD_HIP(hipMemsetAsync(c->d.ctl, 0, sizeof(Ctl), s));
     if (c->learned) { // replay and per-slide counters restart; weights stay
         RTOD_HIP(hipMemsetAsync(c->d.lr.rep_n, 0, sizeof(int32_t) * 2 * kReplaySlides, s));

@@ -657,6 +657,14 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
         if (d.lr.on)
             d.lr.lc->n_units_i = 0;
     }
+    // this slide's outlier bitmap, last read by the epilogue of the slide
+    // two earlier: cleared here, T / 32 words per workgroup
+    {
+        static_assert(T / 32 <= NT, "one word per thread");
+        const int wi = bid * (T / 32) + tid;
+        if (tid < T / 32 && wi < d.bm_words)
+            bitmap_of(d, st.t)[wi] = 0u;
+    }
 
     // round 1: the chunk (keys, slots, batch positions, coordinates; striped,
     // item i of thread t is chunk element i*NT + t), the share of B, and the
@@ -1456,142 +1464,318 @@ __device__ __forceinline__ Stencil<DIM> near_stencil(const Dev &d, const Stencil
 }
 
 // ---------------------------------------------------------------------------
-// Slide epilogue, run by every thread of one workgroup of NT threads once
-// the slide's detect phase is complete (the last detect workgroup, or a
-// workgroup of the next slide's first launch): ascending slot list from the
-// outlier bitmap written straight into pinned host memory, bitmap cleared,
-// counters published and reset for the next slide, device slide counter
-// advanced.  Each thread owns a contiguous run of bitmap words; its loads
-// are issued together.
+// Slide epilogue, in F parts of NT threads (one workgroup each) once the
+// slide's detect phase is complete: the last detect workgroup (F = 1), or
+// F workgroups of the next slide's window-merge launch.  The ascending slot
+// list is written from the slide's outlier bitmap straight into pinned host
+// memory; the last part to finish publishes the counters, resets them for
+// the next slide and stamps the slide.  The bitmap is cut into chunks of
+// kChunkWords words (thread w holds word c * kChunkWords + w of chunk c),
+// chunk c taken by part
+// c mod F: a part ranks its chunk's bits among the slide's (a block sum of
+// the popcounts of the words before it, a block scan within it), stages the
+// chunk's slots in LDS and writes them out by consecutive threads
+// (coalesced writes into host memory).  The bitmap is left set: the window
+// merge of the slide two later, its next user, clears it.
 // ---------------------------------------------------------------------------
-constexpr int kCompactRegs = 16;
+constexpr int kCompactRegs = 16;  // chunks whose words a thread keeps in registers
+constexpr int kChunkWords = 256;  // bitmap words per chunk
+constexpr int kFinishCap = 32 * kChunkWords; // slots of one chunk, staged in LDS
+
+struct FinishLds {
+    int list[kFinishCap];
+    int red[4][1024 / kWave];
+    int scan[1024 / kWave];
+    int last;
+};
+
+// what the last part publishes besides the list: n_o / n_units, the slide's
+// probable-outlier and total unit counts; use_pred, whether its detect
+// followed the prediction (learned schedule)
+struct SlideSummary {
+    int n_o, n_units;
+    bool learn, use_pred;
+};
+
+// block sums of a and b and block OR of m (wave-uniform m); every thread
+// gets all three
+template <int NT>
+__device__ __forceinline__ void block_sum2(int a, int b, uint32_t m, int (*s_red)[1024 / kWave],
+                                           int &sa, int &sb, uint32_t &om)
+{
+    const int lane = lane_id(), wid = threadIdx.x / kWave;
+#pragma unroll
+    for (int o = kWave / 2; o >= 1; o >>= 1) {
+        a += __shfl_xor(a, o, kWave);
+        b += __shfl_xor(b, o, kWave);
+    }
+    if (lane == 0) {
+        s_red[0][wid] = a;
+        s_red[1][wid] = b;
+        s_red[2][wid] = (int)m;
+    }
+    __syncthreads();
+    sa = 0;
+    sb = 0;
+    om = 0;
+#pragma unroll
+    for (int w = 0; w < NT / kWave; ++w) {
+        sa += s_red[0][w];
+        sb += s_red[1][w];
+        om |= (uint32_t)s_red[2][w];
+    }
+    __syncthreads();
+}
+
+// Exclusive block scan of four words per thread, each packing four 8-bit
+// counters; the caller keeps every column sum below 256, so no lane
+// carries into the next.  tot: the block totals.
+template <int NT>
+__device__ __forceinline__ void block_excl_scan4(uint32_t (&v)[4], int (*s_red)[1024 / kWave],
+                                                 uint32_t (&tot)[4])
+{
+    const int lane = lane_id(), wid = threadIdx.x / kWave;
+    uint32_t inc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        inc[q] = v[q];
+#pragma unroll
+    for (int dd = 1; dd < kWave; dd <<= 1) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t y = __shfl_up(inc[q], dd, kWave);
+            inc[q] += lane >= dd ? y : 0u;
+        }
+    }
+    if (lane == kWave - 1) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            s_red[q][wid] = (int)inc[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        uint32_t pre = 0, t = 0;
+#pragma unroll
+        for (int w = 0; w < NT / kWave; ++w) {
+            const uint32_t u = (uint32_t)s_red[q][w];
+            pre += (w < wid) ? u : 0u;
+            t += u;
+        }
+        tot[q] = t;
+        v[q] = pre + inc[q] - v[q];
+    }
+    __syncthreads();
+}
+
+// a slide with at most this many outliers is listed by one part in one
+// pass (block_excl_scan4 ranks the bits of all chunks at once)
+constexpr int kSparseList = 255;
+
+// chunk c (this thread's word: wd) to list positions [P, P + its bits);
+// returns its bit count
+template <int NT>
+__device__ __forceinline__ int emit_chunk(HostOut *ho, FinishLds &fl, int c, uint32_t wd, int P)
+{
+    int ctot;
+    int p = block_excl_scan<NT>(__popc(wd), fl.scan, ctot);
+    const int word = c * kChunkWords + (int)threadIdx.x;
+    while (wd) {
+        fl.list[p++] = (word << 5) + __builtin_ctz(wd);
+        wd &= wd - 1;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < ctot; i += NT)
+        ho->list[P + i] = fl.list[i];
+    __syncthreads();
+    return ctot;
+}
 
 template <int NT>
-__device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int n_cand, int *s_scan)
+__device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int f, int F,
+                                             const SlideSummary &sm, FinishLds &fl)
 {
+    static_assert(NT >= kChunkWords, "a word of the chunk per thread");
     const int tid = threadIdx.x;
     const int nwords = (d.W + 31) >> 5;
-    const int per = (nwords + NT - 1) / NT;
-    const int w0 = min(nwords, tid * per), w1 = min(nwords, w0 + per);
+    const int nch = (nwords + kChunkWords - 1) / kChunkWords;
+    // thread tid holds word tid of each chunk (none past the chunk)
+    const int nmine = tid < kChunkWords ? nwords : 0;
+    const uint32_t *bm = bitmap_of(d, st.t);
     HostOut *ho = host_out(d, st.t);
-    int total = 0;
-    // the slide's counters, requested with the bitmap words
-    unsigned long long pt = 0;
-    if (tid == 0)
-        pt = __hip_atomic_load(&d.ctl->pair_tests, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (per <= kCompactRegs) {
+    int P = 0, total = 0;
+    bool single = F == 1; // this part lists the whole slide
+    int wrote = 0;        // list entries this part wrote
+    if (nch <= kCompactRegs) {
         uint32_t wv[kCompactRegs];
-        int mine = 0;
-#pragma unroll
-        for (int k = 0; k < kCompactRegs; ++k) {
-            wv[k] = 0;
-            if (w0 + k < w1)
-                wv[k] = __hip_atomic_load(&d.bitmap[w0 + k], __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-        }
 #pragma unroll
         for (int k = 0; k < kCompactRegs; ++k)
-            mine += __popc(wv[k]);
-        int pos = block_excl_scan<NT>(mine, s_scan, total);
+            wv[k] = (k < nch && k * kChunkWords + tid < nmine) ? bm[k * kChunkWords + tid] : 0u;
+        int before = 0, all = 0;
+        uint32_t nz = 0; // bit k: chunk k holds outlier bits (this wave's words)
 #pragma unroll
         for (int k = 0; k < kCompactRegs; ++k) {
-            uint32_t bits = wv[k];
-            if (bits)
-                __hip_atomic_store(&d.bitmap[w0 + k], 0u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            while (bits) {
-                const int b = __builtin_ctz(bits);
-                bits &= bits - 1;
-                ho->list[pos++] = ((w0 + k) << 5) + b;
+            const int b = __popc(wv[k]);
+            all += b;
+            before += (k < f) ? b : 0;
+            nz |= __ballot(wv[k] != 0u) ? (1u << k) : 0u;
+        }
+        block_sum2<NT>(before, all, nz, fl.red, P, total, nz);
+        if (total <= kSparseList) {
+            // part 0 alone, one pass: bit b of word w of chunk k goes to
+            // (bits of chunks < k) + (bits of chunk k in threads < w) +
+            // popc(word below b); the per-chunk prefixes over threads come
+            // from one scan of the packed counts (<= 255 each)
+            if (f != 0)
+                return;
+            single = true;
+            uint32_t pk[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int k = 0; k < kCompactRegs; ++k)
+                pk[k >> 2] |= (uint32_t)__popc(wv[k]) << (8 * (k & 3));
+            uint32_t tot[4];
+            block_excl_scan4<NT>(pk, fl.red, tot);
+            int cpre = 0;
+#pragma unroll
+            for (int k = 0; k < kCompactRegs; ++k) {
+                int p = cpre + (int)((pk[k >> 2] >> (8 * (k & 3))) & 255u);
+                uint32_t bits = wv[k];
+                const int word = k * kChunkWords + tid;
+                while (bits) {
+                    fl.list[p++] = (word << 5) + __builtin_ctz(bits);
+                    bits &= bits - 1;
+                }
+                cpre += (int)((tot[k >> 2] >> (8 * (k & 3))) & 255u);
+            }
+            __syncthreads();
+            for (int i = tid; i < total; i += NT)
+                ho->list[i] = fl.list[i];
+            wrote = total;
+        } else {
+            for (int c = f; c < nch; c += F) {
+                uint32_t wd = 0;
+                int mid = 0; // bits of the chunks up to this part's next one
+#pragma unroll
+                for (int k = 0; k < kCompactRegs; ++k) {
+                    wd = (k == c) ? wv[k] : wd;
+                    mid += (k > c && k < c + F) ? __popc(wv[k]) : 0;
+                }
+                if ((nz >> c) & 1u) { // empty chunks are skipped (uniform)
+                    const int n = emit_chunk<NT>(ho, fl, c, wd, P);
+                    P += n;
+                    wrote += n;
+                }
+                const uint32_t between = (nz >> (c + 1)) & ((1u << (F - 1)) - 1u);
+                if (F > 1 && c + F < nch && between) {
+                    int m, unused;
+                    uint32_t un2;
+                    block_sum2<NT>(mid, 0, 0u, fl.red, m, unused, un2);
+                    P += m;
+                }
             }
         }
     } else {
-        int mine = 0;
-        for (int w = w0; w < w1; ++w)
-            mine += __popc(__hip_atomic_load(&d.bitmap[w], __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT));
-        int pos = block_excl_scan<NT>(mine, s_scan, total);
-        for (int w = w0; w < w1; ++w) {
-            uint32_t bits =
-                __hip_atomic_load(&d.bitmap[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (bits)
-                __hip_atomic_store(&d.bitmap[w], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            while (bits) {
-                const int b = __builtin_ctz(bits);
-                bits &= bits - 1;
-                ho->list[pos++] = (w << 5) + b;
+        int before = 0, all = 0;
+        for (int k = 0; k < nch; ++k) {
+            const int wi = k * kChunkWords + tid;
+            const int b = wi < nmine ? __popc(bm[wi]) : 0;
+            all += b;
+            before += (k < f) ? b : 0;
+        }
+        uint32_t unused;
+        block_sum2<NT>(before, all, 0u, fl.red, P, total, unused);
+        for (int c = f; c < nch; c += F) {
+            const int wi = c * kChunkWords + tid;
+            const int n = emit_chunk<NT>(ho, fl, c, wi < nmine ? bm[wi] : 0u, P);
+            P += n;
+            wrote += n;
+            if (F > 1 && c + F < nch) {
+                int mid = 0;
+                for (int k = c + 1; k < c + F; ++k) {
+                    const int wk = k * kChunkWords + tid;
+                    mid += wk < nmine ? __popc(bm[wk]) : 0;
+                }
+                int m, un1;
+                uint32_t un2;
+                block_sum2<NT>(mid, 0, 0u, fl.red, m, un1, un2);
+                P += m;
             }
         }
     }
-    // the slide stamp is written last, after one system-scope release by the
-    // thread that writes it (cumulative over the workgroup's list writes,
-    // ordered before it by the barrier): a host that sees the stamp
-    // (polling, acquire) sees the list and the counters
-    // (err is taken by an exchange: the next slide's window merge may be
-    // running beside this workgroup, and a flag it raises is reported with
-    // this slide or the next, never lost)
+    // every wave's list writes issued and acknowledged; a part of several
+    // that wrote any then releases them at system scope (a release only: no
+    // cache invalidation under the window merge running beside it) before
+    // taking its ticket.  (Acknowledged writes to host memory are not yet
+    // visible to the host without the release.)
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if (tid == 0) {
-        ho->n_out = total;
-        ho->n_cand = n_cand;
-        ho->pair_tests = pt;
-        ho->err = atomicExch(&d.ctl->err, 0);
-        d.ctl->pair_tests = 0;
-        d.ctl->ticket = 0;
-        d.ctl->n_cand = 0;
-        d.ctl->n_units = 0;
-        if (d.lr.on)
-            d.lr.lc->n_units_i = 0;
-        d.ctl->t = st.t + 1;
-        __threadfence_system();
-        __hip_atomic_store(&ho->slide, (int64_t)(st.t + 1), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_SYSTEM);
-    }
-}
-
-// The learned schedule's per-slide bookkeeping (replay bucket sizes, miss
-// count, the fallback rule for the next slide), then finish_slide.  n_o /
-// n_units: the slide's probable-outlier and total unit counts; use_pred:
-// whether its detect followed the prediction.
-template <int NT>
-__device__ __forceinline__ void slide_epilogue(const Dev &d, const Step &st, int n_o, int n_units,
-                                               bool learn, bool use_pred, int *s_scan)
-{
-    if (threadIdx.x == 0) {
-        HostOut *ho = host_out(d, st.t);
-        if (learn) {
-            LearnCtl *lc = d.lr.lc;
-            const int slot = (int)(st.t % kReplaySlides);
-            const int miss =
-                __hip_atomic_load(&lc->misses, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            d.lr.rep_n[2 * slot] = n_o;
-            d.lr.rep_n[2 * slot + 1] = n_units - n_o;
-            ho->misses = miss;
-            ho->used_pred = use_pred ? 1 : 0;
-            ho->train_steps = d.lr.ls->steps_total;
-            ho->train_loss = d.lr.ls->loss_last;
-            // fallback rule: schedule the next slide by prediction only if
-            // this slide's miss rate (over units) is within the threshold
-            if (n_units > 0)
-                lc->use_pred = ((float)miss <= d.lr.miss_thr * (float)n_units) ? 1 : 0;
-            lc->misses = 0;
+        bool last = true;
+        if (!single) {
+            if (wrote)
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+            last = atomicAdd(&d.ctl->fin_ticket, 1u) == (unsigned)F - 1;
         }
-        ho->n_units = n_units;
+        fl.last = last;
     }
-    finish_slide<NT>(d, st, d.ctl->n_cand, s_scan);
+    __syncthreads();
+    if (!fl.last || tid != 0)
+        return;
+    // the last part: counters, learned bookkeeping, then the stamp, written
+    // after one system-scope release (cumulative over the parts' list
+    // writes): a host that sees the stamp (polling, acquire) sees the list
+    // and the counters.  err is taken by an exchange: the next slide's
+    // window merge may run beside the epilogue, and a flag it raises is
+    // reported with this slide or the next, never lost.  (The other parts
+    // wrote nothing this part reads; the counters were written by earlier
+    // launches or by atomics.)
+    if (sm.learn) {
+        LearnCtl *lc = d.lr.lc;
+        const int slot = (int)(st.t % kReplaySlides);
+        const int miss = __hip_atomic_load(&lc->misses, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        d.lr.rep_n[2 * slot] = sm.n_o;
+        d.lr.rep_n[2 * slot + 1] = sm.n_units - sm.n_o;
+        ho->misses = miss;
+        ho->used_pred = sm.use_pred ? 1 : 0;
+        ho->train_steps = d.lr.ls->steps_total;
+        ho->train_loss = d.lr.ls->loss_last;
+        // fallback rule: schedule the next slide by prediction only if this
+        // slide's miss rate (over units) is within the threshold
+        if (sm.n_units > 0)
+            lc->use_pred = ((float)miss <= d.lr.miss_thr * (float)sm.n_units) ? 1 : 0;
+        lc->misses = 0;
+        lc->n_units_i = 0;
+    }
+    ho->n_units = sm.n_units;
+    ho->n_out = total;
+    ho->n_cand = d.ctl->n_cand;
+    ho->pair_tests = __hip_atomic_load(&d.ctl->pair_tests, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+    ho->err = atomicExch(&d.ctl->err, 0);
+    d.ctl->pair_tests = 0;
+    d.ctl->ticket = 0;
+    d.ctl->fin_ticket = 0;
+    d.ctl->n_cand = 0;
+    d.ctl->n_units = 0;
+    d.ctl->t = st.t + 1;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    __hip_atomic_store(&ho->slide, (int64_t)(st.t + 1), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// The epilogue of slide st run after its detect launch (pipelined slides: a
-// workgroup of the next slide's window-merge launch), from the counters the
-// slide's classify and detect left in memory.
+// Part f of F of slide st's epilogue, run after its detect launch
+// (pipelined slides: workgroups of the next slide's window-merge launch),
+// from the counters the slide's classify and detect left in memory.
 template <int NT>
-__device__ __forceinline__ void finish_deferred(const Dev &d, const Step &st, int *s_scan)
+__device__ __forceinline__ void finish_deferred(const Dev &d, const Step &st, int f, int F,
+                                                FinishLds &fl)
 {
-    const bool learn = d.lr.on != 0;
-    const int n_o = d.ctl->n_units;
-    const int n_units = learn ? n_o + d.lr.lc->n_units_i : n_o;
-    const bool use_pred = learn && d.lr.lc->use_pred != 0;
-    slide_epilogue<NT>(d, st, n_o, n_units, learn, use_pred, s_scan);
+    SlideSummary sm;
+    sm.learn = d.lr.on != 0;
+    sm.n_o = d.ctl->n_units;
+    sm.n_units = sm.learn ? sm.n_o + d.lr.lc->n_units_i : sm.n_o;
+    sm.use_pred = sm.learn && d.lr.lc->use_pred != 0;
+    finish_slide<NT>(d, st, f, F, sm, fl);
 }
 
 // ---------------------------------------------------------------------------
@@ -1612,9 +1796,11 @@ __device__ __forceinline__ void finish_deferred(const Dev &d, const Step &st, in
 // ---------------------------------------------------------------------------
 struct DetectLds {
     static constexpr int NW = kDetectNT / kWave;
-    uint64_t samp[kMaxSamples];
+    union { // the sample table, then (last workgroup) the epilogue's
+        uint64_t samp[kMaxSamples];
+        FinishLds fin;
+    };
     int rows[NW][2][64];
-    int scan[NW];
     int last;
     int miss;
     unsigned long long tests;
@@ -1629,7 +1815,6 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
     constexpr int NW = kDetectNT / kWave;
     uint64_t *s_samp = lds.samp;
     auto &s_rows = lds.rows;
-    int *s_scan = lds.scan;
     int &s_last = lds.last;
     unsigned long long &s_tests = lds.tests;
     int &s_miss = lds.miss;
@@ -1642,6 +1827,7 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
     const float *Nx = d.axyz[par];
     const int W = d.W, K = d.K;
     const int n_o = d.ctl->n_units;
+    uint32_t *bm = bitmap_of(d, st.t);
     const int n_units = LEARN ? n_o + d.lr.lc->n_units_i : n_o;
     const bool use_pred = LEARN && d.lr.lc->use_pred != 0;
     const int busy = min(nblk, max(1, (n_units + NW - 1) / NW));
@@ -1702,7 +1888,7 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
         }
         const int capped = lane < n ? min(cnt, K + 1) : K + 1;
         if (lane < n && cnt <= K)
-            atomicOr(&d.bitmap[cslot >> 5], 1u << (cslot & 31));
+            atomicOr(&bm[cslot >> 5], 1u << (cslot & 31));
         const int umin = unit_min_i(capped);
         if (lane == 0) {
             d.cand_cnt[u] = umin;
@@ -1737,7 +1923,7 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
     if (!s_last)
         return;
     __threadfence();
-    slide_epilogue<kDetectNT>(d, st, n_o, n_units, LEARN, use_pred, s_scan);
+    finish_slide<kDetectNT>(d, st, 0, 1, SlideSummary{n_o, n_units, LEARN, use_pred}, lds.fin);
 }
 
 template <int DIM, bool LEARN>
@@ -1759,8 +1945,9 @@ __global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_detect(Dev 
 // the outlier bitmap and the slide counters (slide t's classify, the next
 // launch, is their first user) and writes host-out buffer t-1.
 // ---------------------------------------------------------------------------
-// fin_prev: workgroup 0 runs slide st_prev's epilogue; then nmb merge
-// workgroups, then the tile sorts of st_next (as many as the grid has left)
+// fin_prev > 0: workgroups [0, fin_prev) run slide st_prev's epilogue, in
+// fin_prev parts; then nmb merge workgroups, then the tile sorts of st_next
+// (as many as the grid has left)
 template <int DIM, int IT>
 __global__ __launch_bounds__(256) void k_merge_window_ingest(Dev d, Step st, Step st_next,
                                                             Step st_prev, int nmb, int fin_prev)
@@ -1768,29 +1955,27 @@ __global__ __launch_bounds__(256) void k_merge_window_ingest(Dev d, Step st, Ste
     __shared__ union {
         MergeWindowLds mw;
         IngestLds<DIM, 256, IT> ing;
-        int scan[256 / kWave];
+        FinishLds fin;
     } lds;
     int bid = blockIdx.x;
-    if (fin_prev) {
-        if (bid == 0) {
-            finish_deferred<256>(d, st_prev, lds.scan);
-            return;
-        }
-        --bid;
+    if (bid < fin_prev) {
+        finish_deferred<256>(d, st_prev, bid, fin_prev, lds.fin);
+        return;
     }
+    bid -= fin_prev;
     if (bid < nmb)
-        merge_window_body<DIM>(d, st, bid, lds.mw, !fin_prev);
+        merge_window_body<DIM>(d, st, bid, lds.mw, fin_prev == 0);
     else
         ingest_sort_body<DIM, 256, IT>(d, st_next, kModeSlide, nullptr, d.cpb == 1 ? 1 : 0,
                                        bid - nmb, lds.ing);
 }
 
-// slide st's epilogue in a launch of its own (a pipelined run stopped
-// before the slide that would carry it)
+// slide st's epilogue in a launch of its own, one part per workgroup (a
+// pipelined run stopped before the slide that would carry it)
 __global__ __launch_bounds__(256) void k_finish(Dev d, Step st)
 {
-    __shared__ int s_scan[256 / kWave];
-    finish_deferred<256>(d, st, s_scan);
+    __shared__ FinishLds lds;
+    finish_deferred<256>(d, st, blockIdx.x, gridDim.x, lds);
 }
 
 static_assert(kRunMergeNT == kDetectNT, "the run merge rides in detect workgroups");

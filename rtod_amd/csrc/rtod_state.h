@@ -41,6 +41,7 @@ struct Ctl {
     uint64_t t;            // slides completed since prime
     int32_t n_cand;        // candidates of the current slide
     uint32_t ticket;       // last-workgroup ticket of the detect kernel
+    uint32_t fin_ticket;   // parts done of a deferred slide epilogue
     unsigned long long pair_tests;
     int32_t err;
     int32_t n_units;       // cell units of the current slide
@@ -134,7 +135,9 @@ struct Dev {
     int32_t *cand;
     int2 *units;        // cell units: {start, len}
     int32_t *cand_cnt;  // per-unit minimum count (capped)
-    uint32_t *bitmap;
+    uint32_t *bitmap;   // [2][bm_words]: outlier bits of slide t at (t & 1)
+    int bm_words;       // ceil(W / 32), rounded up to 64
+    int fin_parts;      // parts of a deferred slide epilogue
     Ctl *ctl;
     const float *h_in;
     HostOut *h_out;      // kHostOuts host-out buffers, slide t uses t % kHostOuts
@@ -146,6 +149,13 @@ struct Dev {
 // kHostOuts buffers, so up to that many slides can be in flight while the
 // host reads the oldest one's list
 constexpr int kHostOuts = 4;
+
+// the outlier bitmap of slide t (two, by slide parity: the epilogue of slide
+// t may still read its bits while slide t + 1 is being detected)
+__device__ __forceinline__ uint32_t *bitmap_of(const Dev &d, uint64_t t)
+{
+    return d.bitmap + (size_t)(t & 1) * (size_t)d.bm_words;
+}
 __device__ __forceinline__ HostOut *host_out(const Dev &d, uint64_t t)
 {
     return reinterpret_cast<HostOut *>(reinterpret_cast<char *>(d.h_out) +
