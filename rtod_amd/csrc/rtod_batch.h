@@ -90,10 +90,10 @@ template <int DIM>
 __global__ __launch_bounds__(kBatchNT) void k_batch_merge_window(const BatchItem *items,
                                                                 const BatchStream *streams)
 {
-    __shared__ MergeWindowLds lds;
+    __shared__ MergeWindowLds<256> lds;
     const BatchItem it = items[blockIdx.x];
     const BatchStream &bs = streams[it.stream];
-    merge_window_body<DIM>(bs.d, batch_step(bs), it.bid, lds);
+    merge_window_body<DIM, 256>(bs.d, batch_step(bs), it.bid, lds);
 }
 
 template <int DIM, int MODE>

@@ -34,6 +34,7 @@ struct Lbvh {
     int32_t *flag;     // [U] bottom-up arrival counters
     int32_t *count;    // [U] neighbour counts per sorted leaf
     int ntiles_log2;   // merge passes for U up to W
+    const int32_t *n_cand; // the slide's candidate count (set per launch)
 };
 
 // Morton interleave of up to three 21-bit (3-D) or two 31-bit (2-D) fields
@@ -75,7 +76,7 @@ template <int DIM>
 __global__ __launch_bounds__(256) void k_lbvh_codes(Dev d, Step st, Lbvh L)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int U = d.ctl->n_cand;
+    const int U = *L.n_cand;
     if (i >= U)
         return;
     const uint64_t *Nk = d.akey[st.par ^ 1];
@@ -96,7 +97,7 @@ __global__ __launch_bounds__(NT) void k_lbvh_sort(Dev d, Lbvh L)
     __shared__ uint16_t s_val[CH];
     __shared__ int s_hist[16 * NW];
     __shared__ uint64_t s_red[2][NW];
-    const int U = d.ctl->n_cand;
+    const int U = *L.n_cand;
     const int j0 = blockIdx.x * CH;
     if (j0 >= U)
         return;
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(NT) void k_lbvh_sort(Dev d, Lbvh L)
 __global__ __launch_bounds__(256) void k_lbvh_merge(Dev d, Lbvh L, int Lr, int src)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int U = d.ctl->n_cand;
+    const int U = *L.n_cand;
     if (i >= U)
         return; // Lr >= U still copies (pr * Lr >= U): buffers stay in step
     const uint64_t *sk = L.code[src];
@@ -203,7 +204,7 @@ __global__ __launch_bounds__(256) void k_lbvh_merge(Dev d, Lbvh L, int Lr, int s
 __global__ __launch_bounds__(256) void k_lbvh_settle(Dev d, Lbvh L, int src)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int U = d.ctl->n_cand;
+    const int U = *L.n_cand;
     if (src == 1 || i >= U)
         return;
     L.code[1][i] = L.code[0][i];
@@ -227,7 +228,7 @@ __device__ __forceinline__ int lbvh_delta(const uint64_t *code, int U, int i, in
 __global__ __launch_bounds__(256) void k_lbvh_build(Dev d, Lbvh L)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int U = d.ctl->n_cand;
+    const int U = *L.n_cand;
     if (i == 0 && U > 0)
         L.parent[0] = -1;
     if (i >= U - 1)
@@ -270,7 +271,7 @@ template <int DIM>
 __global__ __launch_bounds__(256) void k_lbvh_boxes(Dev d, Step st, Lbvh L)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int U = d.ctl->n_cand;
+    const int U = *L.n_cand;
     if (i >= U)
         return;
     const float *Nx = d.axyz[st.par ^ 1];
@@ -326,7 +327,7 @@ __global__ __launch_bounds__(256) void k_lbvh_query(Dev d, Step st, Lbvh L)
 {
     __shared__ int s_stack[256 * kLbvhStack];
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int U = d.ctl->n_cand;
+    const int U = *L.n_cand;
     if (U == 0)
         return;
     const float *Nx = d.axyz[st.par ^ 1];
@@ -383,7 +384,7 @@ __global__ __launch_bounds__(256) void k_lbvh_query(Dev d, Step st, Lbvh L)
 __global__ __launch_bounds__(kDetectNT) void k_lbvh_select(Dev d, Step st, Lbvh L)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int U = d.ctl->n_cand;
+    const int U = *L.n_cand;
     if (i < U) {
         const int c = L.count[i];
         d.cand_cnt[L.item[1][i]] = min(c, d.K + 1);

@@ -55,12 +55,15 @@ enum KernelId {
     kKLbvhQuery,
     kKMergeWindowIngest, // pipelined: window merge + the next slide's tile sorts
     kKDetectRuns,        // pipelined: detect + the next slide's run merge
+    kKClassifyStage,     // two launches per slide: classify + epilogue + tile sorts
+    kKDetectStage,       // two launches per slide: detect + window merge + run merge
     kKCount
 };
 const char *kKernelNames[kKCount] = {"ingest_sort",         "merge_pass",  "merge_window",
                                      "classify",            "detect",      "lbvh_build",
                                      "lbvh_query",          "merge_window_ingest",
-                                     "detect_runs"};
+                                     "detect_runs",         "classify_stage",
+                                     "detect_stage"};
 
 // LDS-sort tile shapes (threads x items per thread).  A batch of up to
 // kRunMergeCap points is sorted as tiles of 1024 points on as many
@@ -354,44 +357,111 @@ void launch_finish(rtod_ctx *c, hipStream_t s, const Step &st)
     hipLaunchKernelGGL(k_finish, dim3(c->d.fin_parts), dim3(256), 0, s, c->d, st);
 }
 
+// Two launches per slide (cell-stencil index, the ingest riding): slide
+// st's classify with the epilogue of prev and the tile sorts of ts ...
+template <int DIM, int MODE>
+void launch_classify_stage_dim(rtod_ctx *c, hipStream_t s, const Step &st, const Step *prev,
+                               const Step *ts)
+{
+    const int nfin = prev ? c->d.fin_parts : 0;
+    const int ncls = (c->W + kClassifyNT - 1) / kClassifyNT;
+    const dim3 g(nfin + ncls + (ts ? c->d.cpb : 0));
+    const Step pv = prev ? *prev : Step{}, tv = ts ? *ts : Step{};
+    if (c->sort_shape == 0)
+        hipLaunchKernelGGL((k_classify_stage<DIM, MODE, 2>), g, dim3(kClassifyNT), 0, s, c->d, st,
+                           pv, tv, nfin, ncls);
+    else
+        hipLaunchKernelGGL((k_classify_stage<DIM, MODE, 4>), g, dim3(kClassifyNT), 0, s, c->d, st,
+                           pv, tv, nfin, ncls);
+}
+
+void launch_classify_stage(rtod_ctx *c, hipStream_t s, const Step &st, const Step *prev,
+                           const Step *ts)
+{
+    if (c->dim == 1)
+        c->learned ? launch_classify_stage_dim<1, kClsLearned>(c, s, st, prev, ts)
+                   : launch_classify_stage_dim<1, kClsUnits>(c, s, st, prev, ts);
+    else if (c->dim == 2)
+        c->learned ? launch_classify_stage_dim<2, kClsLearned>(c, s, st, prev, ts)
+                   : launch_classify_stage_dim<2, kClsUnits>(c, s, st, prev, ts);
+    else
+        c->learned ? launch_classify_stage_dim<3, kClsLearned>(c, s, st, prev, ts)
+                   : launch_classify_stage_dim<3, kClsUnits>(c, s, st, prev, ts);
+}
+
+// ... and slide st's detect with the window merge of merge and the run
+// merge of runs
+template <int DIM>
+void launch_detect_stage_dim(rtod_ctx *c, hipStream_t s, const Step &st, const Step *merge,
+                             const Step *runs)
+{
+    const int nmb = merge ? (c->W + kMergeT - 1) / kMergeT : 0;
+    const int nrm = runs ? run_merge_wgs(c->S) : 0;
+    const int ndet = c->detect_blocks;
+    const dim3 g(nmb + nrm + ndet);
+    const size_t lds = detect_stage_lds(merge != nullptr, runs != nullptr, c->S);
+    const Step mv = merge ? *merge : Step{}, rv = runs ? *runs : Step{};
+    if (c->learned)
+        hipLaunchKernelGGL((k_detect_stage<DIM, true>), g, dim3(kDetectNT), lds, s, c->d, st, mv,
+                           rv, nmb, nrm, ndet);
+    else
+        hipLaunchKernelGGL((k_detect_stage<DIM, false>), g, dim3(kDetectNT), lds, s, c->d, st, mv,
+                           rv, nmb, nrm, ndet);
+}
+
+void launch_detect_stage(rtod_ctx *c, hipStream_t s, const Step &st, const Step *merge,
+                         const Step *runs)
+{
+    if (c->dim == 1)
+        launch_detect_stage_dim<1>(c, s, st, merge, runs);
+    else if (c->dim == 2)
+        launch_detect_stage_dim<2>(c, s, st, merge, runs);
+    else
+        launch_detect_stage_dim<3>(c, s, st, merge, runs);
+}
+
 // LBVH index: codes, tile sort, merges up to U = W, hierarchy, boxes
 void launch_lbvh_build(rtod_ctx *c, hipStream_t s, const Step &st)
 {
+    Lbvh L = c->L;
+    L.n_cand = &c->d.ctl->cnt[st.t & 1].n_cand; // device address (not read here)
     const dim3 gw((c->W + 255) / 256), b(256);
     if (c->dim == 1)
-        hipLaunchKernelGGL(k_lbvh_codes<1>, gw, b, 0, s, c->d, st, c->L);
+        hipLaunchKernelGGL(k_lbvh_codes<1>, gw, b, 0, s, c->d, st, L);
     else if (c->dim == 2)
-        hipLaunchKernelGGL(k_lbvh_codes<2>, gw, b, 0, s, c->d, st, c->L);
+        hipLaunchKernelGGL(k_lbvh_codes<2>, gw, b, 0, s, c->d, st, L);
     else
-        hipLaunchKernelGGL(k_lbvh_codes<3>, gw, b, 0, s, c->d, st, c->L);
+        hipLaunchKernelGGL(k_lbvh_codes<3>, gw, b, 0, s, c->d, st, L);
     constexpr int CH = 1024 * 4;
     hipLaunchKernelGGL((k_lbvh_sort<1024, 4>), dim3((c->W + CH - 1) / CH), dim3(1024), 0, s,
-                       c->d, c->L);
+                       c->d, L);
     int src = 1;
     for (int p = 0, Lr = CH; p < c->lbvh_passes; ++p, Lr *= 2) {
-        hipLaunchKernelGGL(k_lbvh_merge, gw, b, 0, s, c->d, c->L, Lr, src);
+        hipLaunchKernelGGL(k_lbvh_merge, gw, b, 0, s, c->d, L, Lr, src);
         src ^= 1;
     }
-    hipLaunchKernelGGL(k_lbvh_settle, gw, b, 0, s, c->d, c->L, src);
-    hipLaunchKernelGGL(k_lbvh_build, gw, b, 0, s, c->d, c->L);
+    hipLaunchKernelGGL(k_lbvh_settle, gw, b, 0, s, c->d, L, src);
+    hipLaunchKernelGGL(k_lbvh_build, gw, b, 0, s, c->d, L);
     if (c->dim == 1)
-        hipLaunchKernelGGL(k_lbvh_boxes<1>, gw, b, 0, s, c->d, st, c->L);
+        hipLaunchKernelGGL(k_lbvh_boxes<1>, gw, b, 0, s, c->d, st, L);
     else if (c->dim == 2)
-        hipLaunchKernelGGL(k_lbvh_boxes<2>, gw, b, 0, s, c->d, st, c->L);
+        hipLaunchKernelGGL(k_lbvh_boxes<2>, gw, b, 0, s, c->d, st, L);
     else
-        hipLaunchKernelGGL(k_lbvh_boxes<3>, gw, b, 0, s, c->d, st, c->L);
+        hipLaunchKernelGGL(k_lbvh_boxes<3>, gw, b, 0, s, c->d, st, L);
 }
 
 void launch_lbvh_detect(rtod_ctx *c, hipStream_t s, const Step &st)
 {
+    Lbvh L = c->L;
+    L.n_cand = &c->d.ctl->cnt[st.t & 1].n_cand;
     const dim3 gw((c->W + 255) / 256), b(256);
     if (c->dim == 1)
-        hipLaunchKernelGGL(k_lbvh_query<1>, gw, b, 0, s, c->d, st, c->L);
+        hipLaunchKernelGGL(k_lbvh_query<1>, gw, b, 0, s, c->d, st, L);
     else if (c->dim == 2)
-        hipLaunchKernelGGL(k_lbvh_query<2>, gw, b, 0, s, c->d, st, c->L);
+        hipLaunchKernelGGL(k_lbvh_query<2>, gw, b, 0, s, c->d, st, L);
     else
-        hipLaunchKernelGGL(k_lbvh_query<3>, gw, b, 0, s, c->d, st, c->L);
-    hipLaunchKernelGGL(k_lbvh_select, gw, dim3(kDetectNT), 0, s, c->d, st, c->L);
+        hipLaunchKernelGGL(k_lbvh_query<3>, gw, b, 0, s, c->d, st, L);
+    hipLaunchKernelGGL(k_lbvh_select, gw, dim3(kDetectNT), 0, s, c->d, st, L);
     launch_finish(c, s, st);
 }
 
@@ -854,6 +924,20 @@ void accumulate_core_times(rtod_ctx *c, bool fused)
     }
 }
 
+// event pair [2, 3]: the classify stage; [3, 4]: the detect stage
+void accumulate_stage_times(rtod_ctx *c)
+{
+    float ms[2];
+    for (int i = 0; i < 2; ++i) {
+        ms[i] = 0.f;
+        (void)hipEventElapsedTime(&ms[i], c->ev[2 + i], c->ev[3 + i]);
+    }
+    c->ksum[kKClassifyStage] += ms[0];
+    c->kcnt[kKClassifyStage]++;
+    c->ksum[kKDetectStage] += ms[1];
+    c->kcnt[kKDetectStage]++;
+}
+
 void accumulate_kernel_times(rtod_ctx *c, double *phase)
 {
     float ms[6];
@@ -1042,9 +1126,104 @@ rtod_status wait_slide(rtod_ctx *c, int64_t t)
 // rtod_slide_device calls.  On an error the stream is drained first.
 // RTOD_FLAG_PHASE_TIMES: HIP events around the core kernels, and each slide
 // is waited for before the next is enqueued (a measuring mode).
+// Pipelined slides in two launches per slide (cell-stencil index, the
+// ingest riding; see k_classify_stage): after a prologue (slide t0's ingest
+// and window merge, slide t0 + 1's ingest), slide t0 + i is the classify
+// stage A(i) and the detect stage B(i), and the last slide's epilogue
+// follows in a launch of its own.  Up to kHostOuts slides are in flight.
+rtod_status slides_two_stage(rtod_ctx *c, const float *dev, int32_t n, int32_t *offs,
+                             int32_t *slots)
+{
+    const int64_t t0 = c->host_t;
+    const size_t per = (size_t)c->S * c->dim;
+    const bool timed = (c->cfg.flags & RTOD_FLAG_PHASE_TIMES) != 0;
+    const bool runs = c->d.cpb > 1;
+    hipStream_t s = c->stream;
+    auto step = [&](int32_t i) {
+        return make_step_at(c, (uint64_t)(t0 + i), dev + (size_t)i * per);
+    };
+    int64_t used = 0;
+    int32_t done = 0;
+    offs[0] = 0;
+    auto collect_next = [&]() -> rtod_status {
+        RTOD_TRY(wait_slide(c, c->host_t));
+        int32_t k = 0;
+        RTOD_TRY(collect(c, slots + used, &k));
+        used += k;
+        offs[++done] = (int32_t)used;
+        return maybe_train(c);
+    };
+    const Step s0 = step(0);
+    launch_ingest_stage(c, s, s0, nullptr);
+    if (n > 1) {
+        const Step s1 = step(1);
+        launch_merge_window(c, s, s0, &s1);
+        if (runs)
+            launch_batch_merges(c, s, s1, kModeSlide, c->S);
+    } else {
+        launch_merge_window(c, s, s0);
+    }
+    rtod_status st = RTOD_OK;
+    int32_t launched = 0; // slides whose two stages are queued
+    for (int32_t i = 0; i < n && st == RTOD_OK; ++i) {
+        if (i >= kHostOuts && (st = collect_next()) != RTOD_OK)
+            break;
+        const int64_t t = t0 + i;
+        if ((st = guard_trainer(c, t, s)) != RTOD_OK)
+            break;
+        const Step cur = step(i);
+        const Step prev = i > 0 ? step(i - 1) : Step{};
+        const Step nx1 = i + 1 < n ? step(i + 1) : Step{};
+        const Step nx2 = i + 2 < n ? step(i + 2) : Step{};
+        if (timed)
+            (void)hipEventRecord(c->ev[2], s);
+        launch_classify_stage(c, s, cur, i > 0 ? &prev : nullptr, i + 2 < n ? &nx2 : nullptr);
+        if (timed)
+            (void)hipEventRecord(c->ev[3], s);
+        launch_detect_stage(c, s, cur, i + 1 < n ? &nx1 : nullptr,
+                            (i + 2 < n && runs) ? &nx2 : nullptr);
+        if (timed)
+            (void)hipEventRecord(c->ev[4], s);
+        if (i + 1 == n)
+            launch_finish(c, s, cur);
+        launched = i + 1;
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            set_error(std::string("launch: ") + hipGetErrorString(e));
+            st = RTOD_ERR_HIP;
+        } else if (timed) {
+            const hipError_t q = hipStreamSynchronize(s);
+            if (q != hipSuccess) {
+                set_error(std::string("HIP error: ") + hipGetErrorString(q));
+                st = RTOD_ERR_HIP;
+            } else {
+                accumulate_stage_times(c);
+            }
+        }
+    }
+    if (st != RTOD_OK) {
+        // the window may have been merged for a slide that was not detected:
+        // the detector has to be primed again
+        if (launched > 0 && launched < n)
+            launch_finish(c, s, step(launched - 1));
+        (void)hipStreamSynchronize(s);
+        c->primed = false;
+        return st;
+    }
+    while (st == RTOD_OK && done < n)
+        st = collect_next();
+    if (st != RTOD_OK) {
+        (void)hipStreamSynchronize(s);
+        c->primed = false;
+    }
+    return st;
+}
+
 rtod_status slides_device_impl(rtod_ctx *c, const float *dev, int32_t n, int32_t *offs,
                                int32_t *slots)
 {
+    if (!c->lbvh && c->ride)
+        return slides_two_stage(c, dev, n, offs, slots);
     const int64_t t0 = c->host_t;
     const size_t per = (size_t)c->S * c->dim;
     const bool timed = (c->cfg.flags & RTOD_FLAG_PHASE_TIMES) != 0;

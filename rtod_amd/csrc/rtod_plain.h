@@ -589,11 +589,16 @@ __device__ __forceinline__ int wave_count_lds(const uint64_t *tab, int n, uint64
 constexpr int kMergeT = 512; // window elements per merge workgroup (half the chunk
                               // of 1024 measured 8% faster on the 1-D headline)
 
-struct MergeWindowLds {
-    static constexpr int NT = 256, IT = 2, T = NT * IT, CAP = 2048, MAXBS = 512;
+// A's sample table as staged for the ranks of B: at most kMergeSamp
+// entries (every other sample when the table is longer), so the workgroup's
+// LDS stays small enough for three 512-thread workgroups per CU
+constexpr int kMergeSamp = kMaxSamples / 2;
+
+template <int NT> struct MergeWindowLds {
+    static constexpr int IT = kMergeT / NT, T = kMergeT, CAP = 2048, MAXBS = 512;
     uint64_t k[T];
     uint64_t bs[MAXBS], es[MAXBS];
-    alignas(16) uint64_t as[kMaxSamples];
+    alignas(16) uint64_t as[kMergeSamp];
     alignas(16) uint64_t br[CAP];
     int rc[IT][NT / kWave];
     int res[8];
@@ -601,14 +606,14 @@ struct MergeWindowLds {
     int pos0, slot0;
 };
 
-// reset: workgroup 0 clears the slide's classify counters (left to the
-// previous slide's epilogue when that runs in the same launch)
-template <int DIM>
+// NT threads, kMergeT / NT chunk elements each; workgroup 0 resets the
+// slide's classify counters
+template <int DIM, int NT>
 __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, int bid,
-                                                  MergeWindowLds &lds, bool reset = true)
+                                                  MergeWindowLds<NT> &lds)
 {
-    constexpr int NT = 256, IT = 2, T = NT * IT, CAP = 2048;
-    static_assert(T == kMergeT, "merge tile");
+    constexpr int IT = kMergeT / NT, T = kMergeT, CAP = 2048;
+    static_assert(IT * NT == T, "merge tile");
     uint64_t *s_k = lds.k;
     uint64_t *s_bs = lds.bs, *s_es = lds.es;
     uint64_t *s_as = lds.as;
@@ -651,11 +656,11 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
     const int bper = (S + nmb - 1) / nmb;
     const int j0 = min(S, bid * bper), j1 = min(S, j0 + bper);
 
-    if (reset && bid == 0 && tid == 0) {
-        d.ctl->n_cand = 0;
-        d.ctl->n_units = 0;
-        if (d.lr.on)
-            d.lr.lc->n_units_i = 0;
+    if (bid == 0 && tid == 0) {
+        SlideCnt *cn = cnt_of(d, st.t);
+        cn->n_cand = 0;
+        cn->n_units = 0;
+        cn->n_units_i = 0;
     }
     // this slide's outlier bitmap, last read by the epilogue of the slide
     // two earlier: cleared here, T / 32 words per workgroup
@@ -706,7 +711,26 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
     }
     copy_to_lds<2>(s_bs, Bsamp, nbs);
     copy_to_lds<2>(s_es, Esamp, nbs);
-    copy_to_lds<kMaxSamples / NT>(s_as, d.samples[par], d.nsamp);
+    // A's sample table, every (1 << sst)-th entry
+    const int sst = d.nsamp > kMergeSamp ? 1 : 0;
+    const int nas = (d.nsamp + (1 << sst) - 1) >> sst;
+    {
+        constexpr int U = kMergeSamp / NT;
+        const uint64_t *Asamp = d.samples[par];
+        uint64_t v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = tid + u * NT;
+            if (i < nas)
+                v[u] = Asamp[i << sst];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = tid + u * NT;
+            if (i < nas)
+                s_as[i] = v[u];
+        }
+    }
     if (tid == 0) {
         s_kn = last ? 0 : Ak[i1];
         s_pos0 = Ap[i0];
@@ -832,10 +856,10 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
             for (int a = 0; a < DIM; ++a)
                 px[a] = Bx[(size_t)a * bxs + j];
         }
-        const int ca = upper_bound_u64(s_as, 0, d.nsamp, key); // samples <= key
+        const int ca = upper_bound_u64(s_as, 0, nas, key); // staged samples <= key
         int ubA = 0;
         if (ca > 0) {
-            const int L = ((ca - 1) << sl2) + 1, H = min(ca << sl2, W);
+            const int L = ((ca - 1) << (sl2 + sst)) + 1, H = min(ca << (sl2 + sst), W);
             ubA = upper_bound_u64(Ak, L, H, key);
         }
         const int ce = upper_bound_u64(s_es, 0, nbs, key);
@@ -865,8 +889,8 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
 template <int DIM>
 __global__ __launch_bounds__(256) void k_merge_window(Dev d, Step st)
 {
-    __shared__ MergeWindowLds lds;
-    merge_window_body<DIM>(d, st, blockIdx.x, lds);
+    __shared__ MergeWindowLds<256> lds;
+    merge_window_body<DIM, 256>(d, st, blockIdx.x, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1009,7 +1033,7 @@ __device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int 
         if (cnt) {
             int base = 0;
             if (lane == 0)
-                base = atomicAdd(&d.ctl->n_cand, cnt);
+                base = atomicAdd(&cnt_of(d, st.t)->n_cand, cnt);
             base = __shfl(base, 0, kWave);
             if (cand)
                 d.cand[base + __popcll(mask & lt)] = i;
@@ -1078,10 +1102,10 @@ __device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int 
                 for (int q = 0; q < 3; ++q)
                     tot[q] += lds.wc[w][q];
             if (tot[0])
-                atomicAdd(&d.ctl->n_cand, tot[0]);
-            lds.base[1] = tot[1] ? atomicAdd(&d.ctl->n_units, tot[1]) : 0;
+                atomicAdd(&cnt_of(d, st.t)->n_cand, tot[0]);
+            lds.base[1] = tot[1] ? atomicAdd(&cnt_of(d, st.t)->n_units, tot[1]) : 0;
             if constexpr (LEARN)
-                lds.base[2] = tot[2] ? atomicAdd(&d.lr.lc->n_units_i, tot[2]) : 0;
+                lds.base[2] = tot[2] ? atomicAdd(&cnt_of(d, st.t)->n_units_i, tot[2]) : 0;
         }
         __syncthreads();
         int bo = lds.base[1], bi = LEARN ? lds.base[2] : 0;
@@ -1744,19 +1768,16 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int f
         if (sm.n_units > 0)
             lc->use_pred = ((float)miss <= d.lr.miss_thr * (float)sm.n_units) ? 1 : 0;
         lc->misses = 0;
-        lc->n_units_i = 0;
     }
     ho->n_units = sm.n_units;
     ho->n_out = total;
-    ho->n_cand = d.ctl->n_cand;
+    ho->n_cand = cnt_of(d, st.t)->n_cand;
     ho->pair_tests = __hip_atomic_load(&d.ctl->pair_tests, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
     ho->err = atomicExch(&d.ctl->err, 0);
     d.ctl->pair_tests = 0;
     d.ctl->ticket = 0;
     d.ctl->fin_ticket = 0;
-    d.ctl->n_cand = 0;
-    d.ctl->n_units = 0;
     d.ctl->t = st.t + 1;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
     __hip_atomic_store(&ho->slide, (int64_t)(st.t + 1), __ATOMIC_RELAXED,
@@ -1772,8 +1793,9 @@ __device__ __forceinline__ void finish_deferred(const Dev &d, const Step &st, in
 {
     SlideSummary sm;
     sm.learn = d.lr.on != 0;
-    sm.n_o = d.ctl->n_units;
-    sm.n_units = sm.learn ? sm.n_o + d.lr.lc->n_units_i : sm.n_o;
+    const SlideCnt *cn = cnt_of(d, st.t);
+    sm.n_o = cn->n_units;
+    sm.n_units = sm.learn ? sm.n_o + cn->n_units_i : sm.n_o;
     sm.use_pred = sm.learn && d.lr.lc->use_pred != 0;
     finish_slide<NT>(d, st, f, F, sm, fl);
 }
@@ -1826,9 +1848,9 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
     const int32_t *Ns = d.aslot[par];
     const float *Nx = d.axyz[par];
     const int W = d.W, K = d.K;
-    const int n_o = d.ctl->n_units;
+    const int n_o = cnt_of(d, st.t)->n_units;
     uint32_t *bm = bitmap_of(d, st.t);
-    const int n_units = LEARN ? n_o + d.lr.lc->n_units_i : n_o;
+    const int n_units = LEARN ? n_o + cnt_of(d, st.t)->n_units_i : n_o;
     const bool use_pred = LEARN && d.lr.lc->use_pred != 0;
     const int busy = min(nblk, max(1, (n_units + NW - 1) / NW));
     if (bid >= busy)
@@ -1953,7 +1975,7 @@ __global__ __launch_bounds__(256) void k_merge_window_ingest(Dev d, Step st, Ste
                                                             Step st_prev, int nmb, int fin_prev)
 {
     __shared__ union {
-        MergeWindowLds mw;
+        MergeWindowLds<256> mw;
         IngestLds<DIM, 256, IT> ing;
         FinishLds fin;
     } lds;
@@ -1964,7 +1986,7 @@ __global__ __launch_bounds__(256) void k_merge_window_ingest(Dev d, Step st, Ste
     }
     bid -= fin_prev;
     if (bid < nmb)
-        merge_window_body<DIM>(d, st, bid, lds.mw, fin_prev == 0);
+        merge_window_body<DIM, 256>(d, st, bid, lds.mw);
     else
         ingest_sort_body<DIM, 256, IT>(d, st_next, kModeSlide, nullptr, d.cpb == 1 ? 1 : 0,
                                        bid - nmb, lds.ing);
@@ -2001,6 +2023,85 @@ __global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_detect_runs
     else
         merge_runs_body<DIM>(d, st_next, kModeSlide, blockIdx.x - ndet,
                              reinterpret_cast<uint64_t *>(dyn_lds));
+}
+
+// ---------------------------------------------------------------------------
+// Two launches per slide (pipelined slides on one stream, cell-stencil
+// index, the ingest riding): the classify stage of slide t carries the
+// epilogue of slide t - 1 and the tile sorts of slide t + 2; the detect
+// stage of slide t carries the window merge of slide t + 1 and the run
+// merge of slide t + 2.  The parts of a launch touch disjoint memory:
+//   classify(t): reads window t; writes the candidate and unit lists and
+//     counters cnt[t & 1];
+//   epilogue(t - 1): reads bitmap (t - 1) & 1 and cnt[(t - 1) & 1]; writes
+//     host-out buffer t - 1;
+//   tile sorts(t + 2): write the tile scratch (one tile: batch-ring entry
+//     enew(t + 2) = eold(t), whose last reader, merge(t), ran a launch
+//     earlier);
+//   detect(t): reads window t, the lists and cnt[t & 1]; sets bitmap t & 1;
+//   merge(t + 1): reads window t and ring entries enew(t + 1), eold(t + 1);
+//     writes window t + 1 (the other buffers); resets cnt[(t + 1) & 1] and
+//     clears bitmap (t + 1) & 1 (both last read by epilogue(t - 1), a
+//     launch earlier);
+//   run merge(t + 2): reads the tile scratch; writes ring entry enew(t + 2),
+//     distinct from enew(t + 1) and eold(t + 1) with m + 2 entries.
+// ---------------------------------------------------------------------------
+template <int DIM, int MODE, int IT>
+__global__ __launch_bounds__(kClassifyNT) void k_classify_stage(Dev d, Step st, Step st_prev,
+                                                               Step st_ts, int nfin, int ncls)
+{
+    static_assert(kClassifyNT == 256, "the tile sorts and epilogue parts take 256 threads");
+    __shared__ union {
+        ClassifyLds<MODE == kClsLearned> cls;
+        FinishLds fin;
+        IngestLds<DIM, 256, IT> ing;
+    } lds;
+    int bid = blockIdx.x;
+    if (bid < nfin) {
+        finish_deferred<256>(d, st_prev, bid, nfin, lds.fin);
+        return;
+    }
+    bid -= nfin;
+    if (bid < ncls) {
+        classify_body<DIM, MODE>(d, st, bid, lds.cls);
+        return;
+    }
+    ingest_sort_body<DIM, 256, IT>(d, st_ts, kModeSlide, nullptr, d.cpb == 1 ? 1 : 0,
+                                   bid - ncls, lds.ing);
+}
+
+// LDS of k_detect_stage, set at launch: the largest of its parts'
+__host__ __device__ constexpr size_t detect_stage_lds(bool merge, bool runs, int S)
+{
+    size_t b = sizeof(DetectLds);
+    if (merge && sizeof(MergeWindowLds<kDetectNT>) > b)
+        b = sizeof(MergeWindowLds<kDetectNT>);
+    if (runs && (size_t)S * sizeof(uint64_t) > b)
+        b = (size_t)S * sizeof(uint64_t);
+    return b;
+}
+
+// workgroups: nmb window-merge (slide st_merge), nrm run-merge (slide
+// st_runs), then the ndet of the detect (slide st), whose busy ones come
+// first in the list
+template <int DIM, bool LEARN>
+__global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_detect_stage(
+    Dev d, Step st, Step st_merge, Step st_runs, int nmb, int nrm, int ndet)
+{
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    int bid = blockIdx.x;
+    if (bid < nmb) {
+        merge_window_body<DIM, kDetectNT>(d, st_merge, bid,
+                                          *reinterpret_cast<MergeWindowLds<kDetectNT> *>(dyn_lds));
+        return;
+    }
+    bid -= nmb;
+    if (bid < nrm) {
+        merge_runs_body<DIM>(d, st_runs, kModeSlide, bid, reinterpret_cast<uint64_t *>(dyn_lds));
+        return;
+    }
+    bid -= nrm;
+    detect_body<DIM, LEARN>(d, st, bid, ndet, false, *reinterpret_cast<DetectLds *>(dyn_lds));
 }
 
 } // namespace rtod

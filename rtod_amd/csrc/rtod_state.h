@@ -37,14 +37,24 @@
 
 namespace rtod {
 
+// A slide's classify counters.  There are two sets, by slide parity: the
+// classify of slide t may run beside the epilogue of slide t - 1, which
+// reads that slide's.
+struct SlideCnt {
+    int32_t n_cand;        // candidates
+    int32_t n_units;       // cell units (learned: the probable-outlier bucket)
+    int32_t n_units_i;     // learned: the probable-inlier bucket
+    int32_t pad;
+};
+
 struct Ctl {
     uint64_t t;            // slides completed since prime
-    int32_t n_cand;        // candidates of the current slide
     uint32_t ticket;       // last-workgroup ticket of the detect kernel
     uint32_t fin_ticket;   // parts done of a deferred slide epilogue
     unsigned long long pair_tests;
     int32_t err;
-    int32_t n_units;       // cell units of the current slide
+    int32_t pad;
+    SlideCnt cnt[2];       // slide t's at cnt[t & 1]; reset by its window merge
 };
 
 struct HostOut {
@@ -63,10 +73,8 @@ struct HostOut {
 
 // Device-side state of the learned scheduler: per stream ...
 struct LearnCtl {
-    int32_t n_units_i;     // predicted-inlier unit bucket size (current slide)
     int32_t use_pred;      // schedule the current slide by prediction
     int32_t misses;        // mispredictions (current slide)
-    int32_t pad;
 };
 
 // ... and of the predictor, which several streams may share (a batch).
@@ -149,6 +157,9 @@ struct Dev {
 // kHostOuts buffers, so up to that many slides can be in flight while the
 // host reads the oldest one's list
 constexpr int kHostOuts = 4;
+
+// the classify counters of slide t
+__device__ __forceinline__ SlideCnt *cnt_of(const Dev &d, uint64_t t) { return &d.ctl->cnt[t & 1]; }
 
 // the outlier bitmap of slide t (two, by slide parity: the epilogue of slide
 // t may still read its bits while slide t + 1 is being detected)
