@@ -444,7 +444,12 @@ def main() -> int:
            "merge_window_ingest": 8 * dim * (W + S),    # + the next slide's tile sorts
            "classify": 4 * dim * W,                     # one pass over the window
            "detect": 4 * dim * n_cand + 4 * n_out,      # candidates in, list out
-           "detect_runs": 4 * dim * n_cand + 4 * n_out + 8 * dim * S}  # + next run merge
+           "detect_runs": 4 * dim * n_cand + 4 * n_out + 8 * dim * S,  # + next run merge
+           # two launches per slide: classify + the previous slide's list +
+           # tile sorts two slides ahead; detect + the next slide's window
+           # merge + the run merge two slides ahead
+           "classify_stage": 4 * dim * W + 8 * dim * S + 4 * n_out,
+           "detect_stage": 4 * dim * n_cand + 8 * dim * W + 8 * dim * S}
     b_dom = alg.get(dom, 4 * dim * (W + S))
     achieved = b_dom / (dom_ms * 1e-3) / 1e9
     b_slide = 4 * dim * (W + S) + 4 * n_out

@@ -358,66 +358,71 @@ void launch_finish(rtod_ctx *c, hipStream_t s, const Step &st)
 }
 
 // Two launches per slide (cell-stencil index, the ingest riding): slide
-// st's classify with the epilogue of prev and the tile sorts of ts ...
+// st's classify with the epilogue of prev ...
 template <int DIM, int MODE>
-void launch_classify_stage_dim(rtod_ctx *c, hipStream_t s, const Step &st, const Step *prev,
-                               const Step *ts)
+void launch_classify_stage_dim(rtod_ctx *c, hipStream_t s, const Step &st, const Step *prev)
 {
     const int nfin = prev ? c->d.fin_parts : 0;
     const int ncls = (c->W + kClassifyNT - 1) / kClassifyNT;
-    const dim3 g(nfin + ncls + (ts ? c->d.cpb : 0));
-    const Step pv = prev ? *prev : Step{}, tv = ts ? *ts : Step{};
-    if (c->sort_shape == 0)
-        hipLaunchKernelGGL((k_classify_stage<DIM, MODE, 2>), g, dim3(kClassifyNT), 0, s, c->d, st,
-                           pv, tv, nfin, ncls);
-    else
-        hipLaunchKernelGGL((k_classify_stage<DIM, MODE, 4>), g, dim3(kClassifyNT), 0, s, c->d, st,
-                           pv, tv, nfin, ncls);
+    const Step pv = prev ? *prev : Step{};
+    hipLaunchKernelGGL((k_classify_stage<DIM, MODE>), dim3(nfin + ncls), dim3(kClassifyNT), 0, s,
+                       c->d, st, pv, nfin, ncls);
 }
 
-void launch_classify_stage(rtod_ctx *c, hipStream_t s, const Step &st, const Step *prev,
-                           const Step *ts)
+void launch_classify_stage(rtod_ctx *c, hipStream_t s, const Step &st, const Step *prev)
 {
     if (c->dim == 1)
-        c->learned ? launch_classify_stage_dim<1, kClsLearned>(c, s, st, prev, ts)
-                   : launch_classify_stage_dim<1, kClsUnits>(c, s, st, prev, ts);
+        c->learned ? launch_classify_stage_dim<1, kClsLearned>(c, s, st, prev)
+                   : launch_classify_stage_dim<1, kClsUnits>(c, s, st, prev);
     else if (c->dim == 2)
-        c->learned ? launch_classify_stage_dim<2, kClsLearned>(c, s, st, prev, ts)
-                   : launch_classify_stage_dim<2, kClsUnits>(c, s, st, prev, ts);
+        c->learned ? launch_classify_stage_dim<2, kClsLearned>(c, s, st, prev)
+                   : launch_classify_stage_dim<2, kClsUnits>(c, s, st, prev);
     else
-        c->learned ? launch_classify_stage_dim<3, kClsLearned>(c, s, st, prev, ts)
-                   : launch_classify_stage_dim<3, kClsUnits>(c, s, st, prev, ts);
+        c->learned ? launch_classify_stage_dim<3, kClsLearned>(c, s, st, prev)
+                   : launch_classify_stage_dim<3, kClsUnits>(c, s, st, prev);
 }
 
-// ... and slide st's detect with the window merge of merge and the run
-// merge of runs
-template <int DIM>
-void launch_detect_stage_dim(rtod_ctx *c, hipStream_t s, const Step &st, const Step *merge,
-                             const Step *runs)
+// ... and slide st's detect with the window merge of merge, the run merge
+// of runs and the tile sorts of ts (tiles of c->CH points: kDetectNT
+// threads with CH / kDetectNT items each)
+template <int DIM, bool LEARN, int TSIT>
+void launch_detect_stage_t(rtod_ctx *c, hipStream_t s, const Step &st, const Step *merge,
+                           const Step *runs, const Step *ts)
 {
     const int nmb = merge ? (c->W + kMergeT - 1) / kMergeT : 0;
     const int nrm = runs ? run_merge_wgs(c->S) : 0;
+    const int nts = ts ? c->d.cpb : 0;
     const int ndet = c->detect_blocks;
-    const dim3 g(nmb + nrm + ndet);
-    const size_t lds = detect_stage_lds(merge != nullptr, runs != nullptr, c->S);
-    const Step mv = merge ? *merge : Step{}, rv = runs ? *runs : Step{};
+    const dim3 g(nmb + nrm + nts + ndet);
+    const size_t lds = detect_stage_lds<DIM, TSIT>(merge != nullptr, runs != nullptr,
+                                                   ts != nullptr, c->S);
+    const Step mv = merge ? *merge : Step{}, rv = runs ? *runs : Step{}, tv = ts ? *ts : Step{};
+    hipLaunchKernelGGL((k_detect_stage<DIM, LEARN, TSIT>), g, dim3(kDetectNT), lds, s, c->d, st,
+                       mv, rv, tv, nmb, nrm, nts, ndet);
+}
+
+template <int DIM>
+void launch_detect_stage_dim(rtod_ctx *c, hipStream_t s, const Step &st, const Step *merge,
+                             const Step *runs, const Step *ts)
+{
+    const bool t1 = c->CH == kDetectNT; // else CH == 2 * kDetectNT (ride_along)
     if (c->learned)
-        hipLaunchKernelGGL((k_detect_stage<DIM, true>), g, dim3(kDetectNT), lds, s, c->d, st, mv,
-                           rv, nmb, nrm, ndet);
+        t1 ? launch_detect_stage_t<DIM, true, 1>(c, s, st, merge, runs, ts)
+           : launch_detect_stage_t<DIM, true, 2>(c, s, st, merge, runs, ts);
     else
-        hipLaunchKernelGGL((k_detect_stage<DIM, false>), g, dim3(kDetectNT), lds, s, c->d, st, mv,
-                           rv, nmb, nrm, ndet);
+        t1 ? launch_detect_stage_t<DIM, false, 1>(c, s, st, merge, runs, ts)
+           : launch_detect_stage_t<DIM, false, 2>(c, s, st, merge, runs, ts);
 }
 
 void launch_detect_stage(rtod_ctx *c, hipStream_t s, const Step &st, const Step *merge,
-                         const Step *runs)
+                         const Step *runs, const Step *ts)
 {
     if (c->dim == 1)
-        launch_detect_stage_dim<1>(c, s, st, merge, runs);
+        launch_detect_stage_dim<1>(c, s, st, merge, runs, ts);
     else if (c->dim == 2)
-        launch_detect_stage_dim<2>(c, s, st, merge, runs);
+        launch_detect_stage_dim<2>(c, s, st, merge, runs, ts);
     else
-        launch_detect_stage_dim<3>(c, s, st, merge, runs);
+        launch_detect_stage_dim<3>(c, s, st, merge, runs, ts);
 }
 
 // LBVH index: codes, tile sort, merges up to U = W, hierarchy, boxes
@@ -629,6 +634,7 @@ struct CreateOpts {
     double cell_scale = 1.0;      // cell side multiplier (<= 1)
     hipStream_t stream = nullptr; // borrowed stream
     bool trainer = true;          // learned: own training stream (batches share one)
+    bool batch_member = false;    // a stream of a batch (its own slide kernels)
 };
 
 rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts &opt = {})
@@ -684,7 +690,8 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
     d.Rp = (double)R * (1.0 + 1e-5);
 
     const size_t W = c->W, S = c->S, dim = c->dim, m = c->m;
-    const size_t tmpn = W > S ? W : S;
+    // the tile scratch: the prime's segments (W), or two slides' tiles
+    const size_t tmpn = W > 2 * S ? W : 2 * S;
     RTOD_TRY(dalloc(c, &d.ring, dim * W));
     d.tmpn = (int)tmpn;
     for (int p = 0; p < 2; ++p) {
@@ -696,7 +703,11 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
         RTOD_TRY(dalloc(c, &d.tslot[p], tmpn));
         RTOD_TRY(dalloc(c, &d.txyz[p], dim * tmpn));
     }
-    d.nent = (int)m + 2;
+    d.nent = (int)m + 3;
+    // single-stream slides sorted as several tiles and joined by the run
+    // merge keep two slides' tiles (pipelined slides sort them two launches
+    // ahead of the merge); a batch's streams merge tiles in passes
+    d.tile_par = (!opt.batch_member && use_run_merge(c->S, c->CH)) ? 1 : 0;
     const size_t ne = (size_t)d.nent;
     RTOD_TRY(dalloc(c, &d.bkey, ne * S));
     RTOD_TRY(dalloc(c, &d.bslot, ne * S));
@@ -1128,9 +1139,10 @@ rtod_status wait_slide(rtod_ctx *c, int64_t t)
 // is waited for before the next is enqueued (a measuring mode).
 // Pipelined slides in two launches per slide (cell-stencil index, the
 // ingest riding; see k_classify_stage): after a prologue (slide t0's ingest
-// and window merge, slide t0 + 1's ingest), slide t0 + i is the classify
-// stage A(i) and the detect stage B(i), and the last slide's epilogue
-// follows in a launch of its own.  Up to kHostOuts slides are in flight.
+// and window merge, the ingest of slides t0 + 1 and t0 + 2), slide t0 + i
+// is the classify stage A(i) and the detect stage B(i), and the last
+// slide's epilogue follows in a launch of its own.  Up to kHostOuts slides
+// are in flight.
 rtod_status slides_two_stage(rtod_ctx *c, const float *dev, int32_t n, int32_t *offs,
                              int32_t *slots)
 {
@@ -1153,6 +1165,8 @@ rtod_status slides_two_stage(rtod_ctx *c, const float *dev, int32_t n, int32_t *
         offs[++done] = (int32_t)used;
         return maybe_train(c);
     };
+    // prologue: slide 0's ingest; slide 0's window merge with slide 1's
+    // tile sorts; slide 1's run merge; slide 2's tile sorts
     const Step s0 = step(0);
     launch_ingest_stage(c, s, s0, nullptr);
     if (n > 1) {
@@ -1160,6 +1174,9 @@ rtod_status slides_two_stage(rtod_ctx *c, const float *dev, int32_t n, int32_t *
         launch_merge_window(c, s, s0, &s1);
         if (runs)
             launch_batch_merges(c, s, s1, kModeSlide, c->S);
+        if (n > 2)
+            launch_ingest_dim(c, s, step(2), kModeSlide, nullptr, c->d.cpb,
+                              c->d.cpb == 1 ? 1 : 0);
     } else {
         launch_merge_window(c, s, s0);
     }
@@ -1175,13 +1192,14 @@ rtod_status slides_two_stage(rtod_ctx *c, const float *dev, int32_t n, int32_t *
         const Step prev = i > 0 ? step(i - 1) : Step{};
         const Step nx1 = i + 1 < n ? step(i + 1) : Step{};
         const Step nx2 = i + 2 < n ? step(i + 2) : Step{};
+        const Step nx3 = i + 3 < n ? step(i + 3) : Step{};
         if (timed)
             (void)hipEventRecord(c->ev[2], s);
-        launch_classify_stage(c, s, cur, i > 0 ? &prev : nullptr, i + 2 < n ? &nx2 : nullptr);
+        launch_classify_stage(c, s, cur, i > 0 ? &prev : nullptr);
         if (timed)
             (void)hipEventRecord(c->ev[3], s);
         launch_detect_stage(c, s, cur, i + 1 < n ? &nx1 : nullptr,
-                            (i + 2 < n && runs) ? &nx2 : nullptr);
+                            (i + 2 < n && runs) ? &nx2 : nullptr, i + 3 < n ? &nx3 : nullptr);
         if (timed)
             (void)hipEventRecord(c->ev[4], s);
         if (i + 1 == n)
@@ -1841,6 +1859,7 @@ rtod_status batch_create_members(rtod_batch *b)
         opt.cell_scale = b->tuning.cell_scale[b->cfg[i].dim - 1];
         opt.stream = b->stream;
         opt.trainer = false;
+        opt.batch_member = true;
         rtod_ctx *c = nullptr;
         rtod_status st = create_impl(&b->cfg[i], &c, opt);
         if (st != RTOD_OK) {

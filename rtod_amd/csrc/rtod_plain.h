@@ -312,9 +312,11 @@ __device__ __forceinline__ void ingest_sort_body(const Dev &d, const Step &st, i
         ox = d.bxyz + (size_t)e * d.S;
         xs = stride_b;
     } else {
-        ok = d.tkey[0] + (size_t)seg * d.S + j0;
-        os = d.tslot[0] + (size_t)seg * d.S + j0;
-        ox = d.txyz[0] + (size_t)seg * d.S + j0;
+        const size_t toff = (size_t)seg * d.S + j0 +
+                            ((mode == kModeSlide && d.tile_par) ? (size_t)(st.t & 1) * d.S : 0);
+        ok = d.tkey[0] + toff;
+        os = d.tslot[0] + toff;
+        ox = d.txyz[0] + toff;
         xs = (size_t)d.tmpn;
     }
     const int bmask = (1 << d.bs_log2) - 1;
@@ -456,7 +458,9 @@ __device__ __forceinline__ void merge_runs_body(const Dev &d, const Step &st, in
     const int wps = run_merge_wgs(S);
     const int seg = bid / wps, w = bid % wps;
     const size_t sxs = (size_t)d.tmpn;
-    const uint64_t *sk = d.tkey[0] + (size_t)seg * S;
+    const size_t toff = (size_t)seg * S +
+                        ((mode == kModeSlide && d.tile_par) ? (size_t)(st.t & 1) * S : 0);
+    const uint64_t *sk = d.tkey[0] + toff;
     int32_t sl[IT];
     float cx[IT][3];
 #pragma unroll
@@ -465,10 +469,10 @@ __device__ __forceinline__ void merge_runs_body(const Dev &d, const Step &st, in
         sl[k] = 0;
         cx[k][0] = cx[k][1] = cx[k][2] = 0.f;
         if (i < S) {
-            sl[k] = d.tslot[0][(size_t)seg * S + i];
+            sl[k] = d.tslot[0][toff + i];
 #pragma unroll
             for (int a = 0; a < DIM; ++a)
-                cx[k][a] = d.txyz[0][(size_t)a * sxs + (size_t)seg * S + i];
+                cx[k][a] = d.txyz[0][(size_t)a * sxs + toff + i];
         }
     }
     copy_to_lds<kRunMergeCap / kRunMergeNT>(s_k, sk, S);
@@ -2028,65 +2032,61 @@ __global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_detect_runs
 // ---------------------------------------------------------------------------
 // Two launches per slide (pipelined slides on one stream, cell-stencil
 // index, the ingest riding): the classify stage of slide t carries the
-// epilogue of slide t - 1 and the tile sorts of slide t + 2; the detect
-// stage of slide t carries the window merge of slide t + 1 and the run
-// merge of slide t + 2.  The parts of a launch touch disjoint memory:
+// epilogue of slide t - 1; the detect stage of slide t carries the window
+// merge of slide t + 1, the run merge of slide t + 2 and the tile sorts of
+// slide t + 3.  The parts of a launch touch disjoint memory:
 //   classify(t): reads window t; writes the candidate and unit lists and
 //     counters cnt[t & 1];
 //   epilogue(t - 1): reads bitmap (t - 1) & 1 and cnt[(t - 1) & 1]; writes
 //     host-out buffer t - 1;
-//   tile sorts(t + 2): write the tile scratch (one tile: batch-ring entry
-//     enew(t + 2) = eold(t), whose last reader, merge(t), ran a launch
-//     earlier);
 //   detect(t): reads window t, the lists and cnt[t & 1]; sets bitmap t & 1;
-//   merge(t + 1): reads window t and ring entries enew(t + 1), eold(t + 1);
-//     writes window t + 1 (the other buffers); resets cnt[(t + 1) & 1] and
-//     clears bitmap (t + 1) & 1 (both last read by epilogue(t - 1), a
-//     launch earlier);
-//   run merge(t + 2): reads the tile scratch; writes ring entry enew(t + 2),
-//     distinct from enew(t + 1) and eold(t + 1) with m + 2 entries.
+//   merge(t + 1): reads window t and batch-ring entries enew(t + 1),
+//     eold(t + 1); writes window t + 1 (the other buffers); resets
+//     cnt[(t + 1) & 1] and clears bitmap (t + 1) & 1 (both last read by
+//     epilogue(t - 1), a launch earlier);
+//   run merge(t + 2): reads tile scratch (t + 2) & 1; writes ring entry
+//     enew(t + 2);
+//   tile sorts(t + 3): write tile scratch (t + 3) & 1 (one tile: ring entry
+//     enew(t + 3)).
+// With m + 3 ring entries, enew(t + 1), eold(t + 1), enew(t + 2) and
+// enew(t + 3) are t - 2, t + 1, t - 1 and t modulo m + 3: all distinct.
 // ---------------------------------------------------------------------------
-template <int DIM, int MODE, int IT>
+template <int DIM, int MODE>
 __global__ __launch_bounds__(kClassifyNT) void k_classify_stage(Dev d, Step st, Step st_prev,
-                                                               Step st_ts, int nfin, int ncls)
+                                                               int nfin, int ncls)
 {
-    static_assert(kClassifyNT == 256, "the tile sorts and epilogue parts take 256 threads");
+    static_assert(kClassifyNT == 256, "the epilogue parts take 256 threads");
     __shared__ union {
         ClassifyLds<MODE == kClsLearned> cls;
         FinishLds fin;
-        IngestLds<DIM, 256, IT> ing;
     } lds;
-    int bid = blockIdx.x;
-    if (bid < nfin) {
+    const int bid = blockIdx.x;
+    if (bid < nfin)
         finish_deferred<256>(d, st_prev, bid, nfin, lds.fin);
-        return;
-    }
-    bid -= nfin;
-    if (bid < ncls) {
-        classify_body<DIM, MODE>(d, st, bid, lds.cls);
-        return;
-    }
-    ingest_sort_body<DIM, 256, IT>(d, st_ts, kModeSlide, nullptr, d.cpb == 1 ? 1 : 0,
-                                   bid - ncls, lds.ing);
+    else if (bid - nfin < ncls)
+        classify_body<DIM, MODE>(d, st, bid - nfin, lds.cls);
 }
 
 // LDS of k_detect_stage, set at launch: the largest of its parts'
-__host__ __device__ constexpr size_t detect_stage_lds(bool merge, bool runs, int S)
+template <int DIM, int TSIT>
+__host__ __device__ constexpr size_t detect_stage_lds(bool merge, bool runs, bool ts, int S)
 {
     size_t b = sizeof(DetectLds);
     if (merge && sizeof(MergeWindowLds<kDetectNT>) > b)
         b = sizeof(MergeWindowLds<kDetectNT>);
     if (runs && (size_t)S * sizeof(uint64_t) > b)
         b = (size_t)S * sizeof(uint64_t);
+    if (ts && sizeof(IngestLds<DIM, kDetectNT, TSIT>) > b)
+        b = sizeof(IngestLds<DIM, kDetectNT, TSIT>);
     return b;
 }
 
 // workgroups: nmb window-merge (slide st_merge), nrm run-merge (slide
-// st_runs), then the ndet of the detect (slide st), whose busy ones come
-// first in the list
-template <int DIM, bool LEARN>
+// st_runs), nts tile-sort (slide st_ts, tiles of kDetectNT * TSIT points),
+// then the ndet of the detect (slide st), whose busy ones come first
+template <int DIM, bool LEARN, int TSIT>
 __global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_detect_stage(
-    Dev d, Step st, Step st_merge, Step st_runs, int nmb, int nrm, int ndet)
+    Dev d, Step st, Step st_merge, Step st_runs, Step st_ts, int nmb, int nrm, int nts, int ndet)
 {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     int bid = blockIdx.x;
@@ -2101,6 +2101,13 @@ __global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_detect_stag
         return;
     }
     bid -= nrm;
+    if (bid < nts) {
+        ingest_sort_body<DIM, kDetectNT, TSIT>(
+            d, st_ts, kModeSlide, nullptr, d.cpb == 1 ? 1 : 0, bid,
+            *reinterpret_cast<IngestLds<DIM, kDetectNT, TSIT> *>(dyn_lds));
+        return;
+    }
+    bid -= nts;
     detect_body<DIM, LEARN>(d, st, bid, ndet, false, *reinterpret_cast<DetectLds *>(dyn_lds));
 }
 

@@ -113,8 +113,10 @@ struct Step {
 
 struct Dev {
     int dim, W, S, m, K;
-    int nent;         // batch-ring entries, m + 2: the next slide's batch can be
-                      // ingested while this slide's merge reads its two
+    int nent;         // batch-ring entries, m + 3: the batches of the next three
+                      // slides can be ingested while this slide's merge reads its two
+    int tile_par;     // slide batches sorted as several tiles go to the tile
+                      // scratch at offset (t & 1) * S (two slides' tiles live)
     int cpb;          // sort chunks per batch
     int nsamp;        // samples in the sorted-window sample table
     int stride_log2;  // sample stride
