@@ -1619,144 +1619,19 @@ __device__ __forceinline__ int emit_chunk(HostOut *ho, FinishLds &fl, int c, uin
     return ctot;
 }
 
-template <int NT>
-__device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int f, int F,
-                                             const SlideSummary &sm, FinishLds &fl)
+// The slide's counters and learned bookkeeping, then the stamp, by one
+// thread once the whole list is written (its own wave's list stores, or
+// every part's, released before their tickets); total: the slide's outlier
+// count.  The stamp is written after one system-scope release (cumulative
+// over the list writes ordered before it): a host that sees the stamp
+// (polling, acquire) sees the list and the counters.  err is taken by an
+// exchange: the next slide's window merge may run beside the epilogue, and
+// a flag it raises is reported with this slide or the next, never lost.
+// (The other parts wrote nothing read here; the counters were written by
+// earlier launches or by atomics.)
+__device__ __forceinline__ void publish_slide(const Dev &d, const Step &st,
+                                              const SlideSummary &sm, HostOut *ho, int total)
 {
-    static_assert(NT >= kChunkWords, "a word of the chunk per thread");
-    const int tid = threadIdx.x;
-    const int nwords = (d.W + 31) >> 5;
-    const int nch = (nwords + kChunkWords - 1) / kChunkWords;
-    // thread tid holds word tid of each chunk (none past the chunk)
-    const int nmine = tid < kChunkWords ? nwords : 0;
-    const uint32_t *bm = bitmap_of(d, st.t);
-    HostOut *ho = host_out(d, st.t);
-    int P = 0, total = 0;
-    bool single = F == 1; // this part lists the whole slide
-    int wrote = 0;        // list entries this part wrote
-    if (nch <= kCompactRegs) {
-        uint32_t wv[kCompactRegs];
-#pragma unroll
-        for (int k = 0; k < kCompactRegs; ++k)
-            wv[k] = (k < nch && k * kChunkWords + tid < nmine) ? bm[k * kChunkWords + tid] : 0u;
-        int before = 0, all = 0;
-        uint32_t nz = 0; // bit k: chunk k holds outlier bits (this wave's words)
-#pragma unroll
-        for (int k = 0; k < kCompactRegs; ++k) {
-            const int b = __popc(wv[k]);
-            all += b;
-            before += (k < f) ? b : 0;
-            nz |= __ballot(wv[k] != 0u) ? (1u << k) : 0u;
-        }
-        block_sum2<NT>(before, all, nz, fl.red, P, total, nz);
-        if (total <= kSparseList) {
-            // part 0 alone, one pass: bit b of word w of chunk k goes to
-            // (bits of chunks < k) + (bits of chunk k in threads < w) +
-            // popc(word below b); the per-chunk prefixes over threads come
-            // from one scan of the packed counts (<= 255 each)
-            if (f != 0)
-                return;
-            single = true;
-            uint32_t pk[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-            for (int k = 0; k < kCompactRegs; ++k)
-                pk[k >> 2] |= (uint32_t)__popc(wv[k]) << (8 * (k & 3));
-            uint32_t tot[4];
-            block_excl_scan4<NT>(pk, fl.red, tot);
-            int cpre = 0;
-#pragma unroll
-            for (int k = 0; k < kCompactRegs; ++k) {
-                int p = cpre + (int)((pk[k >> 2] >> (8 * (k & 3))) & 255u);
-                uint32_t bits = wv[k];
-                const int word = k * kChunkWords + tid;
-                while (bits) {
-                    fl.list[p++] = (word << 5) + __builtin_ctz(bits);
-                    bits &= bits - 1;
-                }
-                cpre += (int)((tot[k >> 2] >> (8 * (k & 3))) & 255u);
-            }
-            __syncthreads();
-            for (int i = tid; i < total; i += NT)
-                ho->list[i] = fl.list[i];
-            wrote = total;
-        } else {
-            for (int c = f; c < nch; c += F) {
-                uint32_t wd = 0;
-                int mid = 0; // bits of the chunks up to this part's next one
-#pragma unroll
-                for (int k = 0; k < kCompactRegs; ++k) {
-                    wd = (k == c) ? wv[k] : wd;
-                    mid += (k > c && k < c + F) ? __popc(wv[k]) : 0;
-                }
-                if ((nz >> c) & 1u) { // empty chunks are skipped (uniform)
-                    const int n = emit_chunk<NT>(ho, fl, c, wd, P);
-                    P += n;
-                    wrote += n;
-                }
-                const uint32_t between = (nz >> (c + 1)) & ((1u << (F - 1)) - 1u);
-                if (F > 1 && c + F < nch && between) {
-                    int m, unused;
-                    uint32_t un2;
-                    block_sum2<NT>(mid, 0, 0u, fl.red, m, unused, un2);
-                    P += m;
-                }
-            }
-        }
-    } else {
-        int before = 0, all = 0;
-        for (int k = 0; k < nch; ++k) {
-            const int wi = k * kChunkWords + tid;
-            const int b = wi < nmine ? __popc(bm[wi]) : 0;
-            all += b;
-            before += (k < f) ? b : 0;
-        }
-        uint32_t unused;
-        block_sum2<NT>(before, all, 0u, fl.red, P, total, unused);
-        for (int c = f; c < nch; c += F) {
-            const int wi = c * kChunkWords + tid;
-            const int n = emit_chunk<NT>(ho, fl, c, wi < nmine ? bm[wi] : 0u, P);
-            P += n;
-            wrote += n;
-            if (F > 1 && c + F < nch) {
-                int mid = 0;
-                for (int k = c + 1; k < c + F; ++k) {
-                    const int wk = k * kChunkWords + tid;
-                    mid += wk < nmine ? __popc(bm[wk]) : 0;
-                }
-                int m, un1;
-                uint32_t un2;
-                block_sum2<NT>(mid, 0, 0u, fl.red, m, un1, un2);
-                P += m;
-            }
-        }
-    }
-    // every wave's list writes issued and acknowledged; a part of several
-    // that wrote any then releases them at system scope (a release only: no
-    // cache invalidation under the window merge running beside it) before
-    // taking its ticket.  (Acknowledged writes to host memory are not yet
-    // visible to the host without the release.)
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();
-    if (tid == 0) {
-        bool last = true;
-        if (!single) {
-            if (wrote)
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
-            last = atomicAdd(&d.ctl->fin_ticket, 1u) == (unsigned)F - 1;
-        }
-        fl.last = last;
-    }
-    __syncthreads();
-    if (!fl.last || tid != 0)
-        return;
-    // the last part: counters, learned bookkeeping, then the stamp, written
-    // after one system-scope release (cumulative over the parts' list
-    // writes): a host that sees the stamp (polling, acquire) sees the list
-    // and the counters.  err is taken by an exchange: the next slide's
-    // window merge may run beside the epilogue, and a flag it raises is
-    // reported with this slide or the next, never lost.  (The other parts
-    // wrote nothing this part reads; the counters were written by earlier
-    // launches or by atomics.)
     if (sm.learn) {
         LearnCtl *lc = d.lr.lc;
         const int slot = (int)(st.t % kReplaySlides);
@@ -1786,6 +1661,140 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int f
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
     __hip_atomic_store(&ho->slide, (int64_t)(st.t + 1), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+template <int NT>
+__device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int f, int F,
+                                             const SlideSummary &sm, FinishLds &fl)
+{
+    static_assert(NT >= kChunkWords, "a word of the chunk per thread");
+    const int tid = threadIdx.x;
+    const int nwords = (d.W + 31) >> 5;
+    const int nch = (nwords + kChunkWords - 1) / kChunkWords;
+    // thread tid holds word tid of each chunk (none past the chunk)
+    const int nmine = tid < kChunkWords ? nwords : 0;
+    const uint32_t *bm = bitmap_of(d, st.t);
+    HostOut *ho = host_out(d, st.t);
+    int P = 0, total = 0;
+    const bool single = F == 1; // this part lists the whole slide
+    if (nch <= kCompactRegs) {
+        uint32_t wv[kCompactRegs];
+#pragma unroll
+        for (int k = 0; k < kCompactRegs; ++k)
+            wv[k] = (k < nch && k * kChunkWords + tid < nmine) ? bm[k * kChunkWords + tid] : 0u;
+        int before = 0, all = 0;
+        uint32_t nz = 0; // bit k: chunk k holds outlier bits (this wave's words)
+#pragma unroll
+        for (int k = 0; k < kCompactRegs; ++k) {
+            const int b = __popc(wv[k]);
+            all += b;
+            before += (k < f) ? b : 0;
+            nz |= __ballot(wv[k] != 0u) ? (1u << k) : 0u;
+        }
+        block_sum2<NT>(before, all, nz, fl.red, P, total, nz);
+        if (total <= kSparseList) {
+            // part 0 alone, one pass: bit b of word w of chunk k goes to
+            // (bits of chunks < k) + (bits of chunk k in threads < w) +
+            // popc(word below b); the per-chunk prefixes over threads come
+            // from one scan of the packed counts (<= 255 each)
+            if (f != 0)
+                return;
+            uint32_t pk[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int k = 0; k < kCompactRegs; ++k)
+                pk[k >> 2] |= (uint32_t)__popc(wv[k]) << (8 * (k & 3));
+            uint32_t tot[4];
+            block_excl_scan4<NT>(pk, fl.red, tot);
+            int cpre = 0;
+#pragma unroll
+            for (int k = 0; k < kCompactRegs; ++k) {
+                int p = cpre + (int)((pk[k >> 2] >> (8 * (k & 3))) & 255u);
+                uint32_t bits = wv[k];
+                const int word = k * kChunkWords + tid;
+                while (bits) {
+                    fl.list[p++] = (word << 5) + __builtin_ctz(bits);
+                    bits &= bits - 1;
+                }
+                cpre += (int)((tot[k >> 2] >> (8 * (k & 3))) & 255u);
+            }
+            // one wave writes the list and publishes: one release (its
+            // stores, then the stamp) and no further barrier
+            __syncthreads();
+            if (tid < kWave) {
+                for (int i = tid; i < total; i += kWave)
+                    ho->list[i] = fl.list[i];
+                if (tid == 0)
+                    publish_slide(d, st, sm, ho, total);
+            }
+            return;
+        } else {
+            for (int c = f; c < nch; c += F) {
+                uint32_t wd = 0;
+                int mid = 0; // bits of the chunks up to this part's next one
+#pragma unroll
+                for (int k = 0; k < kCompactRegs; ++k) {
+                    wd = (k == c) ? wv[k] : wd;
+                    mid += (k > c && k < c + F) ? __popc(wv[k]) : 0;
+                }
+                if ((nz >> c) & 1u) { // empty chunks are skipped (uniform)
+                    const int n = emit_chunk<NT>(ho, fl, c, wd, P);
+                    P += n;
+                }
+                const uint32_t between = (nz >> (c + 1)) & ((1u << (F - 1)) - 1u);
+                if (F > 1 && c + F < nch && between) {
+                    int m, unused;
+                    uint32_t un2;
+                    block_sum2<NT>(mid, 0, 0u, fl.red, m, unused, un2);
+                    P += m;
+                }
+            }
+        }
+    } else {
+        int before = 0, all = 0;
+        for (int k = 0; k < nch; ++k) {
+            const int wi = k * kChunkWords + tid;
+            const int b = wi < nmine ? __popc(bm[wi]) : 0;
+            all += b;
+            before += (k < f) ? b : 0;
+        }
+        uint32_t unused;
+        block_sum2<NT>(before, all, 0u, fl.red, P, total, unused);
+        for (int c = f; c < nch; c += F) {
+            const int wi = c * kChunkWords + tid;
+            const int n = emit_chunk<NT>(ho, fl, c, wi < nmine ? bm[wi] : 0u, P);
+            P += n;
+            if (F > 1 && c + F < nch) {
+                int mid = 0;
+                for (int k = c + 1; k < c + F; ++k) {
+                    const int wk = k * kChunkWords + tid;
+                    mid += wk < nmine ? __popc(bm[wk]) : 0;
+                }
+                int m, un1;
+                uint32_t un2;
+                block_sum2<NT>(mid, 0, 0u, fl.red, m, un1, un2);
+                P += m;
+            }
+        }
+    }
+    // every wave's list writes issued and acknowledged; a part of several
+    // then releases them at system scope (a release only: no cache
+    // invalidation under the window merge running beside it) before taking
+    // its ticket.  (Acknowledged writes to host memory are not yet visible
+    // to the host without the release.)
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (tid == 0) {
+        bool last = true;
+        if (!single) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+            last = atomicAdd(&d.ctl->fin_ticket, 1u) == (unsigned)F - 1;
+        }
+        fl.last = last;
+    }
+    __syncthreads();
+    if (!fl.last || tid != 0)
+        return;
+    publish_slide(d, st, sm, ho, total);
 }
 
 // Part f of F of slide st's epilogue, run after its detect launch
