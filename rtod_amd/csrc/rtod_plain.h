@@ -1619,41 +1619,69 @@ __device__ __forceinline__ int emit_chunk(HostOut *ho, FinishLds &fl, int c, uin
     return ctot;
 }
 
+// The counters publish_slide reports, requested by one thread when the
+// epilogue starts (with the bitmap loads) rather than when it ends: every
+// one of them is final by then (detect and classify are complete), and
+// error flags raised later (by the next slide's merge running beside the
+// epilogue) stay set for the next slide.
+struct PubIn {
+    unsigned long long pair_tests;
+    int32_t err, n_cand, misses, train_steps;
+    float train_loss;
+};
+
+__device__ __forceinline__ PubIn load_pub_inputs(const Dev &d, const Step &st, bool learn)
+{
+    PubIn in;
+    in.pair_tests = __hip_atomic_load(&d.ctl->pair_tests, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+    in.err = __hip_atomic_load(&d.ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    in.n_cand = cnt_of(d, st.t)->n_cand;
+    in.misses = in.train_steps = 0;
+    in.train_loss = 0.f;
+    if (learn) {
+        in.misses = __hip_atomic_load(&d.lr.lc->misses, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+        in.train_steps = d.lr.ls->steps_total;
+        in.train_loss = d.lr.ls->loss_last;
+    }
+    return in;
+}
+
 // The slide's counters and learned bookkeeping, then the stamp, by one
 // thread once the whole list is written (its own wave's list stores, or
 // every part's, released before their tickets); total: the slide's outlier
 // count.  The stamp is written after one system-scope release (cumulative
 // over the list writes ordered before it): a host that sees the stamp
-// (polling, acquire) sees the list and the counters.  err is taken by an
-// exchange: the next slide's window merge may run beside the epilogue, and
-// a flag it raises is reported with this slide or the next, never lost.
-// (The other parts wrote nothing read here; the counters were written by
-// earlier launches or by atomics.)
+// (polling, acquire) sees the list and the counters.  The error flags
+// reported are cleared by an atomic AND of their complement, so a flag
+// raised after they were read is reported with the next slide, never lost.
 __device__ __forceinline__ void publish_slide(const Dev &d, const Step &st,
-                                              const SlideSummary &sm, HostOut *ho, int total)
+                                              const SlideSummary &sm, const PubIn &in,
+                                              HostOut *ho, int total)
 {
     if (sm.learn) {
         LearnCtl *lc = d.lr.lc;
         const int slot = (int)(st.t % kReplaySlides);
-        const int miss = __hip_atomic_load(&lc->misses, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         d.lr.rep_n[2 * slot] = sm.n_o;
         d.lr.rep_n[2 * slot + 1] = sm.n_units - sm.n_o;
-        ho->misses = miss;
+        ho->misses = in.misses;
         ho->used_pred = sm.use_pred ? 1 : 0;
-        ho->train_steps = d.lr.ls->steps_total;
-        ho->train_loss = d.lr.ls->loss_last;
+        ho->train_steps = in.train_steps;
+        ho->train_loss = in.train_loss;
         // fallback rule: schedule the next slide by prediction only if this
         // slide's miss rate (over units) is within the threshold
         if (sm.n_units > 0)
-            lc->use_pred = ((float)miss <= d.lr.miss_thr * (float)sm.n_units) ? 1 : 0;
+            lc->use_pred = ((float)in.misses <= d.lr.miss_thr * (float)sm.n_units) ? 1 : 0;
         lc->misses = 0;
     }
     ho->n_units = sm.n_units;
     ho->n_out = total;
-    ho->n_cand = cnt_of(d, st.t)->n_cand;
-    ho->pair_tests = __hip_atomic_load(&d.ctl->pair_tests, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-    ho->err = atomicExch(&d.ctl->err, 0);
+    ho->n_cand = in.n_cand;
+    ho->pair_tests = in.pair_tests;
+    ho->err = in.err;
+    if (in.err)
+        atomicAnd(&d.ctl->err, ~in.err); // the flags reported here
     d.ctl->pair_tests = 0;
     d.ctl->ticket = 0;
     d.ctl->fin_ticket = 0;
@@ -1675,6 +1703,9 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int f
     const int nmine = tid < kChunkWords ? nwords : 0;
     const uint32_t *bm = bitmap_of(d, st.t);
     HostOut *ho = host_out(d, st.t);
+    PubIn in{};
+    if (tid == 0)
+        in = load_pub_inputs(d, st, sm.learn);
     int P = 0, total = 0;
     const bool single = F == 1; // this part lists the whole slide
     if (nch <= kCompactRegs) {
@@ -1724,7 +1755,7 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int f
                 for (int i = tid; i < total; i += kWave)
                     ho->list[i] = fl.list[i];
                 if (tid == 0)
-                    publish_slide(d, st, sm, ho, total);
+                    publish_slide(d, st, sm, in, ho, total);
             }
             return;
         } else {
@@ -1794,7 +1825,7 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int f
     __syncthreads();
     if (!fl.last || tid != 0)
         return;
-    publish_slide(d, st, sm, ho, total);
+    publish_slide(d, st, sm, in, ho, total);
 }
 
 // Part f of F of slide st's epilogue, run after its detect launch
