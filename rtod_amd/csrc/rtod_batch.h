@@ -35,10 +35,13 @@ struct BatchStream {
                            // nullptr: the pinned staging buffers
 };
 
-__device__ __forceinline__ Step batch_step(const BatchStream &bs)
+// det: for the detect phase, whose last workgroup advances ctl->t while
+// other workgroups of the stream may still be starting (they take the slide
+// that classify recorded)
+__device__ __forceinline__ Step batch_step(const BatchStream &bs, bool det = false)
 {
     const Dev &d = bs.d;
-    const uint64_t t = d.ctl->t;
+    const uint64_t t = det ? d.ctl->t_det : d.ctl->t;
     const uint64_t m = (uint64_t)d.m;
     Step st;
     st.t = t;
@@ -113,7 +116,7 @@ __global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_batch_detec
     __shared__ DetectLds lds;
     const BatchItem it = items[blockIdx.x];
     const BatchStream &bs = streams[it.stream];
-    detect_body<DIM, LEARN>(bs.d, batch_step(bs), it.bid, it.arg, true, lds);
+    detect_body<DIM, LEARN>(bs.d, batch_step(bs, true), it.bid, it.arg, true, lds);
 }
 
 // The batch's shared predictor (learned batches), trained on every stream's

@@ -944,6 +944,8 @@ __device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int 
     constexpr bool LEARN = MODE == kClsLearned;
     uint64_t *s_t = lds.t;
     const int i0 = bid * NT;
+    if (bid == 0 && threadIdx.x == 0)
+        d.ctl->t_det = st.t;
     const int i = i0 + threadIdx.x;
     const uint64_t *Nk = d.akey[st.par ^ 1];
     const int W = d.W, K = d.K;
@@ -1701,7 +1703,13 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int f
     const int nch = (nwords + kChunkWords - 1) / kChunkWords;
     // thread tid holds word tid of each chunk (none past the chunk)
     const int nmine = tid < kChunkWords ? nwords : 0;
-    const uint32_t *bm = bitmap_of(d, st.t);
+    // the bitmap's words are read with agent-scope loads: in the last detect
+    // workgroup they were set by other workgroups' atomics in the same
+    // kernel, possibly on other XCDs, whose L2 lines a plain load could miss
+    uint32_t *bmw = bitmap_of(d, st.t);
+    auto bm_load = [&](int w) {
+        return __hip_atomic_load(&bmw[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
     HostOut *ho = host_out(d, st.t);
     PubIn in{};
     if (tid == 0)
@@ -1712,7 +1720,7 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int f
         uint32_t wv[kCompactRegs];
 #pragma unroll
         for (int k = 0; k < kCompactRegs; ++k)
-            wv[k] = (k < nch && k * kChunkWords + tid < nmine) ? bm[k * kChunkWords + tid] : 0u;
+            wv[k] = (k < nch && k * kChunkWords + tid < nmine) ? bm_load(k * kChunkWords + tid) : 0u;
         int before = 0, all = 0;
         uint32_t nz = 0; // bit k: chunk k holds outlier bits (this wave's words)
 #pragma unroll
@@ -1784,7 +1792,7 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int f
         int before = 0, all = 0;
         for (int k = 0; k < nch; ++k) {
             const int wi = k * kChunkWords + tid;
-            const int b = wi < nmine ? __popc(bm[wi]) : 0;
+            const int b = wi < nmine ? __popc(bm_load(wi)) : 0;
             all += b;
             before += (k < f) ? b : 0;
         }
@@ -1792,13 +1800,13 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int f
         block_sum2<NT>(before, all, 0u, fl.red, P, total, unused);
         for (int c = f; c < nch; c += F) {
             const int wi = c * kChunkWords + tid;
-            const int n = emit_chunk<NT>(ho, fl, c, wi < nmine ? bm[wi] : 0u, P);
+            const int n = emit_chunk<NT>(ho, fl, c, wi < nmine ? bm_load(wi) : 0u, P);
             P += n;
             if (F > 1 && c + F < nch) {
                 int mid = 0;
                 for (int k = c + 1; k < c + F; ++k) {
                     const int wk = k * kChunkWords + tid;
-                    mid += wk < nmine ? __popc(bm[wk]) : 0;
+                    mid += wk < nmine ? __popc(bm_load(wk)) : 0;
                 }
                 int m, un1;
                 uint32_t un2;

@@ -49,6 +49,8 @@ struct SlideCnt {
 
 struct Ctl {
     uint64_t t;            // slides completed since prime
+    uint64_t t_det;        // the slide last classified: a batch's detect takes its
+                           // slide from here, since t advances while it runs
     uint32_t ticket;       // last-workgroup ticket of the detect kernel
     uint32_t fin_ticket;   // parts done of a deferred slide epilogue
     unsigned long long pair_tests;

@@ -160,3 +160,29 @@ def test_learned_batch_exact_and_trains():
     _check(got, want, "learned batch")
     assert all(st["predictor_points"] > 0 for st in stats if st["candidates_total"] > 0)
     assert max(st["train_steps"] for st in stats) > 0  # the shared predictor trained
+
+
+def test_batch_large_streams_plain_and_learned_exact():
+    """256 uneven streams (W up to 100k) over many steps, plain and learned,
+    equal the grid oracle on every step.  In a batch, the last detect
+    workgroup of a stream publishes the slide (and advances the stream's
+    slide counter) while the stream's other detect workgroups may still be
+    starting; these take the slide from the counter classify left, else
+    they would read the next slide's counters and take a detect ticket,
+    making the next slide's epilogue run early and miss outliers."""
+    from rtod_amd import synth
+    case("batch of 256 streams up to W=100k, plain (40 steps) and learned (20)")
+    T = 40
+    descs, data = synth.batch_workload(256, seed=11, slides=T)
+    cfg = [(d.dim, d.W, d.S, d.R, d.K) for d in descs]
+    want = [O.run_stream(xs, d.W, d.S, T, d.R, d.K, "grid", nthreads=16)
+            for d, xs in zip(descs, data)]
+    for sched, steps in ((R.SCHED_PLAIN, T), (R.SCHED_LEARNED, T // 2)):
+        with R.Batch(cfg, sched_kind=sched) as b:
+            ds = _bind(b, cfg, data)
+            b.prime_device()
+            got = [b.step_device() for _ in range(steps)]
+            b.close()
+            for d in ds:
+                d.close()
+        _check(got, want, f"256-stream batch sched {sched}")
