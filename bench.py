@@ -506,7 +506,7 @@ def main() -> int:
                 "+-U(2,5)); 3-D: mixture of 8 drifting anisotropic Gaussians + 0.5% noise",
         "config": {"workload": f"{args.workload} W={W} S={S} R={R} K={K} dim={dim} "
                                "(plain schedule, cell-stencil index; consecutive slides "
-                               "pipelined: the next slide's batch sorted on a second stream)",
+                               "pipelined on one stream, two launches per slide)",
                    "window": W, "slide": S, "R": R, "K": K, "dim": dim},
         "sequential_us_per_slide": round(plain["lib_ms"] * 1e3, 3),
         "pipelined_mismatched_slides": pipe_mism,
