@@ -358,78 +358,78 @@ void launch_finish(rtod_ctx *c, hipStream_t s, const Step &st)
 }
 
 // Two launches per slide (cell-stencil index, the ingest riding): slide
-// st's classify with the epilogue of prev ...
+// st's classify ...
 template <int DIM, int MODE>
-void launch_classify_stage_dim(rtod_ctx *c, hipStream_t s, const Step &st, const Step *prev)
+void launch_classify_stage_dim(rtod_ctx *c, hipStream_t s, const Step &st)
 {
-    const int nfin = prev ? c->d.fin_parts : 0;
     const int ncls = (c->W + kClassifyNT - 1) / kClassifyNT;
-    const Step pv = prev ? *prev : Step{};
-    hipLaunchKernelGGL((k_classify_stage<DIM, MODE>), dim3(nfin + ncls), dim3(kClassifyNT), 0, s,
-                       c->d, st, pv, nfin, ncls);
+    hipLaunchKernelGGL((k_classify_stage<DIM, MODE>), dim3(ncls), dim3(kClassifyNT), 0, s, c->d,
+                       st);
 }
 
-void launch_classify_stage(rtod_ctx *c, hipStream_t s, const Step &st, const Step *prev)
+void launch_classify_stage(rtod_ctx *c, hipStream_t s, const Step &st)
 {
     if (c->dim == 1)
-        c->learned ? launch_classify_stage_dim<1, kClsLearned>(c, s, st, prev)
-                   : launch_classify_stage_dim<1, kClsUnits>(c, s, st, prev);
+        c->learned ? launch_classify_stage_dim<1, kClsLearned>(c, s, st)
+                   : launch_classify_stage_dim<1, kClsUnits>(c, s, st);
     else if (c->dim == 2)
-        c->learned ? launch_classify_stage_dim<2, kClsLearned>(c, s, st, prev)
-                   : launch_classify_stage_dim<2, kClsUnits>(c, s, st, prev);
+        c->learned ? launch_classify_stage_dim<2, kClsLearned>(c, s, st)
+                   : launch_classify_stage_dim<2, kClsUnits>(c, s, st);
     else
-        c->learned ? launch_classify_stage_dim<3, kClsLearned>(c, s, st, prev)
-                   : launch_classify_stage_dim<3, kClsUnits>(c, s, st, prev);
+        c->learned ? launch_classify_stage_dim<3, kClsLearned>(c, s, st)
+                   : launch_classify_stage_dim<3, kClsUnits>(c, s, st);
 }
 
-// ... and slide st's detect with the window merge of merge, the run merge
-// of runs and the tile sorts of ts (tiles of c->CH points: kDetectNT
-// threads with CH / kDetectNT items each)
+// ... and slide st's detect with the epilogue of prev, the window merge of
+// merge, the run merge of runs and the tile sorts of ts (tiles of c->CH
+// points: kDetectNT threads with CH / kDetectNT items each)
+struct StageParts {
+    const Step *prev, *merge, *runs, *ts;
+};
+
 template <int DIM, bool LEARN, int TSIT>
-void launch_detect_stage_t(rtod_ctx *c, hipStream_t s, const Step &st, const Step *merge,
-                           const Step *runs, const Step *ts)
+void launch_detect_stage_t(rtod_ctx *c, hipStream_t s, const Step &st, const StageParts &p)
 {
-    const int nmb = merge ? (c->W + kMergeT - 1) / kMergeT : 0;
-    const int nrm = runs ? run_merge_wgs(c->S) : 0;
-    const int nts = ts ? c->d.cpb : 0;
+    const int nfin = p.prev ? c->d.fin_parts : 0;
+    const int nmb = p.merge ? (c->W + kMergeT - 1) / kMergeT : 0;
+    const int nrm = p.runs ? run_merge_wgs(c->S) : 0;
+    const int nts = p.ts ? c->d.cpb : 0;
     const int ndet = c->detect_blocks;
-    const dim3 g(nmb + nrm + nts + ndet);
-    const size_t lds = detect_stage_lds<DIM, TSIT>(merge != nullptr, runs != nullptr,
-                                                   ts != nullptr, c->S);
-    const Step mv = merge ? *merge : Step{}, rv = runs ? *runs : Step{}, tv = ts ? *ts : Step{};
+    const dim3 g(nfin + nmb + nrm + nts + ndet);
+    const size_t lds = detect_stage_lds<DIM, TSIT>(nfin > 0, nmb > 0, nrm > 0, nts > 0, c->S);
+    const Step pv = p.prev ? *p.prev : Step{}, mv = p.merge ? *p.merge : Step{},
+               rv = p.runs ? *p.runs : Step{}, tv = p.ts ? *p.ts : Step{};
     hipLaunchKernelGGL((k_detect_stage<DIM, LEARN, TSIT>), g, dim3(kDetectNT), lds, s, c->d, st,
-                       mv, rv, tv, nmb, nrm, nts, ndet);
+                       pv, mv, rv, tv, nfin, nmb, nrm, nts, ndet);
 }
 
 template <int DIM>
-void launch_detect_stage_dim(rtod_ctx *c, hipStream_t s, const Step &st, const Step *merge,
-                             const Step *runs, const Step *ts)
+void launch_detect_stage_dim(rtod_ctx *c, hipStream_t s, const Step &st, const StageParts &p)
 {
     const bool t1 = c->CH == kDetectNT; // else CH == 2 * kDetectNT (ride_along)
     if (c->learned)
-        t1 ? launch_detect_stage_t<DIM, true, 1>(c, s, st, merge, runs, ts)
-           : launch_detect_stage_t<DIM, true, 2>(c, s, st, merge, runs, ts);
+        t1 ? launch_detect_stage_t<DIM, true, 1>(c, s, st, p)
+           : launch_detect_stage_t<DIM, true, 2>(c, s, st, p);
     else
-        t1 ? launch_detect_stage_t<DIM, false, 1>(c, s, st, merge, runs, ts)
-           : launch_detect_stage_t<DIM, false, 2>(c, s, st, merge, runs, ts);
+        t1 ? launch_detect_stage_t<DIM, false, 1>(c, s, st, p)
+           : launch_detect_stage_t<DIM, false, 2>(c, s, st, p);
 }
 
-void launch_detect_stage(rtod_ctx *c, hipStream_t s, const Step &st, const Step *merge,
-                         const Step *runs, const Step *ts)
+void launch_detect_stage(rtod_ctx *c, hipStream_t s, const Step &st, const StageParts &p)
 {
     if (c->dim == 1)
-        launch_detect_stage_dim<1>(c, s, st, merge, runs, ts);
+        launch_detect_stage_dim<1>(c, s, st, p);
     else if (c->dim == 2)
-        launch_detect_stage_dim<2>(c, s, st, merge, runs, ts);
+        launch_detect_stage_dim<2>(c, s, st, p);
     else
-        launch_detect_stage_dim<3>(c, s, st, merge, runs, ts);
+        launch_detect_stage_dim<3>(c, s, st, p);
 }
 
 // LBVH index: codes, tile sort, merges up to U = W, hierarchy, boxes
 void launch_lbvh_build(rtod_ctx *c, hipStream_t s, const Step &st)
 {
     Lbvh L = c->L;
-    L.n_cand = &c->d.ctl->cnt[st.t & 1].n_cand; // device address (not read here)
+    L.n_cand = &c->d.ctl->cnt[st.t % kSlideSets].n_cand; // device address (not read here)
     const dim3 gw((c->W + 255) / 256), b(256);
     if (c->dim == 1)
         hipLaunchKernelGGL(k_lbvh_codes<1>, gw, b, 0, s, c->d, st, L);
@@ -458,7 +458,7 @@ void launch_lbvh_build(rtod_ctx *c, hipStream_t s, const Step &st)
 void launch_lbvh_detect(rtod_ctx *c, hipStream_t s, const Step &st)
 {
     Lbvh L = c->L;
-    L.n_cand = &c->d.ctl->cnt[st.t & 1].n_cand;
+    L.n_cand = &c->d.ctl->cnt[st.t % kSlideSets].n_cand;
     const dim3 gw((c->W + 255) / 256), b(256);
     if (c->dim == 1)
         hipLaunchKernelGGL(k_lbvh_query<1>, gw, b, 0, s, c->d, st, L);
@@ -726,10 +726,10 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
     d.bm_words = (((W + 31) / 32) + 63) / 64 * 64;
     // one epilogue part per chunk of the bitmap
     d.fin_parts = std::min(32, std::max(1, (int)(((W + 31) / 32 + kChunkWords - 1) / kChunkWords)));
-    RTOD_TRY(dalloc(c, &d.bitmap, (size_t)2 * d.bm_words));
+    RTOD_TRY(dalloc(c, &d.bitmap, (size_t)kSlideSets * d.bm_words));
     RTOD_TRY(dalloc(c, &d.ctl, 1));
     RTOD_TRY(dalloc(c, &c->d_prime, dim * W));
-    RTOD_HIP(hipMemsetAsync(d.bitmap, 0, sizeof(uint32_t) * 2 * d.bm_words, c->stream));
+    RTOD_HIP(hipMemsetAsync(d.bitmap, 0, sizeof(uint32_t) * kSlideSets * d.bm_words, c->stream));
     RTOD_HIP(hipMemsetAsync(d.ctl, 0, sizeof(Ctl), c->stream));
 
     c->lbvh = cfg->index_kind == RTOD_INDEX_LBVH;
@@ -851,7 +851,7 @@ rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n,
     hipLaunchKernelGGL(k_window_samples,
                        dim3((std::max(c->d.nsamp, c->d.nsamp4) + 255) / 256), dim3(256), 0, s, c->d,
                        0);
-    RTOD_HIP(hipMemsetAsync(c->d.bitmap, 0, sizeof(uint32_t) * 2 * c->d.bm_words, s));
+    RTOD_HIP(hipMemsetAsync(c->d.bitmap, 0, sizeof(uint32_t) * kSlideSets * c->d.bm_words, s));
     RTOD_HIP(hipMemsetAsync(c->d.ctl, 0, sizeof(Ctl), s));
     if (c->learned) { // replay and per-slide counters restart; weights stay
         RTOD_HIP(hipMemsetAsync(c->d.lr.rep_n, 0, sizeof(int32_t) * 2 * kReplaySlides, s));
@@ -1195,11 +1195,13 @@ rtod_status slides_two_stage(rtod_ctx *c, const float *dev, int32_t n, int32_t *
         const Step nx3 = i + 3 < n ? step(i + 3) : Step{};
         if (timed)
             (void)hipEventRecord(c->ev[2], s);
-        launch_classify_stage(c, s, cur, i > 0 ? &prev : nullptr);
+        launch_classify_stage(c, s, cur);
         if (timed)
             (void)hipEventRecord(c->ev[3], s);
-        launch_detect_stage(c, s, cur, i + 1 < n ? &nx1 : nullptr,
-                            (i + 2 < n && runs) ? &nx2 : nullptr, i + 3 < n ? &nx3 : nullptr);
+        launch_detect_stage(c, s, cur,
+                            StageParts{i > 0 ? &prev : nullptr, i + 1 < n ? &nx1 : nullptr,
+                                       (i + 2 < n && runs) ? &nx2 : nullptr,
+                                       i + 3 < n ? &nx3 : nullptr});
         if (timed)
             (void)hipEventRecord(c->ev[4], s);
         if (i + 1 == n)

@@ -667,7 +667,7 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
         cn->n_units_i = 0;
     }
     // this slide's outlier bitmap, last read by the epilogue of the slide
-    // two earlier: cleared here, T / 32 words per workgroup
+    // kSlideSets earlier: cleared here, T / 32 words per workgroup
     {
         static_assert(T / 32 <= NT, "one word per thread");
         const int wi = bid * (T / 32) + tid;
@@ -2079,19 +2079,20 @@ __global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_detect_runs
 
 // ---------------------------------------------------------------------------
 // Two launches per slide (pipelined slides on one stream, cell-stencil
-// index, the ingest riding): the classify stage of slide t carries the
-// epilogue of slide t - 1; the detect stage of slide t carries the window
-// merge of slide t + 1, the run merge of slide t + 2 and the tile sorts of
-// slide t + 3.  The parts of a launch touch disjoint memory:
+// index, the ingest riding): the classify of slide t; then the detect
+// stage of slide t, carrying the epilogue of slide t - 1, the window merge
+// of slide t + 1, the run merge of slide t + 2 and the tile sorts of slide
+// t + 3.  The parts of a launch touch disjoint memory (counters and bitmaps
+// come in kSlideSets = 3 sets, by slide modulo 3):
 //   classify(t): reads window t; writes the candidate and unit lists and
-//     counters cnt[t & 1];
-//   epilogue(t - 1): reads bitmap (t - 1) & 1 and cnt[(t - 1) & 1]; writes
+//     counters cnt[t % 3];
+//   epilogue(t - 1): reads bitmap (t - 1) % 3 and cnt[(t - 1) % 3]; writes
 //     host-out buffer t - 1;
-//   detect(t): reads window t, the lists and cnt[t & 1]; sets bitmap t & 1;
+//   detect(t): reads window t, the lists and cnt[t % 3]; sets bitmap t % 3;
 //   merge(t + 1): reads window t and batch-ring entries enew(t + 1),
 //     eold(t + 1); writes window t + 1 (the other buffers); resets
-//     cnt[(t + 1) & 1] and clears bitmap (t + 1) & 1 (both last read by
-//     epilogue(t - 1), a launch earlier);
+//     cnt[(t + 1) % 3] and clears bitmap (t + 1) % 3 (last read by
+//     epilogue(t - 2), two launches earlier);
 //   run merge(t + 2): reads tile scratch (t + 2) & 1; writes ring entry
 //     enew(t + 2);
 //   tile sorts(t + 3): write tile scratch (t + 3) & 1 (one tile: ring entry
@@ -2100,26 +2101,20 @@ __global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_detect_runs
 // enew(t + 3) are t - 2, t + 1, t - 1 and t modulo m + 3: all distinct.
 // ---------------------------------------------------------------------------
 template <int DIM, int MODE>
-__global__ __launch_bounds__(kClassifyNT) void k_classify_stage(Dev d, Step st, Step st_prev,
-                                                               int nfin, int ncls)
+__global__ __launch_bounds__(kClassifyNT) void k_classify_stage(Dev d, Step st)
 {
-    static_assert(kClassifyNT == 256, "the epilogue parts take 256 threads");
-    __shared__ union {
-        ClassifyLds<MODE == kClsLearned> cls;
-        FinishLds fin;
-    } lds;
-    const int bid = blockIdx.x;
-    if (bid < nfin)
-        finish_deferred<256>(d, st_prev, bid, nfin, lds.fin);
-    else if (bid - nfin < ncls)
-        classify_body<DIM, MODE>(d, st, bid - nfin, lds.cls);
+    __shared__ ClassifyLds<MODE == kClsLearned> lds;
+    classify_body<DIM, MODE>(d, st, blockIdx.x, lds);
 }
 
 // LDS of k_detect_stage, set at launch: the largest of its parts'
 template <int DIM, int TSIT>
-__host__ __device__ constexpr size_t detect_stage_lds(bool merge, bool runs, bool ts, int S)
+__host__ __device__ constexpr size_t detect_stage_lds(bool fin, bool merge, bool runs, bool ts,
+                                                      int S)
 {
     size_t b = sizeof(DetectLds);
+    if (fin && sizeof(FinishLds) > b)
+        b = sizeof(FinishLds);
     if (merge && sizeof(MergeWindowLds<kDetectNT>) > b)
         b = sizeof(MergeWindowLds<kDetectNT>);
     if (runs && (size_t)S * sizeof(uint64_t) > b)
@@ -2129,15 +2124,22 @@ __host__ __device__ constexpr size_t detect_stage_lds(bool merge, bool runs, boo
     return b;
 }
 
-// workgroups: nmb window-merge (slide st_merge), nrm run-merge (slide
-// st_runs), nts tile-sort (slide st_ts, tiles of kDetectNT * TSIT points),
-// then the ndet of the detect (slide st), whose busy ones come first
+// workgroups: nfin epilogue parts (slide st_prev), nmb window-merge (slide
+// st_merge), nrm run-merge (slide st_runs), nts tile-sort (slide st_ts,
+// tiles of kDetectNT * TSIT points), then the ndet of the detect (slide
+// st), whose busy ones come first
 template <int DIM, bool LEARN, int TSIT>
 __global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_detect_stage(
-    Dev d, Step st, Step st_merge, Step st_runs, Step st_ts, int nmb, int nrm, int nts, int ndet)
+    Dev d, Step st, Step st_prev, Step st_merge, Step st_runs, Step st_ts, int nfin, int nmb,
+    int nrm, int nts, int ndet)
 {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     int bid = blockIdx.x;
+    if (bid < nfin) {
+        finish_deferred<kDetectNT>(d, st_prev, bid, nfin, *reinterpret_cast<FinishLds *>(dyn_lds));
+        return;
+    }
+    bid -= nfin;
     if (bid < nmb) {
         merge_window_body<DIM, kDetectNT>(d, st_merge, bid,
                                           *reinterpret_cast<MergeWindowLds<kDetectNT> *>(dyn_lds));

@@ -37,9 +37,10 @@
 
 namespace rtod {
 
-// A slide's classify counters.  There are two sets, by slide parity: the
-// classify of slide t may run beside the epilogue of slide t - 1, which
-// reads that slide's.
+// A slide's classify counters.  There are three sets, by slide modulo 3
+// (kSlideSets): while slide t is classified and detected, the epilogue of
+// slide t - 1 may still read its set, and the window merge of slide t + 1
+// (resetting the next set) may run beside both.
 struct SlideCnt {
     int32_t n_cand;        // candidates
     int32_t n_units;       // cell units (learned: the probable-outlier bucket)
@@ -56,7 +57,7 @@ struct Ctl {
     unsigned long long pair_tests;
     int32_t err;
     int32_t pad;
-    SlideCnt cnt[2];       // slide t's at cnt[t & 1]; reset by its window merge
+    SlideCnt cnt[3];       // slide t's at cnt[t % 3]; reset by its window merge
 };
 
 struct HostOut {
@@ -147,7 +148,7 @@ struct Dev {
     int32_t *cand;
     int2 *units;        // cell units: {start, len}
     int32_t *cand_cnt;  // per-unit minimum count (capped)
-    uint32_t *bitmap;   // [2][bm_words]: outlier bits of slide t at (t & 1)
+    uint32_t *bitmap;   // [3][bm_words]: outlier bits of slide t at (t % 3)
     int bm_words;       // ceil(W / 32), rounded up to 64
     int fin_parts;      // parts of a deferred slide epilogue
     Ctl *ctl;
@@ -163,13 +164,18 @@ struct Dev {
 constexpr int kHostOuts = 4;
 
 // the classify counters of slide t
-__device__ __forceinline__ SlideCnt *cnt_of(const Dev &d, uint64_t t) { return &d.ctl->cnt[t & 1]; }
+constexpr int kSlideSets = 3;
+__device__ __forceinline__ SlideCnt *cnt_of(const Dev &d, uint64_t t)
+{
+    return &d.ctl->cnt[t % kSlideSets];
+}
 
-// the outlier bitmap of slide t (two, by slide parity: the epilogue of slide
-// t may still read its bits while slide t + 1 is being detected)
+// the outlier bitmap of slide t (kSlideSets of them: the epilogue of slide t
+// may still read its bits while slide t + 1 is detected and the bitmap of
+// slide t + 2 is cleared by its window merge)
 __device__ __forceinline__ uint32_t *bitmap_of(const Dev &d, uint64_t t)
 {
-    return d.bitmap + (size_t)(t & 1) * (size_t)d.bm_words;
+    return d.bitmap + (size_t)(t % kSlideSets) * (size_t)d.bm_words;
 }
 __device__ __forceinline__ HostOut *host_out(const Dev &d, uint64_t t)
 {
