@@ -9,6 +9,7 @@ import pytest
 
 import oracle as O
 import rtod_amd as R
+from rtod_amd.synth import calibrate_R
 
 pytestmark = pytest.mark.gpu
 
@@ -225,9 +226,10 @@ PIPE_CASES = [
 
 @pytest.mark.parametrize("name,wl,T,kw", PIPE_CASES, ids=[c[0] for c in PIPE_CASES])
 def test_pipelined_slides_vs_grid(name, wl, T, kw):
-    """rtod_slides_device (the next slide's ingest on a second stream, the
-    host one slide ahead) against the grid oracle, mixed with single slides
-    on the same context: 3 single, T-8 pipelined, 2 single, 3 pipelined."""
+    """rtod_slides_device (two launches per slide on one stream, the slides'
+    sorts, merges and lists riding in neighbouring slides' launches) against
+    the grid oracle, mixed with single slides on the same context: 3 single,
+    T-8 pipelined, 2 single, 3 pipelined."""
     case(f"pipelined {name}")
     xs = wl.data(wl.W + T * wl.S)
     want = O.run_stream(xs, wl.W, wl.S, T, wl.R, wl.K, "grid")
@@ -254,6 +256,33 @@ def test_pipelined_slides_vs_grid(name, wl, T, kw):
         piped(3)
         assert det.stats()["slides"] == T
     assert_same(got, want, f"pipelined {name}")
+
+
+@pytest.mark.parametrize("dim,W,S,K,kind", [
+    (1, 20000, 2500, 20, "walk"),   # three sort tiles: run merge two slides ahead
+    (3, 8000, 400, 20, "gmm"),      # one sort tile: sorted straight into the ring
+    (2, 6000, 1500, 20, "walk"),    # two tiles
+])
+def test_pipelined_short_runs(dim, W, S, K, kind):
+    """Pipelined calls of 1, 2, 3, 1, 4 and 5 slides in a row (every length
+    of prologue and tail: the sorts and merges run up to three slides ahead
+    of the detect, the list one slide behind) equal the grid oracle."""
+    case(f"pipelined short runs dim={dim} W={W} S={S}")
+    runs = [1, 2, 3, 1, 4, 5]
+    T = sum(runs)
+    xs = O.gen_walk(90 + dim, W + T * S, dim) if kind == "walk" else \
+        O.gen_gmm(90 + dim, W + T * S, dim)
+    Rr = calibrate_R(xs[:W], K, seed=dim)
+    want = O.run_stream(xs, W, S, T, Rr, K, "grid")
+    got = []
+    with R.DeviceStream(xs) as ds, R.Detector(dim, W, S, Rr, K) as det:
+        det.prime(xs[:W])
+        t = 0
+        for n in runs:
+            got.extend(det.slides_device(ds.at(W + t * S), n))
+            t += n
+        assert det.stats()["slides"] == T
+    assert_same(got, want, f"short runs dim={dim}")
 
 
 def test_pipelined_slides_argument_checks():
