@@ -15,7 +15,8 @@
 //   k_lbvh_query   every window point walks the tree (LDS stack); at a leaf:
 //                  early-out if the count already exceeds K, exact FP32 test,
 //                  atomicAdd -- counts stay exact up to K+1
-//   k_lbvh_select  count <= K -> outlier bit; last workgroup compacts
+//   k_lbvh_select  count <= K -> outlier bit (the slide epilogue, k_finish,
+//                  lists them)
 // The number of candidates U lives on the device; kernels are sized for W
 // and exit past U.
 #pragma once

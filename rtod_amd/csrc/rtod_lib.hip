@@ -705,8 +705,9 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
     }
     d.nent = (int)m + 3;
     // single-stream slides sorted as several tiles and joined by the run
-    // merge keep two slides' tiles (pipelined slides sort them two launches
-    // ahead of the merge); a batch's streams merge tiles in passes
+    // merge keep two slides' tiles (pipelined slides sort a slide's tiles
+    // one launch before its run merge, beside the previous slide's run
+    // merge); a batch's streams merge tiles in passes
     d.tile_par = (!opt.batch_member && use_run_merge(c->S, c->CH)) ? 1 : 0;
     const size_t ne = (size_t)d.nent;
     RTOD_TRY(dalloc(c, &d.bkey, ne * S));

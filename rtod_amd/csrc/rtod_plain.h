@@ -1870,7 +1870,7 @@ __device__ __forceinline__ void finish_deferred(const Dev &d, const Step &st, in
 // ---------------------------------------------------------------------------
 struct DetectLds {
     static constexpr int NW = kDetectNT / kWave;
-    union { // the sample table, then (last workgroup) the epilogue's
+    union { // the sample table, then (a batch's last workgroup) the epilogue's
         uint64_t samp[kMaxSamples];
         FinishLds fin;
     };
@@ -2009,15 +2009,16 @@ __global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_detect(Dev 
 }
 
 // ---------------------------------------------------------------------------
-// Core-stage launches carrying work of the neighbouring slides in extra
-// workgroups (pipelined slides on one stream): slide t's window merge with
-// slide t-1's epilogue and slide t+1's tile sorts, slide t's detect with
-// slide t+1's run merge.  The parts touch disjoint memory: the ingest writes
-// batch-ring entry enew(t+1) and the tile scratch; slide t's merge reads
-// entries enew(t) and eold(t) (distinct from enew(t+1) with m + 2 entries)
-// and writes the other window buffer; slide t-1's epilogue reads and clears
-// the outlier bitmap and the slide counters (slide t's classify, the next
-// launch, is their first user) and writes host-out buffer t-1.
+// Three launches per slide, for pipelined slides that cannot take two (the
+// window merge of each slide on its own launch): slide t's window merge
+// carries slide t-1's epilogue and slide t+1's tile sorts, slide t's detect
+// slide t+1's run merge; it is also the prologue of the two-launch schedule.
+// The parts touch disjoint memory: the ingest writes batch-ring entry
+// enew(t+1) and the tile scratch of slide t+1; slide t's merge reads entries
+// enew(t) and eold(t) (distinct from enew(t+1) with m + 3 entries), writes
+// the other window buffer and resets set t % 3 of the counters and
+// bitmaps; slide t-1's epilogue reads set (t-1) % 3 and writes host-out
+// buffer t-1.
 // ---------------------------------------------------------------------------
 // fin_prev > 0: workgroups [0, fin_prev) run slide st_prev's epilogue, in
 // fin_prev parts; then nmb merge workgroups, then the tile sorts of st_next

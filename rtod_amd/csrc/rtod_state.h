@@ -8,18 +8,24 @@
 //                               cell-stencil index; ping-pong by slide parity
 //   apos [2][W]                 position of each sorted-window point inside
 //                               its own sorted batch
-//   bkey/bslot [(m+1)*S]        each batch of S arrivals sorted by cell key,
-//                               kept until it expires m slides later
-//   bxyz f32 [d][(m+1)*S]       coordinates of each sorted batch (SoA)
-//   bsamp [(m+1)*nbs]           every 2^bs_log2-th key of each sorted batch
-//   tkey/tslot/txyz [2]         scratch runs for chunked sorts (max(W,S))
+//   bkey/bslot [(m+3)*S]        each batch of S arrivals sorted by cell key,
+//                               kept until it expires m slides later (three
+//                               extra entries: later slides' batches are
+//                               sorted while this slide's merge reads two)
+//   bxyz f32 [d][(m+3)*S]       coordinates of each sorted batch (SoA)
+//   bsamp [(m+3)*nbs]           every 2^bs_log2-th key of each sorted batch
+//   tkey/tslot/txyz [2]         scratch runs for chunked sorts (max(W,2S):
+//                               two slides' tiles at (t & 1) * S)
 //   samples  u64 [2][W/stride]  every stride-th key of each sorted window
+//   samples4 u64 [2][W/4]       every 4th key (when the stride exceeds 4)
 //   cand     i32 [W]            candidate positions (sorted-window index; the
 //                               LBVH index's per-point list)
-//   units    i32x2 [W]          cell units (start, len <= 64): runs of one
+//   units    i32x2 [W]          cell units (start, len <= 16): runs of one
 //                               sparse cell's candidates in the sorted window
-//   bitmap   u32 [W/32]         outlier flags by ring slot
-//   ctl                         slide counter, candidate count, grid ticket
+//   bitmap   u32 [3][W/32]      outlier flags by ring slot, one set per
+//                               slide modulo 3
+//   ctl                         slide counters, tickets, pair tests, error
+//                               flags, classify counters (three sets)
 // Learned scheduler (only when configured):
 //   params/adam_m/adam_v f32 [1313]   predictor MLP, FP32 master + Adam moments
 //   pub f32 [2][1313]           published weights (classify reads pub[lc.pub&1])
@@ -30,7 +36,8 @@
 //   rep_n i32 [20][2]           replay: bucket sizes of the stored slide
 // Pinned host memory, device-mapped:
 //   h_in  f32 [2][S*d]          new slide staging (row-major, as given)
-//   h_out                       outlier count + ascending slot list + counters
+//   h_out [4]                   outlier count + ascending slot list +
+//                               counters + stamp; slide t uses t mod 4
 #pragma once
 
 #include <stdint.h>
