@@ -578,12 +578,6 @@ __global__ __launch_bounds__(kTrainNT) void k_train(Dev d, uint64_t t_next, int 
     });
 }
 
-// the weights classify uses: the published half of the double buffer
-__device__ __forceinline__ const float *current_weights(const Learn &L)
-{
-    const int cur = __hip_atomic_load(&L.ls->pub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    return L.pub + (size_t)(cur & 1) * kMlpNP;
-}
 
 // ---------------------------------------------------------------------------
 // Test entry kernels: the same device code on caller-given inputs.

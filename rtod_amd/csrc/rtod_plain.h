@@ -969,11 +969,21 @@ __device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int 
         if (wv == 0) { // all of a lane's loads in one round trip
             constexpr int PER = (kMlpNP + kWave - 1) / kWave;
             const int lane = threadIdx.x;
-            const float *P = current_weights(d.lr);
+            // the published set, read without an acquire (which would
+            // invalidate this XCD's L2 under the other waves' tile loads):
+            // its index and words are read at agent scope instead.  The
+            // weights only order the exact counts, so a set read while the
+            // trainer publishes the next one cannot change an outcome.
+            const int cur = __hip_atomic_load(&d.lr.ls->pub, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+            const float *P = d.lr.pub + (size_t)(cur & 1) * kMlpNP;
             float v[PER];
 #pragma unroll
             for (int u = 0; u < PER; ++u)
-                v[u] = lane + u * kWave < kMlpNP ? P[lane + u * kWave] : 0.f;
+                v[u] = lane + u * kWave < kMlpNP
+                           ? __hip_atomic_load(&P[lane + u * kWave], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT)
+                           : 0.f;
 #pragma unroll
             for (int u = 0; u < PER; ++u)
                 if (lane + u * kWave < kMlpNP)
