@@ -48,11 +48,13 @@ namespace rtod {
 // (kSlideSets): while slide t is classified and detected, the epilogue of
 // slide t - 1 may still read its set, and the window merge of slide t + 1
 // (resetting the next set) may run beside both.
+// Each counter sits on a cache line of its own: every classify workgroup
+// adds to each with one device-scope atomic, and atomics on one line are
+// performed one after another.
 struct SlideCnt {
-    int32_t n_cand;        // candidates
-    int32_t n_units;       // cell units (learned: the probable-outlier bucket)
-    int32_t n_units_i;     // learned: the probable-inlier bucket
-    int32_t pad;
+    alignas(128) int32_t n_cand; // candidates
+    alignas(128) int32_t n_units; // cell units (learned: the probable-outlier bucket)
+    alignas(128) int32_t n_units_i; // learned: the probable-inlier bucket
 };
 
 struct Ctl {
