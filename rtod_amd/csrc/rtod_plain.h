@@ -922,8 +922,8 @@ __global__ __launch_bounds__(256) void k_merge_window(Dev d, Step st)
 constexpr int kClassifyNT = 256;
 constexpr int kClassifyHalo = 512;
 enum ClassifyMode { kClsPoints = 0, kClsUnits = 1, kClsLearned = 2 };
-constexpr int kUnitMax = 16; // candidates per cell unit (a power of two)
-static_assert(kUnitMax == 16, "unit reductions span lanes 0..15");
+constexpr int kUnitMax = 8; // candidates per cell unit (a power of two)
+static_assert(kUnitMax <= 16, "unit reductions span lanes 0..15");
 
 template <bool MLP> struct ClassifyLds {
     static constexpr int NW = kClassifyNT / kWave;
