@@ -97,8 +97,9 @@ typedef struct rtod_stats {
     int64_t device_bytes;       /* device memory owned by the context      */
     int64_t host_pinned_bytes;  /* pinned host memory owned by the context */
     int64_t predictor_misses;   /* learned scheduler: mispredicted cell
-                                   units (a unit: <= 16 candidates of one
-                                   sparse cell, counted together)         */
+                                   units (a unit: <= 4 candidates of one
+                                   sparse cell in 1-D, <= 8 in 2-D/3-D,
+                                   counted together)                      */
     int64_t predictor_points;   /* learned scheduler: predicted units      */
     int64_t fallbacks;          /* slides run on the plain schedule because
                                    the previous slide's miss rate exceeded

@@ -906,24 +906,31 @@ __global__ __launch_bounds__(256) void k_merge_window(Dev d, Step st)
 // sparse cells -- are the candidates.
 //   kClsPoints  appends every candidate to the per-point list (LBVH index).
 //   kClsUnits   appends cell units: a candidate starts a unit where its cell
-//               run starts or at a multiple of kUnitMax; the unit runs to the
-//               end of the cell or of that aligned block, so a unit is at
-//               most kUnitMax candidates sharing one cell.  One atomic per
-//               wavefront.
+//               run starts or at a multiple of unit_max<DIM>; the unit runs to
+//               the end of the cell or of that aligned block, so a unit is at
+//               most unit_max<DIM> candidates sharing one cell.  One atomic
+//               per counter per workgroup.
 //   kClsLearned kClsUnits plus, per unit, the predictor's features (own-cell
 //               count, the two neighbouring cells' counts along the last
 //               axis, the unit's first point's offsets from the cell centre)
 //               and its prediction on the matrix cores: probable-outlier
-//               units are filed at the front of the list (ctl->n_units),
+//               units are filed at the front of the list (n_units),
 //               probable-inlier units at the back (index W-1-k,
-//               lc->n_units_i); features go to the slide's replay slot under
+//               n_units_i); features go to the slide's replay slot under
 //               the unit's list index.
 // ---------------------------------------------------------------------------
 constexpr int kClassifyNT = 256;
 constexpr int kClassifyHalo = 512;
 enum ClassifyMode { kClsPoints = 0, kClsUnits = 1, kClsLearned = 2 };
-constexpr int kUnitMax = 8; // candidates per cell unit (a power of two)
-static_assert(kUnitMax <= 16, "unit reductions span lanes 0..15");
+// Candidates per cell unit (a power of two, at most 16: the unit reductions
+// span lanes 0..15).  Smaller units put more wavefronts on a slide's
+// candidates; a unit's stencil rows are shared by fewer candidates.
+// Measured per slide: 1-D best at 4 (walk W=100k: 17.5 us at 16, 16.3 at 8,
+// 16.0 at 4), 3-D at 8 (gmm W=10k: 29.2, 26.3, 27.5; W=100k: 128.0, 126.2,
+// 145.8).
+template <int DIM> constexpr int unit_max() { return DIM == 1 ? 4 : 8; }
+static_assert(unit_max<1>() <= 16 && unit_max<2>() <= 16 && unit_max<3>() <= 16,
+              "unit reductions span lanes 0..15");
 
 template <bool MLP> struct ClassifyLds {
     static constexpr int NW = kClassifyNT / kWave;
@@ -951,6 +958,7 @@ __device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int 
     const int W = d.W, K = d.K;
     // keys read: [i-K, i] for the run start, [i, i+63] for a unit's end;
     // learned features reach 2K+2 either side
+    constexpr int kUnitMax = unit_max<DIM>();
     const int H = LEARN ? max(2 * K + 2, kUnitMax) : max(K, kUnitMax);
     const bool tiled = H <= HALO;
     const int t0 = max(0, i0 - H), t1 = min(W, i0 + NT + H);
@@ -1343,7 +1351,7 @@ __device__ __forceinline__ float lane_f(float v, int l)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-// Counts of a cell unit's candidates (<= kUnitMax) over the window points in the cells of
+// Counts of a cell unit's candidates (<= unit_max<DIM>) over the window points in the cells of
 // stencil `st` (a superset of each candidate's own stencil; the predicate
 // decides).  Lane c < n holds candidate c's coordinates in pc and receives
 // its count, exact up to K+1 at least.  Chunks of 64 stencil points stream

@@ -20,7 +20,7 @@
 //   samples4 u64 [2][W/4]       every 4th key (when the stride exceeds 4)
 //   cand     i32 [W]            candidate positions (sorted-window index; the
 //                               LBVH index's per-point list)
-//   units    i32x2 [W]          cell units (start, len <= 16): runs of one
+//   units    i32x2 [W]          cell units (start, len <= 4 in 1-D, 8 else): runs of one
 //                               sparse cell's candidates in the sorted window
 //   bitmap   u32 [3][W/32]      outlier flags by ring slot, one set per
 //                               slide modulo 3
