@@ -709,6 +709,7 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
     // one launch before its run merge, beside the previous slide's run
     // merge); a batch's streams merge tiles in passes
     d.tile_par = (!opt.batch_member && use_run_merge(c->S, c->CH)) ? 1 : 0;
+    d.unit_max = opt.batch_member ? kUnitCap : unit_max_single(c->dim);
     const size_t ne = (size_t)d.nent;
     RTOD_TRY(dalloc(c, &d.bkey, ne * S));
     RTOD_TRY(dalloc(c, &d.bslot, ne * S));

@@ -906,9 +906,9 @@ __global__ __launch_bounds__(256) void k_merge_window(Dev d, Step st)
 // sparse cells -- are the candidates.
 //   kClsPoints  appends every candidate to the per-point list (LBVH index).
 //   kClsUnits   appends cell units: a candidate starts a unit where its cell
-//               run starts or at a multiple of unit_max<DIM>; the unit runs to
+//               run starts or at a multiple of d.unit_max; the unit runs to
 //               the end of the cell or of that aligned block, so a unit is at
-//               most unit_max<DIM> candidates sharing one cell.  One atomic
+//               most d.unit_max candidates sharing one cell.  One atomic
 //               per counter per workgroup.
 //   kClsLearned kClsUnits plus, per unit, the predictor's features (own-cell
 //               count, the two neighbouring cells' counts along the last
@@ -928,9 +928,10 @@ enum ClassifyMode { kClsPoints = 0, kClsUnits = 1, kClsLearned = 2 };
 // Measured per slide: 1-D best at 4 (walk W=100k: 17.5 us at 16, 16.3 at 8,
 // 16.0 at 4), 3-D at 8 (gmm W=10k: 29.2, 26.3, 27.5; W=100k: 128.0, 126.2,
 // 145.8).
-template <int DIM> constexpr int unit_max() { return DIM == 1 ? 4 : 8; }
-static_assert(unit_max<1>() <= 16 && unit_max<2>() <= 16 && unit_max<3>() <= 16,
-              "unit reductions span lanes 0..15");
+// A batch's streams, whose detect is throughput-bound (many units per
+// wavefront), keep units of 16: smaller ones only add per-unit work there.
+constexpr int kUnitCap = 16; // the unit reductions span lanes 0..15
+constexpr int unit_max_single(int dim) { return dim == 1 ? 4 : 8; }
 
 template <bool MLP> struct ClassifyLds {
     static constexpr int NW = kClassifyNT / kWave;
@@ -958,8 +959,8 @@ __device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int 
     const int W = d.W, K = d.K;
     // keys read: [i-K, i] for the run start, [i, i+63] for a unit's end;
     // learned features reach 2K+2 either side
-    constexpr int kUnitMax = unit_max<DIM>();
-    const int H = LEARN ? max(2 * K + 2, kUnitMax) : max(K, kUnitMax);
+    const int kUnitMax = d.unit_max;
+    const int H = LEARN ? max(2 * K + 2, kUnitCap) : max(K, kUnitCap);
     const bool tiled = H <= HALO;
     const int t0 = max(0, i0 - H), t1 = min(W, i0 + NT + H);
     // learned: this point's coordinates are requested first, and wave 0
@@ -1351,7 +1352,7 @@ __device__ __forceinline__ float lane_f(float v, int l)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-// Counts of a cell unit's candidates (<= unit_max<DIM>) over the window points in the cells of
+// Counts of a cell unit's candidates (<= d.unit_max) over the window points in the cells of
 // stencil `st` (a superset of each candidate's own stencil; the predicate
 // decides).  Lane c < n holds candidate c's coordinates in pc and receives
 // its count, exact up to K+1 at least.  Chunks of 64 stencil points stream

@@ -127,6 +127,7 @@ struct Dev {
     int dim, W, S, m, K;
     int nent;         // batch-ring entries, m + 3: the batches of the next three
                       // slides can be ingested while this slide's merge reads its two
+    int unit_max;     // candidates per cell unit (a power of two, <= kUnitCap)
     int tile_par;     // slide batches sorted as several tiles go to the tile
                       // scratch at offset (t & 1) * S (two slides' tiles live)
     int cpb;          // sort chunks per batch
