@@ -55,15 +55,17 @@ enum KernelId {
     kKLbvhQuery,
     kKMergeWindowIngest, // pipelined: window merge + the next slide's tile sorts
     kKDetectRuns,        // pipelined: detect + the next slide's run merge
-    kKClassifyStage,     // two launches per slide: classify + epilogue + tile sorts
-    kKDetectStage,       // two launches per slide: detect + window merge + run merge
+    kKClassifyStage,     // two launches per slide (learned): classify
+    kKDetectStage,       // two launches per slide (learned): detect + epilogue +
+                         // window merge + run merge + tile sorts
+    kKSlideStage,        // one launch per slide (plain): classify + the above
     kKCount
 };
 const char *kKernelNames[kKCount] = {"ingest_sort",         "merge_pass",  "merge_window",
                                      "classify",            "detect",      "lbvh_build",
                                      "lbvh_query",          "merge_window_ingest",
                                      "detect_runs",         "classify_stage",
-                                     "detect_stage"};
+                                     "detect_stage",        "slide_stage"};
 
 // LDS-sort tile shapes (threads x items per thread).  A batch of up to
 // kRunMergeCap points is sorted as tiles of 1024 points on as many
@@ -418,6 +420,36 @@ void launch_detect_stage_dim(rtod_ctx *c, hipStream_t s, const Step &st, const S
            : launch_detect_stage_t<DIM, false, 2>(c, s, st, p);
 }
 
+// one launch per slide (plain schedule): slide st's classify in front of
+// the detect stage's parts
+template <int DIM, int TSIT>
+void launch_slide_stage_t(rtod_ctx *c, hipStream_t s, const Step &st, const StageParts &p)
+{
+    const int ncls = (c->W + kDetectNT - 1) / kDetectNT;
+    const int nfin = p.prev ? c->d.fin_parts : 0;
+    const int nmb = p.merge ? (c->W + kMergeT - 1) / kMergeT : 0;
+    const int nrm = p.runs ? run_merge_wgs(c->S) : 0;
+    const int nts = p.ts ? c->d.cpb : 0;
+    const int ndet = c->detect_blocks;
+    const dim3 g(ncls + nfin + nmb + nrm + nts + ndet);
+    const size_t lds = slide_stage_lds<DIM, TSIT>(nfin > 0, nmb > 0, nrm > 0, nts > 0, c->S);
+    const Step pv = p.prev ? *p.prev : Step{}, mv = p.merge ? *p.merge : Step{},
+               rv = p.runs ? *p.runs : Step{}, tv = p.ts ? *p.ts : Step{};
+    hipLaunchKernelGGL((k_slide_stage<DIM, TSIT>), g, dim3(kDetectNT), lds, s, c->d, st, pv, mv,
+                       rv, tv, ncls, nfin, nmb, nrm, nts, ndet);
+}
+
+void launch_slide_stage(rtod_ctx *c, hipStream_t s, const Step &st, const StageParts &p)
+{
+    const bool t1 = c->CH == kDetectNT; // else CH == 2 * kDetectNT (ride_along)
+    if (c->dim == 1)
+        t1 ? launch_slide_stage_t<1, 1>(c, s, st, p) : launch_slide_stage_t<1, 2>(c, s, st, p);
+    else if (c->dim == 2)
+        t1 ? launch_slide_stage_t<2, 1>(c, s, st, p) : launch_slide_stage_t<2, 2>(c, s, st, p);
+    else
+        t1 ? launch_slide_stage_t<3, 1>(c, s, st, p) : launch_slide_stage_t<3, 2>(c, s, st, p);
+}
+
 void launch_detect_stage(rtod_ctx *c, hipStream_t s, const Step &st, const StageParts &p)
 {
     if (c->dim == 1)
@@ -732,10 +764,16 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
     // one epilogue part per chunk of the bitmap
     d.fin_parts = std::min(32, std::max(1, (int)(((W + 31) / 32 + kChunkWords - 1) / kChunkWords)));
     RTOD_TRY(dalloc(c, &d.bitmap, (size_t)kSlideSets * d.bm_words));
+    if (!opt.batch_member) // a batch's streams never run one launch per slide
+        RTOD_TRY(dalloc(c, &d.ready, (size_t)kSlideSets * kReadyFlags * kReadyStride));
     RTOD_TRY(dalloc(c, &d.ctl, 1));
     RTOD_TRY(dalloc(c, &c->d_prime, dim * W));
     RTOD_HIP(hipMemsetAsync(d.bitmap, 0, sizeof(uint32_t) * kSlideSets * d.bm_words, c->stream));
     RTOD_HIP(hipMemsetAsync(d.ctl, 0, sizeof(Ctl), c->stream));
+    if (d.ready)
+        RTOD_HIP(hipMemsetAsync(d.ready, 0,
+                                sizeof(int32_t) * kSlideSets * kReadyFlags * kReadyStride,
+                                c->stream));
 
     c->lbvh = cfg->index_kind == RTOD_INDEX_LBVH;
     if (c->lbvh) {
@@ -941,8 +979,15 @@ void accumulate_core_times(rtod_ctx *c, bool fused)
 }
 
 // event pair [2, 3]: the classify stage; [3, 4]: the detect stage
-void accumulate_stage_times(rtod_ctx *c)
+void accumulate_stage_times(rtod_ctx *c, bool fused)
 {
+    if (fused) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
+        c->ksum[kKSlideStage] += ms;
+        c->kcnt[kKSlideStage]++;
+        return;
+    }
     float ms[2];
     for (int i = 0; i < 2; ++i) {
         ms[i] = 0.f;
@@ -1155,6 +1200,7 @@ rtod_status slides_two_stage(rtod_ctx *c, const float *dev, int32_t n, int32_t *
     const size_t per = (size_t)c->S * c->dim;
     const bool timed = (c->cfg.flags & RTOD_FLAG_PHASE_TIMES) != 0;
     const bool runs = c->d.cpb > 1;
+    const bool fused = !c->learned && c->d.ready; // plain: one launch per slide
     hipStream_t s = c->stream;
     auto step = [&](int32_t i) {
         return make_step_at(c, (uint64_t)(t0 + i), dev + (size_t)i * per);
@@ -1198,17 +1244,20 @@ rtod_status slides_two_stage(rtod_ctx *c, const float *dev, int32_t n, int32_t *
         const Step nx1 = i + 1 < n ? step(i + 1) : Step{};
         const Step nx2 = i + 2 < n ? step(i + 2) : Step{};
         const Step nx3 = i + 3 < n ? step(i + 3) : Step{};
+        const StageParts parts{i > 0 ? &prev : nullptr, i + 1 < n ? &nx1 : nullptr,
+                               (i + 2 < n && runs) ? &nx2 : nullptr, i + 3 < n ? &nx3 : nullptr};
         if (timed)
             (void)hipEventRecord(c->ev[2], s);
-        launch_classify_stage(c, s, cur);
+        if (fused) {
+            launch_slide_stage(c, s, cur, parts);
+        } else {
+            launch_classify_stage(c, s, cur);
+            if (timed)
+                (void)hipEventRecord(c->ev[3], s);
+            launch_detect_stage(c, s, cur, parts);
+        }
         if (timed)
-            (void)hipEventRecord(c->ev[3], s);
-        launch_detect_stage(c, s, cur,
-                            StageParts{i > 0 ? &prev : nullptr, i + 1 < n ? &nx1 : nullptr,
-                                       (i + 2 < n && runs) ? &nx2 : nullptr,
-                                       i + 3 < n ? &nx3 : nullptr});
-        if (timed)
-            (void)hipEventRecord(c->ev[4], s);
+            (void)hipEventRecord(c->ev[fused ? 3 : 4], s);
         if (i + 1 == n)
             launch_finish(c, s, cur);
         launched = i + 1;
@@ -1222,7 +1271,7 @@ rtod_status slides_two_stage(rtod_ctx *c, const float *dev, int32_t n, int32_t *
                 set_error(std::string("HIP error: ") + hipGetErrorString(q));
                 st = RTOD_ERR_HIP;
             } else {
-                accumulate_stage_times(c);
+                accumulate_stage_times(c, fused);
             }
         }
     }

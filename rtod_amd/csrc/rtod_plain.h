@@ -665,7 +665,10 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
         cn->n_cand = 0;
         cn->n_units = 0;
         cn->n_units_i = 0;
+        cn->n_done = 0;
     }
+    if (d.ready && bid == 0 && tid < kReadyFlags)
+        *ready_of(d, st.t, tid) = 0;
     // this slide's outlier bitmap, last read by the epilogue of the slide
     // kSlideSets earlier: cleared here, T / 32 words per workgroup
     {
@@ -933,9 +936,9 @@ enum ClassifyMode { kClsPoints = 0, kClsUnits = 1, kClsLearned = 2 };
 constexpr int kUnitCap = 16; // the unit reductions span lanes 0..15
 constexpr int unit_max_single(int dim) { return dim == 1 ? 4 : 8; }
 
-template <bool MLP> struct ClassifyLds {
-    static constexpr int NW = kClassifyNT / kWave;
-    uint64_t t[kClassifyNT + 2 * kClassifyHalo];
+template <bool MLP, int NT = kClassifyNT> struct ClassifyLds {
+    static constexpr int NW = NT / kWave;
+    uint64_t t[NT + 2 * kClassifyHalo];
     int wc[NW][3]; // per wave: candidates, units (front list), units (back list)
     int base[3];
     alignas(16) __bf16 h[MLP ? NW : 1][MLP ? kWave * kMlpH : 1];
@@ -944,11 +947,14 @@ template <bool MLP> struct ClassifyLds {
     float w[MLP ? kMlpNP : 1]; // the predictor's weights
 };
 
-template <int DIM, int MODE>
+// SIGNAL: the workgroup counts itself done in the slide's n_done once its
+// unit list entries are visible at agent scope, and the last one raises the
+// slide's ready flags (a detect in the same launch waits for them)
+template <int DIM, int MODE, int NT = kClassifyNT, bool SIGNAL = false>
 __device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int bid,
-                                              ClassifyLds<MODE == kClsLearned> &lds)
+                                              ClassifyLds<MODE == kClsLearned, NT> &lds)
 {
-    constexpr int NT = kClassifyNT, HALO = kClassifyHalo;
+    constexpr int HALO = kClassifyHalo;
     constexpr bool LEARN = MODE == kClsLearned;
     uint64_t *s_t = lds.t;
     const int i0 = bid * NT;
@@ -1123,7 +1129,7 @@ __device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int 
         __syncthreads();
         if (threadIdx.x == 0) {
             int tot[3] = {0, 0, 0};
-            for (int w = 0; w < ClassifyLds<LEARN>::NW; ++w)
+            for (int w = 0; w < ClassifyLds<LEARN, NT>::NW; ++w)
                 for (int q = 0; q < 3; ++q)
                     tot[q] += lds.wc[w][q];
             if (tot[0])
@@ -1141,13 +1147,41 @@ __device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int 
         if (ustart) {
             const bool front = (mo >> lane) & 1ull;
             const int li = front ? bo + __popcll(mo & lt) : W - 1 - (bi + __popcll(mi & lt));
-            d.units[li] = make_int2(i, ulen);
+            if constexpr (SIGNAL) // coherent across XCDs without an L2 write-back
+                __hip_atomic_store(reinterpret_cast<unsigned long long *>(d.units + li),
+                                   (unsigned long long)(unsigned)i |
+                                       ((unsigned long long)(unsigned)ulen << 32),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else
+                d.units[li] = make_int2(i, ulen);
             if constexpr (LEARN) {
                 float *rx = d.lr.rep_x +
                             ((size_t)(st.t % kReplaySlides) * W + li) * kFeatStride;
                 *reinterpret_cast<float4 *>(rx) = make_float4(f[0], f[1], f[2], f[3]);
                 *reinterpret_cast<float4 *>(rx + 4) = make_float4(f[4], f[5], 0.f, 0.f);
             }
+        }
+        if constexpr (SIGNAL) {
+            // the list entries were stored at agent scope and the counters
+            // updated by device atomics: once every wave's are acknowledged
+            // they are visible to agent-scope loads on any XCD, so the
+            // workgroup counts itself done without a release (whose L2
+            // write-back, under the window merge's stores, costs
+            // microseconds per workgroup)
+            // The last to count raises the slide's ready flags: every other
+            // workgroup's entries were acknowledged before its count.
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const int ncls = (W + NT - 1) / NT;
+                lds.base[0] = __hip_atomic_fetch_add(&cnt_of(d, st.t)->n_done, 1,
+                                                     __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT) == ncls - 1;
+            }
+            __syncthreads();
+            if (lds.base[0] && threadIdx.x < kReadyFlags)
+                __hip_atomic_store(ready_of(d, st.t, threadIdx.x), 1, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
@@ -1898,12 +1932,22 @@ struct DetectLds {
     int rows[NW][2][64];
     int last;
     int miss;
+    int ready;
     unsigned long long tests;
 };
 
+// Spins of a detect workgroup waiting for the classify of its own launch
+// (at least 0.1 us each): past this the classify is taken to have failed.
+constexpr int kClassifyWaitSpins = 1 << 20;
+constexpr int kErrClassifyWait = 8; // ctl->err flag of such a failure
+
 // bid / nblk: this workgroup and the number of workgroups of the detect
-// phase of this stream
-template <int DIM, bool LEARN>
+// phase of this stream.  WAIT: the slide's classify runs in the same launch
+// in workgroups of lower index; thread 0 waits (bounded) for one of the
+// slide's ready flags, and the counters and unit list entries the classify
+// wrote are then read at agent scope (an acquire would invalidate this XCD's
+// L2).
+template <int DIM, bool LEARN, bool WAIT = false>
 __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bid, int nblk,
                                             bool fin, DetectLds &lds)
 {
@@ -1921,9 +1965,42 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
     const int32_t *Ns = d.aslot[par];
     const float *Nx = d.axyz[par];
     const int W = d.W, K = d.K;
-    const int n_o = cnt_of(d, st.t)->n_units;
+    SlideCnt *cn = cnt_of(d, st.t);
+    if constexpr (WAIT) {
+        if (tid == 0) {
+            const int32_t *flag = ready_of(d, st.t, bid % kReadyFlags);
+            int ok = 0;
+            for (int it = 0; it < kClassifyWaitSpins; ++it) {
+                if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    ok = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(4);
+            }
+            if (!ok)
+                atomicOr(&d.ctl->err, kErrClassifyWait);
+            lds.ready = ok;
+        }
+        __syncthreads();
+        if (!lds.ready)
+            return;
+    }
+    auto ld_cnt = [&](const int32_t *p) {
+        return WAIT ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
+    };
+    auto ld_unit = [&](int li) {
+        if constexpr (WAIT) {
+            const unsigned long long v =
+                __hip_atomic_load(reinterpret_cast<const unsigned long long *>(d.units + li),
+                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return make_int2((int)(unsigned)v, (int)(unsigned)(v >> 32));
+        } else {
+            return d.units[li];
+        }
+    };
+    const int n_o = ld_cnt(&cn->n_units);
     uint32_t *bm = bitmap_of(d, st.t);
-    const int n_units = LEARN ? n_o + cnt_of(d, st.t)->n_units_i : n_o;
+    const int n_units = LEARN ? n_o + ld_cnt(&cn->n_units_i) : n_o;
     const bool use_pred = LEARN && d.lr.lc->use_pred != 0;
     const int busy = min(nblk, max(1, (n_units + NW - 1) / NW));
     if (bid >= busy)
@@ -1941,7 +2018,7 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
     int u = bid * NW + wid;
     int2 un = make_int2(0, 0);
     if (u < n_units)
-        un = d.units[list_index(u)];
+        un = ld_unit(list_index(u));
     float pc[3] = {0.f, 0.f, 0.f};
     int32_t cslot = 0;
     auto load_candidates = [&](int2 e) {
@@ -1966,7 +2043,7 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
         const int n = __builtin_amdgcn_readfirstlane(un.y);
         int2 un_next = make_int2(0, 0);
         if (u + ustep < n_units)
-            un_next = d.units[list_index(u + ustep)];
+            un_next = ld_unit(list_index(u + ustep));
         const Stencil<DIM> full = unit_stencil<DIM>(d, pc, n);
         int cnt;
         if (LEARN && DIM > 1 && use_pred && u >= n_o) {
@@ -2181,6 +2258,65 @@ __global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_detect_stag
     }
     bid -= nts;
     detect_body<DIM, LEARN>(d, st, bid, ndet, false, *reinterpret_cast<DetectLds *>(dyn_lds));
+}
+
+// ---------------------------------------------------------------------------
+// One launch per slide (plain schedule, cell-stencil index, the ingest
+// riding): k_detect_stage's parts behind ncls workgroups of slide st's
+// classify, which its detect waits for inside the launch.  The classify
+// workgroups have the lowest indices: workgroups are dispatched in index
+// order, and a classify workgroup waits for nothing, so every classify
+// workgroup gets a slot while detect workgroups wait.  The wait is bounded
+// (kClassifyWaitSpins): past it the detect raises kErrClassifyWait and
+// leaves, so the slide fails instead of the launch hanging.  The classify
+// writes slide st's counters and unit list only; no other part of the
+// launch reads them.
+// ---------------------------------------------------------------------------
+template <int DIM, int TSIT>
+__host__ __device__ constexpr size_t slide_stage_lds(bool fin, bool merge, bool runs, bool ts,
+                                                     int S)
+{
+    const size_t b = detect_stage_lds<DIM, TSIT>(fin, merge, runs, ts, S);
+    return sizeof(ClassifyLds<false, kDetectNT>) > b ? sizeof(ClassifyLds<false, kDetectNT>) : b;
+}
+
+template <int DIM, int TSIT>
+__global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_slide_stage(
+    Dev d, Step st, Step st_prev, Step st_merge, Step st_runs, Step st_ts, int ncls, int nfin,
+    int nmb, int nrm, int nts, int ndet)
+{
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    int bid = blockIdx.x;
+    if (bid < ncls) {
+        classify_body<DIM, kClsUnits, kDetectNT, true>(
+            d, st, bid, *reinterpret_cast<ClassifyLds<false, kDetectNT> *>(dyn_lds));
+        return;
+    }
+    bid -= ncls;
+    if (bid < nfin) {
+        finish_deferred<kDetectNT>(d, st_prev, bid, nfin, *reinterpret_cast<FinishLds *>(dyn_lds));
+        return;
+    }
+    bid -= nfin;
+    if (bid < nmb) {
+        merge_window_body<DIM, kDetectNT>(d, st_merge, bid,
+                                          *reinterpret_cast<MergeWindowLds<kDetectNT> *>(dyn_lds));
+        return;
+    }
+    bid -= nmb;
+    if (bid < nrm) {
+        merge_runs_body<DIM>(d, st_runs, kModeSlide, bid, reinterpret_cast<uint64_t *>(dyn_lds));
+        return;
+    }
+    bid -= nrm;
+    if (bid < nts) {
+        ingest_sort_body<DIM, kDetectNT, TSIT>(
+            d, st_ts, kModeSlide, nullptr, d.cpb == 1 ? 1 : 0, bid,
+            *reinterpret_cast<IngestLds<DIM, kDetectNT, TSIT> *>(dyn_lds));
+        return;
+    }
+    bid -= nts;
+    detect_body<DIM, false, true>(d, st, bid, ndet, false, *reinterpret_cast<DetectLds *>(dyn_lds));
 }
 
 } // namespace rtod

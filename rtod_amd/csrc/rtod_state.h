@@ -24,6 +24,8 @@
 //                               sparse cell's candidates in the sorted window
 //   bitmap   u32 [3][W/32]      outlier flags by ring slot, one set per
 //                               slide modulo 3
+//   ready    i32 [3][64 x 1024] one launch per slide: classify-done flags of
+//                               slide t at set t % 3, 4 KB apart
 //   ctl                         slide counters, tickets, pair tests, error
 //                               flags, classify counters (three sets)
 // Learned scheduler (only when configured):
@@ -55,6 +57,7 @@ struct SlideCnt {
     alignas(128) int32_t n_cand; // candidates
     alignas(128) int32_t n_units; // cell units (learned: the probable-outlier bucket)
     alignas(128) int32_t n_units_i; // learned: the probable-inlier bucket
+    alignas(128) int32_t n_done; // one launch per slide: classify workgroups done
 };
 
 struct Ctl {
@@ -161,6 +164,7 @@ struct Dev {
     uint32_t *bitmap;   // [3][bm_words]: outlier bits of slide t at (t % 3)
     int bm_words;       // ceil(W / 32), rounded up to 64
     int fin_parts;      // parts of a deferred slide epilogue
+    int32_t *ready;     // [3][kReadyFlags * kReadyStride]: classify-done flags
     Ctl *ctl;
     const float *h_in;
     HostOut *h_out;      // kHostOuts host-out buffers, slide t uses t % kHostOuts
@@ -186,6 +190,15 @@ __device__ __forceinline__ SlideCnt *cnt_of(const Dev &d, uint64_t t)
 __device__ __forceinline__ uint32_t *bitmap_of(const Dev &d, uint64_t t)
 {
     return d.bitmap + (size_t)(t % kSlideSets) * (size_t)d.bm_words;
+}
+// the classify-done flags of slide t: kReadyFlags copies, kReadyStride
+// words (4 KB) apart, so that the detect workgroups waiting on them spread
+// their polling over the memory channels (polling one address saturates its
+// channel and stalls every other access that maps to it)
+constexpr int kReadyFlags = 64, kReadyStride = 1024;
+__device__ __forceinline__ int32_t *ready_of(const Dev &d, uint64_t t, int j)
+{
+    return d.ready + ((size_t)(t % kSlideSets) * kReadyFlags + j) * kReadyStride;
 }
 __device__ __forceinline__ HostOut *host_out(const Dev &d, uint64_t t)
 {
