@@ -445,11 +445,14 @@ def main() -> int:
            "classify": 4 * dim * W,                     # one pass over the window
            "detect": 4 * dim * n_cand + 4 * n_out,      # candidates in, list out
            "detect_runs": 4 * dim * n_cand + 4 * n_out + 8 * dim * S,  # + next run merge
-           # two launches per slide: classify + the previous slide's list +
-           # tile sorts two slides ahead; detect + the next slide's window
-           # merge + the run merge two slides ahead
-           "classify_stage": 4 * dim * W + 8 * dim * S + 4 * n_out,
-           "detect_stage": 4 * dim * n_cand + 8 * dim * W + 8 * dim * S}
+           # two launches per slide (learned schedule): classify; detect +
+           # the previous slide's list + the next slide's window merge + the
+           # run merge two slides ahead + the tile sorts three ahead
+           "classify_stage": 4 * dim * W,
+           "detect_stage": 4 * dim * n_cand + 4 * n_out + 8 * dim * W + 16 * dim * S,
+           # one launch per slide (plain schedule): classify + all of the above
+           "slide_stage": 4 * dim * W + 4 * dim * n_cand + 4 * n_out + 8 * dim * W
+                          + 16 * dim * S}
     b_dom = alg.get(dom, 4 * dim * (W + S))
     achieved = b_dom / (dom_ms * 1e-3) / 1e9
     b_slide = 4 * dim * (W + S) + 4 * n_out
@@ -506,7 +509,7 @@ def main() -> int:
                 "+-U(2,5)); 3-D: mixture of 8 drifting anisotropic Gaussians + 0.5% noise",
         "config": {"workload": f"{args.workload} W={W} S={S} R={R} K={K} dim={dim} "
                                "(plain schedule, cell-stencil index; consecutive slides "
-                               "pipelined on one stream, two launches per slide)",
+                               "pipelined on one stream, one launch per slide)",
                    "window": W, "slide": S, "R": R, "K": K, "dim": dim},
         "sequential_us_per_slide": round(plain["lib_ms"] * 1e3, 3),
         "pipelined_mismatched_slides": pipe_mism,

@@ -143,11 +143,13 @@ RTOD_API rtod_status rtod_slide_device(rtod_ctx *ctx, const float *dev_new_pts,
  * dev_pts + i*S*dim.  No counterpart in the reference, whose loop
  * (optixScan.cpp:1045-1110) runs one slide per iteration; this runs the same
  * slides software-pipelined on one stream.  With the cell-stencil index and
- * S <= 8192 a slide is two launches: its classify, with the previous
- * slide's outlier list and the tile sorts of the slide two ahead; its
- * detect, with the next slide's window merge and the run merge of the slide
- * two ahead.  (Otherwise a slide's launches carry the next slide's sort and
- * the previous slide's list where they can.)  Up to four slides are queued,
+ * S <= 8192 a slide's detect shares a launch with the previous slide's
+ * outlier list, the next slide's window merge, the run merge of the slide
+ * two ahead and the tile sorts of the slide three ahead; its classify is a
+ * launch of its own before it (learned schedule) or the first workgroups of
+ * that launch, which the detect waits for (plain schedule).  (Otherwise a
+ * slide's launches carry the next slide's sort and the previous slide's list
+ * where they can.)  Up to four slides are queued,
  * and the host collects each slide's list as it completes.  After a failed
  * call the context must be primed again.  Results equal those of n_slides
  * rtod_slide_device calls:
@@ -179,9 +181,10 @@ RTOD_API rtod_status rtod_get_stats(const rtod_ctx *ctx, rtod_stats *out);
 /* Average device duration (ms) per launch of each kernel recorded with HIP
  * events on the context's stream while RTOD_FLAG_PHASE_TIMES is set (with
  * rtod_slides_device the flag also waits on every slide: a measuring mode;
- * its two launches per slide are named "classify_stage" and "detect_stage",
- * otherwise its window merge and detect launches that carry the next slide's
- * sort are "merge_window_ingest" and "detect_runs").
+ * its launch per slide is "slide_stage" (plain schedule) and its two launches
+ * per slide "classify_stage" and "detect_stage" (learned schedule), otherwise
+ * its window merge and detect launches that carry the next slide's sort are
+ * "merge_window_ingest" and "detect_runs").
  * names[i] (static strings) / avg_ms[i] / launches[i] for i < *n (in: capacity). */
 RTOD_API rtod_status rtod_kernel_times(const rtod_ctx *ctx, const char **names, double *avg_ms,
                                        int64_t *launches, int32_t *n);

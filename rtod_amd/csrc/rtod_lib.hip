@@ -362,8 +362,8 @@ void launch_finish(rtod_ctx *c, hipStream_t s, const Step &st)
     hipLaunchKernelGGL(k_finish, dim3(c->d.fin_parts), dim3(256), 0, s, c->d, st);
 }
 
-// Two launches per slide (cell-stencil index, the ingest riding): slide
-// st's classify ...
+// Stage launches (cell-stencil index, the ingest riding), learned
+// schedule: slide st's classify ...
 template <int DIM, int MODE>
 void launch_classify_stage_dim(rtod_ctx *c, hipStream_t s, const Step &st)
 {
@@ -919,7 +919,10 @@ rtod_status collect(rtod_ctx *c, int32_t *outs, int32_t *n_out)
         return RTOD_ERR_INTERNAL;
     }
     if (ho->err != 0) {
-        set_error("device consistency check failed (code " + std::to_string(ho->err) + ")");
+        set_error("device consistency check failed (code " + std::to_string(ho->err) +
+                  ((ho->err & kErrClassifyWait) ? ": a detect timed out waiting for its classify"
+                                                : "") +
+                  ")");
         c->host_t++;
         return RTOD_ERR_INTERNAL;
     }
@@ -1187,12 +1190,12 @@ rtod_status wait_slide(rtod_ctx *c, int64_t t)
 // rtod_slide_device calls.  On an error the stream is drained first.
 // RTOD_FLAG_PHASE_TIMES: HIP events around the core kernels, and each slide
 // is waited for before the next is enqueued (a measuring mode).
-// Pipelined slides in two launches per slide (cell-stencil index, the
-// ingest riding; see k_classify_stage): after a prologue (slide t0's ingest
-// and window merge, the ingest of slides t0 + 1 and t0 + 2), slide t0 + i
-// is the classify stage A(i) and the detect stage B(i), and the last
-// slide's epilogue follows in a launch of its own.  Up to kHostOuts slides
-// are in flight.
+// Pipelined slides in stage launches (cell-stencil index, the ingest
+// riding; see k_classify_stage): after a prologue (slide t0's ingest and
+// window merge, the ingest of slides t0 + 1 and t0 + 2), slide t0 + i is one
+// launch, k_slide_stage (plain schedule), or two, the classify stage and the
+// detect stage (learned schedule), and the last slide's epilogue follows in
+// a launch of its own.  Up to kHostOuts slides are in flight.
 rtod_status slides_two_stage(rtod_ctx *c, const float *dev, int32_t n, int32_t *offs,
                              int32_t *slots)
 {

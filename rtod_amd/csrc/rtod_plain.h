@@ -2107,10 +2107,11 @@ __global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_detect(Dev 
 }
 
 // ---------------------------------------------------------------------------
-// Three launches per slide, for pipelined slides that cannot take two (the
-// window merge of each slide on its own launch): slide t's window merge
-// carries slide t-1's epilogue and slide t+1's tile sorts, slide t's detect
-// slide t+1's run merge; it is also the prologue of the two-launch schedule.
+// Three launches per slide, for pipelined slides that cannot take the stage
+// launches (the window merge of each slide on its own launch): slide t's
+// window merge carries slide t-1's epilogue and slide t+1's tile sorts, slide
+// t's detect slide t+1's run merge; it is also the prologue of the stage
+// launches.
 // The parts touch disjoint memory: the ingest writes batch-ring entry
 // enew(t+1) and the tile scratch of slide t+1; slide t's merge reads entries
 // enew(t) and eold(t) (distinct from enew(t+1) with m + 3 entries), writes
@@ -2177,21 +2178,23 @@ __global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_detect_runs
 }
 
 // ---------------------------------------------------------------------------
-// Two launches per slide (pipelined slides on one stream, cell-stencil
-// index, the ingest riding): the classify of slide t; then the detect
-// stage of slide t, carrying the epilogue of slide t - 1, the window merge
-// of slide t + 1, the run merge of slide t + 2 and the tile sorts of slide
-// t + 3.  The parts of a launch touch disjoint memory (counters and bitmaps
-// come in kSlideSets = 3 sets, by slide modulo 3):
-//   classify(t): reads window t; writes the candidate and unit lists and
-//     counters cnt[t % 3];
+// Pipelined slides on one stream (cell-stencil index, the ingest riding):
+// the detect stage of slide t carries the epilogue of slide t - 1, the
+// window merge of slide t + 1, the run merge of slide t + 2 and the tile
+// sorts of slide t + 3; the classify of slide t is a launch of its own
+// before it (learned schedule: k_classify_stage) or its first workgroups
+// (plain schedule: k_slide_stage).  The parts touch disjoint memory
+// (counters, bitmaps and ready flags come in kSlideSets = 3 sets, by slide
+// modulo 3):
+//   classify(t): reads window t; writes the candidate and unit lists,
+//     counters cnt[t % 3] and (one launch) ready flags t % 3;
 //   epilogue(t - 1): reads bitmap (t - 1) % 3 and cnt[(t - 1) % 3]; writes
 //     host-out buffer t - 1;
 //   detect(t): reads window t, the lists and cnt[t % 3]; sets bitmap t % 3;
 //   merge(t + 1): reads window t and batch-ring entries enew(t + 1),
 //     eold(t + 1); writes window t + 1 (the other buffers); resets
-//     cnt[(t + 1) % 3] and clears bitmap (t + 1) % 3 (last read by
-//     epilogue(t - 2), two launches earlier);
+//     cnt[(t + 1) % 3] and ready flags (t + 1) % 3 and clears bitmap
+//     (t + 1) % 3 (last read by epilogue(t - 2), a detect stage earlier);
 //   run merge(t + 2): reads tile scratch (t + 2) & 1; writes ring entry
 //     enew(t + 2);
 //   tile sorts(t + 3): write tile scratch (t + 3) & 1 (one tile: ring entry

@@ -226,8 +226,9 @@ PIPE_CASES = [
 
 @pytest.mark.parametrize("name,wl,T,kw", PIPE_CASES, ids=[c[0] for c in PIPE_CASES])
 def test_pipelined_slides_vs_grid(name, wl, T, kw):
-    """rtod_slides_device (two launches per slide on one stream, the slides'
-    sorts, merges and lists riding in neighbouring slides' launches) against
+    """rtod_slides_device (stage launches on one stream: one per slide, two
+    with the learned schedule; the slides' sorts, merges and lists riding in
+    neighbouring slides' launches) against
     the grid oracle, mixed with single slides on the same context: 3 single,
     T-8 pipelined, 2 single, 3 pipelined."""
     case(f"pipelined {name}")
