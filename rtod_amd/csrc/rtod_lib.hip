@@ -116,6 +116,7 @@ struct rtod_ctx {
     // dispatcher hands a finished workgroup's place to a waiting one (the
     // large 3-D window: 127.9 -> 123.2 us per slide; four times: 122.4)
     int detect_blocks = 1536;
+    int units_last = -1;     // cell units of the last collected slide (-1: none yet)
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = true; // false: a batch member on the batch's stream
@@ -435,8 +436,13 @@ void launch_slide_stage_t(rtod_ctx *c, hipStream_t s, const Step &st, const Stag
     const size_t lds = slide_stage_lds<DIM, TSIT>(nfin > 0, nmb > 0, nrm > 0, nts > 0, c->S);
     const Step pv = p.prev ? *p.prev : Step{}, mv = p.merge ? *p.merge : Step{},
                rv = p.runs ? *p.runs : Step{}, tv = p.ts ? *p.ts : Step{};
+    // detect workgroups staging their sample table before the wait: twice
+    // the busy ones of the last collected slide (a few slides back)
+    constexpr int NW = kDetectNT / kWave;
+    const int pre =
+        c->units_last < 0 ? 64 : std::min(ndet, std::max(16, 2 * ((c->units_last + NW - 1) / NW)));
     hipLaunchKernelGGL((k_slide_stage<DIM, TSIT>), g, dim3(kDetectNT), lds, s, c->d, st, pv, mv,
-                       rv, tv, ncls, nfin, nmb, nrm, nts, ndet);
+                       rv, tv, ncls, nfin, nmb, nrm, nts, ndet, pre);
 }
 
 void launch_slide_stage(rtod_ctx *c, hipStream_t s, const Step &st, const StageParts &p)
@@ -936,6 +942,7 @@ rtod_status collect(rtod_ctx *c, int32_t *outs, int32_t *n_out)
     if (n_out)
         *n_out = n;
     c->host_t++;
+    c->units_last = ho->n_units;
     c->cand_last = ho->n_cand;
     c->cand_total += ho->n_cand;
     c->tests_last = (int64_t)ho->pair_tests;

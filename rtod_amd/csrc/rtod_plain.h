@@ -1947,9 +1947,12 @@ constexpr int kErrClassifyWait = 8; // ctl->err flag of such a failure
 // slide's ready flags, and the counters and unit list entries the classify
 // wrote are then read at agent scope (an acquire would invalidate this XCD's
 // L2).
+// The first `pre` workgroups stage the sample table before they wait (it
+// comes from the window, not from the classify), so that its round trip
+// overlaps the classify.
 template <int DIM, bool LEARN, bool WAIT = false>
 __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bid, int nblk,
-                                            bool fin, DetectLds &lds)
+                                            bool fin, DetectLds &lds, int pre = 0)
 {
     constexpr int NW = kDetectNT / kWave;
     uint64_t *s_samp = lds.samp;
@@ -1966,6 +1969,9 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
     const float *Nx = d.axyz[par];
     const int W = d.W, K = d.K;
     SlideCnt *cn = cnt_of(d, st.t);
+    const bool staged = WAIT && bid < pre;
+    if (staged)
+        copy_to_lds<kMaxSamples / kDetectNT>(s_samp, d.samples[par], d.nsamp);
     if constexpr (WAIT) {
         if (tid == 0) {
             const int32_t *flag = ready_of(d, st.t, bid % kReadyFlags);
@@ -2031,7 +2037,8 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
             cslot = Ns[start + lane];
         }
     };
-    copy_to_lds<kMaxSamples / kDetectNT>(s_samp, d.samples[par], d.nsamp);
+    if (!staged)
+        copy_to_lds<kMaxSamples / kDetectNT>(s_samp, d.samples[par], d.nsamp);
     if (u < n_units)
         load_candidates(un);
     __syncthreads();
@@ -2286,7 +2293,7 @@ __host__ __device__ constexpr size_t slide_stage_lds(bool fin, bool merge, bool 
 template <int DIM, int TSIT>
 __global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_slide_stage(
     Dev d, Step st, Step st_prev, Step st_merge, Step st_runs, Step st_ts, int ncls, int nfin,
-    int nmb, int nrm, int nts, int ndet)
+    int nmb, int nrm, int nts, int ndet, int pre)
 {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     int bid = blockIdx.x;
@@ -2319,7 +2326,8 @@ __global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_slide_stage
         return;
     }
     bid -= nts;
-    detect_body<DIM, false, true>(d, st, bid, ndet, false, *reinterpret_cast<DetectLds *>(dyn_lds));
+    detect_body<DIM, false, true>(d, st, bid, ndet, false, *reinterpret_cast<DetectLds *>(dyn_lds),
+                                  pre);
 }
 
 } // namespace rtod
