@@ -45,7 +45,7 @@ def _run(cmd: list[str]) -> None:
 
 def build_lib() -> str:
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    tmp = LIB + ".tmp"
+    tmp = f"{LIB}.{os.getpid()}.tmp"  # one per build: concurrent builds never share it
     _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs])
     os.replace(tmp, LIB)
     return LIB
@@ -55,7 +55,7 @@ def build_ods() -> str:
     src = os.path.join(HERE, "host", "ods.cpp")
     if not os.path.exists(src):
         return ""
-    tmp = ODS + ".tmp"
+    tmp = f"{ODS}.{os.getpid()}.tmp"
     _run(["g++", "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}", "-o", tmp, src,
           f"-L{HERE}", "-lrtod", f"-Wl,-rpath,$ORIGIN", "-lpthread"])
     os.replace(tmp, ODS)
@@ -68,7 +68,7 @@ SYNTH = os.path.join(HERE, "librtod_synth.so")
 def build_synth() -> str:
     """Synthetic-stream generators (plain C, gcc)."""
     src = os.path.join(HERE, "host", "synth.c")
-    tmp = SYNTH + ".tmp"
+    tmp = f"{SYNTH}.{os.getpid()}.tmp"
     _run(["gcc", "-O2", "-std=c11", "-ffp-contract=off", "-fPIC", "-shared",
           "-fvisibility=hidden", "-o", tmp, src, "-lm"])
     os.replace(tmp, SYNTH)

@@ -12,7 +12,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -678,6 +680,28 @@ struct CreateOpts {
     bool batch_member = false;    // a stream of a batch (its own slide kernels)
 };
 
+// The kernel-argument structs as the device pass compiled them.  Host and
+// device passes of one build see the same headers; a library whose passes
+// saw different ones (sources edited while it compiled) would hand its
+// kernels misplaced pointers, so it is refused before any launch.
+__device__ int32_t g_abi_layout[3] = {(int32_t)sizeof(Dev), (int32_t)sizeof(Step),
+                                            (int32_t)offsetof(Dev, ctl)};
+
+rtod_status check_abi_layout()
+{
+    int32_t dev[3] = {0, 0, 0};
+    RTOD_HIP(hipMemcpyFromSymbol(dev, HIP_SYMBOL(g_abi_layout), sizeof(dev), 0,
+                                 hipMemcpyDeviceToHost));
+    const int32_t host[3] = {(int32_t)sizeof(Dev), (int32_t)sizeof(Step),
+                             (int32_t)offsetof(Dev, ctl)};
+    if (std::memcmp(dev, host, sizeof(dev)) != 0) {
+        set_error("librtod.so was built from inconsistent sources: kernel argument layout "
+                  "differs between its host and device code; rebuild it");
+        return RTOD_ERR_INTERNAL;
+    }
+    return RTOD_OK;
+}
+
 rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts &opt = {})
 {
     RTOD_TRY(validate(cfg));
@@ -693,6 +717,7 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
     c->K = cfg->K;
     c->device = cfg->device;
     RTOD_HIP(hipSetDevice(c->device));
+    RTOD_TRY(check_abi_layout());
     if (opt.stream) {
         c->stream = opt.stream;
         c->own_stream = false;
@@ -751,6 +776,13 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
     // merge); a batch's streams merge tiles in passes
     d.tile_par = (!opt.batch_member && use_run_merge(c->S, c->CH)) ? 1 : 0;
     d.unit_max = opt.batch_member ? kUnitCap : unit_max_single(c->dim);
+    d.wait_spins = kClassifyWaitSpins;
+    if (const char *e = std::getenv("RTOD_CLASSIFY_WAIT_SPINS")) {
+        char *end = nullptr;
+        const long v = std::strtol(e, &end, 10);
+        if (end != e && *end == '\0' && v >= 0 && v <= kClassifyWaitSpins)
+            d.wait_spins = (int)v;
+    }
     const size_t ne = (size_t)d.nent;
     RTOD_TRY(dalloc(c, &d.bkey, ne * S));
     RTOD_TRY(dalloc(c, &d.bslot, ne * S));

@@ -1938,6 +1938,8 @@ struct DetectLds {
 
 // Spins of a detect workgroup waiting for the classify of its own launch
 // (at least 0.1 us each): past this the classify is taken to have failed.
+// The environment variable RTOD_CLASSIFY_WAIT_SPINS, read when a context is
+// created, replaces it (0 fails every such slide: a test of the error path).
 constexpr int kClassifyWaitSpins = 1 << 20;
 constexpr int kErrClassifyWait = 8; // ctl->err flag of such a failure
 
@@ -1976,7 +1978,7 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
         if (tid == 0) {
             const int32_t *flag = ready_of(d, st.t, bid % kReadyFlags);
             int ok = 0;
-            for (int it = 0; it < kClassifyWaitSpins; ++it) {
+            for (int it = 0; it < d.wait_spins; ++it) {
                 if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
                     ok = 1;
                     break;

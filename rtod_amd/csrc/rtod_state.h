@@ -165,6 +165,7 @@ struct Dev {
     int bm_words;       // ceil(W / 32), rounded up to 64
     int fin_parts;      // parts of a deferred slide epilogue
     int32_t *ready;     // [3][kReadyFlags * kReadyStride]: classify-done flags
+    int wait_spins;     // bound of a detect's wait for its classify (one launch per slide)
     Ctl *ctl;
     const float *h_in;
     HostOut *h_out;      // kHostOuts host-out buffers, slide t uses t % kHostOuts

@@ -306,6 +306,28 @@ def test_pipelined_slides_argument_checks():
     assert_same(got, O.run_stream(xs, W, S, 2, 0.45, 10, "grid"), "pipelined after errors")
 
 
+def test_classify_wait_timeout_fails_slides(monkeypatch):
+    """A pipelined slide's detect waits for the classify of its own launch
+    with a bounded spin.  A context created with RTOD_CLASSIFY_WAIT_SPINS=0
+    gives up at once: the pipelined call fails with error flag 8 (every
+    kernel still exits), and after priming again the context's one-slide
+    calls equal the grid oracle."""
+    case("classify wait bound 0")
+    wl, T = O.C1, 6
+    W, S = wl.W, wl.S
+    xs = wl.data(W + T * S)
+    want = O.run_stream(xs, W, S, T, wl.R, wl.K, "grid")
+    monkeypatch.setenv("RTOD_CLASSIFY_WAIT_SPINS", "0")
+    with R.DeviceStream(xs) as ds, R.Detector(1, W, S, wl.R, wl.K) as det:
+        monkeypatch.delenv("RTOD_CLASSIFY_WAIT_SPINS")
+        det.prime(xs[:W])
+        with pytest.raises(R.RtodError, match="code 8"):
+            det.slides_device(ds.at(W), 4)
+        det.prime(xs[:W])
+        got = [det.slide_device(ds.at(W + t * S)) for t in range(T)]
+    assert_same(got, want, "one-slide calls after a failed wait")
+
+
 def test_reprime_restarts_stream():
     case("reprime")
     W, S, T, K = 2000, 200, 5, 10
