@@ -147,7 +147,9 @@ RTOD_API rtod_status rtod_slide_device(rtod_ctx *ctx, const float *dev_new_pts,
  * outlier list, the next slide's window merge, the run merge of the slide
  * two ahead and the tile sorts of the slide three ahead; its classify is a
  * launch of its own before it (learned schedule) or the first workgroups of
- * that launch, which the detect waits for (plain schedule).  (Otherwise a
+ * that launch, which the detect waits for (plain schedule; the wait is
+ * bounded, 2^20 polls or the environment variable RTOD_CLASSIFY_WAIT_SPINS
+ * read at rtod_create, and past it the call fails with error flag 8).  (Otherwise a
  * slide's launches carry the next slide's sort and the previous slide's list
  * where they can.)  Up to four slides are queued,
  * and the host collects each slide's list as it completes.  After a failed
