@@ -1312,7 +1312,8 @@ __device__ __forceinline__ int refine_bounds(const uint64_t *tab, const int *w_L
 
 // Row bounds of a candidate's stencil: lane b < 2*nrows computes bound b
 // (row b>>1, upper if b&1).  The sample table in LDS (stride S1 = 2^sl2)
-// narrows each bound to one gap of S1 - 1 keys.  With S1 <= 4 those keys are
+// narrows each bound to one gap of S1 - 1 keys.  With S1 <= 4, or few
+// enough bounds that all their gaps load in one round trip, those keys are
 // probed directly; else the every-4th-key table narrows the gap to 3 keys
 // first (S1/4 - 1 probes, then 3): two short coalesced rounds instead of
 // S1 - 1 probes per bound, which is most of a 3-D unit's memory traffic.
@@ -1342,8 +1343,11 @@ __device__ __forceinline__ int row_bounds(const Dev &d, const uint64_t *Nk, cons
         }
     }
     // the answer is 0 when no sample passes; else it lies in
-    // [(lo-1)*S1 + 1, lo*S1], element (lo-1)*S1 passing
-    if (!Nk4) {
+    // [(lo-1)*S1 + 1, lo*S1], element (lo-1)*S1 passing.  Probed directly
+    // when every bound's gap fits the loads refine_bounds keeps in flight
+    // (8 batches of 64): one round trip, over the same few cache lines the
+    // two rounds through the every-4th-key table would touch.
+    if (!Nk4 || (nb << sl2) <= 8 * kWave) {
         if (lane < nb)
             w_L[lane] = lo > 0 ? ((lo - 1) << sl2) + 1 : -1;
         wave_sync_lds();
