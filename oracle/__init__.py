@@ -74,6 +74,14 @@ def lib():
                                         f32p, f32p, C.c_int]
         L.rtod_o_bf16_round.restype = C.c_float
         L.rtod_o_bf16_round.argtypes = [C.c_float]
+        i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+        L.rtod_o_semantics_diff.restype = C.c_int64
+        L.rtod_o_semantics_diff.argtypes = [f32p, C.c_int64, C.c_int, C.c_float, C.c_int, C.c_int,
+                                            i64p]
+        L.rtod_o_semantics_diff_stream.restype = C.c_int64
+        L.rtod_o_semantics_diff_stream.argtypes = [f32p, C.c_int, C.c_int64, C.c_int64, C.c_int64,
+                                                   C.c_int64, C.c_float, C.c_int, C.c_int, i64p,
+                                                   i64p]
         L.rtod_o_run_stream.restype = C.c_int64
         L.rtod_o_run_stream.argtypes = [C.c_int, f32p, C.c_int, C.c_int64, C.c_int64, C.c_int64,
                                         C.c_float, C.c_int, C.c_int, i32p, i32p, C.c_int64,
@@ -120,6 +128,35 @@ def sorted1d(win: np.ndarray, R: float, K: int, counts: bool = False):
     if n < 0:
         raise ValueError("bad arguments")
     return (out[:n].copy(), cnt) if counts else out[:n].copy()
+
+
+def semantics_diff(win: np.ndarray, R: float, K: int, nthreads: int = 8) -> dict:
+    """Contract (FP32 d2 <= r2) vs the reference's FP64 strict rule on one
+    window: points whose outlier status differs, points whose full counts
+    differ, ordered pairs the two rules disagree on."""
+    win = np.ascontiguousarray(win, np.float32)
+    W, dim = win.shape
+    o = np.zeros(3, np.int64)
+    if lib().rtod_o_semantics_diff(win.ravel(), W, dim, R, K, nthreads, o) < 0:
+        raise ValueError("bad arguments")
+    return {"status": int(o[0]), "counts": int(o[1]), "pairs": int(o[2])}
+
+
+def semantics_diff_stream(xs: np.ndarray, W: int, S: int, T: int, R: float, K: int,
+                          every: int = 1, nthreads: int = 8):
+    """semantics_diff over slides 1..T (every `every`-th): totals and the
+    per-slide status differences (-1 for slides not checked)."""
+    xs = np.ascontiguousarray(xs, np.float32)
+    dim = xs.shape[1]
+    if xs.shape[0] < W + T * S:
+        raise ValueError("stream too short")
+    per = np.zeros(max(T, 1), np.int64)
+    o = np.zeros(3, np.int64)
+    if lib().rtod_o_semantics_diff_stream(xs.ravel(), dim, W, S, T, every, R, K, nthreads, per,
+                                          o) < 0:
+        raise ValueError("bad arguments")
+    return {"status": int(o[0]), "counts": int(o[1]), "pairs": int(o[2]),
+            "slides_checked": int((per[:T] >= 0).sum())}, per[:T]
 
 
 def cell_key(p, R: float) -> int:

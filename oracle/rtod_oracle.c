@@ -833,6 +833,232 @@ int64_t rtod_o_run_stream(int method, const float *xs, int dim, int64_t W, int64
 }
 
 /* ------------------------------------------------------------------ */
+/* contract vs reference semantics                                      */
+/* ------------------------------------------------------------------ */
+/* For one window, the full neighbour count of every point under both rules
+ * -- the contract (FP32 d2 <= r2, pred_contract) and the reference's FP64
+ * strict test (pred_reference, optixScan.cu:170-189) -- and the points whose
+ * outlier status (count <= K) differs.  1-D: both neighbour sets of a point
+ * are index intervals of the sorted window containing the point (each
+ * predicate is monotone in |x_i - x_j|), found by binary search.  2-D/3-D:
+ * cells of side R (1 + 1e-4), every pair of neighbouring cells enumerated
+ * (a pair passing either rule is closer than R (1 + 1e-6) in exact
+ * arithmetic, so it lies in adjacent cells).  out3: [0] points whose status
+ * differs, [1] points whose counts differ, [2] ordered pairs on which the
+ * rules differ. */
+typedef struct {
+    uint64_t key;
+    int32_t slot;
+} ks_t;
+
+static int cmp_ks(const void *a, const void *b)
+{
+    const ks_t *x = (const ks_t *)a, *y = (const ks_t *)b;
+    if (x->key != y->key)
+        return x->key < y->key ? -1 : 1;
+    return (x->slot > y->slot) - (x->slot < y->slot);
+}
+
+typedef struct {
+    const float *win;
+    int dim, K;
+    int64_t W;
+    float r2;
+    double R, R2, inv_c;
+    const vs_t *a1;  /* 1-D: window sorted by value */
+    const ks_t *ks;  /* 2-D/3-D: window sorted by cell key */
+    int64_t acc[256][3];
+} sdiff_ctx;
+
+static int64_t ks_lower(const ks_t *ks, int64_t n, uint64_t key)
+{
+    int64_t L = 0, H = n;
+    while (L < H) {
+        int64_t mid = L + (H - L) / 2;
+        if (ks[mid].key < key)
+            L = mid + 1;
+        else
+            H = mid;
+    }
+    return L;
+}
+
+static uint64_t sdiff_key(const int64_t *k, int dim)
+{
+    /* 21 bits per axis around a bias; coordinates beyond are clamped (a
+     * pair there is compared with the clamped cells' points, still exactly) */
+    uint64_t key = 0;
+    for (int a = 0; a < dim; a++) {
+        int64_t v = k[a] + (1ll << 20);
+        if (v < 0)
+            v = 0;
+        if (v > (1ll << 21) - 1)
+            v = (1ll << 21) - 1;
+        key = (key << 21) | (uint64_t)v;
+    }
+    return key;
+}
+
+static void sdiff_range(void *p, int64_t lo, int64_t hi, int tid)
+{
+    sdiff_ctx *c = (sdiff_ctx *)p;
+    int64_t st = 0, cd = 0, pd = 0;
+    for (int64_t i = lo; i < hi; i++) {
+        int64_t n32 = 0, n64 = 0;
+        if (c->dim == 1) {
+            const vs_t *a = c->a1;
+            const float x = a[i].v;
+            int64_t b[4];
+            for (int r = 0; r < 2; r++) {
+                /* first j in [0,i] passing, last j in [i,W) passing */
+                int64_t L = 0, H = i;
+                while (L < H) {
+                    int64_t mid = L + (H - L) / 2;
+                    int ok = r == 0 ? pred_contract(&x, &a[mid].v, 1, c->r2)
+                                    : pred_reference(&x, &a[mid].v, 1, c->R, c->R2);
+                    if (ok)
+                        H = mid;
+                    else
+                        L = mid + 1;
+                }
+                b[2 * r] = L;
+                L = i;
+                H = c->W - 1;
+                while (L < H) {
+                    int64_t mid = L + (H - L + 1) / 2;
+                    int ok = r == 0 ? pred_contract(&x, &a[mid].v, 1, c->r2)
+                                    : pred_reference(&x, &a[mid].v, 1, c->R, c->R2);
+                    if (ok)
+                        L = mid;
+                    else
+                        H = mid - 1;
+                }
+                b[2 * r + 1] = L;
+            }
+            n32 = b[1] - b[0] + 1;
+            n64 = b[3] - b[2] + 1;
+            pd += llabs(b[0] - b[2]) + llabs(b[1] - b[3]);
+        } else {
+            const float *p0 = c->win + (int64_t)c->ks[i].slot * c->dim;
+            int64_t k[3] = {0, 0, 0};
+            for (int a = 0; a < c->dim; a++)
+                k[a] = (int64_t)floor((double)p0[a] * c->inv_c);
+            int64_t nb = 1;
+            for (int a = 0; a < c->dim; a++)
+                nb *= 3;
+            for (int64_t q = 0; q < nb; q++) {
+                int64_t kk[3] = {0, 0, 0}, qq = q;
+                for (int a = 0; a < c->dim; a++) {
+                    kk[a] = k[a] + (qq % 3) - 1;
+                    qq /= 3;
+                }
+                const uint64_t key = sdiff_key(kk, c->dim);
+                for (int64_t j = ks_lower(c->ks, c->W, key); j < c->W && c->ks[j].key == key;
+                     j++) {
+                    const float *q0 = c->win + (int64_t)c->ks[j].slot * c->dim;
+                    const int a32 = pred_contract(p0, q0, c->dim, c->r2);
+                    const int a64 = pred_reference(p0, q0, c->dim, c->R, c->R2);
+                    n32 += a32;
+                    n64 += a64;
+                    pd += a32 != a64;
+                }
+            }
+        }
+        cd += n32 != n64;
+        st += (n32 <= c->K) != (n64 <= c->K);
+    }
+    c->acc[tid][0] += st;
+    c->acc[tid][1] += cd;
+    c->acc[tid][2] += pd;
+}
+
+int64_t rtod_o_semantics_diff(const float *win, int64_t W, int dim, float R, int K, int nthreads,
+                              int64_t *out3)
+{
+    if (!win || W <= 0 || dim < 1 || dim > 3 || K < 1 || !(R > 0.0f))
+        return -1;
+    if (nthreads < 1)
+        nthreads = 1;
+    if (nthreads > 256)
+        nthreads = 256;
+    sdiff_ctx *c = (sdiff_ctx *)calloc(1, sizeof(sdiff_ctx));
+    c->win = win;
+    c->dim = dim;
+    c->K = K;
+    c->W = W;
+    c->r2 = R * R;
+    c->R = (double)R;
+    c->R2 = (double)R * (double)R;
+    c->inv_c = 1.0 / ((double)R * (1.0 + 1e-4));
+    vs_t *a1 = NULL;
+    ks_t *ks = NULL;
+    if (dim == 1) {
+        a1 = (vs_t *)malloc(sizeof(vs_t) * (size_t)W);
+        for (int64_t i = 0; i < W; i++) {
+            a1[i].v = win[i];
+            a1[i].slot = (int32_t)i;
+        }
+        qsort(a1, (size_t)W, sizeof(vs_t), cmp_vs);
+        c->a1 = a1;
+    } else {
+        ks = (ks_t *)malloc(sizeof(ks_t) * (size_t)W);
+        for (int64_t i = 0; i < W; i++) {
+            int64_t k[3] = {0, 0, 0};
+            for (int a = 0; a < dim; a++)
+                k[a] = (int64_t)floor((double)win[i * dim + a] * c->inv_c);
+            ks[i].key = sdiff_key(k, dim);
+            ks[i].slot = (int32_t)i;
+        }
+        qsort(ks, (size_t)W, sizeof(ks_t), cmp_ks);
+        c->ks = ks;
+    }
+    parallel_for(nthreads, W, 256, sdiff_range, c);
+    int64_t r[3] = {0, 0, 0};
+    for (int t = 0; t < 256; t++)
+        for (int q = 0; q < 3; q++)
+            r[q] += c->acc[t][q];
+    if (out3)
+        for (int q = 0; q < 3; q++)
+            out3[q] = r[q];
+    free(a1);
+    free(ks);
+    free(c);
+    return r[0];
+}
+
+int64_t rtod_o_semantics_diff_stream(const float *xs, int dim, int64_t W, int64_t S, int64_t T,
+                                     int64_t every, float R, int K, int nthreads,
+                                     int64_t *per_slide, int64_t *out3)
+{
+    if (dim < 1 || dim > 3 || W <= 0 || S <= 0 || T < 0 || every < 1)
+        return -1;
+    float *win = (float *)malloc(sizeof(float) * (size_t)(W * dim));
+    int64_t tot[3] = {0, 0, 0};
+    for (int64_t t = 1; t <= T; t++) {
+        if (per_slide)
+            per_slide[t - 1] = -1;
+        if ((t - 1) % every != 0)
+            continue;
+        for (int64_t gi = t * S; gi < t * S + W; gi++)
+            memcpy(win + (gi % W) * dim, xs + gi * dim, sizeof(float) * (size_t)dim);
+        int64_t o[3];
+        if (rtod_o_semantics_diff(win, W, dim, R, K, nthreads, o) < 0) {
+            free(win);
+            return -1;
+        }
+        if (per_slide)
+            per_slide[t - 1] = o[0];
+        for (int q = 0; q < 3; q++)
+            tot[q] += o[q];
+    }
+    free(win);
+    if (out3)
+        for (int q = 0; q < 3; q++)
+            out3[q] = tot[q];
+    return tot[0];
+}
+
+/* ------------------------------------------------------------------ */
 /* MLP reference (learned scheduler's predictor)                        */
 /* ------------------------------------------------------------------ */
 #define NIN 6

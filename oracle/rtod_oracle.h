@@ -85,6 +85,21 @@ int64_t rtod_o_run_stream(int method, const float *xs, int dim, int64_t W,
                           int32_t *n_out, int32_t *out, int64_t cap,
                           double *us_per_slide);
 
+/* Contract (FP32 d2 <= r2) against the reference's FP64 strict rule
+ * (optixScan.cu:170-189) on one window: full neighbour counts under both
+ * rules (1-D: sorted-window intervals; else adjacent cells of side
+ * R (1 + 1e-4)).  out3 (nullable): [0] points whose outlier status differs,
+ * [1] points whose counts differ, [2] ordered pairs the rules disagree on.
+ * Returns out3[0], or -1 on bad arguments. */
+int64_t rtod_o_semantics_diff(const float *win, int64_t W, int dim, float R, int K,
+                              int nthreads, int64_t *out3);
+/* The same over slides 1..T of stream xs (every `every`-th slide, from slide
+ * 1); per_slide[T] (nullable) gets each checked slide's status differences
+ * (-1: not checked); out3 the totals.  Returns the total of out3[0]. */
+int64_t rtod_o_semantics_diff_stream(const float *xs, int dim, int64_t W, int64_t S, int64_t T,
+                                     int64_t every, float R, int K, int nthreads,
+                                     int64_t *per_slide, int64_t *out3);
+
 /* ------------------------------------------------------------------ */
 /* Learned scheduler's predictor: MLP 6 -> 32 (ReLU) -> 32 (ReLU) -> 1     */
 /* (sigmoid), loss = mean squared error.  Parameters, flat (RTOD_O_MLP_NP  */

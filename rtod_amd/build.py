@@ -6,9 +6,12 @@ disabled.
 """
 from __future__ import annotations
 
+import hashlib
 import os
+import shutil
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -17,7 +20,6 @@ LIB = os.path.join(HERE, "librtod.so")
 ODS = os.path.join(HERE, "ods")
 
 HIP_SOURCES = ["rtod_lib.hip"]
-HIP_DEPS = ["rtod_device.h", "rtod_state.h", "rtod_plain.h", "rtod_lbvh.h", "rtod_learned.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -43,10 +45,39 @@ def _run(cmd: list[str]) -> None:
             sys.stderr.write(line + "\n")
 
 
+def _snapshot(dst: str) -> str:
+    """Copy the HIP sources and the public header into dst (same relative
+    layout) and return the SHA-256 of their contents, first 16 hex digits.
+    hipcc compiles the snapshot, so a source edited while a build runs cannot
+    reach one of its passes and not the other."""
+    h = hashlib.sha256()
+    files = [("rtod_amd/csrc", f) for f in sorted(os.listdir(CSRC))
+             if f.endswith((".h", ".hip"))]
+    files += [("include", f) for f in sorted(os.listdir(os.path.join(ROOT, "include")))
+              if f.endswith(".h")]
+    for sub, f in files:
+        os.makedirs(os.path.join(dst, sub), exist_ok=True)
+        with open(os.path.join(ROOT, sub, f), "rb") as fh:
+            data = fh.read()
+        with open(os.path.join(dst, sub, f), "wb") as fh:
+            fh.write(data)
+        h.update(f"{sub}/{f}\0".encode())
+        h.update(data)
+        h.update(b"\0")
+    return h.hexdigest()[:16]
+
+
 def build_lib() -> str:
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
     tmp = f"{LIB}.{os.getpid()}.tmp"  # one per build: concurrent builds never share it
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs])
+    snap = tempfile.mkdtemp(prefix="rtod_src_")
+    try:
+        digest = _snapshot(snap)
+        srcs = [os.path.join(snap, "rtod_amd", "csrc", s) for s in HIP_SOURCES]
+        flags = [f if not f.startswith("-I") else f"-I{os.path.join(snap, 'include')}"
+                 for f in HIP_FLAGS]
+        _run([HIPCC, *flags, f'-DRTOD_SRC_HASH="{digest}"', "-shared", "-o", tmp, *srcs])
+    finally:
+        shutil.rmtree(snap, ignore_errors=True)
     os.replace(tmp, LIB)
     return LIB
 

@@ -368,7 +368,7 @@ __global__ __launch_bounds__(256) void k_lbvh_query(Dev d, Step st, Lbvh L)
                 node = L.left[node];
                 continue;
             } else {
-                atomicOr(&d.ctl->err, 8); // stack overflow: cannot happen below depth 64
+                atomicOr(&d.ctl->err, kErrLbvhStack); // cannot happen below depth 64
             }
         }
         if (sp == 0)
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(256) void k_lbvh_query(Dev d, Step st, Lbvh L)
     for (int dd = 1; dd < kWave; dd <<= 1)
         tests += __shfl_xor(tests, dd, kWave);
     if (lane_id() == 0 && tests)
-        atomicAdd(&d.ctl->pair_tests, (unsigned long long)tests);
+        atomicAdd(&cnt_of(d, st.t)->pair_tests, (unsigned long long)tests);
 }
 
 // outlier bits from the leaf counts (the slide epilogue, k_finish, follows)

@@ -139,6 +139,8 @@ struct rtod_ctx {
     std::vector<void *> dev_allocs;
     int64_t dev_bytes = 0, pinned_bytes = 0;
     bool primed = false;
+    int64_t fail_at = -1;  // RTOD_FAIL_AT_SLIDE: collecting this slide reports a
+                           // failure (a host-side test of the error paths)
     bool staged = false;   // phase API: new points copied, kernels pending
     int phase = 0;         // phase API progress
     int64_t host_t = 0;
@@ -680,23 +682,40 @@ struct CreateOpts {
     bool batch_member = false;    // a stream of a batch (its own slide kernels)
 };
 
-// The kernel-argument structs as the device pass compiled them.  Host and
-// device passes of one build see the same headers; a library whose passes
-// saw different ones (sources edited while it compiled) would hand its
-// kernels misplaced pointers, so it is refused before any launch.
-__device__ int32_t g_abi_layout[3] = {(int32_t)sizeof(Dev), (int32_t)sizeof(Step),
-                                            (int32_t)offsetof(Dev, ctl)};
+// The kernel-argument structs as the device pass compiled them: the size of
+// every struct that reaches a kernel and the offsets of the Dev fields the
+// kernels read last, plus the hash of the source snapshot the library was
+// built from (RTOD_SRC_HASH, set by the build).  The build compiles a
+// private snapshot of the sources, so both passes see the same headers; a
+// library whose passes disagreed anyway (or whose host code was linked with
+// another device image) would hand its kernels misplaced pointers, so it is
+// refused before any launch.
+#ifndef RTOD_SRC_HASH
+#define RTOD_SRC_HASH "unhashed"
+#endif
+constexpr int kAbiFields = 14;
+#define RTOD_ABI_FIELDS                                                                           \
+    {(int32_t)sizeof(Dev),         (int32_t)sizeof(Step),        (int32_t)sizeof(Ctl),           \
+     (int32_t)sizeof(SlideCnt),    (int32_t)sizeof(HostOut),     (int32_t)sizeof(Learn),         \
+     (int32_t)sizeof(LearnCtl),    (int32_t)sizeof(LearnShared), (int32_t)sizeof(Lbvh),          \
+     (int32_t)sizeof(BatchItem),   (int32_t)sizeof(BatchStream), (int32_t)offsetof(Dev, ctl),    \
+     (int32_t)offsetof(Dev, h_out), (int32_t)offsetof(Dev, lr)}
+__device__ int32_t g_abi_layout[kAbiFields] = RTOD_ABI_FIELDS;
+__device__ char g_src_hash[sizeof(RTOD_SRC_HASH)] = RTOD_SRC_HASH;
 
 rtod_status check_abi_layout()
 {
-    int32_t dev[3] = {0, 0, 0};
+    int32_t dev[kAbiFields] = {};
     RTOD_HIP(hipMemcpyFromSymbol(dev, HIP_SYMBOL(g_abi_layout), sizeof(dev), 0,
                                  hipMemcpyDeviceToHost));
-    const int32_t host[3] = {(int32_t)sizeof(Dev), (int32_t)sizeof(Step),
-                             (int32_t)offsetof(Dev, ctl)};
-    if (std::memcmp(dev, host, sizeof(dev)) != 0) {
+    const int32_t host[kAbiFields] = RTOD_ABI_FIELDS;
+    char hash[sizeof(RTOD_SRC_HASH)] = {};
+    RTOD_HIP(hipMemcpyFromSymbol(hash, HIP_SYMBOL(g_src_hash), sizeof(hash), 0,
+                                 hipMemcpyDeviceToHost));
+    if (std::memcmp(dev, host, sizeof(dev)) != 0 ||
+        std::memcmp(hash, RTOD_SRC_HASH, sizeof(hash)) != 0) {
         set_error("librtod.so was built from inconsistent sources: kernel argument layout "
-                  "differs between its host and device code; rebuild it");
+                  "or source hash differs between its host and device code; rebuild it");
         return RTOD_ERR_INTERNAL;
     }
     return RTOD_OK;
@@ -777,6 +796,12 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
     d.tile_par = (!opt.batch_member && use_run_merge(c->S, c->CH)) ? 1 : 0;
     d.unit_max = opt.batch_member ? kUnitCap : unit_max_single(c->dim);
     d.wait_spins = kClassifyWaitSpins;
+    if (const char *e = std::getenv("RTOD_FAIL_AT_SLIDE")) {
+        char *end = nullptr;
+        const long long v = std::strtoll(e, &end, 10);
+        if (end != e && *end == '\0' && v >= 0)
+            c->fail_at = v;
+    }
     if (const char *e = std::getenv("RTOD_CLASSIFY_WAIT_SPINS")) {
         char *end = nullptr;
         const long v = std::strtol(e, &end, 10);
@@ -949,18 +974,41 @@ rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n,
     return RTOD_OK;
 }
 
+// the messages of the device error flags set in `err`
+std::string err_text(int32_t err)
+{
+    static const struct {
+        int32_t bit;
+        const char *what;
+    } kMsgs[] = {{kErrMergeSurvivor, "a window-merge survivor ranked out of range"},
+                 {kErrMergeNew, "a window-merge new point ranked out of range"},
+                 {kErrStencilRows, "a detect unit's stencil exceeded 32 rows"},
+                 {kErrClassifyWait, "a detect timed out waiting for its classify"},
+                 {kErrLbvhStack, "an LBVH query overflowed its traversal stack"},
+                 {kErrStageWait, "a fused-slide stage timed out waiting for the one before"}};
+    std::string s;
+    for (const auto &m : kMsgs)
+        if (err & m.bit)
+            s += std::string(" ") + m.what + ";";
+    return s.empty() ? std::string(" unknown flag") : s;
+}
+
 rtod_status collect(rtod_ctx *c, int32_t *outs, int32_t *n_out)
 {
     HostOut *ho = host_out_of(c, c->host_t);
+    if (c->host_t == c->fail_at) {
+        set_error("failure injected at slide " + std::to_string(c->fail_at) +
+                  " (RTOD_FAIL_AT_SLIDE)");
+        c->host_t++;
+        return RTOD_ERR_INTERNAL;
+    }
     if (ho->slide != c->host_t + 1) {
         set_error("device slide counter out of step with the host");
         return RTOD_ERR_INTERNAL;
     }
     if (ho->err != 0) {
-        set_error("device consistency check failed (code " + std::to_string(ho->err) +
-                  ((ho->err & kErrClassifyWait) ? ": a detect timed out waiting for its classify"
-                                                : "") +
-                  ")");
+        set_error("device consistency check failed (code " + std::to_string(ho->err) + ":" +
+                  err_text(ho->err) + ")");
         c->host_t++;
         return RTOD_ERR_INTERNAL;
     }
@@ -1395,8 +1443,12 @@ rtod_status slides_device_impl(rtod_ctx *c, const float *dev, int32_t n, int32_t
         launch_finish(c, c->stream, pending);
     while (s == RTOD_OK && done < n)
         s = collect_next();
-    if (s != RTOD_OK)
+    if (s != RTOD_OK) {
+        // the window may have been merged for slides that were not
+        // collected: the detector has to be primed again
         (void)hipStreamSynchronize(c->stream);
+        c->primed = false;
+    }
     return s;
 }
 
@@ -1852,7 +1904,7 @@ RTOD_API const char *rtod_status_str(rtod_status s)
 
 RTOD_API const char *rtod_last_error(void) { return g_last_error.c_str(); }
 
-RTOD_API const char *rtod_version(void) { return "rtod-mi355x 0.1 (gfx950)"; }
+RTOD_API const char *rtod_version(void) { return "rtod-mi355x 0.2 (gfx950, sources " RTOD_SRC_HASH ")"; }
 
 } // extern "C"
 

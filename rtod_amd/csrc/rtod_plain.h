@@ -666,6 +666,8 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
         cn->n_units = 0;
         cn->n_units_i = 0;
         cn->n_done = 0;
+        cn->pair_tests = 0;
+        cn->misses = 0;
     }
     if (d.ready && bid == 0 && tid < kReadyFlags)
         *ready_of(d, st.t, tid) = 0;
@@ -835,7 +837,7 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
         const int rb = bfit ? b_lo + rbl[i] : lower_bound_u64(Bk, b_lo, b_hi, key);
         const int np = i0 + l - eb + rb;
         if ((unsigned)np >= (unsigned)W) { // impossible unless state is corrupt
-            atomicOr(&d.ctl->err, 1);
+            atomicOr(&d.ctl->err, kErrMergeSurvivor);
             continue;
         }
         Nk[np] = key;
@@ -877,7 +879,7 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
         }
         const int np = j + ubA - ubE;
         if ((unsigned)np >= (unsigned)W) {
-            atomicOr(&d.ctl->err, 2);
+            atomicOr(&d.ctl->err, kErrMergeNew);
             continue;
         }
         Nk[np] = key;
@@ -1408,7 +1410,7 @@ __device__ __forceinline__ int unit_count(const Dev &d, const uint64_t *Nk, cons
     const int W = d.W, K = d.K;
     if (st.nrows > 32) { // cannot happen: the stencils stay within 32 rows
         if (lane == 0)
-            atomicOr(&d.ctl->err, 4);
+            atomicOr(&d.ctl->err, kErrStencilRows);
         st.nrows = 32;
     }
     const int bnd = row_bounds<DIM>(d, Nk, Nk4, s_samp, st, w_a);
@@ -1694,15 +1696,14 @@ struct PubIn {
 __device__ __forceinline__ PubIn load_pub_inputs(const Dev &d, const Step &st, bool learn)
 {
     PubIn in;
-    in.pair_tests = __hip_atomic_load(&d.ctl->pair_tests, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
+    SlideCnt *cn = cnt_of(d, st.t);
+    in.pair_tests = __hip_atomic_load(&cn->pair_tests, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     in.err = __hip_atomic_load(&d.ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    in.n_cand = cnt_of(d, st.t)->n_cand;
+    in.n_cand = cn->n_cand;
     in.misses = in.train_steps = 0;
     in.train_loss = 0.f;
     if (learn) {
-        in.misses = __hip_atomic_load(&d.lr.lc->misses, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
+        in.misses = __hip_atomic_load(&cn->misses, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         in.train_steps = d.lr.ls->steps_total;
         in.train_loss = d.lr.ls->loss_last;
     }
@@ -1730,11 +1731,13 @@ __device__ __forceinline__ void publish_slide(const Dev &d, const Step &st,
         ho->used_pred = sm.use_pred ? 1 : 0;
         ho->train_steps = in.train_steps;
         ho->train_loss = in.train_loss;
-        // fallback rule: schedule the next slide by prediction only if this
-        // slide's miss rate (over units) is within the threshold
-        if (sm.n_units > 0)
-            lc->use_pred = ((float)in.misses <= d.lr.miss_thr * (float)sm.n_units) ? 1 : 0;
-        lc->misses = 0;
+        // fallback rule: schedule slide t + 2 by prediction only if this
+        // slide's miss rate (over units) is within the threshold; a slide
+        // without units keeps the previous decision
+        int32_t *dn = &lc->dec[(st.t + 2) % kDecSlots];
+        *dn = sm.n_units > 0
+                  ? (((float)in.misses <= d.lr.miss_thr * (float)sm.n_units) ? 1 : 0)
+                  : lc->dec[(st.t + 1) % kDecSlots];
     }
     ho->n_units = sm.n_units;
     ho->n_out = total;
@@ -1743,7 +1746,6 @@ __device__ __forceinline__ void publish_slide(const Dev &d, const Step &st,
     ho->err = in.err;
     if (in.err)
         atomicAnd(&d.ctl->err, ~in.err); // the flags reported here
-    d.ctl->pair_tests = 0;
     d.ctl->ticket = 0;
     d.ctl->fin_ticket = 0;
     d.ctl->t = st.t + 1;
@@ -1907,7 +1909,7 @@ __device__ __forceinline__ void finish_deferred(const Dev &d, const Step &st, in
     const SlideCnt *cn = cnt_of(d, st.t);
     sm.n_o = cn->n_units;
     sm.n_units = sm.learn ? sm.n_o + cn->n_units_i : sm.n_o;
-    sm.use_pred = sm.learn && d.lr.lc->use_pred != 0;
+    sm.use_pred = sm.learn && d.lr.lc->dec[st.t % kDecSlots] != 0;
     finish_slide<NT>(d, st, f, F, sm, fl);
 }
 
@@ -1921,7 +1923,7 @@ __device__ __forceinline__ void finish_deferred(const Dev &d, const Step &st, in
 // the stream runs it (finish_deferred).
 // LEARN: the list is the probable-outlier units [0, nO) followed by the
 // probable-inlier units (indices W-1, W-2, ...): the long full counts are
-// handed out first.  When the schedule is on (lc->use_pred), a probable-
+// handed out first.  When the schedule is on (lc->dec of the slide), a probable-
 // inlier unit (2-D/3-D) is first counted over its own cell and the
 // neighbouring cells only; the full stencil is counted only if some
 // candidate is still at <= K.  (In 1-D those cells are most of the stencil.)  Every count is exact; each unit's smallest count goes to the
@@ -1945,7 +1947,6 @@ struct DetectLds {
 // The environment variable RTOD_CLASSIFY_WAIT_SPINS, read when a context is
 // created, replaces it (0 fails every such slide: a test of the error path).
 constexpr int kClassifyWaitSpins = 1 << 20;
-constexpr int kErrClassifyWait = 8; // ctl->err flag of such a failure
 
 // bid / nblk: this workgroup and the number of workgroups of the detect
 // phase of this stream.  WAIT: the slide's classify runs in the same launch
@@ -2013,7 +2014,7 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
     const int n_o = ld_cnt(&cn->n_units);
     uint32_t *bm = bitmap_of(d, st.t);
     const int n_units = LEARN ? n_o + ld_cnt(&cn->n_units_i) : n_o;
-    const bool use_pred = LEARN && d.lr.lc->use_pred != 0;
+    const bool use_pred = LEARN && d.lr.lc->dec[st.t % kDecSlots] != 0;
     const int busy = min(nblk, max(1, (n_units + NW - 1) / NW));
     if (bid >= busy)
         return;
@@ -2094,9 +2095,9 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
     __syncthreads();
     if (tid == 0) {
         if (s_tests)
-            atomicAdd(&d.ctl->pair_tests, s_tests);
+            atomicAdd(&cn->pair_tests, s_tests);
         if (LEARN && s_miss)
-            atomicAdd(&d.lr.lc->misses, s_miss);
+            atomicAdd(&cn->misses, s_miss);
         if (fin) {
             __threadfence();
             s_last = (atomicAdd(&d.ctl->ticket, 1u) == (unsigned)busy - 1);

@@ -58,6 +58,8 @@ struct SlideCnt {
     alignas(128) int32_t n_units; // cell units (learned: the probable-outlier bucket)
     alignas(128) int32_t n_units_i; // learned: the probable-inlier bucket
     alignas(128) int32_t n_done; // one launch per slide: classify workgroups done
+    alignas(128) unsigned long long pair_tests; // predicate evaluations of the slide
+    alignas(128) int32_t misses; // learned: mispredicted units of the slide
 };
 
 struct Ctl {
@@ -66,7 +68,6 @@ struct Ctl {
                            // slide from here, since t advances while it runs
     uint32_t ticket;       // last-workgroup ticket of the detect kernel
     uint32_t fin_ticket;   // parts done of a deferred slide epilogue
-    unsigned long long pair_tests;
     int32_t err;
     int32_t pad;
     SlideCnt cnt[3];       // slide t's at cnt[t % 3]; reset by its window merge
@@ -87,9 +88,15 @@ struct HostOut {
 };
 
 // Device-side state of the learned scheduler: per stream ...
+// The fallback decision for slide t is made by the epilogue of slide t - 2
+// (from that slide's miss rate) and kept in dec[t % kDecSlots]: that
+// epilogue is complete before slide t's detect starts both when slides run
+// one per call and when they are pipelined (where the epilogue of slide
+// t - 1 runs beside the detect of slide t), so the schedule of every slide
+// is the same either way.
+constexpr int kDecSlots = 4;
 struct LearnCtl {
-    int32_t use_pred;      // schedule the current slide by prediction
-    int32_t misses;        // mispredictions (current slide)
+    int32_t dec[kDecSlots]; // 1: slide t is scheduled by prediction (dec[t % 4])
 };
 
 // ... and of the predictor, which several streams may share (a batch).
@@ -172,6 +179,15 @@ struct Dev {
     int64_t h_out_bytes; // their stride
     Learn lr;
 };
+
+// Error flags a slide's kernels raise in ctl->err (one bit each); the slide
+// epilogue reports them in HostOut::err and the host maps each to a message.
+constexpr int kErrMergeSurvivor = 1; // window merge: a survivor's rank out of range
+constexpr int kErrMergeNew = 2;      // window merge: a new point's rank out of range
+constexpr int kErrStencilRows = 4;   // detect: a unit's stencil wider than 32 rows
+constexpr int kErrClassifyWait = 8;  // detect: the bounded wait for its classify ran out
+constexpr int kErrLbvhStack = 16;    // LBVH query: traversal stack overflow
+constexpr int kErrStageWait = 32;    // fused slide: a stage's bounded wait ran out
 
 // the host-out buffer of slide t: consecutive slides rotate through
 // kHostOuts buffers, so up to that many slides can be in flight while the

@@ -141,10 +141,11 @@ def test_learned_fallback_rule():
     W, S, T = 4000, 200, 25
     xs = O.gen_gmm(41, W + T * S, 3)
     want = O.run_stream(xs, W, S, T, 2.0, 20, "grid")
-    # threshold 1: every slide after the first is scheduled by prediction
+    # threshold 1: every slide after the first two (slide t follows the
+    # decision slide t - 2 made) is scheduled by prediction
     got, st = _learned_stream(xs, W, S, T, 2.0, 20, miss_threshold=1.0)
     assert_same(got, want, "threshold 1")
-    assert st["fallbacks"] == 1 and st["sched_kind"] == R.SCHED_LEARNED
+    assert st["fallbacks"] == 2 and st["sched_kind"] == R.SCHED_LEARNED
     # a tiny threshold with a frozen untrained network: never engages
     p0 = np.zeros(R.MLP_NPARAMS, np.float32)
     p0[-1] = 4.0  # every candidate a probable inlier: many misses

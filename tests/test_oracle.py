@@ -346,3 +346,71 @@ def test_mlp_bf16_emulation_close_to_fp32():
     _, gb = O.mlp_grad(P, X, y, bf16=False)
     assert 0 < np.abs(ga - gb).max() < 0.25 * np.abs(gb).max()
     assert np.dot(ga, gb) / (np.linalg.norm(ga) * np.linalg.norm(gb)) > 0.99
+
+
+# ------------------------------------------- contract vs reference semantics
+def _status_diff_brute(win, R, K):
+    """Points whose outlier status differs between the contract and the
+    reference's FP64 strict rule, by brute force under both."""
+    a = set(O.brute(win, R, K, semantics=0).tolist())
+    b = set(O.brute(win, R, K, semantics=1).tolist())
+    return len(a ^ b)
+
+
+def test_semantics_diff_lattice_closed_form():
+    """x_i = i/4, R = 1/2 exactly: the contract counts the two lattice points
+    at distance exactly R, the reference's strict test does not.  Interior
+    points: 5 vs 3 neighbours (K = 4 splits them), the two points at each end
+    are outliers either way; every count differs; 2 (W - 2) ordered pairs sit
+    at distance exactly R."""
+    W = 256
+    win = (np.arange(W, dtype=np.float64) * 0.25).astype(np.float32).reshape(-1, 1)
+    got = O.semantics_diff(win, 0.5, 4)
+    assert got == {"status": W - 4, "counts": W, "pairs": 2 * (W - 2)}
+    assert got["status"] == _status_diff_brute(win, 0.5, 4)
+    # 3-D integer lattice, R = 1: the six axis neighbours sit exactly at R
+    g = np.arange(6, dtype=np.float32)
+    cube = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    got3 = O.semantics_diff(cube, 1.0, 3)
+    assert got3["status"] == _status_diff_brute(cube, 1.0, 3)
+    inner = sum(1 for p in cube for a in range(3) for s in (-1, 1)
+                if 0 <= p[a] + s <= 5)  # ordered pairs at distance exactly 1
+    assert got3["pairs"] == inner
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_semantics_diff_equals_brute(dim):
+    """Windows with many exact ties (values on a 1/8 grid, R a grid distance)
+    and continuous ones: the fast counter's status differences equal the
+    symmetric difference of the two brute-force outlier sets."""
+    rng = np.random.default_rng(100 + dim)
+    for trial in range(6):
+        W = int(rng.integers(300, 1500))
+        if trial % 2 == 0:
+            win = (rng.integers(0, 40, (W, dim)) / 8.0).astype(np.float32)
+            R = float(rng.choice([0.5, 0.625, 1.0]))
+        else:
+            win = rng.normal(0, 3, (W, dim)).astype(np.float32)
+            R = float(rng.uniform(0.2, 1.5))
+        K = int(rng.integers(1, 40))
+        assert O.semantics_diff(win, R, K)["status"] == _status_diff_brute(win, R, K), (dim, trial)
+
+
+def test_semantics_diff_on_configs():
+    """The measured 1-D and 3-D streams, every slide: how often the build's
+    FP32 `d2 <= R*R` and the reference's FP64 `|dx| < R` / `sqdist < R2`
+    (optixScan.cu:170-189) give a different outlier set.  C1 and C2: never
+    (C2 has pairs on which the rules differ, none of which moves a count
+    across K).  C3: two points, in two slides; those slides are re-checked by
+    brute force under both rules."""
+    want = {"walk1d-W10k-S500": 0, "walk1d-W100k-S5k": 0, "gmm3d-W10k-S500": 2}
+    for wl in (O.C1, O.C2, O.C3):
+        xs = wl.data()
+        tot, per = O.semantics_diff_stream(xs, wl.W, wl.S, wl.T, wl.R, wl.K)
+        assert tot["slides_checked"] == wl.T
+        assert tot["status"] == want[wl.name], (wl.name, tot)
+        for t in np.nonzero(per > 0)[0][:2] + 1:
+            idx = np.arange(t * wl.S, t * wl.S + wl.W)
+            win = np.empty((wl.W, wl.dim), np.float32)
+            win[idx % wl.W] = xs[idx]
+            assert _status_diff_brute(win, wl.R, wl.K) == per[t - 1], (wl.name, t)
