@@ -73,15 +73,22 @@ typedef struct rtod_config {
 } rtod_config;
 
 /* Per-slide time breakdown in milliseconds, the reference Timer buckets
- * (timer.h:16-24, printed by showTimeNew timer.h:102-117). */
+ * (timer.h:16-24, printed by showTimeNew timer.h:102-117).  Filled on every
+ * rtod_slide / rtod_slide_device call; the device parts come from the
+ * start stamps (device wall clock) of the slide's kernels. */
 typedef struct rtod_times {
-    double copy_h2d;       /* new slide into the device ring              */
-    double prepare_cell;   /* cell keys, LDS radix sort, per-cell counts,
-                              dense fast-accept, candidate compaction     */
-    double build_index;    /* cell-stencil index / LBVH build             */
+    double copy_h2d;       /* the new slide copied into pinned staging (the
+                              device reads it from there inside the ingest) */
+    double prepare_cell;   /* ingest: cell keys, LDS radix sort, run merge;
+                              classify: dense fast-accept, candidate units  */
+    double build_index;    /* cell-stencil index (the cell-sorted window
+                              merge) / LBVH build                          */
     double detect_outlier; /* range counts with early exit                */
-    double copy_d2h;       /* outlier compaction + list to host           */
+    double copy_d2h;       /* outlier compaction + list into host memory,
+                              and the copy into the caller's array         */
     double total;          /* host wall clock, entry to list in host mem  */
+    double launch;         /* total minus the five buckets: kernel launch,
+                              dispatch and the host's completion polling   */
 } rtod_times;
 
 /* Counters mirroring the reference's DEBUG_INFO / "BVH Node / slide" output
@@ -162,6 +169,19 @@ RTOD_API rtod_status rtod_slides_device(rtod_ctx *ctx, const float *dev_pts, int
                                         int32_t *out_offsets, int32_t *out_slots,
                                         int64_t slots_cap);
 
+/* rtod_slides_device with the slides in host memory: pts [n_slides*S][dim]
+ * row-major, slide i at pts + i*S*dim.  Each slide is copied into one of the
+ * context's pinned staging slots just before the launch that reads it (its
+ * ingest, up to three slides ahead of the slide being detected), and the
+ * kernels read it from there over PCIe: the reference's per-slide H2D copy
+ * (optixScan.cpp:1051-1058) inside the pipelined run, overlapping the
+ * detection of earlier slides.  Outputs, errors and pipelining as
+ * rtod_slides_device; the results equal those of n_slides rtod_slide calls.
+ * No counterpart in the reference, whose loop (optixScan.cpp:1045-1110)
+ * runs one slide per iteration. */
+RTOD_API rtod_status rtod_slides(rtod_ctx *ctx, const float *pts, int32_t n_slides,
+                                 int32_t *out_offsets, int32_t *out_slots, int64_t slots_cap);
+
 /* Phase-level entry points: the same slide split where the reference's loop
  * body calls out.  Call in this order; rtod_slide == all five.
  *   rtod_copy_new_points  <- cudaMemcpy of the new slide (optixScan.cpp:1051-1058)
@@ -191,6 +211,15 @@ RTOD_API rtod_status rtod_get_stats(const rtod_ctx *ctx, rtod_stats *out);
 RTOD_API rtod_status rtod_kernel_times(const rtod_ctx *ctx, const char **names, double *avg_ms,
                                        int64_t *launches, int32_t *n);
 RTOD_API rtod_status rtod_reset_kernel_times(rtod_ctx *ctx);
+
+/* Kernel durations (ms, device wall clock) of the last slide run one per
+ * call (rtod_slide / rtod_slide_device), from the start stamp each of its
+ * kernels' first workgroup records: out[0] ingest (tile sorts and batch
+ * merges), [1] window merge, [2] classify, [3] LBVH build (0 with the cell
+ * stencil), [4] detect, [5] epilogue (list into host memory); n (in) >= 6.
+ * The stamps rtod_times is built from; RTOD_ERR_STATE before a slide.  No
+ * reference counterpart beyond Timer's buckets (timer.h:16-24). */
+RTOD_API rtod_status rtod_stage_times(const rtod_ctx *ctx, double *out, int32_t n);
 
 /* ---- Learned scheduler (RTOD_SCHED_LEARNED) ------------------------------
  * No reference counterpart: the reference schedules every range count the

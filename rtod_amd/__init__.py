@@ -49,7 +49,7 @@ class _Config(C.Structure):
 class _Times(C.Structure):
     _fields_ = [("copy_h2d", C.c_double), ("prepare_cell", C.c_double),
                 ("build_index", C.c_double), ("detect_outlier", C.c_double),
-                ("copy_d2h", C.c_double), ("total", C.c_double)]
+                ("copy_d2h", C.c_double), ("total", C.c_double), ("launch", C.c_double)]
 
 
 class _Stats(C.Structure):
@@ -85,6 +85,7 @@ SIGNATURES = [
                                     C.POINTER(_Times)]),
     ("rtod_slides_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, _I32P, _I32P,
                                      C.c_int64]),
+    ("rtod_slides", C.c_int, [C.c_void_p, _F32P, C.c_int32, _I32P, _I32P, C.c_int64]),
     ("rtod_copy_new_points", C.c_int, [C.c_void_p, _F32P]),
     ("rtod_prepare_cell", C.c_int, [C.c_void_p]),
     ("rtod_build_index", C.c_int, [C.c_void_p]),
@@ -95,6 +96,7 @@ SIGNATURES = [
                                     C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                     C.POINTER(C.c_int32)]),
     ("rtod_reset_kernel_times", C.c_int, [C.c_void_p]),
+    ("rtod_stage_times", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int32]),
     ("rtod_get_predictor", C.c_int, [C.c_void_p, _F32P, C.c_int32]),
     ("rtod_set_predictor", C.c_int, [C.c_void_p, _F32P, C.c_int32, C.c_int32]),
     ("rtod_mlp_forward", C.c_int, [C.c_int32, _F32P, _F32P, C.c_int32, _F32P]),
@@ -151,6 +153,7 @@ class Times:
     detect_outlier: float
     copy_d2h: float
     total: float
+    launch: float  # total minus the five buckets (launch, dispatch, polling)
 
 
 class Detector:
@@ -218,11 +221,23 @@ class Detector:
                                         slots.size))
         return [slots[offs[i]:offs[i + 1]].copy() for i in range(n_slides)]
 
+    def slides(self, pts: np.ndarray, n_slides: int) -> list[np.ndarray]:
+        """n_slides consecutive slides from host memory (slide i = rows
+        i*S .. (i+1)*S of pts), pipelined, each staged into pinned memory just
+        before its ingest; the same lists as n_slides slide() calls."""
+        pts = np.ascontiguousarray(pts, np.float32).reshape(-1)
+        if pts.size < n_slides * self.S * self.dim:
+            raise ValueError("pts holds fewer than n_slides slides")
+        offs = np.zeros(n_slides + 1, np.int32)
+        slots = np.empty(max(1, n_slides * self.W), np.int32)
+        _check(lib().rtod_slides(self._h, pts, n_slides, offs, slots, slots.size))
+        return [slots[offs[i]:offs[i + 1]].copy() for i in range(n_slides)]
+
     @property
     def times(self) -> Times:
         t = self._times
         return Times(t.copy_h2d, t.prepare_cell, t.build_index, t.detect_outlier, t.copy_d2h,
-                     t.total)
+                     t.total, t.launch)
 
     # -- phase-level API (reference loop body) ------------------------------
     def copy_new_points(self, new_pts: np.ndarray) -> None:
@@ -273,6 +288,14 @@ class Detector:
 
     def reset_kernel_times(self) -> None:
         _check(lib().rtod_reset_kernel_times(self._h))
+
+    STAGES = ("ingest", "window_merge", "classify", "lbvh_build", "detect", "epilogue")
+
+    def stage_times(self) -> dict:
+        """Kernel durations (ms, device clock) of the last slide run one per call."""
+        out = (C.c_double * 6)()
+        _check(lib().rtod_stage_times(self._h, out, 6))
+        return dict(zip(self.STAGES, list(out)))
 
 
 def run_stream(xs: np.ndarray, W: int, S: int, T: int, R: float, K: int, **kw):

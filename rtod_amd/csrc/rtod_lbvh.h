@@ -76,6 +76,7 @@ template <int DIM> __device__ __forceinline__ uint64_t morton_of_key(uint64_t ke
 template <int DIM>
 __global__ __launch_bounds__(256) void k_lbvh_codes(Dev d, Step st, Lbvh L)
 {
+    stamp_start(d, kTsIndex);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int U = *L.n_cand;
     if (i >= U)
@@ -327,6 +328,7 @@ template <int DIM>
 __global__ __launch_bounds__(256) void k_lbvh_query(Dev d, Step st, Lbvh L)
 {
     __shared__ int s_stack[256 * kLbvhStack];
+    stamp_start(d, kTsDetect);
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     const int U = *L.n_cand;
     if (U == 0)

@@ -129,7 +129,8 @@ struct rtod_ctx {
                               // slots of kReplaySkip slides from this one
     Dev d{};
     float *d_prime = nullptr; // device copy of the initial window
-    float *h_in = nullptr;
+    float *h_in = nullptr;   // stage_slots pinned staging slots of S*dim floats
+    int stage_slots = 2;     // slide t is staged in slot t % stage_slots
     HostOut *h_out = nullptr;
     const float *dev_src = nullptr; // device-resident new slide of this call
     bool lbvh = false;
@@ -146,6 +147,7 @@ struct rtod_ctx {
     int64_t host_t = 0;
     hipEvent_t ev[8] = {};
     bool ride = false; // pipelined slides: next ingest rides in core launches
+    double wall_khz = 0.0; // the device wall clock's rate (ticks per ms)
     double ksum[kKCount] = {};
     int64_t kcnt[kKCount] = {};
     // stats
@@ -362,9 +364,12 @@ void launch_detect(rtod_ctx *c, hipStream_t s, const Step &st, const Step *runs 
 }
 
 // slide st's epilogue on its own, after a detect that left it
-void launch_finish(rtod_ctx *c, hipStream_t s, const Step &st)
+// (stamps: a slide run one per call, whose epilogue publishes its kernels'
+// start stamps)
+void launch_finish(rtod_ctx *c, hipStream_t s, const Step &st, bool stamps = false)
 {
-    hipLaunchKernelGGL(k_finish, dim3(c->d.fin_parts), dim3(256), 0, s, c->d, st);
+    hipLaunchKernelGGL(k_finish, dim3(c->d.fin_parts), dim3(256), 0, s, c->d, st,
+                       stamps ? 1 : 0);
 }
 
 // Stage launches (cell-stencil index, the ingest riding), learned
@@ -500,7 +505,7 @@ void launch_lbvh_build(rtod_ctx *c, hipStream_t s, const Step &st)
         hipLaunchKernelGGL(k_lbvh_boxes<3>, gw, b, 0, s, c->d, st, L);
 }
 
-void launch_lbvh_detect(rtod_ctx *c, hipStream_t s, const Step &st)
+void launch_lbvh_detect(rtod_ctx *c, hipStream_t s, const Step &st, bool stamps = false)
 {
     Lbvh L = c->L;
     L.n_cand = &c->d.ctl->cnt[st.t % kSlideSets].n_cand;
@@ -512,7 +517,7 @@ void launch_lbvh_detect(rtod_ctx *c, hipStream_t s, const Step &st)
     else
         hipLaunchKernelGGL(k_lbvh_query<3>, gw, b, 0, s, c->d, st, L);
     hipLaunchKernelGGL(k_lbvh_select, gw, dim3(kDetectNT), 0, s, c->d, st, L);
-    launch_finish(c, s, st);
+    launch_finish(c, s, st, stamps);
 }
 
 // slide t's quantities; src: its new points in HBM, or nullptr for the
@@ -526,7 +531,7 @@ Step make_step_at(const rtod_ctx *c, uint64_t t, const float *src)
     st.par = (int)(t & 1);
     st.eold = (int)(t % ne);
     st.enew = (int)(((uint64_t)c->m + t) % ne);
-    st.buf = (int)(t & 1);
+    st.buf = (int)(t % (uint64_t)c->stage_slots);
     st.src = src ? src : c->h_in + (size_t)st.buf * c->S * c->dim;
     return st;
 }
@@ -563,7 +568,8 @@ void launch_ingest_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t 
 // to the next slide's window merge (a run's last slide must not defer),
 // else it follows the detect in a launch of its own.
 void launch_core_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t *ev,
-                       const Step *next, const Step *prev = nullptr, bool defer = false)
+                       const Step *next, const Step *prev = nullptr, bool defer = false,
+                       bool stamps = false)
 {
     if (ev)
         (void)hipEventRecord(ev[2], s);
@@ -578,7 +584,7 @@ void launch_core_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t *e
         launch_lbvh_build(c, s, st);
         if (ev)
             (void)hipEventRecord(ev[5], s);
-        launch_lbvh_detect(c, s, st);
+        launch_lbvh_detect(c, s, st, stamps);
         if (ev)
             (void)hipEventRecord(ev[6], s);
         if (runs)
@@ -586,7 +592,7 @@ void launch_core_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t *e
     } else {
         launch_detect(c, s, st, runs ? next : nullptr, false);
         if (!defer)
-            launch_finish(c, s, st);
+            launch_finish(c, s, st, stamps);
         if (ev) {
             (void)hipEventRecord(ev[5], s);
             (void)hipEventRecord(ev[6], s);
@@ -598,7 +604,7 @@ void launch_slide(rtod_ctx *c, hipStream_t s, hipEvent_t *ev)
 {
     const Step st = make_step(c);
     launch_ingest_stage(c, s, st, ev);
-    launch_core_stage(c, s, st, ev, nullptr);
+    launch_core_stage(c, s, st, ev, nullptr, nullptr, false, true);
 }
 
 rtod_status validate(const rtod_config *cfg)
@@ -830,9 +836,11 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
     if (!opt.batch_member) // a batch's streams never run one launch per slide
         RTOD_TRY(dalloc(c, &d.ready, (size_t)kSlideSets * kReadyFlags * kReadyStride));
     RTOD_TRY(dalloc(c, &d.ctl, 1));
+    RTOD_TRY(dalloc(c, &d.tstamp, (size_t)kTsCount));
     RTOD_TRY(dalloc(c, &c->d_prime, dim * W));
     RTOD_HIP(hipMemsetAsync(d.bitmap, 0, sizeof(uint32_t) * kSlideSets * d.bm_words, c->stream));
     RTOD_HIP(hipMemsetAsync(d.ctl, 0, sizeof(Ctl), c->stream));
+    RTOD_HIP(hipMemsetAsync(d.tstamp, 0, sizeof(uint64_t) * kTsCount, c->stream));
     if (d.ready)
         RTOD_HIP(hipMemsetAsync(d.ready, 0,
                                 sizeof(int32_t) * kSlideSets * kReadyFlags * kReadyStride,
@@ -858,6 +866,11 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
     }
     c->learned = cfg->sched_kind == RTOD_SCHED_LEARNED;
     d.lr.on = c->learned ? 1 : 0;
+    {
+        int khz = 0;
+        RTOD_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+        c->wall_khz = khz > 0 ? (double)khz : 100000.0;
+    }
     if (c->learned) {
         Learn &lr = d.lr;
         lr.miss_thr = cfg->miss_threshold > 0.f ? cfg->miss_threshold : 0.25f;
@@ -886,7 +899,12 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
         RTOD_HIP(hipMemsetAsync(lr.adam_m, 0, sizeof(float) * kMlpNP, c->stream));
         RTOD_HIP(hipMemsetAsync(lr.adam_v, 0, sizeof(float) * kMlpNP, c->stream));
     }
-    const size_t in_bytes = 2 * S * dim * sizeof(float);
+    // staging: a batch's streams use two slots (slide t in t & 1); a single
+    // stream kStageSlots, so that pipelined slides from host memory can be
+    // staged three ahead of the slide being detected (rtod_slides)
+    c->stage_slots =
+        (opt.batch_member || S * dim * sizeof(float) > kStageSlotCap) ? 2 : kStageSlots;
+    const size_t in_bytes = (size_t)c->stage_slots * S * dim * sizeof(float);
     // kHostOuts host-out buffers (slide t: t % kHostOuts), each aligned to
     // 256 bytes
     const size_t out_bytes = (sizeof(HostOut) + W * sizeof(int32_t) + 255) & ~(size_t)255;
@@ -984,8 +1002,7 @@ std::string err_text(int32_t err)
                  {kErrMergeNew, "a window-merge new point ranked out of range"},
                  {kErrStencilRows, "a detect unit's stencil exceeded 32 rows"},
                  {kErrClassifyWait, "a detect timed out waiting for its classify"},
-                 {kErrLbvhStack, "an LBVH query overflowed its traversal stack"},
-                 {kErrStageWait, "a fused-slide stage timed out waiting for the one before"}};
+                 {kErrLbvhStack, "an LBVH query overflowed its traversal stack"}};
     std::string s;
     for (const auto &m : kMsgs)
         if (err & m.bit)
@@ -1116,9 +1133,11 @@ void accumulate_kernel_times(rtod_ctx *c, double *phase)
         c->kcnt[kKDetect]++;
     }
     if (phase) {
-        phase[0] = (double)ms[0] + ms[1] + ms[2] + ms[3];     // prepare cell
-        phase[1] = c->lbvh ? (double)ms[4] : 0.0;              // build index
-        phase[2] = c->lbvh ? (double)ms[5] : (double)ms[4];    // detect (+ compaction)
+        // the cell-sorted window (merge) is the cell-stencil index; with the
+        // LBVH it only feeds the grid filter and the index is the LBVH
+        phase[0] = (double)ms[0] + ms[1] + ms[3] + (c->lbvh ? ms[2] : 0.f); // prepare cell
+        phase[1] = c->lbvh ? (double)ms[4] : (double)ms[2];                 // build index
+        phase[2] = c->lbvh ? (double)ms[5] : (double)ms[4];                 // detect
     }
 }
 
@@ -1167,20 +1186,57 @@ rtod_status guard_trainer(rtod_ctx *c, int64_t t, hipStream_t s)
     return RTOD_OK;
 }
 
+rtod_status wait_slide(rtod_ctx *c, int64_t t);
+
+// the device stamps of a slide run one per call (its kernels' starts, in
+// its host-out buffer) as the reference's buckets, in ms: prepare cell (the
+// ingest, the batch merges and the classify; with the LBVH also the window
+// merge), build index (the cell-sorted window merge, or the LBVH build) and
+// detect; out[3] the epilogue (list into host memory)
+void stamp_phases(const rtod_ctx *c, const HostOut *ho, double *phase)
+{
+    const double k = 1.0 / c->wall_khz;
+    auto span = [&](int a, int b) {
+        return ho->ts[b] >= ho->ts[a] ? (double)(ho->ts[b] - ho->ts[a]) * k : 0.0;
+    };
+    if (c->lbvh) {
+        phase[0] = span(kTsIngest, kTsIndex);
+        phase[1] = span(kTsIndex, kTsDetect);
+    } else {
+        phase[0] = span(kTsIngest, kTsMerge) + span(kTsClassify, kTsDetect);
+        phase[1] = span(kTsMerge, kTsClassify);
+    }
+    phase[2] = span(kTsDetect, kTsFinish);
+    phase[3] = span(kTsFinish, kTsPublished);
+}
+
 // One slide whose new points are already where the ingest kernel reads them
-// (pinned staging, or c->dev_src in HBM).
+// (pinned staging, or c->dev_src in HBM).  The host waits for the slide's
+// stamp in its host-out buffer (polling, not a stream synchronisation: that
+// costs ~5 us more per slide); the breakdown comes from the kernels' device
+// stamps.  RTOD_FLAG_PHASE_TIMES also records HIP events between the
+// kernels (per-kernel averages, rtod_kernel_times) and waits for them.
 rtod_status run_slide(rtod_ctx *c, double t_entry, double t_h2d, int32_t *outs, int32_t *n_out,
                       rtod_times *times)
 {
-    double phase[3] = {0, 0, 0};
+    double phase[4] = {0, 0, 0, 0};
     const bool timed = (c->cfg.flags & RTOD_FLAG_PHASE_TIMES) != 0;
     RTOD_TRY(guard_trainer(c, c->host_t, c->stream));
     launch_slide(c, c->stream, timed ? c->ev : nullptr);
     RTOD_HIP(hipGetLastError());
-    RTOD_HIP(hipStreamSynchronize(c->stream));
-    if (timed)
-        accumulate_kernel_times(c, phase);
-    const double t_dev = now_ms();
+    const rtod_status ws = wait_slide(c, c->host_t);
+    if (ws != RTOD_OK) { // the launches ended without publishing the slide
+        (void)hipStreamSynchronize(c->stream);
+        c->primed = false;
+        return ws;
+    }
+    if (timed) {
+        RTOD_HIP(hipStreamSynchronize(c->stream));
+        double unused[3];
+        accumulate_kernel_times(c, unused);
+    }
+    stamp_phases(c, host_out_of(c, c->host_t), phase);
+    const double t_col = now_ms();
     RTOD_TRY(collect(c, outs, n_out));
     const double t_end = now_ms();
     RTOD_TRY(maybe_train(c));
@@ -1189,8 +1245,10 @@ rtod_status run_slide(rtod_ctx *c, double t_entry, double t_h2d, int32_t *outs, 
         times->prepare_cell = phase[0];
         times->build_index = phase[1];
         times->detect_outlier = phase[2];
-        times->copy_d2h = t_end - t_dev;
+        times->copy_d2h = phase[3] + (t_end - t_col);
         times->total = t_end - t_entry;
+        times->launch = times->total - times->copy_h2d - times->prepare_cell -
+                        times->build_index - times->detect_outlier - times->copy_d2h;
     }
     return RTOD_OK;
 }
@@ -1219,7 +1277,7 @@ rtod_status slide_impl(rtod_ctx *c, const float *new_pts, int32_t *outs, int32_t
     RTOD_TRY(slide_checks(c, new_pts, outs, n_out));
     RTOD_HIP(hipSetDevice(c->device));
     const size_t nf = (size_t)c->S * c->dim;
-    std::memcpy(c->h_in + (size_t)(c->host_t & 1) * nf, new_pts, nf * sizeof(float));
+    std::memcpy(c->h_in + (size_t)(c->host_t % c->stage_slots) * nf, new_pts, nf * sizeof(float));
     c->dev_src = nullptr;
     return run_slide(c, t_entry, now_ms(), outs, n_out, times);
 }
@@ -1270,8 +1328,36 @@ rtod_status wait_slide(rtod_ctx *c, int64_t t)
     }
 }
 
-// n slides from HBM on the context's stream, up to kHostOuts in flight: the
-// host enqueues slide i once slide i - kHostOuts is collected (polling its
+// Where the slides of a pipelined run come from: HBM (dev, slide i at
+// dev + i*S*dim) or host memory (host, same layout).  Host slides are copied
+// into pinned staging slot (t0 + i) % stage_slots just before the launch
+// whose ingest reads them (zero-copy over PCIe): the copy is inside the
+// run, as the reference's per-slide H2D is inside its loop
+// (optixScan.cpp:1051-1058).
+struct SlideSrc {
+    const float *dev = nullptr;
+    const float *host = nullptr;
+};
+
+const float *slide_src(const rtod_ctx *c, const SlideSrc &src, int64_t t0, int32_t i)
+{
+    const size_t per = (size_t)c->S * c->dim;
+    if (src.dev)
+        return src.dev + (size_t)i * per;
+    return c->h_in + (size_t)((t0 + i) % c->stage_slots) * per;
+}
+
+void stage_slide(rtod_ctx *c, const SlideSrc &src, int64_t t0, int32_t i)
+{
+    if (!src.host)
+        return;
+    const size_t per = (size_t)c->S * c->dim;
+    std::memcpy(c->h_in + (size_t)((t0 + i) % c->stage_slots) * per,
+                src.host + (size_t)i * per, per * sizeof(float));
+}
+
+// n slides from HBM or host memory on the context's stream, up to kHostOuts
+// in flight: the host enqueues slide i once slide i - kHostOuts is collected (polling its
 // stamp), and slide i+1's ingest stage rides in slide i's core launches
 // (ride_along; else it follows them).  The results are those of n
 // rtod_slide_device calls.  On an error the stream is drained first.
@@ -1283,17 +1369,16 @@ rtod_status wait_slide(rtod_ctx *c, int64_t t)
 // launch, k_slide_stage (plain schedule), or two, the classify stage and the
 // detect stage (learned schedule), and the last slide's epilogue follows in
 // a launch of its own.  Up to kHostOuts slides are in flight.
-rtod_status slides_two_stage(rtod_ctx *c, const float *dev, int32_t n, int32_t *offs,
+rtod_status slides_two_stage(rtod_ctx *c, const SlideSrc &src, int32_t n, int32_t *offs,
                              int32_t *slots)
 {
     const int64_t t0 = c->host_t;
-    const size_t per = (size_t)c->S * c->dim;
     const bool timed = (c->cfg.flags & RTOD_FLAG_PHASE_TIMES) != 0;
     const bool runs = c->d.cpb > 1;
     const bool fused = !c->learned && c->d.ready; // plain: one launch per slide
     hipStream_t s = c->stream;
     auto step = [&](int32_t i) {
-        return make_step_at(c, (uint64_t)(t0 + i), dev + (size_t)i * per);
+        return make_step_at(c, (uint64_t)(t0 + i), slide_src(c, src, t0, i));
     };
     int64_t used = 0;
     int32_t done = 0;
@@ -1308,6 +1393,8 @@ rtod_status slides_two_stage(rtod_ctx *c, const float *dev, int32_t n, int32_t *
     };
     // prologue: slide 0's ingest; slide 0's window merge with slide 1's
     // tile sorts; slide 1's run merge; slide 2's tile sorts
+    for (int32_t i = 0; i < 3 && i < n; ++i)
+        stage_slide(c, src, t0, i);
     const Step s0 = step(0);
     launch_ingest_stage(c, s, s0, nullptr);
     if (n > 1) {
@@ -1336,6 +1423,8 @@ rtod_status slides_two_stage(rtod_ctx *c, const float *dev, int32_t n, int32_t *
         const Step nx3 = i + 3 < n ? step(i + 3) : Step{};
         const StageParts parts{i > 0 ? &prev : nullptr, i + 1 < n ? &nx1 : nullptr,
                                (i + 2 < n && runs) ? &nx2 : nullptr, i + 3 < n ? &nx3 : nullptr};
+        if (i + 3 < n) // its tile sorts ride in this launch
+            stage_slide(c, src, t0, i + 3);
         if (timed)
             (void)hipEventRecord(c->ev[2], s);
         if (fused) {
@@ -1383,13 +1472,12 @@ rtod_status slides_two_stage(rtod_ctx *c, const float *dev, int32_t n, int32_t *
     return st;
 }
 
-rtod_status slides_device_impl(rtod_ctx *c, const float *dev, int32_t n, int32_t *offs,
-                               int32_t *slots)
+rtod_status slides_impl(rtod_ctx *c, const SlideSrc &src, int32_t n, int32_t *offs,
+                        int32_t *slots)
 {
     if (!c->lbvh && c->ride)
-        return slides_two_stage(c, dev, n, offs, slots);
+        return slides_two_stage(c, src, n, offs, slots);
     const int64_t t0 = c->host_t;
-    const size_t per = (size_t)c->S * c->dim;
     const bool timed = (c->cfg.flags & RTOD_FLAG_PHASE_TIMES) != 0;
     int64_t used = 0;
     int32_t done = 0;
@@ -1403,7 +1491,9 @@ rtod_status slides_device_impl(rtod_ctx *c, const float *dev, int32_t n, int32_t
         return maybe_train(c);
     };
     rtod_status s = RTOD_OK;
-    launch_ingest_stage(c, c->stream, make_step_at(c, (uint64_t)t0, dev), nullptr);
+    stage_slide(c, src, t0, 0);
+    launch_ingest_stage(c, c->stream, make_step_at(c, (uint64_t)t0, slide_src(c, src, t0, 0)),
+                        nullptr);
     // cell-stencil index: every slide but the last leaves its epilogue to
     // the next slide's window-merge launch; pending is that slide
     Step pending{};
@@ -1412,9 +1502,11 @@ rtod_status slides_device_impl(rtod_ctx *c, const float *dev, int32_t n, int32_t
         if (i >= kHostOuts && (s = collect_next()) != RTOD_OK)
             break;
         const uint64_t t = (uint64_t)(t0 + i);
-        const Step st = make_step_at(c, t, dev + (size_t)i * per);
+        const Step st = make_step_at(c, t, slide_src(c, src, t0, i));
         const bool more = i + 1 < n;
-        const Step nx = more ? make_step_at(c, t + 1, dev + (size_t)(i + 1) * per) : Step{};
+        if (more) // its ingest rides in (or follows) this slide's launches
+            stage_slide(c, src, t0, i + 1);
+        const Step nx = more ? make_step_at(c, t + 1, slide_src(c, src, t0, i + 1)) : Step{};
         if ((s = guard_trainer(c, (int64_t)t, c->stream)) != RTOD_OK)
             break;
         const bool fused = more && c->ride;
@@ -1554,7 +1646,43 @@ RTOD_API rtod_status rtod_slides_device(rtod_ctx *ctx, const float *dev_pts, int
             set_error("rtod_slides_device needs a device (HBM) pointer");
             return RTOD_ERR_INVALID_ARGUMENT;
         }
-        return slides_device_impl(ctx, dev_pts, n_slides, out_offsets, out_slots);
+        SlideSrc src;
+        src.dev = dev_pts;
+        return slides_impl(ctx, src, n_slides, out_offsets, out_slots);
+    } catch (...) {
+        return RTOD_ERR_INTERNAL;
+    }
+}
+
+RTOD_API rtod_status rtod_slides(rtod_ctx *ctx, const float *pts, int32_t n_slides,
+                                 int32_t *out_offsets, int32_t *out_slots, int64_t slots_cap)
+{
+    if (!ctx || !pts || !out_offsets || !out_slots) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (n_slides < 1 || slots_cap < (int64_t)n_slides * ctx->W) {
+        set_error("n_slides must be >= 1 and out_slots hold n_slides * W entries");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (!ctx->primed || ctx->staged) {
+        set_error(!ctx->primed ? "rtod_slides before rtod_prime"
+                               : "a phase-API slide is in progress");
+        return RTOD_ERR_STATE;
+    }
+    if (!ctx->own_stream) {
+        set_error("a batch member's slides run through its batch");
+        return RTOD_ERR_UNSUPPORTED;
+    }
+    if (ctx->stage_slots < kStageSlots) {
+        set_error("slides over 64 MB are staged one at a time: use rtod_slide");
+        return RTOD_ERR_UNSUPPORTED;
+    }
+    try {
+        RTOD_HIP(hipSetDevice(ctx->device));
+        SlideSrc src;
+        src.host = pts;
+        return slides_impl(ctx, src, n_slides, out_offsets, out_slots);
     } catch (...) {
         return RTOD_ERR_INTERNAL;
     }
@@ -1571,7 +1699,7 @@ RTOD_API rtod_status rtod_copy_new_points(rtod_ctx *c, const float *new_pts)
         return RTOD_ERR_STATE;
     }
     const size_t nf = (size_t)c->S * c->dim;
-    std::memcpy(c->h_in + (size_t)(c->host_t & 1) * nf, new_pts, nf * sizeof(float));
+    std::memcpy(c->h_in + (size_t)(c->host_t % c->stage_slots) * nf, new_pts, nf * sizeof(float));
     c->staged = true;
     c->phase = 1;
     return RTOD_OK;
@@ -1838,6 +1966,30 @@ RTOD_API rtod_status rtod_kernel_times(const rtod_ctx *c, const char **names, do
         ++k;
     }
     *n = k;
+    return RTOD_OK;
+}
+
+RTOD_API rtod_status rtod_stage_times(const rtod_ctx *c, double *out, int32_t n)
+{
+    if (!c || !out || n < 6) {
+        set_error("rtod_stage_times needs a context and room for 6 values");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (!c->primed || c->host_t < 1) {
+        set_error("no slide has run since the context was primed");
+        return RTOD_ERR_STATE;
+    }
+    const HostOut *ho = host_out_of(c, c->host_t - 1);
+    const double k = 1.0 / c->wall_khz;
+    auto span = [&](int a, int b) {
+        return ho->ts[b] >= ho->ts[a] ? (double)(ho->ts[b] - ho->ts[a]) * k : 0.0;
+    };
+    out[0] = span(kTsIngest, kTsMerge);
+    out[1] = span(kTsMerge, kTsClassify);
+    out[2] = span(kTsClassify, c->lbvh ? kTsIndex : kTsDetect);
+    out[3] = c->lbvh ? span(kTsIndex, kTsDetect) : 0.0;
+    out[4] = span(kTsDetect, kTsFinish);
+    out[5] = span(kTsFinish, kTsPublished);
     return RTOD_OK;
 }
 

@@ -33,6 +33,20 @@ enum SortMode { kModeSlide = 0, kModePrime = 1 };
 // largest sample table of a sorted window (entries); the stride grows with W
 constexpr int kMaxSamples = 4096;
 
+// the device's constant-rate wall clock (its rate: hipDeviceAttributeWallClockRate)
+__device__ __forceinline__ uint64_t wall_clock() { return __builtin_amdgcn_s_memrealtime(); }
+
+// One slide per call: the first workgroup of each of the slide's kernels
+// stamps its start (kernels on a stream run one after another, so a
+// kernel's start is the previous one's end, to the launch gap); the
+// epilogue copies the stamps into the host-out buffer with the list.
+__device__ __forceinline__ void stamp_start(const Dev &d, int k)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        d.tstamp[k] = wall_clock();
+}
+
+
 // ---------------------------------------------------------------------------
 // Key fields: non-negative per-axis cell coordinates (k + 2^B), most
 // significant axis first.
@@ -360,6 +374,8 @@ __global__ __launch_bounds__(NT) void k_ingest_sort(Dev d, Step st, int mode, co
                                                     int dst_batch)
 {
     __shared__ IngestLds<DIM, NT, IT> lds;
+    if (mode == kModeSlide)
+        stamp_start(d, kTsIngest);
     ingest_sort_body<DIM, NT, IT>(d, st, mode, src_dev, dst_batch, blockIdx.x, lds);
 }
 
@@ -899,6 +915,7 @@ template <int DIM>
 __global__ __launch_bounds__(256) void k_merge_window(Dev d, Step st)
 {
     __shared__ MergeWindowLds<256> lds;
+    stamp_start(d, kTsMerge);
     merge_window_body<DIM, 256>(d, st, blockIdx.x, lds);
 }
 
@@ -1192,6 +1209,7 @@ template <int DIM, int MODE>
 __global__ __launch_bounds__(kClassifyNT) void k_classify(Dev d, Step st)
 {
     __shared__ ClassifyLds<MODE == kClsLearned> lds;
+    stamp_start(d, kTsClassify);
     classify_body<DIM, MODE>(d, st, blockIdx.x, lds);
 }
 
@@ -1586,7 +1604,10 @@ struct FinishLds {
 struct SlideSummary {
     int n_o, n_units;
     bool learn, use_pred;
+    bool stamps; // one slide per call: copy the kernels' start stamps into
+                 // the host-out buffer
 };
+
 
 // block sums of a and b and block OR of m (wave-uniform m); every thread
 // gets all three
@@ -1738,6 +1759,12 @@ __device__ __forceinline__ void publish_slide(const Dev &d, const Step &st,
         *dn = sm.n_units > 0
                   ? (((float)in.misses <= d.lr.miss_thr * (float)sm.n_units) ? 1 : 0)
                   : lc->dec[(st.t + 1) % kDecSlots];
+    }
+    if (sm.stamps) {
+#pragma unroll
+        for (int k = 0; k < kTsPublished; ++k)
+            ho->ts[k] = d.tstamp[k];
+        ho->ts[kTsPublished] = wall_clock();
     }
     ho->n_units = sm.n_units;
     ho->n_out = total;
@@ -1902,7 +1929,7 @@ __device__ __forceinline__ void finish_slide(const Dev &d, const Step &st, int f
 // from the counters the slide's classify and detect left in memory.
 template <int NT>
 __device__ __forceinline__ void finish_deferred(const Dev &d, const Step &st, int f, int F,
-                                                FinishLds &fl)
+                                                FinishLds &fl, bool stamps = false)
 {
     SlideSummary sm;
     sm.learn = d.lr.on != 0;
@@ -1910,6 +1937,7 @@ __device__ __forceinline__ void finish_deferred(const Dev &d, const Step &st, in
     sm.n_o = cn->n_units;
     sm.n_units = sm.learn ? sm.n_o + cn->n_units_i : sm.n_o;
     sm.use_pred = sm.learn && d.lr.lc->dec[st.t % kDecSlots] != 0;
+    sm.stamps = stamps;
     finish_slide<NT>(d, st, f, F, sm, fl);
 }
 
@@ -2109,7 +2137,8 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
     if (!s_last)
         return;
     __threadfence();
-    finish_slide<kDetectNT>(d, st, 0, 1, SlideSummary{n_o, n_units, LEARN, use_pred}, lds.fin);
+    finish_slide<kDetectNT>(d, st, 0, 1, SlideSummary{n_o, n_units, LEARN, use_pred, false},
+                            lds.fin);
 }
 
 template <int DIM, bool LEARN>
@@ -2117,6 +2146,7 @@ __global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_detect(Dev 
                                                                            int fin)
 {
     __shared__ DetectLds lds;
+    stamp_start(d, kTsDetect);
     detect_body<DIM, LEARN>(d, st, blockIdx.x, gridDim.x, fin != 0, lds);
 }
 
@@ -2160,10 +2190,12 @@ __global__ __launch_bounds__(256) void k_merge_window_ingest(Dev d, Step st, Ste
 
 // slide st's epilogue in a launch of its own, one part per workgroup (a
 // pipelined run stopped before the slide that would carry it)
-__global__ __launch_bounds__(256) void k_finish(Dev d, Step st)
+__global__ __launch_bounds__(256) void k_finish(Dev d, Step st, int stamps)
 {
     __shared__ FinishLds lds;
-    finish_deferred<256>(d, st, blockIdx.x, gridDim.x, lds);
+    if (stamps)
+        stamp_start(d, kTsFinish);
+    finish_deferred<256>(d, st, blockIdx.x, gridDim.x, lds, stamps != 0);
 }
 
 static_assert(kRunMergeNT == kDetectNT, "the run merge rides in detect workgroups");
@@ -2336,5 +2368,6 @@ __global__ __launch_bounds__(kDetectNT) RTOD_DETECT_OCCUPANCY void k_slide_stage
     detect_body<DIM, false, true>(d, st, bid, ndet, false, *reinterpret_cast<DetectLds *>(dyn_lds),
                                   pre);
 }
+
 
 } // namespace rtod

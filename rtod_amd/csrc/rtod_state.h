@@ -37,7 +37,9 @@
 //                               count, capped at K+1
 //   rep_n i32 [20][2]           replay: bucket sizes of the stored slide
 // Pinned host memory, device-mapped:
-//   h_in  f32 [2][S*d]          new slide staging (row-major, as given)
+//   h_in  f32 [8][S*d]          new slide staging (row-major, as given), slide
+//                               t in slot t % 8 (a batch's streams, and slides
+//                               over 64 MB: [2], t & 1)
 //   h_out [4]                   outlier count + ascending slot list +
 //                               counters + stamp; slide t uses t mod 4
 #pragma once
@@ -84,8 +86,17 @@ struct HostOut {
     int32_t train_steps;   // learned: optimiser steps since creation
     float train_loss;      // learned: mean loss of those steps
     int32_t n_units;       // cell units of the slide
+    int32_t pad;
+    uint64_t ts[8];        // one slide per call: device stamps (kTs*)
     int32_t list[1];       // W entries follow
 };
+
+// Device wall-clock stamps of a slide run one per call, in ticks of the
+// device's constant wall clock: the start of its ingest, window merge,
+// classify, LBVH build (LBVH index only), detect and epilogue kernels, and
+// the slide being published.
+enum { kTsIngest = 0, kTsMerge, kTsClassify, kTsIndex, kTsDetect, kTsFinish, kTsPublished,
+       kTsCount };
 
 // Device-side state of the learned scheduler: per stream ...
 // The fallback decision for slide t is made by the epilogue of slide t - 2
@@ -173,6 +184,7 @@ struct Dev {
     int fin_parts;      // parts of a deferred slide epilogue
     int32_t *ready;     // [3][kReadyFlags * kReadyStride]: classify-done flags
     int wait_spins;     // bound of a detect's wait for its classify (one launch per slide)
+    uint64_t *tstamp;   // [kTsCount] device stamps of the slide run one per call
     Ctl *ctl;
     const float *h_in;
     HostOut *h_out;      // kHostOuts host-out buffers, slide t uses t % kHostOuts
@@ -187,12 +199,21 @@ constexpr int kErrMergeNew = 2;      // window merge: a new point's rank out of 
 constexpr int kErrStencilRows = 4;   // detect: a unit's stencil wider than 32 rows
 constexpr int kErrClassifyWait = 8;  // detect: the bounded wait for its classify ran out
 constexpr int kErrLbvhStack = 16;    // LBVH query: traversal stack overflow
-constexpr int kErrStageWait = 32;    // fused slide: a stage's bounded wait ran out
 
 // the host-out buffer of slide t: consecutive slides rotate through
 // kHostOuts buffers, so up to that many slides can be in flight while the
 // host reads the oldest one's list
 constexpr int kHostOuts = 4;
+// Pinned staging slots of a single stream.  A pipelined run from host
+// memory stages slide i + 3 just before the launch carrying its ingest (one
+// launch per slide); once slide i - kHostOuts is collected, every launch
+// reading a slide before i has completed, so with eight slots the slot
+// rewritten (that of slide i - 5) is no longer read; before the first
+// collection the slots rewritten were last read by an earlier, completed
+// call.
+constexpr int kStageSlots = 8;
+constexpr size_t kStageSlotCap = (size_t)64 << 20; // bytes per slot beyond which
+                                                   // a stream keeps two slots
 
 // the classify counters of slide t
 constexpr int kSlideSets = 3;

@@ -160,3 +160,65 @@ def test_failed_pipelined_run_needs_prime(index, monkeypatch):
         det.prime(xs[:W])  # slides since prime restart at 0: slide 3 fails again
         got = det.slides_device(ds.at(W), 3)
         assert_same(got, O.run_stream(xs, W, S, 3, Rr, K, "grid"), f"{index} after prime")
+
+
+@pytest.mark.parametrize("wl", [O.C1, O.C2, O.C3], ids=lambda w: w.name)
+def test_host_fed_pipelined_every_slide(wl):
+    """rtod_slides: the slides from host memory, staged into pinned memory
+    just before their ingest, pipelined; all slides of the stream, split into
+    runs of uneven length mixed with single calls on one context."""
+    case(f"{wl.name} host-fed pipelined, {wl.T} slides")
+    xs = wl.data()
+    want = O.run_stream(xs, wl.W, wl.S, wl.T, wl.R, wl.K, "grid")
+    got = []
+    with R.Detector(wl.dim, wl.W, wl.S, wl.R, wl.K) as det:
+        det.prime(xs[:wl.W])
+        t, runs = 0, [1, 2, 3, 9, 60]
+        k = 0
+        while t < wl.T:
+            n = min(runs[k % len(runs)], wl.T - t)
+            k += 1
+            rows = xs[wl.W + t * wl.S: wl.W + (t + n) * wl.S]
+            if k % 4 == 0:  # a single call in between
+                got.append(det.slide(rows[: wl.S]))
+                n = 1
+            else:
+                got.extend(det.slides(rows, n))
+            t += n
+    assert_same(got, want, f"{wl.name} host-fed pipelined")
+
+
+def test_host_fed_pipelined_long_run_lbvh():
+    """The non-stage pipelined path (LBVH index) from host memory."""
+    wl = O.C3
+    case("C3 host-fed pipelined, LBVH")
+    xs = wl.data()
+    T = 40
+    with R.Detector(3, wl.W, wl.S, wl.R, wl.K, index_kind=R.INDEX_LBVH) as det:
+        det.prime(xs[:wl.W])
+        got = det.slides(xs[wl.W: wl.W + T * wl.S], T)
+    assert_same(got, O.run_stream(xs, wl.W, wl.S, T, wl.R, wl.K, "grid"), "C3 LBVH host-fed")
+
+
+@pytest.mark.parametrize("index", ["stencil", "lbvh"])
+def test_time_breakdown_on_every_call(index):
+    """The reference's five buckets are filled on every rtod_slide call (no
+    flag), from the device stamps of the slide's kernels; with the launch
+    remainder they add up to the host wall time."""
+    wl = O.C2 if index == "stencil" else O.C3
+    case(f"time breakdown, {index}")
+    xs = wl.data()
+    kind = R.INDEX_CELL_STENCIL if index == "stencil" else R.INDEX_LBVH
+    with R.Detector(wl.dim, wl.W, wl.S, wl.R, wl.K, index_kind=kind) as det:
+        det.prime(xs[:wl.W])
+        for t in range(1, 21):
+            det.slide(xs[wl.W + (t - 1) * wl.S: wl.W + t * wl.S])
+            tm = det.times
+            parts = [tm.copy_h2d, tm.prepare_cell, tm.build_index, tm.detect_outlier, tm.copy_d2h]
+            assert all(p > 0 for p in parts), tm
+            assert tm.launch >= 0, tm
+            assert abs(sum(parts) + tm.launch - tm.total) < 1e-9, tm
+            st = det.stage_times()
+            assert st["ingest"] > 0 and st["classify"] > 0 and st["detect"] > 0, st
+            assert (st["lbvh_build"] > 0) == (index == "lbvh"), st
+            assert sum(st.values()) < tm.total, (st, tm)

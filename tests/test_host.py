@@ -60,10 +60,10 @@ def test_null_arguments_rejected():
 
 
 def test_struct_layouts_match_header():
-    # rtod_config: 10 x 4 bytes; rtod_times: 6 doubles;
+    # rtod_config: 10 x 4 bytes; rtod_times: 7 doubles;
     # rtod_stats: 13 int64 + 1 double + 2 int32
     assert C.sizeof(R._Config) == 40
-    assert C.sizeof(R._Times) == 48
+    assert C.sizeof(R._Times) == 56
     assert C.sizeof(R._Stats) == 14 * 8 + 2 * 4
 
 
