@@ -1,34 +1,50 @@
 #!/usr/bin/env python3
-"""Per-slide latency of the MI355X RTOD path on the 1-D random-walk stream
-W=100k, S=5k, R=0.45, K=50 (1,048,572 points), one JSON line on stdout.
+"""Per-slide time of the MI355X RTOD path on the 1-D random-walk stream
+W=100k, S=5k, R=0.45, K=50 (1,048,572 points); one JSON line on stdout.
 
-A step is one slide: new S points in, the window's exact outlier list out in
-host memory.  `value` is the mean per-slide wall time in microseconds that the
-library measures from rtod_slide_device entry (the stream already resident
-in HBM) to the sorted list in host memory (lower is better), averaged over
---steps slides after --warmup slides.  The same run through rtod_slide, with
-each slide copied from host memory inside the call, is reported beside it as
-pcie_inclusive_us_per_slide.
+A step is one slide: S new points in host memory in, the window's exact
+outlier list out in host memory.
+
+  value    --steps consecutive slides through rtod_slides (pipelined, plain
+           schedule, cell-stencil index), timed on the host from the call's
+           entry to its return: every slide's points are copied from host
+           memory into pinned staging inside the timed call (the reference's
+           per-slide H2D copy is inside its timed loop, optixScan.cpp:1051-
+           1058) and every list is in host memory when it returns.  --warmup
+           slides go through the same call first, untimed.
+  latency_us_per_slide
+           the same slides one rtod_slide call each (new slide in host
+           memory, list in host memory when the call returns), the mean of
+           the library's wall time per call, with the reference's five Timer
+           buckets (timer.h:16-24) from the kernels' device stamps.
 
 Extra objects on the line:
-  roofline      dominant kernel of the slide (HIP events on the library's
-                stream, separate pass with per-phase events) against the
-                8 TB/s HBM peak, algorithmic bytes B = 4*d*(W+S) + 4*|O|
-  cpu_baseline  the grid oracle (same algorithm, C, pthreads) timed on this
-                host's cores on a bounded sample of the same stream
-  batched       256 independent streams of uneven dim/W/S/R/K (the batched
-                workload): per-step time of the batched API after its
-                autotuner, its roofline over all streams' bytes, and the grid
-                oracle over the same streams on the host cores
-  learned       the same workloads with the learned work scheduler (online-
-                trained MLP ordering the exact range counts): per-slide time,
-                speed-up over the plain schedule, predictor miss rate, slides
-                that fell back to the plain schedule, and the number of slides
-                whose outlier list differs from the plain schedule's (must be 0)
+  roofline      the dominant kernel of the measured path (the one launch per
+                slide of the pipelined run), timed with HIP events on the
+                library's stream in a separate slide-by-slide pass, against
+                the 8 TB/s HBM peak; algorithmic bytes per launch = the
+                slide's B = 4*d*W + 4*d*S + 4*|O| (window coordinates read
+                once, new slide written, outlier ids written); traffic = HBM
+                bytes per launch from the committed rocprofv3 counter summary
+  cpu_baseline  the grid oracle (same algorithm, C, pthreads) on the host's
+                cores: all of them and one thread, the faster reported, with
+                the CPU model; and the number of points whose outlier status
+                differs between this build's FP32 d2 <= R*R and the
+                reference's FP64 strict test (optixScan.cu:170-189) over
+                every slide of the stream
+  sweeps        the reference's R, K and S sweeps around this default
+                (script/run.py:76-218)
+  other_workloads, index_comparison, learned, batched
+                the 1-D W=10k and 3-D configurations, cell stencil vs LBVH,
+                the learned work scheduler against the plain one, and 256
+                batched streams
+The line carries "valid": false, and the process exits 1, if any pipelined,
+learned or batched outlier list differs from the plain one-per-call list.
 """
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -41,106 +57,119 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "μs per slide (W=100k,S=5k,K=50) plain vs learned-scheduled; exact outliers"
+BUCKETS = ("copy_h2d", "prepare_cell", "build_index", "detect_outlier", "copy_d2h", "launch",
+           "total")
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def _workloads():
+def workloads():
     from rtod_amd import synth
 
-    # name: (dim, N, W, S, R, K, generator, seed)
-    return {w.name: (w.dim, w.N, w.W, w.S, w.R, w.K, w.kind, w.seed)
-            for w in synth.WORKLOADS.values()}
-
-
-WORKLOADS = _workloads()
+    return synth.WORKLOADS
 
 
 def stream(name: str, slides: int) -> np.ndarray:
-    from rtod_amd import synth
-
-    # the measured streams have N points; longer runs extend the same seeded
-    # generator
-    dim, N, W, S = WORKLOADS[name][:4]
-    return synth.WORKLOADS[name].data(W + slides * S)
+    """The measured stream; runs longer than N extend the same generator."""
+    wl = workloads()[name]
+    return wl.data(wl.W + slides * wl.S)
 
 
-def run_gpu(xs, W, S, R, K, warmup, steps, flags=0, sched=0, device_input=True, index=0,
-            keep_lists=False):
-    """device_input: the stream sits in HBM (uploaded once, untimed) and each
-    slide is fed by device pointer; else each slide's points come from host
-    memory through the pinned staging buffer (PCIe inside the timed call)."""
+def same(a, b) -> int:
+    """Slides whose lists differ."""
+    return sum(0 if np.array_equal(x, y) else 1 for x, y in zip(a, b)) + abs(len(a) - len(b))
+
+
+# --------------------------------------------------------------------------
+# GPU legs
+# --------------------------------------------------------------------------
+def host_pipelined(xs, W, S, R, K, warmup, steps, sched=0, index=0):
+    """rtod_slides from host memory: `warmup` slides in one call, then
+    `steps` slides in one timed call."""
     import rtod_amd
 
     dim = xs.shape[1]
-    det = rtod_amd.Detector(dim, W, S, R, K, flags=flags, sched_kind=sched, index_kind=index)
-    ds = rtod_amd.DeviceStream(xs[: W + (warmup + steps) * S]) if device_input else None
-    det.prime(xs[:W])
-    totals, outs, lists = [], [], []
-    t_outer = 0.0
-    st0 = None
-    for t in range(1, warmup + steps + 1):
-        row = W + (t - 1) * S
+    with rtod_amd.Detector(dim, W, S, R, K, sched_kind=sched, index_kind=index) as det:
+        det.prime(xs[:W])
+        if warmup:
+            det.slides(xs[W: W + warmup * S], warmup)
+        st0 = det.stats()
+        rows = np.ascontiguousarray(xs[W + warmup * S: W + (warmup + steps) * S])
         t0 = time.perf_counter()
-        o = det.slide_device(ds.at(row)) if ds else det.slide(xs[row: row + S])
+        lists = det.slides(rows, steps)
         dt = time.perf_counter() - t0
-        if t == warmup:
-            st0 = det.stats()
-        if t > warmup:
-            totals.append(det.times.total)
-            outs.append(len(o))
-            t_outer += dt
-            if keep_lists:
-                lists.append(o.copy())
-    res = {"lib_ms": float(np.mean(totals)), "outer_ms": t_outer / max(1, steps) * 1e3,
-           "outliers_mean": float(np.mean(outs)) if outs else 0.0,
-           "kernels": det.kernel_times(), "stats": det.stats(), "stats_warm": st0,
-           "lists": lists}
-    det.close()
-    if ds:
-        ds.close()
-    return res
+        st1 = det.stats()
+    return {"us": dt / steps * 1e6, "lists": lists, "stats": st1, "stats_warm": st0}
 
 
-def run_gpu_pipelined(xs, W, S, R, K, warmup, steps, sched=0, index=0, flags=0):
-    """The same slides through rtod_slides_device: slide t+1's new batch is
-    sorted by extra workgroups of slide t's launches, up to four slides are
-    queued and the host collects each list as it completes.  Stream in HBM;
-    `warmup` slides in one call, then `steps` slides in one timed call.
-    flags=FLAG_PHASE_TIMES: HIP events around each kernel (a separate,
-    slide-by-slide measuring pass)."""
+def device_pipelined(xs, W, S, R, K, warmup, steps, sched=0, index=0, flags=0):
+    """rtod_slides_device: the stream uploaded to HBM first (untimed)."""
     import rtod_amd
 
     dim = xs.shape[1]
-    det = rtod_amd.Detector(dim, W, S, R, K, sched_kind=sched, index_kind=index, flags=flags)
-    ds = rtod_amd.DeviceStream(xs[: W + (warmup + steps) * S])
-    det.prime(xs[:W])
-    if warmup:
-        det.slides_device(ds.at(W), warmup)
-    st0 = det.stats()
-    t0 = time.perf_counter()
-    if flags:
-        det.reset_kernel_times()
-    lists = det.slides_device(ds.at(W + warmup * S), steps)
-    dt = time.perf_counter() - t0
-    res = {"us": dt / steps * 1e6, "lists": lists, "stats": det.stats(), "stats_warm": st0,
-           "kernels": det.kernel_times()}
-    det.close()
-    ds.close()
-    return res
+    with rtod_amd.DeviceStream(xs[: W + (warmup + steps) * S]) as ds, \
+            rtod_amd.Detector(dim, W, S, R, K, sched_kind=sched, index_kind=index,
+                              flags=flags) as det:
+        det.prime(xs[:W])
+        if warmup:
+            det.slides_device(ds.at(W), warmup)
+        if flags:
+            det.reset_kernel_times()
+        t0 = time.perf_counter()
+        lists = det.slides_device(ds.at(W + warmup * S), steps)
+        dt = time.perf_counter() - t0
+        kern = det.kernel_times()
+    return {"us": dt / steps * 1e6, "lists": lists, "kernels": kern}
 
 
-def cpu_baseline(xs, W, S, R, K, slides):
-    import oracle
+def single_calls(xs, W, S, R, K, warmup, steps, sched=0, index=0, flags=0):
+    """One rtod_slide call per slide (host input): the library's wall time
+    per call, its buckets and kernel durations (device stamps)."""
+    import rtod_amd
 
-    threads = min(os.cpu_count() or 1, 16)
-    _, us_mt = oracle.run_stream(xs[: W + slides * S], W, S, slides, R, K, "grid",
-                                 nthreads=threads, with_time=True)
-    _, us_st = oracle.run_stream(xs[: W + slides * S], W, S, slides, R, K, "grid",
-                                 nthreads=1, with_time=True)
-    return us_mt, us_st, threads
+    dim = xs.shape[1]
+    with rtod_amd.Detector(dim, W, S, R, K, sched_kind=sched, index_kind=index,
+                           flags=flags) as det:
+        det.prime(xs[:W])
+        lists, bk, stg, outer = [], [], [], 0.0
+        st0 = None
+        for t in range(1, warmup + steps + 1):
+            row = W + (t - 1) * S
+            t0 = time.perf_counter()
+            o = det.slide(xs[row: row + S])
+            dt = time.perf_counter() - t0
+            if t == warmup:
+                st0 = det.stats()
+            if t > warmup:
+                tm = det.times
+                bk.append([getattr(tm, b) for b in BUCKETS])
+                stg.append(list(det.stage_times().values()))
+                lists.append(o)
+                outer += dt
+        st1 = det.stats()
+        kern = det.kernel_times()
+    bk = np.array(bk).mean(0)
+    return {"us": bk[-1] * 1e3, "outer_us": outer / steps * 1e6, "lists": lists,
+            "buckets_ms": {b: round(float(v), 5) for b, v in zip(BUCKETS, bk)},
+            "stages_ms": {n: round(float(v), 5)
+                          for n, v in zip(rtod_amd.Detector.STAGES, np.array(stg).mean(0))},
+            "stats": st1, "stats_warm": st0, "kernels": kern}
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy), or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("kernel=") and f"k_{kernel}<" in line:
+            for tok in line.split():
+                if tok.startswith("traffic_bytes="):
+                    return int(tok.split("=")[1])
+    return None
 
 
 def calib_copy(mb: int = 64, reps: int = 5) -> None:
@@ -163,81 +192,173 @@ def calib_copy(mb: int = 64, reps: int = 5) -> None:
         f"per copy dispatch")
 
 
-def measured_traffic(workload: str, kernel: str):
-    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy), or None."""
-    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.txt")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("kernel=") and f"k_{kernel}<" in line:
-            for tok in line.split():
-                if tok.startswith("traffic_bytes="):
-                    return int(tok.split("=")[1])
-    return None
+# --------------------------------------------------------------------------
+# CPU baseline (the oracle: test infrastructure, timed here as the baseline)
+# --------------------------------------------------------------------------
+def host_threads() -> tuple[int, dict]:
+    """The host CPUs this process may use: its affinity mask, capped by the
+    cgroup's CPU quota and by OMP_NUM_THREADS when set (a machine's share
+    for this job), with what each said."""
+    info = {"cpu_count": os.cpu_count()}
+    n = len(os.sched_getaffinity(0))
+    info["affinity"] = n
+    try:
+        q, p = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            info["cgroup_quota"] = round(int(q) / int(p), 2)
+            n = min(n, max(1, int(int(q) / int(p))))
+    except (OSError, ValueError):
+        pass
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        info["omp_num_threads"] = int(env)
+        n = min(n, int(env))
+    return n, info
 
 
-def index_comparison(names, steps):
-    """Per-slide time of the cell-stencil index and of the LBVH index on the
-    same streams (the north star picks whichever measures faster)."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(xs, W, S, R, K, slides, threads_all):
+    """Grid oracle, slides 1..slides: all host threads and one thread."""
+    import oracle
+
+    _, us_mt = oracle.run_stream(xs[: W + slides * S], W, S, slides, R, K, "grid",
+                                 nthreads=threads_all, with_time=True)
+    _, us_st = oracle.run_stream(xs[: W + slides * S], W, S, slides, R, K, "grid",
+                                 nthreads=1, with_time=True)
+    best_mt = us_mt < us_st
+    return {"value": round(min(us_mt, us_st), 2), "unit": "us/slide",
+            "cores": threads_all if best_mt else 1, "kind": "port",
+            "sample": f"grid oracle (incremental per-cell FIFOs, C), slides 1..{slides} of the "
+                      f"same stream: {threads_all} threads {us_mt:.1f} us/slide, 1 thread "
+                      f"{us_st:.1f} us/slide, the faster reported (range counts split over "
+                      f"the threads only when a slide has >= 256 candidates)",
+            "us_all_threads": round(us_mt, 2), "us_1_thread": round(us_st, 2),
+            "threads_all": threads_all, "host_cpus": host_threads()[1],
+            "cpu_model": cpu_model()}
+
+
+def fp64_disagreements(xs, W, S, T, R, K, threads):
+    """Points whose outlier status differs between the FP32 contract and the
+    reference's FP64 strict rule, every slide (oracle, CPU)."""
+    import oracle
+
+    t0 = time.perf_counter()
+    tot, per = oracle.semantics_diff_stream(xs, W, S, T, R, K, nthreads=threads)
+    return {"slides": int(T), "points_status_differs": tot["status"],
+            "slides_with_a_difference": int((per > 0).sum()),
+            "points_count_differs": tot["counts"], "pairs_differ": tot["pairs"],
+            "seconds": round(time.perf_counter() - t0, 1)}
+
+
+# --------------------------------------------------------------------------
+# workloads beside the headline
+# --------------------------------------------------------------------------
+def one_workload(name, warmup, steps, cpu_slides, with_cpu, threads, R=None, K=None, S=None):
+    wl = workloads()[name]
+    W = wl.W
+    S = S or wl.S
+    R = R or wl.R
+    K = K or wl.K
+    T = min(10000, (wl.N - W) // S)
+    n = min(steps, T - warmup)
+    xs = wl.data(W + T * S)
+    hp = host_pipelined(xs, W, S, R, K, warmup, n)
+    sc = single_calls(xs, W, S, R, K, warmup, n)
+    ent = {"config": {"dim": wl.dim, "window": W, "slide": S, "R": R, "K": K, "slides": n},
+           "gpu_us_per_slide": round(hp["us"], 3),
+           "latency_us_per_slide": round(sc["us"], 3),
+           "breakdown_ms": sc["buckets_ms"],
+           "mismatched_slides": same(hp["lists"], sc["lists"]),
+           "outliers_per_slide": round(float(np.mean([len(x) for x in sc["lists"]])), 2),
+           "candidates_per_slide": round((sc["stats"]["candidates_total"] -
+                                          sc["stats_warm"]["candidates_total"]) / n, 1)}
+    if with_cpu:
+        cb = cpu_baseline(xs, W, S, R, K, min(cpu_slides, T), threads)
+        ent["cpu_us_per_slide"] = cb["value"]
+        ent["cpu_cores"] = cb["cores"]
+        ent["cpu_us_1_thread"] = cb["us_1_thread"]
+        ent["cpu_us_all_threads"] = cb["us_all_threads"]
+        ent["speedup_vs_cpu"] = round(cb["value"] / hp["us"], 2)
+        ent["latency_speedup_vs_cpu"] = round(cb["value"] / sc["us"], 2)
+    return ent
+
+
+def sweeps(warmup, steps, cpu_slides, with_cpu, threads):
+    out = {}
+    for what, vals in (("R", (0.15, 0.3, 0.6, 0.75)), ("K", (30, 70, 100)),
+                       ("S", (1000, 10000))):
+        for v in vals:
+            kw = {what: v}
+            key = f"{what}={v}"
+            out[key] = one_workload("walk1d-W100k-S5k", warmup, steps, cpu_slides, with_cpu,
+                                    threads, **kw)
+            log(f"[bench] sweep {key}: {out[key]}")
+    return out
+
+
+def index_comparison(names, warmup, steps):
+    """Per-slide latency of the cell-stencil index and of the LBVH index on
+    the same streams (the index is chosen by which measures faster)."""
     import rtod_amd
 
     out = {}
     for name in names:
-        dim, N, W, S, R, K, gen, seed = WORKLOADS[name]
-        T = min(10000, (N - W) // S)
-        xs = stream(name, T)
-        n = min(steps, T - 10)
-        st = run_gpu(xs, W, S, R, K, 10, n, index=rtod_amd.INDEX_CELL_STENCIL)
-        lb = run_gpu(xs, W, S, R, K, 10, n, index=rtod_amd.INDEX_LBVH)
-        out[name] = {"cell_stencil_us": round(st["lib_ms"] * 1e3, 3),
-                     "lbvh_us": round(lb["lib_ms"] * 1e3, 3),
-                     "faster": "cell_stencil" if st["lib_ms"] <= lb["lib_ms"] else "lbvh"}
+        wl = workloads()[name]
+        xs = stream(name, warmup + steps)
+        st = single_calls(xs, wl.W, wl.S, wl.R, wl.K, warmup, steps,
+                          index=rtod_amd.INDEX_CELL_STENCIL)
+        lb = single_calls(xs, wl.W, wl.S, wl.R, wl.K, warmup, steps, index=rtod_amd.INDEX_LBVH)
+        out[name] = {"cell_stencil_us": round(st["us"], 3), "lbvh_us": round(lb["us"], 3),
+                     "cell_stencil_stages_ms": st["stages_ms"], "lbvh_stages_ms": lb["stages_ms"],
+                     "mismatched_slides": same(st["lists"], lb["lists"]),
+                     "faster": "cell_stencil" if st["us"] <= lb["us"] else "lbvh"}
         log(f"[bench] index comparison {name}: {out[name]}")
     return out
 
 
 def learned_comparison(names, train_slides, steps):
-    """Plain vs learned schedule on the same slides: the learned detector
-    first trains online for `train_slides` slides (untimed warm-up, as the
-    plain run's warm-up), then both are timed over `steps` slides."""
+    """Plain vs learned schedule on the same slides: each detector first runs
+    `train_slides` slides (untimed warm-up; the learned one trains online
+    meanwhile), then `steps` timed slides, host-fed pipelined and one per
+    call."""
     import rtod_amd
 
     out = {}
     for name in names:
-        dim, N, W, S, R, K, gen, seed = WORKLOADS[name]
+        wl = workloads()[name]
         xs = stream(name, train_slides + steps)
-        pl = run_gpu(xs, W, S, R, K, train_slides, steps, keep_lists=True)
-        le = run_gpu(xs, W, S, R, K, train_slides, steps, sched=rtod_amd.SCHED_LEARNED,
-                     keep_lists=True)
-        mism = sum(0 if np.array_equal(a, b) else 1 for a, b in zip(pl["lists"], le["lists"]))
-        pp = run_gpu_pipelined(xs, W, S, R, K, train_slides, steps)
-        lp = run_gpu_pipelined(xs, W, S, R, K, train_slides, steps, sched=rtod_amd.SCHED_LEARNED)
-        mism += sum(0 if np.array_equal(a, b) else 1 for a, b in zip(pl["lists"], lp["lists"]))
+        W, S, R, K = wl.W, wl.S, wl.R, wl.K
+        pl = single_calls(xs, W, S, R, K, train_slides, steps)
+        le = single_calls(xs, W, S, R, K, train_slides, steps, sched=rtod_amd.SCHED_LEARNED)
+        pp = host_pipelined(xs, W, S, R, K, train_slides, steps)
+        lp = host_pipelined(xs, W, S, R, K, train_slides, steps, sched=rtod_amd.SCHED_LEARNED)
+        mism = same(pl["lists"], le["lists"]) + same(pl["lists"], lp["lists"]) + \
+            same(pl["lists"], pp["lists"])
         s1, s0 = le["stats"], le["stats_warm"]
         pts = max(1, s1["predictor_points"] - s0["predictor_points"])
-        # kernel durations of both schedules (separate event-timed passes)
-        kp = run_gpu(xs, W, S, R, K, train_slides, min(steps, 60),
-                     flags=rtod_amd.FLAG_PHASE_TIMES)["kernels"]
-        kl = run_gpu(xs, W, S, R, K, train_slides, min(steps, 60),
-                     flags=rtod_amd.FLAG_PHASE_TIMES, sched=rtod_amd.SCHED_LEARNED)["kernels"]
-        ent = {"plain_us": round(pl["lib_ms"] * 1e3, 3),
-               "learned_us": round(le["lib_ms"] * 1e3, 3),
-               "speedup_vs_plain": round(pl["lib_ms"] / le["lib_ms"], 3),
+        ent = {"plain_us": round(pl["us"], 3), "learned_us": round(le["us"], 3),
+               "speedup_vs_plain": round(pl["us"] / le["us"], 3),
                "plain_pipelined_us": round(pp["us"], 3),
                "learned_pipelined_us": round(lp["us"], 3),
                "pipelined_speedup_vs_plain": round(pp["us"] / lp["us"], 3),
                "mismatched_slides": mism,
                "miss_rate": round((s1["predictor_misses"] - s0["predictor_misses"]) / pts, 4),
                "fallback_slides": s1["fallbacks"] - s0["fallbacks"],
-               "train_slides": train_slides,
-               "train_steps": s1["train_steps"],
+               "train_slides": train_slides, "train_steps": s1["train_steps"],
                "pair_tests_per_slide": {
                    "plain": round((pl["stats"]["pair_tests_total"] -
                                    pl["stats_warm"]["pair_tests_total"]) / steps),
                    "learned": round((s1["pair_tests_total"] - s0["pair_tests_total"]) / steps)},
-               "kernels_ms_plain": {k: round(v[0], 5) for k, v in kp.items() if v[1] > 0},
-               "kernels_ms_learned": {k: round(v[0], 5) for k, v in kl.items() if v[1] > 0}}
+               "stages_ms_plain": pl["stages_ms"], "stages_ms_learned": le["stages_ms"]}
         out[name] = ent
         log(f"[bench] learned {name}: {ent}")
     return out
@@ -246,8 +367,6 @@ def learned_comparison(names, train_slides, steps):
 def _batch_run(descs, data, sched, warmup, steps, tune_slides):
     """Autotune, warm up (the learned schedule trains meanwhile), then time
     `steps` steps; per-step digests of all outlier lists for comparison."""
-    import hashlib
-
     import rtod_amd
 
     cfg = [(d.dim, d.W, d.S, d.R, d.K) for d in descs]
@@ -275,22 +394,21 @@ def _batch_run(descs, data, sched, warmup, steps, tune_slides):
     return t_tot / steps, outs / steps, digests, tuning, st
 
 
-def batched(n_streams, warmup, steps, tune_slides, cpu_slides, with_cpu, learned_warmup=40):
-    """The batched workload: n_streams independent streams (dim ~ U{1,2,3},
-    W log-uniform on [2k, 100k], S = 1-5% of W, K in {20, 50}, R set for a
-    median neighbour count ~2K), resident in HBM, advanced together; plain,
-    then the learned schedule on the same steps."""
+def batched(n_streams, steps, tune_slides, cpu_slides, with_cpu, threads, learned_warmup=40):
+    """256 independent streams (dim ~ U{1,2,3}, W log-uniform on [2k, 100k],
+    S = 1-5% of W, K in {20, 50}, R set for a median neighbour count ~2K),
+    resident in HBM, advanced together; plain, then the learned schedule on
+    the same steps."""
     import rtod_amd
     from rtod_amd import synth
 
-    slides = tune_slides + max(warmup, learned_warmup) + steps
+    slides = tune_slides + learned_warmup + steps
     t0 = time.perf_counter()
     descs, data = synth.batch_workload(n_streams, slides=slides)
     log(f"[bench] batched: {n_streams} streams generated in {time.perf_counter() - t0:.1f} s, "
         f"sum W {sum(d.W for d in descs)}, sum S {sum(d.S for d in descs)}")
     dt, out_per_step, dig_p, tuning, _ = _batch_run(descs, data, rtod_amd.SCHED_PLAIN,
                                                     learned_warmup, steps, tune_slides)
-    log(f"[bench] batched autotune: {tuning}")
     dt_l, _, dig_l, tuning_l, st_l = _batch_run(descs, data, rtod_amd.SCHED_LEARNED,
                                                 learned_warmup, steps, tune_slides)
     pts = max(1, sum(s["predictor_points"] for s in st_l))
@@ -317,19 +435,18 @@ def batched(n_streams, warmup, steps, tune_slides, cpu_slides, with_cpu, learned
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                         "algorithmic_bytes": int(bytes_step)}}
     if with_cpu:
-        ent["cpu_baseline"] = batched_cpu(descs, data, cpu_slides)
+        ent["cpu_baseline"] = batched_cpu(descs, data, cpu_slides, threads)
         ent["speedup_vs_cpu"] = round(ent["cpu_baseline"]["value"] / us, 2)
     log(f"[bench] batched: {ent}")
     return ent
 
 
-def batched_cpu(descs, data, slides):
+def batched_cpu(descs, data, slides, threads):
     """Grid oracle over the same streams, one stream per task on a pool of
     host threads (ctypes releases the GIL); wall time of `slides` steps."""
     import oracle
     from concurrent.futures import ThreadPoolExecutor
 
-    threads = min(os.cpu_count() or 1, 16)
     grids = [oracle.Grid(d.dim, d.W, d.S, d.R, d.K) for d in descs]
     with ThreadPoolExecutor(threads) as ex:
         list(ex.map(lambda i: grids[i].prime(data[i][:descs[i].W]), range(len(descs))))
@@ -341,36 +458,11 @@ def batched_cpu(descs, data, slides):
         dt = (time.perf_counter() - t0) / slides
     return {"value": round(dt * 1e6, 1), "unit": "us/step", "cores": threads, "kind": "port",
             "sample": f"grid oracle (C), every stream's first {slides} slides, streams spread "
-                      f"over a {threads}-thread pool, one barrier per step"}
+                      f"over a {threads}-thread pool, one barrier per step",
+            "cpu_model": cpu_model()}
 
 
-def other_workloads(names, steps, cpu_slides, with_cpu):
-    out = {}
-    for name in names:
-        dim, N, W, S, R, K, gen, seed = WORKLOADS[name]
-        T = min(10000, (N - W) // S)
-        xs = stream(name, T)
-        n = min(steps, T - 10)
-        g = run_gpu(xs, W, S, R, K, 10, n, keep_lists=True)
-        gp = run_gpu_pipelined(xs, W, S, R, K, 10, n)
-        ent = {"gpu_us_per_slide": round(gp["us"], 3),
-               "sequential_us_per_slide": round(g["lib_ms"] * 1e3, 3),
-               "pipelined_mismatched_slides": sum(
-                   0 if np.array_equal(a, b) else 1 for a, b in zip(g["lists"], gp["lists"])),
-               "outliers_per_slide": g["outliers_mean"],
-               "candidates_per_slide": g["stats"]["candidates_last"]}
-        if with_cpu:
-            ns = min(cpu_slides, T)
-            us_mt, us_st, threads = cpu_baseline(xs, W, S, R, K, ns)
-            ent["cpu_us_per_slide"] = round(us_mt, 2)
-            ent["cpu_threads"] = threads
-            ent["cpu_us_per_slide_1thread"] = round(us_st, 2)
-            ent["speedup_vs_cpu"] = round(min(us_mt, us_st) / ent["gpu_us_per_slide"], 2)
-        out[name] = ent
-        log(f"[bench] {name}: {ent}")
-    return out
-
-
+# --------------------------------------------------------------------------
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -379,120 +471,102 @@ def main() -> int:
     ap.add_argument("--cpu-slides", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-others", action="store_true",
-                    help="skip the 1-D W=10k and 3-D workloads")
-    ap.add_argument("--workload", default="walk1d-W100k-S5k", choices=sorted(WORKLOADS),
+                    help="skip the other workloads, the sweeps and the index comparison")
+    ap.add_argument("--workload", default="walk1d-W100k-S5k", choices=sorted(workloads()),
                     help="profiling runs only: the JSON line always names the workload run")
-    ap.add_argument("--no-batched", action="store_true",
-                    help="skip the batched-streams measurement")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-streams", type=int, default=256)
     ap.add_argument("--batched-only", action="store_true",
                     help="profiling runs: only the batched measurement, its JSON object")
-    ap.add_argument("--no-learned", action="store_true",
-                    help="skip the learned-schedule comparison")
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--train-slides", type=int, default=200,
                     help="online-training warm-up of the learned schedule")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run only a plain 64 MiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs a stream on a single MI355X "
-              "(multi-GPU sharding is design-only); use --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: every GPU box of this build has one, and the "
+              "multi-GPU sharding of a stream is design-only; use --gpus 1", file=sys.stderr)
         return 2
     if args.calib_copy:
         calib_copy()
         return 0
+    threads = host_threads()[0]
     if args.batched_only:
-        print(json.dumps(batched(args.batch_streams, min(args.warmup, 5), args.steps, 4, 3,
-                                 not args.no_cpu)), flush=True)
+        print(json.dumps(batched(args.batch_streams, args.steps, 4, 3, not args.no_cpu,
+                                 threads)), flush=True)
         return 0
-
-    dim, _, W, S, R, K, _, _ = WORKLOADS[args.workload]
-    xs = stream(args.workload, args.warmup + args.steps)
-    log(f"[bench] {args.workload}: {xs.shape[0]} points; warmup {args.warmup} "
-        f"steps {args.steps}")
 
     import rtod_amd
 
-    plain = run_gpu(xs, W, S, R, K, args.warmup, args.steps, keep_lists=True)
-    log(f"[bench] plain, inputs in HBM, one slide per call: {plain['lib_ms'] * 1e3:.1f} us/slide "
-        f"(outer {plain['outer_ms'] * 1e3:.1f})")
-    pipe = run_gpu_pipelined(xs, W, S, R, K, args.warmup, args.steps)
-    pipe_mism = sum(0 if np.array_equal(a, b) else 1 for a, b in zip(plain["lists"], pipe["lists"]))
-    log(f"[bench] plain, inputs in HBM, pipelined slides: {pipe['us']:.1f} us/slide "
-        f"({pipe_mism} slides differ from the one-per-call run)")
-    pcie = run_gpu(xs, W, S, R, K, args.warmup, args.steps, device_input=False)
-    log(f"[bench] plain, inputs from host memory (PCIe inside the call): "
-        f"{pcie['lib_ms'] * 1e3:.1f} us/slide")
-    # per-kernel device durations: separate passes with HIP events around each
-    # kernel, of the pipelined launches (the measured `value`) and of the
-    # one-slide-per-call ones
-    timed = run_gpu_pipelined(xs, W, S, R, K, min(args.warmup, 10), min(args.steps, 60),
-                              flags=rtod_amd.FLAG_PHASE_TIMES)
+    wl = workloads()[args.workload]
+    dim, W, S, R, K = wl.dim, wl.W, wl.S, wl.R, wl.K
+    xs = stream(args.workload, args.warmup + args.steps)
+    log(f"[bench] {args.workload}: {xs.shape[0]} points; warmup {args.warmup} "
+        f"steps {args.steps}; {rtod_amd.version()}")
+
+    # the measured value: host-fed pipelined slides
+    hp = host_pipelined(xs, W, S, R, K, args.warmup, args.steps)
+    log(f"[bench] host-fed pipelined (rtod_slides): {hp['us']:.2f} us/slide")
+    # one call per slide, host input: latency and the reference's buckets
+    sc = single_calls(xs, W, S, R, K, args.warmup, args.steps)
+    log(f"[bench] one slide per call (rtod_slide): {sc['us']:.2f} us/slide "
+        f"(outer {sc['outer_us']:.2f}); buckets {sc['buckets_ms']}; stages {sc['stages_ms']}")
+    dp = device_pipelined(xs, W, S, R, K, args.warmup, args.steps)
+    log(f"[bench] HBM-fed pipelined (rtod_slides_device): {dp['us']:.2f} us/slide")
+    mism = same(hp["lists"], sc["lists"]) + same(dp["lists"], sc["lists"])
+
+    # the dominant kernel of the measured path, HIP events around each launch
+    timed = device_pipelined(xs, W, S, R, K, min(args.warmup, 10), min(args.steps, 60),
+                             flags=rtod_amd.FLAG_PHASE_TIMES)
     kern = {k: v[0] for k, v in timed["kernels"].items() if v[1] > 0}
-    timed_seq = run_gpu(xs, W, S, R, K, min(args.warmup, 10), min(args.steps, 60),
-                        flags=rtod_amd.FLAG_PHASE_TIMES)
-    kern_seq = {k: v[0] for k, v in timed_seq["kernels"].items() if v[1] > 0}
     dom = max(kern, key=kern.get)
     dom_ms = kern[dom]
-    n_out = plain["outliers_mean"]
-    n_cand = plain["stats"]["candidates_last"]
-    # algorithmic bytes per launch: 4*d bytes per point coordinate read or
-    # written once (the per-point figure of the slide, B = 4d(W+S) + 4|O|)
-    alg = {"ingest_sort": 8 * dim * S,                  # new slide in, ring out
-           "merge_pass": 8 * dim * S,
-           "merge_window": 8 * dim * W,                 # window in, window out
-           "merge_window_ingest": 8 * dim * (W + S),    # + the next slide's tile sorts
-           "classify": 4 * dim * W,                     # one pass over the window
-           "detect": 4 * dim * n_cand + 4 * n_out,      # candidates in, list out
-           "detect_runs": 4 * dim * n_cand + 4 * n_out + 8 * dim * S,  # + next run merge
-           # two launches per slide (learned schedule): classify; detect +
-           # the previous slide's list + the next slide's window merge + the
-           # run merge two slides ahead + the tile sorts three ahead
-           "classify_stage": 4 * dim * W,
-           "detect_stage": 4 * dim * n_cand + 4 * n_out + 8 * dim * W + 16 * dim * S,
-           # one launch per slide (plain schedule): classify + all of the above
-           "slide_stage": 4 * dim * W + 4 * dim * n_cand + 4 * n_out + 8 * dim * W
-                          + 16 * dim * S}
-    b_dom = alg.get(dom, 4 * dim * (W + S))
-    achieved = b_dom / (dom_ms * 1e-3) / 1e9
-    b_slide = 4 * dim * (W + S) + 4 * n_out
-    log(f"[bench] kernels (ms): " + ", ".join(f"{k}={v:.4f}" for k, v in kern.items()))
+    n_out = float(np.mean([len(x) for x in sc["lists"]]))
+    b_slide = 4 * dim * W + 4 * dim * S + 4 * n_out
+    achieved = b_slide / (dom_ms * 1e-3) / 1e9
     traffic = measured_traffic(args.workload, dom)
+    log(f"[bench] pipelined launch {dom}: {dom_ms * 1e3:.2f} us; B_slide {b_slide:.0f} B")
 
-    others = None
-    if not args.no_others and args.workload == "walk1d-W100k-S5k":
-        others = other_workloads(["walk1d-W10k-S500", "gmm3d-W10k-S500", "gmm3d-W100k-S5k"],
-                                 120, args.cpu_slides, not args.no_cpu)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(xs, W, S, R, K, args.cpu_slides, threads)
+        cpu["fp64_strict_disagreements"] = fp64_disagreements(wl.data(), W, S, wl.T, R, K,
+                                                              threads)
+        log(f"[bench] cpu: {cpu}")
 
-    indexes = None
+    others = sweep = indexes = None
     if not args.no_others and args.workload == "walk1d-W100k-S5k":
-        indexes = index_comparison(["walk1d-W100k-S5k", "gmm3d-W10k-S500"], 60)
+        others = {}
+        for name in ("walk1d-W10k-S500", "gmm3d-W10k-S500", "gmm3d-W100k-S5k"):
+            others[name] = one_workload(name, 10, 120, args.cpu_slides, not args.no_cpu, threads)
+            if not args.no_cpu and name != "gmm3d-W100k-S5k":
+                w2 = workloads()[name]
+                others[name]["fp64_strict_disagreements"] = fp64_disagreements(
+                    w2.data(), w2.W, w2.S, w2.T, w2.R, w2.K, threads)
+            log(f"[bench] {name}: {others[name]}")
+        sweep = sweeps(5, 40, 30, not args.no_cpu, threads)
+        indexes = index_comparison(["walk1d-W100k-S5k", "gmm3d-W10k-S500"], 10, 40)
 
     batch = None
     if not args.no_batched and args.workload == "walk1d-W100k-S5k":
-        batch = batched(args.batch_streams, 5, 40, 4, 3, not args.no_cpu)
+        batch = batched(args.batch_streams, 40, 4, 3, not args.no_cpu, threads)
 
     learned = None
     if not args.no_learned:
         names = [args.workload]
         if not args.no_others and args.workload == "walk1d-W100k-S5k":
             names += ["walk1d-W10k-S500", "gmm3d-W10k-S500", "gmm3d-W100k-S5k"]
-        learned = learned_comparison(names, args.train_slides, args.steps)
+        learned = learned_comparison(names, args.train_slides, min(args.steps, 120))
 
-    cpu = None
-    if not args.no_cpu:
-        us_mt, us_st, threads = cpu_baseline(xs, W, S, R, K, args.cpu_slides)
-        best_mt = us_mt < us_st
-        cpu = {"value": round(min(us_mt, us_st), 2), "unit": "us/slide",
-               "cores": threads if best_mt else 1, "kind": "port",
-               "sample": f"grid oracle (incremental per-cell FIFOs, C), slides "
-                         f"1..{args.cpu_slides} of the same stream, the faster of a "
-                         f"{threads}-thread pool ({us_mt:.1f} us/slide; range counts split "
-                         f"over the pool only when a slide has >= 256 candidates) and 1 "
-                         f"thread ({us_st:.1f} us/slide)"}
-        log(f"[bench] cpu grid oracle: {us_mt:.1f} us/slide ({threads} thr), {us_st:.1f} (1 thr)")
-
-    value = pipe["us"]
+    invalid = mism
+    invalid += sum(v["mismatched_slides"] for v in (others or {}).values())
+    invalid += sum(v["mismatched_slides"] for v in (sweep or {}).values())
+    invalid += sum(v["mismatched_slides"] for v in (indexes or {}).values())
+    invalid += sum(v["mismatched_slides"] for v in (learned or {}).values())
+    if batch:
+        invalid += batch["learned"]["mismatched_steps"]
+    value = hp["us"]
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -507,30 +581,36 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic: seeded 1-D random walk (start 10, N(0,0.05) steps, 0.2% spikes "
                 "+-U(2,5)); 3-D: mixture of 8 drifting anisotropic Gaussians + 0.5% noise",
-        "config": {"workload": f"{args.workload} W={W} S={S} R={R} K={K} dim={dim} "
-                               "(plain schedule, cell-stencil index; consecutive slides "
+        "config": {"workload": f"{args.workload} W={W} S={S} R={R} K={K} dim={dim} (plain "
+                               "schedule, cell-stencil index; slides from host memory, "
                                "pipelined on one stream, one launch per slide)",
                    "window": W, "slide": S, "R": R, "K": K, "dim": dim},
-        "sequential_us_per_slide": round(plain["lib_ms"] * 1e3, 3),
-        "pipelined_mismatched_slides": pipe_mism,
-        "outer_us_per_slide": round(plain["outer_ms"] * 1e3, 3),
-        "pcie_inclusive_us_per_slide": round(pcie["lib_ms"] * 1e3, 3),
-        "outliers_per_slide": plain["outliers_mean"],
-        "candidates_per_slide": plain["stats"]["candidates_last"],
-        "kernels_ms": {k: round(v, 5) for k, v in kern.items()},
-        "kernels_ms_one_slide_per_call": {k: round(v, 5) for k, v in kern_seq.items()},
+        "valid": invalid == 0,
+        "mismatched_slides": int(mism),
+        "latency_us_per_slide": round(sc["us"], 3),
+        "latency_outer_us_per_slide": round(sc["outer_us"], 3),
+        "breakdown_ms": sc["buckets_ms"],
+        "stages_ms": sc["stages_ms"],
+        "pipelined_hbm_us_per_slide": round(dp["us"], 3),
+        "outliers_per_slide": round(n_out, 2),
+        "candidates_per_slide": round((sc["stats"]["candidates_total"] -
+                                       sc["stats_warm"]["candidates_total"]) / args.steps, 1),
+        "kernels_ms_pipelined": {k: round(v, 5) for k, v in kern.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 3),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
-                     "algorithmic_bytes": int(b_dom), "kernel_avg_ms": round(dom_ms, 5),
-                     "slide_bytes": int(b_slide),
+                     "traffic_over_algorithmic": round(traffic / b_slide, 2) if traffic else None,
+                     "algorithmic_bytes": int(b_slide), "kernel_avg_ms": round(dom_ms, 5),
                      "slide_frac": round(b_slide / (value * 1e-6) / 1e9 / HBM_PEAK_GBS, 6)},
         "cpu_baseline": cpu,
     }
     if cpu:
         line["speedup_vs_cpu"] = round(cpu["value"] / value, 2)
+        line["latency_speedup_vs_cpu"] = round(cpu["value"] / sc["us"], 2)
     if others:
         line["other_workloads"] = others
+    if sweep:
+        line["sweeps"] = sweep
     if indexes:
         line["index_comparison"] = indexes
     if learned:
@@ -538,7 +618,7 @@ def main() -> int:
     if batch:
         line["batched"] = batch
     print(json.dumps(line), flush=True)
-    return 0
+    return 0 if invalid == 0 else 1
 
 
 if __name__ == "__main__":
