@@ -247,6 +247,18 @@ RTOD_API rtod_status rtod_mlp_forward(int32_t device, const float *params, const
 RTOD_API rtod_status rtod_mlp_grad(int32_t device, const float *params, const float *X,
                                    const float *y, int32_t n, float *grad, float *loss);
 
+/* rtod_mlp_grad that allocates nothing: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code that forbids hipMalloc,
+ * tight serving or training loops): the parameters, samples, gradient and
+ * loss are staged in a fixed device scratch area of the library (14592
+ * bytes per GPU, held until the process ends) instead of per-call
+ * allocations, then the same kernel runs.  n in [1, 128]; larger n is
+ * rejected (RTOD_ERR_INVALID_ARGUMENT).  Every call on one GPU shares its
+ * area: calls that overlap on one GPU (from several threads) race. */
+RTOD_API rtod_status rtod_mlp_grad_static(int32_t device, const float *params, const float *X,
+                                          const float *y, int32_t n, float *grad,
+                                          float *loss);
+
 /* ---- Batched streams ----------------------------------------------------
  * No reference counterpart: the reference runs one stream per process.  A
  * batch advances n independent streams (each its own dim, W, S, R, K; plain

@@ -102,6 +102,8 @@ SIGNATURES = [
     ("rtod_mlp_forward", C.c_int, [C.c_int32, _F32P, _F32P, C.c_int32, _F32P]),
     ("rtod_mlp_grad", C.c_int, [C.c_int32, _F32P, _F32P, _F32P, C.c_int32, _F32P,
                                 C.POINTER(C.c_float)]),
+    ("rtod_mlp_grad_static", C.c_int, [C.c_int32, _F32P, _F32P, _F32P, C.c_int32, _F32P,
+                                       C.POINTER(C.c_float)]),
     ("rtod_batch_create", C.c_int, [C.POINTER(_Config), C.c_int32, C.c_int32,
                                     C.POINTER(C.c_void_p)]),
     ("rtod_batch_prime", C.c_int, [C.c_void_p, _PP]),
@@ -455,6 +457,18 @@ def mlp_grad(params: np.ndarray, X: np.ndarray, y: np.ndarray, device: int = 0):
     _check(lib().rtod_mlp_grad(device, np.ascontiguousarray(params, np.float32).ravel(),
                                X.ravel(), np.ascontiguousarray(y, np.float32).ravel(),
                                X.shape[0], g, C.byref(loss)))
+    return float(loss.value), g
+
+
+def mlp_grad_static(params: np.ndarray, X: np.ndarray, y: np.ndarray, device: int = 0):
+    """mlp_grad through rtod_mlp_grad_static (the library's fixed scratch
+    area, no allocation; n <= 128)."""
+    X = np.ascontiguousarray(X, np.float32).reshape(-1, 6)
+    g = np.empty(MLP_NPARAMS, np.float32)
+    loss = C.c_float(0.0)
+    _check(lib().rtod_mlp_grad_static(device, np.ascontiguousarray(params, np.float32).ravel(),
+                                      X.ravel(), np.ascontiguousarray(y, np.float32).ravel(),
+                                      X.shape[0], g, C.byref(loss)))
     return float(loss.value), g
 
 

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -28,9 +29,19 @@ using namespace rtod;
 
 namespace {
 
-thread_local std::string g_last_error = "";
+// the last error message of this thread, in a fixed buffer (setting it
+// allocates nothing)
+thread_local char g_last_error[1024] = "";
 
-void set_error(const std::string &s) { g_last_error = s; }
+void set_error(const char *s)
+{
+    std::snprintf(g_last_error, sizeof(g_last_error), "%s", s);
+}
+void set_error(const std::string &s) { set_error(s.c_str()); }
+void set_error_hip(const char *what, hipError_t e)
+{
+    std::snprintf(g_last_error, sizeof(g_last_error), "%s: %s", what, hipGetErrorString(e));
+}
 
 #define RTOD_HIP(call)                                                                            \
     do {                                                                                          \
@@ -1947,6 +1958,91 @@ RTOD_API rtod_status rtod_mlp_grad(int32_t device, const float *params, const fl
     return RTOD_OK;
 }
 
+// ---- rtod_mlp_grad_static: the gradient in a fixed device scratch area ----
+// Regions of `scratch` (floats, each starting on a 256-byte boundary): the
+// parameters, kTrainRows x kMlpIn features, kTrainRows targets, the
+// gradient and the loss.  k_mlp_grad reads only the first n rows and
+// overwrites the whole gradient and the loss, so nothing in the area has to
+// start at zero; a call leaves its inputs and results there.
+namespace {
+constexpr size_t kScrAlign = 64; // floats: 256 bytes
+constexpr size_t scr_round(size_t n) { return (n + kScrAlign - 1) / kScrAlign * kScrAlign; }
+constexpr size_t kScrParams = 0;
+constexpr size_t kScrX = kScrParams + scr_round(kMlpNP);
+constexpr size_t kScrY = kScrX + scr_round((size_t)kTrainRows * kMlpIn);
+constexpr size_t kScrGrad = kScrY + scr_round(kTrainRows);
+constexpr size_t kScrLoss = kScrGrad + scr_round(kMlpNP);
+constexpr size_t kScrFloats = kScrLoss + kScrAlign; // 3648 floats, 14592 bytes
+constexpr int kScrDevices = 64;
+
+std::mutex g_scr_mu;
+float *g_scr_ptr[kScrDevices] = {}; // device address of `scratch`, per device
+} // namespace
+
+static __device__ __align__(256) float scratch[kScrFloats];
+
+namespace {
+// this device's copy of `scratch` (the current device must be `device`)
+rtod_status scratch_of(int device, float **out)
+{
+    std::lock_guard<std::mutex> lk(g_scr_mu);
+    if (!g_scr_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error_hip("hipGetSymbolAddress(scratch)", e);
+            return RTOD_ERR_HIP;
+        }
+        g_scr_ptr[device] = static_cast<float *>(p);
+    }
+    *out = g_scr_ptr[device];
+    return RTOD_OK;
+}
+} // namespace
+
+// HIP check whose error path allocates nothing either
+#define RTOD_HIP_NA(call)                                                                         \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) {                                                                   \
+            set_error_hip(#call, e_);                                                             \
+            return RTOD_ERR_HIP;                                                                  \
+        }                                                                                         \
+    } while (0)
+
+RTOD_API rtod_status rtod_mlp_grad_static(int32_t device, const float *params, const float *X,
+                                          const float *y, int32_t n, float *grad, float *loss)
+{
+    if (!params || !X || !y || !grad || !loss) {
+        set_error("NULL argument");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (n < 1 || n > kTrainRows) {
+        set_error("n must be in [1, 128]: the static scratch area holds 128 samples");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    int ndev = 0;
+    RTOD_HIP_NA(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev || device >= kScrDevices) {
+        set_error("no such device");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    RTOD_HIP_NA(hipSetDevice(device));
+    float *scr = nullptr;
+    RTOD_TRY(scratch_of(device, &scr));
+    RTOD_HIP_NA(hipMemcpy(scr + kScrParams, params, sizeof(float) * kMlpNP, hipMemcpyHostToDevice));
+    RTOD_HIP_NA(hipMemcpy(scr + kScrX, X, sizeof(float) * kMlpIn * (size_t)n,
+                       hipMemcpyHostToDevice));
+    RTOD_HIP_NA(hipMemcpy(scr + kScrY, y, sizeof(float) * (size_t)n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_mlp_grad, dim3(1), dim3(kTrainNT), 0, nullptr,
+                       (const float *)(scr + kScrParams), (const float *)(scr + kScrX),
+                       (const float *)(scr + kScrY), n, scr + kScrGrad, scr + kScrLoss);
+    RTOD_HIP_NA(hipGetLastError());
+    RTOD_HIP_NA(hipMemcpy(grad, scr + kScrGrad, sizeof(float) * kMlpNP, hipMemcpyDeviceToHost));
+    RTOD_HIP_NA(hipMemcpy(loss, scr + kScrLoss, sizeof(float), hipMemcpyDeviceToHost));
+    return RTOD_OK;
+}
+
 RTOD_API rtod_status rtod_kernel_times(const rtod_ctx *c, const char **names, double *avg_ms,
                                        int64_t *launches, int32_t *n)
 {
@@ -2054,7 +2150,7 @@ RTOD_API const char *rtod_status_str(rtod_status s)
     return "unknown status";
 }
 
-RTOD_API const char *rtod_last_error(void) { return g_last_error.c_str(); }
+RTOD_API const char *rtod_last_error(void) { return g_last_error; }
 
 RTOD_API const char *rtod_version(void) { return "rtod-mi355x 0.2 (gfx950, sources " RTOD_SRC_HASH ")"; }
 
