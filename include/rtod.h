@@ -50,14 +50,32 @@ typedef enum rtod_sched_kind {
     RTOD_SCHED_LEARNED = 1 /* MLP-predicted ordering/bucketing, exact verify  */
 } rtod_sched_kind;
 
+/* Ablations of the reference's OPTIMIZATION macro (optixScan.h:22-24): the
+ * outlier sets never change, only the work done to find them. */
+typedef enum rtod_ablation {
+    RTOD_ABL_NONE = 0,        /* OPTIMIZATION 2 (default): grid filter, then
+                                 the candidates range-counted (cell stencil)
+                                 or an index over the candidates queried from
+                                 every window point (LBVH, the reference's
+                                 ray-BVH inversion)                          */
+    RTOD_ABL_FILTER_ONLY = 1, /* OPTIMIZATION 1: grid filter, the candidates
+                                 queried over an index of the whole window
+                                 (LBVH; with the cell stencil this is the
+                                 default computation)                        */
+    RTOD_ABL_NO_FILTER = 2    /* OPTIMIZATION 0: no grid filter, every window
+                                 point (or launch_ray_num rays) counted over
+                                 an index of the whole window                */
+} rtod_ablation;
+
 /* flags */
 #define RTOD_FLAG_PHASE_TIMES 0x1 /* record HIP events between the slide's
                                      kernels and fill rtod_times per phase  */
 
-/* Replaces the compile-time DIMENSION/MK/OPTIMIZATION macros and the runtime
- * flags --window --slide --R --K of the reference (optixScan.h:4-28,
- * optixScan.cpp:149-212).  dim in {1,2,3}; 1 <= S <= W, W % S == 0;
- * K >= 1; R finite with 1e-18 <= R <= 1e18. */
+/* Replaces the compile-time DIMENSION/MK/OPTIMIZATION/UPDATE_GAS_TYPE macros
+ * and the runtime flags --window --slide --R --K --launch_ray_num of the
+ * reference (optixScan.h:4-28, optixScan.cpp:149-212).  dim in {1,2,3};
+ * 1 <= S <= W, W % S == 0; K >= 1; R finite with 1e-18 <= R <= 1e18.  A
+ * zero-initialised tail (ablation, refit, launch_ray_num) is the default. */
 typedef struct rtod_config {
     int32_t dim;
     int32_t window;
@@ -70,6 +88,16 @@ typedef struct rtod_config {
                              schedule when the miss rate exceeds this      */
     int32_t flags;        /* RTOD_FLAG_*                                   */
     int32_t device;       /* HIP device ordinal                            */
+    int32_t ablation;     /* rtod_ablation (0: the default path)           */
+    int32_t refit;        /* UPDATE_GAS_TYPE 0 (optixScan.cpp:335-376): the
+                             whole-window LBVH keeps the topology built at
+                             rtod_prime and only refits its boxes each slide
+                             (LBVH with RTOD_ABL_FILTER_ONLY / NO_FILTER) */
+    int32_t launch_ray_num; /* --launch_ray_num (optixScan.cpp:200-206,
+                             738-745), RTOD_ABL_NO_FILTER only: rays cast,
+                             ray i from ring slot i mod W; 0 = W.  The list
+                             then holds the ray indices whose count is <= K
+                             (capacity max(W, launch_ray_num))            */
 } rtod_config;
 
 /* Per-slide time breakdown in milliseconds, the reference Timer buckets
@@ -131,7 +159,8 @@ RTOD_API rtod_status rtod_create(const rtod_config *cfg, rtod_ctx **out);
 RTOD_API rtod_status rtod_prime(rtod_ctx *ctx, const float *pts, int64_t n_pts);
 
 /* One slide: new_pts [S][dim] row-major; writes the window's outliers
- * (ring slots, ascending) to outlier_slots (capacity >= W) and their number
+ * (ring slots, ascending; ray indices with launch_ray_num > W) to
+ * outlier_slots (capacity >= max(W, launch_ray_num)) and their number
  * to *n_outliers; times may be NULL.  Replaces the body of the while loop of
  * detect_outlier (optixScan.cpp:1045-1110). */
 RTOD_API rtod_status rtod_slide(rtod_ctx *ctx, const float *new_pts, int32_t *outlier_slots,
@@ -164,7 +193,8 @@ RTOD_API rtod_status rtod_slide_device(rtod_ctx *ctx, const float *dev_new_pts,
  * rtod_slide_device calls:
  * slide i's outliers (ring slots, ascending) are
  * out_slots[out_offsets[i] .. out_offsets[i+1]); out_offsets holds
- * n_slides + 1 entries and out_slots at least n_slides * W (slots_cap). */
+ * n_slides + 1 entries and out_slots at least n_slides * max(W,
+ * launch_ray_num) (slots_cap). */
 RTOD_API rtod_status rtod_slides_device(rtod_ctx *ctx, const float *dev_pts, int32_t n_slides,
                                         int32_t *out_offsets, int32_t *out_slots,
                                         int64_t slots_cap);

@@ -27,6 +27,7 @@ LIB_PATH = os.path.join(_HERE, "librtod.so")
 
 INDEX_AUTO, INDEX_CELL_STENCIL, INDEX_LBVH = 0, 1, 2
 SCHED_PLAIN, SCHED_LEARNED = 0, 1
+ABL_NONE, ABL_FILTER_ONLY, ABL_NO_FILTER = 0, 1, 2
 FLAG_PHASE_TIMES = 0x1
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "unsupported", 3: "call out of order",
@@ -43,7 +44,8 @@ class _Config(C.Structure):
     _fields_ = [("dim", C.c_int32), ("window", C.c_int32), ("slide", C.c_int32),
                 ("R", C.c_float), ("K", C.c_int32), ("index_kind", C.c_int32),
                 ("sched_kind", C.c_int32), ("miss_threshold", C.c_float),
-                ("flags", C.c_int32), ("device", C.c_int32)]
+                ("flags", C.c_int32), ("device", C.c_int32), ("ablation", C.c_int32),
+                ("refit", C.c_int32), ("launch_ray_num", C.c_int32)]
 
 
 class _Times(C.Structure):
@@ -163,14 +165,16 @@ class Detector:
 
     def __init__(self, dim: int, window: int, slide: int, R: float, K: int,
                  index_kind: int = INDEX_AUTO, sched_kind: int = SCHED_PLAIN,
-                 miss_threshold: float = 0.25, flags: int = 0, device: int = 0):
+                 miss_threshold: float = 0.25, flags: int = 0, device: int = 0,
+                 ablation: int = ABL_NONE, refit: bool = False, launch_ray_num: int = 0):
         self.dim, self.W, self.S, self.R, self.K = dim, window, slide, R, K
         cfg = _Config(dim, window, slide, R, K, index_kind, sched_kind, miss_threshold, flags,
-                      device)
+                      device, ablation, 1 if refit else 0, launch_ray_num)
         h = C.c_void_p()
         _check(lib().rtod_create(C.byref(cfg), C.byref(h)))
         self._h = h
-        self._out = np.empty(window, np.int32)
+        self.cap = max(window, launch_ray_num)  # entries a slide's list may hold
+        self._out = np.empty(self.cap, np.int32)
         self._times = _Times()
 
     # -- lifecycle ---------------------------------------------------------
@@ -218,7 +222,7 @@ class Detector:
         """n_slides consecutive slides from HBM (slide i at dev_ptr + i*S*dim
         floats), pipelined; the same lists as n_slides slide_device() calls."""
         offs = np.zeros(n_slides + 1, np.int32)
-        slots = np.empty(max(1, n_slides * self.W), np.int32)
+        slots = np.empty(max(1, n_slides * self.cap), np.int32)
         _check(lib().rtod_slides_device(self._h, C.c_void_p(dev_ptr), n_slides, offs, slots,
                                         slots.size))
         return [slots[offs[i]:offs[i + 1]].copy() for i in range(n_slides)]
@@ -231,7 +235,7 @@ class Detector:
         if pts.size < n_slides * self.S * self.dim:
             raise ValueError("pts holds fewer than n_slides slides")
         offs = np.zeros(n_slides + 1, np.int32)
-        slots = np.empty(max(1, n_slides * self.W), np.int32)
+        slots = np.empty(max(1, n_slides * self.cap), np.int32)
         _check(lib().rtod_slides(self._h, pts, n_slides, offs, slots, slots.size))
         return [slots[offs[i]:offs[i + 1]].copy() for i in range(n_slides)]
 

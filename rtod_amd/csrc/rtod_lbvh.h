@@ -12,13 +12,17 @@
 //   k_lbvh_build   Karras radix-tree hierarchy (one thread per internal node)
 //   k_lbvh_boxes   leaf boxes rounded outward, unions bottom-up (second
 //                  arrival at a node continues)
-//   k_lbvh_query   every window point walks the tree (LDS stack); at a leaf:
-//                  early-out if the count already exceeds K, exact FP32 test,
-//                  atomicAdd -- counts stay exact up to K+1
+//   k_lbvh_query   every window point walks the tree (stackless, by parent
+//                  links); at a leaf: early-out if the count already exceeds
+//                  K, exact FP32 test, atomicAdd -- counts stay exact up to K+1
 //   k_lbvh_select  count <= K -> outlier bit (the slide epilogue, k_finish,
 //                  lists them)
 // The number of candidates U lives on the device; kernels are sized for W
 // and exit past U.
+// Ablations (the reference's OPTIMIZATION 0/1 and UPDATE_GAS_TYPE 0): a tree
+// over the whole window (leaf items = ring slots, U = W), queried from the
+// candidates or from every slot by k_lbvh_count, rebuilt each slide or built
+// once at prime and refit (k_lbvh_unflag + k_lbvh_boxes).
 #pragma once
 
 #include "rtod_plain.h"
@@ -35,7 +39,11 @@ struct Lbvh {
     int32_t *flag;     // [U] bottom-up arrival counters
     int32_t *count;    // [U] neighbour counts per sorted leaf
     int ntiles_log2;   // merge passes for U up to W
-    const int32_t *n_cand; // the slide's candidate count (set per launch)
+    const int32_t *n_cand; // leaves U: the slide's candidate count (set per
+                           // launch), or n_whole
+    int whole;         // the tree holds the whole window: leaf items are ring
+                       // slots (ablations FILTER_ONLY / NO_FILTER)
+    int32_t *n_whole;  // [1] = W, the leaf count of a whole-window tree
 };
 
 // Morton interleave of up to three 21-bit (3-D) or two 31-bit (2-D) fields
@@ -81,8 +89,16 @@ __global__ __launch_bounds__(256) void k_lbvh_codes(Dev d, Step st, Lbvh L)
     const int U = *L.n_cand;
     if (i >= U)
         return;
-    const uint64_t *Nk = d.akey[st.par ^ 1];
-    L.code[0][i] = morton_of_key<DIM>(Nk[d.cand[i]]);
+    if (L.whole) { // leaf i: ring slot i, the point's cell from its coordinates
+        float p[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+            p[a] = d.ring[(size_t)a * d.W + i];
+        L.code[0][i] = morton_of_key<DIM>(cell_key<DIM>(p, d.inv_c));
+    } else {
+        const uint64_t *Nk = d.akey[st.par ^ 1];
+        L.code[0][i] = morton_of_key<DIM>(Nk[d.cand[i]]);
+    }
     L.item[0][i] = i;
 }
 
@@ -276,8 +292,10 @@ __global__ __launch_bounds__(256) void k_lbvh_boxes(Dev d, Step st, Lbvh L)
     const int U = *L.n_cand;
     if (i >= U)
         return;
-    const float *Nx = d.axyz[st.par ^ 1];
-    const int p = d.cand[L.item[1][i]];
+    // the leaf's point: a candidate in the sorted window, or (whole-window
+    // tree) the point in ring slot item
+    const float *Nx = L.whole ? d.ring : d.axyz[st.par ^ 1];
+    const int p = L.whole ? L.item[1][i] : d.cand[L.item[1][i]];
     const int leaf = U - 1 + i;
     float *bx = L.box + (size_t)leaf * 6;
 #pragma unroll
@@ -318,16 +336,64 @@ __global__ __launch_bounds__(256) void k_lbvh_boxes(Dev d, Step st, Lbvh L)
     }
 }
 
-constexpr int kLbvhStack = 64;
+// Node ids of the walk: internal nodes 0..U-2, leaf j at U-1+j (the layout
+// of L.parent and L.box); a child entry c of L.left/L.right is c >= 0 for an
+// internal node, ~j for leaf j.
+__device__ __forceinline__ int lbvh_node(int c, int U) { return c >= 0 ? c : U - 1 + ~c; }
 
-// Every window point (sorted order, so neighbouring threads walk similar
-// paths) descends the tree; at a leaf whose box holds it: early-out when the
-// candidate's count already exceeds K (racy read; counts only grow), exact
-// FP32 test, atomicAdd.
+// Stackless depth-first walk over the nodes whose box holds x, by parent
+// links: arriving from the parent, test the box (descend left if it holds
+// x, else go back up); arriving from the left child, descend right; from
+// the right child, go up.  The root's parent is -1.  No stack, so no LDS.
+// at_leaf(j) handles leaf j whose box holds x and returns false to stop the
+// walk (early exit).  A walk enters each of the 2U - 1 nodes at most three
+// times; past that bound (a malformed tree) it raises kErrLbvhWalk and stops.
+template <int DIM, typename F>
+__device__ __forceinline__ void lbvh_walk(const Lbvh &L, int U, const float (&x)[3], int32_t *err,
+                                          F at_leaf)
+{
+    int cur = 0, prev = -1;
+    const long long bound = 3ll * (2ll * U);
+    for (long long steps = 0; cur >= 0; ++steps) {
+        if (steps > bound) {
+            atomicOr(err, kErrLbvhWalk);
+            return;
+        }
+        const int par = L.parent[cur];
+        int next;
+        if (prev == par) { // coming down
+            const float *b = L.box + (size_t)cur * 6;
+            bool in = true;
+#pragma unroll
+            for (int a = 0; a < DIM; ++a)
+                in = in && (x[a] >= b[a]) && (x[a] <= b[3 + a]);
+            if (!in) {
+                next = par;
+            } else if (cur >= U - 1) {
+                if (!at_leaf(cur - (U - 1)))
+                    return;
+                next = par;
+            } else {
+                next = lbvh_node(L.left[cur], U);
+            }
+        } else if (prev == lbvh_node(L.left[cur], U)) {
+            next = lbvh_node(L.right[cur], U);
+        } else {
+            next = par;
+        }
+        prev = cur;
+        cur = next;
+    }
+}
+
+// Default index (boxes of the slide's candidates): every window point
+// (sorted order, so neighbouring threads walk similar paths) walks the tree;
+// at a leaf whose box holds it: early-out when the candidate's count
+// already exceeds K (racy read; counts only grow), exact FP32 test,
+// atomicAdd.
 template <int DIM>
 __global__ __launch_bounds__(256) void k_lbvh_query(Dev d, Step st, Lbvh L)
 {
-    __shared__ int s_stack[256 * kLbvhStack];
     stamp_start(d, kTsDetect);
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     const int U = *L.n_cand;
@@ -341,46 +407,84 @@ __global__ __launch_bounds__(256) void k_lbvh_query(Dev d, Step st, Lbvh L)
         for (int a = 0; a < DIM; ++a)
             x[a] = Nx[(size_t)a * d.W + q];
     const int K = d.K;
-    int *stk = s_stack + threadIdx.x; // stride 256: conflict-free
-    int sp = 0;
-    int node = (U == 1) ? ~0 : 0;
     long long tests = 0;
-    for (; active;) {
-        const int slot = node >= 0 ? node : U - 1 + ~node;
-        const float *b = L.box + (size_t)slot * 6;
-        bool in = true;
-#pragma unroll
-        for (int a = 0; a < DIM; ++a)
-            in = in && (x[a] >= b[a]) && (x[a] <= b[3 + a]);
-        if (in) {
-            if (node < 0) {
-                const int leaf = ~node;
-                if (__hip_atomic_load(&L.count[leaf], __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT) <= K) {
-                    const int p = d.cand[L.item[1][leaf]];
-                    const float qx = Nx[p];
-                    const float qy = DIM > 1 ? Nx[(size_t)d.W + p] : 0.f;
-                    const float qz = DIM > 2 ? Nx[(size_t)2 * d.W + p] : 0.f;
-                    ++tests;
-                    if (dist2<DIM>(x, qx, qy, qz) <= d.r2)
-                        atomicAdd(&L.count[leaf], 1);
-                }
-            } else if (sp + 2 <= kLbvhStack) {
-                stk[(sp++) * 256] = L.right[node];
-                node = L.left[node];
-                continue;
-            } else {
-                atomicOr(&d.ctl->err, kErrLbvhStack); // cannot happen below depth 64
+    if (active)
+        lbvh_walk<DIM>(L, U, x, &d.ctl->err, [&](int leaf) {
+            if (__hip_atomic_load(&L.count[leaf], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <=
+                K) {
+                const int p = d.cand[L.item[1][leaf]];
+                const float qx = Nx[p];
+                const float qy = DIM > 1 ? Nx[(size_t)d.W + p] : 0.f;
+                const float qz = DIM > 2 ? Nx[(size_t)2 * d.W + p] : 0.f;
+                ++tests;
+                if (dist2<DIM>(x, qx, qy, qz) <= d.r2)
+                    atomicAdd(&L.count[leaf], 1);
             }
-        }
-        if (sp == 0)
-            break;
-        node = stk[(--sp) * 256];
-    }
+            return true;
+        });
     for (int dd = 1; dd < kWave; dd <<= 1)
         tests += __shfl_xor(tests, dd, kWave);
     if (lane_id() == 0 && tests)
         atomicAdd(&cnt_of(d, st.t)->pair_tests, (unsigned long long)tests);
+}
+
+// Whole-window tree (the reference's OPTIMIZATION 0/1: a GAS over every
+// window point, rays counting hits and terminating once past K,
+// optixScan.cu:80-156): query q walks the tree, counts the leaves whose
+// point passes the FP32 test (its own leaf included) and stops at K+1; a
+// count <= K sets the query point's outlier bit.  cands != 0: the queries
+// are the slide's candidates (sorted-window positions, OPTIMIZATION 1);
+// else the ring slots [0, nq) (OPTIMIZATION 0, nq = min(W, launch_ray_num
+// or W): rays past W repeat slots and are expanded on the host).
+template <int DIM>
+__global__ __launch_bounds__(256) void k_lbvh_count(Dev d, Step st, Lbvh L, int cands, int nq)
+{
+    stamp_start(d, kTsDetect);
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int U = *L.n_whole;
+    SlideCnt *cn = cnt_of(d, st.t);
+    if (!cands && q == 0)
+        cn->n_cand = nq; // every query is a candidate
+    const int n = cands ? cn->n_cand : nq;
+    long long tests = 0;
+    if (q < n) {
+        const float *Qx = cands ? d.axyz[st.par ^ 1] : d.ring;
+        const int qp = cands ? d.cand[q] : q;
+        const int slot = cands ? d.aslot[st.par ^ 1][qp] : q;
+        float x[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+            x[a] = Qx[(size_t)a * d.W + qp];
+        const int K = d.K;
+        int cnt = 0;
+        lbvh_walk<DIM>(L, U, x, &d.ctl->err, [&](int leaf) {
+            const int p = L.item[1][leaf];
+            const float qx = d.ring[p];
+            const float qy = DIM > 1 ? d.ring[(size_t)d.W + p] : 0.f;
+            const float qz = DIM > 2 ? d.ring[(size_t)2 * d.W + p] : 0.f;
+            ++tests;
+            cnt += dist2<DIM>(x, qx, qy, qz) <= d.r2 ? 1 : 0;
+            return cnt <= K;
+        });
+        if (cands)
+            d.cand_cnt[q] = min(cnt, K + 1);
+        if (cnt <= K)
+            atomicOr(&bitmap_of(d, st.t)[slot >> 5], 1u << (slot & 31));
+    }
+    for (int dd = 1; dd < kWave; dd <<= 1)
+        tests += __shfl_xor(tests, dd, kWave);
+    if (lane_id() == 0 && tests)
+        atomicAdd(&cn->pair_tests, (unsigned long long)tests);
+}
+
+// refit of a whole-window tree: internal nodes' arrival counters cleared
+// before k_lbvh_boxes recomputes every box over the kept topology
+__global__ __launch_bounds__(256) void k_lbvh_unflag(Dev d, Lbvh L)
+{
+    stamp_start(d, kTsIndex);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < *L.n_whole - 1)
+        L.flag[i] = 0;
 }
 
 // outlier bits from the leaf counts (the slide epilogue, k_finish, follows)

@@ -146,6 +146,9 @@ struct rtod_ctx {
     const float *dev_src = nullptr; // device-resident new slide of this call
     bool lbvh = false;
     bool learned = false;
+    bool refit = false;      // whole-window LBVH: boxes refit each slide
+    int ray_num = 0;         // launch_ray_num (no grid filter; 0: W)
+    int list_cap = 0;        // entries a slide's list may hold: max(W, ray_num)
     Lbvh L{};
     int lbvh_passes = 0;
     std::vector<void *> dev_allocs;
@@ -490,7 +493,9 @@ void launch_detect_stage(rtod_ctx *c, hipStream_t s, const Step &st, const Stage
 void launch_lbvh_build(rtod_ctx *c, hipStream_t s, const Step &st)
 {
     Lbvh L = c->L;
-    L.n_cand = &c->d.ctl->cnt[st.t % kSlideSets].n_cand; // device address (not read here)
+    // the leaf count: the slide's candidates (device address, read by the
+    // kernels), or W for a whole-window tree
+    L.n_cand = L.whole ? L.n_whole : &c->d.ctl->cnt[st.t % kSlideSets].n_cand;
     const dim3 gw((c->W + 255) / 256), b(256);
     if (c->dim == 1)
         hipLaunchKernelGGL(k_lbvh_codes<1>, gw, b, 0, s, c->d, st, L);
@@ -516,8 +521,46 @@ void launch_lbvh_build(rtod_ctx *c, hipStream_t s, const Step &st)
         hipLaunchKernelGGL(k_lbvh_boxes<3>, gw, b, 0, s, c->d, st, L);
 }
 
+// whole-window LBVH, refit: the boxes recomputed over the topology built at
+// rtod_prime (the reference's UPDATE_GAS_TYPE 0, optixScan.cpp:335-376)
+void launch_lbvh_refit(rtod_ctx *c, hipStream_t s, const Step &st)
+{
+    Lbvh L = c->L;
+    L.n_cand = L.n_whole;
+    const dim3 gw((c->W + 255) / 256), b(256);
+    hipLaunchKernelGGL(k_lbvh_unflag, gw, b, 0, s, c->d, L);
+    if (c->dim == 1)
+        hipLaunchKernelGGL(k_lbvh_boxes<1>, gw, b, 0, s, c->d, st, L);
+    else if (c->dim == 2)
+        hipLaunchKernelGGL(k_lbvh_boxes<2>, gw, b, 0, s, c->d, st, L);
+    else
+        hipLaunchKernelGGL(k_lbvh_boxes<3>, gw, b, 0, s, c->d, st, L);
+}
+
+// whole-window LBVH: the queries (candidates, or ring slots) count their
+// neighbours over it
+void launch_lbvh_count(rtod_ctx *c, hipStream_t s, const Step &st, bool stamps)
+{
+    Lbvh L = c->L;
+    L.n_cand = L.n_whole;
+    const int cands = c->cfg.ablation == RTOD_ABL_FILTER_ONLY ? 1 : 0;
+    const int nq = cands ? c->W : std::min(c->W, c->ray_num > 0 ? c->ray_num : c->W);
+    const dim3 g((nq + 255) / 256), b(256);
+    if (c->dim == 1)
+        hipLaunchKernelGGL(k_lbvh_count<1>, g, b, 0, s, c->d, st, L, cands, nq);
+    else if (c->dim == 2)
+        hipLaunchKernelGGL(k_lbvh_count<2>, g, b, 0, s, c->d, st, L, cands, nq);
+    else
+        hipLaunchKernelGGL(k_lbvh_count<3>, g, b, 0, s, c->d, st, L, cands, nq);
+    launch_finish(c, s, st, stamps);
+}
+
 void launch_lbvh_detect(rtod_ctx *c, hipStream_t s, const Step &st, bool stamps = false)
 {
+    if (c->L.whole) {
+        launch_lbvh_count(c, s, st, stamps);
+        return;
+    }
     Lbvh L = c->L;
     L.n_cand = &c->d.ctl->cnt[st.t % kSlideSets].n_cand;
     const dim3 gw((c->W + 255) / 256), b(256);
@@ -592,7 +635,10 @@ void launch_core_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t *e
         (void)hipEventRecord(ev[4], s);
     const bool runs = next && c->d.cpb > 1;
     if (c->lbvh) {
-        launch_lbvh_build(c, s, st);
+        if (c->refit)
+            launch_lbvh_refit(c, s, st);
+        else
+            launch_lbvh_build(c, s, st);
         if (ev)
             (void)hipEventRecord(ev[5], s);
         launch_lbvh_detect(c, s, st, stamps);
@@ -648,6 +694,31 @@ rtod_status validate(const rtod_config *cfg)
         cfg->sched_kind > 1) {
         set_error("unknown index_kind or sched_kind");
         return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (cfg->ablation < RTOD_ABL_NONE || cfg->ablation > RTOD_ABL_NO_FILTER) {
+        set_error("unknown ablation");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (cfg->refit != 0 && cfg->refit != 1) {
+        set_error("refit must be 0 or 1");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (cfg->launch_ray_num < 0 ||
+        (cfg->launch_ray_num > 0 && cfg->ablation != RTOD_ABL_NO_FILTER)) {
+        set_error("launch_ray_num (>= 0) applies to RTOD_ABL_NO_FILTER only, as the "
+                  "reference's --launch_ray_num applies to OPTIMIZATION 0");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    if (cfg->refit &&
+        (cfg->index_kind != RTOD_INDEX_LBVH || cfg->ablation == RTOD_ABL_NONE)) {
+        set_error("refit applies to an LBVH over the whole window (RTOD_INDEX_LBVH with "
+                  "RTOD_ABL_FILTER_ONLY or RTOD_ABL_NO_FILTER): the default LBVH holds the "
+                  "slide's candidates, which change every slide");
+        return RTOD_ERR_UNSUPPORTED;
+    }
+    if (cfg->sched_kind == RTOD_SCHED_LEARNED && cfg->ablation != RTOD_ABL_NONE) {
+        set_error("the learned scheduler orders the default path's range counts");
+        return RTOD_ERR_UNSUPPORTED;
     }
     if (cfg->sched_kind == RTOD_SCHED_LEARNED) {
         if (cfg->index_kind == RTOD_INDEX_LBVH) {
@@ -874,7 +945,17 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
         for (size_t Lr = 4096; Lr < W; Lr *= 2)
             ++passes;
         c->lbvh_passes = passes;
+        // the ablations' tree over the whole window (leaf items: ring slots)
+        L.whole = cfg->ablation != RTOD_ABL_NONE ? 1 : 0;
+        RTOD_TRY(dalloc(c, &L.n_whole, 1));
+        const int32_t w32 = (int32_t)W;
+        RTOD_HIP(hipMemcpy(L.n_whole, &w32, sizeof(w32), hipMemcpyHostToDevice));
     }
+    c->refit = cfg->refit != 0;
+    c->ray_num = cfg->launch_ray_num;
+    c->list_cap = std::max(c->W, c->ray_num);
+    d.no_filter = cfg->ablation == RTOD_ABL_NO_FILTER ? 1 : 0;
+    d.ray_lim = (!c->lbvh && d.no_filter && c->ray_num > 0 && c->ray_num < c->W) ? c->ray_num : 0;
     c->learned = cfg->sched_kind == RTOD_SCHED_LEARNED;
     d.lr.on = c->learned ? 1 : 0;
     {
@@ -986,6 +1067,10 @@ rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n,
     hipLaunchKernelGGL(k_window_samples,
                        dim3((std::max(c->d.nsamp, c->d.nsamp4) + 255) / 256), dim3(256), 0, s, c->d,
                        0);
+    // refit: the whole-window tree is built once, over the initial window
+    // (the ring the ingest just filled), and only refit by the slides
+    if (c->refit)
+        launch_lbvh_build(c, s, st);
     RTOD_HIP(hipMemsetAsync(c->d.bitmap, 0, sizeof(uint32_t) * kSlideSets * c->d.bm_words, s));
     RTOD_HIP(hipMemsetAsync(c->d.ctl, 0, sizeof(Ctl), s));
     if (c->learned) { // replay and per-slide counters restart; weights stay
@@ -1013,7 +1098,7 @@ std::string err_text(int32_t err)
                  {kErrMergeNew, "a window-merge new point ranked out of range"},
                  {kErrStencilRows, "a detect unit's stencil exceeded 32 rows"},
                  {kErrClassifyWait, "a detect timed out waiting for its classify"},
-                 {kErrLbvhStack, "an LBVH query overflowed its traversal stack"}};
+                 {kErrLbvhWalk, "an LBVH walk exceeded its step bound (malformed tree)"}};
     std::string s;
     for (const auto &m : kMsgs)
         if (err & m.bit)
@@ -1045,10 +1130,23 @@ rtod_status collect(rtod_ctx *c, int32_t *outs, int32_t *n_out)
         set_error("outlier count out of range");
         return RTOD_ERR_INTERNAL;
     }
-    if (outs && n)
+    int nl = n;
+    if (c->ray_num > c->W) {
+        // launch_ray_num > W: ray r starts from slot r mod W (as the
+        // reference's raygen, optixScan.cu:29-32), so the ray indices whose
+        // count is <= K are the listed slots repeated every W, below ray_num
+        nl = 0;
+        for (int64_t base = 0; base < c->ray_num; base += c->W)
+            for (int i = 0; i < n && base + ho->list[i] < c->ray_num; ++i)
+                if (outs)
+                    outs[nl++] = (int32_t)(base + ho->list[i]);
+                else
+                    ++nl;
+    } else if (outs && n) {
         std::memcpy(outs, ho->list, sizeof(int32_t) * (size_t)n);
+    }
     if (n_out)
-        *n_out = n;
+        *n_out = nl;
     c->host_t++;
     c->units_last = ho->n_units;
     c->cand_last = ho->n_cand;
@@ -1520,13 +1618,17 @@ rtod_status slides_impl(rtod_ctx *c, const SlideSrc &src, int32_t n, int32_t *of
         const Step nx = more ? make_step_at(c, t + 1, slide_src(c, src, t0, i + 1)) : Step{};
         if ((s = guard_trainer(c, (int64_t)t, c->stream)) != RTOD_OK)
             break;
-        const bool fused = more && c->ride;
+        // the next slide's ingest writes its points into the ring: it rides in
+        // this slide's launches unless they read the ring (a whole-window
+        // LBVH, whose leaves are ring slots), then it follows them
+        const bool ride = c->ride && !c->L.whole;
+        const bool fused = more && ride;
         const bool defer = more && !c->lbvh;
         launch_core_stage(c, c->stream, st, timed ? c->ev : nullptr, fused ? &nx : nullptr,
                           has_pending ? &pending : nullptr, defer);
         pending = st;
         has_pending = defer;
-        if (more && !c->ride)
+        if (more && !ride)
             launch_ingest_stage(c, c->stream, nx, nullptr);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
@@ -1635,8 +1737,9 @@ RTOD_API rtod_status rtod_slides_device(rtod_ctx *ctx, const float *dev_pts, int
         set_error("NULL argument");
         return RTOD_ERR_INVALID_ARGUMENT;
     }
-    if (n_slides < 1 || slots_cap < (int64_t)n_slides * ctx->W) {
-        set_error("n_slides must be >= 1 and out_slots hold n_slides * W entries");
+    if (n_slides < 1 || slots_cap < (int64_t)n_slides * ctx->list_cap) {
+        set_error("n_slides must be >= 1 and out_slots hold n_slides * max(W, "
+                  "launch_ray_num) entries");
         return RTOD_ERR_INVALID_ARGUMENT;
     }
     if (!ctx->primed || ctx->staged) {
@@ -1672,8 +1775,9 @@ RTOD_API rtod_status rtod_slides(rtod_ctx *ctx, const float *pts, int32_t n_slid
         set_error("NULL argument");
         return RTOD_ERR_INVALID_ARGUMENT;
     }
-    if (n_slides < 1 || slots_cap < (int64_t)n_slides * ctx->W) {
-        set_error("n_slides must be >= 1 and out_slots hold n_slides * W entries");
+    if (n_slides < 1 || slots_cap < (int64_t)n_slides * ctx->list_cap) {
+        set_error("n_slides must be >= 1 and out_slots hold n_slides * max(W, "
+                  "launch_ray_num) entries");
         return RTOD_ERR_INVALID_ARGUMENT;
     }
     if (!ctx->primed || ctx->staged) {
@@ -2489,6 +2593,10 @@ RTOD_API rtod_status rtod_batch_create(const rtod_config *cfgs, int32_t n, int32
         RTOD_TRY(validate(&cfgs[i]));
         if (cfgs[i].index_kind == RTOD_INDEX_LBVH) {
             set_error("batched streams run on the cell-stencil index");
+            return RTOD_ERR_UNSUPPORTED;
+        }
+        if (cfgs[i].ablation != RTOD_ABL_NONE) {
+            set_error("batched streams run the default path (no ablation)");
             return RTOD_ERR_UNSUPPORTED;
         }
         if (cfgs[i].sched_kind != cfgs[0].sched_kind) {

@@ -1064,17 +1064,23 @@ __device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int 
         }
         return lo;
     };
+    // ablation "no grid filter" (d.no_filter): no point is fast-accepted; with
+    // a ray limit only the points in ring slots below it are candidates, each
+    // a unit of its own
     bool cand = false;
     uint64_t key = 0;
     if (i < W) {
         key = kat(i);
         bool dense = false;
-        if (K < W && key_guaranteed<DIM>(key)) {
+        if (!d.no_filter && K < W && key_guaranteed<DIM>(key)) {
             const int lo = run_lo(i, key, K);
             dense = (lo + K < W) && (kat(lo + K) == key);
         }
         cand = !dense;
+        if (d.ray_lim > 0)
+            cand = cand && d.aslot[st.par ^ 1][i] < d.ray_lim;
     }
+    const bool singles = d.ray_lim > 0;
     const int lane = lane_id();
     const uint64_t lt = (1ull << lane) - 1ull;
     if constexpr (MODE == kClsPoints) {
@@ -1095,11 +1101,11 @@ __device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int 
         const int wid = threadIdx.x / kWave;
         const uint64_t cmask = __ballot(cand);
         const bool ustart =
-            cand && (i == 0 || (i & (kUnitMax - 1)) == 0 || kat(i - 1) != key);
+            cand && (singles || i == 0 || (i & (kUnitMax - 1)) == 0 || kat(i - 1) != key);
         int ulen = 0;
         if (ustart) {
             const int end = min(W, (i | (kUnitMax - 1)) + 1); // end of the aligned block
-            ulen = run_hi(i, key, end - 1 - i) - i + 1;
+            ulen = singles ? 1 : run_hi(i, key, end - 1 - i) - i + 1;
         }
         uint64_t mo = __ballot(ustart), mi = 0; // front / back list members
         float f[kMlpIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};

@@ -184,6 +184,9 @@ struct Dev {
     int fin_parts;      // parts of a deferred slide epilogue
     int32_t *ready;     // [3][kReadyFlags * kReadyStride]: classify-done flags
     int wait_spins;     // bound of a detect's wait for its classify (one launch per slide)
+    int no_filter;      // ablation "no grid filter": every point is a candidate
+    int ray_lim;        // no grid filter with launch_ray_num < W: only ring slots
+                        // below it are queried (0: all)
     uint64_t *tstamp;   // [kTsCount] device stamps of the slide run one per call
     Ctl *ctl;
     const float *h_in;
@@ -198,7 +201,7 @@ constexpr int kErrMergeSurvivor = 1; // window merge: a survivor's rank out of r
 constexpr int kErrMergeNew = 2;      // window merge: a new point's rank out of range
 constexpr int kErrStencilRows = 4;   // detect: a unit's stencil wider than 32 rows
 constexpr int kErrClassifyWait = 8;  // detect: the bounded wait for its classify ran out
-constexpr int kErrLbvhStack = 16;    // LBVH query: traversal stack overflow
+constexpr int kErrLbvhWalk = 16;     // LBVH: a walk exceeded its step bound
 
 // the host-out buffer of slide t: consecutive slides rotate through
 // kHostOuts buffers, so up to that many slides can be in flight while the

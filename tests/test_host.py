@@ -44,6 +44,19 @@ def test_version_and_status_strings():
     (dict(dim=1, window=100, slide=10, R=1e30, K=5), 1),
     (dict(dim=1, window=100, slide=10, R=0.5, K=0), 1),
     (dict(dim=1, window=0, slide=1, R=0.5, K=5), 1),
+    # ablations (the reference's OPTIMIZATION / UPDATE_GAS_TYPE / --launch_ray_num)
+    (dict(dim=1, window=100, slide=10, R=0.5, K=5, ablation=3), 1),
+    (dict(dim=1, window=100, slide=10, R=0.5, K=5, launch_ray_num=-1,
+          ablation=R.ABL_NO_FILTER), 1),
+    (dict(dim=1, window=100, slide=10, R=0.5, K=5, launch_ray_num=50), 1),  # OPT0 only
+    (dict(dim=1, window=100, slide=10, R=0.5, K=5, launch_ray_num=50,
+          ablation=R.ABL_FILTER_ONLY), 1),
+    (dict(dim=1, window=100, slide=10, R=0.5, K=5, refit=True), 2),  # stencil, default
+    (dict(dim=1, window=100, slide=10, R=0.5, K=5, refit=True, ablation=R.ABL_NO_FILTER), 2),
+    (dict(dim=1, window=100, slide=10, R=0.5, K=5, refit=True,
+          index_kind=R.INDEX_LBVH), 2),  # the default LBVH holds the candidates
+    (dict(dim=1, window=100, slide=10, R=0.5, K=5, sched_kind=R.SCHED_LEARNED,
+          ablation=R.ABL_NO_FILTER), 2),
 ])
 def test_invalid_configs_rejected_without_device(kw, status):
     with pytest.raises(R.RtodError) as e:
@@ -59,12 +72,40 @@ def test_null_arguments_rejected():
     lib.rtod_destroy(None)  # no-op
 
 
+def _header_layout():
+    """sizeof and field offsets of the header's structs, as a C compiler sees
+    include/rtod.h (gcc on a generated program)."""
+    import subprocess
+    import tempfile
+
+    structs = {"rtod_config": R._Config, "rtod_times": R._Times, "rtod_stats": R._Stats,
+               "rtod_batch_tuning": R._BatchTuning}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "rtod.h"', "int main(void){"]
+    for cs, py in structs.items():
+        lines.append(f'printf("{cs} size %zu\\n", sizeof({cs}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cs} {f} %zu\\n", offsetof({cs}, {f}));')
+    lines.append("return 0;}")
+    with tempfile.TemporaryDirectory() as td:
+        src, exe = os.path.join(td, "l.c"), os.path.join(td, "l")
+        open(src, "w").write("\n".join(lines))
+        subprocess.run(["gcc", "-std=c11", f"-I{os.path.join(ROOT, 'include')}", "-o", exe, src],
+                       check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    got = {}
+    for line in out.splitlines():
+        cs, f, v = line.split()
+        got[(cs, f)] = int(v)
+    return structs, got
+
+
 def test_struct_layouts_match_header():
-    # rtod_config: 10 x 4 bytes; rtod_times: 7 doubles;
-    # rtod_stats: 13 int64 + 1 double + 2 int32
-    assert C.sizeof(R._Config) == 40
-    assert C.sizeof(R._Times) == 56
-    assert C.sizeof(R._Stats) == 14 * 8 + 2 * 4
+    """Every ctypes mirror has the header struct's size and field offsets."""
+    structs, got = _header_layout()
+    for cs, py in structs.items():
+        assert C.sizeof(py) == got[(cs, "size")], cs
+        for f, _ in py._fields_:
+            assert getattr(py, f).offset == got[(cs, f)], (cs, f)
 
 
 @pytest.mark.parametrize("kw,status", [
