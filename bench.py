@@ -34,10 +34,11 @@ Extra objects on the line:
                 every slide of the stream
   sweeps        the reference's R, K and S sweeps around this default
                 (script/run.py:76-218)
-  other_workloads, index_comparison, learned, batched
+  other_workloads, index_comparison, ablations, learned, batched
                 the 1-D W=10k and 3-D configurations, cell stencil vs LBVH,
-                the learned work scheduler against the plain one, and 256
-                batched streams
+                the reference's OPTIMIZATION 0/1/2 and refit-vs-rebuild
+                modes and its ray-count runs, the learned work scheduler
+                against the plain one, and 256 batched streams
 The line carries "valid": false, and the process exits 1, if any pipelined,
 learned or batched outlier list differs from the plain one-per-call list.
 """
@@ -325,6 +326,78 @@ def index_comparison(names, warmup, steps):
     return out
 
 
+def ablations(names, warmup, steps):
+    """The reference's OPTIMIZATION 0/1/2 and UPDATE_GAS_TYPE comparison
+    (its Figure 15 runs, script/run.py:254-270) on the same streams: one
+    rtod_slide call per slide in each mode, every list checked against the
+    default path's; and its ray-count experiment (script/run.py:237-251): no
+    grid filter, an LBVH over the window, launch_ray_num rays."""
+    import rtod_amd as RA
+
+    modes = {
+        "stencil_default": dict(),
+        "stencil_no_filter": dict(ablation=RA.ABL_NO_FILTER),
+        "lbvh_default_opt2": dict(index_kind=RA.INDEX_LBVH),
+        "lbvh_filter_only_opt1": dict(index_kind=RA.INDEX_LBVH, ablation=RA.ABL_FILTER_ONLY),
+        "lbvh_filter_only_opt1_refit": dict(index_kind=RA.INDEX_LBVH,
+                                            ablation=RA.ABL_FILTER_ONLY, refit=True),
+        "lbvh_no_filter_opt0": dict(index_kind=RA.INDEX_LBVH, ablation=RA.ABL_NO_FILTER),
+        "lbvh_no_filter_opt0_refit": dict(index_kind=RA.INDEX_LBVH, ablation=RA.ABL_NO_FILTER,
+                                          refit=True),
+    }
+    out = {}
+    for name in names:
+        wl = workloads()[name]
+        xs = stream(name, warmup + steps)
+        ref, ent = None, {}
+        for m, kw in modes.items():
+            r = _single_kw(xs, wl.W, wl.S, wl.R, wl.K, warmup, steps, kw)
+            if ref is None:
+                ref = r["lists"]
+            ent[m] = {"us_per_slide": round(r["us"], 3), "stages_ms": r["stages_ms"],
+                      "pair_tests_per_slide": round(r["tests"] / steps),
+                      "candidates_per_slide": round(r["cands"] / steps, 1),
+                      "mismatched_slides": same(r["lists"], ref)}
+        out[name] = ent
+        log(f"[bench] ablations {name}: {ent}")
+    wl = workloads()["walk1d-W100k-S5k"]
+    xs = stream(wl.name, warmup + steps)
+    rays = {}
+    for L in (1000, 10000, 100000):
+        kw = dict(index_kind=RA.INDEX_LBVH, ablation=RA.ABL_NO_FILTER, launch_ray_num=L)
+        r = _single_kw(xs, wl.W, wl.S, wl.R, wl.K, warmup, steps, kw)
+        rays[str(L)] = {"us_per_slide": round(r["us"], 3),
+                        "detect_us": round(r["stages_ms"]["detect"] * 1e3, 3),
+                        "rays_per_s_detect": round(L / (r["stages_ms"]["detect"] * 1e-3))}
+    out["ray_count_walk1d-W100k-S5k"] = rays
+    log(f"[bench] ray counts: {rays}")
+    return out
+
+
+def _single_kw(xs, W, S, R, K, warmup, steps, kw):
+    """single_calls with detector options; counters of the timed slides."""
+    import rtod_amd
+
+    dim = xs.shape[1]
+    with rtod_amd.Detector(dim, W, S, R, K, **kw) as det:
+        det.prime(xs[:W])
+        lists, tot, stg = [], [], []
+        for t in range(1, warmup + steps + 1):
+            if t == warmup + 1:
+                st0 = det.stats()
+            o = det.slide(xs[W + (t - 1) * S: W + t * S])
+            if t > warmup:
+                lists.append(o)
+                tot.append(det.times.total)
+                stg.append(list(det.stage_times().values()))
+        st1 = det.stats()
+    return {"us": float(np.mean(tot)) * 1e3, "lists": lists,
+            "stages_ms": {n: round(float(v), 5)
+                          for n, v in zip(rtod_amd.Detector.STAGES, np.array(stg).mean(0))},
+            "tests": st1["pair_tests_total"] - st0["pair_tests_total"],
+            "cands": st1["candidates_total"] - st0["candidates_total"]}
+
+
 def learned_comparison(names, train_slides, steps):
     """Plain vs learned schedule on the same slides: each detector first runs
     `train_slides` slides (untimed warm-up; the learned one trains online
@@ -535,7 +608,7 @@ def main() -> int:
                                                               threads)
         log(f"[bench] cpu: {cpu}")
 
-    others = sweep = indexes = None
+    others = sweep = indexes = abl = None
     if not args.no_others and args.workload == "walk1d-W100k-S5k":
         others = {}
         for name in ("walk1d-W10k-S500", "gmm3d-W10k-S500", "gmm3d-W100k-S5k"):
@@ -547,6 +620,7 @@ def main() -> int:
             log(f"[bench] {name}: {others[name]}")
         sweep = sweeps(5, 40, 30, not args.no_cpu, threads)
         indexes = index_comparison(["walk1d-W100k-S5k", "gmm3d-W10k-S500"], 10, 40)
+        abl = ablations(["walk1d-W100k-S5k", "gmm3d-W10k-S500"], 5, 20)
 
     batch = None
     if not args.no_batched and args.workload == "walk1d-W100k-S5k":
@@ -563,6 +637,8 @@ def main() -> int:
     invalid += sum(v["mismatched_slides"] for v in (others or {}).values())
     invalid += sum(v["mismatched_slides"] for v in (sweep or {}).values())
     invalid += sum(v["mismatched_slides"] for v in (indexes or {}).values())
+    invalid += sum(m["mismatched_slides"] for k, v in (abl or {}).items()
+                   if not k.startswith("ray_count") for m in v.values())
     invalid += sum(v["mismatched_slides"] for v in (learned or {}).values())
     if batch:
         invalid += batch["learned"]["mismatched_steps"]
@@ -613,6 +689,8 @@ def main() -> int:
         line["sweeps"] = sweep
     if indexes:
         line["index_comparison"] = indexes
+    if abl:
+        line["ablations"] = abl
     if learned:
         line["learned"] = learned
     if batch:

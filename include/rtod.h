@@ -143,6 +143,12 @@ typedef struct rtod_stats {
     double train_loss_last;     /* learned scheduler: mean loss, last slide */
     int32_t index_kind;         /* index actually used                     */
     int32_t sched_kind;         /* schedule actually used last slide       */
+    int64_t hits_last;          /* neighbours found, last slide: the sum
+                                   over candidates of min(count, K + 1)
+                                   (DEBUG_INFO "Hit / slide",
+                                   optixScan.cpp:765-815, :1181-1186; exact
+                                   despite the early exits)               */
+    int64_t hits_total;
 } rtod_stats;
 
 typedef struct rtod_ctx rtod_ctx;

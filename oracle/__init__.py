@@ -184,10 +184,10 @@ class Grid:
         return self._buf[:n].copy()
 
     def stats(self) -> dict:
-        a = np.zeros(4, np.int64)
+        a = np.zeros(5, np.int64)
         lib().rtod_o_grid_stats(self._h, a)
         return {"candidates": int(a[0]), "active_cells": int(a[1]), "pair_tests": int(a[2]),
-                "dense_cells": int(a[3])}
+                "dense_cells": int(a[3]), "hits": int(a[4])}
 
     def close(self) -> None:
         if self._h:

@@ -464,6 +464,7 @@ struct rtod_o_grid {
     int32_t *cand;
     int32_t *cand_cnt;
     int64_t ncand, ndense;
+    int64_t hits; /* sum over candidates of min(count, K + 1), last slide */
     int64_t *tests_per_thread;
     int64_t pair_tests;
     pool_t *pool;
@@ -743,20 +744,24 @@ int64_t rtod_o_grid_slide(rtod_o_grid *g, const float *new_pts, int32_t *out_slo
         g->pair_tests += g->tests_per_thread[t];
 
     int64_t n = 0;
-    for (int64_t i = 0; i < g->ncand; i++)
+    g->hits = 0;
+    for (int64_t i = 0; i < g->ncand; i++) {
+        g->hits += g->cand_cnt[i]; /* min(count, K + 1): the count stops at K + 1 */
         if (g->cand_cnt[i] <= g->K)
             out_slots[n++] = g->cand[i];
+    }
     qsort(out_slots, (size_t)n, sizeof(int32_t), cmp_i32);
     g->t++;
     return n;
 }
 
-void rtod_o_grid_stats(const rtod_o_grid *g, int64_t *out4)
+void rtod_o_grid_stats(const rtod_o_grid *g, int64_t *out5)
 {
-    out4[0] = g->ncand;
-    out4[1] = g->nactive;
-    out4[2] = g->pair_tests;
-    out4[3] = g->ndense;
+    out5[0] = g->ncand;
+    out5[1] = g->nactive;
+    out5[2] = g->pair_tests;
+    out5[3] = g->ndense;
+    out5[4] = g->hits;
 }
 
 /* ------------------------------------------------------------------ */

@@ -66,8 +66,10 @@ int rtod_o_grid_prime(rtod_o_grid *g, const float *pts);
 int64_t rtod_o_grid_slide(rtod_o_grid *g, const float *new_pts,
                           int32_t *out_slots);
 /* stats: [0] candidates last slide, [1] active cells, [2] pair tests last
- * slide, [3] dense cells last slide */
-void rtod_o_grid_stats(const rtod_o_grid *g, int64_t *out4);
+ * slide, [3] dense cells last slide, [4] hits last slide: the sum over the
+ * candidates of min(count, K + 1) (the reference's DEBUG_INFO hit counts,
+ * optixScan.cpp:765-815, with the count stopping at K + 1) */
+void rtod_o_grid_stats(const rtod_o_grid *g, int64_t *out5);
 void rtod_o_grid_destroy(rtod_o_grid *g);
 
 /* Cell key of a point as the oracle computes it (packing documented in the

@@ -62,7 +62,8 @@ class _Stats(C.Structure):
                 ("host_pinned_bytes", C.c_int64), ("predictor_misses", C.c_int64),
                 ("predictor_points", C.c_int64), ("fallbacks", C.c_int64),
                 ("train_steps", C.c_int64), ("train_loss_last", C.c_double),
-                ("index_kind", C.c_int32), ("sched_kind", C.c_int32)]
+                ("index_kind", C.c_int32), ("sched_kind", C.c_int32),
+                ("hits_last", C.c_int64), ("hits_total", C.c_int64)]
 
 MLP_NPARAMS = 1313
 

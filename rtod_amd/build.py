@@ -88,7 +88,7 @@ def build_ods() -> str:
         return ""
     tmp = f"{ODS}.{os.getpid()}.tmp"
     _run(["g++", "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}", "-o", tmp, src,
-          f"-L{HERE}", "-lrtod", f"-Wl,-rpath,$ORIGIN", "-lpthread"])
+          f"-L{HERE}", "-lrtod", "-lrtod_synth", f"-Wl,-rpath,$ORIGIN", "-lpthread"])
     os.replace(tmp, ODS)
     return ODS
 
@@ -108,8 +108,8 @@ def build_synth() -> str:
 
 def build_all() -> None:
     build_lib()
-    build_ods()
     build_synth()
+    build_ods()
 
 
 if __name__ == "__main__":

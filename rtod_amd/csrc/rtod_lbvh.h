@@ -447,6 +447,7 @@ __global__ __launch_bounds__(256) void k_lbvh_count(Dev d, Step st, Lbvh L, int 
         cn->n_cand = nq; // every query is a candidate
     const int n = cands ? cn->n_cand : nq;
     long long tests = 0;
+    int hits = 0;
     if (q < n) {
         const float *Qx = cands ? d.axyz[st.par ^ 1] : d.ring;
         const int qp = cands ? d.cand[q] : q;
@@ -466,15 +467,20 @@ __global__ __launch_bounds__(256) void k_lbvh_count(Dev d, Step st, Lbvh L, int 
             cnt += dist2<DIM>(x, qx, qy, qz) <= d.r2 ? 1 : 0;
             return cnt <= K;
         });
+        hits = min(cnt, K + 1);
         if (cands)
             d.cand_cnt[q] = min(cnt, K + 1);
         if (cnt <= K)
             atomicOr(&bitmap_of(d, st.t)[slot >> 5], 1u << (slot & 31));
     }
-    for (int dd = 1; dd < kWave; dd <<= 1)
+    for (int dd = 1; dd < kWave; dd <<= 1) {
         tests += __shfl_xor(tests, dd, kWave);
+        hits += __shfl_xor(hits, dd, kWave);
+    }
     if (lane_id() == 0 && tests)
         atomicAdd(&cn->pair_tests, (unsigned long long)tests);
+    if (lane_id() == 0 && hits)
+        atomicAdd(&cn->hits, (unsigned long long)hits);
 }
 
 // refit of a whole-window tree: internal nodes' arrival counters cleared
@@ -492,14 +498,20 @@ __global__ __launch_bounds__(kDetectNT) void k_lbvh_select(Dev d, Step st, Lbvh 
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int U = *L.n_cand;
+    int h = 0;
     if (i < U) {
         const int c = L.count[i];
+        h = min(c, d.K + 1);
         d.cand_cnt[L.item[1][i]] = min(c, d.K + 1);
         if (c <= d.K) {
             const int slot = d.aslot[st.par ^ 1][d.cand[L.item[1][i]]];
             atomicOr(&bitmap_of(d, st.t)[slot >> 5], 1u << (slot & 31));
         }
     }
+    for (int dd = 1; dd < kWave; dd <<= 1)
+        h += __shfl_xor(h, dd, kWave);
+    if (lane_id() == 0 && h)
+        atomicAdd(&cnt_of(d, st.t)->hits, (unsigned long long)h);
 }
 
 } // namespace rtod

@@ -166,6 +166,7 @@ struct rtod_ctx {
     int64_t kcnt[kKCount] = {};
     // stats
     int64_t cand_last = 0, cand_total = 0, tests_last = 0, tests_total = 0;
+    int64_t hits_last = 0, hits_total = 0;
     int64_t out_last = 0, out_total = 0;
     int64_t misses_total = 0, pred_points = 0, fallbacks = 0, train_steps = 0;
     double train_loss_last = 0.0;
@@ -1153,6 +1154,8 @@ rtod_status collect(rtod_ctx *c, int32_t *outs, int32_t *n_out)
     c->cand_total += ho->n_cand;
     c->tests_last = (int64_t)ho->pair_tests;
     c->tests_total += (int64_t)ho->pair_tests;
+    c->hits_last = (int64_t)ho->hits;
+    c->hits_total += (int64_t)ho->hits;
     c->out_last = n;
     c->out_total += n;
     if (c->learned) {
@@ -1917,6 +1920,8 @@ RTOD_API rtod_status rtod_get_stats(const rtod_ctx *c, rtod_stats *o)
     o->candidates_total = c->cand_total;
     o->pair_tests_last = c->tests_last;
     o->pair_tests_total = c->tests_total;
+    o->hits_last = c->hits_last;
+    o->hits_total = c->hits_total;
     o->outliers_last = c->out_last;
     o->outliers_total = c->out_total;
     o->device_bytes = c->dev_bytes;

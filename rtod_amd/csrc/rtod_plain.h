@@ -683,6 +683,7 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
         cn->n_units_i = 0;
         cn->n_done = 0;
         cn->pair_tests = 0;
+        cn->hits = 0;
         cn->misses = 0;
     }
     if (d.ready && bid == 0 && tid < kReadyFlags)
@@ -1715,7 +1716,7 @@ __device__ __forceinline__ int emit_chunk(HostOut *ho, FinishLds &fl, int c, uin
 // error flags raised later (by the next slide's merge running beside the
 // epilogue) stay set for the next slide.
 struct PubIn {
-    unsigned long long pair_tests;
+    unsigned long long pair_tests, hits;
     int32_t err, n_cand, misses, train_steps;
     float train_loss;
 };
@@ -1725,6 +1726,7 @@ __device__ __forceinline__ PubIn load_pub_inputs(const Dev &d, const Step &st, b
     PubIn in;
     SlideCnt *cn = cnt_of(d, st.t);
     in.pair_tests = __hip_atomic_load(&cn->pair_tests, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    in.hits = __hip_atomic_load(&cn->hits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     in.err = __hip_atomic_load(&d.ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     in.n_cand = cn->n_cand;
     in.misses = in.train_steps = 0;
@@ -1776,6 +1778,7 @@ __device__ __forceinline__ void publish_slide(const Dev &d, const Step &st,
     ho->n_out = total;
     ho->n_cand = in.n_cand;
     ho->pair_tests = in.pair_tests;
+    ho->hits = in.hits;
     ho->err = in.err;
     if (in.err)
         atomicAnd(&d.ctl->err, ~in.err); // the flags reported here
@@ -1974,6 +1977,7 @@ struct DetectLds {
     int miss;
     int ready;
     unsigned long long tests;
+    unsigned long long hits;
 };
 
 // Spins of a detect workgroup waiting for the classify of its own launch
@@ -2000,6 +2004,7 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
     auto &s_rows = lds.rows;
     int &s_last = lds.last;
     unsigned long long &s_tests = lds.tests;
+    unsigned long long &s_hits = lds.hits;
     int &s_miss = lds.miss;
 
     const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
@@ -2055,6 +2060,7 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
 
     if (tid == 0) {
         s_tests = 0;
+        s_hits = 0;
         s_miss = 0;
     }
     // a unit's list entry, then its candidates' coordinates and slots: the
@@ -2086,6 +2092,7 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
 
     long long tests = 0;
     int misses = 0;
+    int hits = 0; // this lane's candidates' capped counts
     for (; u < n_units; u += ustep) {
         const int li = list_index(u);
         const int n = __builtin_amdgcn_readfirstlane(un.y);
@@ -2107,6 +2114,7 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
                                   full, tests);
         }
         const int capped = lane < n ? min(cnt, K + 1) : K + 1;
+        hits += lane < n ? capped : 0;
         if (lane < n && cnt <= K)
             atomicOr(&bm[cslot >> 5], 1u << (cslot & 31));
         const int umin = unit_min_i(capped);
@@ -2126,10 +2134,16 @@ __device__ __forceinline__ void detect_body(const Dev &d, const Step &st, int bi
         atomicAdd(&s_tests, (unsigned long long)tests);
     if (LEARN && lane == 0 && misses)
         atomicAdd(&s_miss, misses);
+    for (int dd = 1; dd < kWave; dd <<= 1)
+        hits += __shfl_xor(hits, dd, kWave);
+    if (lane == 0 && hits)
+        atomicAdd(&s_hits, (unsigned long long)hits);
     __syncthreads();
     if (tid == 0) {
         if (s_tests)
             atomicAdd(&cn->pair_tests, s_tests);
+        if (s_hits)
+            atomicAdd(&cn->hits, s_hits);
         if (LEARN && s_miss)
             atomicAdd(&cn->misses, s_miss);
         if (fin) {

@@ -61,6 +61,7 @@ struct SlideCnt {
     alignas(128) int32_t n_units_i; // learned: the probable-inlier bucket
     alignas(128) int32_t n_done; // one launch per slide: classify workgroups done
     alignas(128) unsigned long long pair_tests; // predicate evaluations of the slide
+    alignas(128) unsigned long long hits; // sum over candidates of min(count, K + 1)
     alignas(128) int32_t misses; // learned: mispredicted units of the slide
 };
 
@@ -87,6 +88,7 @@ struct HostOut {
     float train_loss;      // learned: mean loss of those steps
     int32_t n_units;       // cell units of the slide
     int32_t pad;
+    unsigned long long hits; // sum over the slide's candidates of min(count, K + 1)
     uint64_t ts[8];        // one slide per call: device stamps (kTs*)
     int32_t list[1];       // W entries follow
 };

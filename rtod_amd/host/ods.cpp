@@ -2,11 +2,15 @@
 //
 // Counterpart of the reference executable optixScan: main (optixScan.cpp:
 // 1128-1190), parse_args (:149-212), read_data (:95-147) and the Timer report
-// (timer.h:102-117).  Same flags; the reference's compile-time DIMENSION
-// becomes --dim.  Input: text, one point per line, "%lf" (1-D) or
+// (timer.h:102-117).  Same flags; the reference's compile-time DIMENSION,
+// OPTIMIZATION and UPDATE_GAS_TYPE become --dim, --optimization and
+// --update_gas_type.  Input: text, one point per line, "%lf" (1-D) or
 // comma-separated coordinates (2-D/3-D), like read_data; or --gen for the
-// seeded synthetic streams.  Runs `--warmup` untimed passes over the stream
-// (reference: 10) and one timed pass; prints the per-slide averages.
+// seeded synthetic streams of the package's generator library (the measured
+// streams: --gen walk --seed 2 --n 1048572 is the 1-D W=100k stream).  Runs
+// `--warmup` untimed passes over the stream (reference: 10) and one timed
+// pass, one rtod_slide call per slide or (--pipelined) all slides in one
+// rtod_slides call; prints the per-slide averages.
 #include <cerrno>
 #include <chrono>
 #include <cmath>
@@ -16,12 +20,19 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
-#include <random>
 #include <sstream>
 #include <string>
 #include <vector>
 
 #include "rtod.h"
+
+// the package's synthetic-stream generators (rtod_amd/host/synth.c)
+extern "C" {
+void rtod_synth_walk(uint64_t seed, int64_t n, int dim, double start, double sigma,
+                     double p_spike, double spike_lo, double spike_hi, float *out);
+void rtod_synth_gmm(uint64_t seed, int64_t n, int dim, int ncomp, double lo, double hi,
+                    double sig_lo, double sig_hi, double noise_frac, double drift, float *out);
+}
 
 namespace {
 
@@ -32,6 +43,9 @@ struct Args {
     double R = 0.0;
     long long start_copy_pos = 0;
     int launch_ray_num = 0;
+    int optimization = 2;    // the reference's OPTIMIZATION (optixScan.h:22-24)
+    int update_gas_type = 1; // the reference's UPDATE_GAS_TYPE (optixScan.h:18-20)
+    bool pipelined = false;
     int warmup = 10;
     std::string index = "auto", sched = "plain", gen, out;
     uint64_t seed = 1;
@@ -46,8 +60,12 @@ struct Args {
               << "         --window <int> --slide <int> --R <float> --K <int>\n"
               << "         --dim <1|2|3>             point dimension (default 1)\n"
               << "         --start_copy_pos <int>    skip this many slides of input\n"
-              << "         --launch_ray_num <int>    accepted for compatibility, ignored\n"
+              << "         --launch_ray_num <int>    rays cast (--optimization 0), ray i from slot i mod W\n"
+              << "         --optimization 0|1|2      0 no grid filter, 1 grid filter only, 2 grid\n"
+              << "                                   filter + index over the candidates (default)\n"
+              << "         --update_gas_type 0|1     0 refit the whole-window LBVH, 1 rebuild (default)\n"
               << "         --index auto|stencil|lbvh --sched plain|learned\n"
+              << "         --pipelined               the timed pass as one pipelined call\n"
               << "         --warmup <int>            untimed passes (default 10)\n"
               << "         --gen walk|gmm --seed <u64>  synthetic stream instead of --file\n"
               << "         --out <file>              write each slide's outlier slots\n"
@@ -85,6 +103,12 @@ Args parse_args(int argc, char **argv)
             a.start_copy_pos = std::stoll(need(i));
         else if (s == "--launch_ray_num")
             a.launch_ray_num = std::stoi(need(i));
+        else if (s == "--optimization")
+            a.optimization = std::stoi(need(i));
+        else if (s == "--update_gas_type")
+            a.update_gas_type = std::stoi(need(i));
+        else if (s == "--pipelined")
+            a.pipelined = true;
         else if (s == "--index")
             a.index = need(i);
         else if (s == "--sched")
@@ -141,44 +165,17 @@ bool read_data(const Args &a, std::vector<float> &pts)
     return true;
 }
 
-// seeded synthetic streams (same definitions as the measured workloads:
-// random walk with spikes, Gaussian mixture with drift)
+// the package's seeded synthetic streams with the measured workloads'
+// parameters (rtod_amd/synth.py): a random walk from 10 with N(0, 0.05)
+// steps and 0.2% spikes of +-U(2, 5); a mixture of 8 Gaussians in [0, 100]^d,
+// sigma U(0.5, 3), 0.5% uniform noise, centres drifting by 5
 void generate(const Args &a, std::vector<float> &pts)
 {
-    std::mt19937_64 rng(a.seed);
-    std::normal_distribution<double> N01(0.0, 1.0);
-    std::uniform_real_distribution<double> U01(0.0, 1.0);
-    const long long n = a.n;
-    pts.resize((size_t)n * a.dim);
-    if (a.gen == "walk") {
-        double w[3] = {10, 10, 10};
-        for (long long i = 0; i < n; ++i) {
-            for (int d = 0; d < a.dim; ++d) {
-                if (i > 0)
-                    w[d] += 0.05 * N01(rng);
-                double v = w[d];
-                if (U01(rng) < 0.002)
-                    v += (U01(rng) < 0.5 ? -1 : 1) * (2.0 + 3.0 * U01(rng));
-                pts[(size_t)i * a.dim + d] = (float)v;
-            }
-        }
-    } else {
-        const int C = 8;
-        double cen[C][3], sig[C][3];
-        for (int c = 0; c < C; ++c)
-            for (int d = 0; d < 3; ++d) {
-                cen[c][d] = 100.0 * U01(rng);
-                sig[c][d] = 0.5 + 2.5 * U01(rng);
-            }
-        for (long long i = 0; i < n; ++i) {
-            const bool noise = U01(rng) < 0.005;
-            const int c = (int)(U01(rng) * C) % C;
-            for (int d = 0; d < a.dim; ++d) {
-                const double v = noise ? 100.0 * U01(rng) : cen[c][d] + sig[c][d] * N01(rng);
-                pts[(size_t)i * a.dim + d] = (float)v;
-            }
-        }
-    }
+    pts.resize((size_t)a.n * a.dim);
+    if (a.gen == "walk")
+        rtod_synth_walk(a.seed, a.n, a.dim, 10.0, 0.05, 0.002, 2.0, 5.0, pts.data());
+    else
+        rtod_synth_gmm(a.seed, a.n, a.dim, 8, 0.0, 100.0, 0.5, 3.0, 0.005, 5.0, pts.data());
 }
 
 long vm_rss_kb()
@@ -246,26 +243,58 @@ int main(int argc, char **argv)
     cfg.miss_threshold = 0.25f;
     cfg.flags = a.phase_times ? RTOD_FLAG_PHASE_TIMES : 0;
     cfg.device = 0;
+    cfg.ablation = a.optimization == 0   ? RTOD_ABL_NO_FILTER
+                   : a.optimization == 1 ? RTOD_ABL_FILTER_ONLY
+                                         : RTOD_ABL_NONE;
+    cfg.refit = a.update_gas_type == 0 ? 1 : 0;
+    cfg.launch_ray_num = a.launch_ray_num;
 
     std::cout << "Dimension: " << a.dim << "\nR: " << a.R << ", R2: " << (float)a.R * (float)a.R
               << "\nK: " << a.K << "\nWindow size: " << a.window << "\nSlide size: " << a.slide
               << "\nData num: " << rows << "\nIndex: " << a.index << "\nSchedule: " << a.sched
+              << "\nOptimization: " << a.optimization << "\nUpdate GAS type: " << a.update_gas_type
               << "\n";
     if (a.launch_ray_num)
-        std::cout << "(--launch_ray_num ignored: no ray launch in this build)\n";
+        std::cout << "Launch ray num: " << a.launch_ray_num << "\n";
 
     const long rss0 = vm_rss_kb();
     rtod_ctx *ctx = nullptr;
     CHECK(rtod_create(&cfg, &ctx));
-    std::vector<int32_t> outl((size_t)a.window);
+    const size_t cap = (size_t)std::max(a.window, a.launch_ray_num);
+    std::vector<int32_t> outl(cap);
     FILE *fo = a.out.empty() ? nullptr : std::fopen(a.out.c_str(), "w");
     rtod_times acc{};
     int64_t total_out = 0;
+    double pipe_ms = 0.0;
+    auto write_slide = [&](long long t, int n, const int32_t *l) {
+        if (!fo)
+            return;
+        std::fprintf(fo, "%lld %d", t, n);
+        for (int i = 0; i < n; ++i)
+            std::fprintf(fo, " %d", l[i]);
+        std::fprintf(fo, "\n");
+    };
     for (int pass = 0; pass <= a.warmup; ++pass) {
         const bool timed = pass == a.warmup;
         CHECK(rtod_prime(ctx, base, a.window));
         acc = rtod_times{};
         total_out = 0;
+        if (a.pipelined && T > 0) { // all slides in one call, from host memory
+            std::vector<int32_t> offs((size_t)T + 1), slots((size_t)T * cap);
+            const auto t0 = std::chrono::steady_clock::now();
+            CHECK(rtod_slides(ctx, base + (size_t)a.window * a.dim, (int32_t)T, offs.data(),
+                              slots.data(), (int64_t)slots.size()));
+            const auto t1 = std::chrono::steady_clock::now();
+            if (!timed)
+                continue;
+            pipe_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            for (long long t = 1; t <= T; ++t) {
+                const int n = offs[t] - offs[t - 1];
+                total_out += n;
+                write_slide(t, n, slots.data() + offs[t - 1]);
+            }
+            continue;
+        }
         for (long long t = 1; t <= T; ++t) {
             rtod_times tm{};
             int32_t n = 0;
@@ -279,13 +308,9 @@ int main(int argc, char **argv)
             acc.detect_outlier += tm.detect_outlier;
             acc.copy_d2h += tm.copy_d2h;
             acc.total += tm.total;
+            acc.launch += tm.launch;
             total_out += n;
-            if (fo) {
-                std::fprintf(fo, "%lld %d", t, n);
-                for (int i = 0; i < n; ++i)
-                    std::fprintf(fo, " %d", outl[i]);
-                std::fprintf(fo, "\n");
-            }
+            write_slide(t, n, outl.data());
         }
     }
     if (fo)
@@ -293,22 +318,31 @@ int main(int argc, char **argv)
     const double Td = T > 0 ? (double)T : 1.0;
     rtod_stats st{};
     CHECK(rtod_get_stats(ctx, &st));
-    std::cout << "\n###########   Time  ##########\n"
-              << "[Time] copy new points h2d: " << acc.copy_h2d / Td << " ms\n"
-              << "[Time] copy filtered points h2d: 0 ms\n"
-              << "[Time] copy outlier d2h: " << acc.copy_d2h / Td << " ms\n"
-              << "[Time] prepare cell: " << acc.prepare_cell / Td << " ms\n"
-              << "[Time] build BVH: " << acc.build_index / Td << " ms\n"
-              << "[Time] detect outlier: " << acc.detect_outlier / Td << " ms\n"
-              << "[Time] total time for a slide: " << acc.total / Td << " ms\n"
-              << "##############################\n\n";
+    if (a.pipelined) {
+        std::cout << "\n###########   Time  ##########\n"
+                  << "[Time] pipelined slides from host memory: " << pipe_ms / Td
+                  << " ms per slide\n"
+                  << "##############################\n\n";
+    } else {
+        std::cout << "\n###########   Time  ##########\n"
+                  << "[Time] copy new points h2d: " << acc.copy_h2d / Td << " ms\n"
+                  << "[Time] copy filtered points h2d: 0 ms\n"
+                  << "[Time] copy outlier d2h: " << acc.copy_d2h / Td << " ms\n"
+                  << "[Time] prepare cell: " << acc.prepare_cell / Td << " ms\n"
+                  << "[Time] build BVH: " << acc.build_index / Td << " ms\n"
+                  << "[Time] detect outlier: " << acc.detect_outlier / Td << " ms\n"
+                  << "[Time] launch and completion: " << acc.launch / Td << " ms\n"
+                  << "[Time] total time for a slide: " << acc.total / Td << " ms\n"
+                  << "##############################\n\n";
+    }
     std::cout << "[Mem] Host cpu memory used(MB): " << (vm_rss_kb() - rss0) / 1024.0 << "\n"
               << "[Mem] Device memory used for data(MB): " << st.device_bytes / 1048576.0 << "\n"
               << "[Mem] Pinned host memory(MB): " << st.host_pinned_bytes / 1048576.0 << "\n";
     std::cout << "BVH Node / slide: " << (st.slides ? st.candidates_total / st.slides : 0) << "\n"
               << "Outliers / slide: " << total_out / Td << "\n"
               << "Pair tests / slide: " << (st.slides ? st.pair_tests_total / st.slides : 0)
-              << "\n";
+              << "\n"
+              << "Hit / slide: " << (st.slides ? st.hits_total / st.slides : 0) << "\n";
     if (cfg.sched_kind == RTOD_SCHED_LEARNED)
         std::cout << "Predictor misses: " << st.predictor_misses << " / " << st.predictor_points
                   << ", fallbacks: " << st.fallbacks << "\n";

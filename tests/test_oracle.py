@@ -414,3 +414,36 @@ def test_semantics_diff_on_configs():
             win = np.empty((wl.W, wl.dim), np.float32)
             win[idx % wl.W] = xs[idx]
             assert _status_diff_brute(win, wl.R, wl.K) == per[t - 1], (wl.name, t)
+
+
+# ------------------------------------------------------------------ hits
+def test_grid_hits_closed_form_and_brute():
+    """Hits (the sum over candidates of min(count, K + 1)).  K >= W makes
+    every point a candidate whose count stops nowhere, so hits is the sum of
+    the full counts: on the lattice x_i = i/4 with R = 1/2 that is 5W - 6
+    (interior points 5, the two points at each end 3 and 4); on random
+    windows the sum of brute-force counts.  With small K each candidate adds
+    min(count, K + 1): bounded by (K + 1) * candidates."""
+    W, S = 256, 64
+    xs = (np.arange(W + S, dtype=np.float64) * 0.25).astype(np.float32).reshape(-1, 1)
+    g = O.Grid(1, W, S, 0.5, W)
+    g.prime(xs[:W])
+    g.slide(xs[W:W + S])
+    assert g.stats()["hits"] == 5 * W - 6
+    rng = np.random.default_rng(5)
+    for dim in (1, 2, 3):
+        W, S = 600, 100
+        xs = rng.normal(0, 2, (W + S, dim)).astype(np.float32)
+        g = O.Grid(dim, W, S, 0.9, W)
+        g.prime(xs[:W])
+        g.slide(xs[W:W + S])
+        win = np.empty((W, dim), np.float32)
+        idx = np.arange(S, S + W)
+        win[idx % W] = xs[idx]
+        _, cnt = O.brute(win, 0.9, W, counts=True)
+        assert g.stats()["hits"] == int(cnt.sum()), dim
+        g2 = O.Grid(dim, W, S, 0.9, 5)
+        g2.prime(xs[:W])
+        g2.slide(xs[W:W + S])
+        st = g2.stats()
+        assert 0 < st["hits"] <= 6 * st["candidates"]
