@@ -86,13 +86,14 @@ def same(a, b) -> int:
 # --------------------------------------------------------------------------
 # GPU legs
 # --------------------------------------------------------------------------
-def host_pipelined(xs, W, S, R, K, warmup, steps, sched=0, index=0):
+def host_pipelined(xs, W, S, R, K, warmup, steps, sched=0, index=0, flags=0):
     """rtod_slides from host memory: `warmup` slides in one call, then
     `steps` slides in one timed call."""
     import rtod_amd
 
     dim = xs.shape[1]
-    with rtod_amd.Detector(dim, W, S, R, K, sched_kind=sched, index_kind=index) as det:
+    with rtod_amd.Detector(dim, W, S, R, K, sched_kind=sched, index_kind=index,
+                           flags=flags) as det:
         det.prime(xs[:W])
         if warmup:
             det.slides(xs[W: W + warmup * S], warmup)
@@ -402,36 +403,52 @@ def learned_comparison(names, train_slides, steps):
     """Plain vs learned schedule on the same slides: each detector first runs
     `train_slides` slides (untimed warm-up; the learned one trains online
     meanwhile), then `steps` timed slides, host-fed pipelined and one per
-    call."""
+    call.  The learned schedule as shipped applies its cost rule (plain
+    kernels where prediction cannot shorten a slide); `learned_kernels_*`
+    force the learned kernels on every slide (RTOD_FLAG_LEARN_ALWAYS), which
+    shows what inference costs and what the ordering saves."""
     import rtod_amd
 
+    L, ALW = rtod_amd.SCHED_LEARNED, rtod_amd.FLAG_LEARN_ALWAYS
     out = {}
     for name in names:
         wl = workloads()[name]
         xs = stream(name, train_slides + steps)
         W, S, R, K = wl.W, wl.S, wl.R, wl.K
         pl = single_calls(xs, W, S, R, K, train_slides, steps)
-        le = single_calls(xs, W, S, R, K, train_slides, steps, sched=rtod_amd.SCHED_LEARNED)
+        le = single_calls(xs, W, S, R, K, train_slides, steps, sched=L)
+        lk = single_calls(xs, W, S, R, K, train_slides, steps, sched=L, flags=ALW)
         pp = host_pipelined(xs, W, S, R, K, train_slides, steps)
-        lp = host_pipelined(xs, W, S, R, K, train_slides, steps, sched=rtod_amd.SCHED_LEARNED)
-        mism = same(pl["lists"], le["lists"]) + same(pl["lists"], lp["lists"]) + \
-            same(pl["lists"], pp["lists"])
-        s1, s0 = le["stats"], le["stats_warm"]
-        pts = max(1, s1["predictor_points"] - s0["predictor_points"])
+        lp = host_pipelined(xs, W, S, R, K, train_slides, steps, sched=L)
+        lkp = host_pipelined(xs, W, S, R, K, train_slides, steps, sched=L, flags=ALW)
+        mism = sum(same(pl["lists"], x["lists"]) for x in (le, lk, pp, lp, lkp))
+
+        def delta(r, k):
+            return r["stats"][k] - r["stats_warm"][k]
+
+        pts = max(1, delta(lk, "predictor_points"))
         ent = {"plain_us": round(pl["us"], 3), "learned_us": round(le["us"], 3),
                "speedup_vs_plain": round(pl["us"] / le["us"], 3),
                "plain_pipelined_us": round(pp["us"], 3),
                "learned_pipelined_us": round(lp["us"], 3),
                "pipelined_speedup_vs_plain": round(pp["us"] / lp["us"], 3),
+               "plain_kernel_slides": delta(le, "plain_kernel_slides"),
+               "fallback_slides": delta(le, "fallbacks"),
+               "learned_kernels_us": round(lk["us"], 3),
+               "learned_kernels_speedup_vs_plain": round(pl["us"] / lk["us"], 3),
+               "learned_kernels_pipelined_us": round(lkp["us"], 3),
+               "learned_kernels_pipelined_speedup_vs_plain": round(pp["us"] / lkp["us"], 3),
+               "learned_kernels_miss_rate": round(delta(lk, "predictor_misses") / pts, 4),
+               "learned_kernels_fallback_slides": delta(lk, "fallbacks"),
                "mismatched_slides": mism,
-               "miss_rate": round((s1["predictor_misses"] - s0["predictor_misses"]) / pts, 4),
-               "fallback_slides": s1["fallbacks"] - s0["fallbacks"],
-               "train_slides": train_slides, "train_steps": s1["train_steps"],
+               "train_slides": train_slides, "train_steps": lk["stats"]["train_steps"],
+               "units_per_slide": round(delta(lk, "predictor_points") / steps),
                "pair_tests_per_slide": {
-                   "plain": round((pl["stats"]["pair_tests_total"] -
-                                   pl["stats_warm"]["pair_tests_total"]) / steps),
-                   "learned": round((s1["pair_tests_total"] - s0["pair_tests_total"]) / steps)},
-               "stages_ms_plain": pl["stages_ms"], "stages_ms_learned": le["stages_ms"]}
+                   "plain": round(delta(pl, "pair_tests_total") / steps),
+                   "learned": round(delta(le, "pair_tests_total") / steps),
+                   "learned_kernels": round(delta(lk, "pair_tests_total") / steps)},
+               "stages_ms_plain": pl["stages_ms"], "stages_ms_learned": le["stages_ms"],
+               "stages_ms_learned_kernels": lk["stages_ms"]}
         out[name] = ent
         log(f"[bench] learned {name}: {ent}")
     return out
@@ -490,6 +507,7 @@ def batched(n_streams, steps, tune_slides, cpu_slides, with_cpu, threads, learne
                "mismatched_steps": sum(1 for a, c in zip(dig_p, dig_l) if a != c),
                "miss_rate": round(sum(s["predictor_misses"] for s in st_l) / pts, 4),
                "fallback_stream_slides": int(sum(s["fallbacks"] for s in st_l)),
+               "plain_kernel_stream_slides": int(sum(s["plain_kernel_slides"] for s in st_l)),
                "train_steps": int(max(s["train_steps"] for s in st_l)),
                "tuning": tuning_l}
     us = dt * 1e6

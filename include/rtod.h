@@ -70,6 +70,9 @@ typedef enum rtod_ablation {
 /* flags */
 #define RTOD_FLAG_PHASE_TIMES 0x1 /* record HIP events between the slide's
                                      kernels and fill rtod_times per phase  */
+#define RTOD_FLAG_LEARN_ALWAYS 0x2 /* learned schedule: run the learned
+                                      kernels on every slide, the cost rule
+                                      (rtod_stats.plain_kernel_slides) off */
 
 /* Replaces the compile-time DIMENSION/MK/OPTIMIZATION/UPDATE_GAS_TYPE macros
  * and the runtime flags --window --slide --R --K --launch_ray_num of the
@@ -136,9 +139,10 @@ typedef struct rtod_stats {
                                    sparse cell in 1-D, <= 8 in 2-D/3-D,
                                    counted together)                      */
     int64_t predictor_points;   /* learned scheduler: predicted units      */
-    int64_t fallbacks;          /* slides run on the plain schedule because
-                                   the previous slide's miss rate exceeded
-                                   the threshold (or no prediction yet)   */
+    int64_t fallbacks;          /* learned scheduler: slides not ordered by
+                                   prediction -- a miss rate over the
+                                   threshold two slides before (or no
+                                   prediction yet), or the cost rule below */
     int64_t train_steps;        /* learned scheduler: optimiser steps      */
     double train_loss_last;     /* learned scheduler: mean loss, last slide */
     int32_t index_kind;         /* index actually used                     */
@@ -149,6 +153,16 @@ typedef struct rtod_stats {
                                    optixScan.cpp:765-815, :1181-1186; exact
                                    despite the early exits)               */
     int64_t hits_total;
+    int64_t plain_kernel_slides; /* learned scheduler: slides run on the
+                                   plain kernels by the cost rule, which
+                                   skips inference where prediction cannot
+                                   shorten the slide: 1-D (a unit's near
+                                   stencil is its whole stencil), or no
+                                   more cell units on the last collected
+                                   slide than the detect keeps resident
+                                   wavefronts (each unit has a wavefront
+                                   of its own; the slowest unit sets the
+                                   time).  Included in fallbacks.         */
 } rtod_stats;
 
 typedef struct rtod_ctx rtod_ctx;
@@ -306,7 +320,9 @@ typedef struct rtod_batch rtod_batch;
 
 typedef struct rtod_batch_tuning {
     int32_t detect_points_per_wg; /* window points per detect workgroup     */
-    int32_t reserved;
+    int32_t learned_dims;         /* learned batches: bit d-1 set, the d-D
+                                     streams run the learned kernels (2-D
+                                     and 3-D only; autotune chooses)       */
     float cell_scale[3];          /* cell side multiplier by dim (1-D in
                                      [1/16,1], 2-D in [1/4,1], 3-D 1)       */
     double us_per_step;           /* autotune: measured mean step time      */

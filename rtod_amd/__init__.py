@@ -29,6 +29,7 @@ INDEX_AUTO, INDEX_CELL_STENCIL, INDEX_LBVH = 0, 1, 2
 SCHED_PLAIN, SCHED_LEARNED = 0, 1
 ABL_NONE, ABL_FILTER_ONLY, ABL_NO_FILTER = 0, 1, 2
 FLAG_PHASE_TIMES = 0x1
+FLAG_LEARN_ALWAYS = 0x2
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "unsupported", 3: "call out of order",
           4: "HIP runtime error", 5: "out of memory", 6: "internal error"}
@@ -63,13 +64,14 @@ class _Stats(C.Structure):
                 ("predictor_points", C.c_int64), ("fallbacks", C.c_int64),
                 ("train_steps", C.c_int64), ("train_loss_last", C.c_double),
                 ("index_kind", C.c_int32), ("sched_kind", C.c_int32),
-                ("hits_last", C.c_int64), ("hits_total", C.c_int64)]
+                ("hits_last", C.c_int64), ("hits_total", C.c_int64),
+                ("plain_kernel_slides", C.c_int64)]
 
 MLP_NPARAMS = 1313
 
 
 class _BatchTuning(C.Structure):
-    _fields_ = [("detect_points_per_wg", C.c_int32), ("reserved", C.c_int32),
+    _fields_ = [("detect_points_per_wg", C.c_int32), ("learned_dims", C.c_int32),
                 ("cell_scale", C.c_float * 3), ("us_per_step", C.c_double)]
 
 
@@ -346,7 +348,7 @@ class DeviceStream:
 
 
 def _tuning_dict(t: _BatchTuning) -> dict:
-    return {"detect_points_per_wg": t.detect_points_per_wg,
+    return {"detect_points_per_wg": t.detect_points_per_wg, "learned_dims": t.learned_dims,
             "cell_scale": [float(x) for x in t.cell_scale], "us_per_step": t.us_per_step}
 
 
@@ -435,8 +437,11 @@ class Batch:
         _check(lib().rtod_batch_get_tuning(self._h, C.byref(t)))
         return _tuning_dict(t)
 
-    def set_tuning(self, detect_points_per_wg: int, cell_scale=(1.0, 1.0, 1.0)) -> None:
-        t = _BatchTuning(detect_points_per_wg, 0, (C.c_float * 3)(*cell_scale), 0.0)
+    def set_tuning(self, detect_points_per_wg: int, cell_scale=(1.0, 1.0, 1.0),
+                   learned_dims: int = 6) -> None:
+        """learned_dims (learned batches): bit d-1 set, the d-D streams run
+        the learned kernels (bits 1 and 2 only)."""
+        t = _BatchTuning(detect_points_per_wg, learned_dims, (C.c_float * 3)(*cell_scale), 0.0)
         _check(lib().rtod_batch_set_tuning(self._h, C.byref(t)))
 
     def stream_stats(self, i: int) -> dict:

@@ -51,6 +51,7 @@ __device__ __forceinline__ Step batch_step(const BatchStream &bs, bool det = fal
     st.eold = (int)(t % ne);
     st.enew = (int)((m + t) % ne);
     st.buf = (int)(t & 1);
+    st.learn = 0; // a batch's kernels are chosen per dimensionality at plan time
     st.src = bs.src_base ? bs.src_base + ((size_t)d.W + (size_t)t * d.S) * d.dim
                          : d.h_in + (size_t)st.buf * d.S * d.dim;
     return st;

@@ -130,6 +130,16 @@ struct rtod_ctx {
     // large 3-D window: 127.9 -> 123.2 us per slide; four times: 122.4)
     int detect_blocks = 1536;
     int units_last = -1;     // cell units of the last collected slide (-1: none yet)
+    // learned schedule, the cost rule: a slide runs the learned kernels only
+    // with dim > 1 and more units on the last collected slide than
+    // learn_min_units (the detect's resident wavefronts); the choice of
+    // slide t is made once, before its classify is launched, and kept in
+    // learn_of[t % kLearnRing] for its later launches (the epilogue)
+    static constexpr int kLearnRing = 8;
+    int learn_min_units = 0;
+    int8_t learn_of[kLearnRing] = {};
+    int64_t learn_t[kLearnRing] = {-1, -1, -1, -1, -1, -1, -1, -1};
+    bool learn_new = false;  // the last collected slide ran the learned kernels
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = true; // false: a batch member on the batch's stream
@@ -169,6 +179,7 @@ struct rtod_ctx {
     int64_t hits_last = 0, hits_total = 0;
     int64_t out_last = 0, out_total = 0;
     int64_t misses_total = 0, pred_points = 0, fallbacks = 0, train_steps = 0;
+    int64_t plain_slides = 0; // learned: slides on the plain kernels (cost rule)
     double train_loss_last = 0.0;
     bool used_pred_last = false;
 };
@@ -324,7 +335,7 @@ template <int DIM> void launch_classify_dim(rtod_ctx *c, hipStream_t s, const St
     if (c->lbvh) // the LBVH index is built over the per-point candidate list
         hipLaunchKernelGGL((k_classify<DIM, kClsPoints>), dim3(grid), dim3(kClassifyNT), 0, s,
                            c->d, st);
-    else if (c->learned)
+    else if (st.learn)
         hipLaunchKernelGGL((k_classify<DIM, kClsLearned>), dim3(grid), dim3(kClassifyNT), 0, s,
                            c->d, st);
     else
@@ -352,7 +363,7 @@ void launch_detect_dim(rtod_ctx *c, hipStream_t s, const Step &st, const Step *r
     if (runs) {
         const dim3 g(grid + run_merge_wgs(c->S));
         const size_t lds = detect_runs_lds(c->S);
-        if (c->learned)
+        if (st.learn)
             hipLaunchKernelGGL((k_detect_runs<DIM, true>), g, dim3(kDetectNT), lds, s, c->d, st,
                                *runs, grid);
         else
@@ -361,7 +372,7 @@ void launch_detect_dim(rtod_ctx *c, hipStream_t s, const Step &st, const Step *r
         return;
     }
     const int f = fin ? 1 : 0;
-    if (c->learned)
+    if (st.learn)
         hipLaunchKernelGGL((k_detect<DIM, true>), dim3(grid), dim3(kDetectNT), 0, s, c->d, st, f);
     else
         hipLaunchKernelGGL((k_detect<DIM, false>), dim3(grid), dim3(kDetectNT), 0, s, c->d, st, f);
@@ -400,14 +411,14 @@ void launch_classify_stage_dim(rtod_ctx *c, hipStream_t s, const Step &st)
 void launch_classify_stage(rtod_ctx *c, hipStream_t s, const Step &st)
 {
     if (c->dim == 1)
-        c->learned ? launch_classify_stage_dim<1, kClsLearned>(c, s, st)
-                   : launch_classify_stage_dim<1, kClsUnits>(c, s, st);
+        st.learn ? launch_classify_stage_dim<1, kClsLearned>(c, s, st)
+                 : launch_classify_stage_dim<1, kClsUnits>(c, s, st);
     else if (c->dim == 2)
-        c->learned ? launch_classify_stage_dim<2, kClsLearned>(c, s, st)
-                   : launch_classify_stage_dim<2, kClsUnits>(c, s, st);
+        st.learn ? launch_classify_stage_dim<2, kClsLearned>(c, s, st)
+                 : launch_classify_stage_dim<2, kClsUnits>(c, s, st);
     else
-        c->learned ? launch_classify_stage_dim<3, kClsLearned>(c, s, st)
-                   : launch_classify_stage_dim<3, kClsUnits>(c, s, st);
+        st.learn ? launch_classify_stage_dim<3, kClsLearned>(c, s, st)
+                 : launch_classify_stage_dim<3, kClsUnits>(c, s, st);
 }
 
 // ... and slide st's detect with the epilogue of prev, the window merge of
@@ -437,7 +448,7 @@ template <int DIM>
 void launch_detect_stage_dim(rtod_ctx *c, hipStream_t s, const Step &st, const StageParts &p)
 {
     const bool t1 = c->CH == kDetectNT; // else CH == 2 * kDetectNT (ride_along)
-    if (c->learned)
+    if (st.learn)
         t1 ? launch_detect_stage_t<DIM, true, 1>(c, s, st, p)
            : launch_detect_stage_t<DIM, true, 2>(c, s, st, p);
     else
@@ -587,8 +598,34 @@ Step make_step_at(const rtod_ctx *c, uint64_t t, const float *src)
     st.eold = (int)(t % ne);
     st.enew = (int)(((uint64_t)c->m + t) % ne);
     st.buf = (int)(t % (uint64_t)c->stage_slots);
+    const int q = (int)(t % rtod_ctx::kLearnRing);
+    st.learn = c->learn_t[q] == (int64_t)t ? c->learn_of[q] : 0;
     st.src = src ? src : c->h_in + (size_t)st.buf * c->S * c->dim;
     return st;
+}
+
+// The learned schedule's cost rule for slide t (before its classify is
+// launched; later calls keep the first choice).  Inference only pays where
+// the prediction can shorten the slide: in 1-D a unit's near stencil is its
+// whole stencil, so there is nothing to order; and with no more units than
+// the detect keeps resident wavefronts every unit has a wavefront of its
+// own, so the slide lasts as long as its slowest unit, which ordering does
+// not shorten.  The rule reads the last collected slide's unit count (a
+// slide or a few before t); RTOD_FLAG_LEARN_ALWAYS turns it off.
+void decide_learn(rtod_ctx *c, int64_t t)
+{
+    const int q = (int)(t % rtod_ctx::kLearnRing);
+    if (c->learn_t[q] == t)
+        return;
+    bool on = false;
+    if (c->learned) {
+        if (c->cfg.flags & RTOD_FLAG_LEARN_ALWAYS)
+            on = true;
+        else
+            on = c->dim > 1 && c->units_last > c->learn_min_units;
+    }
+    c->learn_t[q] = t;
+    c->learn_of[q] = on ? 1 : 0;
 }
 
 Step make_step(const rtod_ctx *c) { return make_step_at(c, (uint64_t)c->host_t, c->dev_src); }
@@ -660,6 +697,7 @@ void launch_core_stage(rtod_ctx *c, hipStream_t s, const Step &st, hipEvent_t *e
 
 void launch_slide(rtod_ctx *c, hipStream_t s, hipEvent_t *ev)
 {
+    decide_learn(c, c->host_t);
     const Step st = make_step(c);
     launch_ingest_stage(c, s, st, ev);
     launch_core_stage(c, s, st, ev, nullptr, nullptr, false, true);
@@ -960,6 +998,12 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
     c->learned = cfg->sched_kind == RTOD_SCHED_LEARNED;
     d.lr.on = c->learned ? 1 : 0;
     {
+        // the detect's resident wavefronts: at least 6 per SIMD, 4 SIMDs per CU
+        int cus = 0;
+        RTOD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+        c->learn_min_units = std::max(1, cus) * 4 * 6;
+    }
+    {
         int khz = 0;
         RTOD_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
         c->wall_khz = khz > 0 ? (double)khz : 100000.0;
@@ -1082,6 +1126,8 @@ rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n,
     RTOD_HIP(hipStreamSynchronize(s));
     for (int k = 0; k < kHostOuts; ++k)
         host_out_of(c, k)->slide = 0;
+    for (auto &q : c->learn_t)
+        q = -1;
     c->host_t = 0;
     c->primed = true;
     c->staged = false;
@@ -1159,8 +1205,13 @@ rtod_status collect(rtod_ctx *c, int32_t *outs, int32_t *n_out)
     c->out_last = n;
     c->out_total += n;
     if (c->learned) {
-        c->misses_total += ho->misses;
-        c->pred_points += ho->n_units;
+        c->learn_new = ho->learn_kernels != 0;
+        if (c->learn_new) {
+            c->misses_total += ho->misses;
+            c->pred_points += ho->n_units;
+        } else {
+            c->plain_slides++;
+        }
         c->used_pred_last = ho->used_pred != 0;
         if (!ho->used_pred)
             c->fallbacks++;
@@ -1259,7 +1310,8 @@ void accumulate_kernel_times(rtod_ctx *c, double *phase)
 // classify starts after it finishes.
 rtod_status maybe_train(rtod_ctx *c)
 {
-    if (!c->learned || !c->train_stream)
+    // a slide on the plain kernels added nothing to the replay
+    if (!c->learned || !c->train_stream || !c->learn_new)
         return RTOD_OK;
     if (c->train_pending) {
         const hipError_t q = hipEventQuery(c->train_ev);
@@ -1487,7 +1539,6 @@ rtod_status slides_two_stage(rtod_ctx *c, const SlideSrc &src, int32_t n, int32_
     const int64_t t0 = c->host_t;
     const bool timed = (c->cfg.flags & RTOD_FLAG_PHASE_TIMES) != 0;
     const bool runs = c->d.cpb > 1;
-    const bool fused = !c->learned && c->d.ready; // plain: one launch per slide
     hipStream_t s = c->stream;
     auto step = [&](int32_t i) {
         return make_step_at(c, (uint64_t)(t0 + i), slide_src(c, src, t0, i));
@@ -1528,7 +1579,10 @@ rtod_status slides_two_stage(rtod_ctx *c, const SlideSrc &src, int32_t n, int32_
         const int64_t t = t0 + i;
         if ((st = guard_trainer(c, t, s)) != RTOD_OK)
             break;
+        decide_learn(c, t);
         const Step cur = step(i);
+        // plain kernels: one launch per slide; learned: classify, then detect
+        const bool fused = !cur.learn && c->d.ready;
         const Step prev = i > 0 ? step(i - 1) : Step{};
         const Step nx1 = i + 1 < n ? step(i + 1) : Step{};
         const Step nx2 = i + 2 < n ? step(i + 2) : Step{};
@@ -1614,6 +1668,7 @@ rtod_status slides_impl(rtod_ctx *c, const SlideSrc &src, int32_t n, int32_t *of
         if (i >= kHostOuts && (s = collect_next()) != RTOD_OK)
             break;
         const uint64_t t = (uint64_t)(t0 + i);
+        decide_learn(c, (int64_t)t);
         const Step st = make_step_at(c, t, slide_src(c, src, t0, i));
         const bool more = i + 1 < n;
         if (more) // its ingest rides in (or follows) this slide's launches
@@ -1835,6 +1890,7 @@ RTOD_API rtod_status rtod_prepare_cell(rtod_ctx *c)
     }
     RTOD_HIP(hipSetDevice(c->device));
     RTOD_TRY(guard_trainer(c, c->host_t, c->stream));
+    decide_learn(c, c->host_t);
     const Step st = make_step(c);
     launch_ingest_dim(c, c->stream, st, kModeSlide, nullptr, c->d.cpb, c->d.cpb == 1 ? 1 : 0);
     if (c->d.cpb > 1)
@@ -1931,6 +1987,7 @@ RTOD_API rtod_status rtod_get_stats(const rtod_ctx *c, rtod_stats *o)
     o->predictor_misses = c->misses_total;
     o->predictor_points = c->pred_points;
     o->fallbacks = c->fallbacks;
+    o->plain_kernel_slides = c->plain_slides;
     o->train_steps = c->train_steps;
     o->train_loss_last = c->train_loss_last;
     return RTOD_OK;
@@ -2290,6 +2347,7 @@ struct rtod_batch {
     int64_t t = 0;
     // learned batches: one predictor shared by every stream
     bool learned = false;
+    bool learn_any = false; // the plan runs learned kernels for some streams
     Learn shared{};
     std::vector<void *> shared_allocs;
     hipStream_t train_stream = nullptr;
@@ -2328,7 +2386,7 @@ void batch_free_members(rtod_batch *b)
 // step, unless the previous one is still running.
 rtod_status batch_maybe_train(rtod_batch *b)
 {
-    if (!b->learned)
+    if (!b->learned || !b->learn_any) // no stream adds to the replay
         return RTOD_OK;
     if (b->train_pending) {
         const hipError_t q = hipEventQuery(b->train_ev);
@@ -2477,14 +2535,19 @@ rtod_status batch_build_plan(rtod_batch *b)
                        hipMemcpyHostToDevice));
     RTOD_HIP(hipGraphCreate(&b->graph, 0));
     hipGraphNode_t prev[4] = {nullptr, nullptr, nullptr, nullptr};
+    b->learn_any = false;
     for (const auto &l : b->launches) {
         const BatchItem *it = b->d_items + l.first;
         const BatchStream *st = b->d_streams;
         void *args[2] = {(void *)&it, (void *)&st};
         hipKernelNodeParams kp{};
-        kp.func = l.dim == 1 ? batch_kernel<1>(l.kind, b->learned)
-                  : l.dim == 2 ? batch_kernel<2>(l.kind, b->learned)
-                               : batch_kernel<3>(l.kind, b->learned);
+        // learned batches: the learned kernels for the dimensionalities
+        // the tuning names (never 1-D, whose near stencil is the whole one)
+        const bool lk = b->learned && l.dim > 1 && ((b->tuning.learned_dims >> (l.dim - 1)) & 1);
+        b->learn_any = b->learn_any || lk;
+        kp.func = l.dim == 1 ? batch_kernel<1>(l.kind, false)
+                  : l.dim == 2 ? batch_kernel<2>(l.kind, lk)
+                               : batch_kernel<3>(l.kind, lk);
         kp.gridDim = dim3(l.count);
         kp.blockDim = dim3(l.kind == 0 ? kBatchSortNT : l.kind == 4 ? kDetectNT : kBatchNT);
         kp.sharedMemBytes = 0;
@@ -2502,6 +2565,10 @@ rtod_status batch_build_plan(rtod_batch *b)
 
 rtod_status batch_check_tuning(const rtod_batch_tuning *t)
 {
+    if (t->learned_dims & ~6) {
+        set_error("tuning: learned_dims holds bits 1 (2-D) and 2 (3-D) only");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
     if (t->detect_points_per_wg < 1 || t->detect_points_per_wg > (1 << 20)) {
         set_error("tuning: detect_points_per_wg in [1, 2^20]");
         return RTOD_ERR_INVALID_ARGUMENT;
@@ -2626,6 +2693,7 @@ RTOD_API rtod_status rtod_batch_create(const rtod_config *cfgs, int32_t n, int32
             c.flags = 0;
         }
         b->tuning.detect_points_per_wg = 64;
+        b->tuning.learned_dims = 6;
         b->tuning.cell_scale[0] = b->tuning.cell_scale[1] = b->tuning.cell_scale[2] = 1.f;
         b->tuning.us_per_step = 0.0;
         b->learned = cfgs[0].sched_kind == RTOD_SCHED_LEARNED;
@@ -2866,31 +2934,57 @@ RTOD_API rtod_status rtod_batch_autotune(rtod_batch *b, int32_t slides, rtod_bat
         for (auto &c : b->cfg)
             cap += c.window;
         std::vector<int32_t> slots((size_t)cap);
+        auto measure = [&](const rtod_batch_tuning &t, double *us) -> rtod_status {
+            RTOD_TRY(batch_apply_tuning(b, t));
+            RTOD_TRY(batch_prime_device(b));
+            double sum = 0.0;
+            int cnt = 0;
+            for (int k = 0; k < slides; ++k) {
+                const double t0 = now_ms();
+                RTOD_TRY(batch_step_device(b, offs.data(), slots.data()));
+                if (k > 0) { // the first step warms caches and code
+                    sum += now_ms() - t0;
+                    ++cnt;
+                }
+            }
+            *us = sum / cnt * 1e3;
+            return RTOD_OK;
+        };
         for (float cs : c1)
             for (int q : qs) {
-                    rtod_batch_tuning t{};
-                    t.detect_points_per_wg = q;
-                    t.cell_scale[0] = cs;
-                    t.cell_scale[1] = cs;
-                    t.cell_scale[2] = 1.f;
-                    RTOD_TRY(batch_apply_tuning(b, t));
-                    RTOD_TRY(batch_prime_device(b));
-                    double sum = 0.0;
-                    int cnt = 0;
-                    for (int k = 0; k < slides; ++k) {
-                        const double t0 = now_ms();
-                        RTOD_TRY(batch_step_device(b, offs.data(), slots.data()));
-                        if (k > 0) { // the first step warms caches and code
-                            sum += now_ms() - t0;
-                            ++cnt;
-                        }
-                    }
-                    const double us = sum / cnt * 1e3;
-                    if (us < best_us) {
-                        best_us = us;
-                        best = t;
-                    }
+                rtod_batch_tuning t{};
+                t.detect_points_per_wg = q;
+                t.learned_dims = b->tuning.learned_dims;
+                t.cell_scale[0] = cs;
+                t.cell_scale[1] = cs;
+                t.cell_scale[2] = 1.f;
+                double us = 0.0;
+                RTOD_TRY(measure(t, &us));
+                if (us < best_us) {
+                    best_us = us;
+                    best = t;
+                }
             }
+        // learned batches: then which dimensionalities run the learned
+        // kernels (the others the plain ones), at the best shape
+        if (b->learned) {
+            int present = 0;
+            for (auto &c : b->cfg)
+                present |= c.dim > 1 ? 1 << (c.dim - 1) : 0;
+            const int first = best.learned_dims & present;
+            for (int ld : {6, 4, 2, 0}) {
+                if ((ld & present) != ld || ld == first)
+                    continue;
+                rtod_batch_tuning t = best;
+                t.learned_dims = ld;
+                double us = 0.0;
+                RTOD_TRY(measure(t, &us));
+                if (us < best_us) {
+                    best_us = us;
+                    best = t;
+                }
+            }
+        }
         best.us_per_step = best_us;
         RTOD_TRY(batch_apply_tuning(b, best));
         RTOD_TRY(batch_prime_device(b));

@@ -1767,7 +1767,18 @@ __device__ __forceinline__ void publish_slide(const Dev &d, const Step &st,
         *dn = sm.n_units > 0
                   ? (((float)in.misses <= d.lr.miss_thr * (float)sm.n_units) ? 1 : 0)
                   : lc->dec[(st.t + 1) % kDecSlots];
+    } else if (d.lr.on) {
+        // a learned context's slide on the plain kernels: no prediction, no
+        // replay entry (its slot keeps an earlier slide's exact counts); the
+        // miss-rate decision carries over to slide t + 2
+        LearnCtl *lc = d.lr.lc;
+        lc->dec[(st.t + 2) % kDecSlots] = lc->dec[(st.t + 1) % kDecSlots];
+        ho->misses = 0;
+        ho->used_pred = 0;
+        ho->train_steps = d.lr.ls->steps_total;
+        ho->train_loss = d.lr.ls->loss_last;
     }
+    ho->learn_kernels = sm.learn ? 1 : 0;
     if (sm.stamps) {
 #pragma unroll
         for (int k = 0; k < kTsPublished; ++k)
@@ -1941,7 +1952,7 @@ __device__ __forceinline__ void finish_deferred(const Dev &d, const Step &st, in
                                                 FinishLds &fl, bool stamps = false)
 {
     SlideSummary sm;
-    sm.learn = d.lr.on != 0;
+    sm.learn = st.learn != 0;
     const SlideCnt *cn = cnt_of(d, st.t);
     sm.n_o = cn->n_units;
     sm.n_units = sm.learn ? sm.n_o + cn->n_units_i : sm.n_o;

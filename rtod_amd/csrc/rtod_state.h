@@ -87,7 +87,7 @@ struct HostOut {
     int32_t train_steps;   // learned: optimiser steps since creation
     float train_loss;      // learned: mean loss of those steps
     int32_t n_units;       // cell units of the slide
-    int32_t pad;
+    int32_t learn_kernels; // learned: the slide ran the learned kernels
     unsigned long long hits; // sum over the slide's candidates of min(count, K + 1)
     uint64_t ts[8];        // one slide per call: device stamps (kTs*)
     int32_t list[1];       // W entries follow
@@ -143,6 +143,8 @@ struct Step {
     int eold;   // batch-ring entry of the expiring batch (generation t)
     int enew;   // batch-ring entry of the new batch (generation m+t)
     int buf;    // pinned staging buffer holding the new slide
+    int learn;  // learned schedule: this slide runs the learned classify and
+                // detect (else the plain ones)
     const float *src; // the new slide ([S][d] row-major): pinned staging or HBM
 };
 

@@ -158,8 +158,43 @@ def test_learned_batch_exact_and_trains():
         for d in ds:
             d.close()
     _check(got, want, "learned batch")
-    assert all(st["predictor_points"] > 0 for st in stats if st["candidates_total"] > 0)
+    # the learned kernels run for the 2-D and 3-D streams (default tuning);
+    # 1-D streams, whose near stencil is the whole one, keep the plain ones
+    for (dim, *_), st in zip(descs, stats):
+        if dim == 1:
+            assert st["plain_kernel_slides"] == T and st["predictor_points"] == 0, st
+        elif st["candidates_total"] > 0:
+            assert st["plain_kernel_slides"] == 0 and st["predictor_points"] > 0, st
     assert max(st["train_steps"] for st in stats) > 0  # the shared predictor trained
+
+
+def test_learned_batch_learned_dims():
+    """learned_dims picks the dimensionalities on the learned kernels: 3-D
+    only, then none (every stream on the plain kernels, no training); exact
+    in both, and bit 0 (1-D) is refused."""
+    case("learned batch, learned_dims 4 and 0")
+    T = 12
+    descs, data = _streams(MIXED, T, seed0=710)
+    want = _oracle(descs, data, T)
+    for ld in (4, 0):
+        with R.Batch(descs, sched_kind=R.SCHED_LEARNED) as b:
+            ds = _bind(b, descs, data)
+            b.set_tuning(64, learned_dims=ld)
+            assert b.get_tuning()["learned_dims"] == ld
+            b.prime_device()
+            got = [b.step_device() for _ in range(T)]
+            stats = [b.stream_stats(i) for i in range(len(descs))]
+            with pytest.raises(R.RtodError):
+                b.set_tuning(64, learned_dims=1)
+            b.close()
+            for d in ds:
+                d.close()
+        _check(got, want, f"learned batch learned_dims={ld}")
+        for (dim, *_), st in zip(descs, stats):
+            on = dim > 1 and (ld >> (dim - 1)) & 1
+            assert st["plain_kernel_slides"] == (0 if on else T), (dim, ld, st)
+        if ld == 0:
+            assert max(st["train_steps"] for st in stats) == 0
 
 
 def test_batch_large_streams_plain_and_learned_exact():

@@ -81,12 +81,15 @@ def test_c2_sweep_every_slide(what, val):
                 f"sweep {what}={val}: grid vs sorted1d")
 
 
-def test_learned_c2_every_slide():
+@pytest.mark.parametrize("flags", [0, R.FLAG_LEARN_ALWAYS], ids=["cost-rule", "learned-kernels"])
+def test_learned_c2_every_slide(flags):
+    """C2 with the learned schedule: as shipped (1-D, so the cost rule runs
+    the plain kernels) and with the learned kernels forced on every slide."""
     wl = O.C2
-    case(f"{wl.name} learned, all {wl.T} slides")
+    case(f"{wl.name} learned flags={flags}, all {wl.T} slides")
     xs = wl.data()
     want = both_paths(xs, wl.W, wl.S, wl.T, wl.R, wl.K, f"{wl.name} learned",
-                      sched_kind=R.SCHED_LEARNED)
+                      sched_kind=R.SCHED_LEARNED, flags=flags)
     assert sum(len(w) for w in want) > 0
 
 
@@ -117,8 +120,9 @@ def test_pipelined_counters_equal_sequential(sched):
     frozen = None
     if sched == "learned":
         kw["sched_kind"] = R.SCHED_LEARNED
+        kw["flags"] = R.FLAG_LEARN_ALWAYS  # C3 alone would run the plain kernels
         # a trained predictor, so that the near-first path is taken
-        with R.Detector(3, wl.W, wl.S, wl.R, wl.K, sched_kind=R.SCHED_LEARNED) as det:
+        with R.Detector(3, wl.W, wl.S, wl.R, wl.K, **kw) as det:
             det.prime(xs[:wl.W])
             for t in range(1, 61):
                 det.slide(xs[wl.W + (t - 1) * wl.S: wl.W + t * wl.S])

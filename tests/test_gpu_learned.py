@@ -95,9 +95,14 @@ def test_mlp_grad_matches_oracle(n):
     assert O.mlp_loss64(P2, X, y) < O.mlp_loss64(P.astype(np.float64), X, y)
 
 
+ALW = R.FLAG_LEARN_ALWAYS  # the learned kernels on every slide (cost rule off)
+
+
 def _learned_stream(xs, W, S, T, Rr, K, params=None, frozen=False, **kw):
-    """Slides 1..T with the learned schedule; returns (lists, stats)."""
+    """Slides 1..T with the learned schedule, its kernels on every slide
+    unless flags say otherwise; returns (lists, stats)."""
     xs = np.ascontiguousarray(xs, np.float32)
+    kw.setdefault("flags", ALW)
     with R.Detector(xs.shape[1], W, S, Rr, K, sched_kind=R.SCHED_LEARNED, **kw) as det:
         if params is not None:
             det.set_predictor(params, frozen=frozen)
@@ -163,7 +168,7 @@ def test_predictor_api_roundtrip_and_freeze():
     xs = O.gen_gmm(43, W + T * S, 3)
     rng = np.random.default_rng(2)
     p = rng.normal(0, 0.3, R.MLP_NPARAMS).astype(np.float32)
-    with R.Detector(3, W, S, 2.0, 20, sched_kind=R.SCHED_LEARNED) as det:
+    with R.Detector(3, W, S, 2.0, 20, sched_kind=R.SCHED_LEARNED, flags=ALW) as det:
         det.set_predictor(p, frozen=True)
         assert np.array_equal(det.get_predictor(), p)
         det.prime(xs[:W])
@@ -186,7 +191,7 @@ def test_learned_phase_api_equals_slide():
     W, S, T = 3000, 150, 12
     xs = O.gen_gmm(44, W + T * S, 3)
     want = O.run_stream(xs, W, S, T, 2.0, 20, "grid")
-    with R.Detector(3, W, S, 2.0, 20, sched_kind=R.SCHED_LEARNED) as det:
+    with R.Detector(3, W, S, 2.0, 20, sched_kind=R.SCHED_LEARNED, flags=ALW) as det:
         det.prime(xs[:W])
         got = []
         for t in range(1, T + 1):
@@ -215,7 +220,8 @@ def test_trained_predictor_gain():
         return lists, det.stats()["pair_tests_total"] - t0, det.stats()
 
     # trained online over T_train slides, then measured (still training)
-    with R.Detector(3, W, S, wl.R, wl.K, sched_kind=R.SCHED_LEARNED) as det:
+    with R.Detector(3, W, S, wl.R, wl.K, sched_kind=R.SCHED_LEARNED,
+                    flags=ALW) as det:
         det.prime(xs[:W])
         for t in range(1, T_train + 1):
             det.slide(xs[W + (t - 1) * S: W + t * S])
@@ -233,7 +239,8 @@ def test_trained_predictor_gain():
         got, tests_plain, _ = evaluate(det, T_train + 1)
     assert_same(got, want, "plain")
     # the untrained network, frozen (the method without training)
-    with R.Detector(3, W, S, wl.R, wl.K, sched_kind=R.SCHED_LEARNED) as det:
+    with R.Detector(3, W, S, wl.R, wl.K, sched_kind=R.SCHED_LEARNED,
+                    flags=ALW) as det:
         det.set_predictor(det.get_predictor(), frozen=True)
         det.prime(xs[:W])
         for t in range(1, T_train + 1):
@@ -247,3 +254,59 @@ def test_trained_predictor_gain():
     assert fallback_eval <= T_eval // 10
     assert tests_learned < 0.9 * tests_plain
     assert tests_learned < 0.9 * tests_untrained
+
+
+def _plain_stats(xs, W, S, T, Rr, K):
+    with R.Detector(xs.shape[1], W, S, Rr, K) as det:
+        det.prime(xs[:W])
+        for t in range(1, T + 1):
+            det.slide(xs[W + (t - 1) * S: W + t * S])
+        return det.stats()
+
+
+@pytest.mark.parametrize("wl,T", [(O.C1, 40), (O.C3, 30)], ids=["C1", "C3"])
+def test_cost_rule_runs_plain_kernels(wl, T):
+    """The learned schedule's cost rule: in 1-D (a unit's near stencil is its
+    whole stencil) and with no more cell units than the detect's resident
+    wavefronts (C3: about 2.7k units, each with a wavefront of its own), the
+    prediction cannot shorten a slide, so every slide runs the plain kernels:
+    no inference and no training, the plain schedule's outliers and pair
+    tests, every slide counted as a fallback."""
+    case(f"{wl.name} learned, cost rule")
+    xs = wl.data(wl.W + T * wl.S)
+    got, st = _learned_stream(xs, wl.W, wl.S, T, wl.R, wl.K, flags=0)
+    assert_same(got, O.run_stream(xs, wl.W, wl.S, T, wl.R, wl.K, "grid"), wl.name)
+    assert st["plain_kernel_slides"] == T and st["fallbacks"] == T, st
+    assert st["predictor_points"] == 0 and st["train_steps"] == 0, st
+    assert st["sched_kind"] == R.SCHED_PLAIN
+    assert st["pair_tests_total"] == _plain_stats(xs, wl.W, wl.S, T, wl.R, wl.K)["pair_tests_total"]
+
+
+def test_cost_rule_many_units_runs_learned_kernels():
+    """3-D W=100k, S=5k (about 25k cell units, more than the detect's resident
+    wavefronts): the first slide, before any unit count is known, runs the
+    plain kernels and every later one the learned kernels -- one per call,
+    then pipelined (the stage launches switch from one per slide to two
+    within the run), all exact against the grid oracle."""
+    from rtod_amd import synth
+    wl = synth.WORKLOADS["gmm3d-W100k-S5k"]
+    T1, T2 = 6, 12
+    T = T1 + T2
+    case(f"{wl.name} learned, cost rule, {T1} single + {T2} pipelined slides")
+    xs = np.ascontiguousarray(wl.data(wl.W + T * wl.S), np.float32)
+    want = O.run_stream(xs, wl.W, wl.S, T, wl.R, wl.K, "grid", nthreads=8)
+    with R.Detector(3, wl.W, wl.S, wl.R, wl.K, sched_kind=R.SCHED_LEARNED) as det:
+        det.prime(xs[:wl.W])
+        got = [det.slide(xs[wl.W + (t - 1) * wl.S: wl.W + t * wl.S]) for t in range(1, T1 + 1)]
+        st1 = det.stats()
+        det.prime(xs[:wl.W])  # the unit count of the last slide is kept
+        got2 = det.slides(xs[wl.W: wl.W + T2 * wl.S], T2)
+        st2 = det.stats()
+    assert_same(got, want[:T1], "one per call")
+    assert_same(got2, want[:T2], "pipelined")
+    assert st1["plain_kernel_slides"] == 1, st1
+    # pipelined: the decision of slide i reads the last collected slide's
+    # units, known from before the prime
+    assert st2["plain_kernel_slides"] == 1, st2
+    assert st2["predictor_points"] > 6144 * (T1 - 1 + T2), st2
+    assert st2["train_steps"] > 0

@@ -220,6 +220,8 @@ PIPE_CASES = [
     ("C2", O.C2, 40, {}),
     ("C3", O.C3, 60, {}),
     ("C1 learned", O.C1, 60, dict(sched_kind=R.SCHED_LEARNED)),
+    ("C3 learned kernels", O.C3, 40,
+     dict(sched_kind=R.SCHED_LEARNED, flags=R.FLAG_LEARN_ALWAYS)),
     ("C3 lbvh", O.C3, 12, dict(index_kind=R.INDEX_LBVH)),
 ]
 
