@@ -12,9 +12,10 @@
 //   k_lbvh_build   Karras radix-tree hierarchy (one thread per internal node)
 //   k_lbvh_boxes   leaf boxes rounded outward, unions bottom-up (second
 //                  arrival at a node continues)
-//   k_lbvh_query   every window point walks the tree (stackless, by parent
-//                  links); at a leaf: early-out if the count already exceeds
-//                  K, exact FP32 test, atomicAdd -- counts stay exact up to K+1
+//   k_lbvh_query   groups of 64 sorted window points walk the tree together
+//                  (one wavefront, 64 nodes per step); per leaf batch:
+//                  early-out if the count already exceeds K, exact FP32
+//                  test, atomicAdd -- counts stay exact up to K+1
 //   k_lbvh_select  count <= K -> outlier bit (the slide epilogue, k_finish,
 //                  lists them)
 // The number of candidates U lives on the device; kernels are sized for W
@@ -284,7 +285,12 @@ __global__ __launch_bounds__(256) void k_lbvh_build(Dev d, Lbvh L)
 }
 
 // Leaf boxes [x - R', x + R'] rounded outward, then unions up the tree: the
-// second child to reach a node computes its box and continues.
+// second child to reach a node computes its box and continues.  Boxes are
+// written with agent-scope atomic stores and read with agent-scope atomic
+// loads, which the XCDs' L2 caches do not hold back; a child counts its
+// arrival at the parent only once its box stores are acknowledged, so the
+// second arrival reads both children's boxes.  No device-scope fence per
+// level (each one writes back the L2: microseconds, times the tree depth).
 template <int DIM>
 __global__ __launch_bounds__(256) void k_lbvh_boxes(Dev d, Step st, Lbvh L)
 {
@@ -292,46 +298,47 @@ __global__ __launch_bounds__(256) void k_lbvh_boxes(Dev d, Step st, Lbvh L)
     const int U = *L.n_cand;
     if (i >= U)
         return;
+    auto put = [](float *b, const float (&lo)[3], const float (&hi)[3]) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            __hip_atomic_store(&b[a], lo[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&b[3 + a], hi[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    };
+    auto get = [](const float *b, int a) {
+        return __hip_atomic_load(&b[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
     // the leaf's point: a candidate in the sorted window, or (whole-window
     // tree) the point in ring slot item
     const float *Nx = L.whole ? d.ring : d.axyz[st.par ^ 1];
     const int p = L.whole ? L.item[1][i] : d.cand[L.item[1][i]];
     const int leaf = U - 1 + i;
-    float *bx = L.box + (size_t)leaf * 6;
+    float lo[3], hi[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        float lo = 0.f, hi = 0.f;
+        lo[a] = hi[a] = 0.f;
         if (a < DIM) {
             const double x = (double)Nx[(size_t)a * d.W + p];
-            lo = __double2float_rd(x - d.Rp);
-            hi = __double2float_ru(x + d.Rp);
+            lo[a] = __double2float_rd(x - d.Rp);
+            hi[a] = __double2float_ru(x + d.Rp);
         }
-        bx[a] = lo;
-        bx[3 + a] = hi;
     }
     L.count[i] = 0;
-    __threadfence();
+    put(L.box + (size_t)leaf * 6, lo, hi);
     int node = L.parent[leaf];
     while (node >= 0) {
         if (atomicAdd(&L.flag[node], 1) == 0)
             return; // the sibling subtree finishes this node
-        __threadfence();
         const int a0 = L.left[node], b0 = L.right[node];
         const float *ba = L.box + (size_t)(a0 >= 0 ? a0 : U - 1 + ~a0) * 6;
         const float *bb = L.box + (size_t)(b0 >= 0 ? b0 : U - 1 + ~b0) * 6;
-        float *bn = L.box + (size_t)node * 6;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const float la = __hip_atomic_load(&ba[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const float lb = __hip_atomic_load(&bb[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const float ha =
-                __hip_atomic_load(&ba[3 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const float hb =
-                __hip_atomic_load(&bb[3 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            bn[a] = fminf(la, lb);
-            bn[3 + a] = fmaxf(ha, hb);
+            lo[a] = fminf(get(ba, a), get(bb, a));
+            hi[a] = fmaxf(get(ba, 3 + a), get(bb, 3 + a));
         }
-        __threadfence();
+        put(L.box + (size_t)node * 6, lo, hi);
         node = L.parent[node];
     }
 }
@@ -341,145 +348,260 @@ __global__ __launch_bounds__(256) void k_lbvh_boxes(Dev d, Step st, Lbvh L)
 // internal node, ~j for leaf j.
 __device__ __forceinline__ int lbvh_node(int c, int U) { return c >= 0 ? c : U - 1 + ~c; }
 
-// Stackless depth-first walk over the nodes whose box holds x, by parent
-// links: arriving from the parent, test the box (descend left if it holds
-// x, else go back up); arriving from the left child, descend right; from
-// the right child, go up.  The root's parent is -1.  No stack, so no LDS.
-// at_leaf(j) handles leaf j whose box holds x and returns false to stop the
-// walk (early exit).  A walk enters each of the 2U - 1 nodes at most three
-// times; past that bound (a malformed tree) it raises kErrLbvhWalk and stops.
+// ---------------------------------------------------------------------------
+// Group walk: one wavefront walks the tree for a group of up to 64 query
+// points (consecutive in the sorted window, so the group is compact) against
+// the group's bounding box, 64 nodes at a time: each lane tests one node's
+// box for overlap, and the overlapping internal nodes push both children on
+// the wave's node stack in LDS, the overlapping leaves go to a leaf list;
+// every 64 leaves are handed to batch(n) (leaf indices in w.leaf[0..n)),
+// which tests them against every query of the group -- 64 lanes wide -- and
+// returns false to end the walk.  One thread per query walking alone gives a
+// few hundred wavefronts at W = 10k, each a chain of dependent loads; this
+// gives every level of the walk one round trip for 64 nodes.
+// Stack bound: a step pops the top min(64, top) entries and pushes at most
+// 128, so a push leaves at most 64 entries behind once the next step pops,
+// and the pushes still present have strictly increasing least depth; with
+// D levels the stack never holds more than 64 D + 128 entries.  A Karras
+// tree over 63-bit codes with index tie-breaks has at most 98 levels.
+// ---------------------------------------------------------------------------
+constexpr int kLbvhNT = 128;                   // two wavefronts per workgroup
+constexpr int kLbvhStack = 64 * 98 + 128;      // nodes per wavefront
+struct LbvhWaveLds {
+    int32_t stack[kLbvhStack];
+    int32_t leaf[2 * kWave];
+};
+
 template <int DIM, typename F>
-__device__ __forceinline__ void lbvh_walk(const Lbvh &L, int U, const float (&x)[3], int32_t *err,
-                                          F at_leaf)
+__device__ __forceinline__ void lbvh_group_walk(const Lbvh &L, int U, const float (&glo)[3],
+                                                const float (&ghi)[3], LbvhWaveLds &w,
+                                                int32_t *err, F batch)
 {
-    int cur = 0, prev = -1;
-    const long long bound = 3ll * (2ll * U);
-    for (long long steps = 0; cur >= 0; ++steps) {
-        if (steps > bound) {
-            atomicOr(err, kErrLbvhWalk);
-            return;
-        }
-        const int par = L.parent[cur];
-        int next;
-        if (prev == par) { // coming down
-            const float *b = L.box + (size_t)cur * 6;
-            bool in = true;
+    const int lane = lane_id();
+    const uint64_t lt = (1ull << lane) - 1ull;
+    int top = 1, nleaf = 0; // wave-uniform
+    if (lane == 0)
+        w.stack[0] = 0; // the root (leaf 0 itself when U == 1)
+    wave_sync_lds();
+    while (top > 0) {
+        const int n = min(top, kWave);
+        top -= n;
+        int node = 0, l = 0, r = 0;
+        bool hit = false;
+        if (lane < n) {
+            node = w.stack[top + lane];
+            const float *b = L.box + (size_t)node * 6;
+            float bl[3], bh[3];
+#pragma unroll
+            for (int a = 0; a < DIM; ++a) {
+                bl[a] = b[a];
+                bh[a] = b[3 + a];
+            }
+            if (node < U - 1) { // requested with the box: one round trip
+                l = L.left[node];
+                r = L.right[node];
+            }
+            hit = true;
 #pragma unroll
             for (int a = 0; a < DIM; ++a)
-                in = in && (x[a] >= b[a]) && (x[a] <= b[3 + a]);
-            if (!in) {
-                next = par;
-            } else if (cur >= U - 1) {
-                if (!at_leaf(cur - (U - 1)))
-                    return;
-                next = par;
-            } else {
-                next = lbvh_node(L.left[cur], U);
-            }
-        } else if (prev == lbvh_node(L.left[cur], U)) {
-            next = lbvh_node(L.right[cur], U);
-        } else {
-            next = par;
+                hit = hit && bl[a] <= ghi[a] && bh[a] >= glo[a];
         }
-        prev = cur;
-        cur = next;
+        const bool leaf = node >= U - 1;
+        const uint64_t im = __ballot(hit && !leaf), lm = __ballot(hit && leaf);
+        const int ni = __popcll(im), nl = __popcll(lm);
+        if (top + 2 * ni > kLbvhStack) { // cannot happen for a Karras tree
+            if (lane == 0)
+                atomicOr(err, kErrLbvhWalk);
+            return;
+        }
+        wave_sync_lds(); // the pops are read before the pushes overwrite them
+        if (hit && !leaf) {
+            const int k = top + 2 * __popcll(im & lt);
+            w.stack[k] = lbvh_node(l, U);
+            w.stack[k + 1] = lbvh_node(r, U);
+        }
+        if (hit && leaf)
+            w.leaf[nleaf + __popcll(lm & lt)] = node - (U - 1);
+        top += 2 * ni;
+        nleaf += nl;
+        wave_sync_lds();
+        if (nleaf >= kWave) {
+            if (!batch(kWave))
+                return;
+            const int rest = nleaf - kWave;
+            const int v = lane < rest ? w.leaf[kWave + lane] : 0;
+            wave_sync_lds();
+            if (lane < rest)
+                w.leaf[lane] = v;
+            nleaf = rest;
+            wave_sync_lds();
+        }
+    }
+    if (nleaf > 0)
+        (void)batch(nleaf);
+}
+
+// the group's query points: lane q holds query q's coordinates; the box
+// over the active lanes (inactive: +inf / -inf, so an all-inactive group
+// overlaps nothing)
+template <int DIM>
+__device__ __forceinline__ void group_box(const float (&x)[3], bool active, float (&glo)[3],
+                                          float (&ghi)[3])
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float lo = (a < DIM && active) ? x[a] : __builtin_inff();
+        float hi = (a < DIM && active) ? x[a] : -__builtin_inff();
+        for (int m = 1; m < kWave; m <<= 1) {
+            lo = fminf(lo, __shfl_xor(lo, m, kWave));
+            hi = fmaxf(hi, __shfl_xor(hi, m, kWave));
+        }
+        glo[a] = a < DIM ? lo : 0.f;
+        ghi[a] = a < DIM ? hi : 0.f;
     }
 }
 
-// Default index (boxes of the slide's candidates): every window point
-// (sorted order, so neighbouring threads walk similar paths) walks the tree;
-// at a leaf whose box holds it: early-out when the candidate's count
-// already exceeds K (racy read; counts only grow), exact FP32 test,
-// atomicAdd.
+// Default index (boxes of the slide's candidates): every window point is a
+// query; a group is G <= 64 consecutive sorted-window points (one wavefront
+// each: smaller groups have smaller boxes, so fewer leaves each, and give
+// more wavefronts to a small window).  Per batch of
+// leaves (one per lane): the candidate's count is read (racy; counts only
+// grow, so one already past K stays past it and is skipped), each of the
+// group's queries is broadcast and tested against every lane's candidate
+// (exact FP32 predicate; it implies the query lies in the candidate's box),
+// and each lane adds its candidate's hits with one atomic -- counts stay
+// exact up to K+1.
 template <int DIM>
-__global__ __launch_bounds__(256) void k_lbvh_query(Dev d, Step st, Lbvh L)
+__global__ __launch_bounds__(kLbvhNT) void k_lbvh_query(Dev d, Step st, Lbvh L, int G)
 {
+    __shared__ LbvhWaveLds lds[kLbvhNT / kWave];
     stamp_start(d, kTsDetect);
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = lane_id(), wid = threadIdx.x / kWave;
+    const int q0 = (blockIdx.x * (kLbvhNT / kWave) + wid) * G;
     const int U = *L.n_cand;
-    if (U == 0)
+    if (U == 0 || q0 >= d.W)
         return;
     const float *Nx = d.axyz[st.par ^ 1];
-    const bool active = q < d.W;
+    const int nq = min(G, d.W - q0);
+    const bool active = lane < nq;
     float x[3] = {0.f, 0.f, 0.f};
     if (active)
 #pragma unroll
         for (int a = 0; a < DIM; ++a)
-            x[a] = Nx[(size_t)a * d.W + q];
+            x[a] = Nx[(size_t)a * d.W + q0 + lane];
+    float glo[3], ghi[3];
+    group_box<DIM>(x, active, glo, ghi);
     const int K = d.K;
     long long tests = 0;
-    if (active)
-        lbvh_walk<DIM>(L, U, x, &d.ctl->err, [&](int leaf) {
-            if (__hip_atomic_load(&L.count[leaf], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <=
-                K) {
-                const int p = d.cand[L.item[1][leaf]];
-                const float qx = Nx[p];
-                const float qy = DIM > 1 ? Nx[(size_t)d.W + p] : 0.f;
-                const float qz = DIM > 2 ? Nx[(size_t)2 * d.W + p] : 0.f;
-                ++tests;
-                if (dist2<DIM>(x, qx, qy, qz) <= d.r2)
-                    atomicAdd(&L.count[leaf], 1);
-            }
-            return true;
-        });
-    for (int dd = 1; dd < kWave; dd <<= 1)
-        tests += __shfl_xor(tests, dd, kWave);
-    if (lane_id() == 0 && tests)
+    LbvhWaveLds &w = lds[wid];
+    lbvh_group_walk<DIM>(L, U, glo, ghi, w, &d.ctl->err, [&](int nb) {
+        const bool mine = lane < nb;
+        const int j = mine ? w.leaf[lane] : 0;
+        float p[3] = {0.f, 0.f, 0.f};
+        bool open = false;
+        if (mine) {
+            open = __hip_atomic_load(&L.count[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= K;
+            const int cp = d.cand[L.item[1][j]];
+#pragma unroll
+            for (int a = 0; a < DIM; ++a)
+                p[a] = Nx[(size_t)a * d.W + cp];
+        }
+        int acc = 0;
+        for (int qi = 0; qi < nq; ++qi) {
+            const float qx = lane_f(x[0], qi);
+            const float qy = DIM > 1 ? lane_f(x[1], qi) : 0.f;
+            const float qz = DIM > 2 ? lane_f(x[2], qi) : 0.f;
+            acc += (open && dist2<DIM>(p, qx, qy, qz) <= d.r2) ? 1 : 0;
+        }
+        tests += (long long)__popcll(__ballot(open)) * nq;
+        if (acc)
+            atomicAdd(&L.count[j], acc);
+        return true;
+    });
+    if (lane == 0 && tests)
         atomicAdd(&cnt_of(d, st.t)->pair_tests, (unsigned long long)tests);
 }
 
 // Whole-window tree (the reference's OPTIMIZATION 0/1: a GAS over every
 // window point, rays counting hits and terminating once past K,
-// optixScan.cu:80-156): query q walks the tree, counts the leaves whose
-// point passes the FP32 test (its own leaf included) and stops at K+1; a
-// count <= K sets the query point's outlier bit.  cands != 0: the queries
-// are the slide's candidates (sorted-window positions, OPTIMIZATION 1);
-// else the ring slots [0, nq) (OPTIMIZATION 0, nq = min(W, launch_ray_num
-// or W): rays past W repeat slots and are expanded on the host).
+// optixScan.cu:80-156): each query counts the leaves whose point passes the
+// FP32 test (its own leaf included); the group's walk ends once every query
+// of the group has passed K, and a count <= K sets the query point's outlier
+// bit.  The queries, grouped by G <= 64 consecutive sorted-window positions
+// (G divides 64) so that a group is compact: cands != 0, the slide's candidates (their bits in
+// d.cmask, OPTIMIZATION 1); else the points whose ring slot is below nq
+// (OPTIMIZATION 0, nq = min(W, launch_ray_num or W): rays past W repeat
+// slots and are expanded on the host).
 template <int DIM>
-__global__ __launch_bounds__(256) void k_lbvh_count(Dev d, Step st, Lbvh L, int cands, int nq)
+__global__ __launch_bounds__(kLbvhNT) void k_lbvh_count(Dev d, Step st, Lbvh L, int cands, int nq,
+                                                        int G)
 {
+    __shared__ LbvhWaveLds lds[kLbvhNT / kWave];
     stamp_start(d, kTsDetect);
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = lane_id(), wid = threadIdx.x / kWave;
     const int U = *L.n_whole;
     SlideCnt *cn = cnt_of(d, st.t);
-    if (!cands && q == 0)
+    if (!cands && blockIdx.x == 0 && threadIdx.x == 0)
         cn->n_cand = nq; // every query is a candidate
-    const int n = cands ? cn->n_cand : nq;
-    long long tests = 0;
-    int hits = 0;
-    if (q < n) {
-        const float *Qx = cands ? d.axyz[st.par ^ 1] : d.ring;
-        const int qp = cands ? d.cand[q] : q;
-        const int slot = cands ? d.aslot[st.par ^ 1][qp] : q;
-        float x[3] = {0.f, 0.f, 0.f};
+    const int p0 = (blockIdx.x * (kLbvhNT / kWave) + wid) * G;
+    if (p0 >= d.W)
+        return;
+    const float *Nx = d.axyz[st.par ^ 1];
+    const int32_t *Ns = d.aslot[st.par ^ 1];
+    const int qp = p0 + lane;
+    // G divides 64: the group's positions lie in one mask word
+    const uint64_t cm = cands ? d.cmask[p0 >> 6] >> (p0 & 63) : ~0ull;
+    int slot = 0;
+    bool active = false;
+    if (lane < G && qp < d.W && ((cm >> lane) & 1ull)) {
+        slot = Ns[qp];
+        active = cands || slot < nq;
+    }
+    float x[3] = {0.f, 0.f, 0.f};
+    if (active)
 #pragma unroll
         for (int a = 0; a < DIM; ++a)
-            x[a] = Qx[(size_t)a * d.W + qp];
-        const int K = d.K;
-        int cnt = 0;
-        lbvh_walk<DIM>(L, U, x, &d.ctl->err, [&](int leaf) {
-            const int p = L.item[1][leaf];
-            const float qx = d.ring[p];
-            const float qy = DIM > 1 ? d.ring[(size_t)d.W + p] : 0.f;
-            const float qz = DIM > 2 ? d.ring[(size_t)2 * d.W + p] : 0.f;
-            ++tests;
-            cnt += dist2<DIM>(x, qx, qy, qz) <= d.r2 ? 1 : 0;
-            return cnt <= K;
+            x[a] = Nx[(size_t)a * d.W + qp];
+    float glo[3], ghi[3];
+    group_box<DIM>(x, active, glo, ghi);
+    const uint64_t amask = __ballot(active);
+    const int K = d.K;
+    int cnt = 0;
+    long long tests = 0;
+    LbvhWaveLds &w = lds[wid];
+    if (amask)
+        lbvh_group_walk<DIM>(L, U, glo, ghi, w, &d.ctl->err, [&](int nb) {
+            const bool mine = lane < nb;
+            float p[3] = {0.f, 0.f, 0.f};
+            if (mine) {
+                const int r = L.item[1][w.leaf[lane]];
+#pragma unroll
+                for (int a = 0; a < DIM; ++a)
+                    p[a] = d.ring[(size_t)a * d.W + r];
+            }
+            uint64_t open = amask & __ballot(cnt <= K);
+            tests += (long long)__popcll(open) * nb;
+            while (open) {
+                const int qi = __builtin_ctzll(open);
+                open &= open - 1ull;
+                const float qx = lane_f(x[0], qi);
+                const float qy = DIM > 1 ? lane_f(x[1], qi) : 0.f;
+                const float qz = DIM > 2 ? lane_f(x[2], qi) : 0.f;
+                const int c = __popcll(__ballot(mine && dist2<DIM>(p, qx, qy, qz) <= d.r2));
+                cnt += lane == qi ? c : 0;
+            }
+            return (amask & __ballot(cnt <= K)) != 0;
         });
-        hits = min(cnt, K + 1);
-        if (cands)
-            d.cand_cnt[q] = min(cnt, K + 1);
+    int hits = active ? min(cnt, K + 1) : 0;
+    if (active) {
         if (cnt <= K)
             atomicOr(&bitmap_of(d, st.t)[slot >> 5], 1u << (slot & 31));
     }
-    for (int dd = 1; dd < kWave; dd <<= 1) {
-        tests += __shfl_xor(tests, dd, kWave);
+    for (int dd = 1; dd < kWave; dd <<= 1)
         hits += __shfl_xor(hits, dd, kWave);
-    }
-    if (lane_id() == 0 && tests)
+    if (lane == 0 && tests)
         atomicAdd(&cn->pair_tests, (unsigned long long)tests);
-    if (lane_id() == 0 && hits)
+    if (lane == 0 && hits)
         atomicAdd(&cn->hits, (unsigned long long)hits);
 }
 

@@ -161,6 +161,7 @@ struct rtod_ctx {
     int list_cap = 0;        // entries a slide's list may hold: max(W, ray_num)
     Lbvh L{};
     int lbvh_passes = 0;
+    int lbvh_group = 64;     // LBVH walks: queries per wavefront (divides 64)
     std::vector<void *> dev_allocs;
     int64_t dev_bytes = 0, pinned_bytes = 0;
     bool primed = false;
@@ -557,13 +558,15 @@ void launch_lbvh_count(rtod_ctx *c, hipStream_t s, const Step &st, bool stamps)
     L.n_cand = L.n_whole;
     const int cands = c->cfg.ablation == RTOD_ABL_FILTER_ONLY ? 1 : 0;
     const int nq = cands ? c->W : std::min(c->W, c->ray_num > 0 ? c->ray_num : c->W);
-    const dim3 g((nq + 255) / 256), b(256);
+    // one wavefront per lbvh_group queries (candidates, or sorted-window points)
+    const int G = c->lbvh_group, per = (kLbvhNT / kWave) * G;
+    const dim3 g((c->W + per - 1) / per), b(kLbvhNT);
     if (c->dim == 1)
-        hipLaunchKernelGGL(k_lbvh_count<1>, g, b, 0, s, c->d, st, L, cands, nq);
+        hipLaunchKernelGGL(k_lbvh_count<1>, g, b, 0, s, c->d, st, L, cands, nq, G);
     else if (c->dim == 2)
-        hipLaunchKernelGGL(k_lbvh_count<2>, g, b, 0, s, c->d, st, L, cands, nq);
+        hipLaunchKernelGGL(k_lbvh_count<2>, g, b, 0, s, c->d, st, L, cands, nq, G);
     else
-        hipLaunchKernelGGL(k_lbvh_count<3>, g, b, 0, s, c->d, st, L, cands, nq);
+        hipLaunchKernelGGL(k_lbvh_count<3>, g, b, 0, s, c->d, st, L, cands, nq, G);
     launch_finish(c, s, st, stamps);
 }
 
@@ -575,13 +578,16 @@ void launch_lbvh_detect(rtod_ctx *c, hipStream_t s, const Step &st, bool stamps 
     }
     Lbvh L = c->L;
     L.n_cand = &c->d.ctl->cnt[st.t % kSlideSets].n_cand;
-    const dim3 gw((c->W + 255) / 256), b(256);
+    // one wavefront per lbvh_group consecutive sorted-window points
+    const int G = c->lbvh_group, per = (kLbvhNT / kWave) * G;
+    const dim3 gq((c->W + per - 1) / per), bq(kLbvhNT);
     if (c->dim == 1)
-        hipLaunchKernelGGL(k_lbvh_query<1>, gw, b, 0, s, c->d, st, L);
+        hipLaunchKernelGGL(k_lbvh_query<1>, gq, bq, 0, s, c->d, st, L, G);
     else if (c->dim == 2)
-        hipLaunchKernelGGL(k_lbvh_query<2>, gw, b, 0, s, c->d, st, L);
+        hipLaunchKernelGGL(k_lbvh_query<2>, gq, bq, 0, s, c->d, st, L, G);
     else
-        hipLaunchKernelGGL(k_lbvh_query<3>, gw, b, 0, s, c->d, st, L);
+        hipLaunchKernelGGL(k_lbvh_query<3>, gq, bq, 0, s, c->d, st, L, G);
+    const dim3 gw((c->W + 255) / 256);
     hipLaunchKernelGGL(k_lbvh_select, gw, dim3(kDetectNT), 0, s, c->d, st, L);
     launch_finish(c, s, st, stamps);
 }
@@ -923,6 +929,15 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
     d.tile_par = (!opt.batch_member && use_run_merge(c->S, c->CH)) ? 1 : 0;
     d.unit_max = opt.batch_member ? kUnitCap : unit_max_single(c->dim);
     d.wait_spins = kClassifyWaitSpins;
+    // LBVH walks: 16 queries per wavefront in 2-D and 3-D (a smaller box,
+    // fewer leaves per group), 64 in 1-D; RTOD_LBVH_GROUP (16, 32 or 64)
+    // overrides
+    c->lbvh_group = cfg->dim == 1 ? 64 : 16;
+    if (const char *e = std::getenv("RTOD_LBVH_GROUP")) {
+        const int v = std::atoi(e);
+        if (v == 16 || v == 32 || v == 64)
+            c->lbvh_group = v;
+    }
     if (const char *e = std::getenv("RTOD_FAIL_AT_SLIDE")) {
         char *end = nullptr;
         const long long v = std::strtoll(e, &end, 10);
@@ -980,6 +995,7 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
         RTOD_TRY(dalloc(c, &L.box, 2 * W * 6));
         RTOD_TRY(dalloc(c, &L.flag, W));
         RTOD_TRY(dalloc(c, &L.count, W));
+        RTOD_TRY(dalloc(c, &d.cmask, (W + 63) / 64));
         int passes = 0;
         for (size_t Lr = 4096; Lr < W; Lr *= 2)
             ++passes;
@@ -1145,7 +1161,7 @@ std::string err_text(int32_t err)
                  {kErrMergeNew, "a window-merge new point ranked out of range"},
                  {kErrStencilRows, "a detect unit's stencil exceeded 32 rows"},
                  {kErrClassifyWait, "a detect timed out waiting for its classify"},
-                 {kErrLbvhWalk, "an LBVH walk exceeded its step bound (malformed tree)"}};
+                 {kErrLbvhWalk, "an LBVH walk overflowed its node stack (malformed tree)"}};
     std::string s;
     for (const auto &m : kMsgs)
         if (err & m.bit)

@@ -1087,6 +1087,10 @@ __device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int 
     if constexpr (MODE == kClsPoints) {
         const uint64_t mask = __ballot(cand);
         const int cnt = __popcll(mask);
+        // the wave's 64 positions start at a multiple of 64 (NT is one)
+        const int wbase = i0 + (threadIdx.x & ~(kWave - 1));
+        if (d.cmask && lane == 0 && wbase < W)
+            d.cmask[wbase >> 6] = mask;
         if (cnt) {
             int base = 0;
             if (lane == 0)

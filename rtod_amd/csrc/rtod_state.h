@@ -181,6 +181,8 @@ struct Dev {
                            // sample stride exceeds 4; else null)
     int nsamp4;            // entries of samples4 (0: none)
     int32_t *cand;
+    uint64_t *cmask;    // LBVH index: candidate bits of each 64 sorted-window
+                        // points from a multiple of 64 (classify), or nullptr
     int2 *units;        // cell units: {start, len}
     int32_t *cand_cnt;  // per-unit minimum count (capped)
     uint32_t *bitmap;   // [3][bm_words]: outlier bits of slide t at (t % 3)

@@ -95,6 +95,7 @@ def arena_bytes(dim, W, S, lbvh=False):
     if lbvh:
         b += 2 * 8 * W + 2 * 4 * W + 4 * W + 4 * W + 4 * 2 * W + 4 * 12 * W + 4 * W + 4 * W
         b += 4                                                # n_whole (one int)
+        b += 8 * ((W + 63) // 64)                             # candidate masks
     return b
 
 
