@@ -305,7 +305,7 @@ void launch_merge_window_dim(rtod_ctx *c, hipStream_t s, const Step &st, const S
 {
     const int nmb = (c->W + kMergeT - 1) / kMergeT;
     if (!next && !prev) {
-        hipLaunchKernelGGL(k_merge_window<DIM>, dim3(nmb), dim3(256), 0, s, c->d, st);
+        hipLaunchKernelGGL(k_merge_window<DIM>, dim3(nmb), dim3(kMergeT), 0, s, c->d, st);
         return;
     }
     const int fin = prev ? c->d.fin_parts : 0;

@@ -599,27 +599,27 @@ __device__ __forceinline__ int wave_count_lds(const uint64_t *tab, int n, uint64
 //   batch, both batches sorted by key with arrival order on ties.
 //   survivor a at index i:  pos = i - #E before a + #B with key < key_a
 //   b at index j of B:      pos = j + #A with key <= k_b - #E with key <= k_b
-// Workgroup c merges the A chunk [i0, i1) (six chunk-boundary searches, the
-// B and E key ranges the chunk touches staged in LDS) and places an equal
-// share of B, ranked against A and E through their sample tables in LDS,
-// so no workgroup waits on a long tail of new points.  #E before A[i0] is
+// Workgroup c merges the A chunk [i0, i1) (chunk-boundary searches on the
+// batch sample tables, the B keys the chunk touches staged in LDS) and
+// places the new points whose keys fall in its key range
+// [key(A[i0]), key(A[i1])) (the first chunk from the start of B, the last
+// to its end): all of A before the chunk has keys <= key(A[i0]) and all of A
+// after it keys >= key(A[i1]), so #A <= k_b is i0 plus a search of the
+// chunk's keys in LDS, and #E <= k_b is #E before the chunk plus the chunk's
+// expired points among those; no search leaves LDS.  #E before A[i0] is
 // A[i0]'s own batch position when A[i0] expires.  Output positions that are
 // multiples of the sample stride fill A''s sample table.
 // ---------------------------------------------------------------------------
 constexpr int kMergeT = 512; // window elements per merge workgroup (half the chunk
                               // of 1024 measured 8% faster on the 1-D headline)
-
-// A's sample table as staged for the ranks of B: at most kMergeSamp
-// entries (every other sample when the table is longer), so the workgroup's
-// LDS stays small enough for three 512-thread workgroups per CU
-constexpr int kMergeSamp = kMaxSamples / 2;
+constexpr int kMergeRB = 4;   // new points per thread loaded ahead
 
 template <int NT> struct MergeWindowLds {
     static constexpr int IT = kMergeT / NT, T = kMergeT, CAP = 2048, MAXBS = 512;
     uint64_t k[T];
     uint64_t bs[MAXBS], es[MAXBS];
-    alignas(16) uint64_t as[kMergeSamp];
     alignas(16) uint64_t br[CAP];
+    uint64_t exm[T / kWave]; // expired bits of the chunk, 64 positions a word
     int rc[IT][NT / kWave];
     int res[8];
     uint64_t kn;
@@ -636,8 +636,8 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
     static_assert(IT * NT == T, "merge tile");
     uint64_t *s_k = lds.k;
     uint64_t *s_bs = lds.bs, *s_es = lds.es;
-    uint64_t *s_as = lds.as;
     uint64_t *s_br = lds.br;
+    uint64_t *s_exm = lds.exm;
     auto &s_rc = lds.rc;
     int *s_res = lds.res;
     uint64_t &s_kn = lds.kn;
@@ -672,9 +672,7 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
     const bool last = i1 == W;
     const int sl2 = d.stride_log2, smask = (1 << sl2) - 1;
     const int bsl = d.bs_log2, nbs = d.nbs;
-    // this workgroup's share of B
-    const int bper = (S + nmb - 1) / nmb;
-    const int j0 = min(S, bid * bper), j1 = min(S, j0 + bper);
+    (void)nmb;
 
     if (bid == 0 && tid == 0) {
         SlideCnt *cn = cnt_of(d, st.t);
@@ -698,8 +696,8 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
     }
 
     // round 1: the chunk (keys, slots, batch positions, coordinates; striped,
-    // item i of thread t is chunk element i*NT + t), the share of B, and the
-    // sample tables of A, B and E
+    // item i of thread t is chunk element i*NT + t) and the sample tables of
+    // B and E
     int32_t sl[IT], ps[IT];
     float cx[IT][3];
     uint64_t kk[IT];
@@ -723,40 +721,8 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
             ex[i] = sl[i] >= exp_lo && sl[i] < exp_lo + S;
         }
     }
-    // B share: one item per thread per round (bper <= NT in practice)
-    const int jb = j0 + tid;
-    uint64_t bk = 0;
-    int32_t bsl_ = 0;
-    float bx[3] = {0.f, 0.f, 0.f};
-    if (jb < j1) {
-        bk = Bk[jb];
-        bsl_ = Bs[jb];
-#pragma unroll
-        for (int a = 0; a < DIM; ++a)
-            bx[a] = Bx[(size_t)a * bxs + jb];
-    }
     copy_to_lds<2>(s_bs, Bsamp, nbs);
     copy_to_lds<2>(s_es, Esamp, nbs);
-    // A's sample table, every (1 << sst)-th entry
-    const int sst = d.nsamp > kMergeSamp ? 1 : 0;
-    const int nas = (d.nsamp + (1 << sst) - 1) >> sst;
-    {
-        constexpr int U = kMergeSamp / NT;
-        const uint64_t *Asamp = d.samples[par];
-        uint64_t v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int i = tid + u * NT;
-            if (i < nas)
-                v[u] = Asamp[i << sst];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int i = tid + u * NT;
-            if (i < nas)
-                s_as[i] = v[u];
-        }
-    }
     if (tid == 0) {
         s_kn = last ? 0 : Ak[i1];
         s_pos0 = Ap[i0];
@@ -769,20 +735,24 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
             s_k[l] = kk[i];
         const uint64_t m = __ballot(ex[i]);
         below[i] = __popcll(m & ((1ull << lane) - 1ull));
-        if (lane == 0)
+        if (lane == 0) {
             s_rc[i][wid] = __popcll(m);
+            s_exm[((i * NT) >> 6) + wid] = m; // positions i*NT + 64*wid + lane
+        }
     }
     __syncthreads();
 
-    // five chunk-boundary searches on the batch sample tables, one wave each
-    // (waves 0,1 take two):  0: #E <= key(A[i0])  1: lb(B, key(A[i0]))
-    //   2: lb(B, key(A[i1-1]))  3: lb(E, key(A[i0]))  4: lb(E, key(A[i1]))
+    // four chunk-boundary searches on the batch sample tables, one wave each:
+    //   0: #E <= key(A[i0])  1: lb(B, key(A[i0]))  2: lb(B, key(A[i1-1]))
+    //   3: lb(B, key(A[i1]))
     const uint64_t k0 = s_k[0], kl = s_k[n - 1], kn = s_kn;
     const int bstride = 1 << bsl;
-    for (int q = wid; q < 5; q += NT / kWave) {
-        const bool onE = (q == 0 || q >= 3);
+    static_assert(NT / kWave >= 4, "a wave per search");
+    if (wid < 4) {
+        const int q = wid;
+        const bool onE = q == 0;
         const bool upper = q == 0;
-        const uint64_t key = (q == 2) ? kl : (q == 4) ? kn : k0;
+        const uint64_t key = (q == 2) ? kl : (q == 3) ? kn : k0;
         const int c = wave_count_lds(onE ? s_es : s_bs, nbs, key, upper);
         // finish inside one stride of the batch: lanes probe it in one round
         int res = 0;
@@ -807,9 +777,30 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
     const bool ex0 = s_slot0 >= exp_lo && s_slot0 < exp_lo + S;
     const int e_before = ex0 ? s_pos0 : s_res[0];
     const int b_lo = s_res[1], b_hi = s_res[2];
-    (void)first;
-
-    // stage the B range the survivors search
+    // the new points this chunk places: B[jb0, jb1), keys in the chunk's
+    // key range; the first kMergeRB per thread (keys, slots, coordinates)
+    // requested with the staging of the survivors' B range [b_lo, b_hi) (a
+    // slide's new points can crowd into a few chunks' key ranges: a random
+    // walk's slide spans a narrow band of values)
+    const int jb0 = first ? 0 : b_lo, jb1 = last ? S : s_res[3];
+    const int nj = jb1 - jb0;
+    uint64_t bk_[kMergeRB];
+    int32_t bsl_[kMergeRB];
+    float bx[kMergeRB][3];
+#pragma unroll
+    for (int r = 0; r < kMergeRB; ++r) {
+        const int jj = tid + r * NT;
+        bk_[r] = 0;
+        bsl_[r] = 0;
+        bx[r][0] = bx[r][1] = bx[r][2] = 0.f;
+        if (jj < nj) {
+            bk_[r] = Bk[jb0 + jj];
+            bsl_[r] = Bs[jb0 + jj];
+#pragma unroll
+            for (int a = 0; a < DIM; ++a)
+                bx[r][a] = Bx[(size_t)a * bxs + jb0 + jj];
+        }
+    }
     const int nb = b_hi - b_lo;
     const bool bfit = nb <= CAP;
     if (bfit)
@@ -869,35 +860,28 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
             Nsamp4[np >> 2] = key;
     }
 
-    // this workgroup's share of B: #A <= k_b via A's sample table, #E <= k_b
-    // via E's, each finished by a short search inside one stride
-    for (int j = jb; j < j1; j += NT) {
-        uint64_t key = bk;
-        int32_t sb = bsl_;
-        float px[3] = {bx[0], bx[1], bx[2]};
-        if (j != jb) { // more than NT per workgroup: later rounds load here
-            key = Bk[j];
-            sb = Bs[j];
-#pragma unroll
-            for (int a = 0; a < DIM; ++a)
-                px[a] = Bx[(size_t)a * bxs + j];
+    // the new points B[jb0, jb1): #A <= k_b = i0 + the chunk's keys <= k_b
+    // (LDS), #E <= k_b = #E before the chunk + its expired points among those
+    // a thread's keys ascend over its rounds and repeat in long runs (a 1-D
+    // slide covers few cells): the ranks of its last key are kept
+    bool have = false;
+    uint64_t last_key = 0;
+    int ca = 0, ce = 0;
+    auto place_new = [&](int j, uint64_t key, int32_t sb, const float (&px)[3]) {
+        if (!have || key != last_key) {
+            ca = upper_bound_u64(s_k, 0, n, key);
+            ce = 0;
+            for (int w = 0; w < (ca >> 6); ++w)
+                ce += __popcll(s_exm[w]);
+            if (ca & 63)
+                ce += __popcll(s_exm[ca >> 6] & ((1ull << (ca & 63)) - 1ull));
+            have = true;
+            last_key = key;
         }
-        const int ca = upper_bound_u64(s_as, 0, nas, key); // staged samples <= key
-        int ubA = 0;
-        if (ca > 0) {
-            const int L = ((ca - 1) << (sl2 + sst)) + 1, H = min(ca << (sl2 + sst), W);
-            ubA = upper_bound_u64(Ak, L, H, key);
-        }
-        const int ce = upper_bound_u64(s_es, 0, nbs, key);
-        int ubE = 0;
-        if (ce > 0) {
-            const int L = ((ce - 1) << bsl) + 1, H = min(ce << bsl, S);
-            ubE = upper_bound_u64(Ek, L, H, key);
-        }
-        const int np = j + ubA - ubE;
+        const int np = j + i0 + ca - (e_before + ce);
         if ((unsigned)np >= (unsigned)W) {
             atomicOr(&d.ctl->err, kErrMergeNew);
-            continue;
+            return;
         }
         Nk[np] = key;
         Ns[np] = sb;
@@ -909,15 +893,52 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
             Nsamp[np >> sl2] = key;
         if (Nsamp4 && (np & 3) == 0)
             Nsamp4[np >> 2] = key;
+    };
+#pragma unroll
+    for (int r = 0; r < kMergeRB; ++r) {
+        const int j = jb0 + tid + r * NT;
+        if (j >= jb1)
+            break;
+        place_new(j, bk_[r], bsl_[r], bx[r]);
+    }
+    // more than kMergeRB * NT new points in this chunk's range: each round's
+    // loads are issued before the previous round's stores
+    int j = jb0 + tid + kMergeRB * NT;
+    uint64_t key = 0;
+    int32_t sb = 0;
+    float px[3] = {0.f, 0.f, 0.f};
+    auto load_new = [&](int jj, uint64_t &k, int32_t &b, float (&x)[3]) {
+        k = Bk[jj];
+        b = Bs[jj];
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+            x[a] = Bx[(size_t)a * bxs + jj];
+    };
+    if (j < jb1)
+        load_new(j, key, sb, px);
+    while (j < jb1) {
+        const int jn = j + NT;
+        uint64_t kn2 = 0;
+        int32_t sn = 0;
+        float pn[3] = {0.f, 0.f, 0.f};
+        if (jn < jb1)
+            load_new(jn, kn2, sn, pn);
+        place_new(j, key, sb, px);
+        j = jn;
+        key = kn2;
+        sb = sn;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            px[a] = pn[a];
     }
 }
 
 template <int DIM>
-__global__ __launch_bounds__(256) void k_merge_window(Dev d, Step st)
+__global__ __launch_bounds__(kMergeT) void k_merge_window(Dev d, Step st)
 {
-    __shared__ MergeWindowLds<256> lds;
+    __shared__ MergeWindowLds<kMergeT> lds;
     stamp_start(d, kTsMerge);
-    merge_window_body<DIM, 256>(d, st, blockIdx.x, lds);
+    merge_window_body<DIM, kMergeT>(d, st, blockIdx.x, lds);
 }
 
 // ---------------------------------------------------------------------------
