@@ -565,6 +565,8 @@ def main() -> int:
                     help="skip the other workloads, the sweeps and the index comparison")
     ap.add_argument("--workload", default="walk1d-W100k-S5k", choices=sorted(workloads()),
                     help="profiling runs only: the JSON line always names the workload run")
+    ap.add_argument("--index", default="auto", choices=("auto", "stencil", "lbvh"),
+                    help="profiling runs only: the index of the measured workload's legs")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-streams", type=int, default=256)
     ap.add_argument("--batched-only", action="store_true",
@@ -596,20 +598,22 @@ def main() -> int:
     log(f"[bench] {args.workload}: {xs.shape[0]} points; warmup {args.warmup} "
         f"steps {args.steps}; {rtod_amd.version()}")
 
+    ix = {"auto": rtod_amd.INDEX_AUTO, "stencil": rtod_amd.INDEX_CELL_STENCIL,
+          "lbvh": rtod_amd.INDEX_LBVH}[args.index]
     # the measured value: host-fed pipelined slides
-    hp = host_pipelined(xs, W, S, R, K, args.warmup, args.steps)
+    hp = host_pipelined(xs, W, S, R, K, args.warmup, args.steps, index=ix)
     log(f"[bench] host-fed pipelined (rtod_slides): {hp['us']:.2f} us/slide")
     # one call per slide, host input: latency and the reference's buckets
-    sc = single_calls(xs, W, S, R, K, args.warmup, args.steps)
+    sc = single_calls(xs, W, S, R, K, args.warmup, args.steps, index=ix)
     log(f"[bench] one slide per call (rtod_slide): {sc['us']:.2f} us/slide "
         f"(outer {sc['outer_us']:.2f}); buckets {sc['buckets_ms']}; stages {sc['stages_ms']}")
-    dp = device_pipelined(xs, W, S, R, K, args.warmup, args.steps)
+    dp = device_pipelined(xs, W, S, R, K, args.warmup, args.steps, index=ix)
     log(f"[bench] HBM-fed pipelined (rtod_slides_device): {dp['us']:.2f} us/slide")
     mism = same(hp["lists"], sc["lists"]) + same(dp["lists"], sc["lists"])
 
     # the dominant kernel of the measured path, HIP events around each launch
     timed = device_pipelined(xs, W, S, R, K, min(args.warmup, 10), min(args.steps, 60),
-                             flags=rtod_amd.FLAG_PHASE_TIMES)
+                             index=ix, flags=rtod_amd.FLAG_PHASE_TIMES)
     kern = {k: v[0] for k, v in timed["kernels"].items() if v[1] > 0}
     dom = max(kern, key=kern.get)
     dom_ms = kern[dom]
@@ -676,8 +680,9 @@ def main() -> int:
         "data": "synthetic: seeded 1-D random walk (start 10, N(0,0.05) steps, 0.2% spikes "
                 "+-U(2,5)); 3-D: mixture of 8 drifting anisotropic Gaussians + 0.5% noise",
         "config": {"workload": f"{args.workload} W={W} S={S} R={R} K={K} dim={dim} (plain "
-                               "schedule, cell-stencil index; slides from host memory, "
-                               "pipelined on one stream, one launch per slide)",
+                               f"schedule, {'LBVH' if args.index == 'lbvh' else 'cell-stencil'} "
+                               "index; slides from host memory, pipelined on one stream"
+                               f"{'' if args.index == 'lbvh' else ', one launch per slide'})",
                    "window": W, "slide": S, "R": R, "K": K, "dim": dim},
         "valid": invalid == 0,
         "mismatched_slides": int(mism),

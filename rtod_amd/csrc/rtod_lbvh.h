@@ -70,10 +70,15 @@ __device__ __forceinline__ uint64_t spread2(uint64_t x)
     return x;
 }
 
-template <int DIM> __device__ __forceinline__ uint64_t morton_of_key(uint64_t key)
+// fields relative to org (a lower bound of the window's fields; 0: raw)
+template <int DIM>
+__device__ __forceinline__ uint64_t morton_of_key(uint64_t key, const uint64_t (&org)[3])
 {
     uint64_t f[3];
     key_fields<DIM>(key, f);
+#pragma unroll
+    for (int a = 0; a < DIM; ++a)
+        f[a] = f[a] >= org[a] ? f[a] - org[a] : 0;
     if constexpr (DIM == 1)
         return f[0];
     else if constexpr (DIM == 2)
@@ -90,15 +95,21 @@ __global__ __launch_bounds__(256) void k_lbvh_codes(Dev d, Step st, Lbvh L)
     const int U = *L.n_cand;
     if (i >= U)
         return;
+    // the slide's code origin (classify); unset (~0, or 0 at prime): raw fields
+    const SlideCnt *cn = cnt_of(d, st.t);
+    uint64_t org[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        org[a] = cn->fmin[a] == ~0ull ? 0 : cn->fmin[a];
     if (L.whole) { // leaf i: ring slot i, the point's cell from its coordinates
         float p[3] = {0.f, 0.f, 0.f};
 #pragma unroll
         for (int a = 0; a < DIM; ++a)
             p[a] = d.ring[(size_t)a * d.W + i];
-        L.code[0][i] = morton_of_key<DIM>(cell_key<DIM>(p, d.inv_c));
+        L.code[0][i] = morton_of_key<DIM>(cell_key<DIM>(p, d.inv_c), org);
     } else {
         const uint64_t *Nk = d.akey[st.par ^ 1];
-        L.code[0][i] = morton_of_key<DIM>(Nk[d.cand[i]]);
+        L.code[0][i] = morton_of_key<DIM>(Nk[d.cand[i]], org);
     }
     L.item[0][i] = i;
 }

@@ -683,6 +683,7 @@ __device__ __forceinline__ void merge_window_body(const Dev &d, const Step &st, 
         cn->pair_tests = 0;
         cn->hits = 0;
         cn->misses = 0;
+        cn->fmin[0] = cn->fmin[1] = cn->fmin[2] = ~0ull;
     }
     if (d.ready && bid == 0 && tid < kReadyFlags)
         *ready_of(d, st.t, tid) = 0;
@@ -1112,13 +1113,61 @@ __device__ __forceinline__ void classify_body(const Dev &d, const Step &st, int 
         const int wbase = i0 + (threadIdx.x & ~(kWave - 1));
         if (d.cmask && lane == 0 && wbase < W)
             d.cmask[wbase >> 6] = mask;
-        if (cnt) {
-            int base = 0;
-            if (lane == 0)
-                base = atomicAdd(&cnt_of(d, st.t)->n_cand, cnt);
-            base = __shfl(base, 0, kWave);
-            if (cand)
-                d.cand[base + __popcll(mask & lt)] = i;
+        // per workgroup, once every wave is done reading the key tile (which
+        // then holds the waves' figures): one atomic for the candidate list
+        // base, and the window's per-axis minimum key field -- the origin of
+        // the LBVH's Morton codes, so that codes of data straddling a
+        // power-of-two cell coordinate stay narrow (fewer sort passes)
+        uint64_t f[3], fm[3];
+        key_fields<DIM>(key, f);
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) {
+            uint64_t m = i < W ? f[a] : ~0ull;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint64_t v = __shfl_xor(m, o, kWave);
+                m = v < m ? v : m;
+            }
+            fm[a] = m;
+        }
+        constexpr int NWV = NT / kWave;
+        static_assert(4 * NWV + 1 <= NT, "the waves' figures fit the key tile");
+        const int wid = threadIdx.x / kWave;
+        __syncthreads();
+        if (lane == 0) {
+            s_t[4 * wid] = (uint64_t)cnt;
+#pragma unroll
+            for (int a = 0; a < DIM; ++a)
+                s_t[4 * wid + 1 + a] = fm[a];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t tot = 0;
+            for (int w = 0; w < NWV; ++w)
+                tot += s_t[4 * w];
+            s_t[4 * NWV] = tot ? (uint64_t)atomicAdd(&cnt_of(d, st.t)->n_cand, (int)tot) : 0;
+        } else if (threadIdx.x <= DIM && (bid == 0 || threadIdx.x > 1)) {
+            // axis 0 is the key's most significant field, so the window's
+            // first point holds its minimum: workgroup 0 alone records it;
+            // the other axes' minima are recorded where they lower the value
+            // read (minima only decrease), not by every workgroup's atomic
+            // on the one address
+            const int a = threadIdx.x - 1;
+            uint64_t m = ~0ull;
+            for (int w = 0; w < NWV; ++w) {
+                const uint64_t v = s_t[4 * w + 1 + a];
+                m = v < m ? v : m;
+            }
+            unsigned long long *fa = &cnt_of(d, st.t)->fmin[a];
+            if (m != ~0ull &&
+                m < __hip_atomic_load(fa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMin(fa, (unsigned long long)m);
+        }
+        __syncthreads();
+        if (cand) {
+            int base = (int)s_t[4 * NWV];
+            for (int w = 0; w < wid; ++w)
+                base += (int)s_t[4 * w];
+            d.cand[base + __popcll(mask & lt)] = i;
         }
         return;
     } else {

@@ -20,6 +20,8 @@
 //   samples4 u64 [2][W/4]       every 4th key (when the stride exceeds 4)
 //   cand     i32 [W]            candidate positions (sorted-window index; the
 //                               LBVH index's per-point list)
+//   cmask    u64 [W/64]         LBVH index: candidate bits of each 64 sorted
+//                               positions
 //   units    i32x2 [W]          cell units (start, len <= 4 in 1-D, 8 else): runs of one
 //                               sparse cell's candidates in the sorted window
 //   bitmap   u32 [3][W/32]      outlier flags by ring slot, one set per
@@ -63,6 +65,9 @@ struct SlideCnt {
     alignas(128) unsigned long long pair_tests; // predicate evaluations of the slide
     alignas(128) unsigned long long hits; // sum over candidates of min(count, K + 1)
     alignas(128) int32_t misses; // learned: mispredicted units of the slide
+    // LBVH index: per-axis minimum key field over the window (classify), the
+    // origin of the slide's Morton codes (0 before any classify: raw fields)
+    alignas(128) unsigned long long fmin[3];
 };
 
 struct Ctl {
