@@ -81,21 +81,26 @@ const char *kKernelNames[kKCount] = {"ingest_sort",         "merge_pass",  "merg
                                      "detect_stage",        "slide_stage"};
 
 // LDS-sort tile shapes (threads x items per thread).  A batch of up to
-// kRunMergeCap points is sorted as tiles of 1024 points on as many
+// kRunMax * 1024 points is sorted as tiles of 1024 points on as many
 // workgroups (one CU sorts a tile; a whole 8k-point tile on one CU costs
 // several microseconds per 4-bit pass), then merged by one k_merge_runs
-// launch.  Larger batches use the largest tile that fits 160 KB of LDS
-// (coordinates/counters pool, keys, 16-bit values) and pairwise merge passes.
+// launch; up to kRunMergeCap points, as tiles of 2048.  Larger batches use
+// the largest tile that fits 160 KB of LDS (coordinates/counters pool, keys,
+// 16-bit values) and pairwise merge passes.
 struct SortShape {
     int nt, it;
 };
-constexpr SortShape kSortShapes[5] = {{256, 2}, {256, 4}, {512, 4}, {1024, 4}, {1024, 8}};
+constexpr SortShape kSortShapes[6] = {{256, 2},  {256, 4},  {512, 4},
+                                      {1024, 4}, {1024, 8}, {256, 8}};
 constexpr int kBatchSortShape = 3; // batched streams: the 1024 x 4 tile
 static_assert(kSortShapes[kBatchSortShape].nt == kBatchSortNT &&
                   kSortShapes[kBatchSortShape].it == kBatchSortIT,
               "batched ingest tile");
-constexpr int kRunTileShape = 1; // 1024-point tiles merged by k_merge_runs
-static_assert(kSortShapes[kRunTileShape].nt * kSortShapes[kRunTileShape].it == kRunLen,
+constexpr int kRunTileShape = 1;  // 1024-point tiles merged by k_merge_runs
+constexpr int kRunTileShape2 = 5; // 2048-point tiles merged by k_merge_runs
+static_assert(kSortShapes[kRunTileShape].nt * kSortShapes[kRunTileShape].it == kRunLen &&
+                  kSortShapes[kRunTileShape2].nt * kSortShapes[kRunTileShape2].it ==
+                      2 * kRunLen,
               "runs merged by k_merge_runs");
 
 int pick_sort_shape(int S, int dim)
@@ -103,13 +108,19 @@ int pick_sort_shape(int S, int dim)
     const int one = kSortShapes[0].nt * kSortShapes[0].it;
     if (S <= one)
         return 0;
-    if (S <= kRunMergeCap)
+    if (S <= kRunMax * kRunLen)
         return kRunTileShape;
+    if (S <= kRunMergeCap)
+        return kRunTileShape2;
     return dim == 3 ? 3 : 4; // (1024, 8) x 3 floats exceeds LDS
 }
 
-// k_merge_runs covers a batch of several tiles in one launch
-bool use_run_merge(int S, int CH) { return CH == kRunLen && S > CH && S <= kRunMergeCap; }
+// k_merge_runs covers a batch of several tiles (of 1024 or 2048 points, at
+// most kRunMax) in one launch
+bool use_run_merge(int S, int CH)
+{
+    return (CH == kRunLen || CH == 2 * kRunLen) && S > CH && S <= kRunMax * CH;
+}
 
 // the next slide's ingest stage can ride in this slide's core launches:
 // 256-thread sort tiles (the merge's workgroup size), at most one merge
@@ -232,6 +243,10 @@ template <int DIM> void launch_ingest(rtod_ctx *c, hipStream_t s, const Step &st
         hipLaunchKernelGGL((k_ingest_sort<DIM, 1024, 4>), g, dim3(1024), 0, s, c->d, st, mode,
                            src, dst_batch);
         break;
+    case 5:
+        hipLaunchKernelGGL((k_ingest_sort<DIM, 256, 8>), g, dim3(256), 0, s, c->d, st, mode, src,
+                           dst_batch);
+        break;
     default:
         if constexpr (DIM < 3)
             hipLaunchKernelGGL((k_ingest_sort<DIM, 1024, 8>), g, dim3(1024), 0, s, c->d, st, mode,
@@ -313,6 +328,9 @@ void launch_merge_window_dim(rtod_ctx *c, hipStream_t s, const Step &st, const S
     const Step nx = next ? *next : Step{}, pv = prev ? *prev : Step{};
     if (c->sort_shape == 0)
         hipLaunchKernelGGL((k_merge_window_ingest<DIM, 2>), g, dim3(256), 0, s, c->d, st, nx, pv,
+                           nmb, fin);
+    else if (c->sort_shape == kRunTileShape2)
+        hipLaunchKernelGGL((k_merge_window_ingest<DIM, 8>), g, dim3(256), 0, s, c->d, st, nx, pv,
                            nmb, fin);
     else
         hipLaunchKernelGGL((k_merge_window_ingest<DIM, 4>), g, dim3(256), 0, s, c->d, st, nx, pv,
@@ -445,16 +463,25 @@ void launch_detect_stage_t(rtod_ctx *c, hipStream_t s, const Step &st, const Sta
                        pv, mv, rv, tv, nfin, nmb, nrm, nts, ndet);
 }
 
+template <int DIM, bool LEARN>
+void launch_detect_stage_l(rtod_ctx *c, hipStream_t s, const Step &st, const StageParts &p)
+{
+    // tiles of CH = kDetectNT * TSIT points (ride_along: CH is 512, 1024 or 2048)
+    if (c->CH == kDetectNT)
+        launch_detect_stage_t<DIM, LEARN, 1>(c, s, st, p);
+    else if (c->CH == 2 * kDetectNT)
+        launch_detect_stage_t<DIM, LEARN, 2>(c, s, st, p);
+    else
+        launch_detect_stage_t<DIM, LEARN, 4>(c, s, st, p);
+}
+
 template <int DIM>
 void launch_detect_stage_dim(rtod_ctx *c, hipStream_t s, const Step &st, const StageParts &p)
 {
-    const bool t1 = c->CH == kDetectNT; // else CH == 2 * kDetectNT (ride_along)
     if (st.learn)
-        t1 ? launch_detect_stage_t<DIM, true, 1>(c, s, st, p)
-           : launch_detect_stage_t<DIM, true, 2>(c, s, st, p);
+        launch_detect_stage_l<DIM, true>(c, s, st, p);
     else
-        t1 ? launch_detect_stage_t<DIM, false, 1>(c, s, st, p)
-           : launch_detect_stage_t<DIM, false, 2>(c, s, st, p);
+        launch_detect_stage_l<DIM, false>(c, s, st, p);
 }
 
 // one launch per slide (plain schedule): slide st's classify in front of
@@ -481,15 +508,26 @@ void launch_slide_stage_t(rtod_ctx *c, hipStream_t s, const Step &st, const Stag
                        rv, tv, ncls, nfin, nmb, nrm, nts, ndet, pre);
 }
 
+template <int DIM>
+void launch_slide_stage_dim(rtod_ctx *c, hipStream_t s, const Step &st, const StageParts &p)
+{
+    // tiles of CH = kDetectNT * TSIT points (ride_along: CH is 512, 1024 or 2048)
+    if (c->CH == kDetectNT)
+        launch_slide_stage_t<DIM, 1>(c, s, st, p);
+    else if (c->CH == 2 * kDetectNT)
+        launch_slide_stage_t<DIM, 2>(c, s, st, p);
+    else
+        launch_slide_stage_t<DIM, 4>(c, s, st, p);
+}
+
 void launch_slide_stage(rtod_ctx *c, hipStream_t s, const Step &st, const StageParts &p)
 {
-    const bool t1 = c->CH == kDetectNT; // else CH == 2 * kDetectNT (ride_along)
     if (c->dim == 1)
-        t1 ? launch_slide_stage_t<1, 1>(c, s, st, p) : launch_slide_stage_t<1, 2>(c, s, st, p);
+        launch_slide_stage_dim<1>(c, s, st, p);
     else if (c->dim == 2)
-        t1 ? launch_slide_stage_t<2, 1>(c, s, st, p) : launch_slide_stage_t<2, 2>(c, s, st, p);
+        launch_slide_stage_dim<2>(c, s, st, p);
     else
-        t1 ? launch_slide_stage_t<3, 1>(c, s, st, p) : launch_slide_stage_t<3, 2>(c, s, st, p);
+        launch_slide_stage_dim<3>(c, s, st, p);
 }
 
 void launch_detect_stage(rtod_ctx *c, hipStream_t s, const Step &st, const StageParts &p)
@@ -890,6 +928,7 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
     d.m = c->m;
     d.K = c->K;
     d.cpb = (c->S + c->CH - 1) / c->CH;
+    d.run_len = c->CH;
     int sl = 2; // sample stride >= 4, table <= kMaxSamples entries
     while (((int64_t)c->W + (1ll << sl) - 1) >> sl > kMaxSamples)
         ++sl;

@@ -445,9 +445,9 @@ __global__ __launch_bounds__(256) void k_merge_pass(Dev d, Step st, const uint64
 
 // ---------------------------------------------------------------------------
 // k_merge_runs: one launch turning each batch segment's tile-sorted runs
-// (length L = kRunLen, the last one shorter) into the sorted batch: ring
-// entry and sample table.  The segment's keys (S <= kRunMergeCap) are
-// staged in LDS.  Element i of run r lands at
+// (length L = d.run_len, 1024 or 2048, the last one shorter; at most kRunMax
+// runs) into the sorted batch: ring entry and sample table.  The segment's
+// keys (S <= kRunMax * L <= kRunMergeCap) are staged in LDS.  Element i of run r lands at
 //     (i - r*L) + sum_{r' < r} #(run r' <= key) + sum_{r' > r} #(run r' < key)
 // which is stable: an earlier run holds earlier arrivals.  A thread places
 // kRunMergeIT elements (strided by the workgroup size; one, since the
@@ -458,18 +458,18 @@ __global__ __launch_bounds__(256) void k_merge_pass(Dev d, Step st, const uint64
 // log2(L)+1 steps issues up to kRunMax independent LDS reads per element.
 // Slots and coordinates are loaded before the staging barrier.
 // ---------------------------------------------------------------------------
-constexpr int kRunMergeNT = 512, kRunMergeIT = 1, kRunLen = 1024, kRunMergeCap = 8192;
-constexpr int kRunMax = kRunMergeCap / kRunLen;
+constexpr int kRunMergeNT = 512, kRunMergeIT = 1, kRunLen = 1024, kRunMax = 8;
+constexpr int kRunMergeCap = kRunMax * 2 * kRunLen; // 2048-point runs
 constexpr int kRunMergeT = kRunMergeNT * kRunMergeIT; // elements per workgroup
 
 // workgroups of k_merge_runs per batch segment of S points
 __host__ __device__ constexpr int run_merge_wgs(int S) { return (S + kRunMergeT - 1) / kRunMergeT; }
 
-template <int DIM>
-__device__ __forceinline__ void merge_runs_body(const Dev &d, const Step &st, int mode, int bid,
-                                                uint64_t *s_k)
+template <int DIM, int L>
+__device__ __forceinline__ void merge_runs_body_t(const Dev &d, const Step &st, int mode, int bid,
+                                                  uint64_t *s_k)
 {
-    constexpr int L = kRunLen, IT = kRunMergeIT;
+    constexpr int IT = kRunMergeIT;
     const int S = d.S;
     const int wps = run_merge_wgs(S);
     const int seg = bid / wps, w = bid % wps;
@@ -491,7 +491,7 @@ __device__ __forceinline__ void merge_runs_body(const Dev &d, const Step &st, in
                 cx[k][a] = d.txyz[0][(size_t)a * sxs + toff + i];
         }
     }
-    copy_to_lds<kRunMergeCap / kRunMergeNT>(s_k, sk, S);
+    copy_to_lds<kRunMax * kRunLen / kRunMergeNT>(s_k, sk, S);
     __syncthreads();
     const int nruns = (S + L - 1) / L;
     const int e = (mode == kModeSlide) ? st.enew : seg;
@@ -547,6 +547,17 @@ __device__ __forceinline__ void merge_runs_body(const Dev &d, const Step &st, in
         if ((o & ((1 << d.bs_log2) - 1)) == 0)
             d.bsamp[(size_t)e * d.nbs + (o >> d.bs_log2)] = key[k];
     }
+}
+
+// the context's run length (uniform): both searches fully unrolled
+template <int DIM>
+__device__ __forceinline__ void merge_runs_body(const Dev &d, const Step &st, int mode, int bid,
+                                                uint64_t *s_k)
+{
+    if (d.run_len == 2 * kRunLen)
+        merge_runs_body_t<DIM, 2 * kRunLen>(d, st, mode, bid, s_k);
+    else
+        merge_runs_body_t<DIM, kRunLen>(d, st, mode, bid, s_k);
 }
 
 template <int DIM>

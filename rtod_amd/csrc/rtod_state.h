@@ -161,6 +161,7 @@ struct Dev {
     int tile_par;     // slide batches sorted as several tiles go to the tile
                       // scratch at offset (t & 1) * S (two slides' tiles live)
     int cpb;          // sort chunks per batch
+    int run_len;      // points per sorted run k_merge_runs merges (the tile)
     int nsamp;        // samples in the sorted-window sample table
     int stride_log2;  // sample stride
     int nbs;          // samples per sorted batch

@@ -265,6 +265,8 @@ def test_pipelined_slides_vs_grid(name, wl, T, kw):
     (1, 20000, 2500, 20, "walk"),   # three sort tiles: run merge two slides ahead
     (3, 8000, 400, 20, "gmm"),      # one sort tile: sorted straight into the ring
     (2, 6000, 1500, 20, "walk"),    # two tiles
+    (1, 36864, 12288, 20, "walk"),  # six 2048-point tiles
+    (3, 32768, 16384, 20, "gmm"),   # eight 2048-point tiles: the run merge's cap
 ])
 def test_pipelined_short_runs(dim, W, S, K, kind):
     """Pipelined calls of 1, 2, 3, 1, 4 and 5 slides in a row (every length
