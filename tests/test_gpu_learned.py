@@ -310,3 +310,38 @@ def test_cost_rule_many_units_runs_learned_kernels():
     assert st2["plain_kernel_slides"] == 1, st2
     assert st2["predictor_points"] > 6144 * (T1 - 1 + T2), st2
     assert st2["train_steps"] > 0
+
+
+def test_learned_time_vs_plain_large_3d():
+    """Time as well as work, where the learned kernels run (the 3-D W=100k
+    stream, ~34k cell units a slide): after 200 online slides the learned
+    schedule does under 60% of the plain schedule's predicate evaluations
+    and is not slower (per-call device time, the mean over 60 slides; the
+    bench measures it about 5% faster)."""
+    from rtod_amd import synth
+    wl = synth.WORKLOADS["gmm3d-W100k-S5k"]
+    T_train, T_eval = 200, 60
+    case(f"{wl.name} learned vs plain time, {T_train} + {T_eval} slides")
+    xs = np.ascontiguousarray(wl.data(wl.W + (T_train + T_eval) * wl.S), np.float32)
+
+    def run(**kw):
+        with R.Detector(3, wl.W, wl.S, wl.R, wl.K, **kw) as det:
+            det.prime(xs[:wl.W])
+            lists, us = [], []
+            for t in range(1, T_train + T_eval + 1):
+                o = det.slide(xs[wl.W + (t - 1) * wl.S: wl.W + t * wl.S])
+                if t > T_train:
+                    lists.append(o)
+                    us.append(det.times.total * 1e3)
+            st = det.stats()
+        return lists, float(np.mean(us)), st
+
+    got_p, us_p, st_p = run()
+    got_l, us_l, st_l = run(sched_kind=R.SCHED_LEARNED)
+    assert_same(got_l, got_p, "learned vs plain")
+    assert st_l["plain_kernel_slides"] == 1, st_l  # the first slide only
+    print(f"{wl.name}: plain {us_p:.1f} us, learned {us_l:.1f} us per slide "
+          f"({us_p / us_l:.3f}x); pair tests {st_p['pair_tests_total']} -> "
+          f"{st_l['pair_tests_total']}", flush=True)
+    assert st_l["pair_tests_total"] < 0.6 * st_p["pair_tests_total"]
+    assert us_l <= 1.03 * us_p, (us_l, us_p)
