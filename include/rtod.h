@@ -40,9 +40,21 @@ typedef enum rtod_status {
 } rtod_status;
 
 typedef enum rtod_index_kind {
-    RTOD_INDEX_AUTO = 0,         /* cell stencil (fastest measured)           */
+    RTOD_INDEX_AUTO = 0,         /* the fastest measured: the slide-run index
+                                    where it applies (1-D), else the cell
+                                    stencil                                  */
     RTOD_INDEX_CELL_STENCIL = 1, /* cell-sorted window + sampled key table    */
-    RTOD_INDEX_LBVH = 2          /* Morton-code LBVH over candidate boxes     */
+    RTOD_INDEX_LBVH = 2,         /* Morton-code LBVH over candidate boxes     */
+    RTOD_INDEX_SLIDE_RUNS = 3    /* 1-D: each slide bucketed by sub-cell on
+                                    arrival, the window = the m latest runs,
+                                    nothing merged.  Applies with no ablation,
+                                    the plain schedule (or the learned one
+                                    without RTOD_FLAG_LEARN_ALWAYS: in 1-D its
+                                    cost rule runs the plain kernels),
+                                    S <= 16384, (W/S) * ceil(S/8192) <= 256
+                                    and W <= 2^18; requested where it does
+                                    not apply, rtod_create fails with
+                                    RTOD_ERR_UNSUPPORTED                     */
 } rtod_index_kind;
 
 typedef enum rtod_sched_kind {

@@ -25,7 +25,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librtod.so")
 
-INDEX_AUTO, INDEX_CELL_STENCIL, INDEX_LBVH = 0, 1, 2
+INDEX_AUTO, INDEX_CELL_STENCIL, INDEX_LBVH, INDEX_SLIDE_RUNS = 0, 1, 2, 3
 SCHED_PLAIN, SCHED_LEARNED = 0, 1
 ABL_NONE, ABL_FILTER_ONLY, ABL_NO_FILTER = 0, 1, 2
 FLAG_PHASE_TIMES = 0x1

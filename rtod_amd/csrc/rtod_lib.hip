@@ -24,6 +24,7 @@
 #include "../../include/rtod.h"
 #include "rtod_batch.h"
 #include "rtod_lbvh.h"
+#include "rtod_runs.h"
 
 using namespace rtod;
 
@@ -72,13 +73,20 @@ enum KernelId {
     kKDetectStage,       // two launches per slide (learned): detect + epilogue +
                          // window merge + run merge + tile sorts
     kKSlideStage,        // one launch per slide (plain): classify + the above
+    kKRunSort,           // slide-run index: a slide's runs sorted (two launches per slide)
+    kKRunDetect,         // slide-run index: a slide detected (two launches per slide)
+    kKRunSlide,          // slide-run index: one slide sorted and detected, one launch
+    kKRunStage,          // slide-run index, pipelined: the detects of kRunBatch slides
+                         // and the sorts of the next kRunBatch
     kKCount
 };
 const char *kKernelNames[kKCount] = {"ingest_sort",         "merge_pass",  "merge_window",
                                      "classify",            "detect",      "lbvh_build",
                                      "lbvh_query",          "merge_window_ingest",
                                      "detect_runs",         "classify_stage",
-                                     "detect_stage",        "slide_stage"};
+                                     "detect_stage",        "slide_stage",
+                                     "run_sort",            "run_detect",
+                                     "run_slide",           "run_stage"};
 
 // LDS-sort tile shapes (threads x items per thread).  A batch of up to
 // kRunMax * 1024 points is sorted as tiles of 1024 points on as many
@@ -164,6 +172,11 @@ struct rtod_ctx {
     float *h_in = nullptr;   // stage_slots pinned staging slots of S*dim floats
     int stage_slots = 2;     // slide t is staged in slot t % stage_slots
     HostOut *h_out = nullptr;
+    int n_hout = kHostOuts;  // host-out buffers: slide t writes t % n_hout
+    // 1-D slide-run index (rtod_runs.h): the window as the m latest sorted
+    // runs; every other path of the context is unused
+    bool runs = false;
+    RunsDev rd{};
     const float *dev_src = nullptr; // device-resident new slide of this call
     bool lbvh = false;
     bool learned = false;
@@ -678,7 +691,7 @@ Step make_step(const rtod_ctx *c) { return make_step_at(c, (uint64_t)c->host_t, 
 HostOut *host_out_of(const rtod_ctx *c, int64_t t)
 {
     return reinterpret_cast<HostOut *>(reinterpret_cast<char *>(c->h_out) +
-                                       (size_t)(t % kHostOuts) * (size_t)c->d.h_out_bytes);
+                                       (size_t)(t % c->n_hout) * (size_t)c->d.h_out_bytes);
 }
 
 // The slide's kernel sequence in two stages.  Ingest stage: the new batch
@@ -773,7 +786,7 @@ rtod_status validate(const rtod_config *cfg)
         set_error("R must be finite and in [1e-18, 1e18]");
         return RTOD_ERR_INVALID_ARGUMENT;
     }
-    if (cfg->index_kind < 0 || cfg->index_kind > 2 || cfg->sched_kind < 0 ||
+    if (cfg->index_kind < 0 || cfg->index_kind > 3 || cfg->sched_kind < 0 ||
         cfg->sched_kind > 1) {
         set_error("unknown index_kind or sched_kind");
         return RTOD_ERR_INVALID_ARGUMENT;
@@ -892,6 +905,133 @@ rtod_status check_abi_layout()
     return RTOD_OK;
 }
 
+// The slide-run index (rtod_runs.h) serves 1-D streams on the default path:
+// index AUTO or SLIDE_RUNS, no ablation, the plain schedule or the learned
+// one under its cost rule (which runs 1-D slides on the plain kernels), a
+// slide of at most two runs, at most kRunMaxRuns runs per window and a
+// window whose outlier bitmap fits LDS.  why: the reason it does not apply.
+bool runs_eligible(const rtod_config &cfg, const CreateOpts &opt, std::string *why)
+{
+    const int rps = (cfg.slide + kRunMaxLen - 1) / kRunMaxLen;
+    const char *no = nullptr;
+    if (opt.batch_member)
+        no = "batched streams run on the cell-stencil index";
+    else if (cfg.dim != 1)
+        no = "it indexes 1-D streams";
+    else if (cfg.index_kind != RTOD_INDEX_AUTO && cfg.index_kind != RTOD_INDEX_SLIDE_RUNS)
+        no = "another index was requested";
+    else if (cfg.ablation != RTOD_ABL_NONE)
+        no = "the ablation modes run on the cell stencil or the LBVH";
+    else if (cfg.sched_kind == RTOD_SCHED_LEARNED && (cfg.flags & RTOD_FLAG_LEARN_ALWAYS))
+        no = "RTOD_FLAG_LEARN_ALWAYS runs the learned cell-stencil kernels";
+    else if (rps > 2)
+        no = "slides of more than 16384 points";
+    else if ((int64_t)(cfg.window / cfg.slide) * rps > kRunMaxRuns)
+        no = "more than 256 runs per window (W / S too large)";
+    else if (cfg.window > kRunMaxW)
+        no = "windows of more than 2^18 points";
+    if (no && why)
+        *why = no;
+    return no == nullptr;
+}
+
+// the predictor's parameters, published copies, optimiser state and shared
+// control block, set to the initial network
+rtod_status alloc_predictor(rtod_ctx *c)
+{
+    Learn &lr = c->d.lr;
+    RTOD_TRY(dalloc(c, &lr.params, (size_t)kMlpNP));
+    RTOD_TRY(dalloc(c, &lr.pub, (size_t)2 * kMlpNP));
+    RTOD_TRY(dalloc(c, &lr.adam_m, (size_t)kMlpNP));
+    RTOD_TRY(dalloc(c, &lr.adam_v, (size_t)kMlpNP));
+    RTOD_TRY(dalloc(c, &lr.ls, 1));
+    RTOD_HIP(hipMemsetAsync(lr.ls, 0, sizeof(LearnShared), c->stream));
+    std::vector<float> p0 = initial_predictor();
+    RTOD_HIP(hipMemcpyAsync(lr.params, p0.data(), sizeof(float) * kMlpNP, hipMemcpyHostToDevice,
+                            c->stream));
+    for (int h = 0; h < 2; ++h)
+        RTOD_HIP(hipMemcpyAsync(lr.pub + (size_t)h * kMlpNP, p0.data(), sizeof(float) * kMlpNP,
+                                hipMemcpyHostToDevice, c->stream));
+    RTOD_HIP(hipMemsetAsync(lr.adam_m, 0, sizeof(float) * kMlpNP, c->stream));
+    RTOD_HIP(hipMemsetAsync(lr.adam_v, 0, sizeof(float) * kMlpNP, c->stream));
+    RTOD_HIP(hipStreamSynchronize(c->stream));
+    return RTOD_OK;
+}
+
+// A context on the slide-run index: the run ring (NE = m + 2 kRunBatch + 1
+// entries of rps runs: the window's m slides plus the slides a pipelined
+// launch detects and sorts ahead), the prime copy, kRunStageSlots pinned
+// staging slots and kRunHostOuts host-out buffers.  A learned context keeps
+// its predictor (rtod_get/set_predictor) but, in 1-D, never runs it.
+rtod_status runs_create(rtod_ctx *c, const rtod_config *cfg)
+{
+    c->runs = true;
+    RunsDev &r = c->rd;
+    r.W = c->W;
+    r.S = c->S;
+    r.m = c->m;
+    r.K = c->K;
+    r.rps = (c->S + kRunMaxLen - 1) / kRunMaxLen;
+    r.RL = (c->S + r.rps - 1) / r.rps;
+    r.NE = c->m + 2 * kRunBatch + 1;
+    r.n_hout = kRunHostOuts;
+    const float R = cfg->R;
+    r.r2 = R * R;
+    r.inv_c = 1.0 / ((double)R * (1.0 - 1e-5));
+    r.R_in = (double)R * (1.0 - 1e-5);
+    r.R_out = (double)R * (1.0 + 1e-5);
+    c->d.dim = 1;
+    c->d.W = c->W;
+    c->d.S = c->S;
+    c->d.m = c->m;
+    c->d.K = c->K;
+    const size_t nrun = (size_t)r.NE * r.rps, RL = (size_t)r.RL;
+    RTOD_TRY(dalloc(c, &r.rx, nrun * RL));
+    RTOD_TRY(dalloc(c, &r.rslot, nrun * RL));
+    RTOD_TRY(dalloc(c, &r.rpre, nrun * (kRunBMax + 1)));
+    RTOD_TRY(dalloc(c, &r.rcell, nrun * RL));
+    RTOD_TRY(dalloc(c, &r.rcs, nrun * (RL + 1)));
+    RTOD_TRY(dalloc(c, &r.meta, nrun));
+    RTOD_TRY(dalloc(c, &r.err, 1));
+    RTOD_TRY(dalloc(c, &r.tstamp, (size_t)kTsCount));
+    RTOD_TRY(dalloc(c, &c->d_prime, (size_t)c->W));
+    RTOD_HIP(hipMemsetAsync(r.meta, 0, sizeof(RunMeta) * nrun, c->stream));
+    RTOD_HIP(hipMemsetAsync(r.err, 0, sizeof(int32_t), c->stream));
+    RTOD_HIP(hipMemsetAsync(r.tstamp, 0, sizeof(uint64_t) * kTsCount, c->stream));
+    c->learned = cfg->sched_kind == RTOD_SCHED_LEARNED;
+    if (c->learned)
+        RTOD_TRY(alloc_predictor(c));
+    c->list_cap = c->W;
+    {
+        int khz = 0;
+        RTOD_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+        c->wall_khz = khz > 0 ? (double)khz : 100000.0;
+    }
+    if (const char *e = std::getenv("RTOD_FAIL_AT_SLIDE")) {
+        char *end = nullptr;
+        const long long v = std::strtoll(e, &end, 10);
+        if (end != e && *end == '\0' && v >= 0)
+            c->fail_at = v;
+    }
+    c->stage_slots = kRunStageSlots;
+    c->n_hout = kRunHostOuts;
+    const size_t in_bytes = (size_t)c->stage_slots * c->S * sizeof(float);
+    const size_t out_bytes = (sizeof(HostOut) + (size_t)c->W * sizeof(int32_t) + 255) & ~(size_t)255;
+    RTOD_HIP(hipHostMalloc((void **)&c->h_in, in_bytes, hipHostMallocMapped | hipHostMallocCoherent));
+    RTOD_HIP(hipHostMalloc((void **)&c->h_out, c->n_hout * out_bytes,
+                           hipHostMallocMapped | hipHostMallocCoherent));
+    c->pinned_bytes = (int64_t)(in_bytes + c->n_hout * out_bytes);
+    std::memset(c->h_in, 0, in_bytes);
+    std::memset((void *)c->h_out, 0, c->n_hout * out_bytes);
+    c->d.h_in = c->h_in;
+    c->d.h_out = c->h_out;
+    c->d.h_out_bytes = (int64_t)out_bytes;
+    r.h_out = c->h_out;
+    r.h_out_bytes = (int64_t)out_bytes;
+    RTOD_HIP(hipStreamSynchronize(c->stream));
+    return RTOD_OK;
+}
+
 rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts &opt = {})
 {
     RTOD_TRY(validate(cfg));
@@ -916,6 +1056,15 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
     }
     for (auto &e : c->ev)
         RTOD_HIP(hipEventCreate(&e));
+    {
+        std::string why;
+        if (runs_eligible(*cfg, opt, &why))
+            return runs_create(c, cfg);
+        if (cfg->index_kind == RTOD_INDEX_SLIDE_RUNS) {
+            set_error("the slide-run index does not apply: " + why);
+            return RTOD_ERR_UNSUPPORTED;
+        }
+    }
 
     c->sort_shape = opt.sort_shape >= 0 ? opt.sort_shape : pick_sort_shape(c->S, c->dim);
     c->CH = kSortShapes[c->sort_shape].nt * kSortShapes[c->sort_shape].it;
@@ -1113,6 +1262,10 @@ rtod_status create_impl(const rtod_config *cfg, rtod_ctx **out, const CreateOpts
     return RTOD_OK;
 }
 
+rtod_status runs_prime(rtod_ctx *c, const float *pts, hipMemcpyKind kind);
+rtod_status runs_run_slide(rtod_ctx *c, const float *src, double t_entry, double t_h2d,
+                           int32_t *outs, int32_t *n_out, rtod_times *times);
+
 rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n,
                        hipMemcpyKind kind = hipMemcpyHostToDevice)
 {
@@ -1121,6 +1274,8 @@ rtod_status prime_impl(rtod_ctx *c, const float *pts, int64_t n,
         return RTOD_ERR_INVALID_ARGUMENT;
     }
     RTOD_HIP(hipSetDevice(c->device));
+    if (c->runs)
+        return runs_prime(c, pts, kind);
     if (c->train_stream) { // the replay is about to be cleared
         RTOD_HIP(hipStreamSynchronize(c->train_stream));
         c->train_pending = false;
@@ -1496,8 +1651,11 @@ rtod_status slide_impl(rtod_ctx *c, const float *new_pts, int32_t *outs, int32_t
     RTOD_TRY(slide_checks(c, new_pts, outs, n_out));
     RTOD_HIP(hipSetDevice(c->device));
     const size_t nf = (size_t)c->S * c->dim;
-    std::memcpy(c->h_in + (size_t)(c->host_t % c->stage_slots) * nf, new_pts, nf * sizeof(float));
+    float *slot = c->h_in + (size_t)(c->host_t % c->stage_slots) * nf;
+    std::memcpy(slot, new_pts, nf * sizeof(float));
     c->dev_src = nullptr;
+    if (c->runs)
+        return runs_run_slide(c, slot, t_entry, now_ms(), outs, n_out, times);
     return run_slide(c, t_entry, now_ms(), outs, n_out, times);
 }
 
@@ -1514,7 +1672,8 @@ rtod_status slide_device_impl(rtod_ctx *c, const float *dev_pts, int32_t *outs, 
         return RTOD_ERR_INVALID_ARGUMENT;
     }
     c->dev_src = dev_pts;
-    rtod_status s = run_slide(c, t_entry, t_entry, outs, n_out, times);
+    rtod_status s = c->runs ? runs_run_slide(c, dev_pts, t_entry, t_entry, outs, n_out, times)
+                            : run_slide(c, t_entry, t_entry, outs, n_out, times);
     c->dev_src = nullptr;
     return s;
 }
@@ -1526,10 +1685,21 @@ rtod_status wait_slide(rtod_ctx *c, int64_t t)
 {
     const int64_t *stamp = &host_out_of(c, t)->slide;
     const int64_t want = t + 1;
+    // the stream is queried only once the wait has lasted kQueryMs (and then
+    // every kQueryMs): a query takes microseconds, and one that overlaps the
+    // stamp's arrival delays the slide by as much
+    constexpr double kQueryMs = 0.2;
+    double next_query = -1.0;
     for (uint64_t spin = 1;; ++spin) {
         if (__atomic_load_n(stamp, __ATOMIC_ACQUIRE) == want)
             return RTOD_OK;
-        if ((spin & 1023) == 0) {
+        if ((spin & 63) == 0) {
+            const double now = now_ms();
+            if (next_query < 0.0)
+                next_query = now + kQueryMs;
+            if (now < next_query)
+                continue;
+            next_query = now + kQueryMs;
             const hipError_t q = hipStreamQuery(c->stream);
             if (q == hipSuccess) {
                 if (__atomic_load_n(stamp, __ATOMIC_ACQUIRE) == want)
@@ -1693,9 +1863,14 @@ rtod_status slides_two_stage(rtod_ctx *c, const SlideSrc &src, int32_t n, int32_
     return st;
 }
 
+rtod_status runs_slides(rtod_ctx *c, const SlideSrc &src, int32_t n, int32_t *offs,
+                        int32_t *slots);
+
 rtod_status slides_impl(rtod_ctx *c, const SlideSrc &src, int32_t n, int32_t *offs,
                         int32_t *slots)
 {
+    if (c->runs)
+        return runs_slides(c, src, n, offs, slots);
     if (!c->lbvh && c->ride)
         return slides_two_stage(c, src, n, offs, slots);
     const int64_t t0 = c->host_t;
@@ -1768,6 +1943,231 @@ rtod_status slides_impl(rtod_ctx *c, const SlideSrc &src, int32_t n, int32_t *of
         c->primed = false;
     }
     return s;
+}
+
+// ---- slide-run index (1-D): host side of rtod_runs.h ----------------------
+
+void runs_launch(rtod_ctx *c, hipStream_t s, const RunJob &job)
+{
+    const int grid = job.fused ? 1 : job.nsort * c->rd.rps + job.ndet;
+    const bool sort = job.fused || job.nsort > 0, det = job.fused || job.ndet > 0;
+    hipLaunchKernelGGL(k_runs, dim3(grid), dim3(kRunNT), run_lds_bytes(c->W, sort, det), s, c->rd,
+                       job);
+}
+
+// The initial window: batch b (slots b*S + [0, S)) sorted into run b.
+rtod_status runs_prime(rtod_ctx *c, const float *pts, hipMemcpyKind kind)
+{
+    hipStream_t s = c->stream;
+    RTOD_HIP(hipMemcpyAsync(c->d_prime, pts, sizeof(float) * (size_t)c->W, kind, s));
+    RTOD_HIP(hipMemsetAsync(c->rd.err, 0, sizeof(int32_t), s));
+    RunJob job{};
+    job.nsort = c->m;
+    job.prime = 1;
+    job.sort_q0 = 0;
+    job.src[0] = c->d_prime;
+    runs_launch(c, s, job);
+    RTOD_HIP(hipGetLastError());
+    RTOD_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < c->n_hout; ++k)
+        host_out_of(c, k)->slide = 0;
+    c->host_t = 0;
+    c->primed = true;
+    c->staged = false;
+    c->phase = 0;
+    return RTOD_OK;
+}
+
+// Slide host_t one per call: with one run per slide a single launch (one
+// workgroup sorts the run, then detects the slide), else the sort and the
+// detect in two launches.  src: the new points (pinned staging or HBM).
+void runs_launch_slide(rtod_ctx *c, hipStream_t s, const float *src, bool timed)
+{
+    const int64_t t = c->host_t;
+    RunJob job{};
+    job.stamps = 1;
+    job.sort_q0 = c->m + t;
+    job.det_i0 = t;
+    job.src[0] = src;
+    if (timed)
+        (void)hipEventRecord(c->ev[0], s);
+    if (c->rd.rps == 1) {
+        job.fused = 1;
+        runs_launch(c, s, job);
+        if (timed)
+            (void)hipEventRecord(c->ev[1], s);
+        return;
+    }
+    RunJob a = job, b = job;
+    a.nsort = 1;
+    b.ndet = 1;
+    runs_launch(c, s, a);
+    if (timed)
+        (void)hipEventRecord(c->ev[1], s);
+    runs_launch(c, s, b);
+    if (timed)
+        (void)hipEventRecord(c->ev[2], s);
+}
+
+void runs_accumulate_call_times(rtod_ctx *c)
+{
+    float ms[2] = {0.f, 0.f};
+    (void)hipEventElapsedTime(&ms[0], c->ev[0], c->ev[1]);
+    if (c->rd.rps == 1) {
+        c->ksum[kKRunSlide] += ms[0];
+        c->kcnt[kKRunSlide]++;
+        return;
+    }
+    (void)hipEventElapsedTime(&ms[1], c->ev[1], c->ev[2]);
+    c->ksum[kKRunSort] += ms[0];
+    c->kcnt[kKRunSort]++;
+    c->ksum[kKRunDetect] += ms[1];
+    c->kcnt[kKRunDetect]++;
+}
+
+// run_slide on the slide-run index
+rtod_status runs_run_slide(rtod_ctx *c, const float *src, double t_entry, double t_h2d,
+                           int32_t *outs, int32_t *n_out, rtod_times *times)
+{
+    const bool timed = (c->cfg.flags & RTOD_FLAG_PHASE_TIMES) != 0;
+    runs_launch_slide(c, c->stream, src, timed);
+    RTOD_HIP(hipGetLastError());
+    const rtod_status ws = wait_slide(c, c->host_t);
+    if (ws != RTOD_OK) {
+        (void)hipStreamSynchronize(c->stream);
+        c->primed = false;
+        return ws;
+    }
+    if (timed) {
+        RTOD_HIP(hipStreamSynchronize(c->stream));
+        runs_accumulate_call_times(c);
+    }
+    double phase[4] = {0, 0, 0, 0};
+    stamp_phases(c, host_out_of(c, c->host_t), phase);
+    const double t_col = now_ms();
+    RTOD_TRY(collect(c, outs, n_out));
+    const double t_end = now_ms();
+    if (times) {
+        times->copy_h2d = t_h2d - t_entry;
+        times->prepare_cell = phase[0];
+        times->build_index = phase[1];
+        times->detect_outlier = phase[2];
+        times->copy_d2h = phase[3] + (t_end - t_col);
+        times->total = t_end - t_entry;
+        times->launch = times->total - times->copy_h2d - times->prepare_cell -
+                        times->build_index - times->detect_outlier - times->copy_d2h;
+    }
+    return RTOD_OK;
+}
+
+// n consecutive slides, pipelined on the context's stream: after a
+// prologue launch sorting slides [0, B), launch j detects slides [jB, jB+B)
+// and sorts slides [jB+B, jB+2B) (B = kRunBatch), each slide's points
+// staged into pinned memory just before the launch that sorts it.  At most
+// two launches are in flight: launch j is enqueued once every slide before
+// (j-1)B is collected, so the staging slots, host-out buffers and runs that
+// launch j rewrites are no longer read.  Results equal those of n
+// rtod_slide calls.
+rtod_status runs_slides(rtod_ctx *c, const SlideSrc &src, int32_t n, int32_t *offs,
+                        int32_t *slots)
+{
+    constexpr int B = kRunBatch;
+    const int64_t t0 = c->host_t;
+    const bool timed = (c->cfg.flags & RTOD_FLAG_PHASE_TIMES) != 0;
+    hipStream_t s = c->stream;
+    int64_t used = 0;
+    int32_t done = 0;
+    offs[0] = 0;
+    auto collect_next = [&]() -> rtod_status {
+        RTOD_TRY(wait_slide(c, c->host_t));
+        int32_t k = 0;
+        RTOD_TRY(collect(c, slots + used, &k));
+        used += k;
+        offs[++done] = (int32_t)used;
+        return RTOD_OK;
+    };
+    auto sort_job = [&](RunJob &job, int32_t first, int32_t count) {
+        job.nsort = count;
+        job.sort_q0 = c->m + t0 + first;
+        for (int32_t k = 0; k < count; ++k) {
+            stage_slide(c, src, t0, first + k);
+            job.src[k] = slide_src(c, src, t0, first + k);
+        }
+    };
+    rtod_status st = RTOD_OK;
+    {
+        RunJob pj{};
+        sort_job(pj, 0, std::min(B, n));
+        runs_launch(c, s, pj);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            set_error(std::string("launch: ") + hipGetErrorString(e));
+            st = RTOD_ERR_HIP;
+        }
+    }
+    for (int32_t j0 = 0; j0 < n && st == RTOD_OK; j0 += B) {
+        while (st == RTOD_OK && done < j0 - B)
+            st = collect_next();
+        if (st != RTOD_OK)
+            break;
+        RunJob job{};
+        job.ndet = std::min(B, n - j0);
+        job.det_i0 = t0 + j0;
+        const int32_t s0 = j0 + B;
+        if (s0 < n)
+            sort_job(job, s0, std::min(B, n - s0));
+        if (timed)
+            (void)hipEventRecord(c->ev[2], s);
+        runs_launch(c, s, job);
+        if (timed)
+            (void)hipEventRecord(c->ev[3], s);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            set_error(std::string("launch: ") + hipGetErrorString(e));
+            st = RTOD_ERR_HIP;
+        } else if (timed) {
+            const hipError_t q = hipStreamSynchronize(s);
+            if (q != hipSuccess) {
+                set_error(std::string("HIP error: ") + hipGetErrorString(q));
+                st = RTOD_ERR_HIP;
+            } else {
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
+                c->ksum[kKRunStage] += ms;
+                c->kcnt[kKRunStage]++;
+            }
+        }
+    }
+    while (st == RTOD_OK && done < n)
+        st = collect_next();
+    if (st != RTOD_OK) {
+        // runs of slides that were not collected may have been sorted: the
+        // detector has to be primed again
+        (void)hipStreamSynchronize(s);
+        c->primed = false;
+    }
+    return st;
+}
+
+// phase API on the slide-run index: prepare_cell sorts the new run,
+// detect_outlier detects the slide (build_index has nothing to do: the
+// sorted runs are the index)
+rtod_status runs_phase(rtod_ctx *c, bool detect)
+{
+    RunJob job{};
+    job.stamps = 1;
+    if (detect) {
+        job.ndet = 1;
+        job.det_i0 = c->host_t;
+    } else {
+        job.nsort = 1;
+        job.sort_q0 = c->m + c->host_t;
+        job.src[0] = c->h_in + (size_t)(c->host_t % c->stage_slots) * c->S;
+    }
+    runs_launch(c, c->stream, job);
+    RTOD_HIP(hipGetLastError());
+    RTOD_HIP(hipStreamSynchronize(c->stream));
+    return RTOD_OK;
 }
 
 } // namespace
@@ -1944,6 +2344,11 @@ RTOD_API rtod_status rtod_prepare_cell(rtod_ctx *c)
         return RTOD_ERR_STATE;
     }
     RTOD_HIP(hipSetDevice(c->device));
+    if (c->runs) {
+        RTOD_TRY(runs_phase(c, false));
+        c->phase = 2;
+        return RTOD_OK;
+    }
     RTOD_TRY(guard_trainer(c, c->host_t, c->stream));
     decide_learn(c, c->host_t);
     const Step st = make_step(c);
@@ -1991,6 +2396,11 @@ RTOD_API rtod_status rtod_detect_outlier(rtod_ctx *c)
         return RTOD_ERR_STATE;
     }
     RTOD_HIP(hipSetDevice(c->device));
+    if (c->runs) {
+        RTOD_TRY(runs_phase(c, true));
+        c->phase = 4;
+        return RTOD_OK;
+    }
     if (c->lbvh)
         launch_lbvh_detect(c, c->stream, make_step(c));
     else
@@ -2037,7 +2447,8 @@ RTOD_API rtod_status rtod_get_stats(const rtod_ctx *c, rtod_stats *o)
     o->outliers_total = c->out_total;
     o->device_bytes = c->dev_bytes;
     o->host_pinned_bytes = c->pinned_bytes;
-    o->index_kind = c->lbvh ? RTOD_INDEX_LBVH : RTOD_INDEX_CELL_STENCIL;
+    o->index_kind = c->runs ? RTOD_INDEX_SLIDE_RUNS
+                            : (c->lbvh ? RTOD_INDEX_LBVH : RTOD_INDEX_CELL_STENCIL);
     o->sched_kind = c->used_pred_last ? RTOD_SCHED_LEARNED : RTOD_SCHED_PLAIN;
     o->predictor_misses = c->misses_total;
     o->predictor_points = c->pred_points;
@@ -2718,7 +3129,7 @@ RTOD_API rtod_status rtod_batch_create(const rtod_config *cfgs, int32_t n, int32
     }
     for (int i = 0; i < n; ++i) {
         RTOD_TRY(validate(&cfgs[i]));
-        if (cfgs[i].index_kind == RTOD_INDEX_LBVH) {
+        if (cfgs[i].index_kind == RTOD_INDEX_LBVH || cfgs[i].index_kind == RTOD_INDEX_SLIDE_RUNS) {
             set_error("batched streams run on the cell-stencil index");
             return RTOD_ERR_UNSUPPORTED;
         }

@@ -1,0 +1,811 @@
+// rtod_runs.h -- the slide-run index of 1-D streams.
+//
+// Each slide's S points are bucketed once, on arrival, into a "run": a
+// counting sort by sub-cell key (the cell of side c = R (1 - 1e-5) split
+// into 2^s sub-cells, s chosen per run so the run's key range fits
+// kRunBMax buckets).  The window is the m most recent runs; nothing is
+// merged, so a slide costs O(S) work plus the range counts of its
+// candidates, where the cell-sorted window of rtod_plain.h rewrites all W
+// points every slide.
+//
+// A run holds, in HBM: its points in bucket order (coordinates and ring
+// slots), the start of every bucket (an exclusive prefix over the run's
+// buckets), and its cell table (each cell with points, ascending, and the
+// index of its first point).  A run whose cell range exceeds kRunBMax (data
+// far apart, or clamped/non-finite values) is sorted by cell with the LDS
+// radix sort instead and searched by cell ("search" mode, s = 0).
+//
+// A slide's detect (one workgroup):
+//   1. the window's cell counts: the runs' cell tables added into an LDS
+//      histogram over the window's cell range (or, when the range exceeds
+//      kRunHCap or a run is in search mode, summed per cell by searching
+//      every run's cell table);
+//   2. candidates: the points of every cell with at most K points in the
+//      window, or whose cell is clamped (the reference's rule `num > K`
+//      fast-accepts a cell, optixScan.cpp:987-993);
+//   3. each candidate's exact count over each run: with x the candidate and
+//      f(v) = floor(v 2^s / c) its run's sub-cell key, every point whose key
+//      lies in [f(x - R(1-1e-5)) + 1, f(x + R(1-1e-5)) - 1] passes the FP32
+//      test d2 <= fl(R*R) and every point below f(x - R(1+1e-5)) or above
+//      f(x + R(1+1e-5)) fails it (the margins exceed the rounding of the
+//      double keys and of the FP32 test for every unclamped cell), so the
+//      count is a difference of two bucket starts plus the exact FP32 test
+//      of the points in the few boundary buckets;
+//   4. counts <= K set bits by ring slot in an LDS bitmap, listed ascending
+//      straight into pinned host memory with the slide's counters, then the
+//      slide's stamp after a system-scope release.
+//
+// Reference counterparts: prepare_c_non_points_queue (optixScan.cpp:
+// 967-1017: cell map, dense cells, the sparse cells' points), rebuild_gas /
+// kGenAABB + launch + __intersection__cube (optixScan.cpp:378-426, 727-750,
+// optixScan.cu:158-199: the candidates' neighbour counts), D2H + select
+// (optixScan.cpp:1083-1101).
+#pragma once
+
+#include <climits>
+
+#include "rtod_plain.h"
+
+namespace rtod {
+
+constexpr int kRunNT = 1024;                 // threads of every runs workgroup
+constexpr int kRunIT = 8;                    // points per thread in a run's sort
+constexpr int kRunMaxLen = kRunNT * kRunIT;  // points per run (a slide: 1 or 2 runs)
+constexpr int kRunBMax = 4096;               // buckets of a direct-mode run
+constexpr int kRunSMax = 8;                  // at most 256 sub-cells per cell
+constexpr int kRunMaxRuns = 256;             // runs per window (m x runs per slide)
+constexpr int kRunHCap = 8192;               // cells of the window histogram in LDS
+constexpr int kRunCandCap = 2048;            // candidates counted per round
+constexpr int kRunECap = 2048;               // cell-table entries staged in LDS per round
+constexpr int kRunBatch = 8;                 // slides per pipelined launch
+constexpr int kRunHostOuts = 16;             // host-out buffers of a runs context
+// Pinned staging slots of a runs context: a pipelined run stages the slides
+// launch j sorts, [jB + B, jB + 2B), once the slides before (j-1)B are
+// collected; those slots were last read by the sorts of launch j - 2, whose
+// slides' completion only that collection shows when 3B slots rotate (with
+// 2B, launch 1 would restage the prologue's slots, which no collection
+// covers)
+constexpr int kRunStageSlots = 3 * kRunBatch;
+constexpr int kRunMaxW = 1 << 18;            // the outlier bitmap of W bits lives in LDS
+
+constexpr int kErrRunBucket = 32;            // a point's bucket fell outside its run's range
+
+enum { kRunDirect = 0, kRunSearch = 1 };
+
+// One sorted run (written by its sort, read by the detects of the m slides
+// whose windows hold it).
+struct RunMeta {
+    int32_t n;          // points
+    int32_t mode;       // kRunDirect / kRunSearch
+    int32_t s;          // direct: 2^s sub-cells per cell (search: 0)
+    int32_t nb;         // direct: buckets, (c_hi - c_lo + 1) << s
+    int32_t c_lo, c_hi; // cell range (clamped cell coordinates)
+    int32_t ne;         // cells holding points (cell-table entries)
+    int32_t pad;
+};
+
+// Device state of a runs context.  Run (q, j): part j of run id q, where
+// the initial window's batch b is q = b and 0-based slide i is q = m + i;
+// it lives at ring entry (q mod NE) * rps + j and holds ring slots
+// (q mod m) * S + j * RL + [0, n).
+struct RunsDev {
+    int W, S, m, K;
+    int rps;            // runs per slide (ceil(S / kRunMaxLen))
+    int RL;             // points per run (ceil(S / rps))
+    int NE;             // run-ring entries (each rps runs)
+    int n_hout;         // host-out buffers (slide i uses i mod n_hout)
+    float r2;           // fl(R * R)
+    int pad0;
+    double inv_c;       // 1 / cell side
+    double R_in;        // R (1 - 1e-5): keys strictly inside are neighbours
+    double R_out;       // R (1 + 1e-5): keys strictly outside are not
+    float *rx;          // [NE*rps][RL] coordinates in bucket order
+    int32_t *rslot;     // [NE*rps][RL] their ring slots
+    int32_t *rpre;      // [NE*rps][kRunBMax + 1] bucket starts (direct mode)
+    int32_t *rcell;     // [NE*rps][RL] cells holding points, ascending
+    int32_t *rcs;       // [NE*rps][RL + 1] index of each cell's first point (then n)
+    RunMeta *meta;      // [NE*rps]
+    int32_t *err;       // error flags (kErrRunBucket); reported and cleared by a detect
+    uint64_t *tstamp;   // [kTsCount] stamps of a slide run one per call
+    HostOut *h_out;     // n_hout buffers of h_out_bytes each (pinned host memory)
+    int64_t h_out_bytes;
+};
+
+// The work of one k_runs launch: the sorts of nsort slides (rps workgroups
+// each) then the detects of ndet slides (one workgroup each); fused: one
+// workgroup sorts slide det_i0's run and then detects the slide.
+struct RunJob {
+    int nsort;
+    int ndet;
+    int prime;              // sort the initial window: batch k from src[0] + k*S
+    int fused;
+    int stamps;             // one slide per call: stamp its phases
+    int pad;
+    int64_t sort_q0;        // run id of the first slide sorted
+    int64_t det_i0;         // first slide detected (0-based)
+    const float *src[kRunBatch]; // each sorted slide's S new points
+};
+
+__device__ __forceinline__ HostOut *run_host_out(const RunsDev &d, int64_t i)
+{
+    return reinterpret_cast<HostOut *>(reinterpret_cast<char *>(d.h_out) +
+                                       (size_t)(i % d.n_hout) * (size_t)d.h_out_bytes);
+}
+
+// Sub-cell key: floor(v * 2^s / c) in double, clamped to [-2^(30+s),
+// 2^(30+s) - 1].  inv_s = 2^s / c exactly (a power-of-two multiple of 1/c),
+// so key >> s equals cell_coord<1> (the cell key of every other kernel and
+// of the oracle), clamping included, and NaN takes the lowest key.
+__device__ __forceinline__ int64_t sub_coord(double v, double inv_s, int s)
+{
+    double f = floor(v * inv_s);
+    const double lo = -(double)(1ll << (30 + s));
+    const double hi = (double)((1ll << (30 + s)) - 1);
+    if (!(f > lo))
+        f = lo;
+    else if (!(f < hi))
+        f = hi;
+    return (int64_t)f;
+}
+
+// a cell whose diameter bound holds (strictly inside the clamp range)
+__device__ __forceinline__ bool run_cell_guaranteed(int32_t c)
+{
+    return c > -(1 << 30) && c < (1 << 30) - 1;
+}
+
+// In-place exclusive scan of a[0, n) in LDS (n <= kRunNT * 8), a[n] = total.
+// a is 16-byte aligned; up to 4 * kRunNT entries each thread reads and
+// writes its four with one 128-bit LDS access.
+__device__ __forceinline__ void run_lds_scan(int32_t *a, int n, int32_t *s_tmp)
+{
+    if (n <= 4 * kRunNT) {
+        int4 *a4 = reinterpret_cast<int4 *>(a);
+        const int b0 = threadIdx.x * 4;
+        int4 v = make_int4(0, 0, 0, 0);
+        if (b0 < n) {
+            v = a4[threadIdx.x];
+            if (b0 + 1 >= n)
+                v.y = 0;
+            if (b0 + 2 >= n)
+                v.z = 0;
+            if (b0 + 3 >= n)
+                v.w = 0;
+        }
+        int total;
+        const int run = block_excl_scan<kRunNT>(v.x + v.y + v.z + v.w, s_tmp, total);
+        if (b0 < n) {
+            const int4 o = make_int4(run, run + v.x, run + v.x + v.y, run + v.x + v.y + v.z);
+            if (b0 + 3 < n) {
+                a4[threadIdx.x] = o;
+            } else {
+                a[b0] = o.x;
+                if (b0 + 1 < n)
+                    a[b0 + 1] = o.y;
+                if (b0 + 2 < n)
+                    a[b0 + 2] = o.z;
+            }
+        }
+        if (threadIdx.x == 0)
+            a[n] = total;
+        __syncthreads();
+        return;
+    }
+    const int per = (n + kRunNT - 1) / kRunNT;
+    const int b0 = threadIdx.x * per;
+    int loc = 0;
+    for (int j = 0; j < per; ++j)
+        if (b0 + j < n)
+            loc += a[b0 + j];
+    int total;
+    int run = block_excl_scan<kRunNT>(loc, s_tmp, total);
+    for (int j = 0; j < per; ++j)
+        if (b0 + j < n) {
+            const int v = a[b0 + j];
+            a[b0 + j] = run;
+            run += v;
+        }
+    if (threadIdx.x == 0)
+        a[n] = total;
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Sort of one run: S points (or RL of them) from the slide's source (pinned
+// staging, or HBM) into ring entry e.
+// ---------------------------------------------------------------------------
+struct RunSortLds {
+    float x[kRunMaxLen]; // arrival order
+    union {
+        struct {
+            alignas(16) int32_t hist[kRunBMax + 4]; // bucket counts, then starts
+            float sx[kRunMaxLen];       // coordinates in bucket order
+            uint16_t sidx[kRunMaxLen];  // their arrival index
+        } dir;
+        struct {
+            uint64_t key[kRunMaxLen];   // (cell - c_lo) << 13 | arrival index
+            int32_t hist[16 * (kRunNT / kWave)];
+        } sea;
+    } u;
+    float red[3][kRunNT / kWave];
+    int32_t scan[kRunNT / kWave];
+};
+
+__device__ __forceinline__ void run_sort_body(const RunsDev &d, int64_t q, int part,
+                                              const float *src, int stamps, RunSortLds &L)
+{
+    constexpr int NT = kRunNT, IT = kRunIT, NW = NT / kWave;
+    const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
+    const int e = (int)(q % d.NE) * d.rps + part;
+    const int j0 = part * d.RL;
+    const int n = min(d.RL, d.S - j0);
+    const int slot0 = (int)(q % d.m) * d.S + j0;
+
+    // the run's coordinates into LDS: one round of loads
+    {
+        const float *g = src + j0;
+        if ((((uintptr_t)g) & 15) == 0) {
+            const int n4 = n >> 2;
+            copy_to_lds<(kRunMaxLen / 4 + NT - 1) / NT>(reinterpret_cast<float4 *>(L.x),
+                                                       reinterpret_cast<const float4 *>(g), n4);
+            for (int k = (n4 << 2) + tid; k < n; k += NT)
+                L.x[k] = g[k];
+        } else {
+            copy_to_lds<IT>(L.x, g, n);
+        }
+    }
+    __syncthreads();
+
+    // the run's cell range: cell_coord is monotone in x and puts NaN in the
+    // lowest cell, so it is [cell(min x), cell(max x)] over the non-NaN
+    // points, from the lowest cell if a NaN is present
+    float xmn = __builtin_inff(), xmx = -__builtin_inff(), nan = 0.f;
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int l = tid + i * NT;
+        if (l < n) {
+            const float v = L.x[l];
+            if (v != v) {
+                nan = 1.f;
+            } else {
+                xmn = fminf(xmn, v);
+                xmx = fmaxf(xmx, v);
+            }
+        }
+    }
+    for (int o = 1; o < kWave; o <<= 1) {
+        xmn = fminf(xmn, __shfl_xor(xmn, o, kWave));
+        xmx = fmaxf(xmx, __shfl_xor(xmx, o, kWave));
+        nan = fmaxf(nan, __shfl_xor(nan, o, kWave));
+    }
+    if (lane == 0) {
+        L.red[0][wid] = xmn;
+        L.red[1][wid] = xmx;
+        L.red[2][wid] = nan;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        xmn = fminf(xmn, L.red[0][w]);
+        xmx = fmaxf(xmx, L.red[1][w]);
+        nan = fmaxf(nan, L.red[2][w]);
+    }
+    int c_lo = cell_coord<1>((double)xmn, d.inv_c);
+    int c_hi = cell_coord<1>((double)xmx, d.inv_c);
+    if (nan != 0.f)
+        c_lo = -(1 << 30);
+    if (c_hi < c_lo) // every point NaN (min x = +inf)
+        c_hi = c_lo;
+    const int64_t crange = (int64_t)c_hi - (int64_t)c_lo + 1;
+    float *ox = d.rx + (size_t)e * d.RL;
+    int32_t *os = d.rslot + (size_t)e * d.RL;
+    int32_t *oc = d.rcell + (size_t)e * d.RL;
+    int32_t *ocs = d.rcs + (size_t)e * (d.RL + 1);
+
+    if (crange <= kRunBMax) {
+        // ---- direct mode: counting sort by sub-cell bucket -----------------
+        int s = 0;
+        while (s < kRunSMax && (crange << (s + 1)) <= kRunBMax)
+            ++s;
+        const int nb = (int)(crange << s);
+        const double inv_s = d.inv_c * (double)(1 << s);
+        const int64_t base = (int64_t)c_lo * (1ll << s);
+        int32_t *hist = L.u.dir.hist;
+        for (int b = tid; b <= nb; b += NT)
+            hist[b] = 0;
+        __syncthreads();
+        int bk[IT], rk[IT];
+        int bad = 0;
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const int l = tid + i * NT;
+            bk[i] = 0;
+            rk[i] = 0;
+            if (l < n) {
+                int64_t f = sub_coord((double)L.x[l], inv_s, s) - base;
+                if (f < 0 || f >= nb) { // cannot happen: key >> s lies in [c_lo, c_hi]
+                    bad = 1;
+                    f = f < 0 ? 0 : nb - 1;
+                }
+                bk[i] = (int)f;
+                rk[i] = atomicAdd(&hist[f], 1);
+            }
+        }
+        if (bad)
+            atomicOr(d.err, kErrRunBucket);
+        __syncthreads();
+        run_lds_scan(hist, nb, L.scan); // hist[b]: first position of bucket b; hist[nb] = n
+        if (stamps && tid == 0)
+            d.tstamp[kTsMerge] = wall_clock();
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const int l = tid + i * NT;
+            if (l < n) {
+                const int pos = hist[bk[i]] + rk[i];
+                L.u.dir.sx[pos] = L.x[l];
+                L.u.dir.sidx[pos] = (uint16_t)l;
+            }
+        }
+        __syncthreads();
+        for (int p = tid; p < n; p += NT) {
+            ox[p] = L.u.dir.sx[p];
+            os[p] = slot0 + (int)L.u.dir.sidx[p];
+        }
+        int32_t *op = d.rpre + (size_t)e * (kRunBMax + 1);
+        for (int b = tid; b <= nb; b += NT)
+            op[b] = hist[b];
+        // the cell table: cells of [c_lo, c_hi] holding points, ascending
+        const int nc = (int)crange;
+        const int per = (nc + NT - 1) / NT;
+        const int k0 = tid * per;
+        int mine = 0;
+        for (int j = 0; j < per; ++j) {
+            const int k = k0 + j;
+            if (k < nc && hist[(k + 1) << s] > hist[k << s])
+                ++mine;
+        }
+        int ne;
+        int o = block_excl_scan<NT>(mine, L.scan, ne);
+        for (int j = 0; j < per; ++j) {
+            const int k = k0 + j;
+            if (k < nc && hist[(k + 1) << s] > hist[k << s]) {
+                oc[o] = c_lo + k;
+                ocs[o] = hist[k << s];
+                ++o;
+            }
+        }
+        if (tid == 0) {
+            ocs[ne] = n;
+            d.meta[e] = RunMeta{n, kRunDirect, s, nb, c_lo, c_hi, ne, 0};
+        }
+        return;
+    }
+
+    // ---- search mode: LDS radix sort by cell, cell table by run length ----
+    uint64_t k[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int l = wid * kWave * IT + i * kWave + lane;
+        k[i] = ~0ull;
+        if (l < n) {
+            const int c = cell_coord<1>((double)L.x[l], d.inv_c);
+            k[i] = ((uint64_t)((int64_t)c - (int64_t)c_lo) << 13) | (uint64_t)l;
+        }
+        L.u.sea.key[l] = k[i];
+    }
+    __syncthreads();
+    block_radix_sort<NT, IT>(k, 13, bit_length((uint64_t)(crange - 1)), L.u.sea.key,
+                             L.u.sea.hist);
+    if (stamps && tid == 0)
+        d.tstamp[kTsMerge] = wall_clock();
+    for (int p = tid; p < n; p += NT) {
+        const int l = (int)(L.u.sea.key[p] & 8191ull);
+        ox[p] = L.x[l];
+        os[p] = slot0 + l;
+    }
+    // a cell starts at p when its cell differs from p - 1's
+    const int per = (n + NT - 1) / NT;
+    const int p0 = tid * per;
+    int mine = 0;
+    for (int j = 0; j < per; ++j) {
+        const int p = p0 + j;
+        if (p < n && (p == 0 || (L.u.sea.key[p] >> 13) != (L.u.sea.key[p - 1] >> 13)))
+            ++mine;
+    }
+    int ne;
+    int o = block_excl_scan<NT>(mine, L.scan, ne);
+    for (int j = 0; j < per; ++j) {
+        const int p = p0 + j;
+        if (p < n && (p == 0 || (L.u.sea.key[p] >> 13) != (L.u.sea.key[p - 1] >> 13))) {
+            oc[o] = (int32_t)((int64_t)c_lo + (int64_t)(L.u.sea.key[p] >> 13));
+            ocs[o] = p;
+            ++o;
+        }
+    }
+    if (tid == 0) {
+        ocs[ne] = n;
+        d.meta[e] = RunMeta{n, kRunSearch, 0, 0, c_lo, c_hi, ne, 0};
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Detect of one slide over the m runs of its window.
+// ---------------------------------------------------------------------------
+struct RunDetLds {
+    RunMeta meta[kRunMaxRuns];
+    int32_t gidx[kRunMaxRuns];         // ring entry of window run r
+    int32_t ebase[kRunMaxRuns + 4];    // cell-table entries before run r
+    alignas(16) int32_t hist[kRunHCap]; // window cell counts over [cmin, cmax]
+    int32_t ecell[kRunECap];           // the round's cell-table entries: cell,
+    uint32_t epos[kRunECap];           //   first point | points << 16,
+    uint8_t erun[kRunECap];            //   window run
+    int32_t rent[kRunECap];            // candidate ranges: staged entry,
+    alignas(16) int32_t rpos[kRunECap + 4]; //   candidates before it
+    float cx[kRunCandCap];             // the round's candidates
+    int32_t cslot[kRunCandCap];
+    int32_t ccnt[kRunCandCap];
+    int32_t scan[kRunNT / kWave];
+    int32_t red[3][kRunNT / kWave];
+    int32_t nrange;
+    int32_t pad[3];
+    unsigned long long tests, hits;
+    uint32_t bitmap[4];                // ceil(W / 32) words follow (sized at launch)
+};
+
+__host__ __device__ constexpr size_t run_det_lds_bytes(int W)
+{
+    return sizeof(RunDetLds) + (size_t)((W + 31) / 32) * sizeof(uint32_t);
+}
+
+static_assert(sizeof(RunSortLds) <= 160 * 1024 && sizeof(RunDetLds) + kRunMaxW / 8 <= 160 * 1024,
+              "a runs workgroup's LDS fits the 160 KB of a CU");
+
+__host__ __device__ constexpr size_t run_lds_bytes(int W, bool sort, bool det)
+{
+    const size_t a = sort ? sizeof(RunSortLds) : 0;
+    const size_t b = det ? run_det_lds_bytes(W) : 0;
+    return a > b ? a : b;
+}
+
+// the window run holding cell-table entry e: the last r with ebase[r] <= e
+__device__ __forceinline__ int run_of_entry(const int32_t *ebase, int nr, int e)
+{
+    int lo = 0, hi = nr - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ebase[mid] <= e)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// first point of run g whose key (at the run's resolution) is >= f
+__device__ __forceinline__ int run_lb(const RunsDev &d, const RunMeta &M, int g, int64_t f)
+{
+    if (M.mode == kRunDirect) {
+        int64_t b = f - (int64_t)M.c_lo * (1ll << M.s);
+        b = b < 0 ? 0 : (b > M.nb ? M.nb : b);
+        return d.rpre[(size_t)g * (kRunBMax + 1) + (size_t)b];
+    }
+    const int32_t *cells = d.rcell + (size_t)g * d.RL;
+    int lo = 0, hi = M.ne;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((int64_t)cells[mid] < f)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo == M.ne ? M.n : d.rcs[(size_t)g * (d.RL + 1) + lo];
+}
+
+// points of run g in cell c (search of its cell table)
+__device__ __forceinline__ int run_cell_count(const RunsDev &d, const RunMeta &M, int g, int32_t c)
+{
+    const int32_t *cells = d.rcell + (size_t)g * d.RL;
+    int lo = 0, hi = M.ne;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cells[mid] < c)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    if (lo == M.ne || cells[lo] != c)
+        return 0;
+    const int32_t *cs = d.rcs + (size_t)g * (d.RL + 1);
+    return cs[lo + 1] - cs[lo];
+}
+
+// exact count of run g's points q with FP32 (xp - xq)^2 <= r2 (xp finite)
+__device__ __forceinline__ int run_count(const RunsDev &d, const RunMeta &M, int g, float xp,
+                                         unsigned long long &tests)
+{
+    const int s = M.mode == kRunDirect ? M.s : 0;
+    const double inv_s = d.inv_c * (double)(1 << s);
+    const double xd = (double)xp;
+    const int64_t f0 = sub_coord(xd - d.R_out, inv_s, s);     // below: out
+    const int64_t f1 = sub_coord(xd - d.R_in, inv_s, s) + 1;  // [f1, f2): in
+    const int64_t f2 = sub_coord(xd + d.R_in, inv_s, s);
+    const int64_t f3 = sub_coord(xd + d.R_out, inv_s, s) + 1; // from f3: out
+    const int i0 = run_lb(d, M, g, f0);
+    int i1 = run_lb(d, M, g, f1);
+    int i2 = run_lb(d, M, g, f2);
+    int i3 = run_lb(d, M, g, f3);
+    i1 = max(i1, i0);
+    i2 = max(i2, i1);
+    i3 = max(i3, i2);
+    int cnt = i2 - i1;
+    const float *xs = d.rx + (size_t)g * d.RL;
+    const float r2 = d.r2;
+    for (int k = i0; k < i1; ++k) {
+        const float dx = xp - xs[k];
+        cnt += (dx * dx <= r2) ? 1 : 0;
+    }
+    for (int k = i2; k < i3; ++k) {
+        const float dx = xp - xs[k];
+        cnt += (dx * dx <= r2) ? 1 : 0;
+    }
+    tests += (unsigned long long)(i1 - i0 + i3 - i2);
+    return cnt;
+}
+
+// stage cell-table entries [e0, e1) of the window (e1 - e0 <= kRunECap)
+__device__ __forceinline__ void run_stage_entries(const RunsDev &d, RunDetLds &L, int nr, int e0,
+                                                  int e1)
+{
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += kRunNT) {
+        const int r = run_of_entry(L.ebase, nr, e);
+        const int g = L.gidx[r];
+        const int k = e - L.ebase[r];
+        const int32_t c = d.rcell[(size_t)g * d.RL + k];
+        const int32_t *cs = d.rcs + (size_t)g * (d.RL + 1) + k;
+        const int a = cs[0], b = cs[1];
+        L.ecell[e - e0] = c;
+        L.epos[e - e0] = (uint32_t)a | ((uint32_t)(b - a) << 16);
+        L.erun[e - e0] = (uint8_t)r;
+    }
+}
+
+__device__ __forceinline__ void run_detect_body(const RunsDev &d, int64_t i, int stamps,
+                                                RunDetLds &L)
+{
+    constexpr int NT = kRunNT, NW = NT / kWave;
+    const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
+    const int nr = d.m * d.rps;
+    if (stamps && tid == 0)
+        d.tstamp[kTsClassify] = wall_clock();
+    const int words = (d.W + 31) >> 5;
+    for (int w = tid; w < words; w += NT)
+        L.bitmap[w] = 0;
+    if (tid == 0) {
+        L.tests = 0;
+        L.hits = 0;
+    }
+
+    // the window's runs: q = i + 1 .. i + m
+    int cl = INT_MAX, ch = INT_MIN, srch = 0, ne = 0;
+    if (tid < nr) {
+        const int64_t q = i + 1 + tid / d.rps;
+        const int g = (int)(q % d.NE) * d.rps + tid % d.rps;
+        const RunMeta M = d.meta[g];
+        L.meta[tid] = M;
+        L.gidx[tid] = g;
+        ne = M.ne;
+        cl = M.c_lo;
+        ch = M.c_hi;
+        srch = M.mode == kRunSearch ? 1 : 0;
+    }
+    int E;
+    const int pre = block_excl_scan<NT>(ne, L.scan, E);
+    if (tid < nr)
+        L.ebase[tid] = pre;
+    for (int o = 1; o < kWave; o <<= 1) {
+        cl = min(cl, __shfl_xor(cl, o, kWave));
+        ch = max(ch, __shfl_xor(ch, o, kWave));
+        srch |= __shfl_xor(srch, o, kWave);
+    }
+    if (lane == 0) {
+        L.red[0][wid] = cl;
+        L.red[1][wid] = ch;
+        L.red[2][wid] = srch;
+    }
+    __syncthreads();
+    int cmin = INT_MAX, cmax = INT_MIN, any_search = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        cmin = min(cmin, L.red[0][w]);
+        cmax = max(cmax, L.red[1][w]);
+        any_search |= L.red[2][w];
+    }
+    const bool fast = !any_search && (int64_t)cmax - (int64_t)cmin + 1 <= kRunHCap;
+    const bool one_round = E <= kRunECap;
+
+    // 1. window cell counts (fast path: LDS histogram of the staged entries)
+    if (fast) {
+        const int cw = cmax - cmin + 1;
+        for (int c = tid; c < cw; c += NT)
+            L.hist[c] = 0;
+        for (int e0 = 0; e0 < E; e0 += kRunECap) {
+            const int e1 = min(E, e0 + kRunECap);
+            run_stage_entries(d, L, nr, e0, e1);
+            __syncthreads();
+            for (int e = tid; e < e1 - e0; e += NT)
+                atomicAdd(&L.hist[L.ecell[e] - cmin], (int)(L.epos[e] >> 16));
+            __syncthreads();
+        }
+    }
+
+    // 2-4. candidates by rounds of cell-table entries, counted by rounds
+    unsigned long long tests = 0, hits = 0;
+    int ncand = 0, nranges = 0;
+    for (int e0 = 0; e0 < E; e0 += kRunECap) {
+        const int e1 = min(E, e0 + kRunECap);
+        if (!(fast && one_round)) { // else staged by the histogram pass
+            run_stage_entries(d, L, nr, e0, e1);
+        }
+        if (tid == 0)
+            L.nrange = 0;
+        __syncthreads();
+        for (int e = tid; e < e1 - e0; e += NT) {
+            const int32_t c = L.ecell[e];
+            bool sparse = !run_cell_guaranteed(c);
+            if (!sparse) {
+                int total = 0;
+                if (fast) {
+                    total = L.hist[c - cmin];
+                } else {
+                    for (int r2 = 0; r2 < nr && total <= d.K; ++r2)
+                        total += run_cell_count(d, L.meta[r2], L.gidx[r2], c);
+                }
+                sparse = total <= d.K;
+            }
+            if (sparse) {
+                const int slot = atomicAdd(&L.nrange, 1);
+                L.rent[slot] = e;
+                L.rpos[slot] = (int)(L.epos[e] >> 16);
+            }
+        }
+        __syncthreads();
+        const int nrg = L.nrange;
+        nranges += nrg;
+        run_lds_scan(L.rpos, nrg, L.scan);
+        const int nc = L.rpos[nrg];
+        ncand += nc;
+        if (stamps && tid == 0 && e0 == 0)
+            d.tstamp[kTsDetect] = wall_clock();
+        for (int c0 = 0; c0 < nc; c0 += kRunCandCap) {
+            const int ncs = min(kRunCandCap, nc - c0);
+            for (int c = tid; c < ncs; c += NT) {
+                const int cc = c0 + c;
+                int lo = 0, hi = nrg - 1; // the range holding candidate cc
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (L.rpos[mid] <= cc)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                const int e = L.rent[lo];
+                const int g = L.gidx[L.erun[e]];
+                const size_t idx =
+                    (size_t)g * d.RL + (size_t)((int)(L.epos[e] & 0xffffu) + (cc - L.rpos[lo]));
+                L.cx[c] = d.rx[idx];
+                L.cslot[c] = d.rslot[idx];
+                L.ccnt[c] = 0;
+            }
+            __syncthreads();
+            const int P = ncs * nr;
+            for (int p = tid; p < P; p += NT) {
+                const int c = p / nr, r = p - c * nr;
+                const float xp = L.cx[c];
+                if (!__builtin_isfinite(xp)) // NaN and +-inf have no neighbour, not even themselves
+                    continue;
+                const int cnt = run_count(d, L.meta[r], L.gidx[r], xp, tests);
+                if (cnt)
+                    atomicAdd(&L.ccnt[c], cnt);
+            }
+            __syncthreads();
+            for (int c = tid; c < ncs; c += NT) {
+                const int cnt = L.ccnt[c];
+                hits += (unsigned long long)min(cnt, d.K + 1);
+                if (cnt <= d.K) {
+                    const int sl = L.cslot[c];
+                    atomicOr(&L.bitmap[sl >> 5], 1u << (sl & 31));
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (tests)
+        atomicAdd(&L.tests, tests);
+    if (hits)
+        atomicAdd(&L.hits, hits);
+
+    // 5. the bitmap as an ascending slot list in the slide's host-out buffer
+    if (stamps && tid == 0)
+        d.tstamp[kTsFinish] = wall_clock();
+    HostOut *ho = run_host_out(d, i);
+    const int wpt = (words + NT - 1) / NT;
+    const int w0 = tid * wpt, w1 = min(words, w0 + wpt);
+    int mine = 0;
+    for (int w = w0; w < w1; ++w)
+        mine += __popc(L.bitmap[w]);
+    int total;
+    int o = block_excl_scan<NT>(mine, L.scan, total);
+    for (int w = w0; w < w1; ++w) {
+        uint32_t b = L.bitmap[w];
+        while (b) {
+            const int bit = __ffs(b) - 1;
+            b &= b - 1;
+            ho->list[o++] = (w << 5) + bit;
+        }
+    }
+    // every wave's list writes acknowledged, then one system-scope release
+    // before the stamp (acknowledged writes to host memory are not yet
+    // visible to the host without it)
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (tid == 0) {
+        ho->n_out = total;
+        ho->n_cand = ncand;
+        ho->n_units = nranges;
+        ho->pair_tests = L.tests;
+        ho->hits = L.hits;
+        ho->misses = 0;
+        ho->used_pred = 0;
+        ho->learn_kernels = 0;
+        ho->train_steps = 0;
+        ho->train_loss = 0.f;
+        ho->err = atomicExch(d.err, 0);
+        if (stamps) {
+#pragma unroll
+            for (int k = 0; k < kTsPublished; ++k)
+                ho->ts[k] = d.tstamp[k];
+            ho->ts[kTsPublished] = wall_clock();
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        __hip_atomic_store(&ho->slide, (int64_t)(i + 1), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+// One launch: job.nsort slides' runs sorted (rps workgroups each), then
+// job.ndet slides detected (one workgroup each); or, fused, one workgroup
+// that sorts slide det_i0's run and then detects it (one slide per call).
+// The detects of a launch read runs sorted by earlier launches only.
+__global__ __launch_bounds__(kRunNT) void k_runs(RunsDev d, RunJob job)
+{
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    if (job.fused) {
+        if (job.stamps && threadIdx.x == 0)
+            d.tstamp[kTsIngest] = wall_clock();
+        run_sort_body(d, job.sort_q0, 0, job.src[0], job.stamps,
+                      *reinterpret_cast<RunSortLds *>(dyn_lds));
+        // the run's stores reach L2 before any wave of this workgroup reads
+        // them back (the detect's loads go through this CU's L1)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        run_detect_body(d, job.det_i0, job.stamps, *reinterpret_cast<RunDetLds *>(dyn_lds));
+        return;
+    }
+    const int nsw = job.nsort * d.rps;
+    int bid = blockIdx.x;
+    if (bid < nsw) {
+        const int k = bid / d.rps, part = bid - k * d.rps;
+        const int64_t q = job.sort_q0 + k;
+        const float *src = job.prime ? job.src[0] + (size_t)k * d.S : job.src[k];
+        if (job.stamps && bid == 0 && threadIdx.x == 0)
+            d.tstamp[kTsIngest] = wall_clock();
+        run_sort_body(d, q, part, src, job.stamps && bid == 0,
+                      *reinterpret_cast<RunSortLds *>(dyn_lds));
+        return;
+    }
+    bid -= nsw;
+    run_detect_body(d, job.det_i0 + bid, job.stamps, *reinterpret_cast<RunDetLds *>(dyn_lds));
+}
+
+} // namespace rtod

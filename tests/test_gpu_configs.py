@@ -43,11 +43,15 @@ def both_paths(xs, W, S, T, Rr, K, name, **kw):
     return want
 
 
-def test_c2_every_slide():
+INDEX_1D = {"runs": R.INDEX_SLIDE_RUNS, "stencil": R.INDEX_CELL_STENCIL}
+
+
+@pytest.mark.parametrize("index", list(INDEX_1D))
+def test_c2_every_slide(index):
     wl = O.C2
-    case(f"{wl.name} all {wl.T} slides")
+    case(f"{wl.name} all {wl.T} slides, {index}")
     xs = wl.data()
-    want = both_paths(xs, wl.W, wl.S, wl.T, wl.R, wl.K, wl.name)
+    want = both_paths(xs, wl.W, wl.S, wl.T, wl.R, wl.K, wl.name, index_kind=INDEX_1D[index])
     assert_same(want[:8], O.run_stream(xs, wl.W, wl.S, 8, wl.R, wl.K, "sorted1d"),
                 "C2 grid vs sorted1d")
 
@@ -63,8 +67,9 @@ SWEEP = [("R", 0.15), ("R", 0.3), ("R", 0.6), ("R", 0.75), ("K", 30), ("K", 70),
          ("S", 1000), ("S", 10000)]
 
 
+@pytest.mark.parametrize("index", list(INDEX_1D))
 @pytest.mark.parametrize("what,val", SWEEP)
-def test_c2_sweep_every_slide(what, val):
+def test_c2_sweep_every_slide(what, val, index):
     wl = O.C2
     Rr, K, S = wl.R, wl.K, wl.S
     if what == "R":
@@ -74,9 +79,9 @@ def test_c2_sweep_every_slide(what, val):
     else:
         S = val
     T = min(10000, (wl.N - wl.W) // S)
-    case(f"{wl.name} sweep {what}={val}: W={wl.W} S={S} R={Rr} K={K}, {T} slides")
+    case(f"{wl.name} sweep {what}={val}: W={wl.W} S={S} R={Rr} K={K}, {T} slides, {index}")
     xs = wl.data()
-    want = both_paths(xs, wl.W, S, T, Rr, K, f"sweep {what}={val}")
+    want = both_paths(xs, wl.W, S, T, Rr, K, f"sweep {what}={val}", index_kind=INDEX_1D[index])
     assert_same(want[:4], O.run_stream(xs, wl.W, S, 4, Rr, K, "sorted1d"),
                 f"sweep {what}={val}: grid vs sorted1d")
 
@@ -137,7 +142,7 @@ def test_pipelined_counters_equal_sequential(sched):
     assert sa["pair_tests_total"] > 0
 
 
-@pytest.mark.parametrize("index", ["lbvh", "stencil"])
+@pytest.mark.parametrize("index", ["lbvh", "stencil", "runs"])
 def test_failed_pipelined_run_needs_prime(index, monkeypatch):
     """A pipelined run that fails part-way (a failure injected on the host
     while collecting slide 3) leaves the context unprimed: further slides are
@@ -146,7 +151,8 @@ def test_failed_pipelined_run_needs_prime(index, monkeypatch):
     case(f"failed pipelined run, {index}")
     W, S, T, Rr, K = 4000, 200, 8, 0.45, 20
     xs = O.gen_walk(77, W + 2 * T * S)
-    kind = R.INDEX_LBVH if index == "lbvh" else R.INDEX_CELL_STENCIL
+    kind = {"lbvh": R.INDEX_LBVH, "stencil": R.INDEX_CELL_STENCIL,
+            "runs": R.INDEX_SLIDE_RUNS}[index]
     monkeypatch.setenv("RTOD_FAIL_AT_SLIDE", "3")
     det = R.Detector(1, W, S, Rr, K, index_kind=kind)
     monkeypatch.delenv("RTOD_FAIL_AT_SLIDE")
@@ -166,16 +172,18 @@ def test_failed_pipelined_run_needs_prime(index, monkeypatch):
         assert_same(got, O.run_stream(xs, W, S, 3, Rr, K, "grid"), f"{index} after prime")
 
 
-@pytest.mark.parametrize("wl", [O.C1, O.C2, O.C3], ids=lambda w: w.name)
-def test_host_fed_pipelined_every_slide(wl):
+@pytest.mark.parametrize("wl,index", [(O.C1, R.INDEX_AUTO), (O.C2, R.INDEX_AUTO),
+                                      (O.C2, R.INDEX_CELL_STENCIL), (O.C3, R.INDEX_AUTO)],
+                         ids=["C1", "C2", "C2-stencil", "C3"])
+def test_host_fed_pipelined_every_slide(wl, index):
     """rtod_slides: the slides from host memory, staged into pinned memory
     just before their ingest, pipelined; all slides of the stream, split into
     runs of uneven length mixed with single calls on one context."""
-    case(f"{wl.name} host-fed pipelined, {wl.T} slides")
+    case(f"{wl.name} host-fed pipelined, {wl.T} slides, index {index}")
     xs = wl.data()
     want = O.run_stream(xs, wl.W, wl.S, wl.T, wl.R, wl.K, "grid")
     got = []
-    with R.Detector(wl.dim, wl.W, wl.S, wl.R, wl.K) as det:
+    with R.Detector(wl.dim, wl.W, wl.S, wl.R, wl.K, index_kind=index) as det:
         det.prime(xs[:wl.W])
         t, runs = 0, [1, 2, 3, 9, 60]
         k = 0
@@ -204,15 +212,16 @@ def test_host_fed_pipelined_long_run_lbvh():
     assert_same(got, O.run_stream(xs, wl.W, wl.S, T, wl.R, wl.K, "grid"), "C3 LBVH host-fed")
 
 
-@pytest.mark.parametrize("index", ["stencil", "lbvh"])
+@pytest.mark.parametrize("index", ["stencil", "lbvh", "runs"])
 def test_time_breakdown_on_every_call(index):
     """The reference's five buckets are filled on every rtod_slide call (no
     flag), from the device stamps of the slide's kernels; with the launch
     remainder they add up to the host wall time."""
-    wl = O.C2 if index == "stencil" else O.C3
+    wl = O.C3 if index == "lbvh" else O.C2
     case(f"time breakdown, {index}")
     xs = wl.data()
-    kind = R.INDEX_CELL_STENCIL if index == "stencil" else R.INDEX_LBVH
+    kind = {"lbvh": R.INDEX_LBVH, "stencil": R.INDEX_CELL_STENCIL,
+            "runs": R.INDEX_SLIDE_RUNS}[index]
     with R.Detector(wl.dim, wl.W, wl.S, wl.R, wl.K, index_kind=kind) as det:
         det.prime(xs[:wl.W])
         for t in range(1, 21):

@@ -19,17 +19,21 @@ def case(name):
     print(f"[case] {name}", flush=True)
 
 
+KINDS = {"stencil": R.INDEX_CELL_STENCIL, "lbvh": R.INDEX_LBVH, "runs": R.INDEX_SLIDE_RUNS}
+
+
 @pytest.mark.parametrize("wl,index", [(O.C1, "stencil"), (O.C2, "stencil"), (O.C3, "stencil"),
-                                      (O.C3, "lbvh")], ids=["C1", "C2", "C3", "C3-lbvh"])
+                                      (O.C3, "lbvh"), (O.C1, "runs"), (O.C2, "runs")],
+                         ids=["C1", "C2", "C3", "C3-lbvh", "C1-runs", "C2-runs"])
 def test_candidates_and_hits_every_slide(wl, index):
     """Hits = the sum over candidates of min(count, K + 1): exact on the GPU
     despite its early exits, equal to the oracle's on every slide."""
-    T = wl.T if index == "stencil" else 60
+    T = 60 if index == "lbvh" else wl.T
     case(f"{wl.name} {index}: candidates and hits, {T} slides")
     xs = wl.data()
     g = O.Grid(wl.dim, wl.W, wl.S, wl.R, wl.K, nthreads=8)
     g.prime(xs[:wl.W])
-    kind = R.INDEX_LBVH if index == "lbvh" else R.INDEX_CELL_STENCIL
+    kind = KINDS[index]
     with R.Detector(wl.dim, wl.W, wl.S, wl.R, wl.K, index_kind=kind) as det:
         det.prime(xs[:wl.W])
         hits = 0
@@ -45,9 +49,10 @@ def test_candidates_and_hits_every_slide(wl, index):
         assert st["hits_total"] == hits
 
 
-def test_pipelined_hits_total():
+@pytest.mark.parametrize("index", ["runs", "stencil"])
+def test_pipelined_hits_total(index):
     wl = O.C2
-    case("pipelined hits total")
+    case(f"pipelined hits total, {index}")
     xs = wl.data()
     T = 60
     g = O.Grid(1, wl.W, wl.S, wl.R, wl.K)
@@ -56,7 +61,7 @@ def test_pipelined_hits_total():
     for t in range(1, T + 1):
         g.slide(xs[wl.W + (t - 1) * wl.S: wl.W + t * wl.S])
         hits += g.stats()["hits"]
-    with R.Detector(1, wl.W, wl.S, wl.R, wl.K) as det:
+    with R.Detector(1, wl.W, wl.S, wl.R, wl.K, index_kind=KINDS[index]) as det:
         det.prime(xs[:wl.W])
         det.slides(xs[wl.W: wl.W + T * wl.S], T)
         assert det.stats()["hits_total"] == hits
@@ -97,6 +102,28 @@ def arena_bytes(dim, W, S, lbvh=False):
         b += 4                                                # n_whole (one int)
         b += 8 * ((W + 63) // 64)                             # candidate masks
     return b
+
+
+def runs_arena_bytes(W, S):
+    """The slide-run index's arena: m + 17 run-ring entries (the window and
+    the slides a pipelined launch sorts and detects ahead, 8 per launch) of
+    ceil(S / 8192) runs, each with its coordinates, slots, cells and cell
+    starts (RL points, RL + 1 starts) and 4097 bucket starts and a 32-byte
+    descriptor; the error word, the kernels' stamps and the prime copy."""
+    m = W // S
+    rps = (S + 8191) // 8192
+    RL = (S + rps - 1) // rps
+    nrun = (m + 17) * rps
+    return nrun * (4 * RL + 4 * RL + 4 * 4097 + 4 * RL + 4 * (RL + 1) + 32) + 4 + 8 * 7 + 4 * W
+
+
+def test_device_memory_accounting_runs():
+    case("device memory, slide-run index")
+    for W, S in [(100000, 5000), (100000, 10000), (10000, 500)]:
+        with R.Detector(1, W, S, 0.45, 50) as det:
+            st = det.stats()
+            assert st["index_kind"] == R.INDEX_SLIDE_RUNS
+            assert st["device_bytes"] == runs_arena_bytes(W, S), (W, S, st["device_bytes"])
 
 
 @pytest.mark.parametrize("dim,W,S,lbvh", [(1, 100000, 5000, False), (3, 10000, 500, False),

@@ -185,19 +185,26 @@ def test_phase_api_equals_slide():
     assert_same(got, want, "phase api")
 
 
+@pytest.mark.parametrize("index_kind", [R.INDEX_CELL_STENCIL, R.INDEX_SLIDE_RUNS])
 @pytest.mark.parametrize("flags", [0, R.FLAG_PHASE_TIMES])
-def test_launch_modes_agree(flags):
-    case(f"flags={flags}")
+def test_launch_modes_agree(flags, index_kind):
+    """With RTOD_FLAG_PHASE_TIMES the slide's kernels are timed with HIP
+    events: the cell stencil's detect, or the slide-run index's one launch
+    per slide (a run of S <= 8192 points is sorted and detected by one
+    workgroup)."""
+    case(f"flags={flags} index={index_kind}")
     wl = O.C1
     xs = wl.data()
     T = 30
     want = O.run_stream(xs, wl.W, wl.S, T, wl.R, wl.K, "grid")
-    with R.Detector(1, wl.W, wl.S, wl.R, wl.K, flags=flags) as det:
+    with R.Detector(1, wl.W, wl.S, wl.R, wl.K, flags=flags, index_kind=index_kind) as det:
         det.prime(xs[:wl.W])
         got = [det.slide(xs[wl.W + (t - 1) * wl.S: wl.W + t * wl.S]) for t in range(1, T + 1)]
+        assert det.stats()["index_kind"] == index_kind
         if flags & R.FLAG_PHASE_TIMES:
             kt = det.kernel_times()
-            assert kt["detect"][1] == T and kt["detect"][0] > 0
+            k = "detect" if index_kind == R.INDEX_CELL_STENCIL else "run_slide"
+            assert kt[k][1] == T and kt[k][0] > 0
             assert det.times.total > 0
     assert_same(got, want, f"flags={flags}")
 
@@ -322,7 +329,8 @@ def test_classify_wait_timeout_fails_slides(monkeypatch):
     xs = wl.data(W + T * S)
     want = O.run_stream(xs, W, S, T, wl.R, wl.K, "grid")
     monkeypatch.setenv("RTOD_CLASSIFY_WAIT_SPINS", "0")
-    with R.DeviceStream(xs) as ds, R.Detector(1, W, S, wl.R, wl.K) as det:
+    with R.DeviceStream(xs) as ds, R.Detector(1, W, S, wl.R, wl.K,
+                                              index_kind=R.INDEX_CELL_STENCIL) as det:
         monkeypatch.delenv("RTOD_CLASSIFY_WAIT_SPINS")
         det.prime(xs[:W])
         with pytest.raises(R.RtodError, match="code 8"):
