@@ -6,9 +6,10 @@ A step is one slide: S new points in host memory in, the window's exact
 outlier list out in host memory.
 
   value    --steps consecutive slides through rtod_slides (pipelined, plain
-           schedule, cell-stencil index), timed on the host from the call's
-           entry to its return: every slide's points are copied from host
-           memory into pinned staging inside the timed call (the reference's
+           schedule, the index AUTO picks: the slide-run index in 1-D, the
+           cell stencil in 2-D/3-D), timed on the host from the call's entry
+           to its return: every slide's points are copied from host memory
+           into pinned staging inside the timed call (the reference's
            per-slide H2D copy is inside its timed loop, optixScan.cpp:1051-
            1058) and every list is in host memory when it returns.  --warmup
            slides go through the same call first, untimed.
@@ -19,13 +20,16 @@ outlier list out in host memory.
            buckets (timer.h:16-24) from the kernels' device stamps.
 
 Extra objects on the line:
-  roofline      the dominant kernel of the measured path (the one launch per
-                slide of the pipelined run), timed with HIP events on the
-                library's stream in a separate slide-by-slide pass, against
-                the 8 TB/s HBM peak; algorithmic bytes per launch = the
-                slide's B = 4*d*W + 4*d*S + 4*|O| (window coordinates read
-                once, new slide written, outlier ids written); traffic = HBM
-                bytes per launch from the committed rocprofv3 counter summary
+  roofline      the dominant kernel of the measured path (the pipelined
+                run's launch: 16 slides detected and 16 sorted per launch on
+                the slide-run index, one slide per launch on the cell
+                stencil), timed with HIP events on the library's stream in a
+                separate launch-by-launch pass, against the 8 TB/s HBM peak;
+                algorithmic bytes per launch = the slides it detects times
+                the slide's B = 4*d*W + 4*d*S + 4*|O| (window coordinates
+                read once, new slide written, outlier ids written); traffic =
+                HBM bytes per launch from the committed rocprofv3 counter
+                summary
   cpu_baseline  the grid oracle (same algorithm, C, pthreads) on the host's
                 cores: all of them and one thread, the faster reported, with
                 the CPU model; and the number of points whose outlier status
@@ -35,7 +39,8 @@ Extra objects on the line:
   sweeps        the reference's R, K and S sweeps around this default
                 (script/run.py:76-218)
   other_workloads, index_comparison, ablations, learned, batched
-                the 1-D W=10k and 3-D configurations, cell stencil vs LBVH,
+                the 1-D W=10k and 3-D configurations, slide-run index vs
+                cell stencil vs LBVH (1-D) and cell stencil vs LBVH (3-D),
                 the reference's OPTIMIZATION 0/1/2 and refit-vs-rebuild
                 modes and its ray-count runs, the learned work scheduler
                 against the plain one, and 256 batched streams
@@ -167,7 +172,7 @@ def measured_traffic(workload: str, kernel: str):
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("kernel=") and f"k_{kernel}<" in line:
+        if line.startswith("kernel=") and (f"k_{kernel}<" in line or f"k_{kernel}(" in line):
             for tok in line.split():
                 if tok.startswith("traffic_bytes="):
                     return int(tok.split("=")[1])
@@ -308,21 +313,32 @@ def sweeps(warmup, steps, cpu_slides, with_cpu, threads):
 
 
 def index_comparison(names, warmup, steps):
-    """Per-slide latency of the cell-stencil index and of the LBVH index on
-    the same streams (the index is chosen by which measures faster)."""
+    """Per-slide time of each index on the same streams, one slide per call
+    and host-fed pipelined: the slide-run index (1-D only), the cell stencil
+    and the LBVH (AUTO takes the fastest measured)."""
     import rtod_amd
 
+    kinds = {"slide_runs": rtod_amd.INDEX_SLIDE_RUNS,
+             "cell_stencil": rtod_amd.INDEX_CELL_STENCIL, "lbvh": rtod_amd.INDEX_LBVH}
     out = {}
     for name in names:
         wl = workloads()[name]
         xs = stream(name, warmup + steps)
-        st = single_calls(xs, wl.W, wl.S, wl.R, wl.K, warmup, steps,
-                          index=rtod_amd.INDEX_CELL_STENCIL)
-        lb = single_calls(xs, wl.W, wl.S, wl.R, wl.K, warmup, steps, index=rtod_amd.INDEX_LBVH)
-        out[name] = {"cell_stencil_us": round(st["us"], 3), "lbvh_us": round(lb["us"], 3),
-                     "cell_stencil_stages_ms": st["stages_ms"], "lbvh_stages_ms": lb["stages_ms"],
-                     "mismatched_slides": same(st["lists"], lb["lists"]),
-                     "faster": "cell_stencil" if st["us"] <= lb["us"] else "lbvh"}
+        ent, ref = {}, None
+        for k, ix in kinds.items():
+            if k == "slide_runs" and wl.dim != 1:
+                continue
+            sc = single_calls(xs, wl.W, wl.S, wl.R, wl.K, warmup, steps, index=ix)
+            hp = host_pipelined(xs, wl.W, wl.S, wl.R, wl.K, warmup, steps, index=ix)
+            ref = sc["lists"] if ref is None else ref
+            ent[f"{k}_us"] = round(sc["us"], 3)
+            ent[f"{k}_pipelined_us"] = round(hp["us"], 3)
+            ent[f"{k}_stages_ms"] = sc["stages_ms"]
+            ent.setdefault("mismatched_slides", 0)
+            ent["mismatched_slides"] += same(sc["lists"], ref) + same(hp["lists"], ref)
+        ent["faster"] = min((k for k in kinds if f"{k}_us" in ent),
+                            key=lambda k: ent[f"{k}_pipelined_us"])
+        out[name] = ent
         log(f"[bench] index comparison {name}: {out[name]}")
     return out
 
@@ -336,7 +352,8 @@ def ablations(names, warmup, steps):
     import rtod_amd as RA
 
     modes = {
-        "stencil_default": dict(),
+        "default": dict(),
+        "stencil_default": dict(index_kind=RA.INDEX_CELL_STENCIL),
         "stencil_no_filter": dict(ablation=RA.ABL_NO_FILTER),
         "lbvh_default_opt2": dict(index_kind=RA.INDEX_LBVH),
         "lbvh_filter_only_opt1": dict(index_kind=RA.INDEX_LBVH, ablation=RA.ABL_FILTER_ONLY),
@@ -565,7 +582,7 @@ def main() -> int:
                     help="skip the other workloads, the sweeps and the index comparison")
     ap.add_argument("--workload", default="walk1d-W100k-S5k", choices=sorted(workloads()),
                     help="profiling runs only: the JSON line always names the workload run")
-    ap.add_argument("--index", default="auto", choices=("auto", "stencil", "lbvh"),
+    ap.add_argument("--index", default="auto", choices=("auto", "runs", "stencil", "lbvh"),
                     help="profiling runs only: the index of the measured workload's legs")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-streams", type=int, default=256)
@@ -598,8 +615,8 @@ def main() -> int:
     log(f"[bench] {args.workload}: {xs.shape[0]} points; warmup {args.warmup} "
         f"steps {args.steps}; {rtod_amd.version()}")
 
-    ix = {"auto": rtod_amd.INDEX_AUTO, "stencil": rtod_amd.INDEX_CELL_STENCIL,
-          "lbvh": rtod_amd.INDEX_LBVH}[args.index]
+    ix = {"auto": rtod_amd.INDEX_AUTO, "runs": rtod_amd.INDEX_SLIDE_RUNS,
+          "stencil": rtod_amd.INDEX_CELL_STENCIL, "lbvh": rtod_amd.INDEX_LBVH}[args.index]
     # the measured value: host-fed pipelined slides
     hp = host_pipelined(xs, W, S, R, K, args.warmup, args.steps, index=ix)
     log(f"[bench] host-fed pipelined (rtod_slides): {hp['us']:.2f} us/slide")
@@ -615,13 +632,21 @@ def main() -> int:
     timed = device_pipelined(xs, W, S, R, K, min(args.warmup, 10), min(args.steps, 60),
                              index=ix, flags=rtod_amd.FLAG_PHASE_TIMES)
     kern = {k: v[0] for k, v in timed["kernels"].items() if v[1] > 0}
-    dom = max(kern, key=kern.get)
+    launches = {k: v[1] for k, v in timed["kernels"].items() if v[1] > 0}
+    dom = max(kern, key=lambda k: kern[k] * launches[k])
     dom_ms = kern[dom]
+    # slides one launch of the dominant kernel detects (the timed pass runs
+    # min(steps, 60) slides; every one of them is detected by that kernel)
+    per_launch = min(args.steps, 60) / launches[dom]
     n_out = float(np.mean([len(x) for x in sc["lists"]]))
     b_slide = 4 * dim * W + 4 * dim * S + 4 * n_out
-    achieved = b_slide / (dom_ms * 1e-3) / 1e9
+    b_launch = b_slide * per_launch
+    achieved = b_launch / (dom_ms * 1e-3) / 1e9
     traffic = measured_traffic(args.workload, dom)
-    log(f"[bench] pipelined launch {dom}: {dom_ms * 1e3:.2f} us; B_slide {b_slide:.0f} B")
+    log(f"[bench] pipelined launch {dom}: {dom_ms * 1e3:.2f} us for {per_launch:.2f} slides; "
+        f"B_slide {b_slide:.0f} B")
+    index_used = {rtod_amd.INDEX_SLIDE_RUNS: "slide-run", rtod_amd.INDEX_CELL_STENCIL:
+                  "cell-stencil", rtod_amd.INDEX_LBVH: "LBVH"}[hp["stats"]["index_kind"]]
 
     cpu = None
     if not args.no_cpu:
@@ -680,9 +705,9 @@ def main() -> int:
         "data": "synthetic: seeded 1-D random walk (start 10, N(0,0.05) steps, 0.2% spikes "
                 "+-U(2,5)); 3-D: mixture of 8 drifting anisotropic Gaussians + 0.5% noise",
         "config": {"workload": f"{args.workload} W={W} S={S} R={R} K={K} dim={dim} (plain "
-                               f"schedule, {'LBVH' if args.index == 'lbvh' else 'cell-stencil'} "
-                               "index; slides from host memory, pipelined on one stream"
-                               f"{'' if args.index == 'lbvh' else ', one launch per slide'})",
+                               f"schedule, {index_used} index; slides from host memory, "
+                               f"pipelined on one stream, {per_launch:.2f} slides per launch of "
+                               f"the dominant kernel)",
                    "window": W, "slide": S, "R": R, "K": K, "dim": dim},
         "valid": invalid == 0,
         "mismatched_slides": int(mism),
@@ -698,8 +723,10 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 3),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
-                     "traffic_over_algorithmic": round(traffic / b_slide, 2) if traffic else None,
-                     "algorithmic_bytes": int(b_slide), "kernel_avg_ms": round(dom_ms, 5),
+                     "traffic_over_algorithmic": round(traffic / b_launch, 2) if traffic else None,
+                     "algorithmic_bytes": int(b_launch), "slides_per_launch": round(per_launch, 2),
+                     "algorithmic_bytes_per_slide": int(b_slide),
+                     "kernel_avg_ms": round(dom_ms, 5),
                      "slide_frac": round(b_slide / (value * 1e-6) / 1e9 / HBM_PEAK_GBS, 6)},
         "cpu_baseline": cpu,
     }

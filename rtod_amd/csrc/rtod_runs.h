@@ -57,8 +57,10 @@ constexpr int kRunMaxRuns = 256;             // runs per window (m x runs per sl
 constexpr int kRunHCap = 8192;               // cells of the window histogram in LDS
 constexpr int kRunCandCap = 2048;            // candidates counted per round
 constexpr int kRunECap = 2048;               // cell-table entries staged in LDS per round
-constexpr int kRunBatch = 8;                 // slides per pipelined launch
-constexpr int kRunHostOuts = 16;             // host-out buffers of a runs context
+constexpr int kRunBatch = 16;                // slides per pipelined launch
+// host-out buffers of a runs context: a pipelined run holds at most two
+// launches' slides uncollected
+constexpr int kRunHostOuts = 2 * kRunBatch;
 // Pinned staging slots of a runs context: a pipelined run stages the slides
 // launch j sorts, [jB + B, jB + 2B), once the slides before (j-1)B are
 // collected; those slots were last read by the sorts of launch j - 2, whose

@@ -139,9 +139,9 @@ def test_nonfinite_clamped_and_duplicates():
     check(xs, W, S, T, 0.3, 10, "edge data")
 
 
-@pytest.mark.parametrize("n", [1, 7, 8, 9, 16, 17, 33])
+@pytest.mark.parametrize("n", [1, 15, 16, 17, 32, 33, 49])
 def test_pipelined_run_lengths(n):
-    """Runs of n slides around the launch batch (8 slides detected and 8
+    """Runs of n slides around the launch batch (16 slides detected and 16
     sorted per launch), host-fed and from HBM, then single calls on the same
     context."""
     case(f"pipelined run of {n}")
