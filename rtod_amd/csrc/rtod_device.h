@@ -7,6 +7,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <climits>
 #include <stdint.h>
 
 namespace rtod {
@@ -111,16 +112,76 @@ __device__ __forceinline__ bool key_guaranteed(uint64_t key)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
+// Wave-wide scans and reductions on DPP moves (register to register between
+// lanes, a few cycles each) instead of ds_bpermute shuffles (an LDS round
+// trip each): row_shr:1, 2, 4, 8 within each 16-lane row, then row_bcast:15
+// (lane 15 of rows 0 and 2 into rows 1 and 3) and row_bcast:31 (lane 31 into
+// rows 2 and 3).  A lane whose DPP source does not exist receives `id`, the
+// operation's identity, so after the six steps lane i holds the inclusive
+// prefix over lanes 0..i and lane 63 the whole wave's result.
+enum : int {
+    kDppRowShr1 = 0x111,
+    kDppRowShr2 = 0x112,
+    kDppRowShr4 = 0x114,
+    kDppRowShr8 = 0x118,
+    kDppRowBcast15 = 0x142,
+    kDppRowBcast31 = 0x143
+};
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_move(int id, int v)
+{
+    return __builtin_amdgcn_update_dpp(id, v, CTRL, ROWS, 0xf, false);
+}
+
+template <typename OP>
+__device__ __forceinline__ int wave_prefix(int x, int id, OP op)
+{
+    x = op(x, dpp_move<kDppRowShr1, 0xf>(id, x));
+    x = op(x, dpp_move<kDppRowShr2, 0xf>(id, x));
+    x = op(x, dpp_move<kDppRowShr4, 0xf>(id, x));
+    x = op(x, dpp_move<kDppRowShr8, 0xf>(id, x));
+    x = op(x, dpp_move<kDppRowBcast15, 0xa>(id, x));
+    x = op(x, dpp_move<kDppRowBcast31, 0xc>(id, x));
+    return x;
+}
+
 __device__ __forceinline__ int wave_incl_scan(int x)
 {
-    const int lane = lane_id();
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        int y = __shfl_up(x, d, kWave);
-        if (lane >= d)
-            x += y;
-    }
-    return x;
+    return wave_prefix(x, 0, [](int a, int b) { return a + b; });
+}
+
+// whole-wave reductions (every lane gets the result)
+__device__ __forceinline__ int wave_min_i32(int x)
+{
+    return __builtin_amdgcn_readlane(
+        wave_prefix(x, INT_MAX, [](int a, int b) { return a < b ? a : b; }), kWave - 1);
+}
+__device__ __forceinline__ int wave_max_i32(int x)
+{
+    return __builtin_amdgcn_readlane(
+        wave_prefix(x, INT_MIN, [](int a, int b) { return a > b ? a : b; }), kWave - 1);
+}
+__device__ __forceinline__ int wave_or_i32(int x)
+{
+    return __builtin_amdgcn_readlane(wave_prefix(x, 0, [](int a, int b) { return a | b; }),
+                                     kWave - 1);
+}
+// float min/max over the non-NaN values (fminf/fmaxf drop NaN operands)
+__device__ __forceinline__ float wave_min_f32(float x)
+{
+    const int r = wave_prefix(__float_as_int(x), __float_as_int(__builtin_inff()), [](int a, int b) {
+        return __float_as_int(fminf(__int_as_float(a), __int_as_float(b)));
+    });
+    return __int_as_float(__builtin_amdgcn_readlane(r, kWave - 1));
+}
+__device__ __forceinline__ float wave_max_f32(float x)
+{
+    const int r = wave_prefix(__float_as_int(x), __float_as_int(-__builtin_inff()),
+                              [](int a, int b) {
+                                  return __float_as_int(fmaxf(__int_as_float(a), __int_as_float(b)));
+                              });
+    return __int_as_float(__builtin_amdgcn_readlane(r, kWave - 1));
 }
 
 // Exclusive block scan of one int per thread; s_tmp holds NT/64 ints.

@@ -1651,7 +1651,10 @@ rtod_status slide_impl(rtod_ctx *c, const float *new_pts, int32_t *outs, int32_t
     RTOD_TRY(slide_checks(c, new_pts, outs, n_out));
     RTOD_HIP(hipSetDevice(c->device));
     const size_t nf = (size_t)c->S * c->dim;
-    float *slot = c->h_in + (size_t)(c->host_t % c->stage_slots) * nf;
+    // a slide run one per call is staged in slot 0 on the slide-run index:
+    // every launch that read a staging slot has completed by now, and the
+    // device's translations of that slot's pages stay warm
+    float *slot = c->h_in + (c->runs ? 0 : (size_t)(c->host_t % c->stage_slots) * nf);
     std::memcpy(slot, new_pts, nf * sizeof(float));
     c->dev_src = nullptr;
     if (c->runs)
@@ -1950,9 +1953,9 @@ rtod_status slides_impl(rtod_ctx *c, const SlideSrc &src, int32_t n, int32_t *of
 void runs_launch(rtod_ctx *c, hipStream_t s, const RunJob &job)
 {
     const int grid = job.fused ? 1 : job.nsort * c->rd.rps + job.ndet;
-    const bool sort = job.fused || job.nsort > 0, det = job.fused || job.ndet > 0;
-    hipLaunchKernelGGL(k_runs, dim3(grid), dim3(kRunNT), run_lds_bytes(c->W, sort, det), s, c->rd,
-                       job);
+    const size_t lds = job.fused ? run_fused_lds_bytes(c->W)
+                                 : run_lds_bytes(c->W, job.nsort > 0, job.ndet > 0);
+    hipLaunchKernelGGL(k_runs, dim3(grid), dim3(kRunNT), lds, s, c->rd, job);
 }
 
 // The initial window: batch b (slots b*S + [0, S)) sorted into run b.

@@ -48,7 +48,9 @@
 
 namespace rtod {
 
-constexpr int kRunNT = 1024;                 // threads of every runs workgroup
+constexpr int kRunNT = 1024;                 // threads of every runs workgroup (four
+                                             // waves per SIMD hide the LDS and memory
+                                             // latency of its chain; 256 threads: 2x slower)
 constexpr int kRunIT = 8;                    // points per thread in a run's sort
 constexpr int kRunMaxLen = kRunNT * kRunIT;  // points per run (a slide: 1 or 2 runs)
 constexpr int kRunBMax = 4096;               // buckets of a direct-mode run
@@ -150,63 +152,94 @@ __device__ __forceinline__ int64_t sub_coord(double v, double inv_s, int s)
     return (int64_t)f;
 }
 
+// sub_coord's key as a double (an integer below 2^53 in magnitude, exact)
+__device__ __forceinline__ double sub_key_d(double v, double inv_s, int s)
+{
+    double f = floor(v * inv_s);
+    const double lo = -(double)(1ll << (30 + s));
+    const double hi = (double)((1ll << (30 + s)) - 1);
+    if (!(f > lo))
+        f = lo;
+    else if (!(f < hi))
+        f = hi;
+    return f;
+}
+
 // a cell whose diameter bound holds (strictly inside the clamp range)
 __device__ __forceinline__ bool run_cell_guaranteed(int32_t c)
 {
     return c > -(1 << 30) && c < (1 << 30) - 1;
 }
 
-// In-place exclusive scan of a[0, n) in LDS (n <= kRunNT * 8), a[n] = total.
-// a is 16-byte aligned; up to 4 * kRunNT entries each thread reads and
-// writes its four with one 128-bit LDS access.
+// Exclusive scan of one int per thread over the workgroup: the waves'
+// totals go through LDS once, and each wave scans them in registers (DPP),
+// so no thread loops over the waves.  s_tmp: kRunNT / kWave ints.  Two
+// barriers; every thread gets the total.
+__device__ __forceinline__ int run_scan(int x, int32_t *s_tmp, int &total)
+{
+    constexpr int NW = kRunNT / kWave;
+    const int lane = lane_id(), wid = threadIdx.x / kWave;
+    const int inc = wave_incl_scan(x);
+    if (lane == kWave - 1)
+        s_tmp[wid] = inc;
+    __syncthreads();
+    const int v = lane < NW ? s_tmp[lane] : 0;
+    const int sc = wave_incl_scan(v);
+    total = __builtin_amdgcn_readlane(sc, NW - 1);
+    const int before = __builtin_amdgcn_readlane(sc - v, wid);
+    __syncthreads(); // s_tmp is free for the next scan
+    return before + inc - x;
+}
+
+// In-place exclusive scan of a[0, n) in LDS (n <= 32 * kRunNT), a[n] =
+// total.  a is 16-byte aligned with room for n rounded up to 4, plus one;
+// each thread scans a contiguous run of 16-byte groups.
 __device__ __forceinline__ void run_lds_scan(int32_t *a, int n, int32_t *s_tmp)
 {
-    if (n <= 4 * kRunNT) {
-        int4 *a4 = reinterpret_cast<int4 *>(a);
-        const int b0 = threadIdx.x * 4;
-        int4 v = make_int4(0, 0, 0, 0);
-        if (b0 < n) {
-            v = a4[threadIdx.x];
-            if (b0 + 1 >= n)
-                v.y = 0;
-            if (b0 + 2 >= n)
-                v.z = 0;
-            if (b0 + 3 >= n)
-                v.w = 0;
-        }
-        int total;
-        const int run = block_excl_scan<kRunNT>(v.x + v.y + v.z + v.w, s_tmp, total);
-        if (b0 < n) {
-            const int4 o = make_int4(run, run + v.x, run + v.x + v.y, run + v.x + v.y + v.z);
-            if (b0 + 3 < n) {
-                a4[threadIdx.x] = o;
-            } else {
-                a[b0] = o.x;
-                if (b0 + 1 < n)
-                    a[b0 + 1] = o.y;
-                if (b0 + 2 < n)
-                    a[b0 + 2] = o.z;
-            }
-        }
-        if (threadIdx.x == 0)
-            a[n] = total;
-        __syncthreads();
-        return;
-    }
-    const int per = (n + kRunNT - 1) / kRunNT;
-    const int b0 = threadIdx.x * per;
+    constexpr int MAXG = 8;
+    int4 *a4 = reinterpret_cast<int4 *>(a);
+    const int n4 = (n + 3) >> 2;
+    const int per = (n4 + kRunNT - 1) / kRunNT;
+    const int g0 = threadIdx.x * per;
+    int4 v[MAXG];
     int loc = 0;
-    for (int j = 0; j < per; ++j)
-        if (b0 + j < n)
-            loc += a[b0 + j];
-    int total;
-    int run = block_excl_scan<kRunNT>(loc, s_tmp, total);
-    for (int j = 0; j < per; ++j)
-        if (b0 + j < n) {
-            const int v = a[b0 + j];
-            a[b0 + j] = run;
-            run += v;
+#pragma unroll
+    for (int j = 0; j < MAXG; ++j) {
+        const int g = g0 + j;
+        v[j] = make_int4(0, 0, 0, 0);
+        if (j < per && g < n4) {
+            v[j] = a4[g];
+            const int b = g * 4;
+            if (b + 1 >= n)
+                v[j].y = 0;
+            if (b + 2 >= n)
+                v[j].z = 0;
+            if (b + 3 >= n)
+                v[j].w = 0;
         }
+        loc += v[j].x + v[j].y + v[j].z + v[j].w;
+    }
+    int total;
+    int run = run_scan(loc, s_tmp, total);
+#pragma unroll
+    for (int j = 0; j < MAXG; ++j) {
+        const int g = g0 + j;
+        if (j < per && g < n4) {
+            const int b = g * 4;
+            const int4 o = make_int4(run, run + v[j].x, run + v[j].x + v[j].y,
+                                     run + v[j].x + v[j].y + v[j].z);
+            if (b + 3 < n) {
+                a4[g] = o;
+            } else {
+                a[b] = o.x;
+                if (b + 1 < n)
+                    a[b + 1] = o.y;
+                if (b + 2 < n)
+                    a[b + 2] = o.z;
+            }
+            run += v[j].x + v[j].y + v[j].z + v[j].w;
+        }
+    }
     if (threadIdx.x == 0)
         a[n] = total;
     __syncthreads();
@@ -233,40 +266,37 @@ struct RunSortLds {
     int32_t scan[kRunNT / kWave];
 };
 
-__device__ __forceinline__ void run_sort_body(const RunsDev &d, int64_t q, int part,
-                                              const float *src, int stamps, RunSortLds &L)
+// ---- pieces of a run's sort -------------------------------------------------
+
+// n points from g into LDS x: one round of loads (16-byte loads when g is
+// 16-byte aligned)
+__device__ __forceinline__ void run_load_points(float *x, const float *g, int n)
 {
-    constexpr int NT = kRunNT, IT = kRunIT, NW = NT / kWave;
-    const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
-    const int e = (int)(q % d.NE) * d.rps + part;
-    const int j0 = part * d.RL;
-    const int n = min(d.RL, d.S - j0);
-    const int slot0 = (int)(q % d.m) * d.S + j0;
-
-    // the run's coordinates into LDS: one round of loads
-    {
-        const float *g = src + j0;
-        if ((((uintptr_t)g) & 15) == 0) {
-            const int n4 = n >> 2;
-            copy_to_lds<(kRunMaxLen / 4 + NT - 1) / NT>(reinterpret_cast<float4 *>(L.x),
-                                                       reinterpret_cast<const float4 *>(g), n4);
-            for (int k = (n4 << 2) + tid; k < n; k += NT)
-                L.x[k] = g[k];
-        } else {
-            copy_to_lds<IT>(L.x, g, n);
-        }
+    if ((((uintptr_t)g) & 15) == 0) {
+        const int n4 = n >> 2;
+        copy_to_lds<(kRunMaxLen / 4 + kRunNT - 1) / kRunNT>(reinterpret_cast<float4 *>(x),
+                                                           reinterpret_cast<const float4 *>(g), n4);
+        for (int k = (n4 << 2) + (int)threadIdx.x; k < n; k += kRunNT)
+            x[k] = g[k];
+    } else {
+        copy_to_lds<kRunIT>(x, g, n);
     }
-    __syncthreads();
+}
 
-    // the run's cell range: cell_coord is monotone in x and puts NaN in the
-    // lowest cell, so it is [cell(min x), cell(max x)] over the non-NaN
-    // points, from the lowest cell if a NaN is present
+// The cell range of x[0, n): cell_coord is monotone in x and puts NaN in the
+// lowest cell, so it is [cell(min x), cell(max x)] over the non-NaN points,
+// from the lowest cell if a NaN is present.  Block-wide (one barrier).
+__device__ __forceinline__ void run_cell_range(const RunsDev &d, const float *x, int n,
+                                               float (*red)[kRunNT / kWave], int &c_lo, int &c_hi)
+{
+    constexpr int NW = kRunNT / kWave;
+    const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
     float xmn = __builtin_inff(), xmx = -__builtin_inff(), nan = 0.f;
 #pragma unroll
-    for (int i = 0; i < IT; ++i) {
-        const int l = tid + i * NT;
+    for (int i = 0; i < kRunIT; ++i) {
+        const int l = tid + i * kRunNT;
         if (l < n) {
-            const float v = L.x[l];
+            const float v = x[l];
             if (v != v) {
                 nan = 1.f;
             } else {
@@ -275,115 +305,178 @@ __device__ __forceinline__ void run_sort_body(const RunsDev &d, int64_t q, int p
             }
         }
     }
-    for (int o = 1; o < kWave; o <<= 1) {
-        xmn = fminf(xmn, __shfl_xor(xmn, o, kWave));
-        xmx = fmaxf(xmx, __shfl_xor(xmx, o, kWave));
-        nan = fmaxf(nan, __shfl_xor(nan, o, kWave));
-    }
+    xmn = wave_min_f32(xmn);
+    xmx = wave_max_f32(xmx);
+    nan = wave_max_f32(nan);
     if (lane == 0) {
-        L.red[0][wid] = xmn;
-        L.red[1][wid] = xmx;
-        L.red[2][wid] = nan;
+        red[0][wid] = xmn;
+        red[1][wid] = xmx;
+        red[2][wid] = nan;
     }
     __syncthreads();
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-        xmn = fminf(xmn, L.red[0][w]);
-        xmx = fmaxf(xmx, L.red[1][w]);
-        nan = fmaxf(nan, L.red[2][w]);
-    }
-    int c_lo = cell_coord<1>((double)xmn, d.inv_c);
-    int c_hi = cell_coord<1>((double)xmx, d.inv_c);
+    xmn = wave_min_f32(lane < NW ? red[0][lane] : __builtin_inff());
+    xmx = wave_max_f32(lane < NW ? red[1][lane] : -__builtin_inff());
+    nan = wave_max_f32(lane < NW ? red[2][lane] : 0.f);
+    c_lo = cell_coord<1>((double)xmn, d.inv_c);
+    c_hi = cell_coord<1>((double)xmx, d.inv_c);
     if (nan != 0.f)
         c_lo = -(1 << 30);
     if (c_hi < c_lo) // every point NaN (min x = +inf)
         c_hi = c_lo;
-    const int64_t crange = (int64_t)c_hi - (int64_t)c_lo + 1;
+}
+
+// direct mode: the largest s <= kRunSMax with crange << s <= kRunBMax buckets
+__device__ __forceinline__ int run_shift(int64_t crange)
+{
+    int s = 0;
+    while (s < kRunSMax && (crange << (s + 1)) <= kRunBMax)
+        ++s;
+    return s;
+}
+
+// Counting sort of x[0, n) by sub-cell bucket (nb buckets from cell c_lo,
+// 2^s per cell): sx/sidx hold the points in bucket order with their arrival
+// index, hist[0, nb] the bucket starts (hist[nb] = n).  Ends with a barrier.
+__device__ __forceinline__ void run_bucket(const RunsDev &d, const float *x, int n, int c_lo,
+                                           int s, int nb, int32_t *hist, float *sx,
+                                           uint16_t *sidx, int32_t *scan, int stamps)
+{
+    constexpr int NT = kRunNT, IT = kRunIT;
+    const int tid = threadIdx.x;
+    const double inv_s = d.inv_c * (double)(1 << s);
+    const double base = (double)c_lo * (double)(1 << s);
+    for (int b = tid; b <= nb; b += NT)
+        hist[b] = 0;
+    __syncthreads();
+    int bk[IT], rk[IT];
+    int bad = 0;
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int l = tid + i * NT;
+        bk[i] = 0;
+        rk[i] = 0;
+        if (l < n) {
+            // the key relative to the run's first bucket, in double (exact:
+            // integers below 2^53), converted once to int32
+            const double fr = sub_key_d((double)x[l], inv_s, s) - base;
+            int f = (fr >= 0.0 && fr < (double)nb) ? (int)fr : -1;
+            if (f < 0) { // cannot happen: key >> s lies in [c_lo, c_hi]
+                bad = 1;
+                f = fr < 0.0 ? 0 : nb - 1;
+            }
+            bk[i] = f;
+            rk[i] = atomicAdd(&hist[f], 1);
+        }
+    }
+    if (bad)
+        atomicOr(d.err, kErrRunBucket);
+    __syncthreads();
+    run_lds_scan(hist, nb, scan); // hist[b]: first position of bucket b; hist[nb] = n
+    if (stamps && tid == 0)
+        d.tstamp[kTsMerge] = wall_clock();
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int l = tid + i * NT;
+        if (l < n) {
+            const int pos = hist[bk[i]] + rk[i];
+            sx[pos] = x[l];
+            sidx[pos] = (uint16_t)l;
+        }
+    }
+    __syncthreads();
+}
+
+// A direct-mode run from LDS (bucket order sx/sidx, bucket starts hist)
+// into ring entry e: coordinates, slots, bucket starts, cell table and
+// descriptor.  With ecell != nullptr the cell table is also listed as
+// staged window entries of run `er` (first point | points << 16).
+// Returns the cell count (every thread).
+__device__ __forceinline__ int run_write_direct(const RunsDev &d, int e, int n, int slot0, int s,
+                                                int nb, int c_lo, int c_hi, const int32_t *hist,
+                                                const float *sx, const uint16_t *sidx,
+                                                int32_t *scan, int32_t *ecell, uint32_t *epos,
+                                                uint8_t *erun, int er)
+{
+    constexpr int NT = kRunNT;
+    const int tid = threadIdx.x;
     float *ox = d.rx + (size_t)e * d.RL;
     int32_t *os = d.rslot + (size_t)e * d.RL;
     int32_t *oc = d.rcell + (size_t)e * d.RL;
     int32_t *ocs = d.rcs + (size_t)e * (d.RL + 1);
+    for (int p = tid; p < n; p += NT) {
+        ox[p] = sx[p];
+        os[p] = slot0 + (int)sidx[p];
+    }
+    int32_t *op = d.rpre + (size_t)e * (kRunBMax + 1);
+    for (int b = tid; b <= nb; b += NT)
+        op[b] = hist[b];
+    // the cell table: cells of [c_lo, c_hi] holding points, ascending
+    const int nc = c_hi - c_lo + 1;
+    const int per = (nc + NT - 1) / NT;
+    const int k0 = tid * per;
+    int mine = 0;
+    for (int j = 0; j < per; ++j) {
+        const int k = k0 + j;
+        if (k < nc && hist[(k + 1) << s] > hist[k << s])
+            ++mine;
+    }
+    int ne;
+    int o = run_scan(mine, scan, ne);
+    for (int j = 0; j < per; ++j) {
+        const int k = k0 + j;
+        if (k >= nc)
+            break;
+        const int a = hist[k << s];
+        if (hist[(k + 1) << s] > a) {
+            oc[o] = c_lo + k;
+            ocs[o] = a;
+            if (ecell) {
+                ecell[o] = c_lo + k;
+                epos[o] = (uint32_t)a | ((uint32_t)(hist[(k + 1) << s] - a) << 16);
+                erun[o] = (uint8_t)er;
+            }
+            ++o;
+        }
+    }
+    if (tid == 0) {
+        ocs[ne] = n;
+        d.meta[e] = RunMeta{n, kRunDirect, s, nb, c_lo, c_hi, ne, 0};
+    }
+    return ne;
+}
+
+__device__ __forceinline__ void run_sort_body(const RunsDev &d, int64_t q, int part,
+                                              const float *src, int stamps, RunSortLds &L)
+{
+    constexpr int NT = kRunNT, IT = kRunIT;
+    const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
+    const int e = (int)(q % d.NE) * d.rps + part;
+    const int j0 = part * d.RL;
+    const int n = min(d.RL, d.S - j0);
+    const int slot0 = (int)(q % d.m) * d.S + j0;
+
+    run_load_points(L.x, src + j0, n);
+    __syncthreads();
+    int c_lo, c_hi;
+    run_cell_range(d, L.x, n, L.red, c_lo, c_hi);
+    const int64_t crange = (int64_t)c_hi - (int64_t)c_lo + 1;
 
     if (crange <= kRunBMax) {
         // ---- direct mode: counting sort by sub-cell bucket -----------------
-        int s = 0;
-        while (s < kRunSMax && (crange << (s + 1)) <= kRunBMax)
-            ++s;
+        const int s = run_shift(crange);
         const int nb = (int)(crange << s);
-        const double inv_s = d.inv_c * (double)(1 << s);
-        const int64_t base = (int64_t)c_lo * (1ll << s);
-        int32_t *hist = L.u.dir.hist;
-        for (int b = tid; b <= nb; b += NT)
-            hist[b] = 0;
-        __syncthreads();
-        int bk[IT], rk[IT];
-        int bad = 0;
-#pragma unroll
-        for (int i = 0; i < IT; ++i) {
-            const int l = tid + i * NT;
-            bk[i] = 0;
-            rk[i] = 0;
-            if (l < n) {
-                int64_t f = sub_coord((double)L.x[l], inv_s, s) - base;
-                if (f < 0 || f >= nb) { // cannot happen: key >> s lies in [c_lo, c_hi]
-                    bad = 1;
-                    f = f < 0 ? 0 : nb - 1;
-                }
-                bk[i] = (int)f;
-                rk[i] = atomicAdd(&hist[f], 1);
-            }
-        }
-        if (bad)
-            atomicOr(d.err, kErrRunBucket);
-        __syncthreads();
-        run_lds_scan(hist, nb, L.scan); // hist[b]: first position of bucket b; hist[nb] = n
-        if (stamps && tid == 0)
-            d.tstamp[kTsMerge] = wall_clock();
-#pragma unroll
-        for (int i = 0; i < IT; ++i) {
-            const int l = tid + i * NT;
-            if (l < n) {
-                const int pos = hist[bk[i]] + rk[i];
-                L.u.dir.sx[pos] = L.x[l];
-                L.u.dir.sidx[pos] = (uint16_t)l;
-            }
-        }
-        __syncthreads();
-        for (int p = tid; p < n; p += NT) {
-            ox[p] = L.u.dir.sx[p];
-            os[p] = slot0 + (int)L.u.dir.sidx[p];
-        }
-        int32_t *op = d.rpre + (size_t)e * (kRunBMax + 1);
-        for (int b = tid; b <= nb; b += NT)
-            op[b] = hist[b];
-        // the cell table: cells of [c_lo, c_hi] holding points, ascending
-        const int nc = (int)crange;
-        const int per = (nc + NT - 1) / NT;
-        const int k0 = tid * per;
-        int mine = 0;
-        for (int j = 0; j < per; ++j) {
-            const int k = k0 + j;
-            if (k < nc && hist[(k + 1) << s] > hist[k << s])
-                ++mine;
-        }
-        int ne;
-        int o = block_excl_scan<NT>(mine, L.scan, ne);
-        for (int j = 0; j < per; ++j) {
-            const int k = k0 + j;
-            if (k < nc && hist[(k + 1) << s] > hist[k << s]) {
-                oc[o] = c_lo + k;
-                ocs[o] = hist[k << s];
-                ++o;
-            }
-        }
-        if (tid == 0) {
-            ocs[ne] = n;
-            d.meta[e] = RunMeta{n, kRunDirect, s, nb, c_lo, c_hi, ne, 0};
-        }
+        run_bucket(d, L.x, n, c_lo, s, nb, L.u.dir.hist, L.u.dir.sx, L.u.dir.sidx, L.scan,
+                   stamps);
+        run_write_direct(d, e, n, slot0, s, nb, c_lo, c_hi, L.u.dir.hist, L.u.dir.sx,
+                         L.u.dir.sidx, L.scan, nullptr, nullptr, nullptr, 0);
         return;
     }
 
     // ---- search mode: LDS radix sort by cell, cell table by run length ----
+    float *ox = d.rx + (size_t)e * d.RL;
+    int32_t *os = d.rslot + (size_t)e * d.RL;
+    int32_t *oc = d.rcell + (size_t)e * d.RL;
+    int32_t *ocs = d.rcs + (size_t)e * (d.RL + 1);
     uint64_t k[IT];
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
@@ -415,7 +508,7 @@ __device__ __forceinline__ void run_sort_body(const RunsDev &d, int64_t q, int p
             ++mine;
     }
     int ne;
-    int o = block_excl_scan<NT>(mine, L.scan, ne);
+    int o = run_scan(mine, L.scan, ne);
     for (int j = 0; j < per; ++j) {
         const int p = p0 + j;
         if (p < n && (p == 0 || (L.u.sea.key[p] >> 13) != (L.u.sea.key[p - 1] >> 13))) {
@@ -521,26 +614,28 @@ __device__ __forceinline__ int run_cell_count(const RunsDev &d, const RunMeta &M
     return cs[lo + 1] - cs[lo];
 }
 
-// exact count of run g's points q with FP32 (xp - xq)^2 <= r2 (xp finite)
-__device__ __forceinline__ int run_count(const RunsDev &d, const RunMeta &M, int g, float xp,
-                                         unsigned long long &tests)
+// Exact count of the points q of a run with FP32 (xp - xq)^2 <= r2 (xp
+// finite): keys in [f1, f2) are neighbours, keys below f0 or from f3 are not,
+// the points of the boundary buckets [f0, f1) and [f2, f3) are tested.
+// lb(f): the run's first point whose key (at its resolution s) is >= f.
+template <typename LB>
+__device__ __forceinline__ int run_count_keys(const RunsDev &d, int s, const float *xs, float xp,
+                                              unsigned long long &tests, LB lb)
 {
-    const int s = M.mode == kRunDirect ? M.s : 0;
     const double inv_s = d.inv_c * (double)(1 << s);
     const double xd = (double)xp;
     const int64_t f0 = sub_coord(xd - d.R_out, inv_s, s);     // below: out
     const int64_t f1 = sub_coord(xd - d.R_in, inv_s, s) + 1;  // [f1, f2): in
     const int64_t f2 = sub_coord(xd + d.R_in, inv_s, s);
     const int64_t f3 = sub_coord(xd + d.R_out, inv_s, s) + 1; // from f3: out
-    const int i0 = run_lb(d, M, g, f0);
-    int i1 = run_lb(d, M, g, f1);
-    int i2 = run_lb(d, M, g, f2);
-    int i3 = run_lb(d, M, g, f3);
+    const int i0 = lb(f0);
+    int i1 = lb(f1);
+    int i2 = lb(f2);
+    int i3 = lb(f3);
     i1 = max(i1, i0);
     i2 = max(i2, i1);
     i3 = max(i3, i2);
     int cnt = i2 - i1;
-    const float *xs = d.rx + (size_t)g * d.RL;
     const float r2 = d.r2;
     for (int k = i0; k < i1; ++k) {
         const float dx = xp - xs[k];
@@ -552,6 +647,113 @@ __device__ __forceinline__ int run_count(const RunsDev &d, const RunMeta &M, int
     }
     tests += (unsigned long long)(i1 - i0 + i3 - i2);
     return cnt;
+}
+
+// a direct-mode run given by its bucket starts pre[0, nb] and coordinates
+// xs (HBM, or LDS for the run a one-slide workgroup has just sorted): the
+// four keys of run_count_keys as bucket indices clamped to [0, nb], in
+// double (exact integers) and converted once to int32
+__device__ __forceinline__ int run_count_direct(const RunsDev &d, const RunMeta &M,
+                                                const int32_t *pre, const float *xs, float xp,
+                                                unsigned long long &tests)
+{
+    const int s = M.s;
+    const double inv_s = d.inv_c * (double)(1 << s);
+    const double base = (double)M.c_lo * (double)(1 << s);
+    const double nbd = (double)M.nb;
+    const double xd = (double)xp;
+    auto idx = [&](double v, double plus) {
+        const double b = sub_key_d(v, inv_s, s) + plus - base;
+        return (int)(b < 0.0 ? 0.0 : (b > nbd ? nbd : b));
+    };
+    const int i0 = pre[idx(xd - d.R_out, 0.0)];     // below: out
+    int i1 = pre[idx(xd - d.R_in, 1.0)];            // [i1, i2): in
+    int i2 = pre[idx(xd + d.R_in, 0.0)];
+    int i3 = pre[idx(xd + d.R_out, 1.0)];           // from i3: out
+    i1 = max(i1, i0);
+    i2 = max(i2, i1);
+    i3 = max(i3, i2);
+    int cnt = i2 - i1;
+    const float r2 = d.r2;
+    for (int k = i0; k < i1; ++k) {
+        const float dx = xp - xs[k];
+        cnt += (dx * dx <= r2) ? 1 : 0;
+    }
+    for (int k = i2; k < i3; ++k) {
+        const float dx = xp - xs[k];
+        cnt += (dx * dx <= r2) ? 1 : 0;
+    }
+    tests += (unsigned)(i1 - i0 + i3 - i2);
+    return cnt;
+}
+
+// exact count over run g of the window (HBM)
+__device__ __forceinline__ int run_count(const RunsDev &d, const RunMeta &M, int g, float xp,
+                                         unsigned long long &tests)
+{
+    const float *xs = d.rx + (size_t)g * d.RL;
+    if (M.mode == kRunDirect)
+        return run_count_direct(d, M, d.rpre + (size_t)g * (kRunBMax + 1), xs, xp, tests);
+    return run_count_keys(d, 0, xs, xp, tests, [&](int64_t f) { return run_lb(d, M, g, f); });
+}
+
+// The outlier bitmap (ceil(W / 32) words in LDS) listed ascending into
+// slide i's host-out buffer with the slide's counters, then its stamp after
+// one system-scope release.  tests/hits: the workgroup's totals in LDS,
+// final at the first barrier inside.
+__device__ __forceinline__ void run_publish(const RunsDev &d, int64_t i, const uint32_t *bitmap,
+                                            int32_t *scan, int ncand, int nranges,
+                                            const unsigned long long *tests,
+                                            const unsigned long long *hits, int stamps)
+{
+    constexpr int NT = kRunNT;
+    const int tid = threadIdx.x;
+    if (stamps && tid == 0)
+        d.tstamp[kTsFinish] = wall_clock();
+    HostOut *ho = run_host_out(d, i);
+    const int words = (d.W + 31) >> 5;
+    const int wpt = (words + NT - 1) / NT;
+    const int w0 = tid * wpt, w1 = min(words, w0 + wpt);
+    int mine = 0;
+    for (int w = w0; w < w1; ++w)
+        mine += __popc(bitmap[w]);
+    int total;
+    int o = run_scan(mine, scan, total);
+    for (int w = w0; w < w1; ++w) {
+        uint32_t b = bitmap[w];
+        while (b) {
+            const int bit = __ffs(b) - 1;
+            b &= b - 1;
+            ho->list[o++] = (w << 5) + bit;
+        }
+    }
+    // every wave's list writes acknowledged, then one system-scope release
+    // before the stamp (acknowledged writes to host memory are not yet
+    // visible to the host without it)
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (tid == 0) {
+        ho->n_out = total;
+        ho->n_cand = ncand;
+        ho->n_units = nranges;
+        ho->pair_tests = *tests;
+        ho->hits = *hits;
+        ho->misses = 0;
+        ho->used_pred = 0;
+        ho->learn_kernels = 0;
+        ho->train_steps = 0;
+        ho->train_loss = 0.f;
+        ho->err = atomicExch(d.err, 0);
+        if (stamps) {
+#pragma unroll
+            for (int k = 0; k < kTsPublished; ++k)
+                ho->ts[k] = d.tstamp[k];
+            ho->ts[kTsPublished] = wall_clock();
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        __hip_atomic_store(&ho->slide, (int64_t)(i + 1), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 // stage cell-table entries [e0, e1) of the window (e1 - e0 <= kRunECap)
@@ -601,27 +803,21 @@ __device__ __forceinline__ void run_detect_body(const RunsDev &d, int64_t i, int
         srch = M.mode == kRunSearch ? 1 : 0;
     }
     int E;
-    const int pre = block_excl_scan<NT>(ne, L.scan, E);
+    const int pre = run_scan(ne, L.scan, E);
     if (tid < nr)
         L.ebase[tid] = pre;
-    for (int o = 1; o < kWave; o <<= 1) {
-        cl = min(cl, __shfl_xor(cl, o, kWave));
-        ch = max(ch, __shfl_xor(ch, o, kWave));
-        srch |= __shfl_xor(srch, o, kWave);
-    }
+    cl = wave_min_i32(cl);
+    ch = wave_max_i32(ch);
+    srch = wave_or_i32(srch);
     if (lane == 0) {
         L.red[0][wid] = cl;
         L.red[1][wid] = ch;
         L.red[2][wid] = srch;
     }
     __syncthreads();
-    int cmin = INT_MAX, cmax = INT_MIN, any_search = 0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-        cmin = min(cmin, L.red[0][w]);
-        cmax = max(cmax, L.red[1][w]);
-        any_search |= L.red[2][w];
-    }
+    const int cmin = wave_min_i32(lane < NW ? L.red[0][lane] : INT_MAX);
+    const int cmax = wave_max_i32(lane < NW ? L.red[1][lane] : INT_MIN);
+    const int any_search = wave_or_i32(lane < NW ? L.red[2][lane] : 0);
     const bool fast = !any_search && (int64_t)cmax - (int64_t)cmin + 1 <= kRunHCap;
     const bool one_round = E <= kRunECap;
 
@@ -726,57 +922,265 @@ __device__ __forceinline__ void run_detect_body(const RunsDev &d, int64_t i, int
     if (hits)
         atomicAdd(&L.hits, hits);
 
-    // 5. the bitmap as an ascending slot list in the slide's host-out buffer
-    if (stamps && tid == 0)
-        d.tstamp[kTsFinish] = wall_clock();
-    HostOut *ho = run_host_out(d, i);
-    const int wpt = (words + NT - 1) / NT;
-    const int w0 = tid * wpt, w1 = min(words, w0 + wpt);
-    int mine = 0;
-    for (int w = w0; w < w1; ++w)
-        mine += __popc(L.bitmap[w]);
-    int total;
-    int o = block_excl_scan<NT>(mine, L.scan, total);
-    for (int w = w0; w < w1; ++w) {
-        uint32_t b = L.bitmap[w];
-        while (b) {
-            const int bit = __ffs(b) - 1;
-            b &= b - 1;
-            ho->list[o++] = (w << 5) + bit;
-        }
+    run_publish(d, i, L.bitmap, L.scan, ncand, nranges, &L.tests, &L.hits, stamps);
+}
+
+// ---------------------------------------------------------------------------
+// One slide per call (one run per slide): one workgroup sorts the new run
+// and detects the slide, keeping the new run in LDS.  Its copy in HBM is for
+// later slides only, so nothing waits for those stores; the old runs'
+// descriptors load beside the new points and their cell tables while the
+// new run is bucketed.  Where the window does not fit the workgroup's
+// tables (a run in search mode, more than kSlideECap cell-table entries,
+// a cell range beyond kSlideHCap) it sorts the run into HBM and detects
+// from there like the pipelined workgroups.
+// ---------------------------------------------------------------------------
+constexpr int kSlideHCap = 1024;   // window cells counted in LDS
+constexpr int kSlideECap = 2048;   // window cell-table entries staged in LDS
+constexpr int kSlideCand = 1024;   // candidates counted per round
+
+struct RunSlideLds {
+    RunMeta meta[kRunMaxRuns];
+    int32_t gidx[kRunMaxRuns];
+    int32_t ebase[kRunMaxRuns + 4];
+    int32_t ecell[kSlideECap];             // window cell-table entries: cell,
+    uint32_t epos[kSlideECap];             //   first point | points << 16,
+    uint8_t erun[kSlideECap];              //   window run (the new run: m - 1)
+    alignas(16) int32_t bpre[kRunBMax + 4]; // the new run: bucket starts,
+    float sx[kRunMaxLen];                  //   coordinates in bucket order,
+    uint16_t sidx[kRunMaxLen];             //   their arrival index
+    union {
+        float x[kRunMaxLen];               // the new points in arrival order
+        struct {                           // (dead once bucketed)
+            int32_t hist[kSlideHCap];      // window cell counts over [cmin, cmax]
+            int32_t rent[kSlideECap];      // candidate ranges: entry,
+            alignas(16) int32_t rpos[kSlideECap + 4]; // candidates before it
+            float cx[kSlideCand];
+            int32_t cslot[kSlideCand];
+            int32_t ccnt[kSlideCand];
+        } d;
+    } u;
+    int32_t scan[kRunNT / kWave];
+    float redf[3][kRunNT / kWave];
+    int32_t redi[3][kRunNT / kWave];
+    int32_t nrange;
+    int32_t pad[3];
+    unsigned long long tests, hits;
+    uint32_t bitmap[4];                    // ceil(W / 32) words follow (sized at launch)
+};
+
+__host__ __device__ constexpr size_t run_slide_lds_bytes(int W)
+{
+    return sizeof(RunSlideLds) + (size_t)((W + 31) / 32) * sizeof(uint32_t);
+}
+
+// a fused launch: the one-slide layout, or the sort's and the detect's
+__host__ __device__ constexpr size_t run_fused_lds_bytes(int W)
+{
+    const size_t a = run_slide_lds_bytes(W), b = run_lds_bytes(W, true, true);
+    return a > b ? a : b;
+}
+
+static_assert(sizeof(RunSlideLds) + kRunMaxW / 8 <= 160 * 1024,
+              "a one-slide workgroup's LDS fits the 160 KB of a CU");
+
+// returns false (uniformly) when the slide takes the HBM path instead
+__device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, const float *src,
+                                               int stamps, RunSlideLds &L)
+{
+    constexpr int NT = kRunNT, NW = NT / kWave;
+    const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
+    const int nr = d.m, newr = d.m - 1; // one run per slide: the window's last run is new
+    const int64_t qn = i + d.m;
+    const int gn = (int)(qn % d.NE);
+    const int n = d.S;
+    const int slot0 = (int)(qn % d.m) * d.S;
+
+    // 0. the old runs' descriptors and the new points, loaded together
+    RunMeta M{};
+    if (tid < newr) {
+        const int64_t q = i + 1 + tid;
+        const int g = (int)(q % d.NE);
+        L.gidx[tid] = g;
+        M = d.meta[g];
     }
-    // every wave's list writes acknowledged, then one system-scope release
-    // before the stamp (acknowledged writes to host memory are not yet
-    // visible to the host without it)
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    run_load_points(L.u.x, src, n);
+    int ne = 0, cl = INT_MAX, ch = INT_MIN, srch = 0;
+    if (tid < newr) {
+        L.meta[tid] = M;
+        ne = M.ne;
+        cl = M.c_lo;
+        ch = M.c_hi;
+        srch = M.mode == kRunSearch ? 1 : 0;
+    }
     __syncthreads();
-    if (tid == 0) {
-        ho->n_out = total;
-        ho->n_cand = ncand;
-        ho->n_units = nranges;
-        ho->pair_tests = L.tests;
-        ho->hits = L.hits;
-        ho->misses = 0;
-        ho->used_pred = 0;
-        ho->learn_kernels = 0;
-        ho->train_steps = 0;
-        ho->train_loss = 0.f;
-        ho->err = atomicExch(d.err, 0);
-        if (stamps) {
-#pragma unroll
-            for (int k = 0; k < kTsPublished; ++k)
-                ho->ts[k] = d.tstamp[k];
-            ho->ts[kTsPublished] = wall_clock();
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
-        __hip_atomic_store(&ho->slide, (int64_t)(i + 1), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_SYSTEM);
+
+    // 1. the new run's cell range; the old runs' entry bases and cell range
+    int c_lo, c_hi;
+    run_cell_range(d, L.u.x, n, L.redf, c_lo, c_hi);
+    int E_old;
+    const int pre = run_scan(ne, L.scan, E_old);
+    if (tid < newr)
+        L.ebase[tid] = pre;
+    cl = wave_min_i32(cl);
+    ch = wave_max_i32(ch);
+    srch = wave_or_i32(srch);
+    if (lane == 0) {
+        L.redi[0][wid] = cl;
+        L.redi[1][wid] = ch;
+        L.redi[2][wid] = srch;
     }
+    __syncthreads();
+    const int cmin = min(c_lo, wave_min_i32(lane < NW ? L.redi[0][lane] : INT_MAX));
+    const int cmax = max(c_hi, wave_max_i32(lane < NW ? L.redi[1][lane] : INT_MIN));
+    const int any_search = wave_or_i32(lane < NW ? L.redi[2][lane] : 0);
+    const int64_t crange = (int64_t)c_hi - (int64_t)c_lo + 1;
+    const int64_t ne_max = crange < n ? crange : n; // the new run's cells
+    if (crange > kRunBMax || any_search || (int64_t)cmax - (int64_t)cmin + 1 > kSlideHCap ||
+        (int64_t)E_old + ne_max > kSlideECap) {
+        __syncthreads(); // every thread past its reads of this layout
+        return false;
+    }
+
+    // 2. the old runs' cell tables: loads issued now, stored after the bucketing
+    constexpr int EJ = kSlideECap / NT;
+    int32_t sc[EJ], sr[EJ];
+    uint32_t sp[EJ];
+#pragma unroll
+    for (int j = 0; j < EJ; ++j) {
+        const int e = tid + j * NT;
+        sr[j] = -1;
+        if (e < E_old) {
+            const int r = run_of_entry(L.ebase, newr, e);
+            const int g = L.gidx[r];
+            const int k = e - L.ebase[r];
+            const int32_t *cs = d.rcs + (size_t)g * (d.RL + 1) + k;
+            sc[j] = d.rcell[(size_t)g * d.RL + k];
+            const int a = cs[0], b = cs[1];
+            sp[j] = (uint32_t)a | ((uint32_t)(b - a) << 16);
+            sr[j] = r;
+        }
+    }
+
+    // 3. the new run bucketed in LDS, then written to HBM for later slides
+    const int s = run_shift(crange);
+    const int nb = (int)(crange << s);
+    run_bucket(d, L.u.x, n, c_lo, s, nb, L.bpre, L.sx, L.sidx, L.scan, stamps);
+    const int ne_new = run_write_direct(d, gn, n, slot0, s, nb, c_lo, c_hi, L.bpre, L.sx, L.sidx,
+                                        L.scan, L.ecell + E_old, L.epos + E_old,
+                                        L.erun + E_old, newr);
+#pragma unroll
+    for (int j = 0; j < EJ; ++j) {
+        const int e = tid + j * NT;
+        if (sr[j] >= 0) {
+            L.ecell[e] = sc[j];
+            L.epos[e] = sp[j];
+            L.erun[e] = (uint8_t)sr[j];
+        }
+    }
+    if (tid == 0)
+        L.meta[newr] = RunMeta{n, kRunDirect, s, nb, c_lo, c_hi, ne_new, 0};
+    const int words = (d.W + 31) >> 5;
+    for (int w = tid; w < words; w += NT)
+        L.bitmap[w] = 0;
+    if (tid == 0) {
+        L.tests = 0;
+        L.hits = 0;
+    }
+    const int E = E_old + ne_new;
+
+    // 4. window cell counts in LDS
+    if (stamps && tid == 0)
+        d.tstamp[kTsClassify] = wall_clock();
+    const int cw = cmax - cmin + 1;
+    for (int c = tid; c < cw; c += NT)
+        L.u.d.hist[c] = 0;
+    __syncthreads();
+    for (int e = tid; e < E; e += NT)
+        atomicAdd(&L.u.d.hist[L.ecell[e] - cmin], (int)(L.epos[e] >> 16));
+    if (tid == 0)
+        L.nrange = 0;
+    __syncthreads();
+
+    // 5. candidates: the points of the sparse and the clamped cells
+    for (int e = tid; e < E; e += NT) {
+        const int32_t c = L.ecell[e];
+        if (!run_cell_guaranteed(c) || L.u.d.hist[c - cmin] <= d.K) {
+            const int slot = atomicAdd(&L.nrange, 1);
+            L.u.d.rent[slot] = e;
+            L.u.d.rpos[slot] = (int)(L.epos[e] >> 16);
+        }
+    }
+    __syncthreads();
+    const int nrg = L.nrange;
+    run_lds_scan(L.u.d.rpos, nrg, L.scan);
+    const int nc = L.u.d.rpos[nrg];
+    if (stamps && tid == 0)
+        d.tstamp[kTsDetect] = wall_clock();
+
+    // 6. exact counts by rounds of candidates: the new run from LDS, the
+    //    others from HBM
+    unsigned long long tests = 0, hits = 0;
+    for (int c0 = 0; c0 < nc; c0 += kSlideCand) {
+        const int ncs = min(kSlideCand, nc - c0);
+        for (int c = tid; c < ncs; c += NT) {
+            const int cc = c0 + c;
+            int lo = 0, hi = nrg - 1; // the range holding candidate cc
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (L.u.d.rpos[mid] <= cc)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const int e = L.u.d.rent[lo];
+            const int idx = (int)(L.epos[e] & 0xffffu) + (cc - L.u.d.rpos[lo]);
+            const int r = L.erun[e];
+            if (r == newr) {
+                L.u.d.cx[c] = L.sx[idx];
+                L.u.d.cslot[c] = slot0 + (int)L.sidx[idx];
+            } else {
+                const size_t gi = (size_t)L.gidx[r] * d.RL + (size_t)idx;
+                L.u.d.cx[c] = d.rx[gi];
+                L.u.d.cslot[c] = d.rslot[gi];
+            }
+            L.u.d.ccnt[c] = 0;
+        }
+        __syncthreads();
+        const int P = ncs * nr;
+        for (int p = tid; p < P; p += NT) {
+            const int c = p / nr, r = p - c * nr;
+            const float xp = L.u.d.cx[c];
+            if (!__builtin_isfinite(xp)) // NaN and +-inf have no neighbour, not even themselves
+                continue;
+            const int cnt = r == newr ? run_count_direct(d, L.meta[r], L.bpre, L.sx, xp, tests)
+                                      : run_count(d, L.meta[r], L.gidx[r], xp, tests);
+            if (cnt)
+                atomicAdd(&L.u.d.ccnt[c], cnt);
+        }
+        __syncthreads();
+        for (int c = tid; c < ncs; c += NT) {
+            const int cnt = L.u.d.ccnt[c];
+            hits += (unsigned long long)min(cnt, d.K + 1);
+            if (cnt <= d.K) {
+                const int sl = L.u.d.cslot[c];
+                atomicOr(&L.bitmap[sl >> 5], 1u << (sl & 31));
+            }
+        }
+        __syncthreads();
+    }
+    if (tests)
+        atomicAdd(&L.tests, tests);
+    if (hits)
+        atomicAdd(&L.hits, hits);
+    run_publish(d, i, L.bitmap, L.scan, nc, nrg, &L.tests, &L.hits, stamps);
+    return true;
 }
 
 // One launch: job.nsort slides' runs sorted (rps workgroups each), then
-// job.ndet slides detected (one workgroup each); or, fused, one workgroup
-// that sorts slide det_i0's run and then detects it (one slide per call).
+// job.ndet slides detected (one workgroup each); or, fused (one run per
+// slide), one workgroup that sorts slide det_i0's run and detects the slide
+// (run_slide_body, else the sort into HBM and the detect from there).
 // The detects of a launch read runs sorted by earlier launches only.
 __global__ __launch_bounds__(kRunNT) void k_runs(RunsDev d, RunJob job)
 {
@@ -784,6 +1188,9 @@ __global__ __launch_bounds__(kRunNT) void k_runs(RunsDev d, RunJob job)
     if (job.fused) {
         if (job.stamps && threadIdx.x == 0)
             d.tstamp[kTsIngest] = wall_clock();
+        if (run_slide_body(d, job.det_i0, job.src[0], job.stamps,
+                           *reinterpret_cast<RunSlideLds *>(dyn_lds)))
+            return;
         run_sort_body(d, job.sort_q0, 0, job.src[0], job.stamps,
                       *reinterpret_cast<RunSortLds *>(dyn_lds));
         // the run's stores reach L2 before any wave of this workgroup reads
