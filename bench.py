@@ -172,7 +172,8 @@ def measured_traffic(workload: str, kernel: str):
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("kernel=") and (f"k_{kernel}<" in line or f"k_{kernel}(" in line):
+        name = line.split()[0][len("kernel="):] if line.startswith("kernel=") else ""
+        if name == f"rtod::k_{kernel}" or name.startswith(f"rtod::k_{kernel}<"):
             for tok in line.split():
                 if tok.startswith("traffic_bytes="):
                     return int(tok.split("=")[1])

@@ -1955,7 +1955,10 @@ void runs_launch(rtod_ctx *c, hipStream_t s, const RunJob &job)
     const int grid = job.fused ? 1 : job.nsort * c->rd.rps + job.ndet;
     const size_t lds = job.fused ? run_fused_lds_bytes(c->W)
                                  : run_lds_bytes(c->W, job.nsort > 0, job.ndet > 0);
-    hipLaunchKernelGGL(k_runs, dim3(grid), dim3(kRunNT), lds, s, c->rd, job);
+    auto kern = job.fused ? k_run_slide
+                          : (job.nsort && job.ndet ? k_run_stage
+                                                   : (job.nsort ? k_run_sort : k_run_detect));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kRunNT), lds, s, c->rd, job);
 }
 
 // The initial window: batch b (slots b*S + [0, S)) sorted into run b.

@@ -115,7 +115,7 @@ struct RunsDev {
     int64_t h_out_bytes;
 };
 
-// The work of one k_runs launch: the sorts of nsort slides (rps workgroups
+// The work of one runs launch: the sorts of nsort slides (rps workgroups
 // each) then the detects of ndet slides (one workgroup each); fused: one
 // workgroup sorts slide det_i0's run and then detects the slide.
 struct RunJob {
@@ -1182,9 +1182,9 @@ __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, cons
 // slide), one workgroup that sorts slide det_i0's run and detects the slide
 // (run_slide_body, else the sort into HBM and the detect from there).
 // The detects of a launch read runs sorted by earlier launches only.
-__global__ __launch_bounds__(kRunNT) void k_runs(RunsDev d, RunJob job)
+__device__ __forceinline__ void runs_launch_body(const RunsDev &d, const RunJob &job,
+                                                 unsigned char *dyn_lds)
 {
-    extern __shared__ __align__(16) unsigned char dyn_lds[];
     if (job.fused) {
         if (job.stamps && threadIdx.x == 0)
             d.tstamp[kTsIngest] = wall_clock();
@@ -1215,6 +1215,33 @@ __global__ __launch_bounds__(kRunNT) void k_runs(RunsDev d, RunJob job)
     }
     bid -= nsw;
     run_detect_body(d, job.det_i0 + bid, job.stamps, *reinterpret_cast<RunDetLds *>(dyn_lds));
+}
+
+// The launches by role (one body; separate names for the profiles):
+// k_run_slide  one slide per call (fused, one workgroup)
+// k_run_stage  pipelined: 16 slides detected, the next 16 sorted
+// k_run_sort   the initial window's runs, or one slide's runs (two launches
+//              per slide, phase API)
+// k_run_detect one slide detected (two launches per slide, phase API)
+__global__ __launch_bounds__(kRunNT) void k_run_slide(RunsDev d, RunJob job)
+{
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    runs_launch_body(d, job, dyn_lds);
+}
+__global__ __launch_bounds__(kRunNT) void k_run_stage(RunsDev d, RunJob job)
+{
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    runs_launch_body(d, job, dyn_lds);
+}
+__global__ __launch_bounds__(kRunNT) void k_run_sort(RunsDev d, RunJob job)
+{
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    runs_launch_body(d, job, dyn_lds);
+}
+__global__ __launch_bounds__(kRunNT) void k_run_detect(RunsDev d, RunJob job)
+{
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    runs_launch_body(d, job, dyn_lds);
 }
 
 } // namespace rtod
