@@ -988,7 +988,7 @@ static_assert(sizeof(RunSlideLds) + kRunMaxW / 8 <= 160 * 1024,
 __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, const float *src,
                                                int stamps, RunSlideLds &L)
 {
-    constexpr int NT = kRunNT, NW = NT / kWave;
+    constexpr int NT = kRunNT, NW = NT / kWave, IT = kRunIT;
     const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
     const int nr = d.m, newr = d.m - 1; // one run per slide: the window's last run is new
     const int64_t qn = i + d.m;
@@ -996,28 +996,30 @@ __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, cons
     const int n = d.S;
     const int slot0 = (int)(qn % d.m) * d.S;
 
-    // 0. the old runs' descriptors and the new points, loaded together
+    // 0. the old runs' descriptors, then the new points into registers: the
+    //    descriptors (issued first) arrive first, and the old runs' cell
+    //    tables are requested while the points still cross PCIe
     RunMeta M{};
+    int g_old = 0;
     if (tid < newr) {
-        const int64_t q = i + 1 + tid;
-        const int g = (int)(q % d.NE);
-        L.gidx[tid] = g;
-        M = d.meta[g];
+        g_old = (int)((i + 1 + tid) % d.NE);
+        M = d.meta[g_old];
     }
-    run_load_points(L.u.x, src, n);
+    float v[IT];
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+        const int l = tid + k * NT;
+        v[k] = l < n ? src[l] : 0.f;
+    }
     int ne = 0, cl = INT_MAX, ch = INT_MIN, srch = 0;
     if (tid < newr) {
         L.meta[tid] = M;
+        L.gidx[tid] = g_old;
         ne = M.ne;
         cl = M.c_lo;
         ch = M.c_hi;
         srch = M.mode == kRunSearch ? 1 : 0;
     }
-    __syncthreads();
-
-    // 1. the new run's cell range; the old runs' entry bases and cell range
-    int c_lo, c_hi;
-    run_cell_range(d, L.u.x, n, L.redf, c_lo, c_hi);
     int E_old;
     const int pre = run_scan(ne, L.scan, E_old);
     if (tid < newr)
@@ -1031,26 +1033,18 @@ __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, cons
         L.redi[2][wid] = srch;
     }
     __syncthreads();
-    const int cmin = min(c_lo, wave_min_i32(lane < NW ? L.redi[0][lane] : INT_MAX));
-    const int cmax = max(c_hi, wave_max_i32(lane < NW ? L.redi[1][lane] : INT_MIN));
+    const int ocmin = wave_min_i32(lane < NW ? L.redi[0][lane] : INT_MAX);
+    const int ocmax = wave_max_i32(lane < NW ? L.redi[1][lane] : INT_MIN);
     const int any_search = wave_or_i32(lane < NW ? L.redi[2][lane] : 0);
-    const int64_t crange = (int64_t)c_hi - (int64_t)c_lo + 1;
-    const int64_t ne_max = crange < n ? crange : n; // the new run's cells
-    if (crange > kRunBMax || any_search || (int64_t)cmax - (int64_t)cmin + 1 > kSlideHCap ||
-        (int64_t)E_old + ne_max > kSlideECap) {
-        __syncthreads(); // every thread past its reads of this layout
-        return false;
-    }
-
-    // 2. the old runs' cell tables: loads issued now, stored after the bucketing
     constexpr int EJ = kSlideECap / NT;
     int32_t sc[EJ], sr[EJ];
     uint32_t sp[EJ];
+    const bool stage_old = !any_search && E_old <= kSlideECap;
 #pragma unroll
     for (int j = 0; j < EJ; ++j) {
         const int e = tid + j * NT;
         sr[j] = -1;
-        if (e < E_old) {
+        if (stage_old && e < E_old) {
             const int r = run_of_entry(L.ebase, newr, e);
             const int g = L.gidx[r];
             const int k = e - L.ebase[r];
@@ -1062,13 +1056,54 @@ __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, cons
         }
     }
 
-    // 3. the new run bucketed in LDS, then written to HBM for later slides
+    // 1. the new points into LDS and their cell range (from the registers)
+    float xmn = __builtin_inff(), xmx = -__builtin_inff(), nan = 0.f;
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+        const int l = tid + k * NT;
+        if (l < n) {
+            L.u.x[l] = v[k];
+            if (v[k] != v[k]) {
+                nan = 1.f;
+            } else {
+                xmn = fminf(xmn, v[k]);
+                xmx = fmaxf(xmx, v[k]);
+            }
+        }
+    }
+    xmn = wave_min_f32(xmn);
+    xmx = wave_max_f32(xmx);
+    nan = wave_max_f32(nan);
+    if (lane == 0) {
+        L.redf[0][wid] = xmn;
+        L.redf[1][wid] = xmx;
+        L.redf[2][wid] = nan;
+    }
+    __syncthreads();
+    xmn = wave_min_f32(lane < NW ? L.redf[0][lane] : __builtin_inff());
+    xmx = wave_max_f32(lane < NW ? L.redf[1][lane] : -__builtin_inff());
+    nan = wave_max_f32(lane < NW ? L.redf[2][lane] : 0.f);
+    int c_lo = cell_coord<1>((double)xmn, d.inv_c);
+    int c_hi = cell_coord<1>((double)xmx, d.inv_c);
+    if (nan != 0.f)
+        c_lo = -(1 << 30);
+    if (c_hi < c_lo) // every point NaN (min x = +inf)
+        c_hi = c_lo;
+    const int cmin = min(c_lo, ocmin), cmax = max(c_hi, ocmax);
+    const int64_t crange = (int64_t)c_hi - (int64_t)c_lo + 1;
+    const int64_t ne_max = crange < n ? crange : n; // the new run's cells
+    if (!stage_old || crange > kRunBMax || (int64_t)cmax - (int64_t)cmin + 1 > kSlideHCap ||
+        (int64_t)E_old + ne_max > kSlideECap) {
+        __syncthreads(); // every thread past its reads of this layout
+        return false;
+    }
+
+    // 2. the new run bucketed in LDS (its HBM copy is written after the
+    //    slide is published: only later slides read it)
     const int s = run_shift(crange);
     const int nb = (int)(crange << s);
+    const int nc_new = (int)crange;
     run_bucket(d, L.u.x, n, c_lo, s, nb, L.bpre, L.sx, L.sidx, L.scan, stamps);
-    const int ne_new = run_write_direct(d, gn, n, slot0, s, nb, c_lo, c_hi, L.bpre, L.sx, L.sidx,
-                                        L.scan, L.ecell + E_old, L.epos + E_old,
-                                        L.erun + E_old, newr);
 #pragma unroll
     for (int j = 0; j < EJ; ++j) {
         const int e = tid + j * NT;
@@ -1078,37 +1113,50 @@ __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, cons
             L.erun[e] = (uint8_t)sr[j];
         }
     }
-    if (tid == 0)
-        L.meta[newr] = RunMeta{n, kRunDirect, s, nb, c_lo, c_hi, ne_new, 0};
+    if (tid == 0) {
+        L.meta[newr] = RunMeta{n, kRunDirect, s, nb, c_lo, c_hi, 0, 0};
+        L.tests = 0;
+        L.hits = 0;
+        L.nrange = 0;
+    }
     const int words = (d.W + 31) >> 5;
     for (int w = tid; w < words; w += NT)
         L.bitmap[w] = 0;
-    if (tid == 0) {
-        L.tests = 0;
-        L.hits = 0;
-    }
-    const int E = E_old + ne_new;
 
-    // 4. window cell counts in LDS
+    // 3. window cell counts in LDS: the old runs' staged tables and the new
+    //    run's cells straight from its bucket starts
     if (stamps && tid == 0)
         d.tstamp[kTsClassify] = wall_clock();
     const int cw = cmax - cmin + 1;
     for (int c = tid; c < cw; c += NT)
         L.u.d.hist[c] = 0;
     __syncthreads();
-    for (int e = tid; e < E; e += NT)
+    for (int e = tid; e < E_old; e += NT)
         atomicAdd(&L.u.d.hist[L.ecell[e] - cmin], (int)(L.epos[e] >> 16));
-    if (tid == 0)
-        L.nrange = 0;
+    for (int k = tid; k < nc_new; k += NT) {
+        const int cnt = L.bpre[(k + 1) << s] - L.bpre[k << s];
+        if (cnt)
+            atomicAdd(&L.u.d.hist[c_lo + k - cmin], cnt);
+    }
     __syncthreads();
 
-    // 5. candidates: the points of the sparse and the clamped cells
-    for (int e = tid; e < E; e += NT) {
+    // 4. candidates: the points of the sparse and the clamped cells; a range
+    //    is an old run's staged entry e (>= 0) or the new run's cell k (-1 - k)
+    for (int e = tid; e < E_old; e += NT) {
         const int32_t c = L.ecell[e];
         if (!run_cell_guaranteed(c) || L.u.d.hist[c - cmin] <= d.K) {
             const int slot = atomicAdd(&L.nrange, 1);
             L.u.d.rent[slot] = e;
             L.u.d.rpos[slot] = (int)(L.epos[e] >> 16);
+        }
+    }
+    for (int k = tid; k < nc_new; k += NT) {
+        const int cnt = L.bpre[(k + 1) << s] - L.bpre[k << s];
+        const int32_t c = c_lo + k;
+        if (cnt && (!run_cell_guaranteed(c) || L.u.d.hist[c - cmin] <= d.K)) {
+            const int slot = atomicAdd(&L.nrange, 1);
+            L.u.d.rent[slot] = -1 - k;
+            L.u.d.rpos[slot] = cnt;
         }
     }
     __syncthreads();
@@ -1118,7 +1166,7 @@ __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, cons
     if (stamps && tid == 0)
         d.tstamp[kTsDetect] = wall_clock();
 
-    // 6. exact counts by rounds of candidates: the new run from LDS, the
+    // 5. exact counts by rounds of candidates: the new run from LDS, the
     //    others from HBM
     unsigned long long tests = 0, hits = 0;
     for (int c0 = 0; c0 < nc; c0 += kSlideCand) {
@@ -1134,13 +1182,14 @@ __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, cons
                     hi = mid - 1;
             }
             const int e = L.u.d.rent[lo];
-            const int idx = (int)(L.epos[e] & 0xffffu) + (cc - L.u.d.rpos[lo]);
-            const int r = L.erun[e];
-            if (r == newr) {
+            const int off = cc - L.u.d.rpos[lo];
+            if (e < 0) {
+                const int idx = L.bpre[(-1 - e) << s] + off;
                 L.u.d.cx[c] = L.sx[idx];
                 L.u.d.cslot[c] = slot0 + (int)L.sidx[idx];
             } else {
-                const size_t gi = (size_t)L.gidx[r] * d.RL + (size_t)idx;
+                const size_t gi = (size_t)L.gidx[L.erun[e]] * d.RL +
+                                  (size_t)((int)(L.epos[e] & 0xffffu) + off);
                 L.u.d.cx[c] = d.rx[gi];
                 L.u.d.cslot[c] = d.rslot[gi];
             }
@@ -1174,6 +1223,11 @@ __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, cons
     if (hits)
         atomicAdd(&L.hits, hits);
     run_publish(d, i, L.bitmap, L.scan, nc, nrg, &L.tests, &L.hits, stamps);
+
+    // 6. the new run into HBM for the slides after this one (the stream
+    //    orders their launches after this one's end)
+    run_write_direct(d, gn, n, slot0, s, nb, c_lo, c_hi, L.bpre, L.sx, L.sidx, L.scan, nullptr,
+                     nullptr, nullptr, 0);
     return true;
 }
 
