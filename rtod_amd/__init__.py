@@ -221,26 +221,45 @@ class Detector:
                                        C.byref(self._times)))
         return self._out[:n.value].copy()
 
+    def _out_arrays(self, n_slides: int):
+        return np.zeros(n_slides + 1, np.int32), np.empty(max(1, n_slides * self.cap), np.int32)
+
+    @staticmethod
+    def split(offs: np.ndarray, slots: np.ndarray) -> list[np.ndarray]:
+        """Per-slide lists from the (offsets, slots) arrays of a pipelined run."""
+        return [slots[offs[i]:offs[i + 1]].copy() for i in range(len(offs) - 1)]
+
+    def slides_device_raw(self, dev_ptr: int, n_slides: int, out=None):
+        """rtod_slides_device itself: (offsets [n+1], slots); slide i's
+        outliers are slots[offsets[i]:offsets[i+1]].  out: reusable arrays
+        from an earlier call with as many slides."""
+        offs, slots = out if out is not None else self._out_arrays(n_slides)
+        _check(lib().rtod_slides_device(self._h, C.c_void_p(dev_ptr), n_slides, offs, slots,
+                                        slots.size))
+        return offs, slots
+
+    def slides_raw(self, pts: np.ndarray, n_slides: int, out=None):
+        """rtod_slides itself (pts contiguous float32): (offsets, slots) as
+        slides_device_raw."""
+        if pts.dtype != np.float32 or not pts.flags.c_contiguous:
+            raise ValueError("pts must be C-contiguous float32")
+        if pts.size < n_slides * self.S * self.dim:
+            raise ValueError("pts holds fewer than n_slides slides")
+        offs, slots = out if out is not None else self._out_arrays(n_slides)
+        _check(lib().rtod_slides(self._h, pts.reshape(-1), n_slides, offs, slots, slots.size))
+        return offs, slots
+
     def slides_device(self, dev_ptr: int, n_slides: int) -> list[np.ndarray]:
         """n_slides consecutive slides from HBM (slide i at dev_ptr + i*S*dim
         floats), pipelined; the same lists as n_slides slide_device() calls."""
-        offs = np.zeros(n_slides + 1, np.int32)
-        slots = np.empty(max(1, n_slides * self.cap), np.int32)
-        _check(lib().rtod_slides_device(self._h, C.c_void_p(dev_ptr), n_slides, offs, slots,
-                                        slots.size))
-        return [slots[offs[i]:offs[i + 1]].copy() for i in range(n_slides)]
+        return self.split(*self.slides_device_raw(dev_ptr, n_slides))
 
     def slides(self, pts: np.ndarray, n_slides: int) -> list[np.ndarray]:
         """n_slides consecutive slides from host memory (slide i = rows
         i*S .. (i+1)*S of pts), pipelined, each staged into pinned memory just
         before its ingest; the same lists as n_slides slide() calls."""
-        pts = np.ascontiguousarray(pts, np.float32).reshape(-1)
-        if pts.size < n_slides * self.S * self.dim:
-            raise ValueError("pts holds fewer than n_slides slides")
-        offs = np.zeros(n_slides + 1, np.int32)
-        slots = np.empty(max(1, n_slides * self.cap), np.int32)
-        _check(lib().rtod_slides(self._h, pts, n_slides, offs, slots, slots.size))
-        return [slots[offs[i]:offs[i + 1]].copy() for i in range(n_slides)]
+        pts = np.ascontiguousarray(pts, np.float32)
+        return self.split(*self.slides_raw(pts, n_slides))
 
     @property
     def times(self) -> Times:

@@ -64,7 +64,7 @@ struct Args {
               << "         --optimization 0|1|2      0 no grid filter, 1 grid filter only, 2 grid\n"
               << "                                   filter + index over the candidates (default)\n"
               << "         --update_gas_type 0|1     0 refit the whole-window LBVH, 1 rebuild (default)\n"
-              << "         --index auto|stencil|lbvh --sched plain|learned\n"
+              << "         --index auto|runs|stencil|lbvh --sched plain|learned\n"
               << "         --pipelined               the timed pass as one pipelined call\n"
               << "         --warmup <int>            untimed passes (default 10)\n"
               << "         --gen walk|gmm --seed <u64>  synthetic stream instead of --file\n"
@@ -236,8 +236,9 @@ int main(int argc, char **argv)
     cfg.slide = a.slide;
     cfg.R = (float)a.R;
     cfg.K = a.K;
-    cfg.index_kind = a.index == "lbvh" ? RTOD_INDEX_LBVH
+    cfg.index_kind = a.index == "lbvh"      ? RTOD_INDEX_LBVH
                      : a.index == "stencil" ? RTOD_INDEX_CELL_STENCIL
+                     : a.index == "runs"    ? RTOD_INDEX_SLIDE_RUNS
                                             : RTOD_INDEX_AUTO;
     cfg.sched_kind = a.sched == "learned" ? RTOD_SCHED_LEARNED : RTOD_SCHED_PLAIN;
     cfg.miss_threshold = 0.25f;
