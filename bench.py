@@ -21,7 +21,7 @@ outlier list out in host memory.
 
 Extra objects on the line:
   roofline      the dominant kernel of the measured path (the pipelined
-                run's launch: 16 slides detected and 16 sorted per launch on
+                run's launch: 32 slides detected and 32 sorted per launch on
                 the slide-run index, one slide per launch on the cell
                 stencil), timed with HIP events on the library's stream in a
                 separate launch-by-launch pass, against the 8 TB/s HBM peak;
@@ -634,15 +634,16 @@ def main() -> int:
     mism = same(hp["lists"], sc["lists"]) + same(dp["lists"], sc["lists"])
 
     # the dominant kernel of the measured path, HIP events around each launch
-    timed = device_pipelined(xs, W, S, R, K, min(args.warmup, 10), min(args.steps, 60),
+    timed = device_pipelined(xs, W, S, R, K, min(args.warmup, 10), min(args.steps, 64),
                              index=ix, flags=rtod_amd.FLAG_PHASE_TIMES)
     kern = {k: v[0] for k, v in timed["kernels"].items() if v[1] > 0}
     launches = {k: v[1] for k, v in timed["kernels"].items() if v[1] > 0}
     dom = max(kern, key=lambda k: kern[k] * launches[k])
     dom_ms = kern[dom]
     # slides one launch of the dominant kernel detects (the timed pass runs
-    # min(steps, 60) slides; every one of them is detected by that kernel)
-    per_launch = min(args.steps, 60) / launches[dom]
+    # min(steps, 64) slides, two full launches of 32 on the slide-run index;
+    # every one of them is detected by that kernel)
+    per_launch = min(args.steps, 64) / launches[dom]
     n_out = float(np.mean([len(x) for x in sc["lists"]]))
     b_slide = 4 * dim * W + 4 * dim * S + 4 * n_out
     b_launch = b_slide * per_launch

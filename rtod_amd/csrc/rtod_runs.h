@@ -59,7 +59,7 @@ constexpr int kRunMaxRuns = 256;             // runs per window (m x runs per sl
 constexpr int kRunHCap = 8192;               // cells of the window histogram in LDS
 constexpr int kRunCandCap = 2048;            // candidates counted per round
 constexpr int kRunECap = 2048;               // cell-table entries staged in LDS per round
-constexpr int kRunBatch = 16;                // slides per pipelined launch
+constexpr int kRunBatch = 32;                // slides per pipelined launch
 // host-out buffers of a runs context: a pipelined run holds at most two
 // launches' slides uncollected
 constexpr int kRunHostOuts = 2 * kRunBatch;
@@ -1273,7 +1273,7 @@ __device__ __forceinline__ void runs_launch_body(const RunsDev &d, const RunJob 
 
 // The launches by role (one body; separate names for the profiles):
 // k_run_slide  one slide per call (fused, one workgroup)
-// k_run_stage  pipelined: 16 slides detected, the next 16 sorted
+// k_run_stage  pipelined: 32 slides detected, the next 32 sorted
 // k_run_sort   the initial window's runs, or one slide's runs (two launches
 //              per slide, phase API)
 // k_run_detect one slide detected (two launches per slide, phase API)

@@ -139,9 +139,9 @@ def test_nonfinite_clamped_and_duplicates():
     check(xs, W, S, T, 0.3, 10, "edge data")
 
 
-@pytest.mark.parametrize("n", [1, 15, 16, 17, 32, 33, 49])
+@pytest.mark.parametrize("n", [1, 31, 32, 33, 64, 65, 97])
 def test_pipelined_run_lengths(n):
-    """Runs of n slides around the launch batch (16 slides detected and 16
+    """Runs of n slides around the launch batch (32 slides detected and 32
     sorted per launch), host-fed and from HBM, then single calls on the same
     context."""
     case(f"pipelined run of {n}")
