@@ -184,6 +184,21 @@ __device__ __forceinline__ float wave_max_f32(float x)
     return __int_as_float(__builtin_amdgcn_readlane(r, kWave - 1));
 }
 
+// whole-wave sum of a 64-bit counter on 32-bit DPP moves: the high words,
+// bits 8..31 and bits 0..7 summed apart (64 values of 24 bits, or of 8,
+// cannot carry out of 32 bits), exact while the total fits 64 bits
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x)
+{
+    auto add = [](int a, int b) { return (int)((unsigned)a + (unsigned)b); };
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane(
+        wave_prefix((int)(unsigned)(x >> 32), 0, add), kWave - 1);
+    const unsigned mid = (unsigned)__builtin_amdgcn_readlane(
+        wave_prefix((int)((unsigned)(x >> 8) & 0xffffffu), 0, add), kWave - 1);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane(
+        wave_prefix((int)((unsigned)x & 0xffu), 0, add), kWave - 1);
+    return ((unsigned long long)hi << 32) + ((unsigned long long)mid << 8) + lo;
+}
+
 // Exclusive block scan of one int per thread; s_tmp holds NT/64 ints.
 template <int NT>
 __device__ __forceinline__ int block_excl_scan(int x, int *s_tmp, int &total)

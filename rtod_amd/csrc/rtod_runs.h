@@ -336,18 +336,23 @@ __device__ __forceinline__ int run_shift(int64_t crange)
 
 // Counting sort of x[0, n) by sub-cell bucket (nb buckets from cell c_lo,
 // 2^s per cell): sx/sidx hold the points in bucket order with their arrival
-// index, hist[0, nb] the bucket starts (hist[nb] = n).  Ends with a barrier.
+// index, hist[0, nb] the bucket starts (hist[nb] = n).  zeroed: hist[0, nb]
+// is already zero and a barrier has passed since.  Errors go to *err (HBM
+// or LDS), the stamp to ts (none when null).  Ends with a barrier.
 __device__ __forceinline__ void run_bucket(const RunsDev &d, const float *x, int n, int c_lo,
                                            int s, int nb, int32_t *hist, float *sx,
-                                           uint16_t *sidx, int32_t *scan, int stamps)
+                                           uint16_t *sidx, int32_t *scan, int32_t *err,
+                                           uint64_t *ts, bool zeroed)
 {
     constexpr int NT = kRunNT, IT = kRunIT;
     const int tid = threadIdx.x;
     const double inv_s = d.inv_c * (double)(1 << s);
     const double base = (double)c_lo * (double)(1 << s);
-    for (int b = tid; b <= nb; b += NT)
-        hist[b] = 0;
-    __syncthreads();
+    if (!zeroed) {
+        for (int b = tid; b <= nb; b += NT)
+            hist[b] = 0;
+        __syncthreads();
+    }
     int bk[IT], rk[IT];
     int bad = 0;
 #pragma unroll
@@ -369,11 +374,11 @@ __device__ __forceinline__ void run_bucket(const RunsDev &d, const float *x, int
         }
     }
     if (bad)
-        atomicOr(d.err, kErrRunBucket);
+        atomicOr(err, kErrRunBucket);
     __syncthreads();
     run_lds_scan(hist, nb, scan); // hist[b]: first position of bucket b; hist[nb] = n
-    if (stamps && tid == 0)
-        d.tstamp[kTsMerge] = wall_clock();
+    if (ts && tid == 0)
+        ts[kTsMerge] = wall_clock();
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
         const int l = tid + i * NT;
@@ -446,7 +451,7 @@ __device__ __forceinline__ int run_write_direct(const RunsDev &d, int e, int n, 
 }
 
 __device__ __forceinline__ void run_sort_body(const RunsDev &d, int64_t q, int part,
-                                              const float *src, int stamps, RunSortLds &L)
+                                              const float *src, uint64_t *ts, RunSortLds &L)
 {
     constexpr int NT = kRunNT, IT = kRunIT;
     const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
@@ -466,7 +471,7 @@ __device__ __forceinline__ void run_sort_body(const RunsDev &d, int64_t q, int p
         const int s = run_shift(crange);
         const int nb = (int)(crange << s);
         run_bucket(d, L.x, n, c_lo, s, nb, L.u.dir.hist, L.u.dir.sx, L.u.dir.sidx, L.scan,
-                   stamps);
+                   d.err, ts, false);
         run_write_direct(d, e, n, slot0, s, nb, c_lo, c_hi, L.u.dir.hist, L.u.dir.sx,
                          L.u.dir.sidx, L.scan, nullptr, nullptr, nullptr, 0);
         return;
@@ -491,8 +496,8 @@ __device__ __forceinline__ void run_sort_body(const RunsDev &d, int64_t q, int p
     __syncthreads();
     block_radix_sort<NT, IT>(k, 13, bit_length((uint64_t)(crange - 1)), L.u.sea.key,
                              L.u.sea.hist);
-    if (stamps && tid == 0)
-        d.tstamp[kTsMerge] = wall_clock();
+    if (ts && tid == 0)
+        ts[kTsMerge] = wall_clock();
     for (int p = tid; p < n; p += NT) {
         const int l = (int)(L.u.sea.key[p] & 8191ull);
         ox[p] = L.x[l];
@@ -697,19 +702,37 @@ __device__ __forceinline__ int run_count(const RunsDev &d, const RunMeta &M, int
     return run_count_keys(d, 0, xs, xp, tests, [&](int64_t f) { return run_lb(d, M, g, f); });
 }
 
+// the threads' pair-test and hit counts into the workgroup's totals in LDS:
+// summed per wave first, one atomic per wave (called by every thread)
+__device__ __forceinline__ void run_add_counters(unsigned long long tests,
+                                                 unsigned long long hits,
+                                                 unsigned long long *l_tests,
+                                                 unsigned long long *l_hits)
+{
+    tests = wave_sum_u64(tests);
+    hits = wave_sum_u64(hits);
+    if (lane_id() == 0) {
+        if (tests)
+            atomicAdd(l_tests, tests);
+        if (hits)
+            atomicAdd(l_hits, hits);
+    }
+}
+
 // The outlier bitmap (ceil(W / 32) words in LDS) listed ascending into
 // slide i's host-out buffer with the slide's counters, then its stamp after
 // one system-scope release.  tests/hits: the workgroup's totals in LDS,
-// final at the first barrier inside.
+// final at the first barrier inside.  err: the slide's error flags (thread
+// 0's value); ts: the slide's stamps (LDS or HBM; none when null).
 __device__ __forceinline__ void run_publish(const RunsDev &d, int64_t i, const uint32_t *bitmap,
                                             int32_t *scan, int ncand, int nranges,
                                             const unsigned long long *tests,
-                                            const unsigned long long *hits, int stamps)
+                                            const unsigned long long *hits, uint64_t *ts, int err)
 {
     constexpr int NT = kRunNT;
     const int tid = threadIdx.x;
-    if (stamps && tid == 0)
-        d.tstamp[kTsFinish] = wall_clock();
+    if (ts && tid == 0)
+        ts[kTsFinish] = wall_clock();
     HostOut *ho = run_host_out(d, i);
     const int words = (d.W + 31) >> 5;
     const int wpt = (words + NT - 1) / NT;
@@ -743,11 +766,11 @@ __device__ __forceinline__ void run_publish(const RunsDev &d, int64_t i, const u
         ho->learn_kernels = 0;
         ho->train_steps = 0;
         ho->train_loss = 0.f;
-        ho->err = atomicExch(d.err, 0);
-        if (stamps) {
+        ho->err = err;
+        if (ts) {
 #pragma unroll
             for (int k = 0; k < kTsPublished; ++k)
-                ho->ts[k] = d.tstamp[k];
+                ho->ts[k] = ts[k];
             ho->ts[kTsPublished] = wall_clock();
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
@@ -773,14 +796,20 @@ __device__ __forceinline__ void run_stage_entries(const RunsDev &d, RunDetLds &L
     }
 }
 
-__device__ __forceinline__ void run_detect_body(const RunsDev &d, int64_t i, int stamps,
+__device__ __forceinline__ void run_detect_body(const RunsDev &d, int64_t i, uint64_t *ts,
                                                 RunDetLds &L)
 {
     constexpr int NT = kRunNT, NW = NT / kWave;
     const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
     const int nr = d.m * d.rps;
-    if (stamps && tid == 0)
-        d.tstamp[kTsClassify] = wall_clock();
+    // the error flags raised before this slide's detect (its sort, earlier
+    // launches), taken now so that the exchange's round trip overlaps the
+    // detect (a detect raises none of its own)
+    int err = 0;
+    if (tid == 0)
+        err = atomicExch(d.err, 0);
+    if (ts && tid == 0)
+        ts[kTsClassify] = wall_clock();
     const int words = (d.W + 31) >> 5;
     for (int w = tid; w < words; w += NT)
         L.bitmap[w] = 0;
@@ -872,8 +901,8 @@ __device__ __forceinline__ void run_detect_body(const RunsDev &d, int64_t i, int
         run_lds_scan(L.rpos, nrg, L.scan);
         const int nc = L.rpos[nrg];
         ncand += nc;
-        if (stamps && tid == 0 && e0 == 0)
-            d.tstamp[kTsDetect] = wall_clock();
+        if (ts && tid == 0 && e0 == 0)
+            ts[kTsDetect] = wall_clock();
         for (int c0 = 0; c0 < nc; c0 += kRunCandCap) {
             const int ncs = min(kRunCandCap, nc - c0);
             for (int c = tid; c < ncs; c += NT) {
@@ -917,12 +946,8 @@ __device__ __forceinline__ void run_detect_body(const RunsDev &d, int64_t i, int
             __syncthreads();
         }
     }
-    if (tests)
-        atomicAdd(&L.tests, tests);
-    if (hits)
-        atomicAdd(&L.hits, hits);
-
-    run_publish(d, i, L.bitmap, L.scan, ncand, nranges, &L.tests, &L.hits, stamps);
+    run_add_counters(tests, hits, &L.tests, &L.hits);
+    run_publish(d, i, L.bitmap, L.scan, ncand, nranges, &L.tests, &L.hits, ts, err);
 }
 
 // ---------------------------------------------------------------------------
@@ -964,7 +989,9 @@ struct RunSlideLds {
     float redf[3][kRunNT / kWave];
     int32_t redi[3][kRunNT / kWave];
     int32_t nrange;
-    int32_t pad[3];
+    int32_t err;                           // error flags raised by this workgroup
+    int32_t pad[2];
+    uint64_t ts[kTsCount];                 // the slide's stamps
     unsigned long long tests, hits;
     uint32_t bitmap[4];                    // ceil(W / 32) words follow (sized at launch)
 };
@@ -984,9 +1011,11 @@ __host__ __device__ constexpr size_t run_fused_lds_bytes(int W)
 static_assert(sizeof(RunSlideLds) + kRunMaxW / 8 <= 160 * 1024,
               "a one-slide workgroup's LDS fits the 160 KB of a CU");
 
-// returns false (uniformly) when the slide takes the HBM path instead
+// Returns false (uniformly) when the slide takes the HBM path instead, with
+// the error flags as it found them.  t_in: the launch's start stamp (thread
+// 0), kept with the slide's other stamps in LDS when `stamps`.
 __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, const float *src,
-                                               int stamps, RunSlideLds &L)
+                                               bool stamps, uint64_t t_in, RunSlideLds &L)
 {
     constexpr int NT = kRunNT, NW = NT / kWave, IT = kRunIT;
     const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
@@ -1005,6 +1034,10 @@ __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, cons
         g_old = (int)((i + 1 + tid) % d.NE);
         M = d.meta[g_old];
     }
+    // the flags raised before this slide (its round trip overlaps the loads)
+    int err_in = 0;
+    if (tid == 0)
+        err_in = atomicExch(d.err, 0);
     float v[IT];
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
@@ -1020,6 +1053,13 @@ __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, cons
         ch = M.c_hi;
         srch = M.mode == kRunSearch ? 1 : 0;
     }
+    if (tid == 0) {
+        L.err = 0;
+        L.ts[kTsIngest] = t_in;
+    }
+    // the new run's bucket counts zeroed while the points are in flight
+    for (int b = tid; b <= kRunBMax; b += NT)
+        L.bpre[b] = 0;
     int E_old;
     const int pre = run_scan(ne, L.scan, E_old);
     if (tid < newr)
@@ -1094,16 +1134,19 @@ __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, cons
     const int64_t ne_max = crange < n ? crange : n; // the new run's cells
     if (!stage_old || crange > kRunBMax || (int64_t)cmax - (int64_t)cmin + 1 > kSlideHCap ||
         (int64_t)E_old + ne_max > kSlideECap) {
+        if (tid == 0 && err_in)
+            atomicOr(d.err, err_in); // left for the HBM path's detect to report
         __syncthreads(); // every thread past its reads of this layout
         return false;
     }
+    uint64_t *ts = stamps ? L.ts : nullptr;
 
     // 2. the new run bucketed in LDS (its HBM copy is written after the
     //    slide is published: only later slides read it)
     const int s = run_shift(crange);
     const int nb = (int)(crange << s);
     const int nc_new = (int)crange;
-    run_bucket(d, L.u.x, n, c_lo, s, nb, L.bpre, L.sx, L.sidx, L.scan, stamps);
+    run_bucket(d, L.u.x, n, c_lo, s, nb, L.bpre, L.sx, L.sidx, L.scan, &L.err, ts, true);
 #pragma unroll
     for (int j = 0; j < EJ; ++j) {
         const int e = tid + j * NT;
@@ -1125,8 +1168,8 @@ __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, cons
 
     // 3. window cell counts in LDS: the old runs' staged tables and the new
     //    run's cells straight from its bucket starts
-    if (stamps && tid == 0)
-        d.tstamp[kTsClassify] = wall_clock();
+    if (ts && tid == 0)
+        ts[kTsClassify] = wall_clock();
     const int cw = cmax - cmin + 1;
     for (int c = tid; c < cw; c += NT)
         L.u.d.hist[c] = 0;
@@ -1163,8 +1206,8 @@ __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, cons
     const int nrg = L.nrange;
     run_lds_scan(L.u.d.rpos, nrg, L.scan);
     const int nc = L.u.d.rpos[nrg];
-    if (stamps && tid == 0)
-        d.tstamp[kTsDetect] = wall_clock();
+    if (ts && tid == 0)
+        ts[kTsDetect] = wall_clock();
 
     // 5. exact counts by rounds of candidates: the new run from LDS, the
     //    others from HBM
@@ -1218,11 +1261,8 @@ __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, cons
         }
         __syncthreads();
     }
-    if (tests)
-        atomicAdd(&L.tests, tests);
-    if (hits)
-        atomicAdd(&L.hits, hits);
-    run_publish(d, i, L.bitmap, L.scan, nc, nrg, &L.tests, &L.hits, stamps);
+    run_add_counters(tests, hits, &L.tests, &L.hits);
+    run_publish(d, i, L.bitmap, L.scan, nc, nrg, &L.tests, &L.hits, ts, err_in | L.err);
 
     // 6. the new run into HBM for the slides after this one (the stream
     //    orders their launches after this one's end)
@@ -1231,47 +1271,68 @@ __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, cons
     return true;
 }
 
+// The launch roles: each kernel holds the code of its own role only.
+enum RunRole { kRoleSlide, kRoleStage, kRoleSort, kRoleDetect };
+
+// A one-slide launch whose window does not fit the workgroup's tables: the
+// run sorted into HBM, then the slide detected from there.  (Out of line it
+// would give the kernel a stack in scratch memory, which measured 16 us
+// more per launch.)
+__device__ __forceinline__ void run_slide_hbm(const RunsDev &d, const RunJob &job, uint64_t *ts,
+                                           unsigned char *dyn_lds)
+{
+    run_sort_body(d, job.sort_q0, 0, job.src[0], ts, *reinterpret_cast<RunSortLds *>(dyn_lds));
+    // the run's stores reach L2 before any wave of this workgroup reads
+    // them back (the detect's loads go through this CU's L1)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    run_detect_body(d, job.det_i0, ts, *reinterpret_cast<RunDetLds *>(dyn_lds));
+}
+
 // One launch: job.nsort slides' runs sorted (rps workgroups each), then
-// job.ndet slides detected (one workgroup each); or, fused (one run per
-// slide), one workgroup that sorts slide det_i0's run and detects the slide
-// (run_slide_body, else the sort into HBM and the detect from there).
-// The detects of a launch read runs sorted by earlier launches only.
+// job.ndet slides detected (one workgroup each); or, one slide per call
+// (one run per slide), one workgroup that sorts slide det_i0's run and
+// detects the slide (run_slide_body, else the sort into HBM and the detect
+// from there).  The detects of a launch read runs sorted by earlier
+// launches only.
+template <int ROLE>
 __device__ __forceinline__ void runs_launch_body(const RunsDev &d, const RunJob &job,
                                                  unsigned char *dyn_lds)
 {
-    if (job.fused) {
+    uint64_t *ts = job.stamps ? d.tstamp : nullptr; // stamps of split launches: HBM
+    if constexpr (ROLE == kRoleSlide) {
+        uint64_t t_in = 0;
         if (job.stamps && threadIdx.x == 0)
-            d.tstamp[kTsIngest] = wall_clock();
-        if (run_slide_body(d, job.det_i0, job.src[0], job.stamps,
+            t_in = wall_clock();
+        if (run_slide_body(d, job.det_i0, job.src[0], job.stamps != 0, t_in,
                            *reinterpret_cast<RunSlideLds *>(dyn_lds)))
             return;
-        run_sort_body(d, job.sort_q0, 0, job.src[0], job.stamps,
-                      *reinterpret_cast<RunSortLds *>(dyn_lds));
-        // the run's stores reach L2 before any wave of this workgroup reads
-        // them back (the detect's loads go through this CU's L1)
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        run_detect_body(d, job.det_i0, job.stamps, *reinterpret_cast<RunDetLds *>(dyn_lds));
+        if (ts && threadIdx.x == 0)
+            ts[kTsIngest] = t_in;
+        run_slide_hbm(d, job, ts, dyn_lds);
         return;
     }
     const int nsw = job.nsort * d.rps;
     int bid = blockIdx.x;
-    if (bid < nsw) {
+    if (ROLE != kRoleDetect && bid < nsw) {
         const int k = bid / d.rps, part = bid - k * d.rps;
         const int64_t q = job.sort_q0 + k;
         const float *src = job.prime ? job.src[0] + (size_t)k * d.S : job.src[k];
-        if (job.stamps && bid == 0 && threadIdx.x == 0)
-            d.tstamp[kTsIngest] = wall_clock();
-        run_sort_body(d, q, part, src, job.stamps && bid == 0,
+        if (ts && bid == 0 && threadIdx.x == 0)
+            ts[kTsIngest] = wall_clock();
+        run_sort_body(d, q, part, src, bid == 0 ? ts : nullptr,
                       *reinterpret_cast<RunSortLds *>(dyn_lds));
         return;
     }
-    bid -= nsw;
-    run_detect_body(d, job.det_i0 + bid, job.stamps, *reinterpret_cast<RunDetLds *>(dyn_lds));
+    if constexpr (ROLE != kRoleSort) {
+        bid -= nsw;
+        run_detect_body(d, job.det_i0 + bid, ts, *reinterpret_cast<RunDetLds *>(dyn_lds));
+    }
 }
 
-// The launches by role (one body; separate names for the profiles):
+// The launches by role (one body, instantiated per role: each kernel's code
+// holds only what its role runs):
 // k_run_slide  one slide per call (fused, one workgroup)
 // k_run_stage  pipelined: 32 slides detected, the next 32 sorted
 // k_run_sort   the initial window's runs, or one slide's runs (two launches
@@ -1280,22 +1341,22 @@ __device__ __forceinline__ void runs_launch_body(const RunsDev &d, const RunJob 
 __global__ __launch_bounds__(kRunNT) void k_run_slide(RunsDev d, RunJob job)
 {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
-    runs_launch_body(d, job, dyn_lds);
+    runs_launch_body<kRoleSlide>(d, job, dyn_lds);
 }
 __global__ __launch_bounds__(kRunNT) void k_run_stage(RunsDev d, RunJob job)
 {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
-    runs_launch_body(d, job, dyn_lds);
+    runs_launch_body<kRoleStage>(d, job, dyn_lds);
 }
 __global__ __launch_bounds__(kRunNT) void k_run_sort(RunsDev d, RunJob job)
 {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
-    runs_launch_body(d, job, dyn_lds);
+    runs_launch_body<kRoleSort>(d, job, dyn_lds);
 }
 __global__ __launch_bounds__(kRunNT) void k_run_detect(RunsDev d, RunJob job)
 {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
-    runs_launch_body(d, job, dyn_lds);
+    runs_launch_body<kRoleDetect>(d, job, dyn_lds);
 }
 
 } // namespace rtod
