@@ -72,7 +72,8 @@ constexpr int kRunHostOuts = 2 * kRunBatch;
 constexpr int kRunStageSlots = 3 * kRunBatch;
 constexpr int kRunMaxW = 1 << 18;            // the outlier bitmap of W bits lives in LDS
 
-constexpr int kErrRunBucket = 32;            // a point's bucket fell outside its run's range
+constexpr int kErrRunBucket = 32;
+constexpr int kRunRankCap = 256; // outliers listed by rank (beyond: from the bitmap)            // a point's bucket fell outside its run's range
 
 enum { kRunDirect = 0, kRunSearch = 1 };
 
@@ -196,6 +197,28 @@ __device__ __forceinline__ int run_scan(int x, int32_t *s_tmp, int &total)
 // each thread scans a contiguous run of 16-byte groups.
 __device__ __forceinline__ void run_lds_scan(int32_t *a, int n, int32_t *s_tmp)
 {
+    if (n <= 4 * kWave) { // n uniform: up to 256 values scanned by wave 0 alone (one barrier)
+        if (threadIdx.x < kWave) {
+            const int b = 4 * (int)threadIdx.x;
+            int x[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                x[j] = b + j < n ? a[b + j] : 0;
+            const int loc = x[0] + x[1] + x[2] + x[3];
+            const int inc = wave_incl_scan(loc);
+            int run = inc - loc;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (b + j < n)
+                    a[b + j] = run;
+                run += x[j];
+            }
+            if (threadIdx.x == kWave - 1)
+                a[n] = inc;
+        }
+        __syncthreads();
+        return;
+    }
     constexpr int MAXG = 8;
     int4 *a4 = reinterpret_cast<int4 *>(a);
     const int n4 = (n + 3) >> 2;
@@ -546,8 +569,10 @@ struct RunDetLds {
     int32_t ccnt[kRunCandCap];
     int32_t scan[kRunNT / kWave];
     int32_t red[3][kRunNT / kWave];
+    int32_t olist[kRunRankCap];        // the first outliers' slots, unordered
     int32_t nrange;
-    int32_t pad[3];
+    int32_t nout;                      // outliers found
+    int32_t pad[2];
     unsigned long long tests, hits;
     uint32_t bitmap[4];                // ceil(W / 32) words follow (sized at launch)
 };
@@ -702,6 +727,17 @@ __device__ __forceinline__ int run_count(const RunsDev &d, const RunMeta &M, int
     return run_count_keys(d, 0, xs, xp, tests, [&](int64_t f) { return run_lb(d, M, g, f); });
 }
 
+// candidate slot sl is an outlier: its bit, and its slot in the list while
+// the list has room
+__device__ __forceinline__ void run_mark_outlier(uint32_t *bitmap, int32_t *olist, int32_t *nout,
+                                                 int sl)
+{
+    atomicOr(&bitmap[sl >> 5], 1u << (sl & 31));
+    const int k = atomicAdd(nout, 1);
+    if (k < kRunRankCap)
+        olist[k] = sl;
+}
+
 // the threads' pair-test and hit counts into the workgroup's totals in LDS:
 // summed per wave first, one atomic per wave (called by every thread)
 __device__ __forceinline__ void run_add_counters(unsigned long long tests,
@@ -719,12 +755,16 @@ __device__ __forceinline__ void run_add_counters(unsigned long long tests,
     }
 }
 
-// The outlier bitmap (ceil(W / 32) words in LDS) listed ascending into
-// slide i's host-out buffer with the slide's counters, then its stamp after
-// one system-scope release.  tests/hits: the workgroup's totals in LDS,
+// The slide's outliers listed ascending into slide i's host-out buffer
+// with the slide's counters, then its stamp after one system-scope release.
+// Up to kRunRankCap outliers (olist/nout, final like the bitmap at the
+// caller's last barrier) each go straight to their rank among the others;
+// more are listed from the bitmap (ceil(W / 32) words in LDS), a popcount
+// and a scan over all words.  tests/hits: the workgroup's totals in LDS,
 // final at the first barrier inside.  err: the slide's error flags (thread
 // 0's value); ts: the slide's stamps (LDS or HBM; none when null).
 __device__ __forceinline__ void run_publish(const RunsDev &d, int64_t i, const uint32_t *bitmap,
+                                            const int32_t *olist, const int32_t *nout,
                                             int32_t *scan, int ncand, int nranges,
                                             const unsigned long long *tests,
                                             const unsigned long long *hits, uint64_t *ts, int err)
@@ -734,20 +774,30 @@ __device__ __forceinline__ void run_publish(const RunsDev &d, int64_t i, const u
     if (ts && tid == 0)
         ts[kTsFinish] = wall_clock();
     HostOut *ho = run_host_out(d, i);
-    const int words = (d.W + 31) >> 5;
-    const int wpt = (words + NT - 1) / NT;
-    const int w0 = tid * wpt, w1 = min(words, w0 + wpt);
-    int mine = 0;
-    for (int w = w0; w < w1; ++w)
-        mine += __popc(bitmap[w]);
-    int total;
-    int o = run_scan(mine, scan, total);
-    for (int w = w0; w < w1; ++w) {
-        uint32_t b = bitmap[w];
-        while (b) {
-            const int bit = __ffs(b) - 1;
-            b &= b - 1;
-            ho->list[o++] = (w << 5) + bit;
+    int total = *nout;
+    if (total <= kRunRankCap) {
+        for (int k = tid; k < total; k += NT) {
+            const int sl = olist[k];
+            int rank = 0;
+            for (int j = 0; j < total; ++j)
+                rank += olist[j] < sl ? 1 : 0; // slots are distinct
+            ho->list[rank] = sl;
+        }
+    } else {
+        const int words = (d.W + 31) >> 5;
+        const int wpt = (words + NT - 1) / NT;
+        const int w0 = tid * wpt, w1 = min(words, w0 + wpt);
+        int mine = 0;
+        for (int w = w0; w < w1; ++w)
+            mine += __popc(bitmap[w]);
+        int o = run_scan(mine, scan, total);
+        for (int w = w0; w < w1; ++w) {
+            uint32_t b = bitmap[w];
+            while (b) {
+                const int bit = __ffs(b) - 1;
+                b &= b - 1;
+                ho->list[o++] = (w << 5) + bit;
+            }
         }
     }
     // every wave's list writes acknowledged, then one system-scope release
@@ -816,6 +866,7 @@ __device__ __forceinline__ void run_detect_body(const RunsDev &d, int64_t i, uin
     if (tid == 0) {
         L.tests = 0;
         L.hits = 0;
+        L.nout = 0;
     }
 
     // the window's runs: q = i + 1 .. i + m
@@ -938,16 +989,15 @@ __device__ __forceinline__ void run_detect_body(const RunsDev &d, int64_t i, uin
             for (int c = tid; c < ncs; c += NT) {
                 const int cnt = L.ccnt[c];
                 hits += (unsigned long long)min(cnt, d.K + 1);
-                if (cnt <= d.K) {
-                    const int sl = L.cslot[c];
-                    atomicOr(&L.bitmap[sl >> 5], 1u << (sl & 31));
-                }
+                if (cnt <= d.K)
+                    run_mark_outlier(L.bitmap, L.olist, &L.nout, L.cslot[c]);
             }
             __syncthreads();
         }
     }
     run_add_counters(tests, hits, &L.tests, &L.hits);
-    run_publish(d, i, L.bitmap, L.scan, ncand, nranges, &L.tests, &L.hits, ts, err);
+    run_publish(d, i, L.bitmap, L.olist, &L.nout, L.scan, ncand, nranges, &L.tests, &L.hits, ts,
+                err);
 }
 
 // ---------------------------------------------------------------------------
@@ -988,9 +1038,11 @@ struct RunSlideLds {
     int32_t scan[kRunNT / kWave];
     float redf[3][kRunNT / kWave];
     int32_t redi[3][kRunNT / kWave];
+    int32_t olist[kRunRankCap];            // the first outliers' slots, unordered
     int32_t nrange;
     int32_t err;                           // error flags raised by this workgroup
-    int32_t pad[2];
+    int32_t nout;                          // outliers found
+    int32_t pad;
     uint64_t ts[kTsCount];                 // the slide's stamps
     unsigned long long tests, hits;
     uint32_t bitmap[4];                    // ceil(W / 32) words follow (sized at launch)
@@ -1038,12 +1090,30 @@ __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, cons
     int err_in = 0;
     if (tid == 0)
         err_in = atomicExch(d.err, 0);
+    // The new points in 16-byte loads (a quarter of the requests over PCIe;
+    // 4-byte alignment suffices), thread t holding groups t and t + NT
+    // (points 4g..4g+3) and the tail point 4*n4 + t.  Every load is
+    // unconditional, its index clamped into range (with no full group the
+    // group loads read 16 discarded bytes of the run ring): the compiler's
+    // wait counts then know that three loads follow the descriptors' and
+    // wait for those alone before the descriptors are used (with loads
+    // under branches it waited for every point, i.e. for PCIe).
+    const int n4 = n >> 2;
+    const float *b4 = n4 > 0 ? src : d.rx;
+    const int gmax = n4 > 0 ? n4 - 1 : 0;
     float v[IT];
-#pragma unroll
-    for (int k = 0; k < IT; ++k) {
-        const int l = tid + k * NT;
-        v[k] = l < n ? src[l] : 0.f;
-    }
+    float4 va, vb;
+    __builtin_memcpy(&va, b4 + 4 * (tid < gmax ? tid : gmax), sizeof(float4));
+    __builtin_memcpy(&vb, b4 + 4 * (tid + NT < gmax ? tid + NT : gmax), sizeof(float4));
+    const float vt = src[4 * n4 + tid < n ? 4 * n4 + tid : n - 1];
+    v[0] = va.x;
+    v[1] = va.y;
+    v[2] = va.z;
+    v[3] = va.w;
+    v[4] = vb.x;
+    v[5] = vb.y;
+    v[6] = vb.z;
+    v[7] = vb.w;
     int ne = 0, cl = INT_MAX, ch = INT_MIN, srch = 0;
     if (tid < newr) {
         L.meta[tid] = M;
@@ -1098,19 +1168,23 @@ __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, cons
 
     // 1. the new points into LDS and their cell range (from the registers)
     float xmn = __builtin_inff(), xmx = -__builtin_inff(), nan = 0.f;
-#pragma unroll
-    for (int k = 0; k < IT; ++k) {
-        const int l = tid + k * NT;
-        if (l < n) {
-            L.u.x[l] = v[k];
-            if (v[k] != v[k]) {
+    auto take = [&](int l, bool ok, float x) {
+        if (ok) {
+            L.u.x[l] = x;
+            if (x != x) {
                 nan = 1.f;
             } else {
-                xmn = fminf(xmn, v[k]);
-                xmx = fmaxf(xmx, v[k]);
+                xmn = fminf(xmn, x);
+                xmx = fmaxf(xmx, x);
             }
         }
+    };
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+        const int g = k < 4 ? tid : tid + NT;
+        take(4 * g + (k & 3), g < n4, v[k]);
     }
+    take(4 * n4 + tid, 4 * n4 + tid < n, vt);
     xmn = wave_min_f32(xmn);
     xmx = wave_max_f32(xmx);
     nan = wave_max_f32(nan);
@@ -1161,6 +1235,7 @@ __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, cons
         L.tests = 0;
         L.hits = 0;
         L.nrange = 0;
+        L.nout = 0;
     }
     const int words = (d.W + 31) >> 5;
     for (int w = tid; w < words; w += NT)
@@ -1254,15 +1329,14 @@ __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, cons
         for (int c = tid; c < ncs; c += NT) {
             const int cnt = L.u.d.ccnt[c];
             hits += (unsigned long long)min(cnt, d.K + 1);
-            if (cnt <= d.K) {
-                const int sl = L.u.d.cslot[c];
-                atomicOr(&L.bitmap[sl >> 5], 1u << (sl & 31));
-            }
+            if (cnt <= d.K)
+                run_mark_outlier(L.bitmap, L.olist, &L.nout, L.u.d.cslot[c]);
         }
         __syncthreads();
     }
     run_add_counters(tests, hits, &L.tests, &L.hits);
-    run_publish(d, i, L.bitmap, L.scan, nc, nrg, &L.tests, &L.hits, ts, err_in | L.err);
+    run_publish(d, i, L.bitmap, L.olist, &L.nout, L.scan, nc, nrg, &L.tests, &L.hits, ts,
+                err_in | L.err);
 
     // 6. the new run into HBM for the slides after this one (the stream
     //    orders their launches after this one's end)

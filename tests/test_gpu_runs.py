@@ -94,6 +94,18 @@ def test_every_point_an_outlier_and_one_run_window():
     check(xs, W, S, T, 0.2, 10, "S = W")
 
 
+@pytest.mark.parametrize("S", [1, 2, 3, 5, 7, 13])
+def test_tiny_and_ragged_slides(S):
+    """Slides of fewer than four points (no full 16-byte group: the group
+    loads are discarded) and of lengths that leave a tail of one to three
+    points after the groups."""
+    case(f"slides of {S} points")
+    rng = np.random.default_rng(20 + S)
+    W, T = 64 * S, 12
+    xs = np.cumsum(rng.normal(0, 0.2, W + T * S)).astype(np.float32)[:, None]
+    check(xs, W, S, T, 0.45, 4, f"S={S}")
+
+
 def test_two_runs_per_slide():
     """S = 16384 (two runs of 8192 per slide): the one-per-call path is a
     sort launch and a detect launch."""
