@@ -1355,7 +1355,8 @@ std::string err_text(int32_t err)
                  {kErrMergeNew, "a window-merge new point ranked out of range"},
                  {kErrStencilRows, "a detect unit's stencil exceeded 32 rows"},
                  {kErrClassifyWait, "a detect timed out waiting for its classify"},
-                 {kErrLbvhWalk, "an LBVH walk overflowed its node stack (malformed tree)"}};
+                 {kErrLbvhWalk, "an LBVH walk overflowed its node stack (malformed tree)"},
+                 {kErrRunBucket, "a slide-run point's bucket fell outside its run's range"}};
     std::string s;
     for (const auto &m : kMsgs)
         if (err & m.bit)
