@@ -94,6 +94,31 @@ def test_every_point_an_outlier_and_one_run_window():
     check(xs, W, S, T, 0.2, 10, "S = W")
 
 
+def test_outlier_counts_around_the_ranked_list():
+    """Windows whose outlier count steps through 256: a slide with up to 256
+    outliers lists them by rank, one with more from the bitmap.  Each slide
+    holds a dense cluster (inliers) and isolated points 0.5 apart (outliers,
+    each alone within R = 0.45), their number rising every other slide so
+    that the window's count goes from 250 to 340; the window stays within
+    the one-slide workgroup's cell histogram."""
+    case("outlier counts around 256")
+    rng = np.random.default_rng(22)
+    W, S, T = 2000, 200, 24
+    m = W // S
+    slides, nxt = [], 0
+    for t in range(m + T):
+        k = 25 + max(0, t - m) // 2  # isolated points in slide t
+        iso = 2.0 + 0.5 * np.arange(nxt, nxt + k)
+        nxt += k
+        dense = rng.uniform(0.0, 0.1, S - k)
+        pts = np.concatenate([iso, dense])
+        slides.append(rng.permutation(pts))
+    xs = np.concatenate(slides).astype(np.float32)[:, None]
+    want = check(xs, W, S, T, 0.45, 5, "ranked list")
+    counts = [len(w) for w in want]
+    assert min(counts) <= 256 < max(counts), counts
+
+
 @pytest.mark.parametrize("S", [1, 2, 3, 5, 7, 13])
 def test_tiny_and_ragged_slides(S):
     """Slides of fewer than four points (no full 16-byte group: the group
