@@ -1970,9 +1970,8 @@ rtod_status runs_prime(rtod_ctx *c, const float *pts, hipMemcpyKind kind)
     RTOD_HIP(hipMemsetAsync(c->rd.err, 0, sizeof(int32_t), s));
     RunJob job{};
     job.nsort = c->m;
-    job.prime = 1;
     job.sort_q0 = 0;
-    job.src[0] = c->d_prime;
+    job.src = c->d_prime; // batch k at d_prime + k*S
     runs_launch(c, s, job);
     RTOD_HIP(hipGetLastError());
     RTOD_HIP(hipStreamSynchronize(s));
@@ -1995,7 +1994,7 @@ void runs_launch_slide(rtod_ctx *c, hipStream_t s, const float *src, bool timed)
     job.stamps = 1;
     job.sort_q0 = c->m + t;
     job.det_i0 = t;
-    job.src[0] = src;
+    job.src = src;
     if (timed)
         (void)hipEventRecord(c->ev[0], s);
     if (c->rd.rps == 1) {
@@ -2096,9 +2095,15 @@ rtod_status runs_slides(rtod_ctx *c, const SlideSrc &src, int32_t n, int32_t *of
     auto sort_job = [&](RunJob &job, int32_t first, int32_t count) {
         job.nsort = count;
         job.sort_q0 = c->m + t0 + first;
-        for (int32_t k = 0; k < count; ++k) {
+        for (int32_t k = 0; k < count; ++k)
             stage_slide(c, src, t0, first + k);
-            job.src[k] = slide_src(c, src, t0, first + k);
+        if (src.dev) { // slide i of the run at dev + i*S (slide_src)
+            job.src = src.dev;
+            job.src_i0 = first;
+        } else {       // staged in slot (t0 + i) mod stage_slots
+            job.src = c->h_in;
+            job.src_i0 = t0 + first;
+            job.src_ring = c->stage_slots;
         }
     };
     rtod_status st = RTOD_OK;
@@ -2169,7 +2174,9 @@ rtod_status runs_phase(rtod_ctx *c, bool detect)
     } else {
         job.nsort = 1;
         job.sort_q0 = c->m + c->host_t;
-        job.src[0] = c->h_in + (size_t)(c->host_t % c->stage_slots) * c->S;
+        job.src = c->h_in;
+        job.src_i0 = c->host_t;
+        job.src_ring = c->stage_slots;
     }
     runs_launch(c, c->stream, job);
     RTOD_HIP(hipGetLastError());

@@ -122,14 +122,26 @@ struct RunsDev {
 struct RunJob {
     int nsort;
     int ndet;
-    int prime;              // sort the initial window: batch k from src[0] + k*S
     int fused;
     int stamps;             // one slide per call: stamp its phases
+    int src_ring;           // the sorted slides' points: run_job_src
     int pad;
     int64_t sort_q0;        // run id of the first slide sorted
     int64_t det_i0;         // first slide detected (0-based)
-    const float *src[kRunBatch]; // each sorted slide's S new points
+    int64_t src_i0;         // the sorted slides' points: run_job_src
+    const float *src;
 };
+
+// The points of the job's k-th sorted slide: src + ((src_i0 + k) mod
+// src_ring) * S in a staging ring (src_ring > 0), else src + (src_i0 + k) * S
+// (a slide run from HBM, the initial window's batches, one slide per call).
+// A job's description stays a few words however many slides it sorts (the
+// launch's arguments: a larger block made each launch call slower).
+__device__ __forceinline__ const float *run_job_src(const RunsDev &d, const RunJob &job, int k)
+{
+    const int64_t i = job.src_i0 + k;
+    return job.src + (size_t)(job.src_ring > 0 ? i % job.src_ring : i) * d.S;
+}
 
 __device__ __forceinline__ HostOut *run_host_out(const RunsDev &d, int64_t i)
 {
@@ -1355,7 +1367,8 @@ enum RunRole { kRoleSlide, kRoleStage, kRoleSort, kRoleDetect };
 __device__ __forceinline__ void run_slide_hbm(const RunsDev &d, const RunJob &job, uint64_t *ts,
                                            unsigned char *dyn_lds)
 {
-    run_sort_body(d, job.sort_q0, 0, job.src[0], ts, *reinterpret_cast<RunSortLds *>(dyn_lds));
+    run_sort_body(d, job.sort_q0, 0, run_job_src(d, job, 0), ts,
+                  *reinterpret_cast<RunSortLds *>(dyn_lds));
     // the run's stores reach L2 before any wave of this workgroup reads
     // them back (the detect's loads go through this CU's L1)
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -1379,7 +1392,7 @@ __device__ __forceinline__ void runs_launch_body(const RunsDev &d, const RunJob 
         uint64_t t_in = 0;
         if (job.stamps && threadIdx.x == 0)
             t_in = wall_clock();
-        if (run_slide_body(d, job.det_i0, job.src[0], job.stamps != 0, t_in,
+        if (run_slide_body(d, job.det_i0, run_job_src(d, job, 0), job.stamps != 0, t_in,
                            *reinterpret_cast<RunSlideLds *>(dyn_lds)))
             return;
         if (ts && threadIdx.x == 0)
@@ -1392,7 +1405,7 @@ __device__ __forceinline__ void runs_launch_body(const RunsDev &d, const RunJob 
     if (ROLE != kRoleDetect && bid < nsw) {
         const int k = bid / d.rps, part = bid - k * d.rps;
         const int64_t q = job.sort_q0 + k;
-        const float *src = job.prime ? job.src[0] + (size_t)k * d.S : job.src[k];
+        const float *src = run_job_src(d, job, k);
         if (ts && bid == 0 && threadIdx.x == 0)
             ts[kTsIngest] = wall_clock();
         run_sort_body(d, q, part, src, bid == 0 ? ts : nullptr,
