@@ -12,7 +12,10 @@ outlier list out in host memory.
            into pinned staging inside the timed call (the reference's
            per-slide H2D copy is inside its timed loop, optixScan.cpp:1051-
            1058) and every list is in host memory when it returns.  --warmup
-           slides go through the same call first, untimed.
+           slides go through the same call first, untimed.  Before it, one
+           untimed run of the same path on a context of its own warms the
+           process (a process's first such run measured 0.1-0.3 us per slide
+           slower than the next, fresh context or not).
   latency_us_per_slide
            the same slides one rtod_slide call each (new slide in host
            memory, list in host memory when the call returns), the mean of
@@ -622,7 +625,9 @@ def main() -> int:
 
     ix = {"auto": rtod_amd.INDEX_AUTO, "runs": rtod_amd.INDEX_SLIDE_RUNS,
           "stencil": rtod_amd.INDEX_CELL_STENCIL, "lbvh": rtod_amd.INDEX_LBVH}[args.index]
-    # the measured value: host-fed pipelined slides
+    # the measured value: host-fed pipelined slides, after one untimed run of
+    # the same path on a context of its own (the process's first)
+    host_pipelined(xs, W, S, R, K, args.warmup, args.steps, index=ix)
     hp = host_pipelined(xs, W, S, R, K, args.warmup, args.steps, index=ix)
     log(f"[bench] host-fed pipelined (rtod_slides): {hp['us']:.2f} us/slide")
     # one call per slide, host input: latency and the reference's buckets
