@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <emmintrin.h>
 #include <mutex>
 #include <new>
 #include <string>
@@ -57,6 +58,33 @@ double now_ms()
 {
     using namespace std::chrono;
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Copies n floats into pinned staging with streaming (non-temporal) stores:
+// the lines go to memory without first being read for ownership, and the
+// device reads them over PCIe from there.  The fence orders the stores before
+// the launch that reads the slot.  On MI355X's host this staged a 20 KB slide
+// about 0.1 us faster per slide in the pipelined loop than std::memcpy; one
+// slide per call, read by the launch right after the copy, it measured 0.5 us
+// slower, so the one-slide calls keep std::memcpy.
+void stream_copy(float *dst, const float *src, size_t n)
+{
+    size_t k = 0;
+    for (; k < n && (reinterpret_cast<uintptr_t>(dst + k) & 15) != 0; ++k)
+        dst[k] = src[k];
+    for (; k + 16 <= n; k += 16) {
+        const __m128i a = _mm_loadu_si128(reinterpret_cast<const __m128i *>(src + k));
+        const __m128i b = _mm_loadu_si128(reinterpret_cast<const __m128i *>(src + k + 4));
+        const __m128i e = _mm_loadu_si128(reinterpret_cast<const __m128i *>(src + k + 8));
+        const __m128i f = _mm_loadu_si128(reinterpret_cast<const __m128i *>(src + k + 12));
+        _mm_stream_si128(reinterpret_cast<__m128i *>(dst + k), a);
+        _mm_stream_si128(reinterpret_cast<__m128i *>(dst + k + 4), b);
+        _mm_stream_si128(reinterpret_cast<__m128i *>(dst + k + 8), e);
+        _mm_stream_si128(reinterpret_cast<__m128i *>(dst + k + 12), f);
+    }
+    for (; k < n; ++k)
+        dst[k] = src[k];
+    _mm_sfence();
 }
 
 enum KernelId {
@@ -1745,8 +1773,8 @@ void stage_slide(rtod_ctx *c, const SlideSrc &src, int64_t t0, int32_t i)
     if (!src.host)
         return;
     const size_t per = (size_t)c->S * c->dim;
-    std::memcpy(c->h_in + (size_t)((t0 + i) % c->stage_slots) * per,
-                src.host + (size_t)i * per, per * sizeof(float));
+    stream_copy(c->h_in + (size_t)((t0 + i) % c->stage_slots) * per, src.host + (size_t)i * per,
+                per);
 }
 
 // n slides from HBM or host memory on the context's stream, up to kHostOuts
