@@ -988,8 +988,11 @@ __device__ __forceinline__ void run_detect_body(const RunsDev &d, int64_t i, uin
             }
             __syncthreads();
             const int P = ncs * nr;
+            // run-major: a wave's lanes take consecutive candidates against one
+            // run, whose bucket starts and boundary points then share cache
+            // lines (candidate-major, every lane gathered from another run)
             for (int p = tid; p < P; p += NT) {
-                const int c = p / nr, r = p - c * nr;
+                const int r = p / ncs, c = p - r * ncs;
                 const float xp = L.cx[c];
                 if (!__builtin_isfinite(xp)) // NaN and +-inf have no neighbour, not even themselves
                     continue;
@@ -1327,8 +1330,8 @@ __device__ __forceinline__ bool run_slide_body(const RunsDev &d, int64_t i, cons
         }
         __syncthreads();
         const int P = ncs * nr;
-        for (int p = tid; p < P; p += NT) {
-            const int c = p / nr, r = p - c * nr;
+        for (int p = tid; p < P; p += NT) { // run-major, as in run_detect_body
+            const int r = p / ncs, c = p - r * ncs;
             const float xp = L.u.d.cx[c];
             if (!__builtin_isfinite(xp)) // NaN and +-inf have no neighbour, not even themselves
                 continue;
