@@ -639,16 +639,21 @@ def main() -> int:
     mism = same(hp["lists"], sc["lists"]) + same(dp["lists"], sc["lists"])
 
     # the dominant kernel of the measured path, HIP events around each launch
-    timed = device_pipelined(xs, W, S, R, K, min(args.warmup, 10), min(args.steps, 64),
+    n_roof = min(args.steps, 160)
+    timed = device_pipelined(xs, W, S, R, K, min(args.warmup, 10), n_roof,
                              index=ix, flags=rtod_amd.FLAG_PHASE_TIMES)
     kern = {k: v[0] for k, v in timed["kernels"].items() if v[1] > 0}
     launches = {k: v[1] for k, v in timed["kernels"].items() if v[1] > 0}
     dom = max(kern, key=lambda k: kern[k] * launches[k])
     dom_ms = kern[dom]
-    # slides one launch of the dominant kernel detects (the timed pass runs
-    # min(steps, 64) slides, two full launches of 32 on the slide-run index;
-    # every one of them is detected by that kernel)
-    per_launch = min(args.steps, 64) / launches[dom]
+    # slides one launch of the dominant kernel detects: on the slide-run
+    # index a full launch (run_stage) detects RUN_BATCH slides and sorts the
+    # next RUN_BATCH (the run's last launch, which only detects, is timed as
+    # run_detect); one slide per launch on the cell stencil
+    if dom == "run_stage":
+        per_launch = float(rtod_amd.RUN_BATCH)
+    else:
+        per_launch = n_roof / launches[dom]
     n_out = float(np.mean([len(x) for x in sc["lists"]]))
     b_slide = 4 * dim * W + 4 * dim * S + 4 * n_out
     b_launch = b_slide * per_launch

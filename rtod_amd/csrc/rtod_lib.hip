@@ -102,7 +102,8 @@ enum KernelId {
                          // window merge + run merge + tile sorts
     kKSlideStage,        // one launch per slide (plain): classify + the above
     kKRunSort,           // slide-run index: a slide's runs sorted (two launches per slide)
-    kKRunDetect,         // slide-run index: a slide detected (two launches per slide)
+    kKRunDetect,         // slide-run index: a slide detected (two launches per slide),
+                         // or a pipelined run's last launch (its last slides detected)
     kKRunSlide,          // slide-run index: one slide sorted and detected, one launch
     kKRunStage,          // slide-run index, pipelined: the detects of kRunBatch slides
                          // and the sorts of the next kRunBatch
@@ -2173,8 +2174,11 @@ rtod_status runs_slides(rtod_ctx *c, const SlideSrc &src, int32_t n, int32_t *of
             } else {
                 float ms = 0.f;
                 (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
-                c->ksum[kKRunStage] += ms;
-                c->kcnt[kKRunStage]++;
+                // a full launch (k_run_stage), or the run's last, which
+                // only detects (k_run_detect)
+                const int kid = job.nsort ? kKRunStage : kKRunDetect;
+                c->ksum[kid] += ms;
+                c->kcnt[kid]++;
             }
         }
     }

@@ -209,6 +209,27 @@ def test_launch_modes_agree(flags, index_kind):
     assert_same(got, want, f"flags={flags}")
 
 
+def test_pipelined_phase_times_count_full_launches():
+    """With RTOD_FLAG_PHASE_TIMES a pipelined run on the slide-run index
+    times each full launch (RUN_BATCH slides detected, the next RUN_BATCH
+    sorted) as run_stage and its last launch, which only detects, as
+    run_detect; the lists are the oracle's."""
+    case("pipelined phase times")
+    wl = O.C1
+    xs = wl.data()
+    n = 3 * R.RUN_BATCH + 5
+    want = O.run_stream(xs, wl.W, wl.S, n, wl.R, wl.K, "grid")
+    with R.DeviceStream(xs[: wl.W + n * wl.S]) as ds, \
+            R.Detector(1, wl.W, wl.S, wl.R, wl.K, flags=R.FLAG_PHASE_TIMES,
+                       index_kind=R.INDEX_SLIDE_RUNS) as det:
+        det.prime(xs[:wl.W])
+        got = det.slides_device(ds.at(wl.W), n)
+        kt = det.kernel_times()
+    assert kt["run_stage"][1] == 3 and kt["run_stage"][0] > 0
+    assert kt["run_detect"][1] == 1
+    assert_same(got, want, "pipelined phase times")
+
+
 def test_slide_device_equals_slide():
     case("slide_device")
     W, S, T, K = 6000, 300, 15, 20

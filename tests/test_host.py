@@ -108,6 +108,14 @@ def test_struct_layouts_match_header():
             assert getattr(py, f).offset == got[(cs, f)], (cs, f)
 
 
+def test_run_batch_matches_the_kernels():
+    """RUN_BATCH (slides per full pipelined launch, used to price the
+    launch's bytes) is the kernels' kRunBatch."""
+    with open(os.path.join(ROOT, "rtod_amd", "csrc", "rtod_runs.h")) as f:
+        m = re.search(r"constexpr int kRunBatch = (\d+);", f.read())
+    assert m and int(m.group(1)) == R.RUN_BATCH
+
+
 @pytest.mark.parametrize("kw,status", [
     # the learned schedule orders cell-stencil range counts only
     (dict(index_kind=R.INDEX_LBVH), 2),
