@@ -21,6 +21,10 @@ outlier list out in host memory.
            memory, list in host memory when the call returns), the mean of
            the library's wall time per call, with the reference's five Timer
            buckets (timer.h:16-24) from the kernels' device stamps.
+  pipelined_hbm_us_per_slide, pipelined_pinned_us_per_slide
+           the same pipelined slides already in HBM (rtod_slides_device),
+           and in pinned host memory from rtod_host_alloc, which the
+           kernels read in place over PCIe (no staging copy).
 
 Extra objects on the line:
   roofline      the dominant kernel of the measured path (the pipelined
@@ -114,6 +118,28 @@ def host_pipelined(xs, W, S, R, K, warmup, steps, sched=0, index=0, flags=0):
         lists = det.split(*out)
         st1 = det.stats()
     return {"us": dt / steps * 1e6, "lists": lists, "stats": st1, "stats_warm": st0}
+
+
+def pinned_pipelined(xs, W, S, R, K, warmup, steps, index=0):
+    """rtod_slides from a buffer of rtod_host_alloc: the kernels read the
+    slides in place over PCIe, with no staging copy (the rows are copied into
+    the buffer before the timed call)."""
+    import rtod_amd
+
+    dim = xs.shape[1]
+    buf = rtod_amd.pinned_empty(steps * S, dim)
+    with rtod_amd.Detector(dim, W, S, R, K, index_kind=index) as det:
+        det.prime(xs[:W])
+        if warmup:
+            det.slides(xs[W: W + warmup * S], warmup)
+        buf[:] = xs[W + warmup * S: W + (warmup + steps) * S]
+        out = det._out_arrays(steps)
+        t0 = time.perf_counter()
+        det.slides_raw(buf, steps, out)  # the C-ABI call: slides read in place
+        dt = time.perf_counter() - t0
+        lists = det.split(*out)
+    del buf
+    return {"us": dt / steps * 1e6, "lists": lists}
 
 
 def device_pipelined(xs, W, S, R, K, warmup, steps, sched=0, index=0, flags=0):
@@ -636,7 +662,11 @@ def main() -> int:
         f"(outer {sc['outer_us']:.2f}); buckets {sc['buckets_ms']}; stages {sc['stages_ms']}")
     dp = device_pipelined(xs, W, S, R, K, args.warmup, args.steps, index=ix)
     log(f"[bench] HBM-fed pipelined (rtod_slides_device): {dp['us']:.2f} us/slide")
-    mism = same(hp["lists"], sc["lists"]) + same(dp["lists"], sc["lists"])
+    pp = pinned_pipelined(xs, W, S, R, K, args.warmup, args.steps, index=ix)
+    log(f"[bench] host-fed pipelined from rtod_host_alloc memory (read in place): "
+        f"{pp['us']:.2f} us/slide")
+    mism = (same(hp["lists"], sc["lists"]) + same(dp["lists"], sc["lists"]) +
+            same(pp["lists"], sc["lists"]))
 
     # the dominant kernel of the measured path, HIP events around each launch
     n_roof = min(args.steps, 160)
@@ -732,6 +762,7 @@ def main() -> int:
         "breakdown_ms": sc["buckets_ms"],
         "stages_ms": sc["stages_ms"],
         "pipelined_hbm_us_per_slide": round(dp["us"], 3),
+        "pipelined_pinned_us_per_slide": round(pp["us"], 3),
         "outliers_per_slide": round(n_out, 2),
         "candidates_per_slide": round((sc["stats"]["candidates_total"] -
                                        sc["stats_warm"]["candidates_total"]) / args.steps, 1),

@@ -244,6 +244,19 @@ RTOD_API rtod_status rtod_slides_device(rtod_ctx *ctx, const float *dev_pts, int
 RTOD_API rtod_status rtod_slides(rtod_ctx *ctx, const float *pts, int32_t n_slides,
                                  int32_t *out_offsets, int32_t *out_slots, int64_t slots_cap);
 
+/* Pinned (page-locked) host memory mapped for every device: bytes > 0, the
+ * buffer at *out.  A slide that rtod_slide finds inside such a buffer, or a
+ * run of slides that rtod_slides finds wholly inside one, is read by the
+ * kernels in place over PCIe, with no copy into the context's staging; the
+ * caller leaves it unchanged until the call returns.  rtod_host_free
+ * releases a buffer from rtod_host_alloc (NULL: nothing to do; any other
+ * pointer: RTOD_ERR_INVALID_ARGUMENT).  No counterpart in the reference,
+ * whose loop copies each slide from pageable memory
+ * (optixScan.cpp:1051-1058); for callers that can receive their points
+ * straight into such a buffer (a socket or file read, a ring of slides). */
+RTOD_API rtod_status rtod_host_alloc(int64_t bytes, void **out);
+RTOD_API rtod_status rtod_host_free(void *p);
+
 /* Phase-level entry points: the same slide split where the reference's loop
  * body calls out.  Call in this order; rtod_slide == all five.
  *   rtod_copy_new_points  <- cudaMemcpy of the new slide (optixScan.cpp:1051-1058)

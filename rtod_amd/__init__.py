@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -94,6 +95,8 @@ SIGNATURES = [
     ("rtod_slides_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, _I32P, _I32P,
                                      C.c_int64]),
     ("rtod_slides", C.c_int, [C.c_void_p, _F32P, C.c_int32, _I32P, _I32P, C.c_int64]),
+    ("rtod_host_alloc", C.c_int, [C.c_int64, C.POINTER(C.c_void_p)]),
+    ("rtod_host_free", C.c_int, [C.c_void_p]),
     ("rtod_copy_new_points", C.c_int, [C.c_void_p, _F32P]),
     ("rtod_prepare_cell", C.c_int, [C.c_void_p]),
     ("rtod_build_index", C.c_int, [C.c_void_p]),
@@ -336,6 +339,19 @@ def run_stream(xs: np.ndarray, W: int, S: int, T: int, R: float, K: int, **kw):
     with Detector(dim, W, S, R, K, **kw) as det:
         det.prime(xs[:W])
         return [det.slide(xs[W + (t - 1) * S: W + t * S]) for t in range(1, T + 1)]
+
+
+def pinned_empty(n_points: int, dim: int = 1) -> np.ndarray:
+    """A float32 [n_points][dim] array in pinned host memory (rtod_host_alloc,
+    released by rtod_host_free once no view of it is left).  Slides that
+    Detector.slide / slides / slides_raw take from it are read by the device
+    in place over PCIe, with no copy into the context's staging."""
+    n = int(n_points) * int(dim)
+    p = C.c_void_p()
+    _check(lib().rtod_host_alloc(n * 4, C.byref(p)))
+    buf = (C.c_float * n).from_address(p.value)
+    weakref.finalize(buf, lib().rtod_host_free, C.c_void_p(p.value))
+    return np.ctypeslib.as_array(buf).reshape(int(n_points), int(dim))
 
 
 class DeviceStream:

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstddef>
@@ -58,6 +59,31 @@ double now_ms()
 {
     using namespace std::chrono;
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Pinned host buffers from rtod_host_alloc: [base, base + bytes) and the
+// device's address of base (mapped memory)
+struct PinnedBuf {
+    const char *base;
+    size_t bytes;
+    const char *dev;
+};
+std::mutex g_pinned_mu;
+std::vector<PinnedBuf> g_pinned;
+std::atomic<int> g_pinned_n{0};
+
+// The device's address of host range [p, p + bytes) when it lies inside one
+// buffer from rtod_host_alloc, else nullptr (no lock while there is none).
+const float *pinned_device_ptr(const void *p, size_t bytes)
+{
+    if (g_pinned_n.load(std::memory_order_acquire) == 0)
+        return nullptr;
+    const char *q = static_cast<const char *>(p);
+    std::lock_guard<std::mutex> lk(g_pinned_mu);
+    for (const PinnedBuf &b : g_pinned)
+        if (q >= b.base && bytes <= b.bytes && (size_t)(q - b.base) <= b.bytes - bytes)
+            return reinterpret_cast<const float *>(b.dev + (q - b.base));
+    return nullptr;
 }
 
 // Copies n floats into pinned staging with streaming (non-temporal) stores:
@@ -1681,6 +1707,15 @@ rtod_status slide_impl(rtod_ctx *c, const float *new_pts, int32_t *outs, int32_t
     RTOD_TRY(slide_checks(c, new_pts, outs, n_out));
     RTOD_HIP(hipSetDevice(c->device));
     const size_t nf = (size_t)c->S * c->dim;
+    // a slide in a buffer from rtod_host_alloc is read in place over PCIe
+    if (const float *dp = pinned_device_ptr(new_pts, nf * sizeof(float))) {
+        c->dev_src = dp;
+        const rtod_status s = c->runs
+                                  ? runs_run_slide(c, dp, t_entry, t_entry, outs, n_out, times)
+                                  : run_slide(c, t_entry, t_entry, outs, n_out, times);
+        c->dev_src = nullptr;
+        return s;
+    }
     // a slide run one per call is staged in slot 0 on the slide-run index:
     // every launch that read a staging slot has completed by now, and the
     // device's translations of that slot's pages stay warm
@@ -2348,14 +2383,20 @@ RTOD_API rtod_status rtod_slides(rtod_ctx *ctx, const float *pts, int32_t n_slid
         set_error("a batch member's slides run through its batch");
         return RTOD_ERR_UNSUPPORTED;
     }
-    if (ctx->stage_slots < kStageSlots) {
+    // slides wholly inside a buffer from rtod_host_alloc are read in place
+    const float *dp = pinned_device_ptr(
+        pts, (size_t)n_slides * (size_t)ctx->S * (size_t)ctx->dim * sizeof(float));
+    if (!dp && ctx->stage_slots < kStageSlots) {
         set_error("slides over 64 MB are staged one at a time: use rtod_slide");
         return RTOD_ERR_UNSUPPORTED;
     }
     try {
         RTOD_HIP(hipSetDevice(ctx->device));
         SlideSrc src;
-        src.host = pts;
+        if (dp)
+            src.dev = dp;
+        else
+            src.host = pts;
         return slides_impl(ctx, src, n_slides, out_offsets, out_slots);
     } catch (...) {
         return RTOD_ERR_INTERNAL;
@@ -2831,6 +2872,59 @@ RTOD_API const char *rtod_status_str(rtod_status s)
 RTOD_API const char *rtod_last_error(void) { return g_last_error; }
 
 RTOD_API const char *rtod_version(void) { return "rtod-mi355x 0.2 (gfx950, sources " RTOD_SRC_HASH ")"; }
+
+RTOD_API rtod_status rtod_host_alloc(int64_t bytes, void **out)
+{
+    if (!out || bytes <= 0) {
+        set_error("rtod_host_alloc needs bytes > 0 and an output pointer");
+        return RTOD_ERR_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    void *p = nullptr;
+    hipError_t e = hipHostMalloc(&p, (size_t)bytes,
+                                 hipHostMallocMapped | hipHostMallocPortable |
+                                     hipHostMallocCoherent);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        set_error(std::string("hipHostMalloc: ") + hipGetErrorString(e));
+        return RTOD_ERR_NOMEM;
+    }
+    void *dev = nullptr;
+    e = hipHostGetDevicePointer(&dev, p, 0);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipHostFree(p);
+        set_error(std::string("hipHostGetDevicePointer: ") + hipGetErrorString(e));
+        return RTOD_ERR_HIP;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_pinned_mu);
+        g_pinned.push_back(PinnedBuf{static_cast<const char *>(p), (size_t)bytes,
+                                     static_cast<const char *>(dev)});
+        g_pinned_n.fetch_add(1, std::memory_order_release);
+    }
+    *out = p;
+    return RTOD_OK;
+}
+
+RTOD_API rtod_status rtod_host_free(void *p)
+{
+    if (!p)
+        return RTOD_OK;
+    {
+        std::lock_guard<std::mutex> lk(g_pinned_mu);
+        auto it = std::find_if(g_pinned.begin(), g_pinned.end(),
+                               [&](const PinnedBuf &b) { return b.base == p; });
+        if (it == g_pinned.end()) {
+            set_error("rtod_host_free: not a buffer from rtod_host_alloc");
+            return RTOD_ERR_INVALID_ARGUMENT;
+        }
+        g_pinned.erase(it);
+        g_pinned_n.fetch_sub(1, std::memory_order_release);
+    }
+    RTOD_HIP(hipHostFree(p));
+    return RTOD_OK;
+}
 
 } // extern "C"
 

@@ -230,6 +230,31 @@ def test_pipelined_phase_times_count_full_launches():
     assert_same(got, want, "pipelined phase times")
 
 
+@pytest.mark.parametrize("dim", [1, 3])
+def test_pinned_slides_read_in_place(dim):
+    """Slides in a buffer from rtod_host_alloc are read by the kernels in
+    place, pipelined and one per call, on the slide-run index (1-D) and the
+    cell stencil (3-D): the lists are the oracle's."""
+    case(f"pinned dim={dim}")
+    if dim == 1:
+        wl = O.C1
+        xs, W, S, Rr, K, n = wl.data(), wl.W, wl.S, wl.R, wl.K, 70
+    else:
+        W, S, Rr, K, n = 3000, 150, 1.9, 20, 24
+        xs = O.gen_gmm(81, W + n * S, 3)
+    want = O.run_stream(xs, W, S, n, Rr, K, "grid")
+    buf = R.pinned_empty(n * S, dim)
+    buf[:] = xs[W: W + n * S]
+    with R.Detector(dim, W, S, Rr, K) as det:
+        det.prime(xs[:W])
+        got = det.slides(buf, n)
+    assert_same(got, want, f"pinned pipelined dim={dim}")
+    with R.Detector(dim, W, S, Rr, K) as det:
+        det.prime(xs[:W])
+        got = [det.slide(buf[t * S: (t + 1) * S]) for t in range(n)]
+    assert_same(got, want, f"pinned one per call dim={dim}")
+
+
 def test_slide_device_equals_slide():
     case("slide_device")
     W, S, T, K = 6000, 300, 15, 20

@@ -72,6 +72,20 @@ def test_null_arguments_rejected():
     lib.rtod_destroy(None)  # no-op
 
 
+def test_host_buffer_arguments_checked_without_device():
+    """rtod_host_alloc refuses a size or output it cannot use before any HIP
+    call; rtod_host_free takes NULL and refuses a pointer it did not hand
+    out."""
+    lib = R.lib()
+    p = C.c_void_p()
+    assert lib.rtod_host_alloc(0, C.byref(p)) == 1 and not p.value
+    assert lib.rtod_host_alloc(-4, C.byref(p)) == 1
+    assert lib.rtod_host_alloc(64, None) == 1
+    assert lib.rtod_host_free(None) == 0
+    assert lib.rtod_host_free(C.c_void_p(0x1000)) == 1
+    assert b"rtod_host_alloc" in lib.rtod_last_error()
+
+
 def _header_layout():
     """sizeof and field offsets of the header's structs, as a C compiler sees
     include/rtod.h (gcc on a generated program)."""
