@@ -21,10 +21,12 @@ outlier list out in host memory.
            memory, list in host memory when the call returns), the mean of
            the library's wall time per call, with the reference's five Timer
            buckets (timer.h:16-24) from the kernels' device stamps.
-  pipelined_hbm_us_per_slide, pipelined_pinned_us_per_slide
+  pipelined_hbm_us_per_slide, pipelined_pinned_us_per_slide,
+  latency_pinned_us_per_slide
            the same pipelined slides already in HBM (rtod_slides_device),
            and in pinned host memory from rtod_host_alloc, which the
-           kernels read in place over PCIe (no staging copy).
+           kernels read in place over PCIe (no staging copy), pipelined and
+           one rtod_slide call each.
 
 Extra objects on the line:
   roofline      the dominant kernel of the measured path (the pipelined
@@ -164,21 +166,27 @@ def device_pipelined(xs, W, S, R, K, warmup, steps, sched=0, index=0, flags=0):
     return {"us": dt / steps * 1e6, "lists": lists, "kernels": kern}
 
 
-def single_calls(xs, W, S, R, K, warmup, steps, sched=0, index=0, flags=0):
+def single_calls(xs, W, S, R, K, warmup, steps, sched=0, index=0, flags=0, pinned=False):
     """One rtod_slide call per slide (host input): the library's wall time
-    per call, its buckets and kernel durations (device stamps)."""
+    per call, its buckets and kernel durations (device stamps).  pinned: the
+    slides in a buffer from rtod_host_alloc (read in place, no staging copy)."""
     import rtod_amd
 
     dim = xs.shape[1]
+    src, base = xs, W
+    if pinned:
+        src = rtod_amd.pinned_empty((warmup + steps) * S, dim)
+        src[:] = xs[W: W + (warmup + steps) * S]
+        base = 0
     with rtod_amd.Detector(dim, W, S, R, K, sched_kind=sched, index_kind=index,
                            flags=flags) as det:
         det.prime(xs[:W])
         lists, bk, stg, outer = [], [], [], 0.0
         st0 = None
         for t in range(1, warmup + steps + 1):
-            row = W + (t - 1) * S
+            row = base + (t - 1) * S
             t0 = time.perf_counter()
-            o = det.slide(xs[row: row + S])
+            o = det.slide(src[row: row + S])
             dt = time.perf_counter() - t0
             if t == warmup:
                 st0 = det.stats()
@@ -662,11 +670,13 @@ def main() -> int:
         f"(outer {sc['outer_us']:.2f}); buckets {sc['buckets_ms']}; stages {sc['stages_ms']}")
     dp = device_pipelined(xs, W, S, R, K, args.warmup, args.steps, index=ix)
     log(f"[bench] HBM-fed pipelined (rtod_slides_device): {dp['us']:.2f} us/slide")
+    scp = single_calls(xs, W, S, R, K, args.warmup, args.steps, index=ix, pinned=True)
+    log(f"[bench] one slide per call from rtod_host_alloc memory: {scp['us']:.2f} us/slide")
     pp = pinned_pipelined(xs, W, S, R, K, args.warmup, args.steps, index=ix)
     log(f"[bench] host-fed pipelined from rtod_host_alloc memory (read in place): "
         f"{pp['us']:.2f} us/slide")
     mism = (same(hp["lists"], sc["lists"]) + same(dp["lists"], sc["lists"]) +
-            same(pp["lists"], sc["lists"]))
+            same(pp["lists"], sc["lists"]) + same(scp["lists"], sc["lists"]))
 
     # the dominant kernel of the measured path, HIP events around each launch
     n_roof = min(args.steps, 160)
@@ -763,6 +773,7 @@ def main() -> int:
         "stages_ms": sc["stages_ms"],
         "pipelined_hbm_us_per_slide": round(dp["us"], 3),
         "pipelined_pinned_us_per_slide": round(pp["us"], 3),
+        "latency_pinned_us_per_slide": round(scp["us"], 3),
         "outliers_per_slide": round(n_out, 2),
         "candidates_per_slide": round((sc["stats"]["candidates_total"] -
                                        sc["stats_warm"]["candidates_total"]) / args.steps, 1),

@@ -2929,8 +2929,9 @@ RTOD_API rtod_status rtod_host_free(void *p)
 } // extern "C"
 
 // ===========================================================================
-// Batched streams: many contexts advanced together by one persistent kernel
-// (rtod_batch.h) replayed from a hipGraph.
+// Batched streams: many contexts advanced together, each phase one launch
+// over all streams of a dimensionality (rtod_batch.h), the step one hipGraph
+// replay.
 // ===========================================================================
 struct rtod_batch {
     int n = 0, device = 0;
