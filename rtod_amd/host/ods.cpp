@@ -46,6 +46,7 @@ struct Args {
     int optimization = 2;    // the reference's OPTIMIZATION (optixScan.h:22-24)
     int update_gas_type = 1; // the reference's UPDATE_GAS_TYPE (optixScan.h:18-20)
     bool pipelined = false;
+    bool pinned = false;
     int warmup = 10;
     std::string index = "auto", sched = "plain", gen, out;
     uint64_t seed = 1;
@@ -66,6 +67,8 @@ struct Args {
               << "         --update_gas_type 0|1     0 refit the whole-window LBVH, 1 rebuild (default)\n"
               << "         --index auto|runs|stencil|lbvh --sched plain|learned\n"
               << "         --pipelined               the timed pass as one pipelined call\n"
+              << "         --pinned                  hold the stream in pinned memory (rtod_host_alloc):\n"
+              << "                                   the kernels read the slides in place\n"
               << "         --warmup <int>            untimed passes (default 10)\n"
               << "         --gen walk|gmm --seed <u64>  synthetic stream instead of --file\n"
               << "         --out <file>              write each slide's outlier slots\n"
@@ -109,6 +112,8 @@ Args parse_args(int argc, char **argv)
             a.update_gas_type = std::stoi(need(i));
         else if (s == "--pipelined")
             a.pipelined = true;
+        else if (s == "--pinned")
+            a.pinned = true;
         else if (s == "--index")
             a.index = need(i);
         else if (s == "--sched")
@@ -227,6 +232,14 @@ int main(int argc, char **argv)
         return 1;
     }
     const float *base = pts.data() + (size_t)skip * a.dim;
+    // --pinned: the stream copied once into a buffer from rtod_host_alloc,
+    // whose slides rtod_slide / rtod_slides read in place (no staging copy)
+    float *pin = nullptr;
+    if (a.pinned) {
+        CHECK(rtod_host_alloc((int64_t)(pts.size() * sizeof(float)), (void **)&pin));
+        std::memcpy(pin, pts.data(), pts.size() * sizeof(float));
+        base = pin + (size_t)skip * a.dim;
+    }
     const long long avail = rows - skip;
     const long long T = std::min<long long>(10000, (avail - a.window) / a.slide);
 
@@ -321,7 +334,8 @@ int main(int argc, char **argv)
     CHECK(rtod_get_stats(ctx, &st));
     if (a.pipelined) {
         std::cout << "\n###########   Time  ##########\n"
-                  << "[Time] pipelined slides from host memory: " << pipe_ms / Td
+                  << "[Time] pipelined slides from host memory" << (a.pinned ? " (pinned)" : "")
+                  << ": " << pipe_ms / Td
                   << " ms per slide\n"
                   << "##############################\n\n";
     } else {
@@ -348,5 +362,6 @@ int main(int argc, char **argv)
         std::cout << "Predictor misses: " << st.predictor_misses << " / " << st.predictor_points
                   << ", fallbacks: " << st.fallbacks << "\n";
     rtod_destroy(ctx);
+    CHECK(rtod_host_free(pin));
     return 0;
 }

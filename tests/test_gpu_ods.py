@@ -76,12 +76,14 @@ def test_ods_text_input_every_slide(dim, tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("extra", [[], ["--pipelined"], ["--index", "lbvh"],
+@pytest.mark.parametrize("extra", [[], ["--pipelined"], ["--pinned"], ["--pipelined", "--pinned"],
+                                   ["--index", "lbvh"],
                                    ["--optimization", "0", "--index", "lbvh",
                                     "--update_gas_type", "0"],
                                    ["--optimization", "1", "--index", "lbvh"],
                                    ["--optimization", "0"]],
-                         ids=["default", "pipelined", "lbvh", "opt0-lbvh-refit", "opt1-lbvh",
+                         ids=["default", "pipelined", "pinned", "pipelined-pinned", "lbvh",
+                              "opt0-lbvh-refit", "opt1-lbvh",
                               "opt0-stencil"])
 def test_ods_gen_and_modes(extra, tmp_path):
     """--gen walk --seed 2 is the measured 1-D stream (the package's
