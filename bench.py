@@ -30,8 +30,9 @@ outlier list out in host memory.
 
 Extra objects on the line:
   roofline      the dominant kernel of the measured path (the pipelined
-                run's launch: 32 slides detected and 32 sorted per launch on
-                the slide-run index, one slide per launch on the cell
+                run's launch: a batch detected and the next sorted per launch
+                on the slide-run index (32, then 64 slides a batch), one
+                slide per launch on the cell
                 stencil), timed with HIP events on the library's stream in a
                 separate launch-by-launch pass, against the 8 TB/s HBM peak;
                 algorithmic bytes per launch = the slides it detects times
@@ -687,11 +688,14 @@ def main() -> int:
     dom = max(kern, key=lambda k: kern[k] * launches[k])
     dom_ms = kern[dom]
     # slides one launch of the dominant kernel detects: on the slide-run
-    # index a full launch (run_stage) detects RUN_BATCH slides and sorts the
-    # next RUN_BATCH (the run's last launch, which only detects, is timed as
-    # run_detect); one slide per launch on the cell stencil
+    # index the run's batches are RUN_FIRST_BATCH slides, then RUN_BATCH
+    # each, and every launch but the last (run_detect, which only detects)
+    # detects one batch and sorts the next (run_stage); one slide per launch
+    # on the cell stencil
     if dom == "run_stage":
-        per_launch = float(rtod_amd.RUN_BATCH)
+        rest = n_roof - rtod_amd.RUN_FIRST_BATCH
+        last = (rest % rtod_amd.RUN_BATCH or rtod_amd.RUN_BATCH) if rest > 0 else n_roof
+        per_launch = (n_roof - last) / launches[dom]
     else:
         per_launch = n_roof / launches[dom]
     n_out = float(np.mean([len(x) for x in sc["lists"]]))

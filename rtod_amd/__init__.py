@@ -31,9 +31,13 @@ SCHED_PLAIN, SCHED_LEARNED = 0, 1
 ABL_NONE, ABL_FILTER_ONLY, ABL_NO_FILTER = 0, 1, 2
 FLAG_PHASE_TIMES = 0x1
 FLAG_LEARN_ALWAYS = 0x2
-# slides a full pipelined launch detects (and sorts ahead) on the slide-run
-# index: kRunBatch of rtod_amd/csrc/rtod_runs.h
-RUN_BATCH = 32
+# a pipelined run's batches on the slide-run index: the first RUN_FIRST_BATCH
+# slides, then RUN_BATCH each from HBM or a pinned buffer (RUN_HOST_BATCH
+# from pageable host memory); a launch detects one batch and sorts the next
+# (kRunFirstBatch, kRunBatch and kRunHostBatch of rtod_amd/csrc/rtod_runs.h)
+RUN_FIRST_BATCH = 32
+RUN_BATCH = 64
+RUN_HOST_BATCH = 32
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "unsupported", 3: "call out of order",
           4: "HIP runtime error", 5: "out of memory", 6: "internal error"}

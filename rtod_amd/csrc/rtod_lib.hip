@@ -2130,18 +2130,24 @@ rtod_status runs_run_slide(rtod_ctx *c, const float *src, double t_entry, double
     return RTOD_OK;
 }
 
-// n consecutive slides, pipelined on the context's stream: after a
-// prologue launch sorting slides [0, B), launch j detects slides [jB, jB+B)
-// and sorts slides [jB+B, jB+2B) (B = kRunBatch), each slide's points
-// staged into pinned memory just before the launch that sorts it.  At most
-// two launches are in flight: launch j is enqueued once every slide before
-// (j-1)B is collected, so the staging slots, host-out buffers and runs that
-// launch j rewrites are no longer read.  Results equal those of n
-// rtod_slide calls.
+// n consecutive slides, pipelined on the context's stream, in batches (the
+// first kRunFirstBatch slides, then kRunBatch each, or kRunHostBatch from
+// pageable host memory): after a prologue launch
+// sorting batch 0, launch k detects batch k and sorts batch k + 1, each
+// slide's points staged into pinned memory just before the launch that sorts
+// it.  At most two launches are in flight: launch k is enqueued once every
+// batch before k - 1 is collected, so the staging slots, host-out buffers
+// and runs that launch k rewrites are no longer read.  Results equal those
+// of n rtod_slide calls.
 rtod_status runs_slides(rtod_ctx *c, const SlideSrc &src, int32_t n, int32_t *offs,
                         int32_t *slots)
 {
-    constexpr int B = kRunBatch;
+    // batch k: slides [first(k), first(k + 1)) of the run
+    const int64_t B = src.host ? kRunHostBatch : kRunBatch;
+    auto first = [n, B](int32_t k) -> int32_t {
+        const int64_t f = k == 0 ? 0 : kRunFirstBatch + (int64_t)(k - 1) * B;
+        return (int32_t)std::min<int64_t>(f, n);
+    };
     const int64_t t0 = c->host_t;
     const bool timed = (c->cfg.flags & RTOD_FLAG_PHASE_TIMES) != 0;
     hipStream_t s = c->stream;
@@ -2173,7 +2179,7 @@ rtod_status runs_slides(rtod_ctx *c, const SlideSrc &src, int32_t n, int32_t *of
     rtod_status st = RTOD_OK;
     {
         RunJob pj{};
-        sort_job(pj, 0, std::min(B, n));
+        sort_job(pj, 0, first(1));
         runs_launch(c, s, pj);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
@@ -2181,17 +2187,17 @@ rtod_status runs_slides(rtod_ctx *c, const SlideSrc &src, int32_t n, int32_t *of
             st = RTOD_ERR_HIP;
         }
     }
-    for (int32_t j0 = 0; j0 < n && st == RTOD_OK; j0 += B) {
-        while (st == RTOD_OK && done < j0 - B)
+    for (int32_t k = 0; first(k) < n && st == RTOD_OK; ++k) {
+        while (st == RTOD_OK && k >= 1 && done < first(k - 1))
             st = collect_next();
         if (st != RTOD_OK)
             break;
+        const int32_t j0 = first(k), j1 = first(k + 1);
         RunJob job{};
-        job.ndet = std::min(B, n - j0);
+        job.ndet = j1 - j0;
         job.det_i0 = t0 + j0;
-        const int32_t s0 = j0 + B;
-        if (s0 < n)
-            sort_job(job, s0, std::min(B, n - s0));
+        if (j1 < n)
+            sort_job(job, j1, first(k + 2) - j1);
         if (timed)
             (void)hipEventRecord(c->ev[2], s);
         runs_launch(c, s, job);

@@ -211,13 +211,13 @@ def test_launch_modes_agree(flags, index_kind):
 
 def test_pipelined_phase_times_count_full_launches():
     """With RTOD_FLAG_PHASE_TIMES a pipelined run on the slide-run index
-    times each full launch (RUN_BATCH slides detected, the next RUN_BATCH
-    sorted) as run_stage and its last launch, which only detects, as
-    run_detect; the lists are the oracle's."""
+    times each launch that detects a batch and sorts the next as run_stage
+    and its last launch, which only detects, as run_detect; the lists are
+    the oracle's."""
     case("pipelined phase times")
     wl = O.C1
     xs = wl.data()
-    n = 3 * R.RUN_BATCH + 5
+    n = R.RUN_FIRST_BATCH + 2 * R.RUN_BATCH + 5  # batches 32, 64, 64, 5
     want = O.run_stream(xs, wl.W, wl.S, n, wl.R, wl.K, "grid")
     with R.DeviceStream(xs[: wl.W + n * wl.S]) as ds, \
             R.Detector(1, wl.W, wl.S, wl.R, wl.K, flags=R.FLAG_PHASE_TIMES,

@@ -176,10 +176,11 @@ def test_nonfinite_clamped_and_duplicates():
     check(xs, W, S, T, 0.3, 10, "edge data")
 
 
-@pytest.mark.parametrize("n", [1, 31, 32, 33, 64, 65, 97])
+@pytest.mark.parametrize("n", [1, 31, 32, 33, 64, 95, 96, 97, 159, 160, 161, 225])
 def test_pipelined_run_lengths(n):
-    """Runs of n slides around the launch batch (32 slides detected and 32
-    sorted per launch), host-fed and from HBM, then single calls on the same
+    """Runs of n slides around the batches (the first 32 slides, then 32 a
+    batch from host memory and 64 from HBM; a launch detects one batch and
+    sorts the next), host-fed and from HBM, then single calls on the same
     context."""
     case(f"pipelined run of {n}")
     wl = O.C1
