@@ -126,21 +126,25 @@ def host_pipelined(xs, W, S, R, K, warmup, steps, sched=0, index=0, flags=0):
 def pinned_pipelined(xs, W, S, R, K, warmup, steps, index=0):
     """rtod_slides from a buffer of rtod_host_alloc: the kernels read the
     slides in place over PCIe, with no staging copy (the rows are copied into
-    the buffer before the timed call)."""
+    the buffer before the timed call).  The buffer is used as a receive
+    buffer is, again and again: a first, untimed run on a context of its own
+    reads it once (the device's first pass over fresh pinned pages measured
+    0.1-0.2 us per slide slower)."""
     import rtod_amd
 
     dim = xs.shape[1]
     buf = rtod_amd.pinned_empty(steps * S, dim)
-    with rtod_amd.Detector(dim, W, S, R, K, index_kind=index) as det:
-        det.prime(xs[:W])
-        if warmup:
-            det.slides(xs[W: W + warmup * S], warmup)
-        buf[:] = xs[W + warmup * S: W + (warmup + steps) * S]
-        out = det._out_arrays(steps)
-        t0 = time.perf_counter()
-        det.slides_raw(buf, steps, out)  # the C-ABI call: slides read in place
-        dt = time.perf_counter() - t0
-        lists = det.split(*out)
+    buf[:] = xs[W + warmup * S: W + (warmup + steps) * S]
+    for _ in range(2):  # the second run is the one reported
+        with rtod_amd.Detector(dim, W, S, R, K, index_kind=index) as det:
+            det.prime(xs[:W])
+            if warmup:
+                det.slides(xs[W: W + warmup * S], warmup)
+            out = det._out_arrays(steps)
+            t0 = time.perf_counter()
+            det.slides_raw(buf, steps, out)  # the C-ABI call: slides read in place
+            dt = time.perf_counter() - t0
+            lists = det.split(*out)
     del buf
     return {"us": dt / steps * 1e6, "lists": lists}
 
@@ -765,9 +769,10 @@ def main() -> int:
         "data": "synthetic: seeded 1-D random walk (start 10, N(0,0.05) steps, 0.2% spikes "
                 "+-U(2,5)); 3-D: mixture of 8 drifting anisotropic Gaussians + 0.5% noise",
         "config": {"workload": f"{args.workload} W={W} S={S} R={R} K={K} dim={dim} (plain "
-                               f"schedule, {index_used} index; slides from host memory, "
-                               f"pipelined on one stream, {per_launch:.2f} slides per launch of "
-                               f"the dominant kernel)",
+                               f"schedule, {index_used} index; value: slides from pageable "
+                               f"host memory, pipelined on one stream; roofline: the same "
+                               f"kernel on slides resident in HBM, {per_launch:.2f} slides "
+                               f"detected per launch)",
                    "window": W, "slide": S, "R": R, "K": K, "dim": dim},
         "valid": invalid == 0,
         "mismatched_slides": int(mism),
