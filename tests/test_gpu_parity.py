@@ -230,12 +230,14 @@ def test_pipelined_phase_times_count_full_launches():
     assert_same(got, want, "pipelined phase times")
 
 
-@pytest.mark.parametrize("dim", [1, 3])
-def test_pinned_slides_read_in_place(dim):
+@pytest.mark.parametrize("dim,sched", [(1, R.SCHED_PLAIN), (3, R.SCHED_PLAIN),
+                                       (3, R.SCHED_LEARNED)])
+def test_pinned_slides_read_in_place(dim, sched):
     """Slides in a buffer from rtod_host_alloc are read by the kernels in
     place, pipelined and one per call, on the slide-run index (1-D) and the
-    cell stencil (3-D): the lists are the oracle's."""
-    case(f"pinned dim={dim}")
+    cell stencil (3-D, plain and learned schedules): the lists are the
+    oracle's."""
+    case(f"pinned dim={dim} sched={sched}")
     if dim == 1:
         wl = O.C1
         xs, W, S, Rr, K, n = wl.data(), wl.W, wl.S, wl.R, wl.K, 70
@@ -245,11 +247,11 @@ def test_pinned_slides_read_in_place(dim):
     want = O.run_stream(xs, W, S, n, Rr, K, "grid")
     buf = R.pinned_empty(n * S, dim)
     buf[:] = xs[W: W + n * S]
-    with R.Detector(dim, W, S, Rr, K) as det:
+    with R.Detector(dim, W, S, Rr, K, sched_kind=sched) as det:
         det.prime(xs[:W])
         got = det.slides(buf, n)
     assert_same(got, want, f"pinned pipelined dim={dim}")
-    with R.Detector(dim, W, S, Rr, K) as det:
+    with R.Detector(dim, W, S, Rr, K, sched_kind=sched) as det:
         det.prime(xs[:W])
         got = [det.slide(buf[t * S: (t + 1) * S]) for t in range(n)]
     assert_same(got, want, f"pinned one per call dim={dim}")
