@@ -220,7 +220,11 @@ RTOD_API rtod_status rtod_slide_device(rtod_ctx *ctx, const float *dev_new_pts,
  * read at rtod_create, and past it the call fails with error flag 8).  (Otherwise a
  * slide's launches carry the next slide's sort and the previous slide's list
  * where they can.)  Up to four slides are queued,
- * and the host collects each slide's list as it completes.  After a failed
+ * and the host collects each slide's list as it completes.  On the slide-run
+ * index (1-D) the slides go in batches instead, the first 32 slides and then
+ * 64 each (32 each when rtod_slides stages them from pageable memory): a
+ * launch detects one batch and sorts the next, with at most two launches
+ * queued.  After a failed
  * call the context must be primed again.  Results equal those of n_slides
  * rtod_slide_device calls:
  * slide i's outliers (ring slots, ascending) are
