@@ -31,11 +31,11 @@ outlier list out in host memory.
 Extra objects on the line:
   roofline      the dominant kernel of the measured path (the pipelined
                 run's launch: a batch detected and the next sorted per launch
-                on the slide-run index (32, then 64 slides a batch), one
+                on the slide-run index (32, then 128 slides a batch), one
                 slide per launch on the cell
                 stencil), timed with HIP events on the library's stream in a
-                separate launch-by-launch pass over 352 slides from HBM (on
-                the slide-run index: a first batch of 32, then five of 64),
+                separate launch-by-launch pass from HBM (on the slide-run
+                index a first batch of 32, then five full batches of 128),
                 against the 8 TB/s HBM peak;
                 algorithmic bytes per launch = the slides it detects times
                 the slide's B = 4*d*W + 4*d*S + 4*|O| (window coordinates

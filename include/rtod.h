@@ -222,7 +222,8 @@ RTOD_API rtod_status rtod_slide_device(rtod_ctx *ctx, const float *dev_new_pts,
  * where they can.)  Up to four slides are queued,
  * and the host collects each slide's list as it completes.  On the slide-run
  * index (1-D) the slides go in batches instead, the first 32 slides and then
- * 64 each (32 each when rtod_slides stages them from pageable memory): a
+ * 128 each (64 from a rtod_host_alloc buffer, 32 when rtod_slides stages them
+ * from pageable memory): a
  * launch detects one batch and sorts the next, with at most two launches
  * queued.  After a failed
  * call the context must be primed again.  Results equal those of n_slides

@@ -32,11 +32,13 @@ ABL_NONE, ABL_FILTER_ONLY, ABL_NO_FILTER = 0, 1, 2
 FLAG_PHASE_TIMES = 0x1
 FLAG_LEARN_ALWAYS = 0x2
 # a pipelined run's batches on the slide-run index: the first RUN_FIRST_BATCH
-# slides, then RUN_BATCH each from HBM or a pinned buffer (RUN_HOST_BATCH
-# from pageable host memory); a launch detects one batch and sorts the next
-# (kRunFirstBatch, kRunBatch and kRunHostBatch of rtod_amd/csrc/rtod_runs.h)
+# slides, then RUN_BATCH each from HBM, RUN_PINNED_BATCH from a pinned
+# buffer, RUN_HOST_BATCH from pageable host memory; a launch detects one
+# batch and sorts the next (kRunFirstBatch, kRunBatch, kRunPinnedBatch and
+# kRunHostBatch of rtod_amd/csrc/rtod_runs.h)
 RUN_FIRST_BATCH = 32
-RUN_BATCH = 64
+RUN_BATCH = 128
+RUN_PINNED_BATCH = 64
 RUN_HOST_BATCH = 32
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "unsupported", 3: "call out of order",

@@ -1794,6 +1794,7 @@ rtod_status wait_slide(rtod_ctx *c, int64_t t)
 struct SlideSrc {
     const float *dev = nullptr;
     const float *host = nullptr;
+    bool pinned = false; // dev: a rtod_host_alloc buffer's device address
 };
 
 const float *slide_src(const rtod_ctx *c, const SlideSrc &src, int64_t t0, int32_t i)
@@ -2131,8 +2132,9 @@ rtod_status runs_run_slide(rtod_ctx *c, const float *src, double t_entry, double
 }
 
 // n consecutive slides, pipelined on the context's stream, in batches (the
-// first kRunFirstBatch slides, then kRunBatch each, or kRunHostBatch from
-// pageable host memory): after a prologue launch
+// first kRunFirstBatch slides, then kRunBatch each from HBM, kRunPinnedBatch
+// from a rtod_host_alloc buffer, kRunHostBatch from pageable host memory):
+// after a prologue launch
 // sorting batch 0, launch k detects batch k and sorts batch k + 1, each
 // slide's points staged into pinned memory just before the launch that sorts
 // it.  At most two launches are in flight: launch k is enqueued once every
@@ -2143,7 +2145,7 @@ rtod_status runs_slides(rtod_ctx *c, const SlideSrc &src, int32_t n, int32_t *of
                         int32_t *slots)
 {
     // batch k: slides [first(k), first(k + 1)) of the run
-    const int64_t B = src.host ? kRunHostBatch : kRunBatch;
+    const int64_t B = src.host ? kRunHostBatch : src.pinned ? kRunPinnedBatch : kRunBatch;
     auto first = [n, B](int32_t k) -> int32_t {
         const int64_t f = k == 0 ? 0 : kRunFirstBatch + (int64_t)(k - 1) * B;
         return (int32_t)std::min<int64_t>(f, n);
@@ -2399,10 +2401,12 @@ RTOD_API rtod_status rtod_slides(rtod_ctx *ctx, const float *pts, int32_t n_slid
     try {
         RTOD_HIP(hipSetDevice(ctx->device));
         SlideSrc src;
-        if (dp)
+        if (dp) {
             src.dev = dp;
-        else
+            src.pinned = true;
+        } else {
             src.host = pts;
+        }
         return slides_impl(ctx, src, n_slides, out_offsets, out_slots);
     } catch (...) {
         return RTOD_ERR_INTERNAL;

@@ -60,15 +60,17 @@ constexpr int kRunHCap = 8192;               // cells of the window histogram in
 constexpr int kRunCandCap = 2048;            // candidates counted per round
 constexpr int kRunECap = 2048;               // cell-table entries staged in LDS per round
 // A pipelined run's slides go in batches: the first kRunFirstBatch slides
-// (sorted by the prologue), then kRunBatch each (kRunHostBatch for slides
-// staged from pageable host memory); launch k detects batch k and sorts
-// batch k + 1.  The small first batch keeps the prologue, whose sorts
-// nothing overlaps, short.  Staged slides keep 32 a batch: the host's copy
-// of a batch then bounds each launch, and a run's fill and drain cost less
-// (64 measured 0.08 us per slide slower over 160-slide calls, against 0.12
-// faster from HBM).
-constexpr int kRunBatch = 64;                // slides per full pipelined launch
-constexpr int kRunHostBatch = 32;            // the same, slides staged from pageable memory
+// (sorted by the prologue), then kRunBatch each from HBM, kRunPinnedBatch
+// from a rtod_host_alloc buffer and kRunHostBatch staged from pageable host
+// memory; launch k detects batch k and sorts batch k + 1.  The small first
+// batch keeps the prologue, whose sorts nothing overlaps, short.  Over
+// PCIe smaller batches pay: staged, the host's copy of a batch bounds each
+// launch and a run's fill and drain cost less (64 measured 0.08 us per slide
+// slower than 32 over 160-slide calls); from a pinned buffer 128 measured
+// 0.78 us per slide against 0.73-0.75 for 64.
+constexpr int kRunBatch = 128;               // slides per full pipelined launch (from HBM)
+constexpr int kRunPinnedBatch = 64;          // the same, from a rtod_host_alloc buffer
+constexpr int kRunHostBatch = 32;            // the same, staged from pageable memory
 constexpr int kRunFirstBatch = 32;           // a pipelined run's first batch
 // host-out buffers of a runs context: a pipelined run holds at most two
 // batches' slides uncollected

@@ -105,15 +105,15 @@ def arena_bytes(dim, W, S, lbvh=False):
 
 
 def runs_arena_bytes(W, S):
-    """The slide-run index's arena: m + 129 run-ring entries (the window and
-    the slides a pipelined launch sorts and detects ahead, 64 per launch) of
+    """The slide-run index's arena: m + 257 run-ring entries (the window and
+    the slides a pipelined launch sorts and detects ahead, 128 per launch) of
     ceil(S / 8192) runs, each with its coordinates, slots, cells and cell
     starts (RL points, RL + 1 starts) and 4097 bucket starts and a 32-byte
     descriptor; the error word, the kernels' stamps and the prime copy."""
     m = W // S
     rps = (S + 8191) // 8192
     RL = (S + rps - 1) // rps
-    nrun = (m + 129) * rps
+    nrun = (m + 257) * rps
     return nrun * (4 * RL + 4 * RL + 4 * 4097 + 4 * RL + 4 * (RL + 1) + 32) + 4 + 8 * 7 + 4 * W
 
 

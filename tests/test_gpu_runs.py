@@ -179,7 +179,7 @@ def test_nonfinite_clamped_and_duplicates():
 @pytest.mark.parametrize("n", [1, 31, 32, 33, 64, 95, 96, 97, 159, 160, 161, 225])
 def test_pipelined_run_lengths(n):
     """Runs of n slides around the batches (the first 32 slides, then 32 a
-    batch from host memory and 64 from HBM; a launch detects one batch and
+    batch from host memory and 128 from HBM; a launch detects one batch and
     sorts the next), host-fed and from HBM, then single calls on the same
     context."""
     case(f"pipelined run of {n}")

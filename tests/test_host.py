@@ -123,16 +123,19 @@ def test_struct_layouts_match_header():
 
 
 def test_run_batch_matches_the_kernels():
-    """RUN_FIRST_BATCH, RUN_BATCH and RUN_HOST_BATCH (a pipelined run's
-    batches, used to price a launch's bytes) are the library's."""
+    """RUN_FIRST_BATCH, RUN_BATCH, RUN_PINNED_BATCH and RUN_HOST_BATCH (a
+    pipelined run's batches, used to price a launch's bytes) are the
+    library's."""
     with open(os.path.join(ROOT, "rtod_amd", "csrc", "rtod_runs.h")) as f:
         src = f.read()
     m = re.search(r"constexpr int kRunBatch = (\d+);", src)
     m0 = re.search(r"constexpr int kRunFirstBatch = (\d+);", src)
     mh = re.search(r"constexpr int kRunHostBatch = (\d+);", src)
+    mp = re.search(r"constexpr int kRunPinnedBatch = (\d+);", src)
     assert m and int(m.group(1)) == R.RUN_BATCH
     assert m0 and int(m0.group(1)) == R.RUN_FIRST_BATCH
     assert mh and int(mh.group(1)) == R.RUN_HOST_BATCH <= R.RUN_BATCH
+    assert mp and int(mp.group(1)) == R.RUN_PINNED_BATCH <= R.RUN_BATCH
 
 
 @pytest.mark.parametrize("kw,status", [
